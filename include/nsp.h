@@ -1,0 +1,232 @@
+/*
+ * nsp.h -- C ABI of libnsp, the MI355X (gfx950) n-space path tracer.
+ *
+ * This is the drop-in boundary for the painter path of Math Viewer
+ * (templeblock/math, src/painter/).  The reference exposes that path as the
+ * C++ template `paint<N,T>()` over virtual scene interfaces; a GPU cannot call
+ * those virtuals, so the boundary here is the scene DATA those interfaces
+ * carry, passed as plain pointers and sizes.  The C++17 adapter
+ * math_amd/csrc/painter/painter.h rebuilds the reference's own API
+ * (paint<N,T>, PainterNotifier, PaintObjects, Mesh, VisibleSharedMesh,
+ * VisiblePointLight, VisibleParallelProjector, VisibleBarPaintbrush,
+ * single_object_scene) on top of these functions.
+ *
+ * Conventions
+ *  - every function returns NSP_OK (0) or a negative NSP_E_* code and never
+ *    throws; the message is available from nsp_last_error(ctx);
+ *  - all array arguments are host pointers unless the name ends in _device;
+ *  - n is the ambient dimension N (3..6, the reference instantiates
+ *    paint<N,T> for N in {3,4,5,6}: src/painter/painter.cpp:627-643);
+ *  - one context per device; a context and the objects made from it are used
+ *    from one host thread at a time.
+ */
+#ifndef NSP_H
+#define NSP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NSP_MAX_DIM 6
+
+#define NSP_OK 0
+#define NSP_E_INVALID (-1)  /* bad argument or degenerate input (reference error()) */
+#define NSP_E_HIP (-2)      /* HIP runtime error */
+#define NSP_E_NOMEM (-3)    /* device allocation failed */
+#define NSP_E_STOPPED (-4)  /* stop flag observed */
+#define NSP_E_UNSUPPORTED (-5)
+
+typedef struct nsp_context nsp_context;
+typedef struct nsp_mesh nsp_mesh;
+typedef struct nsp_scene nsp_scene;
+
+/* Library and device setup.  No reference counterpart (the reference is CPU
+ * only); every other entry point takes the context. */
+int nsp_context_create(int device, nsp_context** out);
+void nsp_context_destroy(nsp_context* ctx);
+/* Last error message of the calling thread for this context (NULL ctx: for
+ * nsp_context_create).  Replaces PainterNotifier::painter_error_message
+ * (src/painter/painter.h:37) for the ABI's own failures. */
+const char* nsp_last_error(const nsp_context* ctx);
+/* Forces the next kernel launch of this context to report failure (host-side
+ * fault injection for error-path tests; no kernel is launched). */
+int nsp_debug_fail_next_launch(nsp_context* ctx);
+
+/* Mesh<N,T>(const Obj<N>*, Matrix<N+1,N+1,T> vertex_matrix, threads, progress)
+ * (src/painter/shapes/mesh.h:66, mesh.cpp:61-135).
+ *   vertices       V*n floats, row-major (Obj<N>::vertices(), src/obj/obj.h:72)
+ *   facets         F*n vertex indices (Obj<N>::Facet::vertices); facet index =
+ *                  position in this array, as in Obj<N>::facets() order
+ *   normals        VN*n floats or NULL; facet_normals F*n indices or NULL
+ *   vertex_matrix  (n+1)*(n+1) doubles row-major, affine; NULL = identity
+ * Builds the FP64 per-simplex hyperplane/barycentric bases and the 2^n-tree on
+ * the GPU (tree depth and leaf size of src/painter/shapes/mesh.cpp:33-58). */
+int nsp_mesh_create(nsp_context* ctx, int n, const float* vertices, int64_t V, const int32_t* facets, int64_t F,
+                    const float* normals, int64_t VN, const int32_t* facet_normals, const double* vertex_matrix,
+                    nsp_mesh** out);
+void nsp_mesh_destroy(nsp_mesh* mesh);
+/* Mesh::min_max (src/painter/shapes/mesh.cpp, used by compute_ray_offset and
+ * single_object_scene). */
+int nsp_mesh_min_max(const nsp_mesh* mesh, double* min, double* max);
+
+typedef struct {
+    int n;
+    int64_t facets;
+    int64_t tree_nodes;
+    int64_t tree_leaves;
+    int64_t tree_entries;
+    int tree_levels;
+    double basis_ms;  /* FP64 basis build on the GPU */
+    double tree_ms;   /* per-level binning tree build on the GPU */
+    double root_lo[NSP_MAX_DIM];
+    double root_size[NSP_MAX_DIM];
+} nsp_mesh_info;
+int nsp_mesh_get_info(const nsp_mesh* mesh, nsp_mesh_info* info);
+
+/* FP64 per-simplex data of MeshHyperplaneSimplex (mesh_hyperplane_simplex.cpp:59-143,
+ * hyperplane_geometry.h:83-115): normal F*n, plane normals F*(n-1)*n,
+ * plane offsets F*(n-1).  Any pointer may be NULL. */
+int nsp_mesh_records(const nsp_mesh* mesh, double* normal, double* plane_n, double* plane_d);
+/* Leaf lists of the 2^n-tree as a flat int64 stream: per leaf, level, q[n]
+ * (integer cell coordinates at that level), count, then count simplex indices.
+ * Returns the number of int64 values (call with out=NULL for the size), or a
+ * negative error code. */
+int64_t nsp_mesh_tree_dump(const nsp_mesh* mesh, int64_t* out, int64_t cap);
+
+/* First hit of a ray batch: Mesh::intersect_approximate + intersect_precise
+ * (src/painter/shapes/mesh.cpp:137-166) for each ray; the parity entry point.
+ *   org, dir  count*n floats (dir need not be unit: Ray normalises it,
+ *             src/com/ray.h:34-36)
+ *   facet     count int32: simplex index, -1 on miss
+ *   t         count doubles: hit distance along the unit direction
+ *   near      count bytes (or NULL): 1 when the FP32 decision for some simplex
+ *             on the ray fell inside the near-boundary band and was re-taken
+ *             in FP64 */
+int nsp_trace_first_hit(nsp_mesh* mesh, const float* org, const float* dir, int64_t count, int32_t* facet, double* t,
+                        uint8_t* near);
+/* Same with device pointers on the given HIP stream (NULL = the context stream);
+ * asynchronous. */
+int nsp_trace_first_hit_device(nsp_mesh* mesh, const float* org_device, const float* dir_device, int64_t count,
+                               int32_t* facet_device, double* t_device, uint8_t* near_device, void* hip_stream);
+
+/* Scene description: the data behind PaintObjects<N,T> (src/painter/objects.h:212-223). */
+#define NSP_LIGHT_POINT 0    /* PointLight, src/painter/lights/light_source.h:26-57 */
+#define NSP_LIGHT_CONSTANT 1 /* ConstantLight, src/painter/lights/light_source.h:59-78 */
+typedef struct {
+    int kind;
+    double position[NSP_MAX_DIM];
+    float rgb[3];
+    double unit_intensity_distance; /* point light only */
+} nsp_light;
+
+#define NSP_PROJECTOR_PARALLEL 0 /* ParallelProjector, src/painter/projectors/projector.h:153-202 */
+typedef struct {
+    int kind;
+    double camera_org[NSP_MAX_DIM];
+    double camera_dir[NSP_MAX_DIM];
+    double screen_axes[NSP_MAX_DIM - 1][NSP_MAX_DIM];
+    double units_per_pixel;
+    int screen_size[NSP_MAX_DIM - 1];
+} nsp_projector;
+
+/* SurfaceProperties (src/painter/objects.h:27-139): colour, diffuse, light source flag/colour. */
+typedef struct {
+    float rgb[3];
+    float diffuse;
+    int light_source;
+    float light_rgb[3];
+} nsp_surface;
+
+/* PaintObjects: objects(), light_sources(), projector(),
+ * default_surface_properties().  One mesh object (the configurations of
+ * single_object_scene); `object` holds the VisibleSharedMesh surface
+ * properties (colour, diffuse). */
+int nsp_scene_create(nsp_context* ctx, int n, nsp_mesh* mesh, const nsp_surface* object, const nsp_light* lights,
+                     int light_count, const nsp_projector* projector, const nsp_surface* default_surface,
+                     nsp_scene** out);
+void nsp_scene_destroy(nsp_scene* scene);
+
+#define NSP_PRECISION_FP32 0 /* FP32 trace with FP64 near-boundary re-test */
+#define NSP_PRECISION_FP64 1 /* FP64 everywhere (reference-quality renders) */
+typedef struct {
+    int samples_per_pixel; /* requested; rounded up to k^(n-1) (sampler.h:31-52) */
+    int max_depth;         /* MAX_RECURSION_LEVEL (painter.cpp:42), reference value 100 */
+    int smooth_normal;     /* paint(..., smooth_normal) */
+    int precision;         /* NSP_PRECISION_* */
+    int passes;            /* passes to run (>=1); VisibleBarPaintbrush max_pass_count */
+    uint64_t seed;         /* counter-based RNG key */
+    int64_t batch_paths;   /* paths per wavefront batch; 0 = library default/autotuned */
+    int use_graph;         /* capture each batch's per-bounce launches in a hipGraph */
+} nsp_paint_params;
+
+/* Delivery of results: PainterNotifier::painter_pixel_after(pixel, color)
+ * (src/painter/painter.h:26-38) in batches of consecutive pixels in
+ * GlobalIndex order (x0 fastest); rgb holds the running mean colour of each
+ * pixel (Pixels::add_color_and_samples, painter.cpp:57-88).  Called on the
+ * thread that called nsp_paint. */
+typedef struct {
+    void (*pixels_after)(void* user, int64_t first_pixel, int64_t count, const float* rgb_mean);
+    void (*pass_done)(void* user, int pass, double seconds);
+    void (*error_message)(void* user, const char* message);
+    void* user;
+} nsp_callbacks;
+
+/* paint<N,T>(notifier, samples_per_pixel, paint_objects, paintbrush,
+ * thread_count, stop, smooth_normal) (src/painter/painter.h:40-42,
+ * painter.cpp:599-625).  Blocks until `passes` passes are done or *stop becomes
+ * nonzero (polled between wavefront batches, the analogue of the per-pixel
+ * poll at painter.cpp:419).  Errors are reported through callbacks->error_message
+ * as in paint(), and returned. */
+int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callbacks* callbacks,
+              const volatile int32_t* stop);
+
+/* Paintbrush::statistics (src/painter/paintbrushes/paintbrush.h:222-232):
+ * passes, pixels, rays (reference counting rules), samples, last pass time. */
+int nsp_statistics(const nsp_scene* scene, int64_t* pass_count, int64_t* pixel_count, int64_t* ray_count,
+                   int64_t* sample_count, double* previous_pass_duration);
+
+/* Current per-pixel colour sums and sample counts (host copy): rgb_sum
+ * pixels*3 floats, samples pixels int32 (either may be NULL).  The per-pixel
+ * state of Pixels (painter.cpp:57-88); with nsp_image_set it is also the
+ * checkpoint of a progressive render. */
+int nsp_image_get(const nsp_scene* scene, float* rgb_sum, int32_t* samples);
+int nsp_image_set(nsp_scene* scene, const float* rgb_sum, const int32_t* samples);
+/* Number of pixels of the scene's screen. */
+int64_t nsp_scene_pixel_count(const nsp_scene* scene);
+
+/* Per-stage timing of the last pass (milliseconds, HIP events on the render
+ * stream): raygen, closest-hit traversal, shade, shadow traversal, accumulate;
+ * and the number of launches of each. */
+typedef struct {
+    double ms_raygen, ms_closest, ms_shade, ms_shadow, ms_accumulate, ms_total;
+    int64_t launches_closest, launches_shadow;
+    int64_t closest_rays, shadow_rays;
+    /* traversal counters of the closest-hit stage and of the shadow stage
+     * (filled when nsp_set_counters is on) */
+    int64_t node_visits, leaf_visits, simplex_tests, retests;
+    int64_t shadow_node_visits, shadow_leaf_visits, shadow_simplex_tests, shadow_retests;
+} nsp_stage_stats;
+int nsp_last_pass_stats(const nsp_scene* scene, nsp_stage_stats* out);
+/* Enables per-ray traversal counters (node visits, leaf visits, simplex
+ * tests, FP64 re-tests) for roofline accounting; off by default. */
+int nsp_set_counters(nsp_scene* scene, int enable);
+
+/* Synthetic meshes (the configurations' generators; host only, no GPU).
+ * Each returns the vertex and facet counts when the output pointers are NULL. */
+/* Torus R, r on a nu x nv grid, minor radius per vertex r + U(-noise, noise). n=3. */
+int nsp_meshgen_torus(double R, double r, int nu, int nv, double noise, uint64_t seed, float* vertices, int64_t* V,
+                      int32_t* facets, int64_t* F);
+/* Boundary of [-1,1]^n, each facet split into g^(n-1) cubes and each cube into
+ * (n-1)! Kuhn simplices; then optionally projected onto S^(n-1)
+ * (sphere != 0), radially scaled by (1 + N(0, sigma)), and with the simplices
+ * whose centroid lies within angle cap_angle of a seeded direction removed
+ * (cap_angle <= 0: none). */
+int nsp_meshgen_cube_sphere(int n, int g, int sphere, double sigma, double cap_angle, uint64_t seed, float* vertices,
+                            int64_t* V, int32_t* facets, int64_t* F);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

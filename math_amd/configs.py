@@ -1,0 +1,99 @@
+"""Workload configurations of the n-space painter benchmarks (synthetic, seeded).
+
+Scene conventions follow single_object_scene (src/painter/scenes/single_object.h:102-179):
+camera on the last axis looking along -e_{n-1}, screen axes e_0..e_{n-2},
+units per pixel = largest projected extent / largest screen edge, background a
+light source of its luminance.  Colours follow the reference's painter test
+(src/painter/test/test_painter.cpp:36: background sRGB (50,100,150)); the object
+is white with diffuse albedo 0.8; one point light (VisiblePointLight) whose
+unit-intensity distance is its distance from the origin.
+
+  C0  n=3 torus R=1 r=0.35, 200x100 grid (40,000 triangles), radius noise
+      +-0.01 (seed 0), light (3,3,3), 512^2, 16 spp, depth 5
+  C1  n=4 tesseract boundary, 8 x 24^3 cubes x 6 Kuhn tetrahedra (663,552),
+      on S^3 with radial N(0, 0.02) (seed 1), light (2,2,2,2), 128^3, 16 spp, depth 5
+  C2  n=5 penteract boundary, 10 x 10^4 cells x 24 Kuhn 4-simplices (2.4M before
+      the cap), on S^4 with a cap of angular radius 0.5 removed (seed 2), 48^4, 8 spp, depth 4
+"""
+import numpy as np
+
+from . import native
+
+
+def srgb_to_linear(c):
+    """srgb_to_rgb, src/com/color/conversion.cpp:188-205."""
+    c = c / 255.0
+    if c >= 1:
+        return 1.0
+    if c >= 0.04045:
+        return ((c + 0.055) / 1.055) ** 2.4
+    if c > 0:
+        return c / 12.92
+    return 0.0
+
+
+BACKGROUND_RGB = [srgb_to_linear(50), srgb_to_linear(100), srgb_to_linear(150)]
+
+
+def mesh_min_max(vertices, facets):
+    used = vertices[np.unique(facets.ravel())].astype(np.float64)
+    return used.min(axis=0), used.max(axis=0)
+
+
+def camera_for(vertices, facets, screen_edge):
+    """single_object_scene camera with min_screen = max_screen = screen_edge."""
+    n = vertices.shape[1]
+    vmin, vmax = mesh_min_max(vertices, facets)
+    size = vmax - vmin
+    center = vmin + size / 2
+    max_proj = float(np.max(size[: n - 1]))
+    screen = [int(min(max(round(size[i] / max_proj * screen_edge), screen_edge), screen_edge)) for i in range(n - 1)]
+    cam = center.copy()
+    cam[n - 1] = vmax[n - 1] + float(np.linalg.norm(size))
+    cdir = np.zeros(n)
+    cdir[n - 1] = -1
+    axes = np.eye(n)[: n - 1]
+    return dict(camera_org=cam.tolist(), camera_dir=cdir.tolist(), axes=axes.tolist(),
+                units_per_pixel=max_proj / screen_edge, screen=screen)
+
+
+def scene_dict(vertices, facets, screen_edge, light_pos, diffuse=0.8):
+    n = vertices.shape[1]
+    cfg = dict(n=n, object_rgb=[1.0, 1.0, 1.0], diffuse=diffuse, background_rgb=BACKGROUND_RGB,
+               light_pos=list(light_pos), light_rgb=[1.0, 1.0, 1.0],
+               light_unit_distance=float(np.linalg.norm(light_pos)))
+    cfg.update(camera_for(vertices, facets, screen_edge))
+    return cfg
+
+
+CONFIGS = {
+    # name: (generator, generator args, screen edge, spp, depth, light)
+    "C0": ("torus", dict(R=1.0, r=0.35, nu=200, nv=100, noise=0.01, seed=0), 512, 16, 5, (3.0, 3.0, 3.0)),
+    "C1": ("cube_sphere", dict(n=4, g=24, sphere=True, sigma=0.02, cap_angle=0.0, seed=1), 128, 16, 5,
+           (2.0, 2.0, 2.0, 2.0)),
+    "C2": ("cube_sphere", dict(n=5, g=10, sphere=True, sigma=0.0, cap_angle=0.5, seed=2), 48, 8, 4,
+           (2.0, 2.0, 2.0, 2.0, 2.0)),
+}
+
+
+def make_mesh(gen, args):
+    if gen == "torus":
+        return native.meshgen_torus(**args)
+    return native.meshgen_cube_sphere(**args)
+
+
+def config(name, **override):
+    """Returns (vertices, facets, scene dict, spp, depth) of a configuration.
+    override may replace 'screen', 'spp', 'depth' or generator arguments."""
+    gen, args, screen, spp, depth, light = CONFIGS[name]
+    args = dict(args)
+    for k in list(override):
+        if k in args:
+            args[k] = override.pop(k)
+    screen = override.pop("screen", screen)
+    spp = override.pop("spp", spp)
+    depth = override.pop("depth", depth)
+    if override:
+        raise KeyError(f"unknown overrides {sorted(override)}")
+    v, f = make_mesh(gen, args)
+    return v, f, scene_dict(v, f, screen, light), spp, depth

@@ -1,0 +1,703 @@
+// nsp_api.cpp -- C ABI of libnsp (include/nsp.h): contexts, meshes, scenes,
+// the first-hit entry point and the progressive paint loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "nsp.h"
+#include "nsp_build.hpp"
+#include "nsp_internal.hpp"
+#include "nsp_render.hpp"
+
+namespace nsp {
+
+[[noreturn]] void raise(int code, const std::string& msg) { throw Error{code, msg}; }
+
+void check_hip(hipError_t e, const char* what)
+{
+    if (e == hipSuccess) return;
+    (void)hipGetLastError();
+    int code = (e == hipErrorOutOfMemory) ? NSP_E_NOMEM : NSP_E_HIP;
+    raise(code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// TREE_MIN_OBJECTS_PER_BOX and tree_max_depth<N>(), src/painter/shapes/mesh.cpp:33-58
+constexpr int TREE_MIN_OBJECTS_PER_BOX = 10;
+static int tree_max_depth(int n)
+{
+    switch (n) {
+    case 3: return 10;
+    case 4: return 8;
+    case 5: return 6;
+    case 6: return 5;
+    default: {
+        double s = 1e9, r = std::pow(2.0, n);
+        return (int)std::max(2.0, std::floor(std::log(s * (r - 1) + 1) / std::log(r)));
+    }
+    }
+}
+
+// cosine_sphere_coefficient, src/painter/coefficient/cosine_sphere.h:97-162
+static long double cosine_sphere_coefficient(int n)
+{
+    long double p = (n & 1) == 0 ? 0.5L : 1.0L;
+    for (int i = n - 1; i > 1; i -= 2) {
+        p *= i;
+        if (i > 2) p /= (i - 1);
+    }
+    if ((n & 1) == 0) p *= 3.14159265358979323846264338327950288L;
+    return p;
+}
+
+// StratifiedJitteredSampler::one_dimension_size, src/painter/sampling/sampler.h:31-52
+static int one_dimension_size(int dims, int spp)
+{
+    if (spp < 1) raise(NSP_E_INVALID, "Stratified jittered sample count (" + std::to_string(spp) + ") is not a positive integer");
+    double v = std::pow(spp, 1.0 / dims);
+    unsigned f = (unsigned)std::floor(v), c = (unsigned)std::ceil(v);
+    unsigned long long p = 1;
+    for (int i = 0; i < dims; ++i) p *= f;
+    if (p >= (unsigned)spp) return (int)f;
+    p = 1;
+    for (int i = 0; i < dims; ++i) p *= c;
+    if (p >= (unsigned)spp) return (int)c;
+    raise(NSP_E_INVALID, "Could not compute one dimension sample count");
+}
+
+}  // namespace nsp
+
+using namespace nsp;
+
+struct nsp_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string error;
+    bool fail_next = false;
+};
+
+struct nsp_mesh {
+    nsp_context* ctx;
+    MeshDev m;
+};
+
+struct nsp_scene {
+    nsp_context* ctx;
+    nsp_mesh* mesh;
+    int n;
+    nsp_surface object;
+    std::vector<nsp_light> lights;
+    nsp_projector proj;
+    nsp_surface background;
+    int64_t pixels = 0;
+    DevBuf pix_sum, pix_cnt;
+    RenderBuffers buf;
+    DevBuf totals;  // 3 u64
+    StageMarks marks;
+    bool counters = false;
+    // statistics (Paintbrush::statistics)
+    int64_t pass_count = 0, pixel_count = 0, ray_count = 0, sample_count = 0;
+    double last_pass_s = 0;
+    nsp_stage_stats stage{};
+};
+
+static thread_local std::string g_create_error;
+
+template <class F>
+static int guarded(nsp_context* ctx, F&& f)
+{
+    try {
+        if (ctx) NSP_HIP(hipSetDevice(ctx->device));
+        f();
+        return NSP_OK;
+    } catch (const Error& e) {
+        if (ctx) ctx->error = e.msg;
+        else g_create_error = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        if (ctx) ctx->error = "host allocation failed";
+        return NSP_E_NOMEM;
+    } catch (const std::exception& e) {
+        if (ctx) ctx->error = e.what();
+        return NSP_E_INVALID;
+    } catch (...) {
+        if (ctx) ctx->error = "unknown error";
+        return NSP_E_INVALID;
+    }
+}
+
+static void launch_gate(nsp_context* ctx)
+{
+    if (ctx->fail_next) {
+        ctx->fail_next = false;
+        raise(NSP_E_HIP, "kernel launch failed (injected)");
+    }
+}
+
+template <typename T>
+static void fill_scene(const nsp_scene& S, const nsp_paint_params& p, int pass, SceneK<T>& k)
+{
+    std::memset(&k, 0, sizeof k);
+    const int n = S.n;
+    const nsp_projector& pr = S.proj;
+    double cd[NSP_MAX_DIM];
+    {
+        double r = pr.camera_dir[0] * pr.camera_dir[0];
+        for (int i = 1; i < n; ++i) r = std::fma(pr.camera_dir[i], pr.camera_dir[i], r);
+        double l = std::sqrt(r);
+        for (int i = 0; i < n; ++i) cd[i] = pr.camera_dir[i] / l;
+    }
+    for (int i = 0; i < n; ++i) {
+        k.cam_org[i] = (T)pr.camera_org[i];
+        k.cam_dir[i] = (T)cd[i];
+    }
+    for (int a = 0; a < n - 1; ++a) {
+        double r = pr.screen_axes[a][0] * pr.screen_axes[a][0];
+        for (int i = 1; i < n; ++i) r = std::fma(pr.screen_axes[a][i], pr.screen_axes[a][i], r);
+        double l = std::sqrt(r);
+        for (int i = 0; i < n; ++i) k.axes[a][i] = (T)((pr.screen_axes[a][i] / l) * pr.units_per_pixel);
+        k.screen[a] = pr.screen_size[a];
+    }
+    k.pixels = S.pixels;
+    k.k = one_dimension_size(n - 1, p.samples_per_pixel);
+    k.S = 1;
+    for (int i = 0; i < n - 1; ++i) k.S *= k.k;
+    k.k_inv = (T)1 / (T)k.k;
+    k.light_count = (int)S.lights.size();
+    for (int l = 0; l < k.light_count; ++l) {
+        const nsp_light& L = S.lights[l];
+        k.light_kind[l] = L.kind;
+        for (int i = 0; i < n; ++i) k.light_pos[l][i] = (T)L.position[i];
+        for (int c = 0; c < 3; ++c) k.light_rgb[l][c] = L.rgb[c];
+        // PointLight: m_coef = pow(unit_intensity_distance, N - 1), light_source.h:37-40
+        k.light_coef[l] = (T)std::pow(L.unit_intensity_distance, n - 1);
+    }
+    for (int c = 0; c < 3; ++c) {
+        k.bg_rgb[c] = S.background.rgb[c];
+        k.bg_light_rgb[c] = S.background.light_rgb[c];
+        k.obj_rgb[c] = S.object.rgb[c];
+        k.obj_light_rgb[c] = S.object.light_rgb[c];
+    }
+    k.bg_is_light = S.background.light_source;
+    k.obj_is_light = S.object.light_source;
+    k.diffuse = S.object.diffuse;
+    // compute_ray_offset, painter.cpp:539-558: max |coordinate| * 1000 * epsilon(T)
+    double all_max = -INFINITY;
+    for (int i = 0; i < n; ++i)
+        all_max = std::max(all_max, std::max(std::fabs(S.mesh->m.mn[i]), std::fabs(S.mesh->m.mx[i])));
+    const double eps = sizeof(T) == 4 ? 1.1920928955078125e-07 : 2.220446049250313e-16;
+    k.ray_offset = (T)(all_max * (1000 * eps));
+    k.cos_coef = (T)cosine_sphere_coefficient(n);
+    k.max_depth = p.max_depth;
+    k.smooth = p.smooth_normal ? 1 : 0;
+    k.has_normals = S.mesh->m.has_normals ? 1 : 0;
+    k.seed = p.seed;
+    k.pass = pass;
+}
+
+extern "C" {
+
+int nsp_context_create(int device, nsp_context** out)
+{
+    if (!out) return NSP_E_INVALID;
+    *out = nullptr;
+    return guarded(nullptr, [&] {
+        int count = 0;
+        NSP_HIP(hipGetDeviceCount(&count));
+        if (device < 0 || device >= count) raise(NSP_E_INVALID, "no such device: " + std::to_string(device));
+        NSP_HIP(hipSetDevice(device));
+        auto ctx = std::make_unique<nsp_context>();
+        ctx->device = device;
+        NSP_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        *out = ctx.release();
+    });
+}
+
+void nsp_context_destroy(nsp_context* ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* nsp_last_error(const nsp_context* ctx) { return ctx ? ctx->error.c_str() : g_create_error.c_str(); }
+
+int nsp_debug_fail_next_launch(nsp_context* ctx)
+{
+    if (!ctx) return NSP_E_INVALID;
+    ctx->fail_next = true;
+    return NSP_OK;
+}
+
+int nsp_mesh_create(nsp_context* ctx, int n, const float* vertices, int64_t V, const int32_t* facets, int64_t F,
+                    const float* normals, int64_t VN, const int32_t* facet_normals, const double* vertex_matrix,
+                    nsp_mesh** out)
+{
+    if (!ctx || !out) return NSP_E_INVALID;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        if (n < 3 || n > NSP_MAX_DIM) raise(NSP_E_INVALID, "dimension " + std::to_string(n) + " is not in [3, 6]");
+        // create_mesh_object, src/painter/shapes/mesh.cpp:61-125
+        if (V <= 0 || !vertices) raise(NSP_E_INVALID, "No vertices found in obj");
+        if (F <= 0 || !facets) raise(NSP_E_INVALID, "No facets found in obj");
+        if (F >= (int64_t)1 << 31) raise(NSP_E_INVALID, "too many facets");
+        auto mesh = std::make_unique<nsp_mesh>();
+        mesh->ctx = ctx;
+        MeshDev& m = mesh->m;
+        m.n = n;
+        m.V = V;
+        m.F = F;
+        std::vector<double> vtx((size_t)V * n);
+        if (vertex_matrix) {
+            // MatrixMulVector, src/com/matrix_alg.h:30-63
+            const double* M = vertex_matrix;
+            if (M[n * (n + 1) + n] != 1) raise(NSP_E_INVALID, "Wrong matrix for matrix-vector multiplier");
+            for (int i = 0; i < n; ++i)
+                if (M[n * (n + 1) + i] != 0) raise(NSP_E_INVALID, "Wrong matrix for matrix-vector multiplier");
+            for (int64_t v = 0; v < V; ++v)
+                for (int i = 0; i < n; ++i) {
+                    double r = M[i * (n + 1) + n];
+                    for (int c = 0; c < n; ++c) r = std::fma(M[i * (n + 1) + c], (double)vertices[v * n + c], r);
+                    vtx[v * n + i] = r;
+                }
+        } else {
+            for (int64_t k = 0; k < V * n; ++k) vtx[k] = vertices[k];
+        }
+        for (int i = 0; i < n; ++i) {
+            m.mn[i] = INFINITY;
+            m.mx[i] = -INFINITY;
+        }
+        for (int64_t f = 0; f < F; ++f)
+            for (int k = 0; k < n; ++k) {
+                int32_t vi = facets[f * n + k];
+                if (vi < 0 || vi >= V) raise(NSP_E_INVALID, "facet vertex index out of range");
+                for (int i = 0; i < n; ++i) {
+                    m.mn[i] = std::min(m.mn[i], vtx[(size_t)vi * n + i]);
+                    m.mx[i] = std::max(m.mx[i], vtx[(size_t)vi * n + i]);
+                }
+            }
+        hipStream_t s = ctx->stream;
+        m.vtx64.alloc(sizeof(double) * V * n);
+        NSP_HIP(hipMemcpyAsync(m.vtx64.p, vtx.data(), sizeof(double) * V * n, hipMemcpyHostToDevice, s));
+        m.fac.alloc(sizeof(int32_t) * F * n);
+        NSP_HIP(hipMemcpyAsync(m.fac.p, facets, sizeof(int32_t) * F * n, hipMemcpyHostToDevice, s));
+        std::vector<double> vn;
+        if (normals && facet_normals && VN > 0) {
+            m.has_normals = true;
+            m.VN = VN;
+            vn.resize((size_t)VN * n);
+            for (int64_t k = 0; k < VN * n; ++k) vn[k] = normals[k];
+            m.vn64.alloc(sizeof(double) * VN * n);
+            NSP_HIP(hipMemcpyAsync(m.vn64.p, vn.data(), sizeof(double) * VN * n, hipMemcpyHostToDevice, s));
+            m.fnrm.alloc(sizeof(int32_t) * F * n);
+            NSP_HIP(hipMemcpyAsync(m.fnrm.p, facet_normals, sizeof(int32_t) * F * n, hipMemcpyHostToDevice, s));
+        }
+        NSP_HIP(hipStreamSynchronize(s));
+        launch_gate(ctx);
+        build_mesh_device(m, tree_max_depth(n), TREE_MIN_OBJECTS_PER_BOX, s);
+        *out = mesh.release();
+    });
+}
+
+void nsp_mesh_destroy(nsp_mesh* mesh)
+{
+    if (!mesh) return;
+    (void)hipSetDevice(mesh->ctx->device);
+    (void)hipStreamSynchronize(mesh->ctx->stream);
+    delete mesh;
+}
+
+int nsp_mesh_min_max(const nsp_mesh* mesh, double* mn, double* mx)
+{
+    if (!mesh || !mn || !mx) return NSP_E_INVALID;
+    for (int i = 0; i < mesh->m.n; ++i) {
+        mn[i] = mesh->m.mn[i];
+        mx[i] = mesh->m.mx[i];
+    }
+    return NSP_OK;
+}
+
+int nsp_mesh_get_info(const nsp_mesh* mesh, nsp_mesh_info* info)
+{
+    if (!mesh || !info) return NSP_E_INVALID;
+    const MeshDev& m = mesh->m;
+    std::memset(info, 0, sizeof *info);
+    info->n = m.n;
+    info->facets = m.F;
+    info->tree_nodes = m.tree.node_count;
+    info->tree_leaves = m.tree.leaf_count;
+    info->tree_entries = m.tree.entry_count;
+    info->tree_levels = m.tree.levels;
+    info->basis_ms = m.basis_ms;
+    info->tree_ms = m.build_ms;
+    for (int i = 0; i < m.n; ++i) {
+        info->root_lo[i] = m.tree.root_lo[i];
+        info->root_size[i] = m.tree.root_size[i];
+    }
+    return NSP_OK;
+}
+
+int nsp_mesh_records(const nsp_mesh* mesh, double* normal, double* plane_n, double* plane_d)
+{
+    if (!mesh) return NSP_E_INVALID;
+    nsp_context* ctx = mesh->ctx;
+    return guarded(ctx, [&] {
+        const MeshDev& m = mesh->m;
+        const int n = m.n, st = rec64_stride(n);
+        std::vector<double> rec((size_t)m.F * st);
+        NSP_HIP(hipMemcpy(rec.data(), m.rec64.p, sizeof(double) * rec.size(), hipMemcpyDeviceToHost));
+        for (int64_t f = 0; f < m.F; ++f) {
+            const double* r = rec.data() + f * st;
+            if (normal) std::memcpy(normal + f * n, r, sizeof(double) * n);
+            if (plane_n) std::memcpy(plane_n + f * (n - 1) * n, r + 2 * n, sizeof(double) * (n - 1) * n);
+            if (plane_d) std::memcpy(plane_d + f * (n - 1), r + 2 * n + (n - 1) * n, sizeof(double) * (n - 1));
+        }
+    });
+}
+
+int64_t nsp_mesh_tree_dump(const nsp_mesh* mesh, int64_t* out, int64_t cap)
+{
+    if (!mesh) return NSP_E_INVALID;
+    int64_t written = 0;
+    int rc = guarded(mesh->ctx, [&] {
+        const MeshDev& m = mesh->m;
+        const TreeDev& t = m.tree;
+        const int n = m.n;
+        std::vector<int2> nodes(t.node_count);
+        std::vector<int8_t> lv(t.node_count);
+        std::vector<int32_t> q((size_t)t.node_count * n);
+        std::vector<int32_t> ent(std::max<int64_t>(t.entry_count, 1));
+        NSP_HIP(hipMemcpy(nodes.data(), t.nodes.p, sizeof(int2) * t.node_count, hipMemcpyDeviceToHost));
+        NSP_HIP(hipMemcpy(lv.data(), t.node_level.p, t.node_count, hipMemcpyDeviceToHost));
+        NSP_HIP(hipMemcpy(q.data(), t.node_q.p, sizeof(int32_t) * q.size(), hipMemcpyDeviceToHost));
+        if (t.entry_count)
+            NSP_HIP(hipMemcpy(ent.data(), t.entries.p, sizeof(int32_t) * t.entry_count, hipMemcpyDeviceToHost));
+        // leaves in depth-first child order (the order of the reference's recursive build)
+        std::vector<int64_t> stack{0};
+        while (!stack.empty()) {
+            int64_t id = stack.back();
+            stack.pop_back();
+            int2 nd = nodes[id];
+            if (nd.y < 0) {
+                int children = 1 << n;
+                for (int c = children - 1; c >= 0; --c) stack.push_back((int64_t)nd.x + (int64_t)c * (-nd.y));
+                continue;
+            }
+            int64_t need = 2 + n + nd.y;
+            if (out && written + need <= cap) {
+                out[written] = lv[id];
+                for (int k = 0; k < n; ++k) out[written + 1 + k] = q[(size_t)id * n + k];
+                out[written + 1 + n] = nd.y;
+                for (int j = 0; j < nd.y; ++j) out[written + 2 + n + j] = ent[nd.x + j];
+            }
+            written += need;
+        }
+    });
+    return rc == NSP_OK ? written : rc;
+}
+
+int nsp_trace_first_hit_device(nsp_mesh* mesh, const float* org, const float* dir, int64_t count, int32_t* facet,
+                               double* t, uint8_t* near, void* stream)
+{
+    if (!mesh || count < 0) return NSP_E_INVALID;
+    if (count == 0) return NSP_OK;
+    if (!org || !dir || !facet || !t) return NSP_E_INVALID;
+    nsp_context* ctx = mesh->ctx;
+    return guarded(ctx, [&] {
+        launch_gate(ctx);
+        hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+        launch_first_hit(mesh->m, org, dir, count, facet, t, near, nullptr, s);
+    });
+}
+
+int nsp_trace_first_hit(nsp_mesh* mesh, const float* org, const float* dir, int64_t count, int32_t* facet, double* t,
+                        uint8_t* near)
+{
+    if (!mesh || count < 0) return NSP_E_INVALID;
+    if (count == 0) return NSP_OK;
+    if (!org || !dir || !facet || !t) return NSP_E_INVALID;
+    nsp_context* ctx = mesh->ctx;
+    return guarded(ctx, [&] {
+        launch_gate(ctx);
+        const int n = mesh->m.n;
+        hipStream_t s = ctx->stream;
+        DevBuf d_o, d_d, d_i, d_t, d_n;
+        d_o.alloc(sizeof(float) * count * n);
+        d_d.alloc(sizeof(float) * count * n);
+        d_i.alloc(sizeof(int32_t) * count);
+        d_t.alloc(sizeof(double) * count);
+        d_n.alloc(count);
+        NSP_HIP(hipMemcpyAsync(d_o.p, org, sizeof(float) * count * n, hipMemcpyHostToDevice, s));
+        NSP_HIP(hipMemcpyAsync(d_d.p, dir, sizeof(float) * count * n, hipMemcpyHostToDevice, s));
+        launch_first_hit(mesh->m, d_o.as<float>(), d_d.as<float>(), count, d_i.as<int32_t>(), d_t.as<double>(),
+                         d_n.as<uint8_t>(), nullptr, s);
+        NSP_HIP(hipMemcpyAsync(facet, d_i.p, sizeof(int32_t) * count, hipMemcpyDeviceToHost, s));
+        NSP_HIP(hipMemcpyAsync(t, d_t.p, sizeof(double) * count, hipMemcpyDeviceToHost, s));
+        if (near) NSP_HIP(hipMemcpyAsync(near, d_n.p, count, hipMemcpyDeviceToHost, s));
+        NSP_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int nsp_scene_create(nsp_context* ctx, int n, nsp_mesh* mesh, const nsp_surface* object, const nsp_light* lights,
+                     int light_count, const nsp_projector* projector, const nsp_surface* default_surface,
+                     nsp_scene** out)
+{
+    if (!ctx || !out) return NSP_E_INVALID;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        if (!mesh || !object || !projector || !default_surface) raise(NSP_E_INVALID, "null scene argument");
+        if (mesh->m.n != n) raise(NSP_E_INVALID, "mesh dimension differs from scene dimension");
+        if (light_count < 0 || (light_count > 0 && !lights)) raise(NSP_E_INVALID, "bad light list");
+        if (light_count > 1)
+            raise(NSP_E_UNSUPPORTED, "more than one light source is not supported by the GPU painter yet");
+        if (projector->kind != NSP_PROJECTOR_PARALLEL) raise(NSP_E_UNSUPPORTED, "only the parallel projector is supported");
+        int64_t pixels = 1;
+        for (int i = 0; i < n - 1; ++i) {
+            // screen_org, src/painter/projectors/projector.h:51-66; pixel coordinates are int_least16_t
+            if (projector->screen_size[i] < 1)
+                raise(NSP_E_INVALID, "Projection size " + std::to_string(i) + " is not positive (" +
+                                         std::to_string(projector->screen_size[i]) + ")");
+            if (projector->screen_size[i] > 32767) raise(NSP_E_INVALID, "screen size exceeds int_least16_t pixels");
+            pixels *= projector->screen_size[i];
+        }
+        if (!(projector->units_per_pixel > 0)) raise(NSP_E_INVALID, "Error units per pixel for parallel projection");
+        // unit_dir_and_axes + check_vectors_orthogonal, projector.h:30-86
+        auto len = [n](const double* v) {
+            double r = v[0] * v[0];
+            for (int i = 1; i < n; ++i) r = std::fma(v[i], v[i], r);
+            return std::sqrt(r);
+        };
+        double cd[NSP_MAX_DIM], ax[NSP_MAX_DIM - 1][NSP_MAX_DIM];
+        double l0 = len(projector->camera_dir);
+        for (int i = 0; i < n; ++i) cd[i] = projector->camera_dir[i] / l0;
+        for (int a = 0; a < n - 1; ++a) {
+            double la = len(projector->screen_axes[a]);
+            for (int i = 0; i < n; ++i) ax[a][i] = projector->screen_axes[a][i] / la;
+        }
+        const double LIMIT_COS = 2.220446049250313e-16 * 100;
+        auto dotn = [n](const double* a, const double* b) {
+            double r = a[0] * b[0];
+            for (int i = 1; i < n; ++i) r = std::fma(a[i], b[i], r);
+            return r;
+        };
+        for (int a = 0; a < n - 1; ++a) {
+            if (!(std::fabs(dotn(ax[a], cd)) <= LIMIT_COS))
+                raise(NSP_E_INVALID, "The screen axis " + std::to_string(a) + " is not orthogonal to the camera direction");
+            for (int b = a + 1; b < n - 1; ++b)
+                if (!(std::fabs(dotn(ax[a], ax[b])) <= LIMIT_COS))
+                    raise(NSP_E_INVALID, "The screen axis " + std::to_string(a) + " is not orthogonal to the screen axes " +
+                                             std::to_string(b));
+        }
+        auto sc = std::make_unique<nsp_scene>();
+        sc->ctx = ctx;
+        sc->mesh = mesh;
+        sc->n = n;
+        sc->object = *object;
+        sc->lights.assign(lights, lights + light_count);
+        sc->proj = *projector;
+        sc->background = *default_surface;
+        sc->pixels = pixels;
+        sc->pix_sum.alloc(sizeof(float) * 3 * pixels);
+        sc->pix_cnt.alloc(sizeof(int32_t) * pixels);
+        NSP_HIP(hipMemsetAsync(sc->pix_sum.p, 0, sizeof(float) * 3 * pixels, ctx->stream));
+        NSP_HIP(hipMemsetAsync(sc->pix_cnt.p, 0, sizeof(int32_t) * pixels, ctx->stream));
+        sc->totals.alloc(sizeof(unsigned long long) * 3);
+        NSP_HIP(hipStreamSynchronize(ctx->stream));
+        *out = sc.release();
+    });
+}
+
+void nsp_scene_destroy(nsp_scene* scene)
+{
+    if (!scene) return;
+    (void)hipSetDevice(scene->ctx->device);
+    (void)hipStreamSynchronize(scene->ctx->stream);
+    delete scene;
+}
+
+int64_t nsp_scene_pixel_count(const nsp_scene* scene) { return scene ? scene->pixels : NSP_E_INVALID; }
+
+int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callbacks* cb, const volatile int32_t* stop)
+{
+    if (!scene || !params) return NSP_E_INVALID;
+    nsp_context* ctx = scene->ctx;
+    int rc = guarded(ctx, [&] {
+        const nsp_paint_params& p = *params;
+        if (p.max_depth < 0 || p.max_depth > 250) raise(NSP_E_INVALID, "max_depth must be in [0, 250]");
+        if (p.passes < 1) raise(NSP_E_INVALID, "passes must be >= 1");
+        if (p.precision != NSP_PRECISION_FP32 && p.precision != NSP_PRECISION_FP64)
+            raise(NSP_E_INVALID, "unknown precision");
+        launch_gate(ctx);
+        const int n = scene->n;
+        hipStream_t s = ctx->stream;
+        SceneK<float> kf;
+        SceneK<double> kd;
+        fill_scene<float>(*scene, p, 0, kf);
+        fill_scene<double>(*scene, p, 0, kd);
+        const int S = kf.S;
+        int64_t want = p.batch_paths > 0 ? p.batch_paths : (p.precision ? ((int64_t)1 << 22) : ((int64_t)1 << 23));
+        int64_t batch_pixels = std::max<int64_t>(1, want / S);
+        batch_pixels = std::min<int64_t>(batch_pixels, scene->pixels);
+        const int64_t cap = batch_pixels * S;
+        alloc_render_buffers(scene->buf, n, p.precision, cap, p.max_depth);
+        for (int pass = 0; pass < p.passes; ++pass) {
+            if (stop && *stop) raise(NSP_E_STOPPED, "stopped");
+            kf.pass = kd.pass = (int)scene->pass_count + pass;
+            auto t0 = std::chrono::steady_clock::now();
+            scene->marks.reset();
+            NSP_HIP(hipMemsetAsync(scene->totals.p, 0, sizeof(unsigned long long) * 3, s));
+            if (scene->counters)
+                NSP_HIP(hipMemsetAsync(scene->buf.trav_counters.p, 0, sizeof(unsigned long long) * 8, s));
+            std::vector<BatchDesc> descs;
+            for (int64_t px = 0; px < scene->pixels; px += batch_pixels)
+                descs.push_back(BatchDesc{px, std::min(batch_pixels, scene->pixels - px) * S});
+            std::vector<unsigned long long> batch_cnt((size_t)CNT_STRIDE * (p.max_depth + 2));
+            unsigned long long closest = 0, shadow = 0, probes = 0;
+            for (size_t b = 0; b < descs.size(); ++b) {
+                if (stop && *stop) break;
+                NSP_HIP(hipMemcpyAsync(scene->buf.desc.p, &descs[b], sizeof(BatchDesc), hipMemcpyHostToDevice, s));
+                RenderLaunch rl;
+                rl.mesh = &scene->mesh->m;
+                rl.scene = p.precision ? (const void*)&kd : (const void*)&kf;
+                rl.buf = &scene->buf;
+                rl.pix_sum = scene->pix_sum.as<float>();
+                rl.pix_cnt = scene->pix_cnt.as<int32_t>();
+                rl.pixels = scene->pixels;
+                rl.max_depth = p.max_depth;
+                rl.counters = scene->counters;
+                rl.marks = &scene->marks;
+                render_batch(rl, s);
+                NSP_HIP(hipMemcpyAsync(batch_cnt.data(), scene->buf.counters.p,
+                                       sizeof(unsigned long long) * batch_cnt.size(), hipMemcpyDeviceToHost, s));
+                NSP_HIP(hipStreamSynchronize(s));
+                for (int d = 0; d <= p.max_depth; ++d) {
+                    closest += batch_cnt[d * CNT_STRIDE + 0];
+                    shadow += batch_cnt[d * CNT_STRIDE + 1];
+                    probes += batch_cnt[d * CNT_STRIDE + 2];
+                }
+            }
+            NSP_HIP(hipStreamSynchronize(s));
+            auto t1 = std::chrono::steady_clock::now();
+            double secs = std::chrono::duration<double>(t1 - t0).count();
+            // stage timing
+            nsp_stage_stats st{};
+            for (auto& [stage, e0, e1] : scene->marks.marks) {
+                float ms = 0;
+                NSP_HIP(hipEventElapsedTime(&ms, e0, e1));
+                switch (stage) {
+                case 0: st.ms_raygen += ms; break;
+                case 1: st.ms_closest += ms; ++st.launches_closest; break;
+                case 2: st.ms_shade += ms; break;
+                case 3: st.ms_shadow += ms; ++st.launches_shadow; break;
+                case 4: st.ms_accumulate += ms; break;
+                }
+            }
+            st.ms_total = secs * 1e3;
+            st.closest_rays = (int64_t)closest;
+            st.shadow_rays = (int64_t)shadow;
+            if (scene->counters) {
+                unsigned long long tc[8];
+                NSP_HIP(hipMemcpy(tc, scene->buf.trav_counters.p, sizeof tc, hipMemcpyDeviceToHost));
+                st.node_visits = (int64_t)tc[0];
+                st.leaf_visits = (int64_t)tc[1];
+                st.simplex_tests = (int64_t)tc[2];
+                st.retests = (int64_t)tc[3];
+                st.shadow_node_visits = (int64_t)tc[4];
+                st.shadow_leaf_visits = (int64_t)tc[5];
+                st.shadow_simplex_tests = (int64_t)tc[6];
+                st.shadow_retests = (int64_t)tc[7];
+            }
+            scene->stage = st;
+            bool stopped = stop && *stop;
+            scene->pass_count += 1;
+            scene->pixel_count += scene->pixels;
+            scene->ray_count += (int64_t)(closest + shadow + probes);
+            scene->sample_count += scene->pixels * S;
+            scene->last_pass_s = secs;
+            if (cb && cb->pass_done) cb->pass_done(cb->user, (int)scene->pass_count, secs);
+            if (cb && cb->pixels_after) {
+                const int64_t P = scene->pixels;
+                std::vector<float> sum((size_t)3 * P);
+                std::vector<int32_t> cnt(P);
+                NSP_HIP(hipMemcpy(sum.data(), scene->pix_sum.p, sizeof(float) * 3 * P, hipMemcpyDeviceToHost));
+                NSP_HIP(hipMemcpy(cnt.data(), scene->pix_cnt.p, sizeof(int32_t) * P, hipMemcpyDeviceToHost));
+                const int64_t CH = 65536;
+                std::vector<float> mean((size_t)3 * CH);
+                for (int64_t b = 0; b < P; b += CH) {
+                    int64_t c = std::min(CH, P - b);
+                    for (int64_t j = 0; j < c; ++j) {
+                        int32_t k = std::max(1, cnt[b + j]);
+                        for (int ch = 0; ch < 3; ++ch) mean[j * 3 + ch] = sum[(size_t)ch * P + b + j] / k;
+                    }
+                    cb->pixels_after(cb->user, b, c, mean.data());
+                }
+            }
+            if (stopped) raise(NSP_E_STOPPED, "stopped");
+        }
+    });
+    if (rc != NSP_OK && rc != NSP_E_STOPPED && cb && cb->error_message)
+        cb->error_message(cb->user, (std::string("Painter error:\n") + ctx->error).c_str());
+    return rc;
+}
+
+int nsp_statistics(const nsp_scene* scene, int64_t* pass_count, int64_t* pixel_count, int64_t* ray_count,
+                   int64_t* sample_count, double* previous_pass_duration)
+{
+    if (!scene) return NSP_E_INVALID;
+    if (pass_count) *pass_count = scene->pass_count;
+    if (pixel_count) *pixel_count = scene->pixel_count;
+    if (ray_count) *ray_count = scene->ray_count;
+    if (sample_count) *sample_count = scene->sample_count;
+    if (previous_pass_duration) *previous_pass_duration = scene->last_pass_s;
+    return NSP_OK;
+}
+
+int nsp_image_get(const nsp_scene* scene, float* rgb_sum, int32_t* samples)
+{
+    if (!scene) return NSP_E_INVALID;
+    return guarded(scene->ctx, [&] {
+        const int64_t P = scene->pixels;
+        if (rgb_sum) {
+            std::vector<float> s((size_t)3 * P);
+            NSP_HIP(hipMemcpy(s.data(), scene->pix_sum.p, sizeof(float) * 3 * P, hipMemcpyDeviceToHost));
+            for (int64_t j = 0; j < P; ++j)
+                for (int c = 0; c < 3; ++c) rgb_sum[j * 3 + c] = s[(size_t)c * P + j];
+        }
+        if (samples) NSP_HIP(hipMemcpy(samples, scene->pix_cnt.p, sizeof(int32_t) * P, hipMemcpyDeviceToHost));
+    });
+}
+
+int nsp_image_set(nsp_scene* scene, const float* rgb_sum, const int32_t* samples)
+{
+    if (!scene || !rgb_sum || !samples) return NSP_E_INVALID;
+    return guarded(scene->ctx, [&] {
+        const int64_t P = scene->pixels;
+        std::vector<float> s((size_t)3 * P);
+        for (int64_t j = 0; j < P; ++j)
+            for (int c = 0; c < 3; ++c) s[(size_t)c * P + j] = rgb_sum[j * 3 + c];
+        NSP_HIP(hipMemcpy(scene->pix_sum.p, s.data(), sizeof(float) * 3 * P, hipMemcpyHostToDevice));
+        NSP_HIP(hipMemcpy(scene->pix_cnt.p, samples, sizeof(int32_t) * P, hipMemcpyHostToDevice));
+    });
+}
+
+int nsp_last_pass_stats(const nsp_scene* scene, nsp_stage_stats* out)
+{
+    if (!scene || !out) return NSP_E_INVALID;
+    *out = scene->stage;
+    return NSP_OK;
+}
+
+int nsp_set_counters(nsp_scene* scene, int enable)
+{
+    if (!scene) return NSP_E_INVALID;
+    scene->counters = enable != 0;
+    return NSP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,632 @@
+// nsp_render.hip -- first-hit kernel and the wavefront path-tracing stages
+// (gfx950).  One FP32 (or FP64) ray state per lane; stages:
+//   raygen    paint_pixels' sample + ParallelProjector::ray (src/painter/painter.cpp:419-442,
+//             src/painter/sampling/engine.h:86-156, src/painter/projectors/projector.h:194-201)
+//   closest   Mesh::intersect_precise (src/painter/shapes/mesh.cpp:143-166) by tree traversal
+//   shade     trace_path's shading, direct_diffuse_lighting's shadow-ray set-up and
+//             diffuse_lighting's bounce (painter.cpp:197-397); shadow rays and bounce
+//             rays are appended to compacted queues with one atomic per wave
+//   shadow    light_source_is_visible (painter.cpp:142-186) as an any-hit traversal
+//   accumulate Pixels::add_color_and_samples (painter.cpp:57-88), samples summed in a
+//             fixed order so every image is bit-reproducible.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "nsp_device.hpp"
+#include "nsp_render.hpp"
+#include "nsp_trace.hpp"
+
+namespace nsp {
+
+static unsigned grid_for(int64_t n, int block = TRACE_BLOCK, int64_t max_blocks = 8192)
+{
+    int64_t b = (n + block - 1) / block;
+    b = std::max<int64_t>(1, std::min<int64_t>(b, max_blocks));
+    // multiple of 8 so the XCD remap is a permutation
+    b = (b + 7) / 8 * 8;
+    return (unsigned)b;
+}
+
+TreeArgs make_tree_args(const MeshDev& m)
+{
+    const TreeDev& t = m.tree;
+    TreeArgs a{};
+    a.nodes = t.nodes.as<int2>();
+    a.entries = t.entries.as<int32_t>();
+    a.soa = t.soa.as<float>();
+    a.pad = t.entry_pad;
+    a.rec64 = m.rec64.as<double>();
+    a.lmax = t.levels - 1;
+    double ext = 0;
+    for (int i = 0; i < m.n; ++i) {
+        a.lo[i] = (float)t.root_lo[i];
+        a.size[i] = (float)t.root_size[i];
+        a.inv_fine[i] = (float)((double)(1 << a.lmax) / t.root_size[i]);
+        a.lo64[i] = t.root_lo[i];
+        a.size64[i] = t.root_size[i];
+        a.inv_fine64[i] = (double)(1 << a.lmax) / t.root_size[i];
+        ext = std::max(ext, std::fabs(t.root_lo[i]) + t.root_size[i]);
+    }
+    a.tadv = (float)(ext * 2.0e-6);
+    a.tadv64 = ext * 1.0e-12;
+    a.counters = nullptr;
+    return a;
+}
+
+// ---------------------------------------------------------------------------
+// First hit
+
+template <int N>
+__global__ void __launch_bounds__(TRACE_BLOCK) k_first_hit(TreeArgs A, int64_t count, const float* __restrict__ org,
+                                                           const float* __restrict__ dir, int32_t* __restrict__ idx,
+                                                           double* __restrict__ tout, uint8_t* __restrict__ near)
+{
+    __shared__ int stack[MAX_LEVELS * TRACE_BLOCK];
+    int* st = stack + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x; i < count; i += stride) {
+        float o[N], d[N];
+        double o64[N], d64[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            o[k] = org[i * N + k];
+            d[k] = dir[i * N + k];
+            o64[k] = o[k];
+            d64[k] = d[k];
+        }
+        normalize<N, float>(d);
+        normalize<N, double>(d64);  // Ray normalises its direction, src/com/ray.h:34-36
+        HitResult h = traverse<N, float, false>(A, o, d, INFINITY, st, o64, d64);
+        idx[i] = h.idx;
+        tout[i] = h.idx >= 0 ? h.t : 0.0;
+        if (near) near[i] = h.near ? 1 : 0;
+    }
+}
+
+void launch_first_hit(const MeshDev& m, const float* org, const float* dir, int64_t count, int32_t* idx, double* t,
+                      uint8_t* near, unsigned long long* counters, hipStream_t s)
+{
+    TreeArgs a = make_tree_args(m);
+    a.counters = counters;
+    unsigned g = grid_for(count);
+    switch (m.n) {
+    case 3: hipLaunchKernelGGL(k_first_hit<3>, dim3(g), dim3(TRACE_BLOCK), 0, s, a, count, org, dir, idx, t, near); break;
+    case 4: hipLaunchKernelGGL(k_first_hit<4>, dim3(g), dim3(TRACE_BLOCK), 0, s, a, count, org, dir, idx, t, near); break;
+    case 5: hipLaunchKernelGGL(k_first_hit<5>, dim3(g), dim3(TRACE_BLOCK), 0, s, a, count, org, dir, idx, t, near); break;
+    case 6: hipLaunchKernelGGL(k_first_hit<6>, dim3(g), dim3(TRACE_BLOCK), 0, s, a, count, org, dir, idx, t, near); break;
+    default: raise(NSP_E_INVALID, "dimension not supported");
+    }
+    NSP_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Render stages
+
+__global__ void k_batch_reset(unsigned long long* cnt, int ncnt, const BatchDesc* desc)
+{
+    int i = threadIdx.x;
+    for (int k = i; k < ncnt; k += blockDim.x) cnt[k] = 0;
+    __syncthreads();
+    if (i == 0) cnt[0] = (unsigned long long)desc->paths;
+}
+
+template <int N, typename T>
+__global__ void __launch_bounds__(TRACE_BLOCK) k_raygen(SceneK<T> sc, const BatchDesc* __restrict__ desc,
+                                                        RayQueue<T> q, int64_t cap, float* __restrict__ L)
+{
+    const int64_t paths = desc->paths;
+    const int64_t pixel0 = desc->pixel0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)xcd_block() * blockDim.x + threadIdx.x; p < paths; p += stride) {
+        int64_t pixel = pixel0 + p / sc.S;
+        int sample = (int)(p % sc.S);
+        T u[8];
+        uniforms4<T>(sc.seed, pixel, sample, sc.pass, 0, 0, u);
+        if constexpr (N - 1 > 4) uniforms4<T>(sc.seed, pixel, sample, sc.pass, 0, 1, u + 4);
+        int64_t rem = pixel;
+        int sr = sample;
+        T sp[N - 1];
+#pragma unroll
+        for (int i = 0; i < N - 1; ++i) {
+            int64_t x = rem % sc.screen[i];
+            rem /= sc.screen[i];
+            int dgt = sr % sc.k;
+            sr /= sc.k;
+            T off = (T)dgt / (T)sc.k;
+            T spt = off + u[i] * sc.k_inv;
+            sp[i] = (-(T)sc.screen[i] * (T)0.5) + ((T)x + spt);
+        }
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            T sd = sc.axes[0][j] * sp[0];
+#pragma unroll
+            for (int i = 1; i < N - 1; ++i) sd = sd + sc.axes[i][j] * sp[i];
+            q.org[(int64_t)j * cap + p] = sc.cam_org[j] + sd;
+            q.dir[(int64_t)j * cap + p] = sc.cam_dir[j];
+        }
+        q.path[p] = (int32_t)p;
+        q.beta[p] = 1.0f;
+        q.beta[cap + p] = 1.0f;
+        q.beta[2 * cap + p] = 1.0f;
+        q.level[p] = 1.0f;
+        L[p] = 0.0f;
+        L[cap + p] = 0.0f;
+        L[2 * cap + p] = 0.0f;
+    }
+}
+
+template <int N, typename T>
+__global__ void __launch_bounds__(TRACE_BLOCK) k_closest(TreeArgs A, RayQueue<T> q, int64_t cap,
+                                                         const unsigned long long* __restrict__ count_in)
+{
+    __shared__ int stack[MAX_LEVELS * TRACE_BLOCK];
+    int* st = stack + threadIdx.x;
+    const int64_t count = (int64_t)*count_in;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x; i < count; i += stride) {
+        T o[N], d[N];
+        double o64[N], d64[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            o[k] = q.org[(int64_t)k * cap + i];
+            d[k] = q.dir[(int64_t)k * cap + i];
+            o64[k] = o[k];
+            d64[k] = d[k];
+        }
+        HitResult h = traverse<N, T, false>(A, o, d, (T)INFINITY, st, o64, d64);
+        q.hit[i] = h.idx;
+        q.hit_t[i] = (T)h.t;
+    }
+}
+
+template <int N, typename T>
+__device__ __forceinline__ void load_normal(const MeshShadeArgs& M, int idx, T* gn)
+{
+    if constexpr (sizeof(T) == 4) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) gn[k] = M.normal32[(int64_t)idx * N + k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < N; ++k) gn[k] = M.rec64[(int64_t)idx * rec64_stride(N) + k];
+    }
+}
+
+// MeshHyperplaneSimplex::shading_normal, src/painter/shapes/mesh_hyperplane_simplex.cpp:157-186
+template <int N, typename T>
+__device__ void shading_normal(const MeshShadeArgs& M, int idx, const T* p, const T* gn, T* sn)
+{
+    int nt = M.ntype[idx];
+    int type = nt & 3;
+    if (type == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) sn[k] = gn[k];
+        return;
+    }
+    const double* r = M.rec64 + (int64_t)idx * rec64_stride(N);
+    double pd[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) pd[k] = p[k];
+    double bc[N];
+    bc[N - 1] = 1;
+#pragma unroll
+    for (int i = 0; i < N - 1; ++i) {
+        bc[i] = dot<N, double>(pd, r + 2 * N + i * N) - r[2 * N + (N - 1) * N + i];
+        bc[N - 1] -= bc[i];
+    }
+    double acc[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const double* vn = M.vn64 + (int64_t)M.fnrm[(int64_t)idx * N + k] * N;
+        double sgn = (type == 2 && ((nt >> (8 + k)) & 1)) ? -1.0 : 1.0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            double term = bc[k] * (sgn * vn[i]);
+            acc[i] = (k == 0) ? term : acc[i] + term;
+        }
+    }
+    normalize<N, double>(acc);
+#pragma unroll
+    for (int k = 0; k < N; ++k) sn[k] = (T)acc[k];
+    normalize<N, T>(sn);
+}
+
+template <int N, typename T>
+__global__ void __launch_bounds__(TRACE_BLOCK)
+    k_shade(SceneK<T> sc, TreeArgs A, MeshShadeArgs M, const BatchDesc* __restrict__ desc, int depth,
+            RayQueue<T> qin, RayQueue<T> qout, ShadowQueue<T> sq, int64_t cap, unsigned long long* __restrict__ cnt,
+            float* __restrict__ L)
+{
+    __shared__ int stack[MAX_LEVELS * TRACE_BLOCK];
+    int* st = stack + threadIdx.x;
+    const int64_t count = (int64_t)cnt[depth * CNT_STRIDE + 0];
+    unsigned long long* next_count = &cnt[(depth + 1) * CNT_STRIDE + 0];
+    unsigned long long* shadow_count = &cnt[depth * CNT_STRIDE + 1];
+    unsigned long long* probe_count = &cnt[depth * CNT_STRIDE + 2];
+    const int64_t pixel0 = desc->pixel0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t base = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+    // every lane of a wave iterates the same number of times (wave-level appends)
+    const int64_t iters = (count + stride - 1) / stride;
+    for (int64_t it = 0; it < iters; ++it) {
+        const int64_t i = base + it * stride;
+        const bool valid = i < count;
+        bool want_next = false;
+        bool want_shadow[MAX_LIGHTS] = {false};
+        T p[N], nd[N];
+        T sh_org[MAX_LIGHTS][N], sh_dir[MAX_LIGHTS][N], sh_tmax[MAX_LIGHTS];
+        float sh_c[MAX_LIGHTS][3];
+        float nb[3] = {0, 0, 0};
+        float new_level = 0;
+        int32_t path = 0;
+        if (valid) {
+            path = qin.path[i];
+            float beta[3] = {qin.beta[i], qin.beta[cap + i], qin.beta[2 * cap + i]};
+            int idx = qin.hit[i];
+            T o[N], d[N];
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                o[k] = qin.org[(int64_t)k * cap + i];
+                d[k] = qin.dir[(int64_t)k * cap + i];
+            }
+            const bool diffuse_reflection = depth > 0;
+            if (idx < 0) {
+                // miss: background, a light source when reached by a diffuse bounce (painter.cpp:335-340)
+                const float* bg = (sc.bg_is_light && diffuse_reflection) ? sc.bg_light_rgb : sc.bg_rgb;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) L[c * cap + path] += beta[c] * bg[c];
+            } else {
+                T t = qin.hit_t[i];
+#pragma unroll
+                for (int k = 0; k < N; ++k) p[k] = o[k] + d[k] * t;
+                T gn[N], sn[N];
+                load_normal<N, T>(M, idx, gn);
+                T dgn = dot<N, T>(d, gn);
+                bool go = !(fabs(dgn) <= (T)0);
+                if (go && sc.obj_is_light) {
+                    const float* lc = diffuse_reflection ? sc.obj_light_rgb : sc.obj_rgb;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) L[c * cap + path] += beta[c] * lc[c];
+                    go = false;
+                }
+                if (go) {
+                    if (sc.smooth && sc.has_normals)
+                        shading_normal<N, T>(M, idx, p, gn, sn);
+                    else {
+#pragma unroll
+                        for (int k = 0; k < N; ++k) sn[k] = gn[k];
+                    }
+                    if (dgn > 0) {
+#pragma unroll
+                        for (int k = 0; k < N; ++k) {
+                            gn[k] = -gn[k];
+                            sn[k] = -sn[k];
+                        }
+                    }
+                    go = sc.diffuse > 0;
+                }
+                if (go) {
+                    float scol[3] = {sc.diffuse * sc.obj_rgb[0], sc.diffuse * sc.obj_rgb[1],
+                                     sc.diffuse * sc.obj_rgb[2]};
+                    float mx = fmaxf(scol[0], fmaxf(scol[1], scol[2]));
+                    new_level = qin.level[i] * mx;
+                    if (new_level >= 1e-4f) {
+                        // direct_diffuse_lighting, painter.cpp:197-292
+                        for (int l = 0; l < sc.light_count; ++l) {
+                            T vl[N];
+#pragma unroll
+                            for (int k = 0; k < N; ++k) vl[k] = sc.light_pos[l][k] - p[k];
+                            T sq2 = dot<N, T>(vl, vl);
+                            T scale;
+                            if (sc.light_kind[l] == 1)
+                                scale = (T)1;
+                            else if constexpr (N == 3)
+                                scale = sc.light_coef[l] / sq2;
+                            else
+                                scale = sc.light_coef[l] / pow(sq2, (T)(N - 1) / (T)2);
+                            float lc[3];
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) lc[c] = sc.light_rgb[l][c] * (float)scale;
+                            if (fmaxf(lc[0], fmaxf(lc[1], lc[2])) < 1e-4f) continue;
+                            T ld[N];
+#pragma unroll
+                            for (int k = 0; k < N; ++k) ld[k] = vl[k];
+                            normalize<N, T>(ld);
+                            T dln = dot<N, T>(ld, sn);
+                            if (dln <= (T)0) continue;
+                            T dist = sqrt(dot<N, T>(vl, vl));
+                            T o1[N];
+#pragma unroll
+                            for (int k = 0; k < N; ++k) o1[k] = p[k] + ld[k] * sc.ray_offset;
+                            T tmaxs = dist;
+                            if (sc.smooth && dot<N, T>(ld, gn) < (T)0) {
+                                // smooth-normal back face: forward and reverse probes (painter.cpp:238-284)
+                                atomicAdd(probe_count, 1ull);
+                                double o64[N], d64[N];
+#pragma unroll
+                                for (int k = 0; k < N; ++k) {
+                                    o64[k] = o1[k];
+                                    d64[k] = ld[k];
+                                }
+                                HitResult h1 = traverse<N, T, false>(A, o1, ld, (T)INFINITY, st, o64, d64);
+                                if (h1.idx < 0) continue;
+                                T tf = (T)h1.t;
+                                if (tf >= dist) continue;
+                                atomicAdd(probe_count, 1ull);
+                                T rd[N], ro[N];
+#pragma unroll
+                                for (int k = 0; k < N; ++k) {
+                                    rd[k] = -ld[k];
+                                    ro[k] = o1[k] + rd[k] * ((T)2 * sc.ray_offset);
+                                    o64[k] = ro[k];
+                                    d64[k] = rd[k];
+                                }
+                                HitResult h2 = traverse<N, T, false>(A, ro, rd, (T)INFINITY, st, o64, d64);
+                                if (h2.idx >= 0 && (T)h2.t < tf) continue;
+#pragma unroll
+                                for (int k = 0; k < N; ++k) o1[k] = o1[k] + ld[k] * (tf + sc.ray_offset);
+                                tmaxs = dist - tf;
+                            }
+                            float w = (float)(sc.cos_coef * dln);
+                            want_shadow[l] = true;
+#pragma unroll
+                            for (int k = 0; k < N; ++k) {
+                                sh_org[l][k] = o1[k];
+                                sh_dir[l][k] = ld[k];
+                            }
+                            sh_tmax[l] = tmaxs;
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) sh_c[l][c] = beta[c] * (scol[c] * (lc[c] * w));
+                        }
+                        // diffuse_lighting, painter.cpp:298-323
+                        if (depth < sc.max_depth) {
+                            int64_t gp = (pixel0 * sc.S) + path;
+                            int64_t pixel = gp / sc.S;
+                            int sample = (int)(gp % sc.S);
+                            T u[8];
+                            uniforms4<T>(sc.seed, pixel, sample, sc.pass, depth, 2, u);
+                            if constexpr (N >= 6) uniforms4<T>(sc.seed, pixel, sample, sc.pass, depth, 3, u + 4);
+                            cosine_hemisphere<N, T>(sn, u, nd);
+                            normalize<N, T>(nd);
+                            if (dot<N, T>(nd, gn) > (T)0) {
+                                want_next = true;
+#pragma unroll
+                                for (int c = 0; c < 3; ++c) nb[c] = beta[c] * scol[c];
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        // compaction: one atomic per wave per queue
+        int64_t slot = wave_append(want_next, next_count);
+        if (slot >= 0) {
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                qout.org[(int64_t)k * cap + slot] = p[k] + nd[k] * sc.ray_offset;
+                qout.dir[(int64_t)k * cap + slot] = nd[k];
+            }
+            qout.path[slot] = path;
+            qout.beta[slot] = nb[0];
+            qout.beta[cap + slot] = nb[1];
+            qout.beta[2 * cap + slot] = nb[2];
+            qout.level[slot] = new_level;
+        }
+        for (int l = 0; l < MAX_LIGHTS; ++l) {
+            if (l >= sc.light_count) break;
+            int64_t ss = wave_append(want_shadow[l], shadow_count);
+            if (ss >= 0) {
+#pragma unroll
+                for (int k = 0; k < N; ++k) {
+                    sq.org[(int64_t)k * cap * MAX_LIGHTS + ss] = sh_org[l][k];
+                    sq.dir[(int64_t)k * cap * MAX_LIGHTS + ss] = sh_dir[l][k];
+                }
+                sq.tmax[ss] = sh_tmax[l];
+                sq.path[ss] = path;
+                sq.contrib[ss] = sh_c[l][0];
+                sq.contrib[cap * MAX_LIGHTS + ss] = sh_c[l][1];
+                sq.contrib[2 * cap * MAX_LIGHTS + ss] = sh_c[l][2];
+            }
+        }
+    }
+}
+
+// Shadow rays: any hit before the light.  Different shadow rays of one path
+// are applied in separate launches per light slot would be needed for
+// determinism with several lights; with one light per path and bounce each
+// path receives at most one contribution per launch.
+template <int N, typename T>
+__global__ void __launch_bounds__(TRACE_BLOCK) k_shadow(TreeArgs A, ShadowQueue<T> sq, int64_t scap,
+                                                        const unsigned long long* __restrict__ count_in,
+                                                        float* __restrict__ L, int64_t cap)
+{
+    __shared__ int stack[MAX_LEVELS * TRACE_BLOCK];
+    int* st = stack + threadIdx.x;
+    const int64_t count = (int64_t)*count_in;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x; i < count; i += stride) {
+        T o[N], d[N];
+        double o64[N], d64[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            o[k] = sq.org[(int64_t)k * scap + i];
+            d[k] = sq.dir[(int64_t)k * scap + i];
+            o64[k] = o[k];
+            d64[k] = d[k];
+        }
+        T tmax = sq.tmax[i];
+        HitResult h = traverse<N, T, true>(A, o, d, tmax, st, o64, d64);
+        if (h.idx < 0) {
+            int32_t path = sq.path[i];
+            L[path] += sq.contrib[i];
+            L[cap + path] += sq.contrib[scap + i];
+            L[2 * cap + path] += sq.contrib[2 * scap + i];
+        }
+    }
+}
+
+// Sum of the samples of each pixel in sample order; Pixels::add_color_and_samples.
+__global__ void __launch_bounds__(256) k_accumulate(const BatchDesc* __restrict__ desc, int S,
+                                                   const float* __restrict__ L, int64_t cap,
+                                                   float* __restrict__ pix_sum, int32_t* __restrict__ pix_cnt,
+                                                   int64_t pixels)
+{
+    const int64_t npix = desc->paths / S;
+    const int64_t pixel0 = desc->pixel0;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < npix; j += (int64_t)gridDim.x * blockDim.x) {
+        float s0 = 0, s1 = 0, s2 = 0;
+        for (int s = 0; s < S; ++s) {
+            int64_t p = j * S + s;
+            s0 += L[p];
+            s1 += L[cap + p];
+            s2 += L[2 * cap + p];
+        }
+        int64_t px = pixel0 + j;
+        pix_sum[px] += s0;
+        pix_sum[pixels + px] += s1;
+        pix_sum[2 * pixels + px] += s2;
+        pix_cnt[px] += S;
+    }
+}
+
+// ---------------------------------------------------------------------------
+
+void alloc_render_buffers(RenderBuffers& b, int n, int precision, int64_t cap, int max_depth)
+{
+    size_t ts = precision ? sizeof(double) : sizeof(float);
+    if (b.n == n && b.precision == precision && b.cap >= cap && b.depth_slots >= max_depth + 2) return;
+    b.n = n;
+    b.precision = precision;
+    b.cap = cap;
+    b.depth_slots = max_depth + 2;
+    for (DevBuf* q : {b.qa, b.qb}) {
+        q[0].alloc(ts * n * cap);
+        q[1].alloc(ts * n * cap);
+        q[2].alloc(sizeof(int32_t) * cap);
+        q[3].alloc(sizeof(float) * 3 * cap);
+        q[4].alloc(sizeof(float) * cap);
+        q[5].alloc(sizeof(int32_t) * cap);
+        q[6].alloc(ts * cap);
+    }
+    int64_t scap = cap * MAX_LIGHTS;
+    b.sq[0].alloc(ts * n * scap);
+    b.sq[1].alloc(ts * n * scap);
+    b.sq[2].alloc(ts * scap);
+    b.sq[3].alloc(sizeof(int32_t) * scap);
+    b.sq[4].alloc(sizeof(float) * 3 * scap);
+    b.L.alloc(sizeof(float) * 3 * cap);
+    b.counters.alloc(sizeof(unsigned long long) * CNT_STRIDE * (max_depth + 2));
+    b.desc.alloc(sizeof(BatchDesc));
+    b.trav_counters.alloc(sizeof(unsigned long long) * 8);
+}
+
+template <typename T>
+static RayQueue<T> queue_of(DevBuf* q)
+{
+    return RayQueue<T>{q[0].as<T>(), q[1].as<T>(), q[2].as<int32_t>(), q[3].as<float>(), q[4].as<float>(),
+                       q[5].as<int32_t>(), q[6].as<T>()};
+}
+
+struct StageEv {
+    hipEvent_t* ev;
+    int stage;
+};
+
+template <int N, typename T>
+static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
+{
+    RenderBuffers& b = *rl.buf;
+    const SceneK<T>& sc = *static_cast<const SceneK<T>*>(rl.scene);
+    const int64_t cap = b.cap;
+    const int64_t scap = cap * MAX_LIGHTS;
+    TreeArgs A = make_tree_args(*rl.mesh);
+    TreeArgs As = A;
+    if (rl.counters) {
+        A.counters = b.trav_counters.as<unsigned long long>();
+        As.counters = A.counters + 4;
+    }
+    MeshShadeArgs M{rl.mesh->normal32.as<float>(), rl.mesh->rec64.as<double>(),
+                    rl.mesh->has_normals ? rl.mesh->vn64.as<double>() : nullptr,
+                    rl.mesh->has_normals ? rl.mesh->fnrm.as<int32_t>() : nullptr, rl.mesh->ntype.as<int32_t>()};
+    RayQueue<T> qa = queue_of<T>(b.qa), qb = queue_of<T>(b.qb);
+    ShadowQueue<T> sq{b.sq[0].as<T>(), b.sq[1].as<T>(), b.sq[2].as<T>(), b.sq[3].as<int32_t>(), b.sq[4].as<float>()};
+    unsigned long long* cnt = b.counters.as<unsigned long long>();
+    const BatchDesc* desc = b.desc.as<BatchDesc>();
+    float* L = b.L.as<float>();
+    unsigned g = grid_for(cap);
+    auto timed = [&](int stage, auto&& launch) {
+        if (!rl.marks) {
+            launch();
+            NSP_HIP(hipGetLastError());
+            return;
+        }
+        hipEvent_t e0 = rl.marks->get(), e1 = rl.marks->get();
+        NSP_HIP(hipEventRecord(e0, s));
+        launch();
+        NSP_HIP(hipGetLastError());
+        NSP_HIP(hipEventRecord(e1, s));
+        rl.marks->marks.emplace_back(stage, e0, e1);
+    };
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(256), 0, s, cnt, CNT_STRIDE * (rl.max_depth + 2), desc);
+    NSP_HIP(hipGetLastError());
+    timed(0, [&] { hipLaunchKernelGGL((k_raygen<N, T>), dim3(g), dim3(TRACE_BLOCK), 0, s, sc, desc, qa, cap, L); });
+    for (int depth = 0; depth <= rl.max_depth; ++depth) {
+        RayQueue<T>& qin = (depth & 1) ? qb : qa;
+        RayQueue<T>& qout = (depth & 1) ? qa : qb;
+        timed(1, [&] {
+            hipLaunchKernelGGL((k_closest<N, T>), dim3(g), dim3(TRACE_BLOCK), 0, s, A, qin, cap,
+                               cnt + depth * CNT_STRIDE);
+        });
+        timed(2, [&] {
+            hipLaunchKernelGGL((k_shade<N, T>), dim3(g), dim3(TRACE_BLOCK), 0, s, sc, A, M, desc, depth, qin, qout,
+                               sq, cap, cnt, L);
+        });
+        timed(3, [&] {
+            hipLaunchKernelGGL((k_shadow<N, T>), dim3(grid_for(scap)), dim3(TRACE_BLOCK), 0, s, As, sq, scap,
+                               cnt + depth * CNT_STRIDE + 1, L, cap);
+        });
+    }
+    timed(4, [&] {
+        hipLaunchKernelGGL(k_accumulate, dim3(grid_for(cap / std::max(1, sc.S), 256, 4096)), dim3(256), 0, s, desc,
+                           sc.S, L, cap, rl.pix_sum, rl.pix_cnt, rl.pixels);
+    });
+}
+
+hipEvent_t StageMarks::get()
+{
+    if (used == pool.size()) {
+        hipEvent_t e;
+        NSP_HIP(hipEventCreate(&e));
+        pool.push_back(e);
+    }
+    return pool[used++];
+}
+
+StageMarks::~StageMarks()
+{
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+}
+
+void render_batch(const RenderLaunch& rl, hipStream_t s)
+{
+    const int n = rl.mesh->n;
+    const bool dbl = rl.buf->precision != 0;
+#define NSP_CASE(NN)                                 \
+    case NN:                                         \
+        if (dbl)                                     \
+            render_batch_t<NN, double>(rl, s);       \
+        else                                         \
+            render_batch_t<NN, float>(rl, s);        \
+        break;
+    switch (n) {
+        NSP_CASE(3)
+        NSP_CASE(4)
+        NSP_CASE(5)
+        NSP_CASE(6)
+    default: raise(NSP_E_INVALID, "dimension not supported");
+    }
+#undef NSP_CASE
+}
+
+}  // namespace nsp

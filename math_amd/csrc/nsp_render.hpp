@@ -1,0 +1,136 @@
+// nsp_render.hpp -- host/device interface of the trace and render kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <tuple>
+#include <vector>
+
+#include "nsp_internal.hpp"
+#include "nsp_trace.hpp"
+
+namespace nsp {
+
+constexpr int MAX_LIGHTS = 4;
+
+// Scene constants passed by value to the render kernels (the data of
+// PaintObjects: src/painter/objects.h:212-223, with the derived constants of
+// paint_threads, src/painter/painter.cpp:560-575).
+template <typename T>
+struct SceneK {
+    T cam_org[NMAX];
+    T cam_dir[NMAX];
+    T axes[NMAX - 1][NMAX];  // unit screen axes times units_per_pixel
+    int screen[NMAX - 1];
+    int64_t pixels;
+    int k;        // samples per screen dimension
+    int S;        // samples per pixel = k^(n-1)
+    T k_inv;      // 1/k in T
+    int light_count;
+    int light_kind[MAX_LIGHTS];
+    T light_pos[MAX_LIGHTS][NMAX];
+    float light_rgb[MAX_LIGHTS][3];
+    T light_coef[MAX_LIGHTS];
+    float bg_rgb[3];
+    float bg_light_rgb[3];
+    int bg_is_light;
+    float obj_rgb[3];
+    float obj_light_rgb[3];
+    int obj_is_light;
+    float diffuse;
+    T ray_offset;
+    T cos_coef;   // cosine_sphere_coefficient(n)
+    int max_depth;
+    int smooth;
+    int has_normals;
+    uint64_t seed;
+    int pass;
+};
+
+struct BatchDesc {
+    int64_t pixel0;
+    int64_t paths;
+};
+
+// Ray-state queue (SoA, component-major: org[c * cap + i]).
+template <typename T>
+struct RayQueue {
+    T* org;
+    T* dir;
+    int32_t* path;
+    float* beta;   // 3 * cap
+    float* level;  // colour level (painter.cpp:370-373)
+    int32_t* hit;
+    T* hit_t;
+};
+
+template <typename T>
+struct ShadowQueue {
+    T* org;
+    T* dir;
+    T* tmax;
+    int32_t* path;
+    float* contrib;  // 3 * cap
+};
+
+struct MeshShadeArgs {
+    const float* normal32;
+    const double* rec64;
+    const double* vn64;
+    const int32_t* fnrm;
+    const int32_t* ntype;
+};
+
+// Per-depth counters: [d*4+0] closest-hit rays, [d*4+1] shadow rays, [d*4+2] probe rays.
+constexpr int CNT_STRIDE = 4;
+
+struct RenderBuffers {
+    int n = 0;
+    int precision = 0;
+    int64_t cap = 0;        // paths per batch
+    int depth_slots = 0;
+    DevBuf qa[8], qb[8], sq[6];
+    DevBuf L;               // 3 * cap floats
+    DevBuf counters;        // (max_depth + 2) * CNT_STRIDE u64
+    DevBuf desc;            // BatchDesc
+    DevBuf trav_counters;   // 4 u64
+};
+
+TreeArgs make_tree_args(const MeshDev& m);
+
+// First hit of a batch of rays (device pointers).
+void launch_first_hit(const MeshDev& m, const float* org, const float* dir, int64_t count, int32_t* idx, double* t,
+                      uint8_t* near, unsigned long long* counters, hipStream_t s);
+
+// HIP events around each stage launch (stage ids: 0 raygen, 1 closest,
+// 2 shade, 3 shadow, 4 accumulate).
+struct StageMarks {
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> marks;
+    hipEvent_t get();
+    void reset()
+    {
+        used = 0;
+        marks.clear();
+    }
+    ~StageMarks();
+};
+
+// One wavefront batch of the render (all kernels enqueued on s).
+struct RenderLaunch {
+    const MeshDev* mesh;
+    const void* scene;  // SceneK<T>*
+    RenderBuffers* buf;
+    float* pix_sum;     // 3 * pixels (component-major: sum[c * pixels + p])
+    int32_t* pix_cnt;
+    int64_t pixels;
+    int max_depth;
+    bool counters;
+    StageMarks* marks;  // may be null
+};
+void render_batch(const RenderLaunch& rl, hipStream_t s);
+void alloc_render_buffers(RenderBuffers& b, int n, int precision, int64_t cap, int max_depth);
+
+}  // namespace nsp

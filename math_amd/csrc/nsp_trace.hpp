@@ -1,0 +1,325 @@
+// nsp_trace.hpp -- 2^N-tree traversal for FP32 (with FP64 near-boundary
+// re-test) and FP64 rays, shared by the first-hit kernel and the wavefront
+// render stages.
+//
+// Replaces SpatialSubdivisionTree::trace_ray (src/painter/space/tree.h:534-590),
+// which re-descends from the root for every box it steps through and accepts a
+// leaf's nearest simplex only inside that leaf box.  Here the tree is a
+// hierarchy of regular grids, so a point's cell at every level follows from
+// its integer coordinates at the finest level Q = floor((p - lo) / cell_min).
+// Each lane keeps the node ids of its current root-to-leaf path in an LDS
+// stack (one int per level); after leaving a leaf it pops only the levels
+// whose cell differs (bit length of the XOR of old and new coordinates) and
+// descends from there.  Leaves are visited in ray order and the walk stops
+// once the next leaf starts beyond the nearest hit, so the result is the
+// nearest hit over all simplices with the reference's tie rule (smaller
+// index first, src/painter/space/ray_intersection.h:141-150).
+#pragma once
+
+#include "nsp_device.hpp"
+#include "nsp_internal.hpp"
+
+namespace nsp {
+
+constexpr int TRACE_BLOCK = 256;
+constexpr int MAX_LEVELS = 10;  // reference depth limit, src/painter/space/tree.h:428-431
+
+// Half-width of the barycentric band (in barycentric units, scaled by
+// 1 + 1/|cos|) inside which an FP32 simplex decision is re-taken in FP64, and
+// the |cos| below which an FP32 hit is re-taken in FP64 (near-parallel).
+constexpr float RETEST_BAND = 2.0e-4f;
+constexpr float PARALLEL_COS = 0.05f;
+
+struct TreeArgs {
+    const int2* nodes;
+    const int32_t* entries;
+    const float* soa;
+    int64_t pad;
+    const double* rec64;
+    float lo[NMAX], size[NMAX], inv_fine[NMAX];
+    double lo64[NMAX], size64[NMAX], inv_fine64[NMAX];
+    int lmax;
+    float tadv;
+    double tadv64;
+    unsigned long long* counters;  // node visits, leaf visits, simplex tests, retests (or null)
+};
+
+template <typename T>
+struct TreeBox;
+template <>
+struct TreeBox<float> {
+    __device__ static const float* lo(const TreeArgs& a) { return a.lo; }
+    __device__ static const float* size(const TreeArgs& a) { return a.size; }
+    __device__ static const float* inv(const TreeArgs& a) { return a.inv_fine; }
+    __device__ static float adv(const TreeArgs& a) { return a.tadv; }
+};
+template <>
+struct TreeBox<double> {
+    __device__ static const double* lo(const TreeArgs& a) { return a.lo64; }
+    __device__ static const double* size(const TreeArgs& a) { return a.size64; }
+    __device__ static const double* inv(const TreeArgs& a) { return a.inv_fine64; }
+    __device__ static double adv(const TreeArgs& a) { return a.tadv64; }
+};
+
+// FP64 simplex test exactly as the reference states it:
+// HyperplaneSimplexGeometry::intersect + plane_intersect
+// (src/painter/space/hyperplane_geometry.h:40-52, :181-204), no fused ops.
+template <int N>
+__device__ __attribute__((noinline)) bool simplex_test64(const double* __restrict__ r, const double* o,
+                                                         const double* d, double* t_out)
+{
+#pragma clang fp contract(off)
+    const double* nrm = r;
+    const double* v0 = r + N;
+    double s = dot<N, double>(nrm, d);
+    if (s == 0) return false;
+    double diff[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) diff[i] = v0[i] - o[i];
+    double t = dot<N, double>(diff, nrm) / s;
+    if (!(t > 0)) return false;
+    double p[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) p[i] = o[i] + d[i] * t;
+    double last = 1;
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < N - 1; ++i) {
+        double b = dot<N, double>(p, r + 2 * N + i * N) - r[2 * N + (N - 1) * N + i];
+        if (b <= 0 || b >= 1) ok = false;
+        last -= b;
+    }
+    if (!(last > 0)) ok = false;
+    *t_out = t;
+    return ok;
+}
+
+struct HitResult {
+    int idx;
+    double t;
+    bool near;
+};
+
+template <int N>
+__device__ __forceinline__ void load_vec_f(const float* __restrict__ soa, int64_t pad, int64_t e, int c0, float* v)
+{
+#pragma unroll
+    for (int q = 0; q < N; ++q) v[q] = soa[(int64_t)(c0 + q) * pad + e];
+}
+
+// Candidate test of one leaf entry.  Returns true for a hit with 0 < t < tlimit.
+template <int N, typename T>
+__device__ __forceinline__ bool candidate(const TreeArgs& A, int64_t e, const T* o, const T* d, const double* o64,
+                                          const double* d64, T tlimit, double* t_hit, int* idx, bool* near,
+                                          unsigned* retests)
+{
+    if constexpr (sizeof(T) == 8) {
+        int s = A.entries[e];
+        double t;
+        if (!simplex_test64<N>(A.rec64 + (int64_t)s * rec64_stride(N), o, d, &t)) return false;
+        if (!(t < tlimit)) return false;
+        *t_hit = t;
+        *idx = s;
+        return true;
+    } else {
+        const float* soa = A.soa;
+        const int64_t pad = A.pad;
+        float nrm[N];
+        load_vec_f<N>(soa, pad, e, 0, nrm);
+        float s = dot<N, float>(nrm, d);
+        if (s == 0.0f) return false;
+        float dd = soa[(int64_t)N * pad + e];
+        float no = dot<N, float>(nrm, o);
+        float t = (dd - no) / s;
+        float as = fabsf(s);
+        float band = RETEST_BAND * (1.0f + 1.0f / as);
+        // rounding bound of t; outside it the sign of t and the comparison
+        // with the limit are decided in FP32
+        float terr = 4.0e-7f * (fabsf(dd) + fabsf(no)) / as;
+        if (t < -terr) return false;
+        if (t > tlimit + terr) return false;
+        float p[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) p[i] = o[i] + d[i] * t;
+        float last = 1.0f, margin = 3.0e38f;
+        bool in = true;
+#pragma unroll
+        for (int i = 0; i < N - 1; ++i) {
+            float pn[N];
+            load_vec_f<N>(soa, pad, e, N + 1 + i * (N + 1), pn);
+            float b = dot<N, float>(p, pn) - soa[(int64_t)(N + 1 + i * (N + 1) + N) * pad + e];
+            in = in && (b > 0.0f) && (b < 1.0f);
+            last -= b;
+            margin = fminf(margin, b);
+            if (b < -band) return false;  // decided in FP32: outside by more than the band
+        }
+        in = in && (last > 0.0f);
+        margin = fminf(margin, last);
+        bool retest = fabsf(margin) < band || (in && as < PARALLEL_COS) || t < terr;
+        if (!retest) {
+            if (!in || !(t > 0.0f)) return false;
+            if (fabsf(t - tlimit) < terr) *near = true;  // near-tie with the current nearest hit
+            if (!(t < tlimit)) return false;
+            *t_hit = (double)t;
+            *idx = A.entries[e];
+            return true;
+        }
+        ++*retests;
+        *near = true;
+        int sidx = A.entries[e];
+        double t64;
+        if (!simplex_test64<N>(A.rec64 + (int64_t)sidx * rec64_stride(N), o64, d64, &t64)) return false;
+        if (!((float)t64 < tlimit)) return false;
+        *t_hit = t64;
+        *idx = sidx;
+        return true;
+    }
+}
+
+// Slab interval of the axis-aligned box [blo, blo + bsz] (ParallelotopeOrtho
+// semantics, src/painter/space/parallelotope_ortho.h:254-313, incl. s == 0).
+template <int N, typename T>
+__device__ __forceinline__ bool slab(const T* blo, const T* bsz, const T* o, const T* inv, const T* d, T* tn, T* tf)
+{
+    T a = -INFINITY, b = INFINITY;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        T lo = blo[i], hi = blo[i] + bsz[i];
+        if (d[i] == 0) {
+            if (o[i] < lo || o[i] > hi) return false;
+            continue;
+        }
+        T x0 = (lo - o[i]) * inv[i], x1 = (hi - o[i]) * inv[i];
+        a = fmax(a, fmin(x0, x1));
+        b = fmin(b, fmax(x0, x1));
+    }
+    *tn = a;
+    *tf = b;
+    return a <= b;
+}
+
+// Nearest hit (ANY=false) or any hit before tmax (ANY=true).
+// stack: this lane's LDS stack, stack[l * TRACE_BLOCK].
+// o64/d64: the same ray in FP64 for the near-boundary re-test (FP32 rays only).
+template <int N, typename T, bool ANY>
+__device__ HitResult traverse(const TreeArgs& A, const T* o, const T* d, T tmax, int* stack, const double* o64,
+                              const double* d64)
+{
+    HitResult res{-1, 0.0, false};
+    T inv[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) inv[i] = (T)1 / d[i];
+    const T* rlo = TreeBox<T>::lo(A);
+    const T* rsz = TreeBox<T>::size(A);
+    const T* rinv = TreeBox<T>::inv(A);
+    T t_in, t_out;
+    if (!slab<N, T>(rlo, rsz, o, inv, d, &t_in, &t_out)) return res;
+    if (t_out < 0) return res;
+    T t_cur = fmax(t_in, (T)0);
+    T t_end = fmin(t_out, tmax);
+    const T adv0 = TreeBox<T>::adv(A);
+    T adv = adv0;
+    const int lmax = A.lmax;
+    const int R = 1 << lmax;
+    double best_t = (double)tmax;
+    int best = -1;
+    int level = -1;
+    int q[N];
+    int last_leaf = -1;
+    unsigned nvis = 0, lvis = 0, ntest = 0, nret = 0;
+    for (int iter = 0; iter < 65536; ++iter) {
+        if (t_cur > t_end) break;
+        if (best >= 0 && (T)best_t < t_cur - adv0 * 4) break;
+        T tp = t_cur + adv;
+        int Q[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            T x = floor((o[i] + d[i] * tp - rlo[i]) * rinv[i]);
+            int qi = (int)fmin(fmax(x, (T)0), (T)(R - 1));
+            Q[i] = qi;
+        }
+        int node;
+        if (level < 0) {
+            level = 0;
+            node = 0;
+            stack[0] = 0;
+        } else {
+            unsigned diff = 0;
+#pragma unroll
+            for (int i = 0; i < N; ++i) diff |= (unsigned)((Q[i] >> (lmax - level)) ^ q[i]);
+            int pops = diff ? 32 - __clz(diff) : 0;
+            level -= pops;
+            node = stack[level * TRACE_BLOCK];
+        }
+        int2 nd = A.nodes[node];
+        ++nvis;
+        while (nd.y < 0) {
+            int shift = lmax - level - 1;
+            int code = 0;
+#pragma unroll
+            for (int i = 0; i < N; ++i) code |= ((Q[i] >> shift) & 1) << i;
+            node = nd.x + code * (-nd.y);
+            ++level;
+            stack[level * TRACE_BLOCK] = node;
+            nd = A.nodes[node];
+            ++nvis;
+        }
+#pragma unroll
+        for (int i = 0; i < N; ++i) q[i] = Q[i] >> (lmax - level);
+        // leaf box interval
+        T blo[N], bsz[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            T cell = ldexp(rsz[i], -level);
+            bsz[i] = cell;
+            blo[i] = rlo[i] + (T)q[i] * cell;
+        }
+        T ta, tb;
+        if (!slab<N, T>(blo, bsz, o, inv, d, &ta, &tb)) tb = tp;  // rounding: point located outside its own box
+        if (node != last_leaf && nd.y > 0) {
+            ++lvis;
+            for (int64_t e = nd.x; e < (int64_t)nd.x + nd.y; ++e) {
+                ++ntest;
+                double th;
+                int si;
+                bool nr = false;
+                T lim = ANY ? tmax : (T)best_t;
+                if (candidate<N, T>(A, e, o, d, o64, d64, lim, &th, &si, &nr, &nret)) {
+                    if (ANY) {
+                        res.idx = si;
+                        res.t = th;
+                        res.near = res.near || nr;
+                        best = si;
+                        goto done;
+                    }
+                    if (th < best_t || (th == best_t && si < best)) {
+                        best_t = th;
+                        best = si;
+                    }
+                }
+                res.near = res.near || nr;
+            }
+        }
+        last_leaf = node;
+        if (tb > t_cur) {
+            t_cur = tb;
+            adv = adv0;
+        } else {
+            adv = adv * 2;  // no progress: the located cell ended before t_cur (rounding)
+        }
+    }
+    if (!ANY && best >= 0) {
+        res.idx = best;
+        res.t = best_t;
+    }
+done:
+    if (A.counters) {
+        atomicAdd(&A.counters[0], (unsigned long long)nvis);
+        atomicAdd(&A.counters[1], (unsigned long long)lvis);
+        atomicAdd(&A.counters[2], (unsigned long long)ntest);
+        atomicAdd(&A.counters[3], (unsigned long long)nret);
+    }
+    return res;
+}
+
+}  // namespace nsp

@@ -1,0 +1,364 @@
+"""ctypes binding of libnsp.so (include/nsp.h) for tests and benchmarks.
+
+The library must be built (math_amd/build.py); loading fails loudly otherwise.
+There is no CPU fallback: every call here runs the HIP path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnsp.so")
+MAX_DIM = 6
+
+NSP_OK = 0
+NSP_E_INVALID = -1
+NSP_E_HIP = -2
+NSP_E_NOMEM = -3
+NSP_E_STOPPED = -4
+NSP_E_UNSUPPORTED = -5
+
+PRECISION_FP32 = 0
+PRECISION_FP64 = 1
+LIGHT_POINT = 0
+LIGHT_CONSTANT = 1
+
+
+class NspError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"nsp error {code}: {msg}")
+        self.code = code
+
+
+class Light(C.Structure):
+    _fields_ = [("kind", C.c_int), ("position", C.c_double * MAX_DIM), ("rgb", C.c_float * 3),
+                ("unit_intensity_distance", C.c_double)]
+
+
+class Projector(C.Structure):
+    _fields_ = [("kind", C.c_int), ("camera_org", C.c_double * MAX_DIM), ("camera_dir", C.c_double * MAX_DIM),
+                ("screen_axes", (C.c_double * MAX_DIM) * (MAX_DIM - 1)), ("units_per_pixel", C.c_double),
+                ("screen_size", C.c_int * (MAX_DIM - 1))]
+
+
+class Surface(C.Structure):
+    _fields_ = [("rgb", C.c_float * 3), ("diffuse", C.c_float), ("light_source", C.c_int),
+                ("light_rgb", C.c_float * 3)]
+
+
+class PaintParams(C.Structure):
+    _fields_ = [("samples_per_pixel", C.c_int), ("max_depth", C.c_int), ("smooth_normal", C.c_int),
+                ("precision", C.c_int), ("passes", C.c_int), ("seed", C.c_uint64), ("batch_paths", C.c_int64),
+                ("use_graph", C.c_int)]
+
+
+PIXELS_AFTER = C.CFUNCTYPE(None, C.c_void_p, C.c_int64, C.c_int64, C.POINTER(C.c_float))
+PASS_DONE = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_double)
+ERROR_MSG = C.CFUNCTYPE(None, C.c_void_p, C.c_char_p)
+
+
+class Callbacks(C.Structure):
+    _fields_ = [("pixels_after", PIXELS_AFTER), ("pass_done", PASS_DONE), ("error_message", ERROR_MSG),
+                ("user", C.c_void_p)]
+
+
+class MeshInfo(C.Structure):
+    _fields_ = [("n", C.c_int), ("facets", C.c_int64), ("tree_nodes", C.c_int64), ("tree_leaves", C.c_int64),
+                ("tree_entries", C.c_int64), ("tree_levels", C.c_int), ("basis_ms", C.c_double),
+                ("tree_ms", C.c_double), ("root_lo", C.c_double * MAX_DIM), ("root_size", C.c_double * MAX_DIM)]
+
+
+class StageStats(C.Structure):
+    _fields_ = [("ms_raygen", C.c_double), ("ms_closest", C.c_double), ("ms_shade", C.c_double),
+                ("ms_shadow", C.c_double), ("ms_accumulate", C.c_double), ("ms_total", C.c_double),
+                ("launches_closest", C.c_int64), ("launches_shadow", C.c_int64), ("closest_rays", C.c_int64),
+                ("shadow_rays", C.c_int64), ("node_visits", C.c_int64), ("leaf_visits", C.c_int64),
+                ("simplex_tests", C.c_int64), ("retests", C.c_int64), ("shadow_node_visits", C.c_int64),
+                ("shadow_leaf_visits", C.c_int64), ("shadow_simplex_tests", C.c_int64), ("shadow_retests", C.c_int64)]
+
+
+# every symbol of include/nsp.h
+EXPORTS = [
+    "nsp_context_create", "nsp_context_destroy", "nsp_last_error", "nsp_debug_fail_next_launch",
+    "nsp_mesh_create", "nsp_mesh_destroy", "nsp_mesh_min_max", "nsp_mesh_get_info", "nsp_mesh_records",
+    "nsp_mesh_tree_dump", "nsp_trace_first_hit", "nsp_trace_first_hit_device", "nsp_scene_create",
+    "nsp_scene_destroy", "nsp_paint", "nsp_statistics", "nsp_image_get", "nsp_image_set",
+    "nsp_scene_pixel_count", "nsp_last_pass_stats", "nsp_set_counters", "nsp_meshgen_torus",
+    "nsp_meshgen_cube_sphere",
+]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run math_amd/build.py (or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        fp = C.POINTER(C.c_float)
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int32)
+        i64p = C.POINTER(C.c_int64)
+        L.nsp_context_create.argtypes = [C.c_int, C.POINTER(P)]
+        L.nsp_context_destroy.argtypes = [P]
+        L.nsp_context_destroy.restype = None
+        L.nsp_last_error.argtypes = [P]
+        L.nsp_last_error.restype = C.c_char_p
+        L.nsp_debug_fail_next_launch.argtypes = [P]
+        L.nsp_mesh_create.argtypes = [P, C.c_int, fp, C.c_int64, ip, C.c_int64, fp, C.c_int64, ip, dp, C.POINTER(P)]
+        L.nsp_mesh_destroy.argtypes = [P]
+        L.nsp_mesh_destroy.restype = None
+        L.nsp_mesh_min_max.argtypes = [P, dp, dp]
+        L.nsp_mesh_get_info.argtypes = [P, C.POINTER(MeshInfo)]
+        L.nsp_mesh_records.argtypes = [P, dp, dp, dp]
+        L.nsp_mesh_tree_dump.argtypes = [P, i64p, C.c_int64]
+        L.nsp_mesh_tree_dump.restype = C.c_int64
+        L.nsp_trace_first_hit.argtypes = [P, fp, fp, C.c_int64, ip, dp, C.POINTER(C.c_uint8)]
+        L.nsp_trace_first_hit_device.argtypes = [P, P, P, C.c_int64, P, P, P, P]
+        L.nsp_scene_create.argtypes = [P, C.c_int, P, C.POINTER(Surface), C.POINTER(Light), C.c_int,
+                                       C.POINTER(Projector), C.POINTER(Surface), C.POINTER(P)]
+        L.nsp_scene_destroy.argtypes = [P]
+        L.nsp_scene_destroy.restype = None
+        L.nsp_paint.argtypes = [P, C.POINTER(PaintParams), C.POINTER(Callbacks), C.POINTER(C.c_int32)]
+        L.nsp_statistics.argtypes = [P, i64p, i64p, i64p, i64p, dp]
+        L.nsp_image_get.argtypes = [P, fp, ip]
+        L.nsp_image_set.argtypes = [P, fp, ip]
+        L.nsp_scene_pixel_count.argtypes = [P]
+        L.nsp_scene_pixel_count.restype = C.c_int64
+        L.nsp_last_pass_stats.argtypes = [P, C.POINTER(StageStats)]
+        L.nsp_set_counters.argtypes = [P, C.c_int]
+        L.nsp_meshgen_torus.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_uint64, fp, i64p,
+                                        ip, i64p]
+        L.nsp_meshgen_cube_sphere.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_uint64, fp,
+                                              i64p, ip, i64p]
+        _lib = L
+    return _lib
+
+
+def _ptr(a, ctype):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+# ---------------------------------------------------------------------------
+# Mesh generators (host only)
+
+def meshgen_torus(R=1.0, r=0.35, nu=200, nv=100, noise=0.01, seed=0):
+    L = lib()
+    V, F = C.c_int64(), C.c_int64()
+    rc = L.nsp_meshgen_torus(R, r, nu, nv, noise, seed, None, C.byref(V), None, C.byref(F))
+    if rc:
+        raise NspError(rc, "meshgen_torus")
+    v = np.zeros((V.value, 3), np.float32)
+    f = np.zeros((F.value, 3), np.int32)
+    L.nsp_meshgen_torus(R, r, nu, nv, noise, seed, _ptr(v, C.c_float), C.byref(V), _ptr(f, C.c_int32), C.byref(F))
+    return v, f
+
+
+def meshgen_cube_sphere(n, g, sphere=True, sigma=0.0, cap_angle=0.0, seed=0):
+    L = lib()
+    V, F = C.c_int64(), C.c_int64()
+    rc = L.nsp_meshgen_cube_sphere(n, g, int(sphere), sigma, cap_angle, seed, None, C.byref(V), None, C.byref(F))
+    if rc:
+        raise NspError(rc, "meshgen_cube_sphere")
+    v = np.zeros((V.value, n), np.float32)
+    f = np.zeros((F.value, n), np.int32)
+    rc = L.nsp_meshgen_cube_sphere(n, g, int(sphere), sigma, cap_angle, seed, _ptr(v, C.c_float), C.byref(V),
+                                   _ptr(f, C.c_int32), C.byref(F))
+    if rc:
+        raise NspError(rc, "meshgen_cube_sphere")
+    return v, f
+
+
+# ---------------------------------------------------------------------------
+
+class Context:
+    def __init__(self, device=0):
+        self._L = lib()
+        h = C.c_void_p()
+        rc = self._L.nsp_context_create(device, C.byref(h))
+        if rc:
+            raise NspError(rc, self._L.nsp_last_error(None).decode())
+        self.h = h
+
+    def check(self, rc):
+        if rc != NSP_OK:
+            raise NspError(rc, self._L.nsp_last_error(self.h).decode())
+        return rc
+
+    def close(self):
+        if self.h:
+            self._L.nsp_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def mesh(self, vertices, facets, normals=None, facet_normals=None, matrix=None):
+        return Mesh(self, vertices, facets, normals, facet_normals, matrix)
+
+
+class Mesh:
+    def __init__(self, ctx, vertices, facets, normals=None, facet_normals=None, matrix=None):
+        self.ctx = ctx
+        L = ctx._L
+        v = np.ascontiguousarray(vertices, np.float32)
+        f = np.ascontiguousarray(facets, np.int32)
+        self.n = v.shape[1]
+        nrm = None if normals is None else np.ascontiguousarray(normals, np.float32)
+        fn = None if facet_normals is None else np.ascontiguousarray(facet_normals, np.int32)
+        mat = None if matrix is None else np.ascontiguousarray(matrix, np.float64)
+        h = C.c_void_p()
+        ctx.check(L.nsp_mesh_create(ctx.h, self.n, _ptr(v, C.c_float), v.shape[0], _ptr(f, C.c_int32), f.shape[0],
+                                    _ptr(nrm, C.c_float), 0 if nrm is None else nrm.shape[0], _ptr(fn, C.c_int32),
+                                    _ptr(mat, C.c_double), C.byref(h)))
+        self.h = h
+        self.F = f.shape[0]
+
+    def close(self):
+        if self.h:
+            self.ctx._L.nsp_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        i = MeshInfo()
+        self.ctx.check(self.ctx._L.nsp_mesh_get_info(self.h, C.byref(i)))
+        return i
+
+    def min_max(self):
+        mn = np.zeros(self.n)
+        mx = np.zeros(self.n)
+        self.ctx.check(self.ctx._L.nsp_mesh_min_max(self.h, _ptr(mn, C.c_double), _ptr(mx, C.c_double)))
+        return mn, mx
+
+    def records(self):
+        n, F = self.n, self.F
+        nrm = np.zeros((F, n))
+        pn = np.zeros((F, n - 1, n))
+        pd = np.zeros((F, n - 1))
+        self.ctx.check(self.ctx._L.nsp_mesh_records(self.h, _ptr(nrm, C.c_double), _ptr(pn, C.c_double),
+                                                    _ptr(pd, C.c_double)))
+        return nrm, pn, pd
+
+    def tree_dump(self):
+        L = self.ctx._L
+        size = L.nsp_mesh_tree_dump(self.h, None, 0)
+        if size < 0:
+            self.ctx.check(int(size))
+        out = np.zeros(size, np.int64)
+        L.nsp_mesh_tree_dump(self.h, _ptr(out, C.c_int64), size)
+        return out
+
+    def first_hit(self, org, dir):
+        o = np.ascontiguousarray(org, np.float32)
+        d = np.ascontiguousarray(dir, np.float32)
+        cnt = o.shape[0]
+        idx = np.zeros(cnt, np.int32)
+        t = np.zeros(cnt, np.float64)
+        near = np.zeros(cnt, np.uint8)
+        self.ctx.check(self.ctx._L.nsp_trace_first_hit(self.h, _ptr(o, C.c_float), _ptr(d, C.c_float), cnt,
+                                                       _ptr(idx, C.c_int32), _ptr(t, C.c_double),
+                                                       _ptr(near, C.c_uint8)))
+        return idx, t, near
+
+
+class Scene:
+    def __init__(self, ctx, mesh, cfg):
+        """cfg: dict from math_amd.configs (n, camera, axes, units_per_pixel, screen, light, colours)."""
+        self.ctx = ctx
+        self.mesh = mesh
+        n = cfg["n"]
+        self.n = n
+        L = ctx._L
+        obj = Surface()
+        obj.rgb[:] = cfg["object_rgb"]
+        obj.diffuse = cfg["diffuse"]
+        obj.light_source = 0
+        bg = Surface()
+        bg.rgb[:] = cfg["background_rgb"]
+        bg.diffuse = 1.0
+        bg.light_source = 1
+        lum = 0.2126 * cfg["background_rgb"][0] + 0.7152 * cfg["background_rgb"][1] + 0.0722 * cfg["background_rgb"][2]
+        bg.light_rgb[:] = [lum, lum, lum]
+        light = Light()
+        light.kind = LIGHT_POINT
+        light.position[:n] = cfg["light_pos"]
+        light.rgb[:] = cfg["light_rgb"]
+        light.unit_intensity_distance = cfg["light_unit_distance"]
+        pr = Projector()
+        pr.kind = 0
+        pr.camera_org[:n] = cfg["camera_org"]
+        pr.camera_dir[:n] = cfg["camera_dir"]
+        for a in range(n - 1):
+            pr.screen_axes[a][:n] = cfg["axes"][a]
+            pr.screen_size[a] = cfg["screen"][a]
+        pr.units_per_pixel = cfg["units_per_pixel"]
+        h = C.c_void_p()
+        ctx.check(L.nsp_scene_create(ctx.h, n, mesh.h, C.byref(obj), C.byref(light), 1, C.byref(pr), C.byref(bg),
+                                     C.byref(h)))
+        self.h = h
+        self.pixels = L.nsp_scene_pixel_count(h)
+        self.screen = list(cfg["screen"])
+
+    def close(self):
+        if self.h:
+            self.ctx._L.nsp_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def paint(self, spp, max_depth=5, precision=PRECISION_FP32, passes=1, seed=1, batch_paths=0, smooth=False,
+              use_graph=False, stop=None, on_pixels=None):
+        p = PaintParams(spp, max_depth, int(smooth), precision, passes, seed, batch_paths, int(use_graph))
+        cb = None
+        keep = []
+        if on_pixels is not None:
+            def _px(user, first, count, rgb):
+                on_pixels(first, count, np.ctypeslib.as_array(rgb, shape=(count * 3,)).reshape(count, 3).copy())
+            cb = Callbacks(PIXELS_AFTER(_px), PASS_DONE(0), ERROR_MSG(0), None)
+            keep.append(cb)
+        stop_flag = C.c_int32(0) if stop is None else stop
+        rc = self.ctx._L.nsp_paint(self.h, C.byref(p), C.byref(cb) if cb else None, C.byref(stop_flag))
+        self.ctx.check(rc)
+
+    def image(self):
+        s = np.zeros((self.pixels, 3), np.float32)
+        c = np.zeros(self.pixels, np.int32)
+        self.ctx.check(self.ctx._L.nsp_image_get(self.h, _ptr(s, C.c_float), _ptr(c, C.c_int32)))
+        return s, c
+
+    def set_image(self, s, c):
+        s = np.ascontiguousarray(s, np.float32)
+        c = np.ascontiguousarray(c, np.int32)
+        self.ctx.check(self.ctx._L.nsp_image_set(self.h, _ptr(s, C.c_float), _ptr(c, C.c_int32)))
+
+    def statistics(self):
+        a, b, c, d = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        e = C.c_double()
+        self.ctx.check(self.ctx._L.nsp_statistics(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d), C.byref(e)))
+        return dict(passes=a.value, pixels=b.value, rays=c.value, samples=d.value, last_pass_s=e.value)
+
+    def stage_stats(self):
+        s = StageStats()
+        self.ctx.check(self.ctx._L.nsp_last_pass_stats(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in StageStats._fields_}
+
+    def set_counters(self, on):
+        self.ctx.check(self.ctx._L.nsp_set_counters(self.h, int(on)))

@@ -1,0 +1,195 @@
+"""CPU tests of the FP64 checker: known answers, closed forms, invariants.
+
+The reference holds no golden vectors for the painter path (its tests seed from
+/dev/urandom), so the checker is pinned by its known-answer tests (cosine
+sphere coefficient static asserts), published vectors (Philox4x32-10), closed
+forms (barycentric identities, orthonormal complements, the sampler count
+rule), brute force (tree trace == linear scan) and the reference's own
+closed-mesh invariant (src/painter/shapes/test/test_mesh.cpp:106-223).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from math_amd import native
+from oracle import oracle as O
+
+
+def test_philox_known_answers():
+    # Random123 kat_vectors for philox4x32_10
+    assert O.philox([0, 0, 0, 0], [0, 0]) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert O.philox([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2) == [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]
+    assert O.philox([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0]) == [
+        0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+def test_cosine_sphere_coefficient_known_answers():
+    # static_asserts of src/painter/coefficient/cosine_sphere.cpp:35-51
+    kat = {2: math.pi / 2, 3: 2.0, 4: 3 * math.pi / 4, 5: 8 / 3, 6: 15 * math.pi / 16, 7: 16 / 5,
+           8: 35 * math.pi / 32, 9: 128 / 35, 10: 315 * math.pi / 256, 15: 2048 / 429,
+           20: 230945 * math.pi / 131072, 25: 4194304 / 676039}
+    for n, v in kat.items():
+        assert O.cosine_sphere_coefficient(n) == pytest.approx(v, rel=1e-15)
+    # closed form B(1/2,(n-1)/2)/B(1,(n-1)/2), test_cosine_sphere.cpp:47-84
+    for n in range(2, 40):
+        a, b = 0.5, (n - 1) / 2
+        beta1 = math.exp(math.lgamma(a) + math.lgamma(b) - math.lgamma(a + b))
+        beta2 = math.exp(math.lgamma(1) + math.lgamma(b) - math.lgamma(1 + b))
+        assert O.cosine_sphere_coefficient(n) == pytest.approx(beta1 / beta2, rel=1e-12)
+
+
+@pytest.mark.parametrize("dims,spp,expect", [(2, 16, 4), (3, 16, 3), (3, 4, 2), (3, 1024, 11), (3, 4096, 16),
+                                             (4, 8, 2), (4, 4, 2), (2, 1, 1), (3, 1, 1), (2, 17, 5)])
+def test_sampler_count_rule(dims, spp, expect):
+    # StratifiedJitteredSampler::one_dimension_size (sampler.h:31-52): smallest k with k^dims >= spp
+    assert O.samples_per_dimension(dims, spp) == expect
+
+
+@pytest.mark.parametrize("n", [3, 4, 5, 6])
+def test_orthogonal_complement_is_orthonormal(n):
+    rng = np.random.default_rng(n)
+    for _ in range(200):
+        u = rng.normal(size=n)
+        u /= np.linalg.norm(u)
+        if rng.random() < 0.3:
+            u = np.zeros(n)
+            u[rng.integers(n)] = rng.choice([-1.0, 1.0])
+        B = O.orthogonal_complement(u)
+        M = np.vstack([u, B])
+        assert np.allclose(M @ M.T, np.eye(n), atol=1e-12)
+
+
+@pytest.mark.parametrize("n", [3, 4, 5, 6])
+def test_ortho_nn_is_cofactor_vector(n):
+    rng = np.random.default_rng(10 + n)
+    for _ in range(50):
+        V = rng.normal(size=(n - 1, n))
+        w = O.ortho_nn(V)
+        assert np.allclose(V @ w, 0, atol=1e-10)
+        # |w| = (n-1)-volume of the parallelotope = sqrt(det(V V^T))
+        assert np.linalg.norm(w) == pytest.approx(math.sqrt(np.linalg.det(V @ V.T)), rel=1e-9)
+        # orientation: det([V; w]) > 0 when w = sum_i (-1)^i minor_i e_i with the sign of the last row
+        assert np.linalg.det(np.vstack([V, w])) * (-1) ** (n - 1) > 0
+
+
+@pytest.mark.parametrize("n", [3, 4, 5, 6])
+def test_cosine_hemisphere_distribution(n):
+    # test_sphere.cpp:69-123: histogram of cos(theta) is proportional to cos
+    rng = np.random.default_rng(20 + n)
+    normal = rng.normal(size=n)
+    normal /= np.linalg.norm(normal)
+    cs = []
+    for _ in range(20000):
+        d = O.cosine_hemisphere(normal, rng.random(8))
+        assert abs(np.linalg.norm(d) - 1) < 1e-12
+        cs.append(d @ normal)
+    cs = np.array(cs)
+    assert cs.min() > 0
+    hist, edges = np.histogram(cs, bins=10, range=(0, 1))
+    centers = (edges[:-1] + edges[1:]) / 2
+    expect = 2 * centers * 0.1 * len(cs)
+    assert np.all(np.abs(hist - expect) / expect < 0.12)
+
+
+def _mesh(n, g, sigma=0.02, cap=0.0, seed=1):
+    return native.meshgen_cube_sphere(n, g, True, sigma, cap, seed)
+
+
+@pytest.mark.parametrize("n,g", [(3, 6), (4, 3), (5, 2)])
+def test_barycentric_records_closed_form(n, g):
+    v, f = _mesh(n, g)
+    m = O.Mesh(v, f)
+    nrm, pn, pd, v0 = m.records()
+    V = v.astype(np.float64)[f]  # F, n(vertices), n
+    assert np.allclose(np.linalg.norm(nrm, axis=1), 1, atol=1e-14)
+    # normal orthogonal to every edge
+    edges = V[:, 1:, :] - V[:, :1, :]
+    assert np.max(np.abs(np.einsum("fkn,fn->fk", edges, nrm))) < 1e-12
+    # barycentric coordinate i of vertex j is delta_ij (i < n-1); plane normals orthogonal to the simplex normal
+    bc = np.einsum("fvn,fin->fvi", V, pn) - pd[:, None, :]
+    eye = np.zeros((n, n - 1))
+    eye[: n - 1] = np.eye(n - 1)
+    assert np.max(np.abs(bc - eye[None])) < 1e-9
+    assert np.max(np.abs(np.einsum("fin,fn->fi", pn, nrm))) < 1e-9
+
+
+@pytest.mark.parametrize("n,g", [(3, 8), (4, 4), (5, 2)])
+def test_tree_trace_equals_linear_scan(n, g):
+    v, f = _mesh(n, g)
+    m = O.Mesh(v, f)
+    m.build_tree()
+    rng = np.random.default_rng(n)
+    K = 1500
+    org = rng.normal(size=(K, n))
+    org = org / np.linalg.norm(org, axis=1, keepdims=True) * 3
+    d = rng.normal(size=(K, n)) * 0.5 - org
+    i1, t1, near = m.trace_brute(org.astype(np.float32), d.astype(np.float32))
+    i2, t2 = m.trace_tree(org.astype(np.float32), d.astype(np.float32))
+    assert np.array_equal(i1, i2)
+    assert np.array_equal(t1, t2)
+    assert (i1 >= 0).sum() > K // 3
+
+
+@pytest.mark.parametrize("n,g", [(3, 10), (4, 4)])
+def test_closed_mesh_enter_exit_miss(n, g):
+    # test_mesh.cpp:106-223: a ray into a closed convex mesh hits, hits again after
+    # moving past the first hit, then misses; at most 0.05% failures
+    v, f = _mesh(n, g, sigma=0.0)
+    m = O.Mesh(v, f)
+    m.build_tree()
+    rng = np.random.default_rng(3)
+    K = 2000
+    org = rng.normal(size=(K, n))
+    org = org / np.linalg.norm(org, axis=1, keepdims=True) * 4
+    tgt = rng.normal(size=(K, n))
+    tgt *= (0.6 * rng.random(K) / np.linalg.norm(tgt, axis=1))[:, None]  # aim inside the inscribed ball
+    d = tgt - org
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    i1, t1 = m.trace_tree(org.astype(np.float32), d.astype(np.float32))
+    hit = i1 >= 0
+    o2 = org + d * (t1[:, None] + 1e-6)
+    i2, t2 = m.trace_tree(o2.astype(np.float32), d.astype(np.float32))
+    o3 = o2 + d * (t2[:, None] + 1e-6)
+    i3, _ = m.trace_tree(o3.astype(np.float32), d.astype(np.float32))
+    errors = np.sum(~hit) + np.sum(hit & (i2 < 0)) + np.sum(hit & (i2 >= 0) & (i3 >= 0))
+    assert errors <= max(1, 0.0005 * K)
+
+
+def test_tree_leaf_rule_and_containment():
+    # every leaf holds <= 10 objects unless at the maximum depth (tree.h:355-358), and every
+    # simplex whose sampled interior points lie in a leaf box is listed in that leaf
+    n, g = 4, 3
+    v, f = _mesh(n, g)
+    m = O.Mesh(v, f)
+    m.build_tree()
+    dump = m.tree_dump()
+    mn, mx = m.min_max()
+    eps = 2.220446049250313e-16
+    absmax = np.maximum(np.abs(mn), np.abs(mx))
+    lo = mn - absmax * 20 * eps
+    size = (mx + absmax * 20 * eps) - lo
+    leaves = {}
+    k = 0
+    while k < len(dump):
+        lv, q, c = dump[k], tuple(dump[k + 1:k + 1 + n]), dump[k + 1 + n]
+        ids = dump[k + 2 + n:k + 2 + n + c]
+        leaves[(lv, q)] = set(ids.tolist())
+        assert c <= 10 or lv == 8 - 1
+        k += 2 + n + c
+    rng = np.random.default_rng(5)
+    V = v.astype(np.float64)[f]
+    for _ in range(3000):
+        s = rng.integers(len(f))
+        w = rng.dirichlet(np.ones(n))
+        p = w @ V[s]
+        # find the leaf containing p by descending levels
+        found = False
+        for lv in range(0, 8):
+            cell = size / 2 ** lv
+            q = tuple(np.floor((p - lo) / cell).astype(int).tolist())
+            if (lv, q) in leaves:
+                assert s in leaves[(lv, q)]
+                found = True
+                break
+        assert found
