@@ -68,7 +68,23 @@ def build_oracle(verbose=False):
     return os.path.join(d, "libnsp_oracle.so")
 
 
+def build_cpp_tests(verbose=False):
+    """The C++17 drop-in painter test (tests/cpp), compiled by the host compiler
+    against include/nsp.h and libnsp.so."""
+    src = os.path.join(ROOT, "tests", "cpp", "test_painter_api.cpp")
+    exe = os.path.join(ROOT, "tests", "cpp", "test_painter_api")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", CSRC, src, "-o", exe,
+           "-L", HERE, "-lnsp", "-Wl,-rpath,$ORIGIN/../../math_amd"]
+    if verbose:
+        print(" ".join(cmd))
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ test build failed\n{r.stdout}\n{r.stderr}")
+    return exe
+
+
 if __name__ == "__main__":
     v = "-v" in sys.argv
     print(build_library(v))
     print(build_oracle(v))
+    print(build_cpp_tests(v))

@@ -83,11 +83,13 @@ struct nsp_context {
 
 struct nsp_mesh {
     nsp_context* ctx;
+    int device = 0;
     MeshDev m;
 };
 
 struct nsp_scene {
     nsp_context* ctx;
+    int device = 0;
     nsp_mesh* mesh;
     int n;
     nsp_surface object;
@@ -96,6 +98,7 @@ struct nsp_scene {
     nsp_surface background;
     int64_t pixels = 0;
     DevBuf pix_sum, pix_cnt;
+    DevBuf pix_aov;  // (3 + n) * pixels, allocated on the first pass with aov
     RenderBuffers buf;
     DevBuf totals;  // 3 u64
     StageMarks marks;
@@ -249,6 +252,7 @@ int nsp_mesh_create(nsp_context* ctx, int n, const float* vertices, int64_t V, c
         if (F >= (int64_t)1 << 31) raise(NSP_E_INVALID, "too many facets");
         auto mesh = std::make_unique<nsp_mesh>();
         mesh->ctx = ctx;
+        mesh->device = ctx->device;
         MeshDev& m = mesh->m;
         m.n = n;
         m.V = V;
@@ -308,8 +312,9 @@ int nsp_mesh_create(nsp_context* ctx, int n, const float* vertices, int64_t V, c
 void nsp_mesh_destroy(nsp_mesh* mesh)
 {
     if (!mesh) return;
-    (void)hipSetDevice(mesh->ctx->device);
-    (void)hipStreamSynchronize(mesh->ctx->stream);
+    // no use of mesh->ctx here: the context may already be gone
+    (void)hipSetDevice(mesh->device);
+    (void)hipDeviceSynchronize();
     delete mesh;
 }
 
@@ -496,6 +501,7 @@ int nsp_scene_create(nsp_context* ctx, int n, nsp_mesh* mesh, const nsp_surface*
         }
         auto sc = std::make_unique<nsp_scene>();
         sc->ctx = ctx;
+        sc->device = ctx->device;
         sc->mesh = mesh;
         sc->n = n;
         sc->object = *object;
@@ -516,8 +522,8 @@ int nsp_scene_create(nsp_context* ctx, int n, nsp_mesh* mesh, const nsp_surface*
 void nsp_scene_destroy(nsp_scene* scene)
 {
     if (!scene) return;
-    (void)hipSetDevice(scene->ctx->device);
-    (void)hipStreamSynchronize(scene->ctx->stream);
+    (void)hipSetDevice(scene->device);
+    (void)hipDeviceSynchronize();
     delete scene;
 }
 
@@ -546,43 +552,67 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
         batch_pixels = std::min<int64_t>(batch_pixels, scene->pixels);
         const int64_t cap = batch_pixels * S;
         alloc_render_buffers(scene->buf, n, p.precision, cap, p.max_depth);
+        if (p.aov) {
+            scene->buf.aov.alloc(sizeof(float) * (3 + n) * cap);
+            if (!scene->pix_aov.p) {
+                scene->pix_aov.alloc(sizeof(float) * (3 + n) * scene->pixels);
+                NSP_HIP(hipMemsetAsync(scene->pix_aov.p, 0, sizeof(float) * (3 + n) * scene->pixels, s));
+            }
+        }
         for (int pass = 0; pass < p.passes; ++pass) {
             if (stop && *stop) raise(NSP_E_STOPPED, "stopped");
             kf.pass = kd.pass = (int)scene->pass_count + pass;
             auto t0 = std::chrono::steady_clock::now();
             scene->marks.reset();
-            NSP_HIP(hipMemsetAsync(scene->totals.p, 0, sizeof(unsigned long long) * 3, s));
-            if (scene->counters)
-                NSP_HIP(hipMemsetAsync(scene->buf.trav_counters.p, 0, sizeof(unsigned long long) * 8, s));
+            // the pass's batches live in device memory; each batch's launch
+            // sequence selects its batch there, so the sequence is identical
+            // for every batch and can be replayed from one captured graph
             std::vector<BatchDesc> descs;
             for (int64_t px = 0; px < scene->pixels; px += batch_pixels)
                 descs.push_back(BatchDesc{px, std::min(batch_pixels, scene->pixels - px) * S});
-            std::vector<unsigned long long> batch_cnt((size_t)CNT_STRIDE * (p.max_depth + 2));
-            unsigned long long closest = 0, shadow = 0, probes = 0;
+            scene->buf.batch_list.alloc(sizeof(BatchDesc) * descs.size());
+            NSP_HIP(hipMemcpyAsync(scene->buf.batch_list.p, descs.data(), sizeof(BatchDesc) * descs.size(),
+                                   hipMemcpyHostToDevice, s));
+            pass_reset(scene->buf, scene->counters, s);
+            RenderLaunch rl;
+            rl.mesh = &scene->mesh->m;
+            rl.scene = p.precision ? (const void*)&kd : (const void*)&kf;
+            rl.buf = &scene->buf;
+            rl.pix_sum = scene->pix_sum.as<float>();
+            rl.pix_cnt = scene->pix_cnt.as<int32_t>();
+            rl.pix_aov = p.aov ? scene->pix_aov.as<float>() : nullptr;
+            rl.pixels = scene->pixels;
+            rl.max_depth = p.max_depth;
+            rl.counters = scene->counters;
+            rl.marks = p.use_graph ? nullptr : &scene->marks;
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            if (p.use_graph) {
+                NSP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                try {
+                    render_batch(rl, s);
+                } catch (...) {
+                    hipGraph_t g2 = nullptr;
+                    (void)hipStreamEndCapture(s, &g2);
+                    if (g2) (void)hipGraphDestroy(g2);
+                    throw;
+                }
+                NSP_HIP(hipStreamEndCapture(s, &graph));
+                NSP_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            }
             for (size_t b = 0; b < descs.size(); ++b) {
                 if (stop && *stop) break;
-                NSP_HIP(hipMemcpyAsync(scene->buf.desc.p, &descs[b], sizeof(BatchDesc), hipMemcpyHostToDevice, s));
-                RenderLaunch rl;
-                rl.mesh = &scene->mesh->m;
-                rl.scene = p.precision ? (const void*)&kd : (const void*)&kf;
-                rl.buf = &scene->buf;
-                rl.pix_sum = scene->pix_sum.as<float>();
-                rl.pix_cnt = scene->pix_cnt.as<int32_t>();
-                rl.pixels = scene->pixels;
-                rl.max_depth = p.max_depth;
-                rl.counters = scene->counters;
-                rl.marks = &scene->marks;
-                render_batch(rl, s);
-                NSP_HIP(hipMemcpyAsync(batch_cnt.data(), scene->buf.counters.p,
-                                       sizeof(unsigned long long) * batch_cnt.size(), hipMemcpyDeviceToHost, s));
-                NSP_HIP(hipStreamSynchronize(s));
-                for (int d = 0; d <= p.max_depth; ++d) {
-                    closest += batch_cnt[d * CNT_STRIDE + 0];
-                    shadow += batch_cnt[d * CNT_STRIDE + 1];
-                    probes += batch_cnt[d * CNT_STRIDE + 2];
-                }
+                if (exec)
+                    NSP_HIP(hipGraphLaunch(exec, s));
+                else
+                    render_batch(rl, s);
             }
+            unsigned long long tot[3] = {0, 0, 0};
+            NSP_HIP(hipMemcpyAsync(tot, scene->buf.totals.p, sizeof tot, hipMemcpyDeviceToHost, s));
             NSP_HIP(hipStreamSynchronize(s));
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+            const unsigned long long closest = tot[0], shadow = tot[1], probes = tot[2];
             auto t1 = std::chrono::steady_clock::now();
             double secs = std::chrono::duration<double>(t1 - t0).count();
             // stage timing
@@ -683,6 +713,24 @@ int nsp_image_set(nsp_scene* scene, const float* rgb_sum, const int32_t* samples
             for (int c = 0; c < 3; ++c) s[(size_t)c * P + j] = rgb_sum[j * 3 + c];
         NSP_HIP(hipMemcpy(scene->pix_sum.p, s.data(), sizeof(float) * 3 * P, hipMemcpyHostToDevice));
         NSP_HIP(hipMemcpy(scene->pix_cnt.p, samples, sizeof(int32_t) * P, hipMemcpyHostToDevice));
+    });
+}
+
+int nsp_image_aov(const nsp_scene* scene, float* albedo_sum, float* normal_sum)
+{
+    if (!scene) return NSP_E_INVALID;
+    return guarded(scene->ctx, [&] {
+        const int64_t P = scene->pixels;
+        const int n = scene->n;
+        std::vector<float> a((size_t)(3 + n) * P, 0.0f);
+        if (scene->pix_aov.p)
+            NSP_HIP(hipMemcpy(a.data(), scene->pix_aov.p, sizeof(float) * a.size(), hipMemcpyDeviceToHost));
+        for (int64_t j = 0; j < P; ++j) {
+            if (albedo_sum)
+                for (int c = 0; c < 3; ++c) albedo_sum[j * 3 + c] = a[(size_t)c * P + j];
+            if (normal_sum)
+                for (int k = 0; k < n; ++k) normal_sum[j * n + k] = a[(size_t)(3 + k) * P + j];
+        }
     });
 }
 

@@ -104,12 +104,53 @@ void launch_first_hit(const MeshDev& m, const float* org, const float* dir, int6
 // ---------------------------------------------------------------------------
 // Render stages
 
-__global__ void k_batch_reset(unsigned long long* cnt, int ncnt, const BatchDesc* desc)
+// Selects the current batch from the pass's batch list (device memory, so a
+// captured graph replays unchanged for every batch) and clears its counters.
+__global__ void k_batch_reset(unsigned long long* cnt, int ncnt, const BatchDesc* __restrict__ list,
+                              const long long* __restrict__ batch_idx, BatchDesc* __restrict__ cur)
 {
     int i = threadIdx.x;
     for (int k = i; k < ncnt; k += blockDim.x) cnt[k] = 0;
     __syncthreads();
-    if (i == 0) cnt[0] = (unsigned long long)desc->paths;
+    if (i == 0) {
+        BatchDesc d = list[*batch_idx];
+        *cur = d;
+        cnt[0] = (unsigned long long)d.paths;
+    }
+}
+
+// Adds the batch's ray counts to the pass totals and advances the batch index.
+__global__ void k_batch_totals(const unsigned long long* __restrict__ cnt, int depth_slots,
+                               unsigned long long* __restrict__ totals, long long* __restrict__ batch_idx)
+{
+    if (threadIdx.x != 0) return;
+    unsigned long long c = 0, s = 0, p = 0;
+    for (int d = 0; d < depth_slots; ++d) {
+        c += cnt[d * CNT_STRIDE + 0];
+        s += cnt[d * CNT_STRIDE + 1];
+        p += cnt[d * CNT_STRIDE + 2];
+    }
+    totals[0] += c;
+    totals[1] += s;
+    totals[2] += p;
+    *batch_idx += 1;
+}
+
+// Clears the pass totals and the batch index (a kernel, not a memset, so it
+// may sit inside captured work).
+__global__ void k_pass_reset(unsigned long long* totals, long long* batch_idx, unsigned long long* trav, int ntrav)
+{
+    int i = threadIdx.x;
+    if (i < 3) totals[i] = 0;
+    if (i == 0) *batch_idx = 0;
+    if (trav && i < ntrav) trav[i] = 0;
+}
+
+void pass_reset(RenderBuffers& b, bool counters, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pass_reset, dim3(1), dim3(64), 0, s, b.totals.as<unsigned long long>(),
+                       b.batch_idx.as<long long>(), counters ? b.trav_counters.as<unsigned long long>() : nullptr, 8);
+    NSP_HIP(hipGetLastError());
 }
 
 template <int N, typename T>
@@ -236,7 +277,7 @@ template <int N, typename T>
 __global__ void __launch_bounds__(TRACE_BLOCK)
     k_shade(SceneK<T> sc, TreeArgs A, MeshShadeArgs M, const BatchDesc* __restrict__ desc, int depth,
             RayQueue<T> qin, RayQueue<T> qout, ShadowQueue<T> sq, int64_t cap, unsigned long long* __restrict__ cnt,
-            float* __restrict__ L)
+            float* __restrict__ L, float* __restrict__ aov)
 {
     __shared__ int stack[MAX_LEVELS * TRACE_BLOCK];
     int* st = stack + threadIdx.x;
@@ -271,6 +312,20 @@ __global__ void __launch_bounds__(TRACE_BLOCK)
                 d[k] = qin.dir[(int64_t)k * cap + i];
             }
             const bool diffuse_reflection = depth > 0;
+            if (aov && depth == 0) {
+                // first-hit albedo and normal facing the camera (denoiser inputs); background on a miss
+                T an[N];
+                if (idx >= 0) {
+                    load_normal<N, T>(M, idx, an);
+                    if (dot<N, T>(d, an) > (T)0)
+#pragma unroll
+                        for (int k = 0; k < N; ++k) an[k] = -an[k];
+                }
+#pragma unroll
+                for (int c = 0; c < 3; ++c) aov[c * cap + path] = idx >= 0 ? sc.diffuse * sc.obj_rgb[c] : sc.bg_rgb[c];
+#pragma unroll
+                for (int k = 0; k < N; ++k) aov[(3 + k) * cap + path] = idx >= 0 ? (float)an[k] : 0.0f;
+            }
             if (idx < 0) {
                 // miss: background, a light source when reached by a diffuse bounce (painter.cpp:335-340)
                 const float* bg = (sc.bg_is_light && diffuse_reflection) ? sc.bg_light_rgb : sc.bg_rgb;
@@ -470,7 +525,8 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_shadow(TreeArgs A, ShadowQueue<
 __global__ void __launch_bounds__(256) k_accumulate(const BatchDesc* __restrict__ desc, int S,
                                                    const float* __restrict__ L, int64_t cap,
                                                    float* __restrict__ pix_sum, int32_t* __restrict__ pix_cnt,
-                                                   int64_t pixels)
+                                                   int64_t pixels, const float* __restrict__ aov,
+                                                   float* __restrict__ pix_aov, int aovc)
 {
     const int64_t npix = desc->paths / S;
     const int64_t pixel0 = desc->pixel0;
@@ -487,6 +543,13 @@ __global__ void __launch_bounds__(256) k_accumulate(const BatchDesc* __restrict_
         pix_sum[pixels + px] += s1;
         pix_sum[2 * pixels + px] += s2;
         pix_cnt[px] += S;
+        if (aov) {
+            for (int c = 0; c < aovc; ++c) {
+                float a = 0;
+                for (int s = 0; s < S; ++s) a += aov[c * cap + j * S + s];
+                pix_aov[c * pixels + px] += a;
+            }
+        }
     }
 }
 
@@ -518,6 +581,8 @@ void alloc_render_buffers(RenderBuffers& b, int n, int precision, int64_t cap, i
     b.L.alloc(sizeof(float) * 3 * cap);
     b.counters.alloc(sizeof(unsigned long long) * CNT_STRIDE * (max_depth + 2));
     b.desc.alloc(sizeof(BatchDesc));
+    b.totals.alloc(sizeof(unsigned long long) * 3);
+    b.batch_idx.alloc(sizeof(long long));
     b.trav_counters.alloc(sizeof(unsigned long long) * 8);
 }
 
@@ -554,6 +619,12 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
     unsigned long long* cnt = b.counters.as<unsigned long long>();
     const BatchDesc* desc = b.desc.as<BatchDesc>();
     float* L = b.L.as<float>();
+    float* aov = nullptr;
+    if (rl.pix_aov) {
+        // allocated by the caller before any graph capture
+        if (b.aov.bytes < sizeof(float) * (3 + N) * cap) raise(NSP_E_INVALID, "AOV buffer not allocated");
+        aov = b.aov.as<float>();
+    }
     unsigned g = grid_for(cap);
     auto timed = [&](int stage, auto&& launch) {
         if (!rl.marks) {
@@ -568,7 +639,8 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
         NSP_HIP(hipEventRecord(e1, s));
         rl.marks->marks.emplace_back(stage, e0, e1);
     };
-    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(256), 0, s, cnt, CNT_STRIDE * (rl.max_depth + 2), desc);
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(256), 0, s, cnt, CNT_STRIDE * (rl.max_depth + 2),
+                       b.batch_list.as<BatchDesc>(), b.batch_idx.as<long long>(), b.desc.as<BatchDesc>());
     NSP_HIP(hipGetLastError());
     timed(0, [&] { hipLaunchKernelGGL((k_raygen<N, T>), dim3(g), dim3(TRACE_BLOCK), 0, s, sc, desc, qa, cap, L); });
     for (int depth = 0; depth <= rl.max_depth; ++depth) {
@@ -580,7 +652,7 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
         });
         timed(2, [&] {
             hipLaunchKernelGGL((k_shade<N, T>), dim3(g), dim3(TRACE_BLOCK), 0, s, sc, A, M, desc, depth, qin, qout,
-                               sq, cap, cnt, L);
+                               sq, cap, cnt, L, aov);
         });
         timed(3, [&] {
             hipLaunchKernelGGL((k_shadow<N, T>), dim3(grid_for(scap)), dim3(TRACE_BLOCK), 0, s, As, sq, scap,
@@ -589,8 +661,11 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
     }
     timed(4, [&] {
         hipLaunchKernelGGL(k_accumulate, dim3(grid_for(cap / std::max(1, sc.S), 256, 4096)), dim3(256), 0, s, desc,
-                           sc.S, L, cap, rl.pix_sum, rl.pix_cnt, rl.pixels);
+                           sc.S, L, cap, rl.pix_sum, rl.pix_cnt, rl.pixels, aov, rl.pix_aov, 3 + N);
     });
+    hipLaunchKernelGGL(k_batch_totals, dim3(1), dim3(64), 0, s, cnt, rl.max_depth + 1,
+                       b.totals.as<unsigned long long>(), b.batch_idx.as<long long>());
+    NSP_HIP(hipGetLastError());
 }
 
 hipEvent_t StageMarks::get()

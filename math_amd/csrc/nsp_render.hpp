@@ -92,9 +92,13 @@ struct RenderBuffers {
     int depth_slots = 0;
     DevBuf qa[8], qb[8], sq[6];
     DevBuf L;               // 3 * cap floats
+    DevBuf aov;             // (3 + n) * cap floats: first-hit albedo and normal per path
     DevBuf counters;        // (max_depth + 2) * CNT_STRIDE u64
-    DevBuf desc;            // BatchDesc
-    DevBuf trav_counters;   // 4 u64
+    DevBuf desc;            // BatchDesc of the batch in flight
+    DevBuf batch_list;      // BatchDesc per batch of the pass
+    DevBuf batch_idx;       // long long: index of the next batch
+    DevBuf totals;          // 3 u64: closest, shadow and probe rays of the pass
+    DevBuf trav_counters;   // 8 u64: traversal counters (closest stage, shadow stage)
 };
 
 TreeArgs make_tree_args(const MeshDev& m);
@@ -125,12 +129,15 @@ struct RenderLaunch {
     RenderBuffers* buf;
     float* pix_sum;     // 3 * pixels (component-major: sum[c * pixels + p])
     int32_t* pix_cnt;
+    float* pix_aov;     // (3 + n) * pixels sums of first-hit albedo and normal, or null
     int64_t pixels;
     int max_depth;
     bool counters;
     StageMarks* marks;  // may be null
 };
 void render_batch(const RenderLaunch& rl, hipStream_t s);
+// Clears the pass totals, the batch index and (when counting) the traversal counters.
+void pass_reset(RenderBuffers& b, bool counters, hipStream_t s);
 void alloc_render_buffers(RenderBuffers& b, int n, int precision, int64_t cap, int max_depth);
 
 }  // namespace nsp

@@ -108,10 +108,12 @@ __device__ __forceinline__ void load_vec_f(const float* __restrict__ soa, int64_
 }
 
 // Candidate test of one leaf entry.  Returns true for a hit with 0 < t < tlimit.
+// best_idx: the simplex that currently holds tlimit (a simplex listed in
+// several leaves meets itself again and is no tie).
 template <int N, typename T>
 __device__ __forceinline__ bool candidate(const TreeArgs& A, int64_t e, const T* o, const T* d, const double* o64,
-                                          const double* d64, T tlimit, double* t_hit, int* idx, bool* near,
-                                          unsigned* retests)
+                                          const double* d64, T tlimit, int best_idx, double* t_hit, int* idx,
+                                          bool* near, unsigned* retests)
 {
     if constexpr (sizeof(T) == 8) {
         int s = A.entries[e];
@@ -158,7 +160,7 @@ __device__ __forceinline__ bool candidate(const TreeArgs& A, int64_t e, const T*
         bool retest = fabsf(margin) < band || (in && as < PARALLEL_COS) || t < terr;
         if (!retest) {
             if (!in || !(t > 0.0f)) return false;
-            if (fabsf(t - tlimit) < terr) *near = true;  // near-tie with the current nearest hit
+            if (fabsf(t - tlimit) < terr && A.entries[e] != best_idx) *near = true;  // near-tie with another simplex
             if (!(t < tlimit)) return false;
             *t_hit = (double)t;
             *idx = A.entries[e];
@@ -284,7 +286,7 @@ __device__ HitResult traverse(const TreeArgs& A, const T* o, const T* d, T tmax,
                 int si;
                 bool nr = false;
                 T lim = ANY ? tmax : (T)best_t;
-                if (candidate<N, T>(A, e, o, d, o64, d64, lim, &th, &si, &nr, &nret)) {
+                if (candidate<N, T>(A, e, o, d, o64, d64, lim, best, &th, &si, &nr, &nret)) {
                     if (ANY) {
                         res.idx = si;
                         res.t = th;

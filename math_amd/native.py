@@ -50,7 +50,7 @@ class Surface(C.Structure):
 class PaintParams(C.Structure):
     _fields_ = [("samples_per_pixel", C.c_int), ("max_depth", C.c_int), ("smooth_normal", C.c_int),
                 ("precision", C.c_int), ("passes", C.c_int), ("seed", C.c_uint64), ("batch_paths", C.c_int64),
-                ("use_graph", C.c_int)]
+                ("use_graph", C.c_int), ("aov", C.c_int)]
 
 
 PIXELS_AFTER = C.CFUNCTYPE(None, C.c_void_p, C.c_int64, C.c_int64, C.POINTER(C.c_float))
@@ -83,7 +83,7 @@ EXPORTS = [
     "nsp_context_create", "nsp_context_destroy", "nsp_last_error", "nsp_debug_fail_next_launch",
     "nsp_mesh_create", "nsp_mesh_destroy", "nsp_mesh_min_max", "nsp_mesh_get_info", "nsp_mesh_records",
     "nsp_mesh_tree_dump", "nsp_trace_first_hit", "nsp_trace_first_hit_device", "nsp_scene_create",
-    "nsp_scene_destroy", "nsp_paint", "nsp_statistics", "nsp_image_get", "nsp_image_set",
+    "nsp_scene_destroy", "nsp_paint", "nsp_statistics", "nsp_image_get", "nsp_image_set", "nsp_image_aov",
     "nsp_scene_pixel_count", "nsp_last_pass_stats", "nsp_set_counters", "nsp_meshgen_torus",
     "nsp_meshgen_cube_sphere",
 ]
@@ -126,6 +126,7 @@ def lib():
         L.nsp_statistics.argtypes = [P, i64p, i64p, i64p, i64p, dp]
         L.nsp_image_get.argtypes = [P, fp, ip]
         L.nsp_image_set.argtypes = [P, fp, ip]
+        L.nsp_image_aov.argtypes = [P, fp, fp]
         L.nsp_scene_pixel_count.argtypes = [P]
         L.nsp_scene_pixel_count.restype = C.c_int64
         L.nsp_last_pass_stats.argtypes = [P, C.POINTER(StageStats)]
@@ -325,8 +326,8 @@ class Scene:
             pass
 
     def paint(self, spp, max_depth=5, precision=PRECISION_FP32, passes=1, seed=1, batch_paths=0, smooth=False,
-              use_graph=False, stop=None, on_pixels=None):
-        p = PaintParams(spp, max_depth, int(smooth), precision, passes, seed, batch_paths, int(use_graph))
+              use_graph=False, stop=None, on_pixels=None, aov=False):
+        p = PaintParams(spp, max_depth, int(smooth), precision, passes, seed, batch_paths, int(use_graph), int(aov))
         cb = None
         keep = []
         if on_pixels is not None:
@@ -343,6 +344,12 @@ class Scene:
         c = np.zeros(self.pixels, np.int32)
         self.ctx.check(self.ctx._L.nsp_image_get(self.h, _ptr(s, C.c_float), _ptr(c, C.c_int32)))
         return s, c
+
+    def aov(self):
+        a = np.zeros((self.pixels, 3), np.float32)
+        nrm = np.zeros((self.pixels, self.n), np.float32)
+        self.ctx.check(self.ctx._L.nsp_image_aov(self.h, _ptr(a, C.c_float), _ptr(nrm, C.c_float)))
+        return a, nrm
 
     def set_image(self, s, c):
         s = np.ascontiguousarray(s, np.float32)

@@ -15,6 +15,5 @@ def pytest_configure(config):
 @pytest.fixture(scope="session")
 def ctx():
     from math_amd import native
-    c = native.Context(0)
-    yield c
-    c.close()
+    # closed by reference counting after the meshes and scenes that hold it
+    return native.Context(0)

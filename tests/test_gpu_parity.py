@@ -230,3 +230,35 @@ def test_error_paths(ctx):
     # the context stays usable after the injected failure
     idx, _, _ = gm.first_hit(np.full((1, 4), 3.0, np.float32), np.full((1, 4), -1.0, np.float32))
     assert idx[0] >= 0
+
+
+def test_cpp_painter_api_drop_in():
+    """The reference's headless painter test written against the C++17 drop-in
+    (paint<4,double>, single_object_scene, VisibleBarPaintbrush) on the GPU."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "test_painter_api")
+    assert os.path.exists(exe), "built by __graft_entry__.build()"
+    r = subprocess.run([exe, "4", "12", "4"], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0 and r.stdout.startswith("OK"), r.stdout
+    _, pixels, rays, lum, passes = r.stdout.split()
+    assert int(pixels) > 0 and int(rays) > int(pixels) and float(lum) > 0 and int(passes) == 1
+
+
+def test_graph_replay_equals_eager(ctx):
+    """Each batch's launch sequence captured once in a hipGraph and replayed per
+    batch (several replays per pass, eager work and syncs between passes) gives
+    the eager image bit for bit, pass after pass."""
+    v, f, cfg, spp, depth = small_scene(4, 4, 12, spp=8)
+    gm = ctx.mesh(v, f)
+    eager = native.Scene(ctx, gm, cfg)
+    graph = native.Scene(ctx, gm, cfg)
+    for p in range(3):
+        eager.paint(spp, depth, seed=6, batch_paths=27 * 200)
+        graph.paint(spp, depth, seed=6, batch_paths=27 * 200, use_graph=True)
+        gm.first_hit(np.full((8, 4), 3.0, np.float32), np.full((8, 4), -1.0, np.float32))  # eager work between
+        a, ca = eager.image()
+        b, cb = graph.image()
+        assert np.array_equal(a, b) and np.array_equal(ca, cb), p
+    assert eager.statistics()["rays"] == graph.statistics()["rays"]
