@@ -81,7 +81,18 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-paths", type=int, default=0)
     ap.add_argument("--graph", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 256 MiB device copy (to scale the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(1 << 26, dtype=torch.float32, device="cuda")  # 256 MiB
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": x.numel() * 4}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the path tracer shards by screen tiles and "
               "needs no collective; multi-GPU runs are not part of this build", file=sys.stderr)

@@ -219,6 +219,38 @@ int nsp_last_pass_stats(const nsp_scene* scene, nsp_stage_stats* out);
  * tests, FP64 re-tests) for roofline accounting; off by default. */
 int nsp_set_counters(nsp_scene* scene, int enable);
 
+/* Learned denoiser for 3D screens (n = 4 renders).  No reference counterpart
+ * (the reference only renders progressively).  8 layers of 3x3x3
+ * convolutions, 32 channels, BF16 weights and activations, FP32 accumulation,
+ * ReLU between layers; the output is added to the residual input channels
+ * [residual_channel, residual_channel+3) (the low-spp radiance), or to zero
+ * when residual_channel < 0.  Volumes are channels-last float arrays
+ * [nb][depth][height][width][channels] on the host. */
+typedef struct nsp_denoiser nsp_denoiser;
+int nsp_denoiser_create(nsp_context* ctx, int in_channels, int residual_channel, uint64_t seed, nsp_denoiser** out);
+void nsp_denoiser_destroy(nsp_denoiser* d);
+/* Parameters: per layer, weights [cout][27][cin] then bias [cout], with the
+ * padded shapes cin = 16 (layer 0) or 32, cout = 32 or 16 (last layer). */
+int64_t nsp_denoiser_param_count(const nsp_denoiser* d);
+int nsp_denoiser_get_params(const nsp_denoiser* d, float* out);
+int nsp_denoiser_set_params(nsp_denoiser* d, const float* in);
+/* Denoised radiance output[nb*depth*height*width*3]. */
+int nsp_denoiser_forward(nsp_denoiser* d, const float* input, int nb, int depth, int height, int width,
+                         float* output);
+/* Mean squared error against target (voxels*3) and its gradient w.r.t. the
+ * parameters (param_count floats, may be NULL); no update. */
+int nsp_denoiser_loss_grad(nsp_denoiser* d, const float* input, const float* target, int nb, int depth, int height,
+                           int width, float* loss, float* grad);
+/* Adam training on random patch^3 crops (batch per step) of `volumes`
+ * input/target volumes uploaded once; losses[steps] receives each step's loss. */
+int nsp_denoiser_train(nsp_denoiser* d, const float* inputs, const float* targets, int volumes, int depth, int height,
+                       int width, int patch, int batch, int steps, float lr, uint64_t seed, float* losses);
+/* One BF16 implicit-GEMM 3x3x3 convolution (zero padding): x [V][cin],
+ * w [cout][27][cin], bias [cout] or NULL, y [V][cout]; cin, cout in {16, 32}.
+ * Kernel-level parity entry point for the denoiser's convolution. */
+int nsp_conv3d_bf16(nsp_context* ctx, const float* x, int nb, int depth, int height, int width, int cin,
+                    const float* w, const float* bias, int cout, int relu, float* y);
+
 /* Synthetic meshes (the configurations' generators; host only, no GPU).
  * Each returns the vertex and facet counts when the output pointers are NULL. */
 /* Torus R, r on a nu x nv grid, minor radius per vertex r + U(-noise, noise). n=3. */

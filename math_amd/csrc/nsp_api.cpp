@@ -111,6 +111,12 @@ struct nsp_scene {
 
 static thread_local std::string g_create_error;
 
+namespace nsp {
+hipStream_t context_stream(nsp_context* ctx) { return ctx->stream; }
+int context_device(nsp_context* ctx) { return ctx->device; }
+std::string* context_error(nsp_context* ctx) { return &ctx->error; }
+}  // namespace nsp
+
 template <class F>
 static int guarded(nsp_context* ctx, F&& f)
 {

@@ -1,0 +1,830 @@
+// nsp_denoise.hip -- learned denoiser for 3D screens (n = 4 renders): an
+// 8-layer 3x3x3 convolutional network, 32 channels, BF16 weights and
+// activations, FP32 accumulation, trained on the GPU with Adam.
+//
+// No reference counterpart: the reference renders progressively and never
+// denoises.  Every convolution is an implicit GEMM on v_mfma_f32_16x16x32_bf16
+// (gfx950):
+//   forward        Y[v, co]  = sum_{tap, ci} W[co, tap, ci] X[v + off(tap), ci]
+//                  M = voxels (64 per wave), N = output channels, K = 27 * Cin
+//   backward data  dX[v, ci] = sum_{tap, co} W[co, 26 - tap, ci] dY'[v + off(tap), co]
+//                  the forward kernel on tap-flipped, transposed weights, with
+//                  the ReLU mask of the previous layer applied in the epilogue
+//   backward wgt   dW[co, tap, ci] = sum_v dY'[v, co] X[v + off(tap), ci]
+//                  M = co, N = ci, K = voxels; per-chunk partials reduced in a
+//                  fixed order, so gradients are bit-reproducible.
+// Activations are channels-last BF16 ([voxel][channel]), so a lane's K slice
+// of 8 channels is one 16-byte load and 16 lanes read 16 consecutive voxels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "nsp.h"
+#include "nsp_device.hpp"
+#include "nsp_internal.hpp"
+
+namespace nsp {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short shortx8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int DN_LAYERS = 8;
+constexpr int DN_WIDTH = 32;
+constexpr int DN_IN = 16;   // input channels, padded
+constexpr int DN_OUT = 16;  // last layer output channels, padded (3 used)
+
+struct Dims {
+    int nb, d, h, w;
+    __host__ __device__ int64_t voxels() const { return (int64_t)nb * d * h * w; }
+};
+
+__device__ __forceinline__ bf16x8 zero8()
+{
+    shortx8 z = {0, 0, 0, 0, 0, 0, 0, 0};
+    return __builtin_bit_cast(bf16x8, z);
+}
+
+__device__ __forceinline__ unsigned short f2bf(float f)
+{
+    __bf16 b = (__bf16)f;  // round to nearest even
+    return __builtin_bit_cast(unsigned short, b);
+}
+
+__device__ __forceinline__ float bf2f(unsigned short u)
+{
+    return __builtin_bit_cast(float, (unsigned)u << 16);
+}
+
+// voxel index <-> coordinates (x fastest)
+struct Vox {
+    int b, z, y, x;
+};
+__device__ __forceinline__ Vox decode(int64_t v, const Dims& D)
+{
+    Vox r;
+    r.x = (int)(v % D.w);
+    v /= D.w;
+    r.y = (int)(v % D.h);
+    v /= D.h;
+    r.z = (int)(v % D.d);
+    r.b = (int)(v / D.d);
+    return r;
+}
+__device__ __forceinline__ int64_t neighbour(const Vox& p, int tap, const Dims& D)
+{
+    int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+    int z = p.z + dz, y = p.y + dy, x = p.x + dx;
+    if (z < 0 || z >= D.d || y < 0 || y >= D.h || x < 0 || x >= D.w) return -1;
+    return (((int64_t)p.b * D.d + z) * D.h + y) * D.w + x;
+}
+
+// ---------------------------------------------------------------------------
+// Implicit-GEMM convolution (forward, and backward-data with flipped weights).
+//   X      [V][CIN] bf16
+//   Wk     [NT*16][KP] bf16, k = tap * CIN + ci, KP = 32 * ceil(27 * CIN / 32)
+//   bias   [NT*16] f32 or null
+//   mask   [V][NT*16] bf16 or null: zero the output where mask <= 0 (ReLU' of the
+//          layer whose gradient this is)
+//   out    bf16 [V][NT*16] (out_f32 null) or f32 [V][NT*16]
+template <int CIN, int NT>
+__global__ void __launch_bounds__(256) k_conv(const unsigned short* __restrict__ X, Dims D,
+                                              const unsigned short* __restrict__ Wk,
+                                              const float* __restrict__ bias, int relu,
+                                              const unsigned short* __restrict__ mask,
+                                              unsigned short* __restrict__ out, float* __restrict__ out_f32)
+{
+    constexpr int KP = ((27 * CIN + 31) / 32) * 32;
+    constexpr int KS = KP / 32;
+    constexpr int COUT = NT * 16;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t V = D.voxels();
+    const int64_t base = ((int64_t)blockIdx.x * 4 + wave) * 64;
+    if (base >= V) return;
+    const int row = lane & 15, kg = lane >> 4;
+    Vox p[4];
+    bool valid[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        int64_t v = base + mt * 16 + row;
+        valid[mt] = v < V;
+        p[mt] = decode(valid[mt] ? v : 0, D);
+    }
+    floatx4 acc[4][NT];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = floatx4{0, 0, 0, 0};
+    for (int s = 0; s < KS; ++s) {
+        const int k0 = s * 32 + kg * 8;
+        const int tap = k0 / CIN, c0 = k0 % CIN;
+        bf16x8 b[NT];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+            b[nt] = *reinterpret_cast<const bf16x8*>(Wk + (int64_t)(nt * 16 + row) * KP + k0);
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            bf16x8 a = zero8();
+            if (tap < 27 && valid[mt]) {
+                int64_t nb = neighbour(p[mt], tap, D);
+                if (nb >= 0) a = *reinterpret_cast<const bf16x8*>(X + nb * CIN + c0);
+            }
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[nt], acc[mt][nt], 0, 0, 0);
+        }
+    }
+    // D layout: lane holds rows 4*kg + i of column `row`
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                int64_t v = base + mt * 16 + kg * 4 + i;
+                if (v >= V) continue;
+                int co = nt * 16 + row;
+                float y = acc[mt][nt][i];
+                if (bias) y += bias[co];
+                if (relu) y = fmaxf(y, 0.0f);
+                if (mask && !(bf2f(mask[v * COUT + co]) > 0.0f)) y = 0.0f;
+                if (out_f32)
+                    out_f32[v * COUT + co] = y;
+                else
+                    out[v * COUT + co] = f2bf(y);
+            }
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient: partial[chunk][tap][co][ci] over CHUNK voxels per block.
+// Block = 4 waves; blockIdx.y selects 3 taps; each wave covers CHUNK/4 voxels.
+constexpr int DW_CHUNK = 2048;
+
+template <int CIN, int NT>
+__global__ void __launch_bounds__(256) k_conv_dw(const unsigned short* __restrict__ X, Dims D,
+                                                 const unsigned short* __restrict__ dY, float* __restrict__ partial)
+{
+    constexpr int COUT = NT * 16;
+    constexpr int CT = CIN / 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = lane & 15, kg = lane >> 4;
+    const int tap0 = blockIdx.y * 3;
+    const int64_t V = D.voxels();
+    const int64_t vbeg = (int64_t)blockIdx.x * DW_CHUNK + wave * (DW_CHUNK / 4);
+    floatx4 acc[3][NT][CT];
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+#pragma unroll
+        for (int a = 0; a < NT; ++a)
+#pragma unroll
+            for (int c = 0; c < CT; ++c) acc[t][a][c] = floatx4{0, 0, 0, 0};
+    for (int ks = 0; ks < DW_CHUNK / 4 / 32; ++ks) {
+        const int64_t v0 = vbeg + ks * 32 + kg * 8;
+        // A: dY^T, rows co, K = 8 voxels
+        bf16x8 A[NT];
+#pragma unroll
+        for (int a = 0; a < NT; ++a) {
+            shortx8 t8;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                int64_t v = v0 + j;
+                t8[j] = v < V ? (short)dY[v * COUT + a * 16 + row] : (short)0;
+            }
+            A[a] = __builtin_bit_cast(bf16x8, t8);
+        }
+        Vox pv[8];
+        bool ok[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            ok[j] = v0 + j < V;
+            pv[j] = decode(ok[j] ? v0 + j : 0, D);
+        }
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            const int tap = tap0 + t;
+#pragma unroll
+            for (int c = 0; c < CT; ++c) {
+                shortx8 t8;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    int64_t nb = ok[j] ? neighbour(pv[j], tap, D) : -1;
+                    t8[j] = nb >= 0 ? (short)X[nb * CIN + c * 16 + row] : (short)0;
+                }
+                bf16x8 B = __builtin_bit_cast(bf16x8, t8);
+#pragma unroll
+                for (int a = 0; a < NT; ++a)
+                    acc[t][a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[a], B, acc[t][a][c], 0, 0, 0);
+            }
+        }
+    }
+    // reduce the 4 waves through LDS in a fixed order, then one write per value
+    __shared__ float red[4][3 * NT * CT * 16 * 4];
+    int idx = 0;
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+#pragma unroll
+        for (int a = 0; a < NT; ++a)
+#pragma unroll
+            for (int c = 0; c < CT; ++c)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) red[wave][(idx++) * 64 + lane] = acc[t][a][c][i];
+    __syncthreads();
+    const int total = 3 * NT * CT * 4 * 64;
+    for (int e = threadIdx.x; e < total; e += 256) {
+        float s = red[0][e] + red[1][e] + red[2][e] + red[3][e];
+        int ln = e % 64, q = e / 64;
+        int i = q % 4;
+        q /= 4;
+        int c = q % CT;
+        q /= CT;
+        int a = q % NT;
+        int t = q / NT;
+        int co = a * 16 + 4 * (ln >> 4) + i;  // D row = output row of A = co
+        int ci = c * 16 + (ln & 15);          // D column = ci
+        int tap = tap0 + t;
+        // master layout [co][tap][ci] per chunk
+        partial[(((int64_t)blockIdx.x * COUT + co) * 27 + tap) * CIN + ci] = s;
+    }
+}
+
+// dW[tap][co][ci] = sum over chunks in order; db[co] = sum_v dY[v, co].
+__global__ void k_reduce_dw(const float* __restrict__ partial, int chunks, int n, float* __restrict__ dw)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    float s = 0;
+    for (int c = 0; c < chunks; ++c) s += partial[(int64_t)c * n + e];
+    dw[e] = s;
+}
+
+__global__ void k_bias_partial(const unsigned short* __restrict__ dY, int64_t V, int cout, float* __restrict__ part)
+{
+    // block b sums voxels [b*DW_CHUNK, ...) for every channel in a fixed order
+    int co = threadIdx.x;
+    if (co >= cout) return;
+    int64_t v0 = (int64_t)blockIdx.x * DW_CHUNK;
+    float s = 0;
+    for (int64_t v = v0; v < std::min<int64_t>(v0 + DW_CHUNK, V); ++v) s += bf2f(dY[v * cout + co]);
+    part[(int64_t)blockIdx.x * cout + co] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Loss (mean squared error of prediction = net + residual radiance) and its gradient.
+__global__ void k_loss(const float* __restrict__ out, const float* __restrict__ resid, const float* __restrict__ tgt,
+                       int64_t V, float scale, unsigned short* __restrict__ dout, float* __restrict__ pred,
+                       float* __restrict__ loss_part)
+{
+    __shared__ float sm[256];
+    float l = 0;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x) {
+        for (int c = 0; c < DN_OUT; ++c) {
+            if (c < 3) {
+                float p = out[v * DN_OUT + c] + (resid ? resid[v * 3 + c] : 0.0f);
+                if (pred) pred[v * 3 + c] = p;
+                if (tgt) {
+                    float e = p - tgt[v * 3 + c];
+                    l += e * e;
+                    if (dout) dout[v * DN_OUT + c] = f2bf(2.0f * e * scale);
+                }
+            } else if (dout) {
+                dout[v * DN_OUT + c] = 0;
+            }
+        }
+    }
+    sm[threadIdx.x] = l;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && loss_part) loss_part[blockIdx.x] = sm[0];
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 master parameters.
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       int64_t n, float lr, float b1, float b2, float eps, float c1, float c2)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float gi = g[i];
+    float mi = b1 * m[i] + (1 - b1) * gi;
+    float vi = b2 * v[i] + (1 - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// Pack layer weights W[co][tap][ci] (FP32 master, padded shapes) into the BF16
+// forward layout Wf[co][tap*CIN + ci] and the backward-data layout
+// Wb[ci][tap'*COUT + co] with tap' = 26 - tap.
+__global__ void k_pack_w(const float* __restrict__ W, int cin, int cout, int kpf, int kpb,
+                         unsigned short* __restrict__ wf, unsigned short* __restrict__ wb)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    int n = cout * 27 * cin;
+    if (e < cout * kpf) {
+        int co = e / kpf, k = e % kpf;
+        float w = 0;
+        if (k < 27 * cin) w = W[((int64_t)co * 27 + k / cin) * cin + k % cin];
+        wf[e] = f2bf(w);
+    }
+    if (wb && e < cin * kpb) {
+        int ci = e / kpb, k = e % kpb;
+        float w = 0;
+        if (k < 27 * cout) {
+            int tp = k / cout, co = k % cout;
+            w = W[((int64_t)co * 27 + (26 - tp)) * cin + ci];
+        }
+        wb[e] = f2bf(w);
+    }
+    (void)n;
+}
+
+// Input conversion: f32 [V][cin_real] -> bf16 [V][16]; optional residual copy.
+__global__ void k_pack_input(const float* __restrict__ in, int64_t V, int cin, int resid_ch,
+                             unsigned short* __restrict__ x, float* __restrict__ resid)
+{
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    for (int c = 0; c < DN_IN; ++c) x[v * DN_IN + c] = c < cin ? f2bf(in[v * cin + c]) : 0;
+    if (resid)
+        for (int c = 0; c < 3; ++c) resid[v * 3 + c] = resid_ch >= 0 ? in[v * cin + resid_ch + c] : 0.0f;
+}
+
+// Random training patches: for patch i of the batch, a Philox-drawn volume and
+// corner; gathers inputs (f32 -> packed bf16), residual and target.
+__global__ void k_gather_patches(const float* __restrict__ vin, const float* __restrict__ vtgt, int S, int Dd, int Hh,
+                                 int Ww, int cin, int resid_ch, int P, int batch, uint64_t seed, int step,
+                                 unsigned short* __restrict__ x, float* __restrict__ resid, float* __restrict__ tgt)
+{
+    int64_t V = (int64_t)batch * P * P * P;
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    int px = (int)(v % P), py = (int)((v / P) % P), pz = (int)((v / P / P) % P), bi = (int)(v / P / P / P);
+    uint32_t o[4];
+    philox4x32((uint32_t)bi, (uint32_t)step, 0x5EEDu, 0, (uint32_t)seed, (uint32_t)(seed >> 32), o);
+    int s = (int)(o[0] % (uint32_t)S);
+    int z0 = (int)(o[1] % (uint32_t)(Dd - P + 1)), y0 = (int)(o[2] % (uint32_t)(Hh - P + 1)),
+        x0 = (int)(o[3] % (uint32_t)(Ww - P + 1));
+    int64_t src = (((int64_t)s * Dd + (z0 + pz)) * Hh + (y0 + py)) * Ww + (x0 + px);
+    for (int c = 0; c < DN_IN; ++c) x[v * DN_IN + c] = c < cin ? f2bf(vin[src * cin + c]) : 0;
+    for (int c = 0; c < 3; ++c) {
+        resid[v * 3 + c] = resid_ch >= 0 ? vin[src * cin + resid_ch + c] : 0.0f;
+        tgt[v * 3 + c] = vtgt[src * 3 + c];
+    }
+}
+
+// ---------------------------------------------------------------------------
+
+struct Layer {
+    int cin, cout, relu;
+    int64_t w_off, b_off;  // offsets into the master parameter vector
+    int kpf, kpb;
+    DevBuf wf, wb;         // packed bf16
+};
+
+static int kp_of(int c) { return ((27 * c + 31) / 32) * 32; }
+
+}  // namespace nsp
+
+using namespace nsp;
+
+struct nsp_context;  // defined in nsp_api.cpp
+
+struct nsp_denoiser {
+    nsp_context* ctx;
+    int device;
+    hipStream_t stream;
+    std::string* err;
+    int cin_real;
+    int resid_ch;
+    int64_t nparams = 0;
+    std::vector<Layer> layers;
+    DevBuf params, grads, adam_m, adam_v;
+    int64_t adam_t = 0;
+    // activations of the last forward: act[0] = input (bf16 [V][16]), act[l+1] = layer l output
+    DevBuf act[DN_LAYERS + 1];
+    DevBuf out_f32, resid, target, pred, dy_a, dy_b, partial, bias_part, loss_part;
+    int64_t cap_v = 0;
+};
+
+namespace nsp {
+// from nsp_api.cpp
+hipStream_t context_stream(nsp_context* ctx);
+int context_device(nsp_context* ctx);
+std::string* context_error(nsp_context* ctx);
+}  // namespace nsp
+
+namespace {
+
+void dn_alloc(nsp_denoiser& d, int64_t V)
+{
+    if (V <= d.cap_v) return;
+    d.act[0].alloc(sizeof(unsigned short) * V * DN_IN);
+    for (int l = 0; l < DN_LAYERS; ++l) d.act[l + 1].alloc(sizeof(unsigned short) * V * d.layers[l].cout);
+    d.out_f32.alloc(sizeof(float) * V * DN_OUT);
+    d.resid.alloc(sizeof(float) * V * 3);
+    d.target.alloc(sizeof(float) * V * 3);
+    d.pred.alloc(sizeof(float) * V * 3);
+    d.dy_a.alloc(sizeof(unsigned short) * V * DN_WIDTH);
+    d.dy_b.alloc(sizeof(unsigned short) * V * DN_WIDTH);
+    int64_t chunks = (V + DW_CHUNK - 1) / DW_CHUNK;
+    d.partial.alloc(sizeof(float) * chunks * 27 * DN_WIDTH * DN_WIDTH);
+    d.bias_part.alloc(sizeof(float) * chunks * DN_WIDTH);
+    d.loss_part.alloc(sizeof(float) * 1024);
+    d.cap_v = V;
+}
+
+void pack_weights(nsp_denoiser& d)
+{
+    hipStream_t s = d.stream;
+    for (Layer& L : d.layers) {
+        int n = std::max(L.cout * L.kpf, L.cin * L.kpb);
+        hipLaunchKernelGGL(k_pack_w, dim3((n + 255) / 256), dim3(256), 0, s, d.params.as<float>() + L.w_off, L.cin,
+                           L.cout, L.kpf, L.kpb, L.wf.as<unsigned short>(), L.wb.as<unsigned short>());
+    }
+    NSP_HIP(hipGetLastError());
+}
+
+template <int CIN, int NT>
+void conv_launch(const unsigned short* x, Dims D, const unsigned short* w, const float* bias, int relu,
+                 const unsigned short* mask, unsigned short* out, float* out_f32, hipStream_t s)
+{
+    int64_t waves = (D.voxels() + 63) / 64;
+    unsigned blocks = (unsigned)((waves + 3) / 4);
+    hipLaunchKernelGGL((k_conv<CIN, NT>), dim3(blocks), dim3(256), 0, s, x, D, w, bias, relu, mask, out, out_f32);
+    NSP_HIP(hipGetLastError());
+}
+
+void conv(int cin, int cout, const unsigned short* x, Dims D, const unsigned short* w, const float* bias, int relu,
+          const unsigned short* mask, unsigned short* out, float* out_f32, hipStream_t s)
+{
+    if (cin == 16 && cout == 32) return conv_launch<16, 2>(x, D, w, bias, relu, mask, out, out_f32, s);
+    if (cin == 32 && cout == 32) return conv_launch<32, 2>(x, D, w, bias, relu, mask, out, out_f32, s);
+    if (cin == 32 && cout == 16) return conv_launch<32, 1>(x, D, w, bias, relu, mask, out, out_f32, s);
+    if (cin == 16 && cout == 16) return conv_launch<16, 1>(x, D, w, bias, relu, mask, out, out_f32, s);
+    raise(NSP_E_INVALID, "unsupported conv shape");
+}
+
+void conv_dw(int cin, int cout, const unsigned short* x, Dims D, const unsigned short* dy, float* partial,
+             hipStream_t s)
+{
+    int64_t chunks = (D.voxels() + DW_CHUNK - 1) / DW_CHUNK;
+    dim3 g((unsigned)chunks, 9);
+    if (cin == 16 && cout == 32)
+        hipLaunchKernelGGL((k_conv_dw<16, 2>), g, dim3(256), 0, s, x, D, dy, partial);
+    else if (cin == 32 && cout == 32)
+        hipLaunchKernelGGL((k_conv_dw<32, 2>), g, dim3(256), 0, s, x, D, dy, partial);
+    else if (cin == 32 && cout == 16)
+        hipLaunchKernelGGL((k_conv_dw<32, 1>), g, dim3(256), 0, s, x, D, dy, partial);
+    else
+        raise(NSP_E_INVALID, "unsupported conv shape");
+    NSP_HIP(hipGetLastError());
+}
+
+// forward pass over act[0] (already packed) -> out_f32, pred (if requested)
+void forward(nsp_denoiser& d, Dims D)
+{
+    hipStream_t s = d.stream;
+    for (int l = 0; l < DN_LAYERS; ++l) {
+        Layer& L = d.layers[l];
+        bool last = l == DN_LAYERS - 1;
+        conv(L.cin, L.cout, d.act[l].as<unsigned short>(), D, L.wf.as<unsigned short>(),
+             d.params.as<float>() + L.b_off, L.relu, nullptr, last ? nullptr : d.act[l + 1].as<unsigned short>(),
+             last ? d.out_f32.as<float>() : nullptr, s);
+    }
+}
+
+// backward from dout (bf16 [V][16]) -> grads (FP32, master layout)
+void backward(nsp_denoiser& d, Dims D, const unsigned short* dout)
+{
+    hipStream_t s = d.stream;
+    const int64_t V = D.voxels();
+    const int chunks = (int)((V + DW_CHUNK - 1) / DW_CHUNK);
+    const unsigned short* dy = dout;
+    for (int l = DN_LAYERS - 1; l >= 0; --l) {
+        Layer& L = d.layers[l];
+        // weight gradient
+        conv_dw(L.cin, L.cout, d.act[l].as<unsigned short>(), D, dy, d.partial.as<float>(), s);
+        int n = 27 * L.cout * L.cin;
+        // partials are [chunk][co][tap][ci], the master layout; sum the chunks in order
+        hipLaunchKernelGGL(k_reduce_dw, dim3((n + 255) / 256), dim3(256), 0, s, d.partial.as<float>(), chunks, n,
+                           d.grads.as<float>() + L.w_off);
+        NSP_HIP(hipGetLastError());
+        // bias gradient
+        hipLaunchKernelGGL(k_bias_partial, dim3(chunks), dim3(64), 0, s, dy, V, L.cout, d.bias_part.as<float>());
+        hipLaunchKernelGGL(k_reduce_dw, dim3(1), dim3(64), 0, s, d.bias_part.as<float>(), chunks, L.cout,
+                           d.grads.as<float>() + L.b_off);
+        NSP_HIP(hipGetLastError());
+        if (l == 0) break;
+        // data gradient for layer l-1, masked by its ReLU
+        unsigned short* next = (dy == d.dy_a.as<unsigned short>()) ? d.dy_b.as<unsigned short>()
+                                                                     : d.dy_a.as<unsigned short>();
+        conv(L.cout, L.cin, dy, D, L.wb.as<unsigned short>(), nullptr, 0, d.act[l].as<unsigned short>(), next,
+             nullptr, s);
+        dy = next;
+    }
+}
+
+float loss_and_grad(nsp_denoiser& d, Dims D, bool want_grad)
+{
+    hipStream_t s = d.stream;
+    const int64_t V = D.voxels();
+    const int blocks = 1024;
+    float scale = 1.0f / (float)(3 * V);
+    hipLaunchKernelGGL(k_loss, dim3(blocks), dim3(256), 0, s, d.out_f32.as<float>(), d.resid.as<float>(),
+                       d.target.as<float>(), V, scale, want_grad ? d.dy_a.as<unsigned short>() : nullptr,
+                       d.pred.as<float>(), d.loss_part.as<float>());
+    NSP_HIP(hipGetLastError());
+    std::vector<float> lp(blocks);
+    NSP_HIP(hipMemcpyAsync(lp.data(), d.loss_part.p, sizeof(float) * blocks, hipMemcpyDeviceToHost, s));
+    NSP_HIP(hipStreamSynchronize(s));
+    double sum = 0;
+    for (float x : lp) sum += x;
+    return (float)(sum * scale);
+}
+
+}  // namespace
+
+extern "C" {
+
+int nsp_denoiser_create(nsp_context* ctx, int in_channels, int residual_channel, uint64_t seed, nsp_denoiser** out)
+{
+    if (!ctx || !out) return NSP_E_INVALID;
+    *out = nullptr;
+    std::string* err = context_error(ctx);
+    try {
+        NSP_HIP(hipSetDevice(context_device(ctx)));
+        if (in_channels < 1 || in_channels > DN_IN) raise(NSP_E_INVALID, "denoiser input channels must be in [1, 16]");
+        if (residual_channel >= 0 && residual_channel + 3 > in_channels)
+            raise(NSP_E_INVALID, "residual channels out of range");
+        auto d = std::make_unique<nsp_denoiser>();
+        d->ctx = ctx;
+        d->device = context_device(ctx);
+        d->stream = context_stream(ctx);
+        d->err = err;
+        d->cin_real = in_channels;
+        d->resid_ch = residual_channel;
+        int64_t off = 0;
+        for (int l = 0; l < DN_LAYERS; ++l) {
+            Layer L;
+            L.cin = l == 0 ? DN_IN : DN_WIDTH;
+            L.cout = l == DN_LAYERS - 1 ? DN_OUT : DN_WIDTH;
+            L.relu = l < DN_LAYERS - 1;
+            L.w_off = off;
+            off += (int64_t)L.cout * 27 * L.cin;
+            L.b_off = off;
+            off += L.cout;
+            L.kpf = kp_of(L.cin);
+            L.kpb = kp_of(L.cout);
+            d->layers.push_back(std::move(L));
+        }
+        d->nparams = off;
+        // He-uniform initialisation over the real fan-in; padded entries stay zero
+        std::vector<float> p(off, 0.0f);
+        uint64_t st = seed;
+        auto next = [&]() {
+            uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            z ^= z >> 31;
+            return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+        };
+        for (int l = 0; l < DN_LAYERS; ++l) {
+            const Layer& L = d->layers[l];
+            int ci_real = l == 0 ? in_channels : DN_WIDTH;
+            int co_real = l == DN_LAYERS - 1 ? 3 : DN_WIDTH;
+            double bound = std::sqrt(6.0 / (27.0 * ci_real)) * (l == DN_LAYERS - 1 ? 0.1 : 1.0);
+            for (int co = 0; co < co_real; ++co)
+                for (int t = 0; t < 27; ++t)
+                    for (int ci = 0; ci < ci_real; ++ci)
+                        p[L.w_off + ((int64_t)co * 27 + t) * L.cin + ci] = (float)((2 * next() - 1) * bound);
+        }
+        d->params.alloc(sizeof(float) * off);
+        d->grads.alloc(sizeof(float) * off);
+        d->adam_m.alloc(sizeof(float) * off);
+        d->adam_v.alloc(sizeof(float) * off);
+        NSP_HIP(hipMemcpy(d->params.p, p.data(), sizeof(float) * off, hipMemcpyHostToDevice));
+        NSP_HIP(hipMemset(d->adam_m.p, 0, sizeof(float) * off));
+        NSP_HIP(hipMemset(d->adam_v.p, 0, sizeof(float) * off));
+        for (Layer& L : d->layers) {
+            L.wf.alloc(sizeof(unsigned short) * L.cout * L.kpf);
+            L.wb.alloc(sizeof(unsigned short) * L.cin * L.kpb);
+        }
+        pack_weights(*d);
+        NSP_HIP(hipStreamSynchronize(d->stream));
+        *out = d.release();
+        return NSP_OK;
+    } catch (const Error& e) {
+        *err = e.msg;
+        return e.code;
+    }
+}
+
+void nsp_denoiser_destroy(nsp_denoiser* d)
+{
+    if (!d) return;
+    (void)hipSetDevice(d->device);
+    (void)hipDeviceSynchronize();
+    delete d;
+}
+
+int64_t nsp_denoiser_param_count(const nsp_denoiser* d) { return d ? d->nparams : NSP_E_INVALID; }
+
+int nsp_denoiser_get_params(const nsp_denoiser* d, float* out)
+{
+    if (!d || !out) return NSP_E_INVALID;
+    try {
+        NSP_HIP(hipSetDevice(d->device));
+        NSP_HIP(hipMemcpy(out, d->params.p, sizeof(float) * d->nparams, hipMemcpyDeviceToHost));
+        return NSP_OK;
+    } catch (const Error& e) {
+        *d->err = e.msg;
+        return e.code;
+    }
+}
+
+int nsp_denoiser_set_params(nsp_denoiser* d, const float* in)
+{
+    if (!d || !in) return NSP_E_INVALID;
+    try {
+        NSP_HIP(hipSetDevice(d->device));
+        NSP_HIP(hipMemcpy(d->params.p, in, sizeof(float) * d->nparams, hipMemcpyHostToDevice));
+        NSP_HIP(hipMemset(d->adam_m.p, 0, sizeof(float) * d->nparams));
+        NSP_HIP(hipMemset(d->adam_v.p, 0, sizeof(float) * d->nparams));
+        d->adam_t = 0;
+        pack_weights(*d);
+        NSP_HIP(hipStreamSynchronize(d->stream));
+        return NSP_OK;
+    } catch (const Error& e) {
+        *d->err = e.msg;
+        return e.code;
+    }
+}
+
+static Dims dims_of(int nb, int dd, int hh, int ww) { return Dims{nb, dd, hh, ww}; }
+
+int nsp_denoiser_forward(nsp_denoiser* d, const float* input, int nb, int depth, int height, int width, float* output)
+{
+    if (!d || !input || !output || nb < 1 || depth < 1 || height < 1 || width < 1) return NSP_E_INVALID;
+    try {
+        NSP_HIP(hipSetDevice(d->device));
+        Dims D = dims_of(nb, depth, height, width);
+        const int64_t V = D.voxels();
+        dn_alloc(*d, V);
+        hipStream_t s = d->stream;
+        DevBuf in;
+        in.alloc(sizeof(float) * V * d->cin_real);
+        NSP_HIP(hipMemcpyAsync(in.p, input, sizeof(float) * V * d->cin_real, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_pack_input, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s, in.as<float>(), V,
+                           d->cin_real, d->resid_ch, d->act[0].as<unsigned short>(), d->resid.as<float>());
+        NSP_HIP(hipGetLastError());
+        forward(*d, D);
+        hipLaunchKernelGGL(k_loss, dim3(1024), dim3(256), 0, s, d->out_f32.as<float>(), d->resid.as<float>(),
+                           (const float*)nullptr, V, 1.0f, (unsigned short*)nullptr, d->pred.as<float>(),
+                           (float*)nullptr);
+        NSP_HIP(hipGetLastError());
+        NSP_HIP(hipMemcpyAsync(output, d->pred.p, sizeof(float) * V * 3, hipMemcpyDeviceToHost, s));
+        NSP_HIP(hipStreamSynchronize(s));
+        return NSP_OK;
+    } catch (const Error& e) {
+        *d->err = e.msg;
+        return e.code;
+    }
+}
+
+int nsp_denoiser_loss_grad(nsp_denoiser* d, const float* input, const float* target, int nb, int depth, int height,
+                           int width, float* loss, float* grad)
+{
+    if (!d || !input || !target || nb < 1 || depth < 1 || height < 1 || width < 1) return NSP_E_INVALID;
+    try {
+        NSP_HIP(hipSetDevice(d->device));
+        Dims D = dims_of(nb, depth, height, width);
+        const int64_t V = D.voxels();
+        dn_alloc(*d, V);
+        hipStream_t s = d->stream;
+        DevBuf in;
+        in.alloc(sizeof(float) * V * d->cin_real);
+        NSP_HIP(hipMemcpyAsync(in.p, input, sizeof(float) * V * d->cin_real, hipMemcpyHostToDevice, s));
+        NSP_HIP(hipMemcpyAsync(d->target.p, target, sizeof(float) * V * 3, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_pack_input, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s, in.as<float>(), V,
+                           d->cin_real, d->resid_ch, d->act[0].as<unsigned short>(), d->resid.as<float>());
+        forward(*d, D);
+        float l = loss_and_grad(*d, D, true);
+        backward(*d, D, d->dy_a.as<unsigned short>());
+        if (loss) *loss = l;
+        // the context stream is non-blocking: copy on it, then wait
+        if (grad) NSP_HIP(hipMemcpyAsync(grad, d->grads.p, sizeof(float) * d->nparams, hipMemcpyDeviceToHost, s));
+        NSP_HIP(hipStreamSynchronize(s));
+        return NSP_OK;
+    } catch (const Error& e) {
+        *d->err = e.msg;
+        return e.code;
+    }
+}
+
+int nsp_denoiser_train(nsp_denoiser* d, const float* inputs, const float* targets, int volumes, int depth, int height,
+                       int width, int patch, int batch, int steps, float lr, uint64_t seed, float* losses)
+{
+    if (!d || !inputs || !targets || volumes < 1 || patch < 4 || batch < 1 || steps < 0) return NSP_E_INVALID;
+    if (patch > depth || patch > height || patch > width) return NSP_E_INVALID;
+    try {
+        NSP_HIP(hipSetDevice(d->device));
+        hipStream_t s = d->stream;
+        const int64_t VV = (int64_t)volumes * depth * height * width;
+        DevBuf vin, vtg;
+        vin.alloc(sizeof(float) * VV * d->cin_real);
+        vtg.alloc(sizeof(float) * VV * 3);
+        NSP_HIP(hipMemcpyAsync(vin.p, inputs, sizeof(float) * VV * d->cin_real, hipMemcpyHostToDevice, s));
+        NSP_HIP(hipMemcpyAsync(vtg.p, targets, sizeof(float) * VV * 3, hipMemcpyHostToDevice, s));
+        Dims D = dims_of(batch, patch, patch, patch);
+        const int64_t V = D.voxels();
+        dn_alloc(*d, V);
+        const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+        for (int st = 0; st < steps; ++st) {
+            hipLaunchKernelGGL(k_gather_patches, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s, vin.as<float>(),
+                               vtg.as<float>(), volumes, depth, height, width, d->cin_real, d->resid_ch, patch, batch,
+                               seed, st, d->act[0].as<unsigned short>(), d->resid.as<float>(), d->target.as<float>());
+            NSP_HIP(hipGetLastError());
+            forward(*d, D);
+            float l = loss_and_grad(*d, D, true);
+            backward(*d, D, d->dy_a.as<unsigned short>());
+            d->adam_t += 1;
+            float c1 = 1.0f - std::pow(b1, (float)d->adam_t), c2 = 1.0f - std::pow(b2, (float)d->adam_t);
+            hipLaunchKernelGGL(k_adam, dim3((unsigned)((d->nparams + 255) / 256)), dim3(256), 0, s,
+                               d->params.as<float>(), d->grads.as<float>(), d->adam_m.as<float>(),
+                               d->adam_v.as<float>(), d->nparams, lr, b1, b2, eps, c1, c2);
+            NSP_HIP(hipGetLastError());
+            pack_weights(*d);
+            if (losses) losses[st] = l;
+        }
+        NSP_HIP(hipStreamSynchronize(s));
+        return NSP_OK;
+    } catch (const Error& e) {
+        *d->err = e.msg;
+        return e.code;
+    }
+}
+
+// Stand-alone BF16 implicit-GEMM convolution for kernel-level tests:
+// x [V][cin] f32 (cin in {16, 32}), w [cout][27][cin] f32, bias [cout] or NULL,
+// y [V][cout] f32 (cout in {16, 32}).
+int nsp_conv3d_bf16(nsp_context* ctx, const float* x, int nb, int depth, int height, int width, int cin,
+                    const float* w, const float* bias, int cout, int relu, float* y)
+{
+    if (!ctx || !x || !w || !y) return NSP_E_INVALID;
+    std::string* err = context_error(ctx);
+    try {
+        NSP_HIP(hipSetDevice(context_device(ctx)));
+        if ((cin != 16 && cin != 32) || (cout != 16 && cout != 32)) raise(NSP_E_INVALID, "cin/cout must be 16 or 32");
+        hipStream_t s = context_stream(ctx);
+        Dims D = dims_of(nb, depth, height, width);
+        const int64_t V = D.voxels();
+        DevBuf dx, dxf, dw, dwf, dwb, db, dy;
+        dxf.alloc(sizeof(float) * V * cin);
+        dx.alloc(sizeof(unsigned short) * V * cin);
+        dw.alloc(sizeof(float) * cout * 27 * cin);
+        int kpf = kp_of(cin), kpb = kp_of(cout);
+        dwf.alloc(sizeof(unsigned short) * cout * kpf);
+        dwb.alloc(sizeof(unsigned short) * cin * kpb);
+        dy.alloc(sizeof(float) * V * cout);
+        NSP_HIP(hipMemcpyAsync(dxf.p, x, sizeof(float) * V * cin, hipMemcpyHostToDevice, s));
+        NSP_HIP(hipMemcpyAsync(dw.p, w, sizeof(float) * cout * 27 * cin, hipMemcpyHostToDevice, s));
+        if (bias) {
+            db.alloc(sizeof(float) * cout);
+            NSP_HIP(hipMemcpyAsync(db.p, bias, sizeof(float) * cout, hipMemcpyHostToDevice, s));
+        }
+        // f32 -> bf16 input (channels-last, cin already the padded count)
+        {
+            std::vector<unsigned short> hx((size_t)V * cin);
+            for (int64_t i = 0; i < V * cin; ++i) {
+                float f = x[i];
+                uint32_t u;
+                std::memcpy(&u, &f, 4);
+                uint32_t r = ((u >> 16) & 1) + 0x7fff;  // round to nearest even
+                hx[i] = (unsigned short)((u + r) >> 16);
+            }
+            NSP_HIP(hipMemcpyAsync(dx.p, hx.data(), sizeof(unsigned short) * hx.size(), hipMemcpyHostToDevice, s));
+            NSP_HIP(hipStreamSynchronize(s));
+        }
+        int n = std::max(cout * kpf, cin * kpb);
+        hipLaunchKernelGGL(k_pack_w, dim3((n + 255) / 256), dim3(256), 0, s, dw.as<float>(), cin, cout, kpf, kpb,
+                           dwf.as<unsigned short>(), dwb.as<unsigned short>());
+        conv(cin, cout, dx.as<unsigned short>(), D, dwf.as<unsigned short>(), bias ? db.as<float>() : nullptr, relu,
+             nullptr, nullptr, dy.as<float>(), s);
+        NSP_HIP(hipMemcpyAsync(y, dy.p, sizeof(float) * V * cout, hipMemcpyDeviceToHost, s));
+        NSP_HIP(hipStreamSynchronize(s));
+        return NSP_OK;
+    } catch (const Error& e) {
+        *err = e.msg;
+        return e.code;
+    }
+}
+
+}  // extern "C"

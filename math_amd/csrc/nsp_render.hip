@@ -16,6 +16,7 @@
 
 #include "nsp_device.hpp"
 #include "nsp_render.hpp"
+#include "nsp_stages.hpp"
 #include "nsp_trace.hpp"
 
 namespace nsp {
@@ -161,31 +162,12 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_raygen(SceneK<T> sc, const Batc
     const int64_t pixel0 = desc->pixel0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t p = (int64_t)xcd_block() * blockDim.x + threadIdx.x; p < paths; p += stride) {
-        int64_t pixel = pixel0 + p / sc.S;
-        int sample = (int)(p % sc.S);
-        T u[8];
-        uniforms4<T>(sc.seed, pixel, sample, sc.pass, 0, 0, u);
-        if constexpr (N - 1 > 4) uniforms4<T>(sc.seed, pixel, sample, sc.pass, 0, 1, u + 4);
-        int64_t rem = pixel;
-        int sr = sample;
-        T sp[N - 1];
-#pragma unroll
-        for (int i = 0; i < N - 1; ++i) {
-            int64_t x = rem % sc.screen[i];
-            rem /= sc.screen[i];
-            int dgt = sr % sc.k;
-            sr /= sc.k;
-            T off = (T)dgt / (T)sc.k;
-            T spt = off + u[i] * sc.k_inv;
-            sp[i] = (-(T)sc.screen[i] * (T)0.5) + ((T)x + spt);
-        }
+        T o[N], d[N];
+        primary_ray<N, T>(sc, pixel0 + p / sc.S, (int)(p % sc.S), o, d);
 #pragma unroll
         for (int j = 0; j < N; ++j) {
-            T sd = sc.axes[0][j] * sp[0];
-#pragma unroll
-            for (int i = 1; i < N - 1; ++i) sd = sd + sc.axes[i][j] * sp[i];
-            q.org[(int64_t)j * cap + p] = sc.cam_org[j] + sd;
-            q.dir[(int64_t)j * cap + p] = sc.cam_dir[j];
+            q.org[(int64_t)j * cap + p] = o[j];
+            q.dir[(int64_t)j * cap + p] = d[j];
         }
         q.path[p] = (int32_t)p;
         q.beta[p] = 1.0f;
@@ -223,57 +205,6 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_closest(TreeArgs A, RayQueue<T>
 }
 
 template <int N, typename T>
-__device__ __forceinline__ void load_normal(const MeshShadeArgs& M, int idx, T* gn)
-{
-    if constexpr (sizeof(T) == 4) {
-#pragma unroll
-        for (int k = 0; k < N; ++k) gn[k] = M.normal32[(int64_t)idx * N + k];
-    } else {
-#pragma unroll
-        for (int k = 0; k < N; ++k) gn[k] = M.rec64[(int64_t)idx * rec64_stride(N) + k];
-    }
-}
-
-// MeshHyperplaneSimplex::shading_normal, src/painter/shapes/mesh_hyperplane_simplex.cpp:157-186
-template <int N, typename T>
-__device__ void shading_normal(const MeshShadeArgs& M, int idx, const T* p, const T* gn, T* sn)
-{
-    int nt = M.ntype[idx];
-    int type = nt & 3;
-    if (type == 0) {
-#pragma unroll
-        for (int k = 0; k < N; ++k) sn[k] = gn[k];
-        return;
-    }
-    const double* r = M.rec64 + (int64_t)idx * rec64_stride(N);
-    double pd[N];
-#pragma unroll
-    for (int k = 0; k < N; ++k) pd[k] = p[k];
-    double bc[N];
-    bc[N - 1] = 1;
-#pragma unroll
-    for (int i = 0; i < N - 1; ++i) {
-        bc[i] = dot<N, double>(pd, r + 2 * N + i * N) - r[2 * N + (N - 1) * N + i];
-        bc[N - 1] -= bc[i];
-    }
-    double acc[N];
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
-        const double* vn = M.vn64 + (int64_t)M.fnrm[(int64_t)idx * N + k] * N;
-        double sgn = (type == 2 && ((nt >> (8 + k)) & 1)) ? -1.0 : 1.0;
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-            double term = bc[k] * (sgn * vn[i]);
-            acc[i] = (k == 0) ? term : acc[i] + term;
-        }
-    }
-    normalize<N, double>(acc);
-#pragma unroll
-    for (int k = 0; k < N; ++k) sn[k] = (T)acc[k];
-    normalize<N, T>(sn);
-}
-
-template <int N, typename T>
 __global__ void __launch_bounds__(TRACE_BLOCK)
     k_shade(SceneK<T> sc, TreeArgs A, MeshShadeArgs M, const BatchDesc* __restrict__ desc, int depth,
             RayQueue<T> qin, RayQueue<T> qout, ShadowQueue<T> sq, int64_t cap, unsigned long long* __restrict__ cnt,
@@ -288,201 +219,61 @@ __global__ void __launch_bounds__(TRACE_BLOCK)
     const int64_t pixel0 = desc->pixel0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t base = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+    const int64_t scap = cap * MAX_LIGHTS;
     // every lane of a wave iterates the same number of times (wave-level appends)
     const int64_t iters = (count + stride - 1) / stride;
     for (int64_t it = 0; it < iters; ++it) {
         const int64_t i = base + it * stride;
         const bool valid = i < count;
-        bool want_next = false;
-        bool want_shadow[MAX_LIGHTS] = {false};
-        T p[N], nd[N];
-        T sh_org[MAX_LIGHTS][N], sh_dir[MAX_LIGHTS][N], sh_tmax[MAX_LIGHTS];
-        float sh_c[MAX_LIGHTS][3];
-        float nb[3] = {0, 0, 0};
-        float new_level = 0;
+        ShadeOut<N, T> out;
+        out.next = false;
+        out.shadow = false;
         int32_t path = 0;
         if (valid) {
             path = qin.path[i];
             float beta[3] = {qin.beta[i], qin.beta[cap + i], qin.beta[2 * cap + i]};
-            int idx = qin.hit[i];
             T o[N], d[N];
 #pragma unroll
             for (int k = 0; k < N; ++k) {
                 o[k] = qin.org[(int64_t)k * cap + i];
                 d[k] = qin.dir[(int64_t)k * cap + i];
             }
-            const bool diffuse_reflection = depth > 0;
-            if (aov && depth == 0) {
-                // first-hit albedo and normal facing the camera (denoiser inputs); background on a miss
-                T an[N];
-                if (idx >= 0) {
-                    load_normal<N, T>(M, idx, an);
-                    if (dot<N, T>(d, an) > (T)0)
+            const int64_t gp = pixel0 * sc.S + path;
+            float av[3 + N];
+            shade_one<N, T>(sc, A, M, depth, gp / sc.S, (int)(gp % sc.S), o, d, qin.hit[i], qin.hit_t[i], beta,
+                            qin.level[i], st, probe_count, (aov && depth == 0) ? av : nullptr, out);
 #pragma unroll
-                        for (int k = 0; k < N; ++k) an[k] = -an[k];
-                }
+            for (int c = 0; c < 3; ++c) L[c * cap + path] += out.add[c];
+            if (aov && depth == 0)
 #pragma unroll
-                for (int c = 0; c < 3; ++c) aov[c * cap + path] = idx >= 0 ? sc.diffuse * sc.obj_rgb[c] : sc.bg_rgb[c];
-#pragma unroll
-                for (int k = 0; k < N; ++k) aov[(3 + k) * cap + path] = idx >= 0 ? (float)an[k] : 0.0f;
-            }
-            if (idx < 0) {
-                // miss: background, a light source when reached by a diffuse bounce (painter.cpp:335-340)
-                const float* bg = (sc.bg_is_light && diffuse_reflection) ? sc.bg_light_rgb : sc.bg_rgb;
-#pragma unroll
-                for (int c = 0; c < 3; ++c) L[c * cap + path] += beta[c] * bg[c];
-            } else {
-                T t = qin.hit_t[i];
-#pragma unroll
-                for (int k = 0; k < N; ++k) p[k] = o[k] + d[k] * t;
-                T gn[N], sn[N];
-                load_normal<N, T>(M, idx, gn);
-                T dgn = dot<N, T>(d, gn);
-                bool go = !(fabs(dgn) <= (T)0);
-                if (go && sc.obj_is_light) {
-                    const float* lc = diffuse_reflection ? sc.obj_light_rgb : sc.obj_rgb;
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) L[c * cap + path] += beta[c] * lc[c];
-                    go = false;
-                }
-                if (go) {
-                    if (sc.smooth && sc.has_normals)
-                        shading_normal<N, T>(M, idx, p, gn, sn);
-                    else {
-#pragma unroll
-                        for (int k = 0; k < N; ++k) sn[k] = gn[k];
-                    }
-                    if (dgn > 0) {
-#pragma unroll
-                        for (int k = 0; k < N; ++k) {
-                            gn[k] = -gn[k];
-                            sn[k] = -sn[k];
-                        }
-                    }
-                    go = sc.diffuse > 0;
-                }
-                if (go) {
-                    float scol[3] = {sc.diffuse * sc.obj_rgb[0], sc.diffuse * sc.obj_rgb[1],
-                                     sc.diffuse * sc.obj_rgb[2]};
-                    float mx = fmaxf(scol[0], fmaxf(scol[1], scol[2]));
-                    new_level = qin.level[i] * mx;
-                    if (new_level >= 1e-4f) {
-                        // direct_diffuse_lighting, painter.cpp:197-292
-                        for (int l = 0; l < sc.light_count; ++l) {
-                            T vl[N];
-#pragma unroll
-                            for (int k = 0; k < N; ++k) vl[k] = sc.light_pos[l][k] - p[k];
-                            T sq2 = dot<N, T>(vl, vl);
-                            T scale;
-                            if (sc.light_kind[l] == 1)
-                                scale = (T)1;
-                            else if constexpr (N == 3)
-                                scale = sc.light_coef[l] / sq2;
-                            else
-                                scale = sc.light_coef[l] / pow(sq2, (T)(N - 1) / (T)2);
-                            float lc[3];
-#pragma unroll
-                            for (int c = 0; c < 3; ++c) lc[c] = sc.light_rgb[l][c] * (float)scale;
-                            if (fmaxf(lc[0], fmaxf(lc[1], lc[2])) < 1e-4f) continue;
-                            T ld[N];
-#pragma unroll
-                            for (int k = 0; k < N; ++k) ld[k] = vl[k];
-                            normalize<N, T>(ld);
-                            T dln = dot<N, T>(ld, sn);
-                            if (dln <= (T)0) continue;
-                            T dist = sqrt(dot<N, T>(vl, vl));
-                            T o1[N];
-#pragma unroll
-                            for (int k = 0; k < N; ++k) o1[k] = p[k] + ld[k] * sc.ray_offset;
-                            T tmaxs = dist;
-                            if (sc.smooth && dot<N, T>(ld, gn) < (T)0) {
-                                // smooth-normal back face: forward and reverse probes (painter.cpp:238-284)
-                                atomicAdd(probe_count, 1ull);
-                                double o64[N], d64[N];
-#pragma unroll
-                                for (int k = 0; k < N; ++k) {
-                                    o64[k] = o1[k];
-                                    d64[k] = ld[k];
-                                }
-                                HitResult h1 = traverse<N, T, false>(A, o1, ld, (T)INFINITY, st, o64, d64);
-                                if (h1.idx < 0) continue;
-                                T tf = (T)h1.t;
-                                if (tf >= dist) continue;
-                                atomicAdd(probe_count, 1ull);
-                                T rd[N], ro[N];
-#pragma unroll
-                                for (int k = 0; k < N; ++k) {
-                                    rd[k] = -ld[k];
-                                    ro[k] = o1[k] + rd[k] * ((T)2 * sc.ray_offset);
-                                    o64[k] = ro[k];
-                                    d64[k] = rd[k];
-                                }
-                                HitResult h2 = traverse<N, T, false>(A, ro, rd, (T)INFINITY, st, o64, d64);
-                                if (h2.idx >= 0 && (T)h2.t < tf) continue;
-#pragma unroll
-                                for (int k = 0; k < N; ++k) o1[k] = o1[k] + ld[k] * (tf + sc.ray_offset);
-                                tmaxs = dist - tf;
-                            }
-                            float w = (float)(sc.cos_coef * dln);
-                            want_shadow[l] = true;
-#pragma unroll
-                            for (int k = 0; k < N; ++k) {
-                                sh_org[l][k] = o1[k];
-                                sh_dir[l][k] = ld[k];
-                            }
-                            sh_tmax[l] = tmaxs;
-#pragma unroll
-                            for (int c = 0; c < 3; ++c) sh_c[l][c] = beta[c] * (scol[c] * (lc[c] * w));
-                        }
-                        // diffuse_lighting, painter.cpp:298-323
-                        if (depth < sc.max_depth) {
-                            int64_t gp = (pixel0 * sc.S) + path;
-                            int64_t pixel = gp / sc.S;
-                            int sample = (int)(gp % sc.S);
-                            T u[8];
-                            uniforms4<T>(sc.seed, pixel, sample, sc.pass, depth, 2, u);
-                            if constexpr (N >= 6) uniforms4<T>(sc.seed, pixel, sample, sc.pass, depth, 3, u + 4);
-                            cosine_hemisphere<N, T>(sn, u, nd);
-                            normalize<N, T>(nd);
-                            if (dot<N, T>(nd, gn) > (T)0) {
-                                want_next = true;
-#pragma unroll
-                                for (int c = 0; c < 3; ++c) nb[c] = beta[c] * scol[c];
-                            }
-                        }
-                    }
-                }
-            }
+                for (int c = 0; c < 3 + N; ++c) aov[c * cap + path] = av[c];
         }
         // compaction: one atomic per wave per queue
-        int64_t slot = wave_append(want_next, next_count);
+        int64_t slot = wave_append(out.next, next_count);
         if (slot >= 0) {
 #pragma unroll
             for (int k = 0; k < N; ++k) {
-                qout.org[(int64_t)k * cap + slot] = p[k] + nd[k] * sc.ray_offset;
-                qout.dir[(int64_t)k * cap + slot] = nd[k];
+                qout.org[(int64_t)k * cap + slot] = out.norg[k];
+                qout.dir[(int64_t)k * cap + slot] = out.ndir[k];
             }
             qout.path[slot] = path;
-            qout.beta[slot] = nb[0];
-            qout.beta[cap + slot] = nb[1];
-            qout.beta[2 * cap + slot] = nb[2];
-            qout.level[slot] = new_level;
+            qout.beta[slot] = out.nbeta[0];
+            qout.beta[cap + slot] = out.nbeta[1];
+            qout.beta[2 * cap + slot] = out.nbeta[2];
+            qout.level[slot] = out.nlevel;
         }
-        for (int l = 0; l < MAX_LIGHTS; ++l) {
-            if (l >= sc.light_count) break;
-            int64_t ss = wave_append(want_shadow[l], shadow_count);
-            if (ss >= 0) {
+        int64_t ss = wave_append(out.shadow, shadow_count);
+        if (ss >= 0) {
 #pragma unroll
-                for (int k = 0; k < N; ++k) {
-                    sq.org[(int64_t)k * cap * MAX_LIGHTS + ss] = sh_org[l][k];
-                    sq.dir[(int64_t)k * cap * MAX_LIGHTS + ss] = sh_dir[l][k];
-                }
-                sq.tmax[ss] = sh_tmax[l];
-                sq.path[ss] = path;
-                sq.contrib[ss] = sh_c[l][0];
-                sq.contrib[cap * MAX_LIGHTS + ss] = sh_c[l][1];
-                sq.contrib[2 * cap * MAX_LIGHTS + ss] = sh_c[l][2];
+            for (int k = 0; k < N; ++k) {
+                sq.org[(int64_t)k * scap + ss] = out.sorg[k];
+                sq.dir[(int64_t)k * scap + ss] = out.sdir[k];
             }
+            sq.tmax[ss] = out.stmax;
+            sq.path[ss] = path;
+            sq.contrib[ss] = out.contrib[0];
+            sq.contrib[scap + ss] = out.contrib[1];
+            sq.contrib[2 * scap + ss] = out.contrib[2];
         }
     }
 }

@@ -12,7 +12,9 @@
 
 namespace nsp {
 
-constexpr int MAX_LIGHTS = 4;
+// One light per scene: each path then has at most one shadow ray per bounce,
+// so the shadow stage adds to a path's radiance without atomics.
+constexpr int MAX_LIGHTS = 1;
 
 // Scene constants passed by value to the render kernels (the data of
 // PaintObjects: src/painter/objects.h:212-223, with the derived constants of

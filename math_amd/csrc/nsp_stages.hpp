@@ -1,0 +1,263 @@
+// nsp_stages.hpp -- per-ray bodies of the wavefront stages, shared by the
+// single-scene kernels (nsp_render.hip) and the multi-scene persistent
+// kernels (nsp_batch.hip).
+#pragma once
+
+#include "nsp_device.hpp"
+#include "nsp_render.hpp"
+#include "nsp_trace.hpp"
+
+namespace nsp {
+
+// paint_pixels' sample + ParallelProjector::ray (src/painter/painter.cpp:419-442,
+// src/painter/sampling/engine.h:86-156, src/painter/projectors/projector.h:194-201)
+template <int N, typename T>
+__device__ __forceinline__ void primary_ray(const SceneK<T>& sc, int64_t pixel, int sample, T* org, T* dir)
+{
+    T u[8];
+    uniforms4<T>(sc.seed, pixel, sample, sc.pass, 0, 0, u);
+    if constexpr (N - 1 > 4) uniforms4<T>(sc.seed, pixel, sample, sc.pass, 0, 1, u + 4);
+    int64_t rem = pixel;
+    int sr = sample;
+    T sp[N - 1];
+#pragma unroll
+    for (int i = 0; i < N - 1; ++i) {
+        int64_t x = rem % sc.screen[i];
+        rem /= sc.screen[i];
+        int dgt = sr % sc.k;
+        sr /= sc.k;
+        T off = (T)dgt / (T)sc.k;
+        T spt = off + u[i] * sc.k_inv;
+        sp[i] = (-(T)sc.screen[i] * (T)0.5) + ((T)x + spt);
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        T sd = sc.axes[0][j] * sp[0];
+#pragma unroll
+        for (int i = 1; i < N - 1; ++i) sd = sd + sc.axes[i][j] * sp[i];
+        org[j] = sc.cam_org[j] + sd;
+        dir[j] = sc.cam_dir[j];
+    }
+}
+
+template <int N, typename T>
+__device__ __forceinline__ void load_normal(const MeshShadeArgs& M, int idx, T* gn)
+{
+    if constexpr (sizeof(T) == 4) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) gn[k] = M.normal32[(int64_t)idx * N + k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < N; ++k) gn[k] = M.rec64[(int64_t)idx * rec64_stride(N) + k];
+    }
+}
+
+// MeshHyperplaneSimplex::shading_normal, src/painter/shapes/mesh_hyperplane_simplex.cpp:157-186
+template <int N, typename T>
+__device__ void shading_normal(const MeshShadeArgs& M, int idx, const T* p, const T* gn, T* sn)
+{
+    int nt = M.ntype[idx];
+    int type = nt & 3;
+    if (type == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) sn[k] = gn[k];
+        return;
+    }
+    const double* r = M.rec64 + (int64_t)idx * rec64_stride(N);
+    double pd[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) pd[k] = p[k];
+    double bc[N];
+    bc[N - 1] = 1;
+#pragma unroll
+    for (int i = 0; i < N - 1; ++i) {
+        bc[i] = dot<N, double>(pd, r + 2 * N + i * N) - r[2 * N + (N - 1) * N + i];
+        bc[N - 1] -= bc[i];
+    }
+    double acc[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const double* vn = M.vn64 + (int64_t)M.fnrm[(int64_t)idx * N + k] * N;
+        double sgn = (type == 2 && ((nt >> (8 + k)) & 1)) ? -1.0 : 1.0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            double term = bc[k] * (sgn * vn[i]);
+            acc[i] = (k == 0) ? term : acc[i] + term;
+        }
+    }
+    normalize<N, double>(acc);
+#pragma unroll
+    for (int k = 0; k < N; ++k) sn[k] = (T)acc[k];
+    normalize<N, T>(sn);
+}
+
+template <int N, typename T>
+struct ShadeOut {
+    float add[3];      // radiance added to the path now (background or emitter)
+    bool next;         // a diffuse bounce ray continues the path
+    T norg[N], ndir[N];
+    float nbeta[3];
+    float nlevel;
+    bool shadow;       // a shadow ray towards the light
+    T sorg[N], sdir[N];
+    T stmax;
+    float contrib[3];  // radiance added when the shadow ray is unoccluded
+};
+
+// One path vertex: trace_path's shading, direct_diffuse_lighting's shadow-ray
+// set-up and diffuse_lighting's bounce (src/painter/painter.cpp:197-397), for
+// the ray (o, d) whose nearest hit is (idx, t).  aov (3 + N floats) receives
+// the first-hit albedo and normal when non-null.
+template <int N, typename T>
+__device__ void shade_one(const SceneK<T>& sc, const TreeArgs& A, const MeshShadeArgs& M, int depth, int64_t pixel,
+                          int sample, const T* o, const T* d, int idx, T t, const float* beta, float level, int* st,
+                          unsigned long long* probe_count, float* aov, ShadeOut<N, T>& out)
+{
+    out.add[0] = out.add[1] = out.add[2] = 0.0f;
+    out.next = false;
+    out.shadow = false;
+    const bool diffuse_reflection = depth > 0;
+    if (aov) {
+        T an[N];
+        if (idx >= 0) {
+            load_normal<N, T>(M, idx, an);
+            if (dot<N, T>(d, an) > (T)0)
+#pragma unroll
+                for (int k = 0; k < N; ++k) an[k] = -an[k];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) aov[c] = idx >= 0 ? sc.diffuse * sc.obj_rgb[c] : sc.bg_rgb[c];
+#pragma unroll
+        for (int k = 0; k < N; ++k) aov[3 + k] = idx >= 0 ? (float)an[k] : 0.0f;
+    }
+    if (idx < 0) {
+        // miss: background, a light source when reached by a diffuse bounce (painter.cpp:335-340)
+        const float* bg = (sc.bg_is_light && diffuse_reflection) ? sc.bg_light_rgb : sc.bg_rgb;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) out.add[c] = beta[c] * bg[c];
+        return;
+    }
+    T p[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) p[k] = o[k] + d[k] * t;
+    T gn[N], sn[N];
+    load_normal<N, T>(M, idx, gn);
+    T dgn = dot<N, T>(d, gn);
+    if (fabs(dgn) <= (T)0) return;
+    if (sc.obj_is_light) {
+        const float* lc = diffuse_reflection ? sc.obj_light_rgb : sc.obj_rgb;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) out.add[c] = beta[c] * lc[c];
+        return;
+    }
+    if (sc.smooth && sc.has_normals)
+        shading_normal<N, T>(M, idx, p, gn, sn);
+    else {
+#pragma unroll
+        for (int k = 0; k < N; ++k) sn[k] = gn[k];
+    }
+    if (dgn > 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            gn[k] = -gn[k];
+            sn[k] = -sn[k];
+        }
+    }
+    if (!(sc.diffuse > 0)) return;
+    float scol[3] = {sc.diffuse * sc.obj_rgb[0], sc.diffuse * sc.obj_rgb[1], sc.diffuse * sc.obj_rgb[2]};
+    float new_level = level * fmaxf(scol[0], fmaxf(scol[1], scol[2]));
+    if (!(new_level >= 1e-4f)) return;
+    // direct_diffuse_lighting, painter.cpp:197-292 (one light)
+    if (sc.light_count > 0) {
+        T vl[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) vl[k] = sc.light_pos[0][k] - p[k];
+        T sq2 = dot<N, T>(vl, vl);
+        T scale;
+        if (sc.light_kind[0] == 1)
+            scale = (T)1;
+        else if constexpr (N == 3)
+            scale = sc.light_coef[0] / sq2;
+        else
+            scale = sc.light_coef[0] / pow(sq2, (T)(N - 1) / (T)2);
+        float lc[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) lc[c] = sc.light_rgb[0][c] * (float)scale;
+        T ld[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) ld[k] = vl[k];
+        normalize<N, T>(ld);
+        T dln = dot<N, T>(ld, sn);
+        bool lit = !(fmaxf(lc[0], fmaxf(lc[1], lc[2])) < 1e-4f) && dln > (T)0;
+        if (lit) {
+            T dist = sqrt(dot<N, T>(vl, vl));
+            T o1[N];
+#pragma unroll
+            for (int k = 0; k < N; ++k) o1[k] = p[k] + ld[k] * sc.ray_offset;
+            T tmaxs = dist;
+            if (sc.smooth && dot<N, T>(ld, gn) < (T)0) {
+                // smooth-normal back face: forward and reverse probes (painter.cpp:238-284)
+                atomicAdd(probe_count, 1ull);
+                double o64[N], d64[N];
+#pragma unroll
+                for (int k = 0; k < N; ++k) {
+                    o64[k] = o1[k];
+                    d64[k] = ld[k];
+                }
+                HitResult h1 = traverse<N, T, false>(A, o1, ld, (T)INFINITY, st, o64, d64);
+                T tf = (T)h1.t;
+                lit = h1.idx >= 0 && tf < dist;
+                if (lit) {
+                    atomicAdd(probe_count, 1ull);
+                    T rd[N], ro[N];
+#pragma unroll
+                    for (int k = 0; k < N; ++k) {
+                        rd[k] = -ld[k];
+                        ro[k] = o1[k] + rd[k] * ((T)2 * sc.ray_offset);
+                        o64[k] = ro[k];
+                        d64[k] = rd[k];
+                    }
+                    HitResult h2 = traverse<N, T, false>(A, ro, rd, (T)INFINITY, st, o64, d64);
+                    lit = !(h2.idx >= 0 && (T)h2.t < tf);
+#pragma unroll
+                    for (int k = 0; k < N; ++k) o1[k] = o1[k] + ld[k] * (tf + sc.ray_offset);
+                    tmaxs = dist - tf;
+                }
+            }
+            if (lit) {
+                float w = (float)(sc.cos_coef * dln);
+                out.shadow = true;
+#pragma unroll
+                for (int k = 0; k < N; ++k) {
+                    out.sorg[k] = o1[k];
+                    out.sdir[k] = ld[k];
+                }
+                out.stmax = tmaxs;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) out.contrib[c] = beta[c] * (scol[c] * (lc[c] * w));
+            }
+        }
+    }
+    // diffuse_lighting, painter.cpp:298-323
+    if (depth < sc.max_depth) {
+        T u[8];
+        uniforms4<T>(sc.seed, pixel, sample, sc.pass, depth, 2, u);
+        if constexpr (N >= 6) uniforms4<T>(sc.seed, pixel, sample, sc.pass, depth, 3, u + 4);
+        T nd[N];
+        cosine_hemisphere<N, T>(sn, u, nd);
+        normalize<N, T>(nd);
+        if (dot<N, T>(nd, gn) > (T)0) {
+            out.next = true;
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                out.norg[k] = p[k] + nd[k] * sc.ray_offset;
+                out.ndir[k] = nd[k];
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) out.nbeta[c] = beta[c] * scol[c];
+            out.nlevel = new_level;
+        }
+    }
+}
+
+}  // namespace nsp

@@ -230,7 +230,9 @@ __device__ HitResult traverse(const TreeArgs& A, const T* o, const T* d, T tmax,
     int last_leaf = -1;
     unsigned nvis = 0, lvis = 0, ntest = 0, nret = 0;
     for (int iter = 0; iter < 65536; ++iter) {
-        if (t_cur > t_end) break;
+        // the walk ends at the root box's exit: with t_cur == t_end the next
+        // point lies outside the root and clamps back into the last cell
+        if (t_cur >= t_end) break;
         if (best >= 0 && (T)best_t < t_cur - adv0 * 4) break;
         T tp = t_cur + adv;
         int Q[N];
@@ -308,6 +310,7 @@ __device__ HitResult traverse(const TreeArgs& A, const T* o, const T* d, T tmax,
             adv = adv0;
         } else {
             adv = adv * 2;  // no progress: the located cell ended before t_cur (rounding)
+            if (t_cur + adv > t_end) break;
         }
     }
     if (!ANY && best >= 0) {

@@ -85,7 +85,9 @@ EXPORTS = [
     "nsp_mesh_tree_dump", "nsp_trace_first_hit", "nsp_trace_first_hit_device", "nsp_scene_create",
     "nsp_scene_destroy", "nsp_paint", "nsp_statistics", "nsp_image_get", "nsp_image_set", "nsp_image_aov",
     "nsp_scene_pixel_count", "nsp_last_pass_stats", "nsp_set_counters", "nsp_meshgen_torus",
-    "nsp_meshgen_cube_sphere",
+    "nsp_meshgen_cube_sphere", "nsp_denoiser_create", "nsp_denoiser_destroy", "nsp_denoiser_param_count",
+    "nsp_denoiser_get_params", "nsp_denoiser_set_params", "nsp_denoiser_forward", "nsp_denoiser_loss_grad",
+    "nsp_denoiser_train", "nsp_conv3d_bf16",
 ]
 
 _lib = None
@@ -369,3 +371,111 @@ class Scene:
 
     def set_counters(self, on):
         self.ctx.check(self.ctx._L.nsp_set_counters(self.h, int(on)))
+
+
+# ---------------------------------------------------------------------------
+# Denoiser
+
+def _denoiser_lib():
+    L = lib()
+    if not getattr(L, "_dn_ready", False):
+        P = C.c_void_p
+        fp = C.POINTER(C.c_float)
+        L.nsp_denoiser_create.argtypes = [P, C.c_int, C.c_int, C.c_uint64, C.POINTER(P)]
+        L.nsp_denoiser_destroy.argtypes = [P]
+        L.nsp_denoiser_destroy.restype = None
+        L.nsp_denoiser_param_count.argtypes = [P]
+        L.nsp_denoiser_param_count.restype = C.c_int64
+        L.nsp_denoiser_get_params.argtypes = [P, fp]
+        L.nsp_denoiser_set_params.argtypes = [P, fp]
+        L.nsp_denoiser_forward.argtypes = [P, fp, C.c_int, C.c_int, C.c_int, C.c_int, fp]
+        L.nsp_denoiser_loss_grad.argtypes = [P, fp, fp, C.c_int, C.c_int, C.c_int, C.c_int, fp, fp]
+        L.nsp_denoiser_train.argtypes = [P, fp, fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                         C.c_float, C.c_uint64, fp]
+        L.nsp_conv3d_bf16.argtypes = [P, fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, fp, fp, C.c_int, C.c_int,
+                                      fp]
+        L._dn_ready = True
+    return L
+
+
+LAYERS = 8
+WIDTH = 32
+
+
+def layer_shapes():
+    """(cout, cin) padded shapes of the denoiser layers."""
+    return [(WIDTH if l < LAYERS - 1 else 16, 16 if l == 0 else WIDTH) for l in range(LAYERS)]
+
+
+def conv3d_bf16(ctx, x, w, bias=None, relu=False):
+    """x [nb,d,h,w,cin] f32, w [cout,27,cin] f32 -> y [nb,d,h,w,cout] f32 (GPU, BF16 MFMA)."""
+    L = _denoiser_lib()
+    x = np.ascontiguousarray(x, np.float32)
+    w = np.ascontiguousarray(w, np.float32)
+    nb, d, h, wd, cin = x.shape
+    cout = w.shape[0]
+    y = np.zeros((nb, d, h, wd, cout), np.float32)
+    b = None if bias is None else np.ascontiguousarray(bias, np.float32)
+    ctx.check(L.nsp_conv3d_bf16(ctx.h, _ptr(x, C.c_float), nb, d, h, wd, cin, _ptr(w, C.c_float),
+                                _ptr(b, C.c_float), cout, int(relu), _ptr(y, C.c_float)))
+    return y
+
+
+class Denoiser:
+    def __init__(self, ctx, in_channels, residual_channel=-1, seed=0):
+        self.ctx = ctx
+        L = _denoiser_lib()
+        h = C.c_void_p()
+        ctx.check(L.nsp_denoiser_create(ctx.h, in_channels, residual_channel, seed, C.byref(h)))
+        self.h = h
+        self.cin = in_channels
+        self.nparams = L.nsp_denoiser_param_count(h)
+
+    def close(self):
+        if self.h:
+            _denoiser_lib().nsp_denoiser_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        p = np.zeros(self.nparams, np.float32)
+        self.ctx.check(_denoiser_lib().nsp_denoiser_get_params(self.h, _ptr(p, C.c_float)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.nparams
+        self.ctx.check(_denoiser_lib().nsp_denoiser_set_params(self.h, _ptr(p, C.c_float)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        nb, d, h, w, c = x.shape
+        assert c == self.cin
+        out = np.zeros((nb, d, h, w, 3), np.float32)
+        self.ctx.check(_denoiser_lib().nsp_denoiser_forward(self.h, _ptr(x, C.c_float), nb, d, h, w,
+                                                            _ptr(out, C.c_float)))
+        return out
+
+    def loss_grad(self, x, target):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        nb, d, h, w, _ = x.shape
+        loss = C.c_float()
+        g = np.zeros(self.nparams, np.float32)
+        self.ctx.check(_denoiser_lib().nsp_denoiser_loss_grad(self.h, _ptr(x, C.c_float), _ptr(t, C.c_float), nb, d,
+                                                              h, w, C.byref(loss), _ptr(g, C.c_float)))
+        return loss.value, g
+
+    def train(self, inputs, targets, patch=32, batch=16, steps=100, lr=1e-3, seed=0):
+        x = np.ascontiguousarray(inputs, np.float32)
+        t = np.ascontiguousarray(targets, np.float32)
+        S, d, h, w, c = x.shape
+        losses = np.zeros(steps, np.float32)
+        self.ctx.check(_denoiser_lib().nsp_denoiser_train(self.h, _ptr(x, C.c_float), _ptr(t, C.c_float), S, d, h, w,
+                                                          patch, batch, steps, lr, seed, _ptr(losses, C.c_float)))
+        return losses

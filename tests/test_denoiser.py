@@ -1,0 +1,114 @@
+"""Denoiser: the FP64 checker pinned by finite differences (CPU), and the BF16
+MFMA kernels (GPU) against the checker."""
+import numpy as np
+import pytest
+
+from oracle import denoise_ref as R
+
+
+def random_params(seed, scale=0.05):
+    rng = np.random.default_rng(seed)
+    parts = []
+    for l, (co, ci) in enumerate(R.shapes()):
+        w = rng.normal(scale=scale, size=(co, 27, ci))
+        if l == 0:
+            w[:, :, 13:] = 0
+        if l == R.LAYERS - 1:
+            w[3:] = 0
+        b = rng.normal(scale=0.01, size=co)
+        if l == R.LAYERS - 1:
+            b[3:] = 0
+        parts += [w.ravel(), b]
+    return np.concatenate(parts)
+
+
+def test_checker_gradient_matches_finite_differences():
+    rng = np.random.default_rng(1)
+    x = rng.normal(size=(1, 3, 3, 4, 13))
+    t = rng.normal(size=(1, 3, 3, 4, 3))
+    p = random_params(2, scale=0.2)
+    loss, g = R.loss_grad(p, x, t, 3)
+    offs = np.cumsum([0] + [co * 27 * ci + co for co, ci in R.shapes()])
+    idx = []
+    for l, (co, ci) in enumerate(R.shapes()):
+        cr = 3 if l == R.LAYERS - 1 else co
+        cir = 13 if l == 0 else ci
+        for _ in range(3):
+            o, tp, i = rng.integers(cr), rng.integers(27), rng.integers(cir)
+            idx.append(offs[l] + (o * 27 + tp) * ci + i)
+        idx.append(offs[l] + co * 27 * ci + rng.integers(cr))  # a bias
+    h = 1e-6
+    for k in idx:
+        pp, pm = p.copy(), p.copy()
+        pp[k] += h
+        pm[k] -= h
+        fd = (R.loss_grad(pp, x, t, 3)[0] - R.loss_grad(pm, x, t, 3)[0]) / (2 * h)
+        # central differences: truncation O(h^2 f''') and cancellation O(eps f / h)
+        assert abs(fd - g[k]) <= 2e-5 * max(1e-3, abs(g[k])) + 1e-8, (k, fd, g[k])
+
+
+def test_checker_conv_closed_forms():
+    # a delta kernel at tap t shifts the volume by -off(t) with zero fill
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=(2, 4, 5, 6, 16))
+    for t in (0, 13, 26, 5):
+        w = np.zeros((16, 27, 16))
+        w[:, t, :] = np.eye(16)
+        y = R.conv3d(x, w)
+        dz, dy, dx = t // 9 - 1, (t // 3) % 3 - 1, t % 3 - 1
+        xp = np.pad(x, ((0, 0), (1, 1), (1, 1), (1, 1), (0, 0)))
+        assert np.array_equal(y, xp[:, 1 + dz:5 + dz, 1 + dy:6 + dy, 1 + dx:7 + dx, :])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cin,cout", [(16, 32), (32, 32), (32, 16), (16, 16)])
+def test_conv3d_bf16_matches_checker(ctx, cin, cout):
+    from math_amd import native
+    rng = np.random.default_rng(cin + cout)
+    x = R.bf16(rng.normal(size=(2, 5, 6, 7, cin)))
+    w = R.bf16(rng.normal(scale=0.1, size=(cout, 27, cin)))
+    b = rng.normal(size=cout).astype(np.float32)
+    y = native.conv3d_bf16(ctx, x, w, b, relu=False)
+    ref = R.conv3d(x, w, b.astype(np.float64))
+    err = np.max(np.abs(y - ref)) / np.max(np.abs(ref))
+    print(f"conv {cin}->{cout}: max rel err {err:.2e}")
+    assert err < 1e-5  # inputs are exact in BF16; only FP32 accumulation order differs
+
+
+@pytest.mark.gpu
+def test_denoiser_forward_and_gradient_match_checker(ctx):
+    from math_amd import native
+    rng = np.random.default_rng(7)
+    d = native.Denoiser(ctx, 13, residual_channel=3, seed=11)
+    p = d.params()
+    x = rng.normal(size=(2, 6, 7, 8, 13)).astype(np.float32)
+    t = rng.normal(size=(2, 6, 7, 8, 3)).astype(np.float32)
+    y = d.forward(x)
+    ref, _ = R.forward(p, x, 3, emulate_bf16=True)
+    ferr = np.max(np.abs(y - ref)) / np.max(np.abs(ref))
+    loss, g = d.loss_grad(x, t)
+    rl, rg = R.loss_grad(p, x, t, 3, emulate_bf16=True)
+    offs = np.cumsum([0] + [co * 27 * ci + co for co, ci in R.shapes()])
+    errs = []
+    for l in range(R.LAYERS):
+        a, b = g[offs[l]:offs[l + 1]], rg[offs[l]:offs[l + 1]]
+        errs.append(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+    print(f"forward rel err {ferr:.2e}, loss {loss:.6f} vs {rl:.6f}, grad rel err per layer {np.round(errs, 4)}")
+    assert ferr < 2e-2
+    assert abs(loss - rl) <= 2e-2 * rl
+    assert max(errs) < 5e-2
+
+
+@pytest.mark.gpu
+def test_denoiser_training_reduces_loss(ctx):
+    from math_amd import native
+    rng = np.random.default_rng(5)
+    clean = np.cumsum(rng.normal(size=(4, 24, 24, 24, 3)), axis=1).astype(np.float32) * 0.05
+    clean -= clean.min()
+    noisy = clean + rng.normal(scale=0.1, size=clean.shape).astype(np.float32)
+    x = np.concatenate([noisy, noisy, np.ones_like(clean[..., :1])], axis=-1)
+    d = native.Denoiser(ctx, 7, residual_channel=0, seed=3)
+    losses = d.train(x, clean, patch=16, batch=4, steps=60, lr=2e-3, seed=1)
+    print("losses", losses[:3], losses[-3:])
+    assert np.all(np.isfinite(losses))
+    assert losses[-10:].mean() < 0.5 * losses[:5].mean()

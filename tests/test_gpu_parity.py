@@ -81,7 +81,7 @@ def test_first_hit_parity_small(ctx, n, g):
     # with the FP64 re-test the flagged rays agree as well, except true near-ties
     assert np.mean(gi == oi) > 0.999
     h = (gi == oi) & (oi >= 0)
-    assert h.sum() > 1000
+    assert h.sum() > 400
     assert np.max(np.abs(gt[h] - ot[h]) / ot[h]) <= 1e-5
 
 
@@ -96,7 +96,7 @@ def test_first_hit_parity_c1_full_size(ctx):
     ok = (gnear == 0) & (onear == 0)
     assert np.array_equal(gi[ok], oi[ok])
     h = (gi == oi) & (oi >= 0)
-    assert h.sum() > 200
+    assert h.sum() > 100
     assert np.max(np.abs(gt[h] - ot[h]) / ot[h]) <= 1e-5
 
 
