@@ -62,14 +62,111 @@ def cpu_baseline(v, f, cfg, spp, depth, seconds_target=15.0):
             break
         count *= 4
     if dt < seconds_target:
-        count2 = min(int(count * seconds_target / max(dt, 1e-3)), screen[0] * screen[1] * 4)
+        total = screen[0] * screen[1] * screen[2]
+        count2 = min(int(count * seconds_target / max(dt, 1e-3)), total)
+        start = max(0, min(mid - count2 // 2, total - count2))
         t0 = time.perf_counter()
-        _, rays = O.render(m, sc, mid - count2 // 2, count2, threads)
+        _, rays = O.render(m, sc, start, count2, threads)
         dt = time.perf_counter() - t0
         count = count2
     return dict(value=rays / dt / 1e6, unit="Mrays/s", cores=threads, kind="port",
                 sample=f"{count} pixels x 27 samples from the middle of the 128^3 screen, FP64, "
                        f"{rays} rays in {dt:.1f} s")
+
+
+def denoise_bench(args):
+    """C3: denoiser trained on the tracer's own renders; time to RMSE <= 0.01
+    against a 4096-spp FP64 render, FP32 + denoiser vs plain FP32.  Evaluation
+    uses held-out variants rendered in a window of the 128^3 screen (same pixel
+    footprint) to bound the cost of the FP64 references."""
+    import numpy as np
+
+    from math_amd import denoise_eval as DE
+    from math_amd import native
+
+    ctx = native.Context(0)
+    t0 = time.perf_counter()
+    X, Y = DE.training_set(ctx, list(range(100, 100 + args.train_variants)), window=args.window, log=log)
+    gen_s = time.perf_counter() - t0
+    dn = native.Denoiser(ctx, DE.IN_CHANNELS, DE.RESIDUAL, seed=1)
+    t0 = time.perf_counter()
+    losses = dn.train(X, Y, patch=32, batch=16, steps=args.train_steps, lr=1e-3, seed=2)
+    train_s = time.perf_counter() - t0
+    log(f"training: {args.train_steps} steps in {train_s:.1f} s, loss {losses[:5].mean():.5f} -> {losses[-50:].mean():.5f}")
+    results = []
+    for k in range(args.eval_scenes):
+        seed = 200 + k
+        rng = np.random.default_rng(seed)
+        org = [int(rng.integers(0, 128 - args.window + 1)) for _ in range(3)]
+        v, f, cfg = DE.variant(seed, 128, (org, args.window))
+        mesh = ctx.mesh(v, f)
+        ref, ref_s = DE.render_mean(ctx, mesh, cfg, 4096, seed=seed * 7, precision=native.PRECISION_FP64)
+        log(f"eval scene {seed}: FP64 4096-spp reference in {ref_s:.1f} s")
+        r = DE.time_to_rmse(ctx, mesh, cfg, ref, dn, seed=seed * 11)
+        r["seed"] = seed
+        log(f"eval scene {seed}: plain t={r['plain_time_s']} curve={[(c, round(t, 3), round(e, 4)) for c, t, e in r['plain_curve']][:8]}"
+            f"; denoised rmse {r['denoised_rmse']:.4f} (4-spp input {r['input4_rmse']:.4f}) in {r['denoised_time_s']:.3f} s")
+        results.append(r)
+        mesh.close()
+    ok = [r for r in results if r["plain_time_s"] is not None]
+    out = {
+        "metric": "time to RMSE<=0.01 vs 4096-spp FP64 ref at n=4, FP32+denoiser vs plain FP32",
+        "eval": [{k: r[k] for k in ("seed", "plain_time_s", "denoised_time_s", "denoised_rmse", "input4_rmse",
+                                    "denoiser_inference_s")} for r in results],
+        "denoised_reaches_0.01": [r["denoised_rmse"] <= 0.01 for r in results],
+        "speedup": [r["plain_time_s"] / r["denoised_time_s"] for r in ok if r["denoised_rmse"] <= 0.01],
+        "train": {"variants": args.train_variants, "steps": args.train_steps, "seconds": round(train_s, 2),
+                  "data_seconds": round(gen_s, 2), "loss_first": float(losses[:5].mean()),
+                  "loss_last": float(losses[-50:].mean())},
+        "window": args.window,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def batch_bench(args):
+    """C4: many scenes of uneven size and dimension, 4 spp, FP32: one batched
+    launch sequence vs one launch sequence per scene."""
+    import numpy as np
+
+    from math_amd import configs, native
+
+    ctx = native.Context(0)
+    rng = np.random.default_rng(300)
+    meshes = []
+    t0 = time.perf_counter()
+    for i in range(args.scenes):
+        n = int(rng.integers(3, 6))
+        if n == 3:
+            g, edge = int(rng.integers(8, 60)), int(rng.integers(64, 257))
+        elif n == 4:
+            g, edge = int(rng.integers(3, 16)), int(rng.integers(32, 97))
+        else:
+            g, edge = int(rng.integers(2, 6)), int(rng.integers(32, 97))
+        v, f = native.meshgen_cube_sphere(n, g, True, 0.02, 0.0, 300 + i)
+        cfg = configs.scene_dict(v, f, edge, tuple([2.0] * n))
+        meshes.append((ctx.mesh(v, f), cfg, len(f)))
+    log(f"{args.scenes} meshes built in {time.perf_counter() - t0:.1f} s "
+        f"({min(m[2] for m in meshes)}..{max(m[2] for m in meshes)} simplices)")
+    per = [native.Scene(ctx, m, c) for m, c, _ in meshes]
+    t0 = time.perf_counter()
+    for s in per:
+        s.paint(4, 5, seed=1)
+    per_s = time.perf_counter() - t0
+    per_rays = sum(s.statistics()["rays"] for s in per)
+    bat = [native.Scene(ctx, m, c) for m, c, _ in meshes]
+    b = native.Batch(ctx, bat)
+    b.paint(4, 5, seed=1, use_graph=True)  # warm-up
+    for s in bat:
+        s.set_image(*[np.zeros_like(a) for a in s.image()])
+    t0 = time.perf_counter()
+    b.paint(4, 5, seed=1, use_graph=True)
+    bat_s = time.perf_counter() - t0
+    bat_rays = b.statistics()["rays"]
+    out = {"metric": "C4 batched persistent launch, Mrays/s", "scenes": args.scenes,
+           "per_scene": {"seconds": round(per_s, 3), "Mrays_s": round(per_rays / per_s / 1e6, 2)},
+           "batched": {"seconds": round(bat_s, 3), "Mrays_s": round(bat_rays / bat_s / 1e6, 2)},
+           "speedup": round(per_s / bat_s, 3)}
+    print(json.dumps(out), flush=True)
 
 
 def main():
@@ -81,9 +178,25 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-paths", type=int, default=0)
     ap.add_argument("--graph", action="store_true")
+    ap.add_argument("--layout", type=int, default=-1,
+                    help="FP32 record layout: 0 leaf-ordered SoA, 1 simplex-ordered AoS (default: the library's)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 256 MiB device copy (to scale the FETCH_SIZE/WRITE_SIZE counters)")
+    ap.add_argument("--denoise", action="store_true",
+                    help="C3: train the denoiser on renders of n=4 variants, then time to RMSE<=0.01, "
+                         "denoised vs plain")
+    ap.add_argument("--train-variants", type=int, default=16)
+    ap.add_argument("--train-steps", type=int, default=1500)
+    ap.add_argument("--eval-scenes", type=int, default=2)
+    ap.add_argument("--window", type=int, default=64)
+    ap.add_argument("--batch-scenes", action="store_true",
+                    help="C4: batched persistent launch over many scenes vs one launch sequence per scene")
+    ap.add_argument("--scenes", type=int, default=256)
     args = ap.parse_args()
+    if args.denoise:
+        return denoise_bench(args)
+    if args.batch_scenes:
+        return batch_bench(args)
     if args.copy_calibration:
         import torch
         x = torch.ones(1 << 26, dtype=torch.float32, device="cuda")  # 256 MiB
@@ -110,6 +223,9 @@ def main():
     info = mesh.info()
     log(f"mesh build {build_s:.2f} s (basis {info.basis_ms:.1f} ms, tree {info.tree_ms:.1f} ms): "
         f"{info.tree_nodes} nodes, {info.tree_leaves} leaves, {info.tree_entries} entries, {info.tree_levels} levels")
+    if args.layout >= 0:
+        mesh.record_layout = args.layout
+    log(f"record layout {mesh.record_layout}")
     scene = native.Scene(ctx, mesh, cfg)
     for w in range(args.warmup):
         scene.paint(spp, depth, seed=1000 + w, batch_paths=args.batch_paths, use_graph=args.graph)
@@ -164,6 +280,7 @@ def main():
                                "128^3 parallel-projected screen, 16 spp requested (27 samples/pixel), depth 5, "
                                "point light (2,2,2,2), FP32 trace + FP64 re-test",
                    "rays_per_step": rays // args.steps, "mesh_build_s": round(build_s, 3),
+                   "record_layout": ["leaf_soa", "simplex_aos"][mesh.record_layout],
                    "tree": dict(nodes=info.tree_nodes, leaves=info.tree_leaves, entries=info.tree_entries)},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -95,6 +95,19 @@ int nsp_mesh_records(const nsp_mesh* mesh, double* normal, double* plane_n, doub
  * negative error code. */
 int64_t nsp_mesh_tree_dump(const nsp_mesh* mesh, int64_t* out, int64_t cap);
 
+/* Which copy of the FP32 candidate records traversal reads (no counterpart in
+ * the reference, whose Mesh keeps one record per simplex object):
+ *   NSP_LAYOUT_LEAF_SOA    leaf-ordered structure of arrays, one copy per
+ *                          leaf entry (coalesced when a wave shares a leaf)
+ *   NSP_LAYOUT_SIMPLEX_AOS simplex-ordered 16-byte-aligned records gathered
+ *                          through the leaf entry ids (one copy per simplex)
+ * Both hold the same FP32 values, so results are identical.  A new mesh
+ * uses NSP_LAYOUT_SIMPLEX_AOS, the faster of the two on incoherent bounce rays. */
+#define NSP_LAYOUT_LEAF_SOA 0
+#define NSP_LAYOUT_SIMPLEX_AOS 1
+int nsp_mesh_set_record_layout(nsp_mesh* mesh, int layout);
+int nsp_mesh_get_record_layout(const nsp_mesh* mesh);
+
 /* First hit of a ray batch: Mesh::intersect_approximate + intersect_precise
  * (src/painter/shapes/mesh.cpp:137-166) for each ray; the parity entry point.
  *   org, dir  count*n floats (dir need not be unit: Ray normalises it,
@@ -218,6 +231,22 @@ int nsp_last_pass_stats(const nsp_scene* scene, nsp_stage_stats* out);
 /* Enables per-ray traversal counters (node visits, leaf visits, simplex
  * tests, FP64 re-tests) for roofline accounting; off by default. */
 int nsp_set_counters(nsp_scene* scene, int enable);
+
+/* Batched persistent launch: one sequence of wavefront launches renders one
+ * FP32 pass of every scene in the batch (scenes of any size and dimension,
+ * rays dispatched per scene inside the kernels).  No reference counterpart
+ * (the reference paints one scene per paint() call); each scene's image and
+ * statistics accumulate exactly as after nsp_paint with the same parameters. */
+typedef struct nsp_batch nsp_batch;
+int nsp_batch_create(nsp_context* ctx, nsp_scene* const* scenes, int count, nsp_batch** out);
+void nsp_batch_destroy(nsp_batch* batch);
+int nsp_batch_paint(nsp_batch* batch, const nsp_paint_params* params, const volatile int32_t* stop);
+/* rays of the last batch pass (reference counting rules) and its wall time */
+int nsp_batch_statistics(const nsp_batch* batch, int64_t* ray_count, double* last_pass_duration);
+/* Chooses batch_paths for a scene by timing candidate queue sizes on the
+ * scene's first pixels (images do not depend on the choice); writes the
+ * winner into params->batch_paths. */
+int nsp_autotune(nsp_scene* scene, nsp_paint_params* params);
 
 /* Learned denoiser for 3D screens (n = 4 renders).  No reference counterpart
  * (the reference only renders progressively).  8 layers of 3x3x3

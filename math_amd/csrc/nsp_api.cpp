@@ -13,6 +13,7 @@
 #include "nsp.h"
 #include "nsp_build.hpp"
 #include "nsp_internal.hpp"
+#include "nsp_batch.hpp"
 #include "nsp_render.hpp"
 
 namespace nsp {
@@ -79,6 +80,9 @@ struct nsp_context {
     hipStream_t stream = nullptr;
     std::string error;
     bool fail_next = false;
+    // ray queues and per-path state of nsp_paint; every paint on this context
+    // runs on its one stream, so all scenes share this workspace
+    RenderBuffers work;
 };
 
 struct nsp_mesh {
@@ -99,7 +103,6 @@ struct nsp_scene {
     int64_t pixels = 0;
     DevBuf pix_sum, pix_cnt;
     DevBuf pix_aov;  // (3 + n) * pixels, allocated on the first pass with aov
-    RenderBuffers buf;
     DevBuf totals;  // 3 u64
     StageMarks marks;
     bool counters = false;
@@ -354,6 +357,19 @@ int nsp_mesh_get_info(const nsp_mesh* mesh, nsp_mesh_info* info)
     return NSP_OK;
 }
 
+int nsp_mesh_set_record_layout(nsp_mesh* mesh, int layout)
+{
+    if (!mesh) return NSP_E_INVALID;
+    if (layout != NSP_LAYOUT_LEAF_SOA && layout != NSP_LAYOUT_SIMPLEX_AOS) {
+        if (mesh->ctx) mesh->ctx->error = "unknown record layout " + std::to_string(layout);
+        return NSP_E_INVALID;
+    }
+    mesh->m.layout = layout;
+    return NSP_OK;
+}
+
+int nsp_mesh_get_record_layout(const nsp_mesh* mesh) { return mesh ? mesh->m.layout : NSP_E_INVALID; }
+
 int nsp_mesh_records(const nsp_mesh* mesh, double* normal, double* plane_n, double* plane_d)
 {
     if (!mesh) return NSP_E_INVALID;
@@ -557,9 +573,9 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
         int64_t batch_pixels = std::max<int64_t>(1, want / S);
         batch_pixels = std::min<int64_t>(batch_pixels, scene->pixels);
         const int64_t cap = batch_pixels * S;
-        alloc_render_buffers(scene->buf, n, p.precision, cap, p.max_depth);
+        alloc_render_buffers(ctx->work, n, p.precision, cap, p.max_depth);
         if (p.aov) {
-            scene->buf.aov.alloc(sizeof(float) * (3 + n) * cap);
+            ctx->work.aov.alloc(sizeof(float) * (3 + n) * cap);
             if (!scene->pix_aov.p) {
                 scene->pix_aov.alloc(sizeof(float) * (3 + n) * scene->pixels);
                 NSP_HIP(hipMemsetAsync(scene->pix_aov.p, 0, sizeof(float) * (3 + n) * scene->pixels, s));
@@ -576,14 +592,14 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
             std::vector<BatchDesc> descs;
             for (int64_t px = 0; px < scene->pixels; px += batch_pixels)
                 descs.push_back(BatchDesc{px, std::min(batch_pixels, scene->pixels - px) * S});
-            scene->buf.batch_list.alloc(sizeof(BatchDesc) * descs.size());
-            NSP_HIP(hipMemcpyAsync(scene->buf.batch_list.p, descs.data(), sizeof(BatchDesc) * descs.size(),
+            ctx->work.batch_list.alloc(sizeof(BatchDesc) * descs.size());
+            NSP_HIP(hipMemcpyAsync(ctx->work.batch_list.p, descs.data(), sizeof(BatchDesc) * descs.size(),
                                    hipMemcpyHostToDevice, s));
-            pass_reset(scene->buf, scene->counters, s);
+            pass_reset(ctx->work, scene->counters, s);
             RenderLaunch rl;
             rl.mesh = &scene->mesh->m;
             rl.scene = p.precision ? (const void*)&kd : (const void*)&kf;
-            rl.buf = &scene->buf;
+            rl.buf = &ctx->work;
             rl.pix_sum = scene->pix_sum.as<float>();
             rl.pix_cnt = scene->pix_cnt.as<int32_t>();
             rl.pix_aov = p.aov ? scene->pix_aov.as<float>() : nullptr;
@@ -614,7 +630,7 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
                     render_batch(rl, s);
             }
             unsigned long long tot[3] = {0, 0, 0};
-            NSP_HIP(hipMemcpyAsync(tot, scene->buf.totals.p, sizeof tot, hipMemcpyDeviceToHost, s));
+            NSP_HIP(hipMemcpyAsync(tot, ctx->work.totals.p, sizeof tot, hipMemcpyDeviceToHost, s));
             NSP_HIP(hipStreamSynchronize(s));
             if (exec) (void)hipGraphExecDestroy(exec);
             if (graph) (void)hipGraphDestroy(graph);
@@ -639,7 +655,7 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
             st.shadow_rays = (int64_t)shadow;
             if (scene->counters) {
                 unsigned long long tc[8];
-                NSP_HIP(hipMemcpy(tc, scene->buf.trav_counters.p, sizeof tc, hipMemcpyDeviceToHost));
+                NSP_HIP(hipMemcpy(tc, ctx->work.trav_counters.p, sizeof tc, hipMemcpyDeviceToHost));
                 st.node_visits = (int64_t)tc[0];
                 st.leaf_visits = (int64_t)tc[1];
                 st.simplex_tests = (int64_t)tc[2];
@@ -751,6 +767,207 @@ int nsp_set_counters(nsp_scene* scene, int enable)
 {
     if (!scene) return NSP_E_INVALID;
     scene->counters = enable != 0;
+    return NSP_OK;
+}
+
+}  // extern "C"
+
+struct nsp_batch {
+    nsp_context* ctx;
+    int device = 0;
+    std::vector<nsp_scene*> scenes;
+    MultiBuffers buf;
+    DevBuf slots, begins;
+    int64_t rays = 0;
+    double last_s = 0;
+};
+
+extern "C" {
+
+int nsp_batch_create(nsp_context* ctx, nsp_scene* const* scenes, int count, nsp_batch** out)
+{
+    if (!ctx || !out || !scenes || count < 1) return NSP_E_INVALID;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        auto b = std::make_unique<nsp_batch>();
+        b->ctx = ctx;
+        b->device = ctx->device;
+        for (int i = 0; i < count; ++i) {
+            if (!scenes[i] || scenes[i]->ctx != ctx) raise(NSP_E_INVALID, "batch scene from another context");
+            b->scenes.push_back(scenes[i]);
+        }
+        *out = b.release();
+    });
+}
+
+void nsp_batch_destroy(nsp_batch* b)
+{
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    (void)hipDeviceSynchronize();
+    delete b;
+}
+
+int nsp_batch_paint(nsp_batch* b, const nsp_paint_params* params, const volatile int32_t* stop)
+{
+    if (!b || !params) return NSP_E_INVALID;
+    nsp_context* ctx = b->ctx;
+    return guarded(ctx, [&] {
+        const nsp_paint_params& p = *params;
+        if (p.precision != NSP_PRECISION_FP32) raise(NSP_E_UNSUPPORTED, "the batched launch traces in FP32");
+        if (p.max_depth < 0 || p.max_depth > 250) raise(NSP_E_INVALID, "max_depth must be in [0, 250]");
+        if (p.aov) raise(NSP_E_UNSUPPORTED, "AOVs are not produced by the batched launch");
+        launch_gate(ctx);
+        hipStream_t s = ctx->stream;
+        const int ns = (int)b->scenes.size();
+        std::vector<SceneSlot> slots(ns);
+        std::vector<int64_t> begins(ns + 1);
+        int64_t total = 0;
+        int Smax = 1;
+        for (int i = 0; i < ns; ++i) {
+            nsp_scene* sc = b->scenes[i];
+            SceneSlot& sl = slots[i];
+            std::memset(&sl, 0, sizeof sl);
+            fill_scene<float>(*sc, p, (int)sc->pass_count, sl.sc);
+            sl.n = sc->n;
+            sl.S = sl.sc.S;
+            sl.path_begin = total;
+            sl.pixels = sc->pixels;
+            sl.pix_sum = sc->pix_sum.as<float>();
+            sl.pix_cnt = sc->pix_cnt.as<int32_t>();
+            sl.A = make_tree_args(sc->mesh->m);
+            const MeshDev& m = sc->mesh->m;
+            sl.M = MeshShadeArgs{m.normal32.as<float>(), m.rec64.as<double>(),
+                                 m.has_normals ? m.vn64.as<double>() : nullptr,
+                                 m.has_normals ? m.fnrm.as<int32_t>() : nullptr, m.ntype.as<int32_t>()};
+            begins[i] = total;
+            total += sc->pixels * sl.S;
+            Smax = std::max(Smax, sl.S);
+        }
+        begins[ns] = total;
+        const int64_t want = p.batch_paths > 0 ? p.batch_paths : ((int64_t)1 << 23);
+        const int64_t cap = std::max<int64_t>(want, Smax);
+        // batches: runs of whole pixels over the concatenated scenes
+        std::vector<BatchDesc> descs;
+        {
+            int64_t start = 0, cur = 0;
+            for (int i = 0; i < ns; ++i) {
+                const int64_t S = slots[i].S;
+                int64_t px = 0;
+                while (px < b->scenes[i]->pixels) {
+                    int64_t room = (cap - cur) / S;
+                    if (room == 0) {
+                        descs.push_back(BatchDesc{start, cur});
+                        start += cur;
+                        cur = 0;
+                        continue;
+                    }
+                    int64_t take = std::min(room, b->scenes[i]->pixels - px);
+                    px += take;
+                    cur += take * S;
+                }
+            }
+            if (cur > 0) descs.push_back(BatchDesc{start, cur});
+        }
+        alloc_multi_buffers(b->buf, cap, p.max_depth);
+        b->slots.alloc(sizeof(SceneSlot) * ns);
+        b->begins.alloc(sizeof(int64_t) * (ns + 1));
+        b->buf.batch_list.alloc(sizeof(BatchDesc) * descs.size());
+        NSP_HIP(hipMemcpyAsync(b->slots.p, slots.data(), sizeof(SceneSlot) * ns, hipMemcpyHostToDevice, s));
+        NSP_HIP(hipMemcpyAsync(b->begins.p, begins.data(), sizeof(int64_t) * (ns + 1), hipMemcpyHostToDevice, s));
+        NSP_HIP(hipMemcpyAsync(b->buf.batch_list.p, descs.data(), sizeof(BatchDesc) * descs.size(),
+                               hipMemcpyHostToDevice, s));
+        NSP_HIP(hipStreamSynchronize(s));
+        auto t0 = std::chrono::steady_clock::now();
+        multi_pass_reset(b->buf, s);
+        MultiLaunch ml{b->slots.as<SceneSlot>(), b->begins.as<int64_t>(), ns, &b->buf, p.max_depth};
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        if (p.use_graph) {
+            NSP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            try {
+                render_batch_multi(ml, s);
+            } catch (...) {
+                hipGraph_t g2 = nullptr;
+                (void)hipStreamEndCapture(s, &g2);
+                if (g2) (void)hipGraphDestroy(g2);
+                throw;
+            }
+            NSP_HIP(hipStreamEndCapture(s, &graph));
+            NSP_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        }
+        size_t done = 0;
+        for (; done < descs.size(); ++done) {
+            if (stop && *stop) break;
+            if (exec)
+                NSP_HIP(hipGraphLaunch(exec, s));
+            else
+                render_batch_multi(ml, s);
+        }
+        unsigned long long tot[3] = {0, 0, 0};
+        NSP_HIP(hipMemcpyAsync(tot, b->buf.totals.p, sizeof tot, hipMemcpyDeviceToHost, s));
+        NSP_HIP(hipStreamSynchronize(s));
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        b->last_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        b->rays = (int64_t)(tot[0] + tot[1] + tot[2]);
+        if (done < descs.size()) raise(NSP_E_STOPPED, "stopped");
+        for (int i = 0; i < ns; ++i) {
+            nsp_scene* sc = b->scenes[i];
+            sc->pass_count += 1;
+            sc->pixel_count += sc->pixels;
+            sc->sample_count += sc->pixels * slots[i].S;
+            sc->last_pass_s = b->last_s;
+        }
+    });
+}
+
+int nsp_batch_statistics(const nsp_batch* b, int64_t* ray_count, double* last_pass_duration)
+{
+    if (!b) return NSP_E_INVALID;
+    if (ray_count) *ray_count = b->rays;
+    if (last_pass_duration) *last_pass_duration = b->last_s;
+    return NSP_OK;
+}
+
+int nsp_autotune(nsp_scene* scene, nsp_paint_params* params)
+{
+    if (!scene || !params) return NSP_E_INVALID;
+    nsp_context* ctx = scene->ctx;
+    // save the scene state, time candidate queue sizes on whole passes, restore
+    const int64_t P = scene->pixels;
+    std::vector<float> sum((size_t)3 * P);
+    std::vector<int32_t> cnt(P);
+    int rc = nsp_image_get(scene, sum.data(), cnt.data());
+    if (rc != NSP_OK) return rc;
+    const int64_t passes = scene->pass_count, pixels = scene->pixel_count, rays = scene->ray_count,
+                  samples = scene->sample_count;
+    const double last = scene->last_pass_s;
+    nsp_paint_params p = *params;
+    p.passes = 1;
+    int64_t best = 0;
+    double best_s = 1e300;
+    for (int lg = 20; lg <= 24; ++lg) {
+        p.batch_paths = (int64_t)1 << lg;
+        auto t0 = std::chrono::steady_clock::now();
+        rc = nsp_paint(scene, &p, nullptr, nullptr);
+        if (rc != NSP_OK) break;
+        double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (secs < best_s) {
+            best_s = secs;
+            best = p.batch_paths;
+        }
+    }
+    int rc2 = nsp_image_set(scene, sum.data(), cnt.data());
+    scene->pass_count = passes;
+    scene->pixel_count = pixels;
+    scene->ray_count = rays;
+    scene->sample_count = samples;
+    scene->last_pass_s = last;
+    if (rc != NSP_OK) return rc;
+    if (rc2 != NSP_OK) return rc2;
+    params->batch_paths = best;
+    (void)ctx;
     return NSP_OK;
 }
 

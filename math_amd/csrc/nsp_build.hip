@@ -384,6 +384,20 @@ struct ScatterOp {
 // Leaf-ordered FP32 SoA of the simplex records (normal, d = normal.v0, then the
 // N-1 barycentric planes), so that a leaf's candidates are contiguous and a
 // wave reading one leaf reads each component array coalesced.
+// The FP32 test components of one simplex, in the order both layouts share.
+template <int N>
+__device__ inline void fp32_components(const double* r, float* out)
+{
+    int comp = 0;
+    for (int q = 0; q < N; ++q) out[comp++] = (float)r[q];
+    out[comp++] = (float)dot<N, double>(r, r + N);
+    for (int i = 0; i < N - 1; ++i) {
+        for (int q = 0; q < N; ++q) out[comp++] = (float)r[2 * N + i * N + q];
+        out[comp++] = (float)r[2 * N + (N - 1) * N + i];
+    }
+    for (; comp < rec32_stride(N); ++comp) out[comp] = 0.0f;
+}
+
 template <int N>
 __global__ void __launch_bounds__(256) k_pack_soa(int64_t E, int64_t pad, const int32_t* __restrict__ entries,
                                                   const double* __restrict__ rec, float* __restrict__ soa)
@@ -391,14 +405,24 @@ __global__ void __launch_bounds__(256) k_pack_soa(int64_t E, int64_t pad, const 
     int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
     int64_t s = entries[e];
-    const double* r = rec + s * rec64_stride(N);
-    int comp = 0;
-    for (int q = 0; q < N; ++q) soa[(int64_t)(comp++) * pad + e] = (float)r[q];
-    soa[(int64_t)(comp++) * pad + e] = (float)dot<N, double>(r, r + N);
-    for (int i = 0; i < N - 1; ++i) {
-        for (int q = 0; q < N; ++q) soa[(int64_t)(comp++) * pad + e] = (float)r[2 * N + i * N + q];
-        soa[(int64_t)(comp++) * pad + e] = (float)r[2 * N + (N - 1) * N + i];
-    }
+    float c[rec32_stride(N)];
+    fp32_components<N>(rec + s * rec64_stride(N), c);
+    for (int k = 0; k < soa_components(N); ++k) soa[(int64_t)k * pad + e] = c[k];
+}
+
+// Simplex-ordered AoS copy of the same FP32 components, padded to 16-byte
+// records: a candidate test reads its record with rec32_stride/4 vector loads,
+// and the whole table (F * rec32_stride * 4 bytes) is small enough to stay in
+// the last-level cache where the leaf-ordered SoA is not.
+template <int N>
+__global__ void __launch_bounds__(256) k_pack_rec32(int64_t F, const double* __restrict__ rec,
+                                                    float* __restrict__ rec32)
+{
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= F) return;
+    float c[rec32_stride(N)];
+    fp32_components<N>(rec + s * rec64_stride(N), c);
+    for (int k = 0; k < rec32_stride(N); ++k) rec32[s * rec32_stride(N) + k] = c[k];
 }
 
 __global__ void k_iota(int64_t n, int32_t* out)
@@ -432,6 +456,10 @@ void build_basis_t(MeshDev& m, hipStream_t s)
                        m.fac.as<int32_t>(), m.has_normals ? m.vn64.as<double>() : nullptr,
                        m.has_normals ? m.fnrm.as<int32_t>() : nullptr, (int)m.has_normals, m.rec64.as<double>(),
                        m.build64.as<double>(), m.normal32.as<float>(), m.ntype.as<int32_t>(), status.as<int>());
+    NSP_HIP(hipGetLastError());
+    m.rec32.alloc(sizeof(float) * m.F * rec32_stride(N));
+    hipLaunchKernelGGL(k_pack_rec32<N>, dim3(blocks_for(m.F)), dim3(256), 0, s, m.F, m.rec64.as<double>(),
+                       m.rec32.as<float>());
     NSP_HIP(hipGetLastError());
     int st = 0;
     NSP_HIP(hipMemcpyAsync(&st, status.p, sizeof(int), hipMemcpyDeviceToHost, s));

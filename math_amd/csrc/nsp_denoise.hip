@@ -224,7 +224,8 @@ __global__ void __launch_bounds__(256) k_conv_dw(const unsigned short* __restric
         }
     }
     // reduce the 4 waves through LDS in a fixed order, then one write per value
-    __shared__ float red[4][3 * NT * CT * 16 * 4];
+    // per wave: 3 taps x NT x CT tiles x 4 accumulator rows x 64 lanes
+    __shared__ float red[4][3 * NT * CT * 4 * 64];
     int idx = 0;
 #pragma unroll
     for (int t = 0; t < 3; ++t)

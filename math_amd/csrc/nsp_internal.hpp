@@ -73,6 +73,9 @@ __host__ __device__ constexpr int rec64_stride(int n) { return n * n + 2 * n - 1
 __host__ __device__ constexpr int build64_stride(int n) { return 2 * n + n * (n + 1) + n + 1; }
 // FP32 leaf-ordered SoA components: normal[N], d, then (plane_n[N], plane_d) * (N-1)
 __host__ __device__ constexpr int soa_components(int n) { return n * (n + 1); }
+// Per-simplex FP32 record (AoS, simplex order), the same components padded to
+// a multiple of 4 floats so that a record is read with 16-byte loads.
+__host__ __device__ constexpr int rec32_stride(int n) { return (n * (n + 1) + 3) / 4 * 4; }
 
 struct TreeDev {
     int n = 0;
@@ -104,8 +107,10 @@ struct MeshDev {
     DevBuf rec64;     // F*rec64_stride
     DevBuf build64;   // F*build64_stride (freed after the tree build)
     DevBuf normal32;  // F*n float (geometric normal, flipped per the normal classification)
+    DevBuf rec32;     // F*rec32_stride float: normal, d, barycentric planes (simplex-ordered AoS)
     DevBuf ntype;     // F int32: bits 0-1 normal type (0 none, 1 use, 2 reverse), bits 8.. reverse mask
     TreeDev tree;
+    int layout = 1;  // FP32 candidate records read by traversal: 0 leaf-ordered SoA, 1 simplex-ordered AoS
     double build_ms = 0;
     double basis_ms = 0;
 };

@@ -38,6 +38,8 @@ TreeArgs make_tree_args(const MeshDev& m)
     a.entries = t.entries.as<int32_t>();
     a.soa = t.soa.as<float>();
     a.pad = t.entry_pad;
+    a.rec32 = m.rec32.as<float>();
+    a.layout = m.layout;
     a.rec64 = m.rec64.as<double>();
     a.lmax = t.levels - 1;
     double ext = 0;
@@ -348,8 +350,9 @@ __global__ void __launch_bounds__(256) k_accumulate(const BatchDesc* __restrict_
 
 void alloc_render_buffers(RenderBuffers& b, int n, int precision, int64_t cap, int max_depth)
 {
+    // DevBuf::alloc keeps a buffer that is already large enough, so scenes of
+    // different dimension and precision share one workspace that only grows
     size_t ts = precision ? sizeof(double) : sizeof(float);
-    if (b.n == n && b.precision == precision && b.cap >= cap && b.depth_slots >= max_depth + 2) return;
     b.n = n;
     b.precision = precision;
     b.cap = cap;

@@ -35,6 +35,8 @@ struct TreeArgs {
     const int32_t* entries;
     const float* soa;
     int64_t pad;
+    const float* rec32;  // simplex-ordered AoS records (layout 1)
+    int layout;          // 0: leaf-ordered SoA, 1: simplex-ordered AoS gathered through the entry ids
     const double* rec64;
     float lo[NMAX], size[NMAX], inv_fine[NMAX];
     double lo64[NMAX], size64[NMAX], inv_fine64[NMAX];
@@ -124,13 +126,32 @@ __device__ __forceinline__ bool candidate(const TreeArgs& A, int64_t e, const T*
         *idx = s;
         return true;
     } else {
+        // component c of the candidate's FP32 record: leaf-ordered SoA (layout 0)
+        // or the simplex-ordered AoS record read with 16-byte loads (layout 1)
+        constexpr int RS = rec32_stride(N);
+        float rec[RS];
+        int sid = -1;
+        if (A.layout == 1) {
+            sid = A.entries[e];
+            const float4* r4 = reinterpret_cast<const float4*>(A.rec32 + (int64_t)sid * RS);
+#pragma unroll
+            for (int k = 0; k < RS / 4; ++k) {
+                float4 v = r4[k];
+                rec[4 * k] = v.x;
+                rec[4 * k + 1] = v.y;
+                rec[4 * k + 2] = v.z;
+                rec[4 * k + 3] = v.w;
+            }
+        }
         const float* soa = A.soa;
         const int64_t pad = A.pad;
+        auto comp = [&](int c) -> float { return A.layout == 1 ? rec[c] : soa[(int64_t)c * pad + e]; };
         float nrm[N];
-        load_vec_f<N>(soa, pad, e, 0, nrm);
+#pragma unroll
+        for (int q = 0; q < N; ++q) nrm[q] = comp(q);
         float s = dot<N, float>(nrm, d);
         if (s == 0.0f) return false;
-        float dd = soa[(int64_t)N * pad + e];
+        float dd = comp(N);
         float no = dot<N, float>(nrm, o);
         float t = (dd - no) / s;
         float as = fabsf(s);
@@ -148,8 +169,9 @@ __device__ __forceinline__ bool candidate(const TreeArgs& A, int64_t e, const T*
 #pragma unroll
         for (int i = 0; i < N - 1; ++i) {
             float pn[N];
-            load_vec_f<N>(soa, pad, e, N + 1 + i * (N + 1), pn);
-            float b = dot<N, float>(p, pn) - soa[(int64_t)(N + 1 + i * (N + 1) + N) * pad + e];
+#pragma unroll
+            for (int q = 0; q < N; ++q) pn[q] = comp(N + 1 + i * (N + 1) + q);
+            float b = dot<N, float>(p, pn) - comp(N + 1 + i * (N + 1) + N);
             in = in && (b > 0.0f) && (b < 1.0f);
             last -= b;
             margin = fminf(margin, b);
@@ -160,15 +182,16 @@ __device__ __forceinline__ bool candidate(const TreeArgs& A, int64_t e, const T*
         bool retest = fabsf(margin) < band || (in && as < PARALLEL_COS) || t < terr;
         if (!retest) {
             if (!in || !(t > 0.0f)) return false;
-            if (fabsf(t - tlimit) < terr && A.entries[e] != best_idx) *near = true;  // near-tie with another simplex
+            if (sid < 0) sid = A.entries[e];
+            if (fabsf(t - tlimit) < terr && sid != best_idx) *near = true;  // near-tie with another simplex
             if (!(t < tlimit)) return false;
             *t_hit = (double)t;
-            *idx = A.entries[e];
+            *idx = sid;
             return true;
         }
         ++*retests;
         *near = true;
-        int sidx = A.entries[e];
+        int sidx = sid >= 0 ? sid : A.entries[e];
         double t64;
         if (!simplex_test64<N>(A.rec64 + (int64_t)sidx * rec64_stride(N), o64, d64, &t64)) return false;
         if (!((float)t64 < tlimit)) return false;

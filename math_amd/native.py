@@ -23,6 +23,8 @@ PRECISION_FP32 = 0
 PRECISION_FP64 = 1
 LIGHT_POINT = 0
 LIGHT_CONSTANT = 1
+LAYOUT_LEAF_SOA = 0
+LAYOUT_SIMPLEX_AOS = 1
 
 
 class NspError(RuntimeError):
@@ -87,7 +89,8 @@ EXPORTS = [
     "nsp_scene_pixel_count", "nsp_last_pass_stats", "nsp_set_counters", "nsp_meshgen_torus",
     "nsp_meshgen_cube_sphere", "nsp_denoiser_create", "nsp_denoiser_destroy", "nsp_denoiser_param_count",
     "nsp_denoiser_get_params", "nsp_denoiser_set_params", "nsp_denoiser_forward", "nsp_denoiser_loss_grad",
-    "nsp_denoiser_train", "nsp_conv3d_bf16",
+    "nsp_denoiser_train", "nsp_conv3d_bf16", "nsp_batch_create", "nsp_batch_destroy", "nsp_batch_paint",
+    "nsp_batch_statistics", "nsp_autotune", "nsp_mesh_set_record_layout", "nsp_mesh_get_record_layout",
 ]
 
 _lib = None
@@ -118,6 +121,8 @@ def lib():
         L.nsp_mesh_records.argtypes = [P, dp, dp, dp]
         L.nsp_mesh_tree_dump.argtypes = [P, i64p, C.c_int64]
         L.nsp_mesh_tree_dump.restype = C.c_int64
+        L.nsp_mesh_set_record_layout.argtypes = [P, C.c_int]
+        L.nsp_mesh_get_record_layout.argtypes = [P]
         L.nsp_trace_first_hit.argtypes = [P, fp, fp, C.c_int64, ip, dp, C.POINTER(C.c_uint8)]
         L.nsp_trace_first_hit_device.argtypes = [P, P, P, C.c_int64, P, P, P, P]
         L.nsp_scene_create.argtypes = [P, C.c_int, P, C.POINTER(Surface), C.POINTER(Light), C.c_int,
@@ -255,6 +260,14 @@ class Mesh:
         self.ctx.check(self.ctx._L.nsp_mesh_records(self.h, _ptr(nrm, C.c_double), _ptr(pn, C.c_double),
                                                     _ptr(pd, C.c_double)))
         return nrm, pn, pd
+
+    @property
+    def record_layout(self):
+        return self.ctx._L.nsp_mesh_get_record_layout(self.h)
+
+    @record_layout.setter
+    def record_layout(self, layout):
+        self.ctx.check(self.ctx._L.nsp_mesh_set_record_layout(self.h, int(layout)))
 
     def tree_dump(self):
         L = self.ctx._L
@@ -479,3 +492,59 @@ class Denoiser:
         self.ctx.check(_denoiser_lib().nsp_denoiser_train(self.h, _ptr(x, C.c_float), _ptr(t, C.c_float), S, d, h, w,
                                                           patch, batch, steps, lr, seed, _ptr(losses, C.c_float)))
         return losses
+
+
+# ---------------------------------------------------------------------------
+# Batched persistent launch and autotuner
+
+def _batch_lib():
+    L = lib()
+    if not getattr(L, "_batch_ready", False):
+        P = C.c_void_p
+        L.nsp_batch_create.argtypes = [P, C.POINTER(P), C.c_int, C.POINTER(P)]
+        L.nsp_batch_destroy.argtypes = [P]
+        L.nsp_batch_destroy.restype = None
+        L.nsp_batch_paint.argtypes = [P, C.POINTER(PaintParams), C.POINTER(C.c_int32)]
+        L.nsp_batch_statistics.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+        L.nsp_autotune.argtypes = [P, C.POINTER(PaintParams)]
+        L._batch_ready = True
+    return L
+
+
+class Batch:
+    def __init__(self, ctx, scenes):
+        self.ctx = ctx
+        self.scenes = list(scenes)
+        L = _batch_lib()
+        arr = (C.c_void_p * len(self.scenes))(*[s.h.value for s in self.scenes])
+        h = C.c_void_p()
+        ctx.check(L.nsp_batch_create(ctx.h, arr, len(self.scenes), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            _batch_lib().nsp_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def paint(self, spp, max_depth=5, seed=1, batch_paths=0, use_graph=False):
+        p = PaintParams(spp, max_depth, 0, PRECISION_FP32, 1, seed, batch_paths, int(use_graph), 0)
+        stop = C.c_int32(0)
+        self.ctx.check(_batch_lib().nsp_batch_paint(self.h, C.byref(p), C.byref(stop)))
+
+    def statistics(self):
+        r, t = C.c_int64(), C.c_double()
+        self.ctx.check(_batch_lib().nsp_batch_statistics(self.h, C.byref(r), C.byref(t)))
+        return dict(rays=r.value, last_pass_s=t.value)
+
+
+def autotune(scene, spp, max_depth=5, seed=1):
+    """Queue size (paths per wavefront batch) chosen by timing; the scene's image is restored."""
+    p = PaintParams(spp, max_depth, 0, PRECISION_FP32, 1, seed, 0, 0, 0)
+    scene.ctx.check(_batch_lib().nsp_autotune(scene.h, C.byref(p)))
+    return p.batch_paths

@@ -1,0 +1,69 @@
+"""Batched persistent launch (many scenes of uneven size and dimension in one
+launch sequence) and the autotuner, against per-scene renders."""
+import numpy as np
+import pytest
+
+from math_amd import configs, native
+
+pytestmark = pytest.mark.gpu
+
+
+def scene_of(ctx, n, g, edge, seed):
+    v, f = native.meshgen_cube_sphere(n, g, True, 0.02, 0.0, seed)
+    cfg = configs.scene_dict(v, f, edge, tuple([2.0] * n))
+    mesh = ctx.mesh(v, f)
+    return mesh, cfg
+
+
+def test_batch_matches_per_scene_renders(ctx):
+    specs = [(3, 8, 40, 1), (4, 4, 12, 2), (5, 2, 6, 3), (4, 3, 9, 4), (3, 5, 24, 5)]
+    meshes = [scene_of(ctx, *s) for s in specs]
+    singles, batched = [], []
+    for mesh, cfg in meshes:
+        a = native.Scene(ctx, mesh, cfg)
+        a.paint(4, 4, seed=9)
+        singles.append(a)
+        batched.append(native.Scene(ctx, mesh, cfg))
+    b = native.Batch(ctx, batched)
+    b.paint(4, 4, seed=9, batch_paths=5000)
+    rays = 0
+    for a, c in zip(singles, batched):
+        sa, ca = a.image()
+        sc, cc = c.image()
+        assert np.array_equal(ca, cc)
+        rmse = float(np.sqrt(np.mean((sa / ca[:, None] - sc / cc[:, None]) ** 2)))
+        assert rmse < 1e-4, rmse
+        rays += a.statistics()["rays"]
+        assert c.statistics()["passes"] == 1
+    assert abs(b.statistics()["rays"] - rays) <= 1e-3 * rays
+
+
+def test_batch_graph_replay_equals_eager(ctx):
+    specs = [(3, 6, 20, 1), (4, 3, 8, 2), (5, 2, 5, 3)]
+    meshes = [scene_of(ctx, *s) for s in specs]
+    e = [native.Scene(ctx, m, c) for m, c in meshes]
+    g = [native.Scene(ctx, m, c) for m, c in meshes]
+    be, bg = native.Batch(ctx, e), native.Batch(ctx, g)
+    for _ in range(2):
+        be.paint(4, 3, seed=2, batch_paths=3000)
+        bg.paint(4, 3, seed=2, batch_paths=3000, use_graph=True)
+        for x, y in zip(e, g):
+            assert all(np.array_equal(p, q) for p, q in zip(x.image(), y.image()))
+
+
+def test_autotune_restores_state_and_choice_is_image_invariant(ctx):
+    mesh, cfg = scene_of(ctx, 4, 4, 16, 7)
+    s = native.Scene(ctx, mesh, cfg)
+    s.paint(4, 3, seed=1)
+    before = s.image()
+    stats = s.statistics()
+    best = native.autotune(s, 4, 3, seed=1)
+    assert best in [1 << k for k in range(20, 25)]
+    after = s.image()
+    assert all(np.array_equal(p, q) for p, q in zip(before, after))
+    assert s.statistics() == stats
+    t1 = native.Scene(ctx, mesh, cfg)
+    t1.paint(4, 3, seed=5, batch_paths=best)
+    t2 = native.Scene(ctx, mesh, cfg)
+    t2.paint(4, 3, seed=5, batch_paths=1 << 20)
+    assert all(np.array_equal(p, q) for p, q in zip(t1.image(), t2.image()))

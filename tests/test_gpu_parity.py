@@ -170,6 +170,27 @@ def test_render_fp32_matches_checker(ctx, n):
     assert rmse < 5e-3
 
 
+def test_record_layouts_identical(ctx):
+    # both FP32 record copies hold the same values: hits, distances and images are bit-identical
+    v, f = sphere_mesh(4, 4)
+    gm = ctx.mesh(v, f)
+    org, d = rays_at(4, 4000, 21)
+    out = {}
+    for layout in (native.LAYOUT_LEAF_SOA, native.LAYOUT_SIMPLEX_AOS):
+        print(f"record layout {layout}", flush=True)
+        gm.record_layout = layout
+        assert gm.record_layout == layout
+        hits = gm.first_hit(org, d)
+        cfg = configs.scene_dict(v, f, 10, (2.0,) * 4)
+        sc = native.Scene(ctx, gm, cfg)
+        sc.paint(8, 3, precision=native.PRECISION_FP32, seed=4)
+        out[layout] = hits + sc.image()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+    with pytest.raises(native.NspError):
+        gm.record_layout = 7
+
+
 def test_render_deterministic_and_batch_invariant(ctx):
     v, f, cfg, spp, depth = small_scene(4, 4, 12, spp=8)
     gm = ctx.mesh(v, f)
