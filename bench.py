@@ -39,9 +39,25 @@ def closest_bytes(st, n):
             + 4 * (n * n + n) * st["simplex_tests"])
 
 
+def profiled_traffic(config, layout):
+    """HBM bytes per k_closest launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of the same workload and record layout (scaled by the
+    copy-kernel calibration stated there), or None."""
+    path = os.path.join(ROOT, "profiles", f"k_closest_{config.lower()}.md")
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    if f"Record layout: {layout}" not in text:
+        return None
+    for line in text.splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def cpu_baseline(v, f, cfg, spp, depth, seconds_target=15.0):
-    """FP64 CPU restatement on a crop of the C1 screen, all host threads."""
-    import numpy as np  # noqa: F401
+    """FP64 CPU restatement on a crop of the same screen, all host threads."""
+    import numpy as np
 
     from oracle import oracle as O
 
@@ -51,18 +67,18 @@ def cpu_baseline(v, f, cfg, spp, depth, seconds_target=15.0):
     m.build_tree()
     sc = O.make_scene(cfg, spp, depth, 12345)
     screen = cfg["screen"]
-    # crop: rows of pixels through the middle of the screen, grown until ~seconds_target
-    mid = (screen[2] // 2) * screen[0] * screen[1] + (screen[1] // 2) * screen[0]
+    # crop: consecutive pixels around the middle of the screen, grown until ~seconds_target
+    total = int(np.prod(screen))
+    mid = total // 2 + screen[0] // 2
     count = 64 * threads
     while True:
         t0 = time.perf_counter()
         _, rays = O.render(m, sc, mid, count, threads)
         dt = time.perf_counter() - t0
-        if dt > seconds_target / 4 or count >= screen[0] * screen[1]:
+        if dt > seconds_target / 4 or count >= total // 8:
             break
         count *= 4
     if dt < seconds_target:
-        total = screen[0] * screen[1] * screen[2]
         count2 = min(int(count * seconds_target / max(dt, 1e-3)), total)
         start = max(0, min(mid - count2 // 2, total - count2))
         t0 = time.perf_counter()
@@ -70,7 +86,7 @@ def cpu_baseline(v, f, cfg, spp, depth, seconds_target=15.0):
         dt = time.perf_counter() - t0
         count = count2
     return dict(value=rays / dt / 1e6, unit="Mrays/s", cores=threads, kind="port",
-                sample=f"{count} pixels x 27 samples from the middle of the 128^3 screen, FP64, "
+                sample=f"{count} of {total} pixels ({screen}), all samples, from the middle of the screen, FP64, "
                        f"{rays} rays in {dt:.1f} s")
 
 
@@ -149,8 +165,10 @@ def batch_bench(args):
         f"({min(m[2] for m in meshes)}..{max(m[2] for m in meshes)} simplices)")
     per = [native.Scene(ctx, m, c) for m, c, _ in meshes]
     t0 = time.perf_counter()
-    for s in per:
+    for i, s in enumerate(per):
         s.paint(4, 5, seed=1)
+        log(f"per-scene {i}: n={s.n} {s.pixels} pixels, {s.statistics()['rays']} rays, "
+            f"{time.perf_counter() - t0:.1f} s cumulative")
     per_s = time.perf_counter() - t0
     per_rays = sum(s.statistics()["rays"] for s in per)
     bat = [native.Scene(ctx, m, c) for m, c, _ in meshes]
@@ -251,8 +269,9 @@ def main():
     scene.set_counters(False)
     bytes_pass = closest_bytes(cst, n)
     achieved = bytes_pass / (closest_ms / args.steps * 1e-3) / 1e9
+    layout_name = ["leaf_soa", "simplex_aos"][mesh.record_layout]
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=profiled_traffic(args.config, layout_name),
                     kernel="k_closest", launches_per_step=closest_launches // max(args.steps, 1),
                     avg_launch_ms=round(closest_ms / max(closest_launches, 1), 4),
                     bytes_per_step=int(bytes_pass),
@@ -276,11 +295,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"{args.config}: n=4 plain path trace, 663,552-tetrahedron noisy S^3 mesh, "
-                               "128^3 parallel-projected screen, 16 spp requested (27 samples/pixel), depth 5, "
-                               "point light (2,2,2,2), FP32 trace + FP64 re-test",
+        "config": {"workload": configs.WORKLOADS[args.config] + ", FP32 trace + FP64 re-test",
                    "rays_per_step": rays // args.steps, "mesh_build_s": round(build_s, 3),
-                   "record_layout": ["leaf_soa", "simplex_aos"][mesh.record_layout],
+                   "record_layout": layout_name,
                    "tree": dict(nodes=info.tree_nodes, leaves=info.tree_leaves, entries=info.tree_entries)},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -76,6 +76,16 @@ CONFIGS = {
 }
 
 
+WORKLOADS = {
+    "C0": "C0: n=3 plain path trace, 40,000-triangle noisy torus, 512^2 parallel-projected screen, 16 spp, depth 5, "
+          "point light (3,3,3)",
+    "C1": "C1: n=4 plain path trace, 663,552-tetrahedron noisy S^3 mesh, 128^3 parallel-projected screen, "
+          "16 spp requested (27 samples/pixel), depth 5, point light (2,2,2,2)",
+    "C2": "C2: n=5 plain path trace, S^4 penteract-boundary mesh (2.4M 4-simplices before a 0.5 rad cap is removed), "
+          "48^4 parallel-projected screen, 8 spp requested (16 samples/pixel), depth 4, point light (2,2,2,2,2)",
+}
+
+
 def make_mesh(gen, args):
     if gen == "torus":
         return native.meshgen_torus(**args)
