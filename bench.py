@@ -92,9 +92,8 @@ def cpu_baseline(v, f, cfg, spp, depth, seconds_target=15.0):
 
 def denoise_bench(args):
     """C3: denoiser trained on the tracer's own renders; time to RMSE <= 0.01
-    against a 4096-spp FP64 render, FP32 + denoiser vs plain FP32.  Evaluation
-    uses held-out variants rendered in a window of the 128^3 screen (same pixel
-    footprint) to bound the cost of the FP64 references."""
+    against a 4096-sample FP64 render, FP32 + denoiser vs plain FP32, on
+    held-out 128^3 scenes (or a centred window of them with --eval-window)."""
     import numpy as np
 
     from math_amd import denoise_eval as DE
@@ -104,37 +103,52 @@ def denoise_bench(args):
     t0 = time.perf_counter()
     X, Y = DE.training_set(ctx, list(range(100, 100 + args.train_variants)), window=args.window, log=log)
     gen_s = time.perf_counter() - t0
+    log(f"training set: {X.shape[0]} volumes of {X.shape[1]}^3 in {gen_s:.1f} s")
     dn = native.Denoiser(ctx, DE.IN_CHANNELS, DE.RESIDUAL, seed=1)
     t0 = time.perf_counter()
-    losses = dn.train(X, Y, patch=32, batch=16, steps=args.train_steps, lr=1e-3, seed=2)
+    chunks = []
+    for c in range(0, args.train_steps, 500):
+        k = min(500, args.train_steps - c)
+        frac = c / args.train_steps
+        lr = 1e-3 if frac < 0.6 else (3e-4 if frac < 0.85 else 1e-4)
+        chunks.append(dn.train(X, Y, patch=32, batch=16, steps=k, lr=lr, seed=2 + c))
+        log(f"training steps {c + k}: lr {lr:g}, loss {chunks[-1][-50:].mean():.5f} ({time.perf_counter() - t0:.1f} s)")
+    losses = np.concatenate(chunks)
     train_s = time.perf_counter() - t0
-    log(f"training: {args.train_steps} steps in {train_s:.1f} s, loss {losses[:5].mean():.5f} -> {losses[-50:].mean():.5f}")
+    del X, Y
     results = []
     for k in range(args.eval_scenes):
         seed = 200 + k
-        rng = np.random.default_rng(seed)
-        org = [int(rng.integers(0, 128 - args.window + 1)) for _ in range(3)]
-        v, f, cfg = DE.variant(seed, 128, (org, args.window))
+        if args.eval_window:
+            w = args.eval_window
+            v, f, cfg = DE.variant(seed, 128, ([(128 - w) // 2] * 3, w))
+        else:
+            v, f, cfg = DE.variant(seed, 128)
         mesh = ctx.mesh(v, f)
-        ref, ref_s = DE.render_mean(ctx, mesh, cfg, 4096, seed=seed * 7, precision=native.PRECISION_FP64)
-        log(f"eval scene {seed}: FP64 4096-spp reference in {ref_s:.1f} s")
+        # 8 passes of 8^3 = 512 stratified samples: 4096 samples per pixel
+        ref, ref_s = DE.render_mean(ctx, mesh, cfg, 512, seed=seed * 7, precision=native.PRECISION_FP64, passes=8,
+                                    log=log)
+        log(f"eval scene {seed}: FP64 4096-sample reference in {ref_s:.1f} s")
         r = DE.time_to_rmse(ctx, mesh, cfg, ref, dn, seed=seed * 11)
         r["seed"] = seed
-        log(f"eval scene {seed}: plain t={r['plain_time_s']} curve={[(c, round(t, 3), round(e, 4)) for c, t, e in r['plain_curve']][:8]}"
-            f"; denoised rmse {r['denoised_rmse']:.4f} (4-spp input {r['input4_rmse']:.4f}) in {r['denoised_time_s']:.3f} s")
+        log(f"eval scene {seed}: plain {r['plain_time_s']} s, curve "
+            f"{[(c, round(t, 3), round(e, 4)) for c, t, e in r['plain_curve']]}")
+        log(f"eval scene {seed}: denoised {r['denoised_time_s']} s (level-1 rmse {r['denoised_rmse']:.4f}, "
+            f"its 4-spp input {r['input4_rmse']:.4f}), curve "
+            f"{[(c, round(t, 3), round(e, 4)) for c, t, e in r['denoised_curve']]}")
         results.append(r)
         mesh.close()
-    ok = [r for r in results if r["plain_time_s"] is not None]
+    ok = [r for r in results if r["plain_time_s"] is not None and r["denoised_time_s"] is not None]
     out = {
-        "metric": "time to RMSE<=0.01 vs 4096-spp FP64 ref at n=4, FP32+denoiser vs plain FP32",
+        "metric": "time to RMSE<=0.01 vs 4096-sample FP64 ref at n=4, FP32+denoiser vs plain FP32",
+        "eval_screen": f"{args.eval_window}^3 centred window" if args.eval_window else "128^3",
         "eval": [{k: r[k] for k in ("seed", "plain_time_s", "denoised_time_s", "denoised_rmse", "input4_rmse",
                                     "denoiser_inference_s")} for r in results],
-        "denoised_reaches_0.01": [r["denoised_rmse"] <= 0.01 for r in results],
-        "speedup": [r["plain_time_s"] / r["denoised_time_s"] for r in ok if r["denoised_rmse"] <= 0.01],
-        "train": {"variants": args.train_variants, "steps": args.train_steps, "seconds": round(train_s, 2),
-                  "data_seconds": round(gen_s, 2), "loss_first": float(losses[:5].mean()),
-                  "loss_last": float(losses[-50:].mean())},
-        "window": args.window,
+        "speedup": [round(r["plain_time_s"] / r["denoised_time_s"], 3) for r in ok],
+        "curves": [{"seed": r["seed"], "plain": r["plain_curve"], "denoised": r["denoised_curve"]} for r in results],
+        "train": {"variants": args.train_variants, "levels": list(DE.LEVELS), "window": args.window,
+                  "steps": args.train_steps, "seconds": round(train_s, 2), "data_seconds": round(gen_s, 2),
+                  "loss_first": float(losses[:5].mean()), "loss_last": float(losses[-50:].mean())},
     }
     print(json.dumps(out), flush=True)
 
@@ -203,10 +217,11 @@ def main():
     ap.add_argument("--denoise", action="store_true",
                     help="C3: train the denoiser on renders of n=4 variants, then time to RMSE<=0.01, "
                          "denoised vs plain")
-    ap.add_argument("--train-variants", type=int, default=16)
-    ap.add_argument("--train-steps", type=int, default=1500)
+    ap.add_argument("--train-variants", type=int, default=32)
+    ap.add_argument("--train-steps", type=int, default=6000)
     ap.add_argument("--eval-scenes", type=int, default=2)
     ap.add_argument("--window", type=int, default=64)
+    ap.add_argument("--eval-window", type=int, default=0, help="evaluate on a centred window (0: full 128^3)")
     ap.add_argument("--batch-scenes", action="store_true",
                     help="C4: batched persistent launch over many scenes vs one launch sequence per scene")
     ap.add_argument("--scenes", type=int, default=256)

@@ -172,7 +172,7 @@ typedef struct {
     uint64_t seed;         /* counter-based RNG key */
     int64_t batch_paths;   /* paths per wavefront batch; 0 = library default/autotuned */
     int use_graph;         /* capture each batch's per-bounce launches in a hipGraph */
-    int aov;               /* also accumulate first-hit albedo and normal per pixel (denoiser inputs) */
+    int aov;               /* also accumulate first-hit albedo, normal and direct light per pixel (denoiser inputs) */
 } nsp_paint_params;
 
 /* Delivery of results: PainterNotifier::painter_pixel_after(pixel, color)
@@ -207,11 +207,13 @@ int nsp_statistics(const nsp_scene* scene, int64_t* pass_count, int64_t* pixel_c
  * checkpoint of a progressive render. */
 int nsp_image_get(const nsp_scene* scene, float* rgb_sum, int32_t* samples);
 int nsp_image_set(nsp_scene* scene, const float* rgb_sum, const int32_t* samples);
-/* Per-pixel sums of the first-hit albedo (pixels*3) and of the first-hit
- * normal facing the camera (pixels*n) over the samples of passes painted with
- * aov = 1 (divide by the sample counts of nsp_image_get).  No reference
- * counterpart: these are the denoiser's auxiliary inputs. */
-int nsp_image_aov(const nsp_scene* scene, float* albedo_sum, float* normal_sum);
+/* Per-pixel sums of the first-hit albedo (pixels*3), of the first-hit normal
+ * facing the camera (pixels*n) and of the radiance the first hit receives
+ * directly (pixels*3: background or emitter seen by the primary ray plus the
+ * unoccluded first-hit light contributions) over the samples of passes painted
+ * with aov = 1 (divide by the sample counts of nsp_image_get).  Any pointer may
+ * be NULL.  No reference counterpart: these are the denoiser's auxiliary inputs. */
+int nsp_image_aov(const nsp_scene* scene, float* albedo_sum, float* normal_sum, float* direct_sum);
 /* Number of pixels of the scene's screen. */
 int64_t nsp_scene_pixel_count(const nsp_scene* scene);
 

@@ -102,7 +102,7 @@ struct nsp_scene {
     nsp_surface background;
     int64_t pixels = 0;
     DevBuf pix_sum, pix_cnt;
-    DevBuf pix_aov;  // (3 + n) * pixels, allocated on the first pass with aov
+    DevBuf pix_aov;  // aov_channels(n) * pixels, allocated on the first pass with aov
     DevBuf totals;  // 3 u64
     StageMarks marks;
     bool counters = false;
@@ -575,10 +575,10 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
         const int64_t cap = batch_pixels * S;
         alloc_render_buffers(ctx->work, n, p.precision, cap, p.max_depth);
         if (p.aov) {
-            ctx->work.aov.alloc(sizeof(float) * (3 + n) * cap);
+            ctx->work.aov.alloc(sizeof(float) * aov_channels(n) * cap);
             if (!scene->pix_aov.p) {
-                scene->pix_aov.alloc(sizeof(float) * (3 + n) * scene->pixels);
-                NSP_HIP(hipMemsetAsync(scene->pix_aov.p, 0, sizeof(float) * (3 + n) * scene->pixels, s));
+                scene->pix_aov.alloc(sizeof(float) * aov_channels(n) * scene->pixels);
+                NSP_HIP(hipMemsetAsync(scene->pix_aov.p, 0, sizeof(float) * aov_channels(n) * scene->pixels, s));
             }
         }
         for (int pass = 0; pass < p.passes; ++pass) {
@@ -738,13 +738,13 @@ int nsp_image_set(nsp_scene* scene, const float* rgb_sum, const int32_t* samples
     });
 }
 
-int nsp_image_aov(const nsp_scene* scene, float* albedo_sum, float* normal_sum)
+int nsp_image_aov(const nsp_scene* scene, float* albedo_sum, float* normal_sum, float* direct_sum)
 {
     if (!scene) return NSP_E_INVALID;
     return guarded(scene->ctx, [&] {
         const int64_t P = scene->pixels;
         const int n = scene->n;
-        std::vector<float> a((size_t)(3 + n) * P, 0.0f);
+        std::vector<float> a((size_t)aov_channels(n) * P, 0.0f);
         if (scene->pix_aov.p)
             NSP_HIP(hipMemcpy(a.data(), scene->pix_aov.p, sizeof(float) * a.size(), hipMemcpyDeviceToHost));
         for (int64_t j = 0; j < P; ++j) {
@@ -752,6 +752,8 @@ int nsp_image_aov(const nsp_scene* scene, float* albedo_sum, float* normal_sum)
                 for (int c = 0; c < 3; ++c) albedo_sum[j * 3 + c] = a[(size_t)c * P + j];
             if (normal_sum)
                 for (int k = 0; k < n; ++k) normal_sum[j * n + k] = a[(size_t)(3 + k) * P + j];
+            if (direct_sum)
+                for (int c = 0; c < 3; ++c) direct_sum[j * 3 + c] = a[(size_t)(3 + n + c) * P + j];
         }
     });
 }

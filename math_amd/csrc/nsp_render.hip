@@ -13,6 +13,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
 
 #include "nsp_device.hpp"
 #include "nsp_render.hpp"
@@ -28,6 +30,26 @@ static unsigned grid_for(int64_t n, int block = TRACE_BLOCK, int64_t max_blocks 
     // multiple of 8 so the XCD remap is a permutation
     b = (b + 7) / 8 * 8;
     return (unsigned)b;
+}
+
+// Blocks of a persistent launch: every CU filled to the kernel's occupancy
+// (computed once per kernel), rounded to a multiple of 8 for the XCD remap.
+template <class K>
+static unsigned persistent_grid(K kernel)
+{
+    static std::mutex mu;
+    static std::map<const void*, unsigned> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    unsigned& blocks = cache[reinterpret_cast<const void*>(kernel)];
+    if (!blocks) {
+        int dev = 0, cus = 0, per_cu = 0;
+        NSP_HIP(hipGetDevice(&dev));
+        NSP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        NSP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, TRACE_BLOCK, 0));
+        unsigned b = (unsigned)std::max(1, per_cu) * (unsigned)std::max(1, cus);
+        blocks = (b + 7) / 8 * 8;
+    }
+    return blocks;
 }
 
 TreeArgs make_tree_args(const MeshDev& m)
@@ -183,30 +205,6 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_raygen(SceneK<T> sc, const Batc
 }
 
 template <int N, typename T>
-__global__ void __launch_bounds__(TRACE_BLOCK) k_closest(TreeArgs A, RayQueue<T> q, int64_t cap,
-                                                         const unsigned long long* __restrict__ count_in)
-{
-    __shared__ int stack[MAX_LEVELS * TRACE_BLOCK];
-    int* st = stack + threadIdx.x;
-    const int64_t count = (int64_t)*count_in;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x; i < count; i += stride) {
-        T o[N], d[N];
-        double o64[N], d64[N];
-#pragma unroll
-        for (int k = 0; k < N; ++k) {
-            o[k] = q.org[(int64_t)k * cap + i];
-            d[k] = q.dir[(int64_t)k * cap + i];
-            o64[k] = o[k];
-            d64[k] = d[k];
-        }
-        HitResult h = traverse<N, T, false>(A, o, d, (T)INFINITY, st, o64, d64);
-        q.hit[i] = h.idx;
-        q.hit_t[i] = (T)h.t;
-    }
-}
-
-template <int N, typename T>
 __global__ void __launch_bounds__(TRACE_BLOCK)
     k_shade(SceneK<T> sc, TreeArgs A, MeshShadeArgs M, const BatchDesc* __restrict__ desc, int depth,
             RayQueue<T> qin, RayQueue<T> qout, ShadowQueue<T> sq, int64_t cap, unsigned long long* __restrict__ cnt,
@@ -246,9 +244,14 @@ __global__ void __launch_bounds__(TRACE_BLOCK)
                             qin.level[i], st, probe_count, (aov && depth == 0) ? av : nullptr, out);
 #pragma unroll
             for (int c = 0; c < 3; ++c) L[c * cap + path] += out.add[c];
-            if (aov && depth == 0)
+            if (aov && depth == 0) {
 #pragma unroll
                 for (int c = 0; c < 3 + N; ++c) aov[c * cap + path] = av[c];
+                // direct channels start with what the primary ray sees itself
+                // (background or emitter); first-hit shadow rays add to them
+#pragma unroll
+                for (int c = 0; c < 3; ++c) aov[(3 + N + c) * cap + path] = out.add[c];
+            }
         }
         // compaction: one atomic per wave per queue
         int64_t slot = wave_append(out.next, next_count);
@@ -272,7 +275,9 @@ __global__ void __launch_bounds__(TRACE_BLOCK)
                 sq.dir[(int64_t)k * scap + ss] = out.sdir[k];
             }
             sq.tmax[ss] = out.stmax;
-            sq.path[ss] = path;
+            // a first-hit shadow ray is marked by a negative path index while
+            // the direct-light AOV is collected
+            sq.path[ss] = (aov && depth == 0) ? -(path + 1) : path;
             sq.contrib[ss] = out.contrib[0];
             sq.contrib[scap + ss] = out.contrib[1];
             sq.contrib[2 * scap + ss] = out.contrib[2];
@@ -280,37 +285,200 @@ __global__ void __launch_bounds__(TRACE_BLOCK)
     }
 }
 
-// Shadow rays: any hit before the light.  Different shadow rays of one path
-// are applied in separate launches per light slot would be needed for
-// determinism with several lights; with one light per path and bounce each
-// path receives at most one contribution per launch.
+// Adds an unoccluded shadow ray's contribution to its path (and, for a
+// first-hit shadow ray while AOVs are collected, to the direct-light AOV).
+// With one light per path and bounce, each path receives at most one
+// contribution per shadow launch, so the additions need no atomics.
 template <int N, typename T>
-__global__ void __launch_bounds__(TRACE_BLOCK) k_shadow(TreeArgs A, ShadowQueue<T> sq, int64_t scap,
-                                                        const unsigned long long* __restrict__ count_in,
-                                                        float* __restrict__ L, int64_t cap)
+__device__ __forceinline__ void shadow_unoccluded(const ShadowQueue<T>& sq, int64_t scap, int64_t i,
+                                                  float* __restrict__ L, int64_t cap, float* __restrict__ direct)
+{
+    const int32_t raw = sq.path[i];
+    const int32_t path = raw < 0 ? -raw - 1 : raw;
+    const float c0 = sq.contrib[i], c1 = sq.contrib[scap + i], c2 = sq.contrib[2 * scap + i];
+    L[path] += c0;
+    L[cap + path] += c1;
+    L[2 * cap + path] += c2;
+    if (raw < 0 && direct) {
+        direct[path] += c0;
+        direct[cap + path] += c1;
+        direct[2 * cap + path] += c2;
+    }
+}
+
+// Persistent traversal (shadow stage): each lane holds one ray walk (RayWalk)
+// across the iterations of the loop and does one leaf step per iteration; when
+// at least REFILL_LANES lanes of a wave have finished, the wave takes that
+// many new rays with one atomic.  The queue is split into 8 contiguous slices, one per XCD
+// (workgroups are dispatched round robin over the 8 XCDs): a wave takes rays
+// from its own XCD's slice, so neighbouring rays share that XCD's L2, and
+// moves on to the next slices once its own is empty.  Every wave leaves the
+// loop after all 8 slices are empty and its own rays are done.
+constexpr int REFILL_LANES = 32;
+constexpr int XCDS = 8;
+
+struct RayPool {
+    unsigned long long* work;  // XCDS slice counters
+    int64_t count;
+    int slice, visited;
+    bool drained;
+
+    __device__ __forceinline__ RayPool(unsigned long long* w, int64_t n)
+        : work(w), count(n), slice((int)(blockIdx.x % XCDS)), visited(0), drained(false)
+    {
+    }
+
+    // Ray index for each lane with want set (or -1); wave-uniform call.
+    __device__ __forceinline__ int64_t take(bool want)
+    {
+        const unsigned long long idle = __ballot(want);
+        const int nidle = __popcll(idle);
+        if (nidle < REFILL_LANES) return -1;
+        const int64_t lo = count * slice / XCDS, hi = count * (slice + 1) / XCDS;
+        const int lane = (int)(threadIdx.x & 63);
+        const int leader = __ffsll((unsigned long long)idle) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(&work[slice], (unsigned long long)nidle);
+        const unsigned blo = __shfl((unsigned)base, leader), bhi = __shfl((unsigned)(base >> 32), leader);
+        base = ((unsigned long long)bhi << 32) | blo;
+        if (lo + (int64_t)base + nidle >= hi) {
+            slice = (slice + 1) % XCDS;
+            if (++visited == XCDS) drained = true;
+        }
+        if (!want) return -1;
+        const int64_t i = lo + (int64_t)base + __popcll(idle & ((1ull << lane) - 1ull));
+        return i < hi ? i : -1;
+    }
+};
+
+// Closest hit: lane = ray, grid-stride over the queue, the whole walk with
+// the FP64 re-test inline.  Consecutive rays are samples of one pixel, so the
+// lanes of a wave walk nearly the same nodes and leaves; this static
+// assignment keeps them together, which measured faster here than the
+// persistent refill loop the shadow stage uses.
+template <int N, typename T, int LAY>
+__global__ void __launch_bounds__(TRACE_BLOCK) k_closest(TreeArgs A, RayQueue<T> q, int64_t cap,
+                                                         const unsigned long long* __restrict__ cnt)
 {
     __shared__ int stack[MAX_LEVELS * TRACE_BLOCK];
     int* st = stack + threadIdx.x;
-    const int64_t count = (int64_t)*count_in;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x; i < count; i += stride) {
+    const int64_t count = (int64_t)cnt[0];
+    for (int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        RayWalk<N, T, false, LAY, false> w;
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            w.o[k] = q.org[(int64_t)k * cap + i];
+            w.d[k] = q.dir[(int64_t)k * cap + i];
+        }
+        HitResult h{-1, 0.0, false};
+        if (w.start(A, (T)INFINITY)) {
+            while (!w.step(A, st, nullptr, nullptr)) {
+            }
+            w.flush_counters(A);
+            h = w.result();
+        }
+        q.hit[i] = h.idx;
+        q.hit_t[i] = (T)h.t;
+    }
+}
+
+template <int N, typename T, int LAY>
+__global__ void __launch_bounds__(TRACE_BLOCK) k_shadow(TreeArgs A, ShadowQueue<T> sq, int64_t scap,
+                                                        unsigned long long* __restrict__ cnt,
+                                                        float* __restrict__ L, int64_t cap,
+                                                        float* __restrict__ direct,
+                                                        unsigned long long* __restrict__ redo_count,
+                                                        int32_t* __restrict__ redo,
+                                                        unsigned long long* __restrict__ res_count,
+                                                        int32_t* __restrict__ res)
+{
+    __shared__ int stack[MAX_LEVELS * TRACE_BLOCK];
+    int* st = stack + threadIdx.x;
+    RayPool pool(cnt + 15, (int64_t)cnt[0]);
+    RayWalk<N, T, true, LAY, true> w;
+    int64_t ray = -1;
+    for (;;) {
+        if (!pool.drained) {
+            const int64_t i = pool.take(ray < 0);
+            if (i >= 0) {
+#pragma unroll
+                for (int k = 0; k < N; ++k) {
+                    w.o[k] = sq.org[(int64_t)k * scap + i];
+                    w.d[k] = sq.dir[(int64_t)k * scap + i];
+                }
+                if (w.start(A, sq.tmax[i]))
+                    ray = i;
+                else
+                    shadow_unoccluded<N, T>(sq, scap, i, L, cap, direct);
+            }
+        }
+        if (ray >= 0 && w.step(A, st, nullptr, nullptr)) {
+            if (w.deferred) {
+                redo[atomicAdd(redo_count, 1ull)] = (int32_t)ray;
+            } else if (w.best < 0) {
+                if (w.npend > 0) {
+                    const int64_t k = (int64_t)atomicAdd(res_count, 1ull);
+                    res[3 * k] = (int32_t)ray;
+                    res[3 * k + 1] = w.pend[0];
+                    res[3 * k + 2] = w.npend > 1 ? w.pend[PENDING - 1] : -1;
+                } else {
+                    shadow_unoccluded<N, T>(sq, scap, ray, L, cap, direct);
+                }
+            }
+            w.flush_counters(A);
+            ray = -1;
+        }
+        if (pool.drained && __ballot(ray >= 0) == 0) break;
+    }
+}
+
+template <int N, typename T>
+__global__ void __launch_bounds__(TRACE_BLOCK) k_shadow_exact(TreeArgs A, ShadowQueue<T> sq, int64_t scap,
+                                                              float* __restrict__ L, int64_t cap,
+                                                              float* __restrict__ direct,
+                                                              const unsigned long long* __restrict__ redo_count,
+                                                              const int32_t* __restrict__ redo)
+{
+    __shared__ int stack[MAX_LEVELS * TRACE_BLOCK];
+    int* st = stack + threadIdx.x;
+    const int64_t count = (int64_t)*redo_count;
+    for (int64_t j = (int64_t)xcd_block() * blockDim.x + threadIdx.x; j < count;
+         j += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = redo[j];
         T o[N], d[N];
-        double o64[N], d64[N];
 #pragma unroll
         for (int k = 0; k < N; ++k) {
             o[k] = sq.org[(int64_t)k * scap + i];
             d[k] = sq.dir[(int64_t)k * scap + i];
-            o64[k] = o[k];
-            d64[k] = d[k];
         }
-        T tmax = sq.tmax[i];
-        HitResult h = traverse<N, T, true>(A, o, d, tmax, st, o64, d64);
-        if (h.idx < 0) {
-            int32_t path = sq.path[i];
-            L[path] += sq.contrib[i];
-            L[cap + path] += sq.contrib[scap + i];
-            L[2 * cap + path] += sq.contrib[2 * scap + i];
+        if (traverse<N, T, true>(A, o, d, sq.tmax[i], st, nullptr, nullptr).idx < 0)
+            shadow_unoccluded<N, T>(sq, scap, i, L, cap, direct);
+    }
+}
+
+// FP64 decisions of the pending candidates of FP32 shadow walks that found
+// no FP32 hit: the light is visible when none of them is a hit before tmax.
+template <int N>
+__global__ void __launch_bounds__(256) k_shadow_resolve(TreeArgs A, ShadowQueue<float> sq, int64_t scap,
+                                                        float* __restrict__ L, int64_t cap, float* __restrict__ direct,
+                                                        const unsigned long long* __restrict__ res_count,
+                                                        const int32_t* __restrict__ res)
+{
+    const int64_t count = (int64_t)*res_count;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < count;
+         j += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = res[3 * j];
+        float o[N], d[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            o[k] = sq.org[(int64_t)k * scap + i];
+            d[k] = sq.dir[(int64_t)k * scap + i];
         }
+        int best = -1;
+        double best_t = (double)sq.tmax[i];
+        if (!pending_resolve<N, true>(A, o, d, res[3 * j + 1], res[3 * j + 2], sq.tmax[i], &best, &best_t))
+            shadow_unoccluded<N, float>(sq, scap, i, L, cap, direct);
     }
 }
 
@@ -373,6 +541,8 @@ void alloc_render_buffers(RenderBuffers& b, int n, int precision, int64_t cap, i
     b.sq[3].alloc(sizeof(int32_t) * scap);
     b.sq[4].alloc(sizeof(float) * 3 * scap);
     b.L.alloc(sizeof(float) * 3 * cap);
+    b.redo.alloc(sizeof(int32_t) * scap);
+    b.res.alloc(sizeof(int32_t) * 3 * scap);
     b.counters.alloc(sizeof(unsigned long long) * CNT_STRIDE * (max_depth + 2));
     b.desc.alloc(sizeof(BatchDesc));
     b.totals.alloc(sizeof(unsigned long long) * 3);
@@ -416,10 +586,18 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
     float* aov = nullptr;
     if (rl.pix_aov) {
         // allocated by the caller before any graph capture
-        if (b.aov.bytes < sizeof(float) * (3 + N) * cap) raise(NSP_E_INVALID, "AOV buffer not allocated");
+        if (b.aov.bytes < sizeof(float) * aov_channels(N) * cap) raise(NSP_E_INVALID, "AOV buffer not allocated");
         aov = b.aov.as<float>();
     }
     unsigned g = grid_for(cap);
+    // FP32 walks read the record layout chosen at compile time; FP64 walks read the FP64 records
+    const bool aos = sizeof(T) == 4 && A.layout == 1;
+    auto kc = aos ? k_closest<N, T, 1> : k_closest<N, T, 0>;
+    auto ks = aos ? k_shadow<N, T, 1> : k_shadow<N, T, 0>;
+    const unsigned gs = aos ? persistent_grid(k_shadow<N, T, 1>) : persistent_grid(k_shadow<N, T, 0>);
+    const unsigned gx = 8 * 32;  // FP64 decisions and re-walks of shadow rays: a few per cent or fewer
+    int32_t* redo_s = b.redo.as<int32_t>();
+    int32_t* res_s = b.res.as<int32_t>();
     auto timed = [&](int stage, auto&& launch) {
         if (!rl.marks) {
             launch();
@@ -440,22 +618,29 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
     for (int depth = 0; depth <= rl.max_depth; ++depth) {
         RayQueue<T>& qin = (depth & 1) ? qb : qa;
         RayQueue<T>& qout = (depth & 1) ? qa : qb;
-        timed(1, [&] {
-            hipLaunchKernelGGL((k_closest<N, T>), dim3(g), dim3(TRACE_BLOCK), 0, s, A, qin, cap,
-                               cnt + depth * CNT_STRIDE);
-        });
+        unsigned long long* dc = cnt + depth * CNT_STRIDE;
+        timed(1, [&] { hipLaunchKernelGGL(kc, dim3(g), dim3(TRACE_BLOCK), 0, s, A, qin, cap, dc); });
         timed(2, [&] {
             hipLaunchKernelGGL((k_shade<N, T>), dim3(g), dim3(TRACE_BLOCK), 0, s, sc, A, M, desc, depth, qin, qout,
                                sq, cap, cnt, L, aov);
         });
         timed(3, [&] {
-            hipLaunchKernelGGL((k_shadow<N, T>), dim3(grid_for(scap)), dim3(TRACE_BLOCK), 0, s, As, sq, scap,
-                               cnt + depth * CNT_STRIDE + 1, L, cap);
+            float* direct = (aov && depth == 0) ? aov + (int64_t)(3 + N) * cap : nullptr;
+            hipLaunchKernelGGL(ks, dim3(gs), dim3(TRACE_BLOCK), 0, s, As, sq, scap, dc + 1, L, cap, direct,
+                               dc + 6, redo_s, dc + 4, res_s);
+            NSP_HIP(hipGetLastError());
+            if constexpr (sizeof(T) == 4) {
+                hipLaunchKernelGGL((k_shadow_resolve<N>), dim3(gx), dim3(256), 0, s, As, sq, scap, L, cap, direct,
+                                   dc + 4, res_s);
+                NSP_HIP(hipGetLastError());
+            }
+            hipLaunchKernelGGL((k_shadow_exact<N, T>), dim3(gx), dim3(TRACE_BLOCK), 0, s, As, sq, scap, L, cap,
+                               direct, dc + 6, redo_s);
         });
     }
     timed(4, [&] {
         hipLaunchKernelGGL(k_accumulate, dim3(grid_for(cap / std::max(1, sc.S), 256, 4096)), dim3(256), 0, s, desc,
-                           sc.S, L, cap, rl.pix_sum, rl.pix_cnt, rl.pixels, aov, rl.pix_aov, 3 + N);
+                           sc.S, L, cap, rl.pix_sum, rl.pix_cnt, rl.pixels, aov, rl.pix_aov, aov_channels(N));
     });
     hipLaunchKernelGGL(k_batch_totals, dim3(1), dim3(64), 0, s, cnt, rl.max_depth + 1,
                        b.totals.as<unsigned long long>(), b.batch_idx.as<long long>());

@@ -67,6 +67,11 @@ struct RayQueue {
     T* hit_t;
 };
 
+// Per-path AOV channels: first-hit albedo (3), first-hit normal (N) and the
+// radiance the first hit receives directly (3: background or emitter seen by
+// the primary ray plus the unoccluded first-hit light contributions).
+__host__ __device__ constexpr int aov_channels(int n) { return 6 + n; }
+
 template <typename T>
 struct ShadowQueue {
     T* org;
@@ -84,8 +89,11 @@ struct MeshShadeArgs {
     const int32_t* ntype;
 };
 
-// Per-depth counters: [d*4+0] closest-hit rays, [d*4+1] shadow rays, [d*4+2] probe rays.
-constexpr int CNT_STRIDE = 4;
+// Per-depth counters, CNT_STRIDE u64 per depth: 0 queued rays, 1 shadow rays, 2 probe rays,
+// 4 shadow FP64 decisions, 6 shadow re-walks, 16-23 shadow work index per queue slice (the others
+// unused).  The stride keeps the counters that one stage's waves append to with atomics
+// (next depth's queue, shadow queue) on different cache lines.
+constexpr int CNT_STRIDE = 24;
 
 struct RenderBuffers {
     int n = 0;
@@ -94,7 +102,9 @@ struct RenderBuffers {
     int depth_slots = 0;
     DevBuf qa[8], qb[8], sq[6];
     DevBuf L;               // 3 * cap floats
-    DevBuf aov;             // (3 + n) * cap floats: first-hit albedo and normal per path
+    DevBuf redo;            // int32 shadow-ray indices to walk again with the FP64 re-test inline
+    DevBuf res;             // int32 (shadow ray, pending candidate, pending candidate) for the FP64 decisions
+    DevBuf aov;             // aov_channels(n) * cap floats per path
     DevBuf counters;        // (max_depth + 2) * CNT_STRIDE u64
     DevBuf desc;            // BatchDesc of the batch in flight
     DevBuf batch_list;      // BatchDesc per batch of the pass

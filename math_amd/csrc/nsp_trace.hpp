@@ -109,29 +109,35 @@ __device__ __forceinline__ void load_vec_f(const float* __restrict__ soa, int64_
     for (int q = 0; q < N; ++q) v[q] = soa[(int64_t)(c0 + q) * pad + e];
 }
 
-// Candidate test of one leaf entry.  Returns true for a hit with 0 < t < tlimit.
-// best_idx: the simplex that currently holds tlimit (a simplex listed in
-// several leaves meets itself again and is no tie).
-template <int N, typename T>
-__device__ __forceinline__ bool candidate(const TreeArgs& A, int64_t e, const T* o, const T* d, const double* o64,
-                                          const double* d64, T tlimit, int best_idx, double* t_hit, int* idx,
-                                          bool* near, unsigned* retests)
+// Candidate test of one leaf entry: CAND_HIT for a hit with 0 < t < tlimit,
+// CAND_MISS otherwise.  best_idx: the simplex that currently holds tlimit (a
+// simplex listed in several leaves meets itself again and is no tie).
+// LAY: the FP32 record layout (0 leaf-ordered SoA, 1 simplex-ordered AoS,
+// -1 read from A.layout).  DEFER: instead of the FP64 re-test, return
+// CAND_DEFER; the caller then re-walks the ray with the re-test inline.
+constexpr int CAND_MISS = 0, CAND_HIT = 1, CAND_DEFER = 2;
+
+template <int N, typename T, int LAY = -1, bool DEFER = false>
+__device__ __forceinline__ int candidate(const TreeArgs& A, int64_t e, const T* o, const T* d, const double* o64,
+                                         const double* d64, T tlimit, int best_idx, double* t_hit, int* idx,
+                                         bool* near, unsigned* retests)
 {
     if constexpr (sizeof(T) == 8) {
         int s = A.entries[e];
         double t;
-        if (!simplex_test64<N>(A.rec64 + (int64_t)s * rec64_stride(N), o, d, &t)) return false;
-        if (!(t < tlimit)) return false;
+        if (!simplex_test64<N>(A.rec64 + (int64_t)s * rec64_stride(N), o, d, &t)) return CAND_MISS;
+        if (!(t < tlimit)) return CAND_MISS;
         *t_hit = t;
         *idx = s;
-        return true;
+        return CAND_HIT;
     } else {
         // component c of the candidate's FP32 record: leaf-ordered SoA (layout 0)
         // or the simplex-ordered AoS record read with 16-byte loads (layout 1)
         constexpr int RS = rec32_stride(N);
         float rec[RS];
         int sid = -1;
-        if (A.layout == 1) {
+        const bool aos = LAY < 0 ? A.layout == 1 : LAY == 1;
+        if (aos) {
             sid = A.entries[e];
             const float4* r4 = reinterpret_cast<const float4*>(A.rec32 + (int64_t)sid * RS);
 #pragma unroll
@@ -145,12 +151,12 @@ __device__ __forceinline__ bool candidate(const TreeArgs& A, int64_t e, const T*
         }
         const float* soa = A.soa;
         const int64_t pad = A.pad;
-        auto comp = [&](int c) -> float { return A.layout == 1 ? rec[c] : soa[(int64_t)c * pad + e]; };
+        auto comp = [&](int c) -> float { return aos ? rec[c] : soa[(int64_t)c * pad + e]; };
         float nrm[N];
 #pragma unroll
         for (int q = 0; q < N; ++q) nrm[q] = comp(q);
         float s = dot<N, float>(nrm, d);
-        if (s == 0.0f) return false;
+        if (s == 0.0f) return CAND_MISS;
         float dd = comp(N);
         float no = dot<N, float>(nrm, o);
         float t = (dd - no) / s;
@@ -159,8 +165,8 @@ __device__ __forceinline__ bool candidate(const TreeArgs& A, int64_t e, const T*
         // rounding bound of t; outside it the sign of t and the comparison
         // with the limit are decided in FP32
         float terr = 4.0e-7f * (fabsf(dd) + fabsf(no)) / as;
-        if (t < -terr) return false;
-        if (t > tlimit + terr) return false;
+        if (t < -terr) return CAND_MISS;
+        if (t > tlimit + terr) return CAND_MISS;
         float p[N];
 #pragma unroll
         for (int i = 0; i < N; ++i) p[i] = o[i] + d[i] * t;
@@ -175,29 +181,41 @@ __device__ __forceinline__ bool candidate(const TreeArgs& A, int64_t e, const T*
             in = in && (b > 0.0f) && (b < 1.0f);
             last -= b;
             margin = fminf(margin, b);
-            if (b < -band) return false;  // decided in FP32: outside by more than the band
+            if (b < -band) return CAND_MISS;  // decided in FP32: outside by more than the band
         }
         in = in && (last > 0.0f);
         margin = fminf(margin, last);
         bool retest = fabsf(margin) < band || (in && as < PARALLEL_COS) || t < terr;
         if (!retest) {
-            if (!in || !(t > 0.0f)) return false;
+            if (!in || !(t > 0.0f)) return CAND_MISS;
             if (sid < 0) sid = A.entries[e];
             if (fabsf(t - tlimit) < terr && sid != best_idx) *near = true;  // near-tie with another simplex
-            if (!(t < tlimit)) return false;
+            if (!(t < tlimit)) return CAND_MISS;
             *t_hit = (double)t;
             *idx = sid;
-            return true;
+            return CAND_HIT;
         }
+        if constexpr (DEFER) return CAND_DEFER;
         ++*retests;
         *near = true;
         int sidx = sid >= 0 ? sid : A.entries[e];
+        // without a separate FP64 ray the re-test uses the FP32 ray exactly
+        double ob[N], db[N];
+        if (!o64) {
+#pragma unroll
+            for (int q = 0; q < N; ++q) {
+                ob[q] = o[q];
+                db[q] = d[q];
+            }
+            o64 = ob;
+            d64 = db;
+        }
         double t64;
-        if (!simplex_test64<N>(A.rec64 + (int64_t)sidx * rec64_stride(N), o64, d64, &t64)) return false;
-        if (!((float)t64 < tlimit)) return false;
+        if (!simplex_test64<N>(A.rec64 + (int64_t)sidx * rec64_stride(N), o64, d64, &t64)) return CAND_MISS;
+        if (!((float)t64 < tlimit)) return CAND_MISS;
         *t_hit = t64;
         *idx = sidx;
-        return true;
+        return CAND_HIT;
     }
 }
 
@@ -223,47 +241,86 @@ __device__ __forceinline__ bool slab(const T* blo, const T* bsz, const T* o, con
     return a <= b;
 }
 
-// Nearest hit (ANY=false) or any hit before tmax (ANY=true).
-// stack: this lane's LDS stack, stack[l * TRACE_BLOCK].
-// o64/d64: the same ray in FP64 for the near-boundary re-test (FP32 rays only).
-template <int N, typename T, bool ANY>
-__device__ HitResult traverse(const TreeArgs& A, const T* o, const T* d, T tmax, int* stack, const double* o64,
-                              const double* d64)
-{
-    HitResult res{-1, 0.0, false};
-    T inv[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) inv[i] = (T)1 / d[i];
-    const T* rlo = TreeBox<T>::lo(A);
-    const T* rsz = TreeBox<T>::size(A);
-    const T* rinv = TreeBox<T>::inv(A);
-    T t_in, t_out;
-    if (!slab<N, T>(rlo, rsz, o, inv, d, &t_in, &t_out)) return res;
-    if (t_out < 0) return res;
-    T t_cur = fmax(t_in, (T)0);
-    T t_end = fmin(t_out, tmax);
-    const T adv0 = TreeBox<T>::adv(A);
-    T adv = adv0;
-    const int lmax = A.lmax;
-    const int R = 1 << lmax;
-    double best_t = (double)tmax;
-    int best = -1;
-    int level = -1;
+// The tree walk of one ray, split into start() and step() so that a lane can
+// hold it across the iterations of a persistent kernel.  Nearest hit
+// (ANY=false) or any hit before tmax (ANY=true).  The walk locates the leaf
+// that contains the point t_cur + adv (integer cell coordinates at the finest
+// level; the per-level node stack in LDS is kept down to the deepest common
+// ancestor of the previous leaf), tests the leaf's candidates, and moves t_cur
+// to the leaf box's exit.  It visits leaves in ray order and keeps the global
+// nearest hit; it ends at the root box's exit or once the nearest hit lies
+// before the current leaf.
+//
+// LAY: as for candidate().  DEFER: a candidate within the FP64 re-test band
+// is kept as pending (up to PENDING per walk) and the walk goes on with the
+// FP32 decisions only; the caller decides the pending candidates in FP64
+// afterwards (pending_resolve) against the walk's nearest hit.  The nearest
+// hit is the same as with the re-test inline: a pending candidate would only
+// have lowered the limit later candidates are tested against, and the walk
+// ends no earlier for it.  A walk with more pending candidates ends with
+// deferred set, for a re-walk with the re-test inline.
+constexpr int PENDING = 2;
+
+template <int N, typename T, bool ANY, int LAY = -1, bool DEFER = false>
+struct RayWalk {
+    T o[N], d[N], inv[N];
+    T t_cur, t_end, adv, tmax;
+    double best_t;
+    int best, level, last_leaf;
     int q[N];
-    int last_leaf = -1;
-    unsigned nvis = 0, lvis = 0, ntest = 0, nret = 0;
-    for (int iter = 0; iter < 65536; ++iter) {
+    int pend[PENDING];
+    int npend;
+    unsigned steps;
+    bool near, deferred;
+    unsigned nvis, lvis, ntest, nret;
+
+    // Enters the root box with o and d set; false when the ray misses it.
+    __device__ __forceinline__ bool start(const TreeArgs& A, T tmax_)
+    {
+        tmax = tmax_;
+        best_t = (double)tmax;
+        best = -1;
+        level = -1;
+        last_leaf = -1;
+        steps = 0;
+        near = false;
+        deferred = false;
+        npend = 0;
+        nvis = lvis = ntest = nret = 0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) inv[i] = (T)1 / d[i];
+        T t_in, t_out;
+        if (!slab<N, T>(TreeBox<T>::lo(A), TreeBox<T>::size(A), o, inv, d, &t_in, &t_out)) return false;
+        if (t_out < 0) return false;
+        t_cur = fmax(t_in, (T)0);
+        t_end = fmin(t_out, tmax);
+        adv = TreeBox<T>::adv(A);
+        return true;
+    }
+
+    // One leaf of the walk; true when the walk has ended.
+    // stack: this lane's LDS stack, stack[l * TRACE_BLOCK].
+    // o64/d64: the ray in FP64 for the near-boundary re-test, or null to use
+    // the FP32 ray converted exactly.
+    __device__ __forceinline__ bool step(const TreeArgs& A, int* stack, const double* o64, const double* d64)
+    {
+        const T* rlo = TreeBox<T>::lo(A);
+        const T* rsz = TreeBox<T>::size(A);
+        const T* rinv = TreeBox<T>::inv(A);
+        const T adv0 = TreeBox<T>::adv(A);
+        const int lmax = A.lmax;
+        const int R = 1 << lmax;
+        if (++steps > 65536) return true;
         // the walk ends at the root box's exit: with t_cur == t_end the next
         // point lies outside the root and clamps back into the last cell
-        if (t_cur >= t_end) break;
-        if (best >= 0 && (T)best_t < t_cur - adv0 * 4) break;
+        if (t_cur >= t_end) return true;
+        if (best >= 0 && (T)best_t < t_cur - adv0 * 4) return true;
         T tp = t_cur + adv;
         int Q[N];
 #pragma unroll
         for (int i = 0; i < N; ++i) {
             T x = floor((o[i] + d[i] * tp - rlo[i]) * rinv[i]);
-            int qi = (int)fmin(fmax(x, (T)0), (T)(R - 1));
-            Q[i] = qi;
+            Q[i] = (int)fmin(fmax(x, (T)0), (T)(R - 1));
         }
         int node;
         if (level < 0) {
@@ -311,20 +368,37 @@ __device__ HitResult traverse(const TreeArgs& A, const T* o, const T* d, T tmax,
                 int si;
                 bool nr = false;
                 T lim = ANY ? tmax : (T)best_t;
-                if (candidate<N, T>(A, e, o, d, o64, d64, lim, best, &th, &si, &nr, &nret)) {
+                const int c = candidate<N, T, LAY, DEFER>(A, e, o, d, o64, d64, lim, best, &th, &si, &nr, &nret);
+                near = near || nr;
+                if (DEFER && c == CAND_DEFER) {
+                    near = true;
+                    const int sid = A.entries[e];
+                    bool known = false;
+#pragma unroll
+                    for (int k = 0; k < PENDING; ++k) known = known || (k < npend && pend[k] == sid);
+                    if (!known) {
+                        if (npend == PENDING) {
+                            deferred = true;
+                            return true;
+                        }
+#pragma unroll
+                        for (int k = 0; k < PENDING; ++k)
+                            if (k == npend) pend[k] = sid;
+                        ++npend;
+                    }
+                    continue;
+                }
+                if (c == CAND_HIT) {
                     if (ANY) {
-                        res.idx = si;
-                        res.t = th;
-                        res.near = res.near || nr;
+                        best_t = th;
                         best = si;
-                        goto done;
+                        return true;
                     }
                     if (th < best_t || (th == best_t && si < best)) {
                         best_t = th;
                         best = si;
                     }
                 }
-                res.near = res.near || nr;
             }
         }
         last_leaf = node;
@@ -333,21 +407,79 @@ __device__ HitResult traverse(const TreeArgs& A, const T* o, const T* d, T tmax,
             adv = adv0;
         } else {
             adv = adv * 2;  // no progress: the located cell ended before t_cur (rounding)
-            if (t_cur + adv > t_end) break;
+            if (t_cur + adv > t_end) return true;
+        }
+        return false;
+    }
+
+    __device__ __forceinline__ HitResult result() const
+    {
+        HitResult r{-1, 0.0, near};
+        if (best >= 0) {
+            r.idx = best;
+            r.t = best_t;
+        }
+        return r;
+    }
+
+    __device__ __forceinline__ void flush_counters(const TreeArgs& A) const
+    {
+        if (A.counters) {
+            atomicAdd(&A.counters[0], (unsigned long long)nvis);
+            atomicAdd(&A.counters[1], (unsigned long long)lvis);
+            atomicAdd(&A.counters[2], (unsigned long long)ntest);
+            atomicAdd(&A.counters[3], (unsigned long long)nret);
         }
     }
-    if (!ANY && best >= 0) {
-        res.idx = best;
-        res.t = best_t;
+};
+
+// FP64 decision of the pending candidates p0, p1 (-1: none) of an FP32 walk
+// whose nearest hit is (best, best_t): the inline re-test's acceptance and
+// tie rules.  ANY: returns true as soon as one is a hit before tmax.
+template <int N, bool ANY>
+__device__ __forceinline__ bool pending_resolve(const TreeArgs& A, const float* o, const float* d, int p0, int p1,
+                                                float tmax, int* best, double* best_t)
+{
+    double o64[N], d64[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        o64[i] = o[i];
+        d64[i] = d[i];
     }
-done:
-    if (A.counters) {
-        atomicAdd(&A.counters[0], (unsigned long long)nvis);
-        atomicAdd(&A.counters[1], (unsigned long long)lvis);
-        atomicAdd(&A.counters[2], (unsigned long long)ntest);
-        atomicAdd(&A.counters[3], (unsigned long long)nret);
+    bool any = false;
+    for (int k = 0; k < PENDING; ++k) {
+        const int sid = k == 0 ? p0 : p1;
+        if (sid < 0) continue;
+        double t64;
+        if (!simplex_test64<N>(A.rec64 + (int64_t)sid * rec64_stride(N), o64, d64, &t64)) continue;
+        const float lim = ANY ? tmax : (float)*best_t;
+        if (!((float)t64 < lim)) continue;
+        if (ANY) return true;
+        if (t64 < *best_t || (t64 == *best_t && sid < *best)) {
+            *best_t = t64;
+            *best = sid;
+            any = true;
+        }
     }
-    return res;
+    return any;
+}
+
+// The whole walk of one ray in one call (first-hit queries and FP64 paths).
+template <int N, typename T, bool ANY>
+__device__ HitResult traverse(const TreeArgs& A, const T* o, const T* d, T tmax, int* stack, const double* o64,
+                              const double* d64)
+{
+    RayWalk<N, T, ANY> w;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        w.o[i] = o[i];
+        w.d[i] = d[i];
+    }
+    if (!w.start(A, tmax)) return HitResult{-1, 0.0, false};
+    while (!w.step(A, stack, o64, d64)) {
+    }
+    w.flush_counters(A);
+    return w.result();
 }
 
 }  // namespace nsp

@@ -4,12 +4,17 @@ the time-to-RMSE comparison of denoised against plain FP32 rendering.
   variants   n=4 tesseract-boundary meshes of the C1 generator with radial noise
              sigma ~ U[0, 0.05] and a point light at distance 4 in a direction
              uniform on S^3 (seeded)
-  inputs     1-spp and 4-spp (8 samples) radiance, first-hit albedo and normal
-             (13 channels per voxel of the 3D screen)
+  inputs     16 channels per voxel of the 3D screen: the 1-spp radiance and
+             the 4-spp (8-sample) radiance, and from the 4-spp pass the
+             first-hit albedo, normal and direct light (the radiance the first
+             hit receives from the light and the background it sees).
+             Rendering is progressive: at level j the slots hold j one-sample
+             passes and j eight-sample passes, so level 1 is the plain 1-spp +
+             4-spp input and the network can be applied at any sample budget
   target     1024-spp (1331 samples) FP32 render
   network    8 layers 3x3x3, 32 channels, BF16/FP32-accumulate, residual on the
              4-spp radiance, trained on 32^3 patches, batch 16, Adam
-  judge      RMSE over pixels x 3 linear-RGB channels against a 4096-spp FP64
+  judge      RMSE over pixels x 3 linear-RGB channels against a 4096-sample FP64
              render of the same scene; the plain render is the fallback
 """
 import time
@@ -18,8 +23,9 @@ import numpy as np
 
 from . import configs, native
 
-IN_CHANNELS = 13
+IN_CHANNELS = 16
 RESIDUAL = 3  # the 4-spp radiance channels
+LEVELS = (1, 2, 4, 8, 16)  # progressive levels stored for training
 
 
 def variant(seed, screen=128, window=None, g=24):
@@ -48,26 +54,55 @@ def to_volume(arr, screen):
     return arr.reshape(screen[2], screen[1], screen[0], -1)
 
 
-def render_inputs(ctx, mesh, cfg, seed, depth=5):
-    """13-channel denoiser input volume and the render seconds spent."""
+def _paint_timed(scene, spp, depth, seed, **kw):
+    """One paint call; returns its wall seconds (nsp_paint synchronises its stream before returning)."""
     t0 = time.perf_counter()
-    s1 = native.Scene(ctx, mesh, cfg)
-    s1.paint(1, depth, seed=seed, aov=True)
-    r1, c1 = s1.image()
-    alb, nrm = s1.aov()
-    s4 = native.Scene(ctx, mesh, cfg)
-    s4.paint(4, depth, seed=seed + 7919)
-    r4, c4 = s4.image()
-    secs = time.perf_counter() - t0
-    k1 = np.maximum(c1, 1)[:, None]
-    x = np.concatenate([r1 / k1, r4 / np.maximum(c4, 1)[:, None], alb / k1, nrm / k1], axis=1)
-    return to_volume(x.astype(np.float32), cfg["screen"]), secs
+    scene.paint(spp, depth, seed=seed, **kw)
+    return time.perf_counter() - t0
 
 
-def render_mean(ctx, mesh, cfg, spp, seed, depth=5, precision=native.PRECISION_FP32, passes=1):
+class ProgressiveInputs:
+    """The denoiser input of one scene, rendered progressively: each advance()
+    adds one 1-spp pass and one 4-spp (8-sample) pass with its AOVs."""
+
+    def __init__(self, ctx, mesh, cfg, seed, depth=5):
+        self.s1 = native.Scene(ctx, mesh, cfg)
+        self.s4 = native.Scene(ctx, mesh, cfg)
+        self.screen = cfg["screen"]
+        self.seed, self.depth = seed, depth
+        self.level = 0
+        self.secs = 0.0  # paint seconds so far
+
+    def advance(self):
+        self.secs += _paint_timed(self.s1, 1, self.depth, self.seed)
+        self.secs += _paint_timed(self.s4, 4, self.depth, self.seed + 7919, aov=True)
+        self.level += 1
+
+    def volume(self):
+        r1, c1 = self.s1.image()
+        r4, c4 = self.s4.image()
+        alb, nrm, dl = self.s4.aov()
+        k1 = np.maximum(c1, 1)[:, None]
+        k4 = np.maximum(c4, 1)[:, None]
+        x = np.concatenate([r1 / k1, r4 / k4, alb / k4, nrm / k4, dl / k4], axis=1)
+        return to_volume(x.astype(np.float32), self.screen)
+
+
+def render_inputs(ctx, mesh, cfg, seed, depth=5):
+    """Level-1 (1-spp + 4-spp) denoiser input volume and the paint seconds spent."""
+    p = ProgressiveInputs(ctx, mesh, cfg, seed, depth)
+    p.advance()
+    return p.volume(), p.secs
+
+
+def render_mean(ctx, mesh, cfg, spp, seed, depth=5, precision=native.PRECISION_FP32, passes=1, log=None):
+    """Mean image of `passes` progressive passes of `spp` samples each."""
     sc = native.Scene(ctx, mesh, cfg)
     t0 = time.perf_counter()
-    sc.paint(spp, depth, seed=seed, precision=precision, passes=passes)
+    for p in range(passes):
+        sc.paint(spp, depth, seed=seed, precision=precision)
+        if log:
+            log(f"  pass {p + 1}/{passes}: {time.perf_counter() - t0:.1f} s")
     secs = time.perf_counter() - t0
     s, c = sc.image()
     return to_volume((s / np.maximum(c, 1)[:, None]).astype(np.float32), cfg["screen"]), secs
@@ -77,48 +112,76 @@ def rmse(a, b):
     return float(np.sqrt(np.mean((np.asarray(a, np.float64) - np.asarray(b, np.float64)) ** 2)))
 
 
-def training_set(ctx, variants, screen=128, window=64, target_spp=1024, depth=5, log=print):
+def training_set(ctx, variants, screen=128, window=64, target_spp=1024, depth=5, levels=LEVELS, log=print):
+    """Inputs at each progressive level in `levels` and the target of every
+    variant: X [len(variants) * len(levels), w, w, w, IN_CHANNELS], Y [.., 3]."""
     xs, ys = [], []
     for i, seed in enumerate(variants):
         rng = np.random.default_rng(seed)
         org = [int(rng.integers(0, screen - window + 1)) for _ in range(3)]
         v, f, cfg = variant(seed, screen, (org, window))
         mesh = ctx.mesh(v, f)
-        x, _ = render_inputs(ctx, mesh, cfg, seed=10 * seed + 1, depth=depth)
         y, secs = render_mean(ctx, mesh, cfg, target_spp, seed=10 * seed + 2, depth=depth)
-        xs.append(x)
-        ys.append(y)
-        log(f"training volume {i}: seed {seed}, window {org}, target {secs:.2f} s")
+        p = ProgressiveInputs(ctx, mesh, cfg, seed=10 * seed + 1, depth=depth)
+        for lv in range(1, max(levels) + 1):
+            p.advance()
+            if lv in levels:
+                xs.append(p.volume())
+                ys.append(y)
+        log(f"training volume {i}: seed {seed}, window {org}, target {secs:.2f} s, inputs {p.secs:.2f} s")
         mesh.close()
     return np.stack(xs), np.stack(ys)
 
 
-def time_to_rmse(ctx, mesh, cfg, ref, denoiser, depth=5, threshold=0.01, max_passes=64, seed=77):
-    """Plain: progressive 4-spp (8-sample) passes until RMSE <= threshold.
-    Denoised: 1-spp + 4-spp inputs -> network.  Returns a dict of times and errors."""
+def _mean_volume(scene, screen):
+    s, c = scene.image()
+    return to_volume(s / np.maximum(c, 1)[:, None], screen), c
+
+
+def time_to_rmse(ctx, mesh, cfg, ref, denoiser, depth=5, threshold=0.01, max_passes=256, seed=77):
+    """Time for each method to reach RMSE <= threshold against `ref`.  Only GPU
+    work is timed (paint calls and the network forward); image read-back and
+    the RMSE evaluation between passes are not.
+
+      plain     progressive 4-spp (8-sample) FP32 passes
+      denoised  progressive levels of ProgressiveInputs (1 + 8 samples per
+                level); after each level the network maps the input to an
+                image, level 1 being the standard 1-spp + 4-spp input.  The
+                time at a level is its passes plus one inference.
+    """
+    screen = cfg["screen"]
     out = {}
     sc = native.Scene(ctx, mesh, cfg)
-    t_plain = None
-    curve = []
-    t0 = time.perf_counter()
+    t, t_plain, curve = 0.0, None, []
     for p in range(max_passes):
-        sc.paint(4, depth, seed=seed)
-        s, c = sc.image()
-        img = to_volume(s / np.maximum(c, 1)[:, None], cfg["screen"])
+        t += _paint_timed(sc, 4, depth, seed)
+        img, c = _mean_volume(sc, screen)
         e = rmse(img, ref)
-        el = time.perf_counter() - t0
-        curve.append((int(c[0]), el, e))
-        if e <= threshold and t_plain is None:
-            t_plain = el
+        curve.append((int(c[0]), t, e))
+        if e <= threshold:
+            t_plain = t
             break
     out["plain_curve"] = curve
     out["plain_time_s"] = t_plain
-    x, t_in = render_inputs(ctx, mesh, cfg, seed=seed + 1, depth=depth)
-    t1 = time.perf_counter()
-    y = denoiser.forward(x[None])[0]
-    t_net = time.perf_counter() - t1
-    out["denoised_rmse"] = rmse(y, ref)
-    out["input4_rmse"] = rmse(x[..., 3:6], ref)
-    out["denoised_time_s"] = t_in + t_net
-    out["denoiser_inference_s"] = t_net
+    p = ProgressiveInputs(ctx, mesh, cfg, seed + 1, depth)
+    t_den, dcurve = None, []
+    for k in range(max_passes):
+        p.advance()
+        x = p.volume()
+        t1 = time.perf_counter()
+        y = denoiser.forward(x[None])[0]
+        dt = time.perf_counter() - t1
+        e = rmse(y, ref)
+        if k == 0:
+            out["denoised_rmse"] = e
+            out["input4_rmse"] = rmse(x[..., 3:6], ref)
+            out["denoiser_inference_s"] = dt
+        # a render stopped at this level costs its passes and one inference
+        samples = 9 * p.level
+        dcurve.append((samples, p.secs + dt, e))
+        if e <= threshold:
+            t_den = p.secs + dt
+            break
+    out["denoised_curve"] = dcurve
+    out["denoised_time_s"] = t_den
     return out

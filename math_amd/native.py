@@ -133,7 +133,7 @@ def lib():
         L.nsp_statistics.argtypes = [P, i64p, i64p, i64p, i64p, dp]
         L.nsp_image_get.argtypes = [P, fp, ip]
         L.nsp_image_set.argtypes = [P, fp, ip]
-        L.nsp_image_aov.argtypes = [P, fp, fp]
+        L.nsp_image_aov.argtypes = [P, fp, fp, fp]
         L.nsp_scene_pixel_count.argtypes = [P]
         L.nsp_scene_pixel_count.restype = C.c_int64
         L.nsp_last_pass_stats.argtypes = [P, C.POINTER(StageStats)]
@@ -361,10 +361,13 @@ class Scene:
         return s, c
 
     def aov(self):
+        """Per-pixel sums of first-hit albedo, normal and direct light (see nsp_image_aov)."""
         a = np.zeros((self.pixels, 3), np.float32)
         nrm = np.zeros((self.pixels, self.n), np.float32)
-        self.ctx.check(self.ctx._L.nsp_image_aov(self.h, _ptr(a, C.c_float), _ptr(nrm, C.c_float)))
-        return a, nrm
+        dl = np.zeros((self.pixels, 3), np.float32)
+        self.ctx.check(self.ctx._L.nsp_image_aov(self.h, _ptr(a, C.c_float), _ptr(nrm, C.c_float),
+                                                 _ptr(dl, C.c_float)))
+        return a, nrm, dl
 
     def set_image(self, s, c):
         s = np.ascontiguousarray(s, np.float32)

@@ -112,3 +112,25 @@ def test_denoiser_training_reduces_loss(ctx):
     print("losses", losses[:3], losses[-3:])
     assert np.all(np.isfinite(losses))
     assert losses[-10:].mean() < 0.5 * losses[:5].mean()
+
+
+@pytest.mark.gpu
+def test_trained_denoiser_beats_its_noisy_input_on_heldout_render(ctx):
+    # what the learned part gains: trained on the tracer's own low/high-spp
+    # renders, the network's output on a held-out scene is closer to a
+    # high-spp render than the 4-spp image it starts from
+    from math_amd import denoise_eval as DE
+    from math_amd import native
+    X, Y = DE.training_set(ctx, list(range(100, 108)), window=32, target_spp=256, levels=(1, 2), log=print)
+    d = native.Denoiser(ctx, DE.IN_CHANNELS, DE.RESIDUAL, seed=1)
+    losses = d.train(X, Y, patch=16, batch=16, steps=3000, lr=2e-4, seed=3)
+    print("loss", losses[:5].mean(), "->", losses[-25:].mean(), flush=True)
+    v, f, cfg = DE.variant(200, 128, ([48, 48, 48], 32))
+    mesh = ctx.mesh(v, f)
+    ref, _ = DE.render_mean(ctx, mesh, cfg, 1024, seed=2001)
+    x, _ = DE.render_inputs(ctx, mesh, cfg, seed=2002)
+    y = d.forward(x[None])[0]
+    e_in, e_out = DE.rmse(x[..., 3:6], ref), DE.rmse(y, ref)
+    print(f"held-out rmse: 4-spp input {e_in:.4f}, denoised {e_out:.4f}", flush=True)
+    assert np.all(np.isfinite(y))
+    assert e_out < 0.9 * e_in

@@ -170,6 +170,31 @@ def test_render_fp32_matches_checker(ctx, n):
     assert rmse < 5e-3
 
 
+@pytest.mark.parametrize("n,precision", [(3, native.PRECISION_FP64), (4, native.PRECISION_FP64),
+                                         (4, native.PRECISION_FP32)])
+def test_render_smooth_normals_matches_checker(ctx, n, precision):
+    # vertex normals (radial, so they disagree with the noisy facets' normals) with smooth_normal:
+    # MeshHyperplaneSimplex normal classification, shading_normal and the back-face probes of
+    # direct_diffuse_lighting (painter.cpp:238-284), whose rays are counted as the reference counts them
+    g = {3: 8, 4: 4}[n]
+    edge = {3: 24, 4: 10}[n]
+    v, f = native.meshgen_cube_sphere(n, g, True, 0.05, 0.0, 3)
+    vn = v / np.linalg.norm(v, axis=1, keepdims=True)
+    cfg = configs.scene_dict(v, f, edge, tuple([2.0] * n))
+    gm = ctx.mesh(v, f, normals=vn, facet_normals=f)
+    sc = native.Scene(ctx, gm, cfg)
+    sc.paint(4, 3, precision=precision, seed=8, smooth=True)
+    img, cnt = sc.image()
+    st = sc.statistics()
+    om = O.Mesh(v, f, normals=vn, facet_normals=f)
+    om.build_tree()
+    ref, rays = O.render(om, O.make_scene(cfg, 4, 3, 8, smooth=True), threads=8)
+    rmse = float(np.sqrt(np.mean((img / cnt[:, None] - ref / cnt[:, None]) ** 2)))
+    print(f"smooth n={n} precision={precision} rmse {rmse:.3e} rays gpu {st['rays']} cpu {rays}", flush=True)
+    assert rmse < (1e-4 if precision == native.PRECISION_FP64 else 5e-3)
+    assert abs(st["rays"] - rays) <= 0.002 * rays
+
+
 def test_record_layouts_identical(ctx):
     # both FP32 record copies hold the same values: hits, distances and images are bit-identical
     v, f = sphere_mesh(4, 4)
@@ -189,6 +214,27 @@ def test_record_layouts_identical(ctx):
         assert np.array_equal(a, b)
     with pytest.raises(native.NspError):
         gm.record_layout = 7
+
+
+def test_direct_light_aov(ctx):
+    # without bounces the whole radiance of a path is direct: the AOV equals the image bit for bit;
+    # with bounces the direct part is what a depth-0 render of the same samples collects
+    v, f, cfg, _, _ = small_scene(4, 4, 10)
+    gm = ctx.mesh(v, f)
+    sc = native.Scene(ctx, gm, cfg)
+    sc.paint(8, 0, seed=6, aov=True)
+    img, cnt = sc.image()
+    alb, nrm, dl = sc.aov()
+    assert np.array_equal(img, dl)
+    hit = alb[:, 0] > 0.5 * cnt  # white object, diffuse 0.8, against the darker background
+    assert hit.any() and np.all(np.linalg.norm(nrm[hit], axis=1) <= cnt[hit] * (1 + 1e-5))
+    sc2 = native.Scene(ctx, gm, cfg)
+    sc2.paint(8, 4, seed=6, aov=True)
+    img2, _ = sc2.image()
+    _, _, dl2 = sc2.aov()
+    assert np.array_equal(dl2, dl)
+    assert np.all(img2 >= dl2 - 1e-5 * np.abs(dl2))
+    assert img2.sum() > dl2.sum()
 
 
 def test_render_deterministic_and_batch_invariant(ctx):
