@@ -210,6 +210,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-paths", type=int, default=0)
     ap.add_argument("--graph", action="store_true")
+    ap.add_argument("--primary", choices=["auto", "walk"], default="auto",
+                    help="primary visibility: rasterised (auto) or the tree walk")
     ap.add_argument("--layout", type=int, default=-1,
                     help="FP32 record layout: 0 leaf-ordered SoA, 1 simplex-ordered AoS (default: the library's)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -260,15 +262,16 @@ def main():
         mesh.record_layout = args.layout
     log(f"record layout {mesh.record_layout}")
     scene = native.Scene(ctx, mesh, cfg)
+    primary = native.PRIMARY_AUTO if args.primary == "auto" else native.PRIMARY_WALK
     for w in range(args.warmup):
-        scene.paint(spp, depth, seed=1000 + w, batch_paths=args.batch_paths, use_graph=args.graph)
+        scene.paint(spp, depth, seed=1000 + w, batch_paths=args.batch_paths, use_graph=args.graph, primary=primary)
         log(f"warmup {w}: {scene.stage_stats()['ms_total']:.1f} ms")
     r0 = scene.statistics()["rays"]
     closest_ms = 0.0
     closest_launches = 0
     t0 = time.perf_counter()
     for k in range(args.steps):
-        scene.paint(spp, depth, seed=k, batch_paths=args.batch_paths, use_graph=args.graph)
+        scene.paint(spp, depth, seed=k, batch_paths=args.batch_paths, use_graph=args.graph, primary=primary)
         st = scene.stage_stats()
         closest_ms += st["ms_closest"]
         closest_launches += st["launches_closest"]
@@ -279,7 +282,7 @@ def main():
     mrays = rays / elapsed / 1e6
     # traversal counters in a separate pass (the atomics would perturb the timing)
     scene.set_counters(True)
-    scene.paint(spp, depth, seed=0, batch_paths=args.batch_paths)
+    scene.paint(spp, depth, seed=0, batch_paths=args.batch_paths, primary=primary)
     cst = scene.stage_stats()
     scene.set_counters(False)
     bytes_pass = closest_bytes(cst, n)

@@ -173,7 +173,16 @@ typedef struct {
     int64_t batch_paths;   /* paths per wavefront batch; 0 = library default/autotuned */
     int use_graph;         /* capture each batch's per-bounce launches in a hipGraph */
     int aov;               /* also accumulate first-hit albedo, normal and direct light per pixel (denoiser inputs) */
+    int primary;           /* NSP_PRIMARY_*: how FP32 renders find the primary rays' hits */
 } nsp_paint_params;
+
+/* Primary visibility of FP32 renders.  Both give the nearest accepted hit of
+ * every primary ray (the same FP32 test with the FP64 re-test inline):
+ *   NSP_PRIMARY_AUTO    rasterise the simplices into the parallel projection
+ *                       (every primary ray has the projector's direction)
+ *   NSP_PRIMARY_WALK    walk the tree per ray, as for the other bounces */
+#define NSP_PRIMARY_AUTO 0
+#define NSP_PRIMARY_WALK 1
 
 /* Delivery of results: PainterNotifier::painter_pixel_after(pixel, color)
  * (src/painter/painter.h:26-38) in batches of consecutive pixels in

@@ -30,6 +30,7 @@ SOURCES = {
     "nsp_render.hip": [],
     "nsp_denoise.hip": [],
     "nsp_batch.hip": [],
+    "nsp_raster.hip": [],
     "nsp_api.cpp": ["-x", "hip"],
     "nsp_meshgen.cpp": ["-x", "hip"],
 }

@@ -14,6 +14,7 @@
 #include "nsp_build.hpp"
 #include "nsp_internal.hpp"
 #include "nsp_batch.hpp"
+#include "nsp_raster.hpp"
 #include "nsp_render.hpp"
 
 namespace nsp {
@@ -103,6 +104,7 @@ struct nsp_scene {
     int64_t pixels = 0;
     DevBuf pix_sum, pix_cnt;
     DevBuf pix_aov;  // aov_channels(n) * pixels, allocated on the first pass with aov
+    RasterPrep raster;  // rasterised primary visibility (boxes and work units), built on first use
     DevBuf totals;  // 3 u64
     StageMarks marks;
     bool counters = false;
@@ -574,6 +576,12 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
         batch_pixels = std::min<int64_t>(batch_pixels, scene->pixels);
         const int64_t cap = batch_pixels * S;
         alloc_render_buffers(ctx->work, n, p.precision, cap, p.max_depth);
+        if (p.primary != NSP_PRIMARY_AUTO && p.primary != NSP_PRIMARY_WALK) raise(NSP_E_INVALID, "unknown primary mode");
+        const bool raster = p.precision == NSP_PRECISION_FP32 && p.primary == NSP_PRIMARY_AUTO;
+        if (raster) {
+            raster_prepare(scene->raster, scene->mesh->m, kd, S, s);
+            ctx->work.keys.alloc(sizeof(unsigned long long) * cap);
+        }
         if (p.aov) {
             ctx->work.aov.alloc(sizeof(float) * aov_channels(n) * cap);
             if (!scene->pix_aov.p) {
@@ -607,6 +615,7 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
             rl.max_depth = p.max_depth;
             rl.counters = scene->counters;
             rl.marks = p.use_graph ? nullptr : &scene->marks;
+            rl.raster = raster ? &scene->raster : nullptr;
             hipGraph_t graph = nullptr;
             hipGraphExec_t exec = nullptr;
             if (p.use_graph) {

@@ -17,6 +17,7 @@
 #include <mutex>
 
 #include "nsp_device.hpp"
+#include "nsp_raster.hpp"
 #include "nsp_render.hpp"
 #include "nsp_stages.hpp"
 #include "nsp_trace.hpp"
@@ -204,7 +205,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_raygen(SceneK<T> sc, const Batc
     }
 }
 
-template <int N, typename T>
+template <int N, typename T, bool SMOOTH>
 __global__ void __launch_bounds__(TRACE_BLOCK)
     k_shade(SceneK<T> sc, TreeArgs A, MeshShadeArgs M, const BatchDesc* __restrict__ desc, int depth,
             RayQueue<T> qin, RayQueue<T> qout, ShadowQueue<T> sq, int64_t cap, unsigned long long* __restrict__ cnt,
@@ -240,7 +241,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK)
             }
             const int64_t gp = pixel0 * sc.S + path;
             float av[3 + N];
-            shade_one<N, T>(sc, A, M, depth, gp / sc.S, (int)(gp % sc.S), o, d, qin.hit[i], qin.hit_t[i], beta,
+            shade_one<N, T, SMOOTH>(sc, A, M, depth, gp / sc.S, (int)(gp % sc.S), o, d, qin.hit[i], qin.hit_t[i], beta,
                             qin.level[i], st, probe_count, (aov && depth == 0) ? av : nullptr, out);
 #pragma unroll
             for (int c = 0; c < 3; ++c) L[c * cap + path] += out.add[c];
@@ -593,6 +594,7 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
     // FP32 walks read the record layout chosen at compile time; FP64 walks read the FP64 records
     const bool aos = sizeof(T) == 4 && A.layout == 1;
     auto kc = aos ? k_closest<N, T, 1> : k_closest<N, T, 0>;
+    auto kshade = sc.smooth ? k_shade<N, T, true> : k_shade<N, T, false>;
     auto ks = aos ? k_shadow<N, T, 1> : k_shadow<N, T, 0>;
     const unsigned gs = aos ? persistent_grid(k_shadow<N, T, 1>) : persistent_grid(k_shadow<N, T, 0>);
     const unsigned gx = 8 * 32;  // FP64 decisions and re-walks of shadow rays: a few per cent or fewer
@@ -619,10 +621,18 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
         RayQueue<T>& qin = (depth & 1) ? qb : qa;
         RayQueue<T>& qout = (depth & 1) ? qa : qb;
         unsigned long long* dc = cnt + depth * CNT_STRIDE;
-        timed(1, [&] { hipLaunchKernelGGL(kc, dim3(g), dim3(TRACE_BLOCK), 0, s, A, qin, cap, dc); });
+        timed(1, [&] {
+            if constexpr (sizeof(T) == 4) {
+                if (depth == 0 && rl.raster) {
+                    raster_launch(*rl.raster, *rl.mesh, A, desc, qin, cap, b.keys.as<unsigned long long>(), s);
+                    return;
+                }
+            }
+            hipLaunchKernelGGL(kc, dim3(g), dim3(TRACE_BLOCK), 0, s, A, qin, cap, dc);
+        });
         timed(2, [&] {
-            hipLaunchKernelGGL((k_shade<N, T>), dim3(g), dim3(TRACE_BLOCK), 0, s, sc, A, M, desc, depth, qin, qout,
-                               sq, cap, cnt, L, aov);
+            hipLaunchKernelGGL(kshade, dim3(g), dim3(TRACE_BLOCK), 0, s, sc, A, M, desc, depth, qin, qout, sq, cap,
+                               cnt, L, aov);
         });
         timed(3, [&] {
             float* direct = (aov && depth == 0) ? aov + (int64_t)(3 + N) * cap : nullptr;

@@ -104,6 +104,7 @@ struct RenderBuffers {
     DevBuf L;               // 3 * cap floats
     DevBuf redo;            // int32 shadow-ray indices to walk again with the FP64 re-test inline
     DevBuf res;             // int32 (shadow ray, pending candidate, pending candidate) for the FP64 decisions
+    DevBuf keys;            // u64 per path: (t, simplex) of the rasterised primary hits
     DevBuf aov;             // aov_channels(n) * cap floats per path
     DevBuf counters;        // (max_depth + 2) * CNT_STRIDE u64
     DevBuf desc;            // BatchDesc of the batch in flight
@@ -135,6 +136,8 @@ struct StageMarks {
 };
 
 // One wavefront batch of the render (all kernels enqueued on s).
+struct RasterPrep;
+
 struct RenderLaunch {
     const MeshDev* mesh;
     const void* scene;  // SceneK<T>*
@@ -146,6 +149,7 @@ struct RenderLaunch {
     int max_depth;
     bool counters;
     StageMarks* marks;  // may be null
+    const RasterPrep* raster = nullptr;  // FP32 only: rasterised depth-0 hits (nsp_raster.hpp), or null to walk
 };
 void render_batch(const RenderLaunch& rl, hipStream_t s);
 // Clears the pass totals, the batch index and (when counting) the traversal counters.

@@ -108,7 +108,9 @@ struct ShadeOut {
 // set-up and diffuse_lighting's bounce (src/painter/painter.cpp:197-397), for
 // the ray (o, d) whose nearest hit is (idx, t).  aov (3 + N floats) receives
 // the first-hit albedo and normal when non-null.
-template <int N, typename T>
+// SMOOTH: false compiles out the smooth-normal probes (scenes painted without
+// smooth normals never take them).
+template <int N, typename T, bool SMOOTH = true>
 __device__ void shade_one(const SceneK<T>& sc, const TreeArgs& A, const MeshShadeArgs& M, int depth, int64_t pixel,
                           int sample, const T* o, const T* d, int idx, T t, const float* beta, float level, int* st,
                           unsigned long long* probe_count, float* aov, ShadeOut<N, T>& out)
@@ -195,7 +197,7 @@ __device__ void shade_one(const SceneK<T>& sc, const TreeArgs& A, const MeshShad
 #pragma unroll
             for (int k = 0; k < N; ++k) o1[k] = p[k] + ld[k] * sc.ray_offset;
             T tmaxs = dist;
-            if (sc.smooth && dot<N, T>(ld, gn) < (T)0) {
+            if (SMOOTH && sc.smooth && dot<N, T>(ld, gn) < (T)0) {
                 // smooth-normal back face: forward and reverse probes (painter.cpp:238-284)
                 atomicAdd(probe_count, 1ull);
                 double o64[N], d64[N];

@@ -67,7 +67,7 @@ struct TreeBox<double> {
 // HyperplaneSimplexGeometry::intersect + plane_intersect
 // (src/painter/space/hyperplane_geometry.h:40-52, :181-204), no fused ops.
 template <int N>
-__device__ __attribute__((noinline)) bool simplex_test64(const double* __restrict__ r, const double* o,
+__device__ __forceinline__ bool simplex_test64(const double* __restrict__ r, const double* o,
                                                          const double* d, double* t_out)
 {
 #pragma clang fp contract(off)
@@ -109,14 +109,90 @@ __device__ __forceinline__ void load_vec_f(const float* __restrict__ soa, int64_
     for (int q = 0; q < N; ++q) v[q] = soa[(int64_t)(c0 + q) * pad + e];
 }
 
+// Results of a candidate test.
+constexpr int CAND_MISS = 0, CAND_HIT = 1, CAND_DEFER = 2;
+
+// The FP32 test of one simplex against the ray (o, d): comp(c) returns
+// component c of the simplex's FP32 record (normal[N], d, then the N-1
+// barycentric planes), sid() its index.  Shared by the tree walk and the
+// primary-visibility rasteriser, so both make the same decisions.
+template <int N, bool DEFER, class Comp, class Sid>
+__device__ __forceinline__ int fp32_test(const TreeArgs& A, const Comp& comp, const Sid& get_sid, const float* o,
+                                         const float* d, const double* o64, const double* d64, float tlimit,
+                                         int best_idx, double* t_hit, int* idx, bool* near, unsigned* retests)
+{
+    float nrm[N];
+#pragma unroll
+    for (int q = 0; q < N; ++q) nrm[q] = comp(q);
+    float s = dot<N, float>(nrm, d);
+    if (s == 0.0f) return CAND_MISS;
+    float dd = comp(N);
+    float no = dot<N, float>(nrm, o);
+    float t = (dd - no) / s;
+    float as = fabsf(s);
+    float band = RETEST_BAND * (1.0f + 1.0f / as);
+    // rounding bound of t; outside it the sign of t and the comparison
+    // with the limit are decided in FP32
+    float terr = 4.0e-7f * (fabsf(dd) + fabsf(no)) / as;
+    if (t < -terr) return CAND_MISS;
+    if (t > tlimit + terr) return CAND_MISS;
+    float p[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) p[i] = o[i] + d[i] * t;
+    float last = 1.0f, margin = 3.0e38f;
+    bool in = true;
+#pragma unroll
+    for (int i = 0; i < N - 1; ++i) {
+        float pn[N];
+#pragma unroll
+        for (int q = 0; q < N; ++q) pn[q] = comp(N + 1 + i * (N + 1) + q);
+        float b = dot<N, float>(p, pn) - comp(N + 1 + i * (N + 1) + N);
+        in = in && (b > 0.0f) && (b < 1.0f);
+        last -= b;
+        margin = fminf(margin, b);
+        if (b < -band) return CAND_MISS;  // decided in FP32: outside by more than the band
+    }
+    in = in && (last > 0.0f);
+    margin = fminf(margin, last);
+    bool retest = fabsf(margin) < band || (in && as < PARALLEL_COS) || t < terr;
+    if (!retest) {
+        if (!in || !(t > 0.0f)) return CAND_MISS;
+        const int sid = get_sid();
+        if (fabsf(t - tlimit) < terr && sid != best_idx) *near = true;  // near-tie with another simplex
+        if (!(t < tlimit)) return CAND_MISS;
+        *t_hit = (double)t;
+        *idx = sid;
+        return CAND_HIT;
+    }
+    if constexpr (DEFER) return CAND_DEFER;
+    ++*retests;
+    *near = true;
+    const int sidx = get_sid();
+    // without a separate FP64 ray the re-test uses the FP32 ray exactly
+    double ob[N], db[N];
+    if (!o64) {
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+            ob[q] = o[q];
+            db[q] = d[q];
+        }
+        o64 = ob;
+        d64 = db;
+    }
+    double t64;
+    if (!simplex_test64<N>(A.rec64 + (int64_t)sidx * rec64_stride(N), o64, d64, &t64)) return CAND_MISS;
+    if (!((float)t64 < tlimit)) return CAND_MISS;
+    *t_hit = t64;
+    *idx = sidx;
+    return CAND_HIT;
+}
+
 // Candidate test of one leaf entry: CAND_HIT for a hit with 0 < t < tlimit,
 // CAND_MISS otherwise.  best_idx: the simplex that currently holds tlimit (a
 // simplex listed in several leaves meets itself again and is no tie).
 // LAY: the FP32 record layout (0 leaf-ordered SoA, 1 simplex-ordered AoS,
 // -1 read from A.layout).  DEFER: instead of the FP64 re-test, return
-// CAND_DEFER; the caller then re-walks the ray with the re-test inline.
-constexpr int CAND_MISS = 0, CAND_HIT = 1, CAND_DEFER = 2;
-
+// CAND_DEFER (the caller decides the candidate later).
 template <int N, typename T, int LAY = -1, bool DEFER = false>
 __device__ __forceinline__ int candidate(const TreeArgs& A, int64_t e, const T* o, const T* d, const double* o64,
                                          const double* d64, T tlimit, int best_idx, double* t_hit, int* idx,
@@ -152,70 +228,8 @@ __device__ __forceinline__ int candidate(const TreeArgs& A, int64_t e, const T* 
         const float* soa = A.soa;
         const int64_t pad = A.pad;
         auto comp = [&](int c) -> float { return aos ? rec[c] : soa[(int64_t)c * pad + e]; };
-        float nrm[N];
-#pragma unroll
-        for (int q = 0; q < N; ++q) nrm[q] = comp(q);
-        float s = dot<N, float>(nrm, d);
-        if (s == 0.0f) return CAND_MISS;
-        float dd = comp(N);
-        float no = dot<N, float>(nrm, o);
-        float t = (dd - no) / s;
-        float as = fabsf(s);
-        float band = RETEST_BAND * (1.0f + 1.0f / as);
-        // rounding bound of t; outside it the sign of t and the comparison
-        // with the limit are decided in FP32
-        float terr = 4.0e-7f * (fabsf(dd) + fabsf(no)) / as;
-        if (t < -terr) return CAND_MISS;
-        if (t > tlimit + terr) return CAND_MISS;
-        float p[N];
-#pragma unroll
-        for (int i = 0; i < N; ++i) p[i] = o[i] + d[i] * t;
-        float last = 1.0f, margin = 3.0e38f;
-        bool in = true;
-#pragma unroll
-        for (int i = 0; i < N - 1; ++i) {
-            float pn[N];
-#pragma unroll
-            for (int q = 0; q < N; ++q) pn[q] = comp(N + 1 + i * (N + 1) + q);
-            float b = dot<N, float>(p, pn) - comp(N + 1 + i * (N + 1) + N);
-            in = in && (b > 0.0f) && (b < 1.0f);
-            last -= b;
-            margin = fminf(margin, b);
-            if (b < -band) return CAND_MISS;  // decided in FP32: outside by more than the band
-        }
-        in = in && (last > 0.0f);
-        margin = fminf(margin, last);
-        bool retest = fabsf(margin) < band || (in && as < PARALLEL_COS) || t < terr;
-        if (!retest) {
-            if (!in || !(t > 0.0f)) return CAND_MISS;
-            if (sid < 0) sid = A.entries[e];
-            if (fabsf(t - tlimit) < terr && sid != best_idx) *near = true;  // near-tie with another simplex
-            if (!(t < tlimit)) return CAND_MISS;
-            *t_hit = (double)t;
-            *idx = sid;
-            return CAND_HIT;
-        }
-        if constexpr (DEFER) return CAND_DEFER;
-        ++*retests;
-        *near = true;
-        int sidx = sid >= 0 ? sid : A.entries[e];
-        // without a separate FP64 ray the re-test uses the FP32 ray exactly
-        double ob[N], db[N];
-        if (!o64) {
-#pragma unroll
-            for (int q = 0; q < N; ++q) {
-                ob[q] = o[q];
-                db[q] = d[q];
-            }
-            o64 = ob;
-            d64 = db;
-        }
-        double t64;
-        if (!simplex_test64<N>(A.rec64 + (int64_t)sidx * rec64_stride(N), o64, d64, &t64)) return CAND_MISS;
-        if (!((float)t64 < tlimit)) return CAND_MISS;
-        *t_hit = t64;
-        *idx = sidx;
-        return CAND_HIT;
+        auto get_sid = [&]() -> int { return sid >= 0 ? sid : A.entries[e]; };
+        return fp32_test<N, DEFER>(A, comp, get_sid, o, d, o64, d64, tlimit, best_idx, t_hit, idx, near, retests);
     }
 }
 

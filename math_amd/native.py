@@ -25,6 +25,8 @@ LIGHT_POINT = 0
 LIGHT_CONSTANT = 1
 LAYOUT_LEAF_SOA = 0
 LAYOUT_SIMPLEX_AOS = 1
+PRIMARY_AUTO = 0
+PRIMARY_WALK = 1
 
 
 class NspError(RuntimeError):
@@ -52,7 +54,7 @@ class Surface(C.Structure):
 class PaintParams(C.Structure):
     _fields_ = [("samples_per_pixel", C.c_int), ("max_depth", C.c_int), ("smooth_normal", C.c_int),
                 ("precision", C.c_int), ("passes", C.c_int), ("seed", C.c_uint64), ("batch_paths", C.c_int64),
-                ("use_graph", C.c_int), ("aov", C.c_int)]
+                ("use_graph", C.c_int), ("aov", C.c_int), ("primary", C.c_int)]
 
 
 PIXELS_AFTER = C.CFUNCTYPE(None, C.c_void_p, C.c_int64, C.c_int64, C.POINTER(C.c_float))
@@ -341,8 +343,9 @@ class Scene:
             pass
 
     def paint(self, spp, max_depth=5, precision=PRECISION_FP32, passes=1, seed=1, batch_paths=0, smooth=False,
-              use_graph=False, stop=None, on_pixels=None, aov=False):
-        p = PaintParams(spp, max_depth, int(smooth), precision, passes, seed, batch_paths, int(use_graph), int(aov))
+              use_graph=False, stop=None, on_pixels=None, aov=False, primary=PRIMARY_AUTO):
+        p = PaintParams(spp, max_depth, int(smooth), precision, passes, seed, batch_paths, int(use_graph), int(aov),
+                        int(primary))
         cb = None
         keep = []
         if on_pixels is not None:
@@ -536,7 +539,7 @@ class Batch:
             pass
 
     def paint(self, spp, max_depth=5, seed=1, batch_paths=0, use_graph=False):
-        p = PaintParams(spp, max_depth, 0, PRECISION_FP32, 1, seed, batch_paths, int(use_graph), 0)
+        p = PaintParams(spp, max_depth, 0, PRECISION_FP32, 1, seed, batch_paths, int(use_graph), 0, 0)
         stop = C.c_int32(0)
         self.ctx.check(_batch_lib().nsp_batch_paint(self.h, C.byref(p), C.byref(stop)))
 

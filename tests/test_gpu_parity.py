@@ -329,3 +329,23 @@ def test_graph_replay_equals_eager(ctx):
         b, cb = graph.image()
         assert np.array_equal(a, b) and np.array_equal(ca, cb), p
     assert eager.statistics()["rays"] == graph.statistics()["rays"]
+
+
+@pytest.mark.parametrize("n", [3, 4, 5])
+def test_rasterised_primary_visibility_equals_tree_walk(ctx, n):
+    # FP32 renders find the primary hits by rasterising the simplices into the
+    # parallel projection; the tree walk gives the same nearest hits, so the
+    # images are bit-identical
+    g = {3: 10, 4: 4, 5: 2}[n]
+    edge = {3: 40, 4: 14, 5: 6}[n]
+    v, f = sphere_mesh(n, g, sigma=0.05)
+    cfg = configs.scene_dict(v, f, edge, tuple([2.0] * n))
+    gm = ctx.mesh(v, f)
+    imgs = []
+    for primary in (native.PRIMARY_AUTO, native.PRIMARY_WALK):
+        print(f"primary mode {primary} n={n}", flush=True)
+        sc = native.Scene(ctx, gm, cfg)
+        sc.paint(8, 3, seed=12, primary=primary, batch_paths=3000)
+        imgs.append(sc.image() + (sc.statistics()["rays"],))
+    for a, b in zip(imgs[0], imgs[1]):
+        assert np.array_equal(a, b)
