@@ -30,20 +30,29 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def closest_bytes(st, n):
-    """Algorithmic bytes of all closest-hit launches of one pass: ray read
-    4(2N+2) + hit write 8 per ray, 8 per node visit, 4 per leaf candidate and
-    4(N^2+N) per simplex test (the FP32 SoA record)."""
-    rays = st["closest_rays"]
-    return (rays * (4 * (2 * n + 2) + 8) + 8 * st["node_visits"] + 4 * st["simplex_tests"]
-            + 4 * (n * n + n) * st["simplex_tests"])
+def walk_bytes(st, n):
+    """Algorithmic bytes of the closest-hit tree walks of one pass (the bounce
+    rays; rasterised primary rays are not walked): ray read 4(2N+2) + hit write
+    8 per ray, 8 per node visit, 4 per leaf candidate (its simplex index) and
+    4(N^2+N) per simplex test (the FP32 record)."""
+    rays = st["closest_rays"] - st["primary_rays"]
+    return (rays * (4 * (2 * n + 2) + 8) + 8 * st["node_visits"]
+            + (4 + 4 * (n * n + n)) * st["simplex_tests"])
 
 
-def profiled_traffic(config, layout):
-    """HBM bytes per k_closest launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of the same workload and record layout (scaled by the
-    copy-kernel calibration stated there), or None."""
-    path = os.path.join(ROOT, "profiles", f"k_closest_{config.lower()}.md")
+def shadow_bytes(st, n):
+    """Algorithmic bytes of the shadow (any-hit) walks of one pass: ray read
+    4(2N+1) + path 4 + contribution 12 per ray, 8 per node visit, 4 + 4(N^2+N)
+    per simplex test."""
+    return (st["shadow_rays"] * (4 * (2 * n + 1) + 16) + 8 * st["shadow_node_visits"]
+            + (4 + 4 * (n * n + n)) * st["shadow_simplex_tests"])
+
+
+def profiled_traffic(config, kernel, layout):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE
+    / WRITE_SIZE summary of the same workload and record layout (scaled by
+    the copy-kernel calibration stated there), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{config.lower()}.md")
     if not os.path.exists(path):
         return None
     text = open(path).read()
@@ -267,16 +276,18 @@ def main():
         scene.paint(spp, depth, seed=1000 + w, batch_paths=args.batch_paths, use_graph=args.graph, primary=primary)
         log(f"warmup {w}: {scene.stage_stats()['ms_total']:.1f} ms")
     r0 = scene.statistics()["rays"]
-    closest_ms = 0.0
-    closest_launches = 0
+    stage_ms = dict(raster=0.0, closest=0.0, shade=0.0, shadow=0.0)
+    launches = dict(closest=0, shadow=0)
     t0 = time.perf_counter()
     for k in range(args.steps):
         scene.paint(spp, depth, seed=k, batch_paths=args.batch_paths, use_graph=args.graph, primary=primary)
         st = scene.stage_stats()
-        closest_ms += st["ms_closest"]
-        closest_launches += st["launches_closest"]
-        log(f"step {k}: {st['ms_total']:.1f} ms, closest {st['ms_closest']:.1f} ms, shade {st['ms_shade']:.1f} ms, "
-            f"shadow {st['ms_shadow']:.1f} ms")
+        for key in stage_ms:
+            stage_ms[key] += st["ms_" + key]
+        for key in launches:
+            launches[key] += st["launches_" + key]
+        log(f"step {k}: {st['ms_total']:.1f} ms, raster {st['ms_raster']:.1f} ms, closest {st['ms_closest']:.1f} ms, "
+            f"shade {st['ms_shade']:.1f} ms, shadow {st['ms_shadow']:.1f} ms")
     elapsed = time.perf_counter() - t0
     rays = scene.statistics()["rays"] - r0
     mrays = rays / elapsed / 1e6
@@ -285,18 +296,30 @@ def main():
     scene.paint(spp, depth, seed=0, batch_paths=args.batch_paths, primary=primary)
     cst = scene.stage_stats()
     scene.set_counters(False)
-    bytes_pass = closest_bytes(cst, n)
-    achieved = bytes_pass / (closest_ms / args.steps * 1e-3) / 1e9
+    # roofline of the dominant traversal kernel: the closest-hit walk or the shadow walk,
+    # whichever took longer (both are HBM/gather kernels with no MFMA work)
+    kernel = "k_shadow" if stage_ms["shadow"] > stage_ms["closest"] else "k_closest"
+    if kernel == "k_closest":
+        bytes_pass = walk_bytes(cst, n)
+        ms, nl = stage_ms["closest"], launches["closest"]
+        walk_rays = cst["closest_rays"] - cst["primary_rays"]
+        per_ray = dict(nodes=cst["node_visits"] / max(walk_rays, 1), leaves=cst["leaf_visits"] / max(walk_rays, 1),
+                       tests=cst["simplex_tests"] / max(walk_rays, 1),
+                       retest_frac=cst["retests"] / max(cst["simplex_tests"], 1))
+    else:
+        bytes_pass = shadow_bytes(cst, n)
+        ms, nl = stage_ms["shadow"], launches["shadow"]
+        sr = max(cst["shadow_rays"], 1)
+        per_ray = dict(nodes=cst["shadow_node_visits"] / sr, leaves=cst["shadow_leaf_visits"] / sr,
+                       tests=cst["shadow_simplex_tests"] / sr,
+                       retest_frac=cst["shadow_retests"] / max(cst["shadow_simplex_tests"], 1))
+    achieved = bytes_pass / (ms / args.steps * 1e-3) / 1e9
     layout_name = ["leaf_soa", "simplex_aos"][mesh.record_layout]
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=profiled_traffic(args.config, layout_name),
-                    kernel="k_closest", launches_per_step=closest_launches // max(args.steps, 1),
-                    avg_launch_ms=round(closest_ms / max(closest_launches, 1), 4),
-                    bytes_per_step=int(bytes_pass),
-                    per_ray=dict(nodes=cst["node_visits"] / max(cst["closest_rays"], 1),
-                                 leaves=cst["leaf_visits"] / max(cst["closest_rays"], 1),
-                                 tests=cst["simplex_tests"] / max(cst["closest_rays"], 1),
-                                 retest_frac=cst["retests"] / max(cst["simplex_tests"], 1)))
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=profiled_traffic(args.config, kernel, layout_name),
+                    kernel=kernel, launches_per_step=nl // max(args.steps, 1),
+                    avg_launch_ms=round(ms / max(nl, 1), 4), bytes_per_step=int(bytes_pass), per_ray=per_ray,
+                    stage_ms_per_step={k: round(v / args.steps, 2) for k, v in stage_ms.items()})
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(v, f, cfg, spp, depth)

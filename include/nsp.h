@@ -237,6 +237,10 @@ typedef struct {
      * (filled when nsp_set_counters is on) */
     int64_t node_visits, leaf_visits, simplex_tests, retests;
     int64_t shadow_node_visits, shadow_leaf_visits, shadow_simplex_tests, shadow_retests;
+    /* rasterised primary visibility (NSP_PRIMARY_AUTO): its time is not part
+     * of ms_closest, its rays are part of closest_rays (primary_rays of them) */
+    double ms_raster;
+    int64_t launches_raster, primary_rays;
 } nsp_stage_stats;
 int nsp_last_pass_stats(const nsp_scene* scene, nsp_stage_stats* out);
 /* Enables per-ray traversal counters (node visits, leaf visits, simplex

@@ -657,10 +657,12 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
                 case 2: st.ms_shade += ms; break;
                 case 3: st.ms_shadow += ms; ++st.launches_shadow; break;
                 case 4: st.ms_accumulate += ms; break;
+                case 5: st.ms_raster += ms; ++st.launches_raster; break;
                 }
             }
             st.ms_total = secs * 1e3;
             st.closest_rays = (int64_t)closest;
+            st.primary_rays = raster ? scene->pixels * S : 0;
             st.shadow_rays = (int64_t)shadow;
             if (scene->counters) {
                 unsigned long long tc[8];

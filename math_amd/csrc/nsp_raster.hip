@@ -24,6 +24,7 @@ struct RasterDims {
     int last;
     int hmax;
     int64_t ppu;
+    float dir[NSP_MAX_DIM];  // the projector's direction: every primary ray's dir
 };
 
 // Pixel box of each simplex: the projection u_i = (x - cam).a_i / |a_i|^2 +
@@ -110,7 +111,8 @@ __global__ void __launch_bounds__(RASTER_BLOCK) k_raster(TreeArgs A, const int32
     const int64_t u1 = ustart[std::min<int64_t>(rd.last, xhi + 1)];
     const int lane = (int)(threadIdx.x & 63);
     const int64_t waves = (int64_t)gridDim.x * (RASTER_BLOCK / 64);
-    for (int64_t u = u0 + (int64_t)blockIdx.x * (RASTER_BLOCK / 64) + (threadIdx.x >> 6); u < u1; u += waves) {
+    // consecutive units (neighbouring boxes) on one XCD, so their samples share its L2
+    for (int64_t u = u0 + (int64_t)xcd_block() * (RASTER_BLOCK / 64) + (threadIdx.x >> 6); u < u1; u += waves) {
         const int2 un = unit[u];
         const int sid = un.x;
         int lo[D], ext[D];
@@ -152,7 +154,7 @@ __global__ void __launch_bounds__(RASTER_BLOCK) k_raster(TreeArgs A, const int32
 #pragma unroll
             for (int k = 0; k < N; ++k) {
                 o[k] = q.org[(int64_t)k * cap + path];
-                d[k] = q.dir[(int64_t)k * cap + path];
+                d[k] = rd.dir[k];
             }
             double th;
             int si;
@@ -200,8 +202,10 @@ void prepare_t(RasterPrep& rp, const MeshDev& m, const SceneK<double>& kd, int S
     rp.plane = 1;
     for (int i = 0; i < D; ++i) rp.screen[i] = kd.screen[i];
     for (int i = 0; i < D - 1; ++i) rp.plane *= kd.screen[i];
-    // units ordered by the box's first pixel on the last screen axis
-    std::vector<std::tuple<int32_t, int32_t, int32_t>> us;
+    // units ordered by the box's low corner, last screen axis major (so that a
+    // batch, a range of the last axis, is a contiguous range of units, and
+    // consecutive units cover neighbouring pixels)
+    std::vector<std::tuple<int64_t, int32_t, int32_t>> us;
     int hmax = 0;
     for (int64_t f = 0; f < F; ++f) {
         const int32_t* b = hb.data() + f * 2 * D;
@@ -210,14 +214,16 @@ void prepare_t(RasterPrep& rp, const MeshDev& m, const SceneK<double>& kd, int S
         if (vol == 0) continue;
         hmax = std::max(hmax, b[2 * D - 1] - b[D - 1]);
         const int64_t chunks = (vol + rp.ppu - 1) / rp.ppu;
-        for (int64_t c = 0; c < chunks; ++c) us.emplace_back(b[D - 1], (int32_t)f, (int32_t)c);
+        int64_t corner = 0;
+        for (int i = D - 1; i >= 0; --i) corner = corner * kd.screen[i] + b[i];
+        for (int64_t c = 0; c < chunks; ++c) us.emplace_back(corner, (int32_t)f, (int32_t)c);
     }
     std::sort(us.begin(), us.end());
     std::vector<int2> hu(us.size());
     std::vector<int64_t> start((size_t)rp.last + 1, (int64_t)us.size());
     for (size_t k = us.size(); k-- > 0;) {
         hu[k] = int2{std::get<1>(us[k]), std::get<2>(us[k])};
-        start[std::get<0>(us[k])] = (int64_t)k;
+        start[std::get<0>(us[k]) / rp.plane] = (int64_t)k;
     }
     for (int v = rp.last - 1; v >= 0; --v) start[v] = std::min(start[v], start[v + 1]);
     rp.hmax = hmax;
@@ -234,7 +240,7 @@ void prepare_t(RasterPrep& rp, const MeshDev& m, const SceneK<double>& kd, int S
 
 template <int N>
 void launch_t(const RasterPrep& rp, const TreeArgs& A, const BatchDesc* desc, RayQueue<float> q, int64_t cap,
-              unsigned long long* keys, hipStream_t s)
+              unsigned long long* keys, const float* dir, hipStream_t s)
 {
     const unsigned gclear = 4096;
     hipLaunchKernelGGL(k_raster_clear, dim3(gclear), dim3(256), 0, s, desc, keys);
@@ -245,6 +251,7 @@ void launch_t(const RasterPrep& rp, const TreeArgs& A, const BatchDesc* desc, Ra
     rd.last = rp.last;
     rd.hmax = rp.hmax;
     rd.ppu = rp.ppu;
+    for (int k = 0; k < N; ++k) rd.dir[k] = dir[k];
     const unsigned blocks = 2048;  // 8192 waves; each wave loops over the batch's units
     hipLaunchKernelGGL(k_raster<N>, dim3(blocks), dim3(RASTER_BLOCK), 0, s, A, rp.box.as<int32_t>(),
                        rp.unit.as<int2>(), rp.ustart.as<int64_t>(), rd, desc, rp.S, q, cap, keys);
@@ -268,13 +275,13 @@ void raster_prepare(RasterPrep& rp, const MeshDev& m, const SceneK<double>& kd, 
 }
 
 void raster_launch(const RasterPrep& rp, const MeshDev& m, const TreeArgs& A, const BatchDesc* desc,
-                   RayQueue<float> q, int64_t cap, unsigned long long* keys, hipStream_t s)
+                   RayQueue<float> q, int64_t cap, unsigned long long* keys, const float* dir, hipStream_t s)
 {
     switch (m.n) {
-    case 3: launch_t<3>(rp, A, desc, q, cap, keys, s); break;
-    case 4: launch_t<4>(rp, A, desc, q, cap, keys, s); break;
-    case 5: launch_t<5>(rp, A, desc, q, cap, keys, s); break;
-    case 6: launch_t<6>(rp, A, desc, q, cap, keys, s); break;
+    case 3: launch_t<3>(rp, A, desc, q, cap, keys, dir, s); break;
+    case 4: launch_t<4>(rp, A, desc, q, cap, keys, dir, s); break;
+    case 5: launch_t<5>(rp, A, desc, q, cap, keys, dir, s); break;
+    case 6: launch_t<6>(rp, A, desc, q, cap, keys, dir, s); break;
     default: raise(NSP_E_INVALID, "dimension not supported");
     }
 }

@@ -25,7 +25,7 @@ struct RasterPrep {
     int64_t units = 0;
     int screen[NSP_MAX_DIM - 1] = {0};
     DevBuf box;           // int32 [F][2(n-1)]: first and last pixel per screen axis
-    DevBuf unit;          // int2 (simplex, chunk of its box) in the order of the box's first last-axis pixel
+    DevBuf unit;          // int2 (simplex, chunk of its box) in the order of the box's low corner, last axis major
     DevBuf ustart;        // int64 [last + 1]: first unit whose box starts at last-axis pixel v
 };
 
@@ -34,8 +34,9 @@ struct RasterPrep {
 void raster_prepare(RasterPrep& rp, const MeshDev& m, const SceneK<double>& kd, int S, hipStream_t s);
 
 // Depth-0 closest hits of the batch in flight (desc) into the ray queue q:
-// clears the per-path keys, rasterises, writes q.hit / q.hit_t.
+// clears the per-path keys, rasterises, writes q.hit / q.hit_t.  dir: the
+// direction raygen gave every primary ray (the projector's, in FP32).
 void raster_launch(const RasterPrep& rp, const MeshDev& m, const TreeArgs& A, const BatchDesc* desc,
-                   RayQueue<float> q, int64_t cap, unsigned long long* keys, hipStream_t s);
+                   RayQueue<float> q, int64_t cap, unsigned long long* keys, const float* dir, hipStream_t s);
 
 }  // namespace nsp

@@ -307,6 +307,18 @@ __device__ __forceinline__ void shadow_unoccluded(const ShadowQueue<T>& sq, int6
     }
 }
 
+// Entries of the FP64-decision lists: ray index, then PENDING candidates (-1: none).
+constexpr int RES_STRIDE = 1 + PENDING;
+
+__device__ __forceinline__ void push_pending(unsigned long long* res_count, int32_t* res, int64_t ray,
+                                             const int* pend, int npend)
+{
+    const int64_t k = (int64_t)atomicAdd(res_count, 1ull);
+    res[RES_STRIDE * k] = (int32_t)ray;
+#pragma unroll
+    for (int c = 0; c < PENDING; ++c) res[RES_STRIDE * k + 1 + c] = c < npend ? pend[c] : -1;
+}
+
 // Persistent traversal (shadow stage): each lane holds one ray walk (RayWalk)
 // across the iterations of the loop and does one leaf step per iteration; when
 // at least REFILL_LANES lanes of a wave have finished, the wave takes that
@@ -352,11 +364,12 @@ struct RayPool {
     }
 };
 
-// Closest hit: lane = ray, grid-stride over the queue, the whole walk with
-// the FP64 re-test inline.  Consecutive rays are samples of one pixel, so the
-// lanes of a wave walk nearly the same nodes and leaves; this static
-// assignment keeps them together, which measured faster here than the
-// persistent refill loop the shadow stage uses.
+// Closest hit of primary rays (when they are walked, not rasterised): lane =
+// ray, grid-stride over the queue, the whole walk with the FP64 re-test
+// inline.  Consecutive rays are samples of one pixel, so the lanes of a wave
+// walk nearly the same nodes and leaves; this static assignment keeps them
+// together, which measured faster for them than the persistent refill loop
+// (k_closest_pool) that the incoherent bounce rays use.
 template <int N, typename T, int LAY>
 __global__ void __launch_bounds__(TRACE_BLOCK) k_closest(TreeArgs A, RayQueue<T> q, int64_t cap,
                                                          const unsigned long long* __restrict__ cnt)
@@ -379,6 +392,104 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_closest(TreeArgs A, RayQueue<T>
             w.flush_counters(A);
             h = w.result();
         }
+        q.hit[i] = h.idx;
+        q.hit_t[i] = (T)h.t;
+    }
+}
+
+// Closest hit of bounce rays with the persistent loop of the shadow stage:
+// these rays are incoherent, so refilling finished lanes keeps waves full.
+// Candidates within the FP64 re-test band are decided afterwards by
+// k_closest_resolve; walks with more than PENDING of them are walked again by
+// k_closest_exact.
+template <int N, typename T, int LAY>
+__global__ void __launch_bounds__(TRACE_BLOCK) k_closest_pool(TreeArgs A, RayQueue<T> q, int64_t cap,
+                                                              unsigned long long* __restrict__ cnt,
+                                                              unsigned long long* __restrict__ redo_count,
+                                                              int32_t* __restrict__ redo,
+                                                              unsigned long long* __restrict__ res_count,
+                                                              int32_t* __restrict__ res)
+{
+    __shared__ int stack[MAX_LEVELS * TRACE_BLOCK];
+    int* st = stack + threadIdx.x;
+    RayPool pool(cnt + 8, (int64_t)cnt[0]);
+    RayWalk<N, T, false, LAY, true> w;
+    int64_t ray = -1;
+    for (;;) {
+        if (!pool.drained) {
+            const int64_t i = pool.take(ray < 0);
+            if (i >= 0) {
+#pragma unroll
+                for (int k = 0; k < N; ++k) {
+                    w.o[k] = q.org[(int64_t)k * cap + i];
+                    w.d[k] = q.dir[(int64_t)k * cap + i];
+                }
+                if (w.start(A, (T)INFINITY)) {
+                    ray = i;
+                } else {
+                    q.hit[i] = -1;
+                    q.hit_t[i] = (T)0;
+                }
+            }
+        }
+        if (ray >= 0 && w.step(A, st, nullptr, nullptr)) {
+            if (w.deferred) {
+                redo[atomicAdd(redo_count, 1ull)] = (int32_t)ray;
+            } else {
+                const HitResult h = w.result();
+                q.hit[ray] = h.idx;
+                q.hit_t[ray] = (T)h.t;
+                if (w.npend > 0) push_pending(res_count, res, ray, w.pend, w.npend);
+            }
+            w.flush_counters(A);
+            ray = -1;
+        }
+        if (pool.drained && __ballot(ray >= 0) == 0) break;
+    }
+}
+
+template <int N>
+__global__ void __launch_bounds__(256) k_closest_resolve(TreeArgs A, RayQueue<float> q, int64_t cap,
+                                                         const unsigned long long* __restrict__ res_count,
+                                                         const int32_t* __restrict__ res)
+{
+    const int64_t count = (int64_t)*res_count;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < count;
+         j += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = res[RES_STRIDE * j];
+        float o[N], d[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            o[k] = q.org[(int64_t)k * cap + i];
+            d[k] = q.dir[(int64_t)k * cap + i];
+        }
+        int best = q.hit[i];
+        double best_t = best >= 0 ? (double)q.hit_t[i] : (double)INFINITY;
+        if (pending_resolve<N, false>(A, o, d, res + RES_STRIDE * j + 1, INFINITY, &best, &best_t)) {
+            q.hit[i] = best;
+            q.hit_t[i] = (float)best_t;
+        }
+    }
+}
+
+template <int N, typename T>
+__global__ void __launch_bounds__(TRACE_BLOCK) k_closest_exact(TreeArgs A, RayQueue<T> q, int64_t cap,
+                                                               const unsigned long long* __restrict__ redo_count,
+                                                               const int32_t* __restrict__ redo)
+{
+    __shared__ int stack[MAX_LEVELS * TRACE_BLOCK];
+    int* st = stack + threadIdx.x;
+    const int64_t count = (int64_t)*redo_count;
+    for (int64_t j = (int64_t)xcd_block() * blockDim.x + threadIdx.x; j < count;
+         j += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = redo[j];
+        T o[N], d[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            o[k] = q.org[(int64_t)k * cap + i];
+            d[k] = q.dir[(int64_t)k * cap + i];
+        }
+        const HitResult h = traverse<N, T, false>(A, o, d, (T)INFINITY, st, nullptr, nullptr);
         q.hit[i] = h.idx;
         q.hit_t[i] = (T)h.t;
     }
@@ -419,10 +530,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_shadow(TreeArgs A, ShadowQueue<
                 redo[atomicAdd(redo_count, 1ull)] = (int32_t)ray;
             } else if (w.best < 0) {
                 if (w.npend > 0) {
-                    const int64_t k = (int64_t)atomicAdd(res_count, 1ull);
-                    res[3 * k] = (int32_t)ray;
-                    res[3 * k + 1] = w.pend[0];
-                    res[3 * k + 2] = w.npend > 1 ? w.pend[PENDING - 1] : -1;
+                    push_pending(res_count, res, ray, w.pend, w.npend);
                 } else {
                     shadow_unoccluded<N, T>(sq, scap, ray, L, cap, direct);
                 }
@@ -469,7 +577,7 @@ __global__ void __launch_bounds__(256) k_shadow_resolve(TreeArgs A, ShadowQueue<
     const int64_t count = (int64_t)*res_count;
     for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < count;
          j += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = res[3 * j];
+        const int64_t i = res[RES_STRIDE * j];
         float o[N], d[N];
 #pragma unroll
         for (int k = 0; k < N; ++k) {
@@ -478,7 +586,7 @@ __global__ void __launch_bounds__(256) k_shadow_resolve(TreeArgs A, ShadowQueue<
         }
         int best = -1;
         double best_t = (double)sq.tmax[i];
-        if (!pending_resolve<N, true>(A, o, d, res[3 * j + 1], res[3 * j + 2], sq.tmax[i], &best, &best_t))
+        if (!pending_resolve<N, true>(A, o, d, res + RES_STRIDE * j + 1, sq.tmax[i], &best, &best_t))
             shadow_unoccluded<N, float>(sq, scap, i, L, cap, direct);
     }
 }
@@ -543,7 +651,9 @@ void alloc_render_buffers(RenderBuffers& b, int n, int precision, int64_t cap, i
     b.sq[4].alloc(sizeof(float) * 3 * scap);
     b.L.alloc(sizeof(float) * 3 * cap);
     b.redo.alloc(sizeof(int32_t) * scap);
-    b.res.alloc(sizeof(int32_t) * 3 * scap);
+    b.redo_c.alloc(sizeof(int32_t) * cap);
+    b.res_c.alloc(sizeof(int32_t) * RES_STRIDE * cap);
+    b.res.alloc(sizeof(int32_t) * RES_STRIDE * scap);
     b.counters.alloc(sizeof(unsigned long long) * CNT_STRIDE * (max_depth + 2));
     b.desc.alloc(sizeof(BatchDesc));
     b.totals.alloc(sizeof(unsigned long long) * 3);
@@ -594,12 +704,17 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
     // FP32 walks read the record layout chosen at compile time; FP64 walks read the FP64 records
     const bool aos = sizeof(T) == 4 && A.layout == 1;
     auto kc = aos ? k_closest<N, T, 1> : k_closest<N, T, 0>;
+    auto kcp = aos ? k_closest_pool<N, T, 1> : k_closest_pool<N, T, 0>;
+    const unsigned gcp = aos ? persistent_grid(k_closest_pool<N, T, 1>) : persistent_grid(k_closest_pool<N, T, 0>);
+
     auto kshade = sc.smooth ? k_shade<N, T, true> : k_shade<N, T, false>;
     auto ks = aos ? k_shadow<N, T, 1> : k_shadow<N, T, 0>;
     const unsigned gs = aos ? persistent_grid(k_shadow<N, T, 1>) : persistent_grid(k_shadow<N, T, 0>);
     const unsigned gx = 8 * 32;  // FP64 decisions and re-walks of shadow rays: a few per cent or fewer
     int32_t* redo_s = b.redo.as<int32_t>();
     int32_t* res_s = b.res.as<int32_t>();
+    int32_t* redo_c = b.redo_c.as<int32_t>();
+    int32_t* res_c = b.res_c.as<int32_t>();
     auto timed = [&](int stage, auto&& launch) {
         if (!rl.marks) {
             launch();
@@ -621,12 +736,26 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
         RayQueue<T>& qin = (depth & 1) ? qb : qa;
         RayQueue<T>& qout = (depth & 1) ? qa : qb;
         unsigned long long* dc = cnt + depth * CNT_STRIDE;
-        timed(1, [&] {
+        const bool rasterise = sizeof(T) == 4 && depth == 0 && rl.raster;
+        timed(rasterise ? 5 : 1, [&] {
             if constexpr (sizeof(T) == 4) {
-                if (depth == 0 && rl.raster) {
-                    raster_launch(*rl.raster, *rl.mesh, A, desc, qin, cap, b.keys.as<unsigned long long>(), s);
+                if (rasterise) {
+                    raster_launch(*rl.raster, *rl.mesh, A, desc, qin, cap, b.keys.as<unsigned long long>(),
+                                  sc.cam_dir, s);
                     return;
                 }
+            }
+            if (depth > 0) {
+                hipLaunchKernelGGL(kcp, dim3(gcp), dim3(TRACE_BLOCK), 0, s, A, qin, cap, dc, dc + 5, redo_c, dc + 3,
+                                   res_c);
+                NSP_HIP(hipGetLastError());
+                if constexpr (sizeof(T) == 4) {
+                    hipLaunchKernelGGL((k_closest_resolve<N>), dim3(gx), dim3(256), 0, s, A, qin, cap, dc + 3, res_c);
+                    NSP_HIP(hipGetLastError());
+                }
+                hipLaunchKernelGGL((k_closest_exact<N, T>), dim3(gx), dim3(TRACE_BLOCK), 0, s, A, qin, cap, dc + 5,
+                                   redo_c);
+                return;
             }
             hipLaunchKernelGGL(kc, dim3(g), dim3(TRACE_BLOCK), 0, s, A, qin, cap, dc);
         });

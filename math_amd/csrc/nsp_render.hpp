@@ -90,8 +90,8 @@ struct MeshShadeArgs {
 };
 
 // Per-depth counters, CNT_STRIDE u64 per depth: 0 queued rays, 1 shadow rays, 2 probe rays,
-// 4 shadow FP64 decisions, 6 shadow re-walks, 16-23 shadow work index per queue slice (the others
-// unused).  The stride keeps the counters that one stage's waves append to with atomics
+// 3 closest-hit FP64 decisions, 4 shadow FP64 decisions, 5 closest-hit re-walks, 6 shadow re-walks,
+// 8-15 closest-hit work index per queue slice, 16-23 shadow work index per queue slice (7 unused).  The stride keeps the counters that one stage's waves append to with atomics
 // (next depth's queue, shadow queue) on different cache lines.
 constexpr int CNT_STRIDE = 24;
 
@@ -103,8 +103,9 @@ struct RenderBuffers {
     DevBuf qa[8], qb[8], sq[6];
     DevBuf L;               // 3 * cap floats
     DevBuf redo;            // int32 shadow-ray indices to walk again with the FP64 re-test inline
-    DevBuf res;             // int32 (shadow ray, pending candidate, pending candidate) for the FP64 decisions
+    DevBuf res;             // int32 (shadow ray, PENDING candidates) for the FP64 decisions
     DevBuf keys;            // u64 per path: (t, simplex) of the rasterised primary hits
+    DevBuf redo_c, res_c;   // the same two lists for closest-hit walks of bounce rays
     DevBuf aov;             // aov_channels(n) * cap floats per path
     DevBuf counters;        // (max_depth + 2) * CNT_STRIDE u64
     DevBuf desc;            // BatchDesc of the batch in flight
@@ -121,7 +122,7 @@ void launch_first_hit(const MeshDev& m, const float* org, const float* dir, int6
                       uint8_t* near, unsigned long long* counters, hipStream_t s);
 
 // HIP events around each stage launch (stage ids: 0 raygen, 1 closest,
-// 2 shade, 3 shadow, 4 accumulate).
+// 2 shade, 3 shadow, 4 accumulate, 5 rasterised primary visibility).
 struct StageMarks {
     std::vector<hipEvent_t> pool;
     size_t used = 0;

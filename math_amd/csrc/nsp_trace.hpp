@@ -273,7 +273,7 @@ __device__ __forceinline__ bool slab(const T* blo, const T* bsz, const T* o, con
 // have lowered the limit later candidates are tested against, and the walk
 // ends no earlier for it.  A walk with more pending candidates ends with
 // deferred set, for a re-walk with the re-test inline.
-constexpr int PENDING = 2;
+constexpr int PENDING = 4;
 
 template <int N, typename T, bool ANY, int LAY = -1, bool DEFER = false>
 struct RayWalk {
@@ -447,11 +447,12 @@ struct RayWalk {
     }
 };
 
-// FP64 decision of the pending candidates p0, p1 (-1: none) of an FP32 walk
-// whose nearest hit is (best, best_t): the inline re-test's acceptance and
-// tie rules.  ANY: returns true as soon as one is a hit before tmax.
+// FP64 decision of the pending candidates pend[0..PENDING) (-1: none) of an
+// FP32 walk whose nearest hit is (best, best_t): the inline re-test's
+// acceptance and tie rules.  ANY: returns true as soon as one is a hit before
+// tmax; otherwise returns true when one replaced the nearest hit.
 template <int N, bool ANY>
-__device__ __forceinline__ bool pending_resolve(const TreeArgs& A, const float* o, const float* d, int p0, int p1,
+__device__ __forceinline__ bool pending_resolve(const TreeArgs& A, const float* o, const float* d, const int* pend,
                                                 float tmax, int* best, double* best_t)
 {
     double o64[N], d64[N];
@@ -462,7 +463,7 @@ __device__ __forceinline__ bool pending_resolve(const TreeArgs& A, const float* 
     }
     bool any = false;
     for (int k = 0; k < PENDING; ++k) {
-        const int sid = k == 0 ? p0 : p1;
+        const int sid = pend[k];
         if (sid < 0) continue;
         double t64;
         if (!simplex_test64<N>(A.rec64 + (int64_t)sid * rec64_stride(N), o64, d64, &t64)) continue;

@@ -79,7 +79,8 @@ class StageStats(C.Structure):
                 ("launches_closest", C.c_int64), ("launches_shadow", C.c_int64), ("closest_rays", C.c_int64),
                 ("shadow_rays", C.c_int64), ("node_visits", C.c_int64), ("leaf_visits", C.c_int64),
                 ("simplex_tests", C.c_int64), ("retests", C.c_int64), ("shadow_node_visits", C.c_int64),
-                ("shadow_leaf_visits", C.c_int64), ("shadow_simplex_tests", C.c_int64), ("shadow_retests", C.c_int64)]
+                ("shadow_leaf_visits", C.c_int64), ("shadow_simplex_tests", C.c_int64), ("shadow_retests", C.c_int64),
+                ("ms_raster", C.c_double), ("launches_raster", C.c_int64), ("primary_rays", C.c_int64)]
 
 
 # every symbol of include/nsp.h
