@@ -296,30 +296,37 @@ def main():
     scene.paint(spp, depth, seed=0, batch_paths=args.batch_paths, primary=primary)
     cst = scene.stage_stats()
     scene.set_counters(False)
-    # roofline of the dominant traversal kernel: the closest-hit walk or the shadow walk,
-    # whichever took longer (both are HBM/gather kernels with no MFMA work)
-    kernel = "k_shadow" if stage_ms["shadow"] > stage_ms["closest"] else "k_closest"
-    if kernel == "k_closest":
-        bytes_pass = walk_bytes(cst, n)
-        ms, nl = stage_ms["closest"], launches["closest"]
-        walk_rays = cst["closest_rays"] - cst["primary_rays"]
-        per_ray = dict(nodes=cst["node_visits"] / max(walk_rays, 1), leaves=cst["leaf_visits"] / max(walk_rays, 1),
-                       tests=cst["simplex_tests"] / max(walk_rays, 1),
-                       retest_frac=cst["retests"] / max(cst["simplex_tests"], 1))
-    else:
-        bytes_pass = shadow_bytes(cst, n)
-        ms, nl = stage_ms["shadow"], launches["shadow"]
-        sr = max(cst["shadow_rays"], 1)
-        per_ray = dict(nodes=cst["shadow_node_visits"] / sr, leaves=cst["shadow_leaf_visits"] / sr,
-                       tests=cst["shadow_simplex_tests"] / sr,
-                       retest_frac=cst["shadow_retests"] / max(cst["shadow_simplex_tests"], 1))
-    achieved = bytes_pass / (ms / args.steps * 1e-3) / 1e9
+    # roofline of the dominant traversal kernel (the closest-hit walks of the bounce rays or the
+    # shadow walks, whichever took longer; both are HBM/gather kernels with no MFMA work); the
+    # other one is reported beside it
     layout_name = ["leaf_soa", "simplex_aos"][mesh.record_layout]
-    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=profiled_traffic(args.config, kernel, layout_name),
-                    kernel=kernel, launches_per_step=nl // max(args.steps, 1),
-                    avg_launch_ms=round(ms / max(nl, 1), 4), bytes_per_step=int(bytes_pass), per_ray=per_ray,
-                    stage_ms_per_step={k: round(v / args.steps, 2) for k, v in stage_ms.items()})
+    walk_rays = max(cst["closest_rays"] - cst["primary_rays"], 1)
+    sr = max(cst["shadow_rays"], 1)
+    kernels = {
+        "k_closest": dict(bytes=walk_bytes(cst, n), ms=stage_ms["closest"], launches=launches["closest"],
+                          per_ray=dict(nodes=cst["node_visits"] / walk_rays, leaves=cst["leaf_visits"] / walk_rays,
+                                       tests=cst["simplex_tests"] / walk_rays,
+                                       fp64_frac=cst["retests"] / max(cst["simplex_tests"], 1))),
+        "k_shadow": dict(bytes=shadow_bytes(cst, n), ms=stage_ms["shadow"], launches=launches["shadow"],
+                         per_ray=dict(nodes=cst["shadow_node_visits"] / sr, leaves=cst["shadow_leaf_visits"] / sr,
+                                      tests=cst["shadow_simplex_tests"] / sr,
+                                      fp64_frac=cst["shadow_retests"] / max(cst["shadow_simplex_tests"], 1))),
+    }
+
+    def line(name):
+        k = kernels[name]
+        achieved = k["bytes"] / (k["ms"] / args.steps * 1e-3) / 1e9
+        return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=profiled_traffic(args.config, name, layout_name),
+                    kernel=name, launches_per_step=k["launches"] // max(args.steps, 1),
+                    avg_launch_ms=round(k["ms"] / max(k["launches"], 1), 4),
+                    bytes_per_launch=int(k["bytes"] / max(k["launches"] // max(args.steps, 1), 1)),
+                    per_ray=k["per_ray"])
+
+    dominant = max(kernels, key=lambda name: kernels[name]["ms"])
+    roofline = line(dominant)
+    roofline["other"] = line("k_shadow" if dominant == "k_closest" else "k_closest")
+    roofline["stage_ms_per_step"] = {k: round(v / args.steps, 2) for k, v in stage_ms.items()}
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(v, f, cfg, spp, depth)

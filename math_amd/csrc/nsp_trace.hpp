@@ -386,6 +386,7 @@ struct RayWalk {
                 near = near || nr;
                 if (DEFER && c == CAND_DEFER) {
                     near = true;
+                    ++nret;  // decided in FP64 after the walk
                     const int sid = A.entries[e];
                     bool known = false;
 #pragma unroll
