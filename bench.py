@@ -100,9 +100,9 @@ def cpu_baseline(v, f, cfg, spp, depth, seconds_target=15.0):
 
 
 def denoise_bench(args):
-    """C3: denoiser trained on the tracer's own renders; time to RMSE <= 0.01
-    against a 4096-sample FP64 render, FP32 + denoiser vs plain FP32, on
-    held-out 128^3 scenes (or a centred window of them with --eval-window)."""
+    """C3: denoiser trained on the tracer's own renders; time to RMSE <= 0.03,
+    0.02 and 0.01 against a 4096-sample FP64 render, FP32 + denoiser vs plain
+    FP32, on held-out 128^3 scenes (or a centred window with --eval-window)."""
     import numpy as np
 
     from math_amd import denoise_eval as DE
@@ -110,7 +110,8 @@ def denoise_bench(args):
 
     ctx = native.Context(0)
     t0 = time.perf_counter()
-    X, Y = DE.training_set(ctx, list(range(100, 100 + args.train_variants)), window=args.window, log=log)
+    X, Y = DE.training_set(ctx, list(range(100, 100 + args.train_variants)), window=args.window,
+                           target_spp=args.target_spp, log=log)
     gen_s = time.perf_counter() - t0
     log(f"training set: {X.shape[0]} volumes of {X.shape[1]}^3 in {gen_s:.1f} s")
     dn = native.Denoiser(ctx, DE.IN_CHANNELS, DE.RESIDUAL, seed=1)
@@ -118,10 +119,9 @@ def denoise_bench(args):
     chunks = []
     for c in range(0, args.train_steps, 500):
         k = min(500, args.train_steps - c)
-        frac = c / args.train_steps
-        lr = 1e-3 if frac < 0.6 else (3e-4 if frac < 0.85 else 1e-4)
+        lr = 2e-4 if c < 0.8 * args.train_steps else 5e-5
         chunks.append(dn.train(X, Y, patch=32, batch=16, steps=k, lr=lr, seed=2 + c))
-        log(f"training steps {c + k}: lr {lr:g}, loss {chunks[-1][-50:].mean():.5f} ({time.perf_counter() - t0:.1f} s)")
+        log(f"training steps {c + k}: lr {lr:g}, loss {chunks[-1][-50:].mean():.6f} ({time.perf_counter() - t0:.1f} s)")
     losses = np.concatenate(chunks)
     train_s = time.perf_counter() - t0
     del X, Y
@@ -140,24 +140,21 @@ def denoise_bench(args):
         log(f"eval scene {seed}: FP64 4096-sample reference in {ref_s:.1f} s")
         r = DE.time_to_rmse(ctx, mesh, cfg, ref, dn, seed=seed * 11)
         r["seed"] = seed
-        log(f"eval scene {seed}: plain {r['plain_time_s']} s, curve "
-            f"{[(c, round(t, 3), round(e, 4)) for c, t, e in r['plain_curve']]}")
-        log(f"eval scene {seed}: denoised {r['denoised_time_s']} s (level-1 rmse {r['denoised_rmse']:.4f}, "
-            f"its 4-spp input {r['input4_rmse']:.4f}), curve "
-            f"{[(c, round(t, 3), round(e, 4)) for c, t, e in r['denoised_curve']]}")
+        log(f"eval scene {seed}: {r['thresholds']}")
+        log(f"eval scene {seed}: plain curve {[(c, round(t, 3), round(e, 4)) for c, t, e in r['plain_curve']]}")
+        log(f"eval scene {seed}: denoised curve {[(c, round(t, 3), round(e, 4)) for c, t, e in r['denoised_curve']]}")
         results.append(r)
         mesh.close()
-    ok = [r for r in results if r["plain_time_s"] is not None and r["denoised_time_s"] is not None]
     out = {
-        "metric": "time to RMSE<=0.01 vs 4096-sample FP64 ref at n=4, FP32+denoiser vs plain FP32",
+        "metric": "time to RMSE<=thr vs 4096-sample FP64 ref at n=4, FP32+denoiser vs plain FP32",
         "eval_screen": f"{args.eval_window}^3 centred window" if args.eval_window else "128^3",
-        "eval": [{k: r[k] for k in ("seed", "plain_time_s", "denoised_time_s", "denoised_rmse", "input4_rmse",
-                                    "denoiser_inference_s")} for r in results],
-        "speedup": [round(r["plain_time_s"] / r["denoised_time_s"], 3) for r in ok],
+        "eval": [{"seed": r["seed"], "thresholds": r["thresholds"], "level1_rmse": r["denoised_rmse"],
+                  "level1_input_rmse": r["input4_rmse"], "inference_s": r["denoiser_inference_s"]} for r in results],
         "curves": [{"seed": r["seed"], "plain": r["plain_curve"], "denoised": r["denoised_curve"]} for r in results],
         "train": {"variants": args.train_variants, "levels": list(DE.LEVELS), "window": args.window,
-                  "steps": args.train_steps, "seconds": round(train_s, 2), "data_seconds": round(gen_s, 2),
-                  "loss_first": float(losses[:5].mean()), "loss_last": float(losses[-50:].mean())},
+                  "target_spp": args.target_spp, "steps": args.train_steps, "seconds": round(train_s, 2),
+                  "data_seconds": round(gen_s, 2), "loss_first": float(losses[:5].mean()),
+                  "loss_last": float(losses[-50:].mean())},
     }
     print(json.dumps(out), flush=True)
 
@@ -187,15 +184,20 @@ def batch_bench(args):
     log(f"{args.scenes} meshes built in {time.perf_counter() - t0:.1f} s "
         f"({min(m[2] for m in meshes)}..{max(m[2] for m in meshes)} simplices)")
     per = [native.Scene(ctx, m, c) for m, c, _ in meshes]
+    # warm-up: per-scene set-up (raster boxes, workspaces) is not timed, as for the batch
+    for s in per:
+        s.paint(4, 5, seed=1)
+        s.set_image(*[np.zeros_like(a) for a in s.image()])
+    r0 = sum(s.statistics()["rays"] for s in per)
     t0 = time.perf_counter()
     for i, s in enumerate(per):
         s.paint(4, 5, seed=1)
-        log(f"per-scene {i}: n={s.n} {s.pixels} pixels, {s.statistics()['rays']} rays, "
-            f"{time.perf_counter() - t0:.1f} s cumulative")
+        log(f"per-scene {i}: n={s.n} {s.pixels} pixels, {time.perf_counter() - t0:.1f} s cumulative")
     per_s = time.perf_counter() - t0
-    per_rays = sum(s.statistics()["rays"] for s in per)
+    per_rays = sum(s.statistics()["rays"] for s in per) - r0
     bat = [native.Scene(ctx, m, c) for m, c, _ in meshes]
     b = native.Batch(ctx, bat)
+    b.set_lanes(args.lanes)
     b.paint(4, 5, seed=1, use_graph=True)  # warm-up
     for s in bat:
         s.set_image(*[np.zeros_like(a) for a in s.image()])
@@ -203,7 +205,8 @@ def batch_bench(args):
     b.paint(4, 5, seed=1, use_graph=True)
     bat_s = time.perf_counter() - t0
     bat_rays = b.statistics()["rays"]
-    out = {"metric": "C4 batched persistent launch, Mrays/s", "scenes": args.scenes,
+    out = {"metric": "C4 batched launch (one hipGraph, concurrent lanes), Mrays/s", "scenes": args.scenes,
+           "lanes": args.lanes,
            "per_scene": {"seconds": round(per_s, 3), "Mrays_s": round(per_rays / per_s / 1e6, 2)},
            "batched": {"seconds": round(bat_s, 3), "Mrays_s": round(bat_rays / bat_s / 1e6, 2)},
            "speedup": round(per_s / bat_s, 3)}
@@ -232,10 +235,12 @@ def main():
     ap.add_argument("--train-steps", type=int, default=6000)
     ap.add_argument("--eval-scenes", type=int, default=2)
     ap.add_argument("--window", type=int, default=64)
+    ap.add_argument("--target-spp", type=int, default=2048)
     ap.add_argument("--eval-window", type=int, default=0, help="evaluate on a centred window (0: full 128^3)")
     ap.add_argument("--batch-scenes", action="store_true",
                     help="C4: batched persistent launch over many scenes vs one launch sequence per scene")
     ap.add_argument("--scenes", type=int, default=256)
+    ap.add_argument("--lanes", type=int, default=4)
     args = ap.parse_args()
     if args.denoise:
         return denoise_bench(args)

@@ -247,15 +247,21 @@ int nsp_last_pass_stats(const nsp_scene* scene, nsp_stage_stats* out);
  * tests, FP64 re-tests) for roofline accounting; off by default. */
 int nsp_set_counters(nsp_scene* scene, int enable);
 
-/* Batched persistent launch: one sequence of wavefront launches renders one
- * FP32 pass of every scene in the batch (scenes of any size and dimension,
- * rays dispatched per scene inside the kernels).  No reference counterpart
- * (the reference paints one scene per paint() call); each scene's image and
- * statistics accumulate exactly as after nsp_paint with the same parameters. */
+/* Batched launch: one FP32 pass of every scene in the batch (scenes of any
+ * size and dimension, all of the batch's context) in one launch.  The scenes
+ * are spread over concurrent lanes (streams with their own workspaces) by
+ * longest processing time first; each lane runs its scenes' wavefront
+ * sequences back to back, and with use_graph the whole batch, every bounce of
+ * every scene, is captured into one hipGraph and launched once.  No reference
+ * counterpart (the reference paints one scene per paint() call); each scene's
+ * image and statistics accumulate exactly as after nsp_paint with the same
+ * parameters. */
 typedef struct nsp_batch nsp_batch;
 int nsp_batch_create(nsp_context* ctx, nsp_scene* const* scenes, int count, nsp_batch** out);
 void nsp_batch_destroy(nsp_batch* batch);
 int nsp_batch_paint(nsp_batch* batch, const nsp_paint_params* params, const volatile int32_t* stop);
+/* Number of concurrent lanes (1..64, default 4). */
+int nsp_batch_set_lanes(nsp_batch* batch, int lanes);
 /* rays of the last batch pass (reference counting rules) and its wall time */
 int nsp_batch_statistics(const nsp_batch* batch, int64_t* ray_count, double* last_pass_duration);
 /* Chooses batch_paths for a scene by timing candidate queue sizes on the
@@ -272,6 +278,19 @@ int nsp_autotune(nsp_scene* scene, nsp_paint_params* params);
  * [nb][depth][height][width][channels] on the host. */
 typedef struct nsp_denoiser nsp_denoiser;
 int nsp_denoiser_create(nsp_context* ctx, int in_channels, int residual_channel, uint64_t seed, nsp_denoiser** out);
+/* Heads of the last layer (no reference counterpart):
+ *   NSP_DENOISER_RESIDUAL  3 outputs added to the residual channels (as nsp_denoiser_create)
+ *   NSP_DENOISER_KERNEL    27 outputs per voxel are the logits of a 3x3x3 filter; the
+ *                          prediction is the softmax-weighted sum, over the taps inside
+ *                          the volume, of the channels [residual_channel,
+ *                          residual_channel+3) of the tap voxels (kernel prediction, as in
+ *                          Bako et al. 2017, "Kernel-predicting convolutional networks for
+ *                          denoising Monte Carlo renderings"); the last layer starts at zero,
+ *                          a box filter.  The last layer has 32 outputs (27 used). */
+#define NSP_DENOISER_RESIDUAL 0
+#define NSP_DENOISER_KERNEL 1
+int nsp_denoiser_create_head(nsp_context* ctx, int in_channels, int residual_channel, int head, uint64_t seed,
+                             nsp_denoiser** out);
 void nsp_denoiser_destroy(nsp_denoiser* d);
 /* Parameters: per layer, weights [cout][27][cin] then bias [cout], with the
  * padded shapes cin = 16 (layer 0) or 32, cout = 32 or 16 (last layer). */

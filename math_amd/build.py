@@ -29,7 +29,6 @@ SOURCES = {
     "nsp_build.hip": ["-ffp-contract=off"],   # FP64 records bit-equal to the CPU checker
     "nsp_render.hip": [],
     "nsp_denoise.hip": [],
-    "nsp_batch.hip": [],
     "nsp_raster.hip": [],
     "nsp_api.cpp": ["-x", "hip"],
     "nsp_meshgen.cpp": ["-x", "hip"],

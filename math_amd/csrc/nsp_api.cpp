@@ -13,7 +13,6 @@
 #include "nsp.h"
 #include "nsp_build.hpp"
 #include "nsp_internal.hpp"
-#include "nsp_batch.hpp"
 #include "nsp_raster.hpp"
 #include "nsp_render.hpp"
 
@@ -785,12 +784,28 @@ int nsp_set_counters(nsp_scene* scene, int enable)
 
 }  // extern "C"
 
+// A batch of scenes painted by one launch: every scene keeps its own pass
+// state (batch list, batch index, ray totals); the scenes are spread over
+// `lanes` concurrent streams, each with its own workspace, by longest
+// processing time first on pixels x samples; each lane runs its scenes'
+// wavefront sequences back to back, and the whole batch is captured into one
+// hipGraph (fork and join through events on the context's stream).
+struct BatchScene {
+    DevBuf list, idx, totals;
+    SceneK<float> kf;
+    int64_t batch_pixels = 0, batches = 0;
+};
+
 struct nsp_batch {
     nsp_context* ctx;
     int device = 0;
+    int lanes = 4;
     std::vector<nsp_scene*> scenes;
-    MultiBuffers buf;
-    DevBuf slots, begins;
+    std::vector<BatchScene> per;
+    std::vector<std::unique_ptr<RenderBuffers>> work;
+    std::vector<hipStream_t> streams;
+    std::vector<hipEvent_t> joins;
+    hipEvent_t fork = nullptr;
     int64_t rays = 0;
     double last_s = 0;
 };
@@ -806,9 +821,11 @@ int nsp_batch_create(nsp_context* ctx, nsp_scene* const* scenes, int count, nsp_
         b->ctx = ctx;
         b->device = ctx->device;
         for (int i = 0; i < count; ++i) {
-            if (!scenes[i] || scenes[i]->ctx != ctx) raise(NSP_E_INVALID, "batch scene from another context");
+            if (!scenes[i] || scenes[i]->ctx != ctx) raise(NSP_E_INVALID, "batch scenes must belong to the context");
             b->scenes.push_back(scenes[i]);
         }
+        b->per.resize(count);
+        NSP_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
         *out = b.release();
     });
 }
@@ -818,7 +835,17 @@ void nsp_batch_destroy(nsp_batch* b)
     if (!b) return;
     (void)hipSetDevice(b->device);
     (void)hipDeviceSynchronize();
+    for (hipStream_t st : b->streams) (void)hipStreamDestroy(st);
+    for (hipEvent_t e : b->joins) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
     delete b;
+}
+
+int nsp_batch_set_lanes(nsp_batch* b, int lanes)
+{
+    if (!b || lanes < 1 || lanes > 64) return NSP_E_INVALID;
+    b->lanes = lanes;
+    return NSP_OK;
 }
 
 int nsp_batch_paint(nsp_batch* b, const nsp_paint_params* params, const volatile int32_t* stop)
@@ -830,76 +857,104 @@ int nsp_batch_paint(nsp_batch* b, const nsp_paint_params* params, const volatile
         if (p.precision != NSP_PRECISION_FP32) raise(NSP_E_UNSUPPORTED, "the batched launch traces in FP32");
         if (p.max_depth < 0 || p.max_depth > 250) raise(NSP_E_INVALID, "max_depth must be in [0, 250]");
         if (p.aov) raise(NSP_E_UNSUPPORTED, "AOVs are not produced by the batched launch");
+        if (p.primary != NSP_PRIMARY_AUTO && p.primary != NSP_PRIMARY_WALK) raise(NSP_E_INVALID, "unknown primary mode");
         launch_gate(ctx);
         hipStream_t s = ctx->stream;
         const int ns = (int)b->scenes.size();
-        std::vector<SceneSlot> slots(ns);
-        std::vector<int64_t> begins(ns + 1);
-        int64_t total = 0;
-        int Smax = 1;
+        const int lanes = std::min(b->lanes, ns);
+        while ((int)b->streams.size() < lanes) {
+            hipStream_t st;
+            NSP_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            b->streams.push_back(st);
+            hipEvent_t e;
+            NSP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            b->joins.push_back(e);
+            b->work.push_back(std::make_unique<RenderBuffers>());
+        }
+        const bool raster = p.primary == NSP_PRIMARY_AUTO;
+        const int64_t want = p.batch_paths > 0 ? p.batch_paths : ((int64_t)1 << 23);
+        // per scene: its launch constants, batch rule (that of nsp_paint) and pass state
+        std::vector<double> cost(ns);
         for (int i = 0; i < ns; ++i) {
             nsp_scene* sc = b->scenes[i];
-            SceneSlot& sl = slots[i];
-            std::memset(&sl, 0, sizeof sl);
-            fill_scene<float>(*sc, p, (int)sc->pass_count, sl.sc);
-            sl.n = sc->n;
-            sl.S = sl.sc.S;
-            sl.path_begin = total;
-            sl.pixels = sc->pixels;
-            sl.pix_sum = sc->pix_sum.as<float>();
-            sl.pix_cnt = sc->pix_cnt.as<int32_t>();
-            sl.A = make_tree_args(sc->mesh->m);
-            const MeshDev& m = sc->mesh->m;
-            sl.M = MeshShadeArgs{m.normal32.as<float>(), m.rec64.as<double>(),
-                                 m.has_normals ? m.vn64.as<double>() : nullptr,
-                                 m.has_normals ? m.fnrm.as<int32_t>() : nullptr, m.ntype.as<int32_t>()};
-            begins[i] = total;
-            total += sc->pixels * sl.S;
-            Smax = std::max(Smax, sl.S);
+            BatchScene& bs = b->per[i];
+            fill_scene<float>(*sc, p, (int)sc->pass_count, bs.kf);
+            SceneK<double> kd;
+            fill_scene<double>(*sc, p, (int)sc->pass_count, kd);
+            const int S = bs.kf.S;
+            bs.batch_pixels = std::min<int64_t>(std::max<int64_t>(1, want / S), sc->pixels);
+            std::vector<BatchDesc> descs;
+            for (int64_t px = 0; px < sc->pixels; px += bs.batch_pixels)
+                descs.push_back(BatchDesc{px, std::min(bs.batch_pixels, sc->pixels - px) * S});
+            bs.batches = (int64_t)descs.size();
+            bs.list.alloc(sizeof(BatchDesc) * descs.size());
+            bs.idx.alloc(sizeof(long long));
+            bs.totals.alloc(sizeof(unsigned long long) * 3);
+            NSP_HIP(hipMemcpyAsync(bs.list.p, descs.data(), sizeof(BatchDesc) * descs.size(), hipMemcpyHostToDevice,
+                                   s));
+            if (raster) raster_prepare(sc->raster, sc->mesh->m, kd, S, s);
+            cost[i] = (double)sc->pixels * S * std::log2(2.0 + (double)sc->mesh->m.F);
         }
-        begins[ns] = total;
-        const int64_t want = p.batch_paths > 0 ? p.batch_paths : ((int64_t)1 << 23);
-        const int64_t cap = std::max<int64_t>(want, Smax);
-        // batches: runs of whole pixels over the concatenated scenes
-        std::vector<BatchDesc> descs;
-        {
-            int64_t start = 0, cur = 0;
-            for (int i = 0; i < ns; ++i) {
-                const int64_t S = slots[i].S;
-                int64_t px = 0;
-                while (px < b->scenes[i]->pixels) {
-                    int64_t room = (cap - cur) / S;
-                    if (room == 0) {
-                        descs.push_back(BatchDesc{start, cur});
-                        start += cur;
-                        cur = 0;
-                        continue;
-                    }
-                    int64_t take = std::min(room, b->scenes[i]->pixels - px);
-                    px += take;
-                    cur += take * S;
-                }
+        // lanes by longest processing time first
+        std::vector<int> order(ns);
+        for (int i = 0; i < ns; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cost[a] > cost[c]; });
+        std::vector<std::vector<int>> lane_scenes(lanes);
+        std::vector<double> load(lanes, 0.0);
+        for (int i : order) {
+            int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            lane_scenes[l].push_back(i);
+            load[l] += cost[i];
+        }
+        for (int l = 0; l < lanes; ++l) {
+            int64_t cap = 1;
+            int nmax = 3;
+            for (int i : lane_scenes[l]) {
+                cap = std::max(cap, b->per[i].batch_pixels * b->per[i].kf.S);
+                nmax = std::max(nmax, b->scenes[i]->n);
             }
-            if (cur > 0) descs.push_back(BatchDesc{start, cur});
+            alloc_render_buffers(*b->work[l], nmax, NSP_PRECISION_FP32, cap, p.max_depth);
+            if (raster) b->work[l]->keys.alloc(sizeof(unsigned long long) * cap);
         }
-        alloc_multi_buffers(b->buf, cap, p.max_depth);
-        b->slots.alloc(sizeof(SceneSlot) * ns);
-        b->begins.alloc(sizeof(int64_t) * (ns + 1));
-        b->buf.batch_list.alloc(sizeof(BatchDesc) * descs.size());
-        NSP_HIP(hipMemcpyAsync(b->slots.p, slots.data(), sizeof(SceneSlot) * ns, hipMemcpyHostToDevice, s));
-        NSP_HIP(hipMemcpyAsync(b->begins.p, begins.data(), sizeof(int64_t) * (ns + 1), hipMemcpyHostToDevice, s));
-        NSP_HIP(hipMemcpyAsync(b->buf.batch_list.p, descs.data(), sizeof(BatchDesc) * descs.size(),
-                               hipMemcpyHostToDevice, s));
         NSP_HIP(hipStreamSynchronize(s));
+        if (stop && *stop) raise(NSP_E_STOPPED, "stopped");
         auto t0 = std::chrono::steady_clock::now();
-        multi_pass_reset(b->buf, s);
-        MultiLaunch ml{b->slots.as<SceneSlot>(), b->begins.as<int64_t>(), ns, &b->buf, p.max_depth};
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
+        auto enqueue = [&]() {
+            NSP_HIP(hipEventRecord(b->fork, s));
+            for (int l = 0; l < lanes; ++l) {
+                hipStream_t ls = b->streams[l];
+                NSP_HIP(hipStreamWaitEvent(ls, b->fork, 0));
+                for (int i : lane_scenes[l]) {
+                    nsp_scene* sc = b->scenes[i];
+                    BatchScene& bs = b->per[i];
+                    pass_reset_state(bs.totals.as<unsigned long long>(), bs.idx.as<long long>(), ls);
+                    RenderLaunch rl;
+                    rl.mesh = &sc->mesh->m;
+                    rl.scene = &bs.kf;
+                    rl.buf = b->work[l].get();
+                    rl.pix_sum = sc->pix_sum.as<float>();
+                    rl.pix_cnt = sc->pix_cnt.as<int32_t>();
+                    rl.pix_aov = nullptr;
+                    rl.pixels = sc->pixels;
+                    rl.max_depth = p.max_depth;
+                    rl.counters = false;
+                    rl.marks = nullptr;
+                    rl.raster = raster ? &sc->raster : nullptr;
+                    rl.batch_list = bs.list.as<BatchDesc>();
+                    rl.batch_idx = bs.idx.as<long long>();
+                    rl.totals = bs.totals.as<unsigned long long>();
+                    for (int64_t k = 0; k < bs.batches; ++k) render_batch(rl, ls);
+                }
+                NSP_HIP(hipEventRecord(b->joins[l], ls));
+                NSP_HIP(hipStreamWaitEvent(s, b->joins[l], 0));
+            }
+        };
         if (p.use_graph) {
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
             NSP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             try {
-                render_batch_multi(ml, s);
+                enqueue();
             } catch (...) {
                 hipGraph_t g2 = nullptr;
                 (void)hipStreamEndCapture(s, &g2);
@@ -907,29 +962,29 @@ int nsp_batch_paint(nsp_batch* b, const nsp_paint_params* params, const volatile
                 throw;
             }
             NSP_HIP(hipStreamEndCapture(s, &graph));
-            NSP_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (ie == hipSuccess) ie = hipGraphLaunch(exec, s);
+            if (ie == hipSuccess) ie = hipStreamSynchronize(s);
+            if (exec) (void)hipGraphExecDestroy(exec);
+            (void)hipGraphDestroy(graph);
+            NSP_HIP(ie);
+        } else {
+            enqueue();
+            NSP_HIP(hipStreamSynchronize(s));
         }
-        size_t done = 0;
-        for (; done < descs.size(); ++done) {
-            if (stop && *stop) break;
-            if (exec)
-                NSP_HIP(hipGraphLaunch(exec, s));
-            else
-                render_batch_multi(ml, s);
-        }
-        unsigned long long tot[3] = {0, 0, 0};
-        NSP_HIP(hipMemcpyAsync(tot, b->buf.totals.p, sizeof tot, hipMemcpyDeviceToHost, s));
-        NSP_HIP(hipStreamSynchronize(s));
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
         b->last_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        b->rays = (int64_t)(tot[0] + tot[1] + tot[2]);
-        if (done < descs.size()) raise(NSP_E_STOPPED, "stopped");
+        b->rays = 0;
         for (int i = 0; i < ns; ++i) {
             nsp_scene* sc = b->scenes[i];
+            unsigned long long tot[3] = {0, 0, 0};
+            NSP_HIP(hipMemcpyAsync(tot, b->per[i].totals.p, sizeof tot, hipMemcpyDeviceToHost, s));
+            NSP_HIP(hipStreamSynchronize(s));
+            const int64_t r = (int64_t)(tot[0] + tot[1] + tot[2]);
+            b->rays += r;
             sc->pass_count += 1;
             sc->pixel_count += sc->pixels;
-            sc->sample_count += sc->pixels * slots[i].S;
+            sc->ray_count += r;
+            sc->sample_count += sc->pixels * b->per[i].kf.S;
             sc->last_pass_s = b->last_s;
         }
     });

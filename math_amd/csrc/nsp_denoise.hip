@@ -37,7 +37,8 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 constexpr int DN_LAYERS = 8;
 constexpr int DN_WIDTH = 32;
 constexpr int DN_IN = 16;   // input channels, padded
-constexpr int DN_OUT = 16;  // last layer output channels, padded (3 used)
+constexpr int DN_OUT = 16;  // last layer output channels of the residual head, padded (3 used)
+constexpr int DN_KOUT = 32; // last layer output channels of the kernel head, padded (27 used)
 
 struct Dims {
     int nb, d, h, w;
@@ -307,6 +308,81 @@ __global__ void k_loss(const float* __restrict__ out, const float* __restrict__ 
     if (threadIdx.x == 0 && loss_part) loss_part[blockIdx.x] = sm[0];
 }
 
+// Kernel head: out[v][0..27) are the logits of a 3x3x3 filter; the prediction
+// is the softmax-weighted sum (over the taps inside the volume) of the 3
+// radiance channels rad[.][3] of the tap voxels.  With tgt: squared error
+// into loss_part and, with dout, the logits' gradient (bf16 [V][32]):
+// dL/dz_t = w_t (g_t - sum_s w_s g_s), g_t = dL/dpred . rad(tap t).
+__global__ void k_kp_loss(const float* __restrict__ out, const float* __restrict__ rad,
+                          const float* __restrict__ tgt, Dims D, float scale, unsigned short* __restrict__ dout,
+                          float* __restrict__ pred, float* __restrict__ loss_part)
+{
+    __shared__ float sm[256];
+    const int64_t V = D.voxels();
+    float l = 0;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x) {
+        const Vox p = decode(v, D);
+        float z[27];
+        int64_t nb[27];
+        float m = -INFINITY;
+#pragma unroll
+        for (int t = 0; t < 27; ++t) {
+            nb[t] = neighbour(p, t, D);
+            z[t] = out[v * DN_KOUT + t];
+            if (nb[t] >= 0) m = fmaxf(m, z[t]);
+        }
+        float sum = 0;
+#pragma unroll
+        for (int t = 0; t < 27; ++t) {
+            z[t] = nb[t] >= 0 ? expf(z[t] - m) : 0.0f;
+            sum += z[t];
+        }
+        const float inv = 1.0f / sum;
+        float pr[3] = {0, 0, 0};
+#pragma unroll
+        for (int t = 0; t < 27; ++t) {
+            z[t] *= inv;  // weight
+            if (nb[t] >= 0)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) pr[c] += z[t] * rad[nb[t] * 3 + c];
+        }
+        if (pred)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) pred[v * 3 + c] = pr[c];
+        if (tgt) {
+            float gp[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float e = pr[c] - tgt[v * 3 + c];
+                l += e * e;
+                gp[c] = 2.0f * e * scale;
+            }
+            if (dout) {
+                float g[27], wg = 0;
+#pragma unroll
+                for (int t = 0; t < 27; ++t) {
+                    g[t] = 0;
+                    if (nb[t] >= 0)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) g[t] += gp[c] * rad[nb[t] * 3 + c];
+                    wg += z[t] * g[t];
+                }
+#pragma unroll
+                for (int t = 0; t < 27; ++t) dout[v * DN_KOUT + t] = f2bf(nb[t] >= 0 ? z[t] * (g[t] - wg) : 0.0f);
+#pragma unroll
+                for (int t = 27; t < DN_KOUT; ++t) dout[v * DN_KOUT + t] = 0;
+            }
+        }
+    }
+    sm[threadIdx.x] = l;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && loss_part) loss_part[blockIdx.x] = sm[0];
+}
+
 // Adam (Kingma & Ba 2015) on the FP32 master parameters.
 __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
                        int64_t n, float lr, float b1, float b2, float eps, float c1, float c2)
@@ -404,7 +480,8 @@ struct nsp_denoiser {
     hipStream_t stream;
     std::string* err;
     int cin_real;
-    int resid_ch;
+    int resid_ch;  // residual head: the residual channels; kernel head: the radiance channels it filters
+    int head = NSP_DENOISER_RESIDUAL;
     int64_t nparams = 0;
     std::vector<Layer> layers;
     DevBuf params, grads, adam_m, adam_v;
@@ -429,7 +506,7 @@ void dn_alloc(nsp_denoiser& d, int64_t V)
     if (V <= d.cap_v) return;
     d.act[0].alloc(sizeof(unsigned short) * V * DN_IN);
     for (int l = 0; l < DN_LAYERS; ++l) d.act[l + 1].alloc(sizeof(unsigned short) * V * d.layers[l].cout);
-    d.out_f32.alloc(sizeof(float) * V * DN_OUT);
+    d.out_f32.alloc(sizeof(float) * V * d.layers[DN_LAYERS - 1].cout);
     d.resid.alloc(sizeof(float) * V * 3);
     d.target.alloc(sizeof(float) * V * 3);
     d.pred.alloc(sizeof(float) * V * 3);
@@ -539,9 +616,14 @@ float loss_and_grad(nsp_denoiser& d, Dims D, bool want_grad)
     const int64_t V = D.voxels();
     const int blocks = 1024;
     float scale = 1.0f / (float)(3 * V);
-    hipLaunchKernelGGL(k_loss, dim3(blocks), dim3(256), 0, s, d.out_f32.as<float>(), d.resid.as<float>(),
-                       d.target.as<float>(), V, scale, want_grad ? d.dy_a.as<unsigned short>() : nullptr,
-                       d.pred.as<float>(), d.loss_part.as<float>());
+    if (d.head == NSP_DENOISER_KERNEL)
+        hipLaunchKernelGGL(k_kp_loss, dim3(blocks), dim3(256), 0, s, d.out_f32.as<float>(), d.resid.as<float>(),
+                           d.target.as<float>(), D, scale, want_grad ? d.dy_a.as<unsigned short>() : nullptr,
+                           d.pred.as<float>(), d.loss_part.as<float>());
+    else
+        hipLaunchKernelGGL(k_loss, dim3(blocks), dim3(256), 0, s, d.out_f32.as<float>(), d.resid.as<float>(),
+                           d.target.as<float>(), V, scale, want_grad ? d.dy_a.as<unsigned short>() : nullptr,
+                           d.pred.as<float>(), d.loss_part.as<float>());
     NSP_HIP(hipGetLastError());
     std::vector<float> lp(blocks);
     NSP_HIP(hipMemcpyAsync(lp.data(), d.loss_part.p, sizeof(float) * blocks, hipMemcpyDeviceToHost, s));
@@ -557,13 +639,20 @@ extern "C" {
 
 int nsp_denoiser_create(nsp_context* ctx, int in_channels, int residual_channel, uint64_t seed, nsp_denoiser** out)
 {
+    return nsp_denoiser_create_head(ctx, in_channels, residual_channel, NSP_DENOISER_RESIDUAL, seed, out);
+}
+
+int nsp_denoiser_create_head(nsp_context* ctx, int in_channels, int residual_channel, int head, uint64_t seed,
+                             nsp_denoiser** out)
+{
     if (!ctx || !out) return NSP_E_INVALID;
     *out = nullptr;
     std::string* err = context_error(ctx);
     try {
         NSP_HIP(hipSetDevice(context_device(ctx)));
         if (in_channels < 1 || in_channels > DN_IN) raise(NSP_E_INVALID, "denoiser input channels must be in [1, 16]");
-        if (residual_channel >= 0 && residual_channel + 3 > in_channels)
+        if (head != NSP_DENOISER_RESIDUAL && head != NSP_DENOISER_KERNEL) raise(NSP_E_INVALID, "unknown head");
+        if (residual_channel + 3 > in_channels || (head == NSP_DENOISER_KERNEL && residual_channel < 0))
             raise(NSP_E_INVALID, "residual channels out of range");
         auto d = std::make_unique<nsp_denoiser>();
         d->ctx = ctx;
@@ -572,11 +661,12 @@ int nsp_denoiser_create(nsp_context* ctx, int in_channels, int residual_channel,
         d->err = err;
         d->cin_real = in_channels;
         d->resid_ch = residual_channel;
+        d->head = head;
         int64_t off = 0;
         for (int l = 0; l < DN_LAYERS; ++l) {
             Layer L;
             L.cin = l == 0 ? DN_IN : DN_WIDTH;
-            L.cout = l == DN_LAYERS - 1 ? DN_OUT : DN_WIDTH;
+            L.cout = l == DN_LAYERS - 1 ? (head == NSP_DENOISER_KERNEL ? DN_KOUT : DN_OUT) : DN_WIDTH;
             L.relu = l < DN_LAYERS - 1;
             L.w_off = off;
             off += (int64_t)L.cout * 27 * L.cin;
@@ -587,7 +677,8 @@ int nsp_denoiser_create(nsp_context* ctx, int in_channels, int residual_channel,
             d->layers.push_back(std::move(L));
         }
         d->nparams = off;
-        // He-uniform initialisation over the real fan-in; padded entries stay zero
+        // He-uniform initialisation over the real fan-in; padded entries stay zero.  The
+        // kernel head's last layer starts at zero: uniform weights, a 3x3x3 box filter.
         std::vector<float> p(off, 0.0f);
         uint64_t st = seed;
         auto next = [&]() {
@@ -600,7 +691,7 @@ int nsp_denoiser_create(nsp_context* ctx, int in_channels, int residual_channel,
         for (int l = 0; l < DN_LAYERS; ++l) {
             const Layer& L = d->layers[l];
             int ci_real = l == 0 ? in_channels : DN_WIDTH;
-            int co_real = l == DN_LAYERS - 1 ? 3 : DN_WIDTH;
+            int co_real = l == DN_LAYERS - 1 ? (head == NSP_DENOISER_KERNEL ? 0 : 3) : DN_WIDTH;
             double bound = std::sqrt(6.0 / (27.0 * ci_real)) * (l == DN_LAYERS - 1 ? 0.1 : 1.0);
             for (int co = 0; co < co_real; ++co)
                 for (int t = 0; t < 27; ++t)
@@ -687,9 +778,14 @@ int nsp_denoiser_forward(nsp_denoiser* d, const float* input, int nb, int depth,
                            d->cin_real, d->resid_ch, d->act[0].as<unsigned short>(), d->resid.as<float>());
         NSP_HIP(hipGetLastError());
         forward(*d, D);
-        hipLaunchKernelGGL(k_loss, dim3(1024), dim3(256), 0, s, d->out_f32.as<float>(), d->resid.as<float>(),
-                           (const float*)nullptr, V, 1.0f, (unsigned short*)nullptr, d->pred.as<float>(),
-                           (float*)nullptr);
+        if (d->head == NSP_DENOISER_KERNEL)
+            hipLaunchKernelGGL(k_kp_loss, dim3(1024), dim3(256), 0, s, d->out_f32.as<float>(), d->resid.as<float>(),
+                               (const float*)nullptr, D, 1.0f, (unsigned short*)nullptr, d->pred.as<float>(),
+                               (float*)nullptr);
+        else
+            hipLaunchKernelGGL(k_loss, dim3(1024), dim3(256), 0, s, d->out_f32.as<float>(), d->resid.as<float>(),
+                               (const float*)nullptr, V, 1.0f, (unsigned short*)nullptr, d->pred.as<float>(),
+                               (float*)nullptr);
         NSP_HIP(hipGetLastError());
         NSP_HIP(hipMemcpyAsync(output, d->pred.p, sizeof(float) * V * 3, hipMemcpyDeviceToHost, s));
         NSP_HIP(hipStreamSynchronize(s));

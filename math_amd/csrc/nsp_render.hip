@@ -172,6 +172,12 @@ __global__ void k_pass_reset(unsigned long long* totals, long long* batch_idx, u
     if (trav && i < ntrav) trav[i] = 0;
 }
 
+void pass_reset_state(unsigned long long* totals, long long* batch_idx, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pass_reset, dim3(1), dim3(64), 0, s, totals, batch_idx, (unsigned long long*)nullptr, 0);
+    NSP_HIP(hipGetLastError());
+}
+
 void pass_reset(RenderBuffers& b, bool counters, hipStream_t s)
 {
     hipLaunchKernelGGL(k_pass_reset, dim3(1), dim3(64), 0, s, b.totals.as<unsigned long long>(),
@@ -693,6 +699,9 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
     ShadowQueue<T> sq{b.sq[0].as<T>(), b.sq[1].as<T>(), b.sq[2].as<T>(), b.sq[3].as<int32_t>(), b.sq[4].as<float>()};
     unsigned long long* cnt = b.counters.as<unsigned long long>();
     const BatchDesc* desc = b.desc.as<BatchDesc>();
+    const BatchDesc* list = rl.batch_list ? rl.batch_list : b.batch_list.as<BatchDesc>();
+    long long* bidx = rl.batch_idx ? rl.batch_idx : b.batch_idx.as<long long>();
+    unsigned long long* totals = rl.totals ? rl.totals : b.totals.as<unsigned long long>();
     float* L = b.L.as<float>();
     float* aov = nullptr;
     if (rl.pix_aov) {
@@ -729,7 +738,7 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
         rl.marks->marks.emplace_back(stage, e0, e1);
     };
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(256), 0, s, cnt, CNT_STRIDE * (rl.max_depth + 2),
-                       b.batch_list.as<BatchDesc>(), b.batch_idx.as<long long>(), b.desc.as<BatchDesc>());
+                       list, bidx, b.desc.as<BatchDesc>());
     NSP_HIP(hipGetLastError());
     timed(0, [&] { hipLaunchKernelGGL((k_raygen<N, T>), dim3(g), dim3(TRACE_BLOCK), 0, s, sc, desc, qa, cap, L); });
     for (int depth = 0; depth <= rl.max_depth; ++depth) {
@@ -781,8 +790,7 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
         hipLaunchKernelGGL(k_accumulate, dim3(grid_for(cap / std::max(1, sc.S), 256, 4096)), dim3(256), 0, s, desc,
                            sc.S, L, cap, rl.pix_sum, rl.pix_cnt, rl.pixels, aov, rl.pix_aov, aov_channels(N));
     });
-    hipLaunchKernelGGL(k_batch_totals, dim3(1), dim3(64), 0, s, cnt, rl.max_depth + 1,
-                       b.totals.as<unsigned long long>(), b.batch_idx.as<long long>());
+    hipLaunchKernelGGL(k_batch_totals, dim3(1), dim3(64), 0, s, cnt, rl.max_depth + 1, totals, bidx);
     NSP_HIP(hipGetLastError());
 }
 

@@ -151,10 +151,16 @@ struct RenderLaunch {
     bool counters;
     StageMarks* marks;  // may be null
     const RasterPrep* raster = nullptr;  // FP32 only: rasterised depth-0 hits (nsp_raster.hpp), or null to walk
+    // a scene's own pass state when several scenes share one workspace (null: the workspace's)
+    const BatchDesc* batch_list = nullptr;
+    long long* batch_idx = nullptr;
+    unsigned long long* totals = nullptr;
 };
 void render_batch(const RenderLaunch& rl, hipStream_t s);
 // Clears the pass totals, the batch index and (when counting) the traversal counters.
 void pass_reset(RenderBuffers& b, bool counters, hipStream_t s);
+// Clears a scene's own pass totals (3 u64) and batch index.
+void pass_reset_state(unsigned long long* totals, long long* batch_idx, hipStream_t s);
 void alloc_render_buffers(RenderBuffers& b, int n, int precision, int64_t cap, int max_depth);
 
 }  // namespace nsp

@@ -193,6 +193,22 @@ __device__ __forceinline__ int fp32_test(const TreeArgs& A, const Comp& comp, co
 // LAY: the FP32 record layout (0 leaf-ordered SoA, 1 simplex-ordered AoS,
 // -1 read from A.layout).  DEFER: instead of the FP64 re-test, return
 // CAND_DEFER (the caller decides the candidate later).
+// The simplex-ordered FP32 record of simplex sid, read with 16-byte loads.
+template <int N>
+__device__ __forceinline__ void load_rec32(const TreeArgs& A, int sid, float* rec)
+{
+    constexpr int RS = rec32_stride(N);
+    const float4* r4 = reinterpret_cast<const float4*>(A.rec32 + (int64_t)sid * RS);
+#pragma unroll
+    for (int k = 0; k < RS / 4; ++k) {
+        const float4 v = r4[k];
+        rec[4 * k] = v.x;
+        rec[4 * k + 1] = v.y;
+        rec[4 * k + 2] = v.z;
+        rec[4 * k + 3] = v.w;
+    }
+}
+
 template <int N, typename T, int LAY = -1, bool DEFER = false>
 __device__ __forceinline__ int candidate(const TreeArgs& A, int64_t e, const T* o, const T* d, const double* o64,
                                          const double* d64, T tlimit, int best_idx, double* t_hit, int* idx,
@@ -215,15 +231,7 @@ __device__ __forceinline__ int candidate(const TreeArgs& A, int64_t e, const T* 
         const bool aos = LAY < 0 ? A.layout == 1 : LAY == 1;
         if (aos) {
             sid = A.entries[e];
-            const float4* r4 = reinterpret_cast<const float4*>(A.rec32 + (int64_t)sid * RS);
-#pragma unroll
-            for (int k = 0; k < RS / 4; ++k) {
-                float4 v = r4[k];
-                rec[4 * k] = v.x;
-                rec[4 * k + 1] = v.y;
-                rec[4 * k + 2] = v.z;
-                rec[4 * k + 3] = v.w;
-            }
+            load_rec32<N>(A, sid, rec);
         }
         const float* soa = A.soa;
         const int64_t pad = A.pad;
@@ -376,18 +384,12 @@ struct RayWalk {
         if (!slab<N, T>(blo, bsz, o, inv, d, &ta, &tb)) tb = tp;  // rounding: point located outside its own box
         if (node != last_leaf && nd.y > 0) {
             ++lvis;
-            for (int64_t e = nd.x; e < (int64_t)nd.x + nd.y; ++e) {
-                ++ntest;
-                double th;
-                int si;
-                bool nr = false;
-                T lim = ANY ? tmax : (T)best_t;
-                const int c = candidate<N, T, LAY, DEFER>(A, e, o, d, o64, d64, lim, best, &th, &si, &nr, &nret);
+            // outcome of one candidate test; true ends the walk
+            auto take = [&](int c, double th, int si, bool nr, int sid) -> bool {
                 near = near || nr;
                 if (DEFER && c == CAND_DEFER) {
                     near = true;
                     ++nret;  // decided in FP64 after the walk
-                    const int sid = A.entries[e];
                     bool known = false;
 #pragma unroll
                     for (int k = 0; k < PENDING; ++k) known = known || (k < npend && pend[k] == sid);
@@ -401,7 +403,7 @@ struct RayWalk {
                             if (k == npend) pend[k] = sid;
                         ++npend;
                     }
-                    continue;
+                    return false;
                 }
                 if (c == CAND_HIT) {
                     if (ANY) {
@@ -413,6 +415,45 @@ struct RayWalk {
                         best_t = th;
                         best = si;
                     }
+                }
+                return false;
+            };
+            const int64_t e1 = (int64_t)nd.x + nd.y;
+            if constexpr (LAY == 1 && sizeof(T) == 4) {
+                // two candidates at a time: both entry ids, then both records, are
+                // loaded before either is tested (two independent load chains)
+                constexpr int RS = rec32_stride(N);
+                for (int64_t e = nd.x; e < e1; e += 2) {
+                    const bool two = e + 1 < e1;
+                    const int sa = A.entries[e];
+                    const int sb = two ? A.entries[e + 1] : sa;
+                    float ra[RS], rb[RS];
+                    load_rec32<N>(A, sa, ra);
+                    if (two) load_rec32<N>(A, sb, rb);
+#pragma unroll
+                    for (int g = 0; g < 2; ++g) {
+                        if (g == 1 && !two) break;
+                        const int sid = g ? sb : sa;
+                        auto comp = [&](int cc) -> float { return g ? rb[cc] : ra[cc]; };
+                        auto get_sid = [&]() -> int { return sid; };
+                        ++ntest;
+                        double th;
+                        int si;
+                        bool nr = false;
+                        const int c = fp32_test<N, DEFER>(A, comp, get_sid, o, d, o64, d64, ANY ? tmax : (T)best_t,
+                                                          best, &th, &si, &nr, &nret);
+                        if (take(c, th, si, nr, sid)) return true;
+                    }
+                }
+            } else {
+                for (int64_t e = nd.x; e < e1; ++e) {
+                    ++ntest;
+                    double th;
+                    int si;
+                    bool nr = false;
+                    T lim = ANY ? tmax : (T)best_t;
+                    const int c = candidate<N, T, LAY, DEFER>(A, e, o, d, o64, d64, lim, best, &th, &si, &nr, &nret);
+                    if (take(c, th, si, nr, DEFER && c == CAND_DEFER ? A.entries[e] : -1)) return true;
                 }
             }
         }

@@ -25,7 +25,8 @@ from . import configs, native
 
 IN_CHANNELS = 16
 RESIDUAL = 3  # the 4-spp radiance channels
-LEVELS = (1, 2, 4, 8, 16)  # progressive levels stored for training
+LEVELS = (1, 2, 4, 8, 16, 32)  # progressive levels stored for training
+THRESHOLDS = (0.03, 0.02, 0.01)
 
 
 def variant(seed, screen=128, window=None, g=24):
@@ -138,10 +139,21 @@ def _mean_volume(scene, screen):
     return to_volume(s / np.maximum(c, 1)[:, None], screen), c
 
 
-def time_to_rmse(ctx, mesh, cfg, ref, denoiser, depth=5, threshold=0.01, max_passes=256, seed=77):
-    """Time for each method to reach RMSE <= threshold against `ref`.  Only GPU
-    work is timed (paint calls and the network forward); image read-back and
-    the RMSE evaluation between passes are not.
+def first_time(curve, threshold):
+    """First time of a (samples, seconds, rmse) curve at which rmse <= threshold, or None."""
+    for _, t, e in curve:
+        if e <= threshold:
+            return t
+    return None
+
+
+def time_to_rmse(ctx, mesh, cfg, ref, denoiser, depth=5, thresholds=THRESHOLDS, max_passes=256, max_levels=96,
+                 seed=77):
+    """Error-versus-time curves of both methods against `ref`, and the time at
+    which each first reaches each threshold.  Only GPU work is timed (paint
+    calls and the network forward); image read-back and the RMSE evaluation
+    between passes are not.  Each curve stops at the smallest threshold (or
+    after max_passes plain passes, max_levels denoised levels).
 
       plain     progressive 4-spp (8-sample) FP32 passes
       denoised  progressive levels of ProgressiveInputs (1 + 8 samples per
@@ -150,22 +162,21 @@ def time_to_rmse(ctx, mesh, cfg, ref, denoiser, depth=5, threshold=0.01, max_pas
                 time at a level is its passes plus one inference.
     """
     screen = cfg["screen"]
+    goal = min(thresholds)
     out = {}
     sc = native.Scene(ctx, mesh, cfg)
-    t, t_plain, curve = 0.0, None, []
+    t, curve = 0.0, []
     for p in range(max_passes):
         t += _paint_timed(sc, 4, depth, seed)
         img, c = _mean_volume(sc, screen)
         e = rmse(img, ref)
         curve.append((int(c[0]), t, e))
-        if e <= threshold:
-            t_plain = t
+        if e <= goal:
             break
     out["plain_curve"] = curve
-    out["plain_time_s"] = t_plain
     p = ProgressiveInputs(ctx, mesh, cfg, seed + 1, depth)
-    t_den, dcurve = None, []
-    for k in range(max_passes):
+    dcurve = []
+    for k in range(max_levels):
         p.advance()
         x = p.volume()
         t1 = time.perf_counter()
@@ -177,11 +188,13 @@ def time_to_rmse(ctx, mesh, cfg, ref, denoiser, depth=5, threshold=0.01, max_pas
             out["input4_rmse"] = rmse(x[..., 3:6], ref)
             out["denoiser_inference_s"] = dt
         # a render stopped at this level costs its passes and one inference
-        samples = 9 * p.level
-        dcurve.append((samples, p.secs + dt, e))
-        if e <= threshold:
-            t_den = p.secs + dt
+        dcurve.append((9 * p.level, p.secs + dt, e))
+        if e <= goal:
             break
     out["denoised_curve"] = dcurve
-    out["denoised_time_s"] = t_den
+    out["thresholds"] = {}
+    for thr in thresholds:
+        tp, td = first_time(curve, thr), first_time(dcurve, thr)
+        out["thresholds"][str(thr)] = dict(plain_s=tp, denoised_s=td,
+                                           speedup=(tp / td) if (tp is not None and td is not None) else None)
     return out

@@ -90,10 +90,11 @@ EXPORTS = [
     "nsp_mesh_tree_dump", "nsp_trace_first_hit", "nsp_trace_first_hit_device", "nsp_scene_create",
     "nsp_scene_destroy", "nsp_paint", "nsp_statistics", "nsp_image_get", "nsp_image_set", "nsp_image_aov",
     "nsp_scene_pixel_count", "nsp_last_pass_stats", "nsp_set_counters", "nsp_meshgen_torus",
-    "nsp_meshgen_cube_sphere", "nsp_denoiser_create", "nsp_denoiser_destroy", "nsp_denoiser_param_count",
+    "nsp_meshgen_cube_sphere", "nsp_denoiser_create", "nsp_denoiser_create_head", "nsp_denoiser_destroy",
+    "nsp_denoiser_param_count",
     "nsp_denoiser_get_params", "nsp_denoiser_set_params", "nsp_denoiser_forward", "nsp_denoiser_loss_grad",
     "nsp_denoiser_train", "nsp_conv3d_bf16", "nsp_batch_create", "nsp_batch_destroy", "nsp_batch_paint",
-    "nsp_batch_statistics", "nsp_autotune", "nsp_mesh_set_record_layout", "nsp_mesh_get_record_layout",
+    "nsp_batch_statistics", "nsp_batch_set_lanes", "nsp_autotune", "nsp_mesh_set_record_layout", "nsp_mesh_get_record_layout",
 ]
 
 _lib = None
@@ -402,6 +403,7 @@ def _denoiser_lib():
         P = C.c_void_p
         fp = C.POINTER(C.c_float)
         L.nsp_denoiser_create.argtypes = [P, C.c_int, C.c_int, C.c_uint64, C.POINTER(P)]
+        L.nsp_denoiser_create_head.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(P)]
         L.nsp_denoiser_destroy.argtypes = [P]
         L.nsp_denoiser_destroy.restype = None
         L.nsp_denoiser_param_count.argtypes = [P]
@@ -422,9 +424,10 @@ LAYERS = 8
 WIDTH = 32
 
 
-def layer_shapes():
-    """(cout, cin) padded shapes of the denoiser layers."""
-    return [(WIDTH if l < LAYERS - 1 else 16, 16 if l == 0 else WIDTH) for l in range(LAYERS)]
+def layer_shapes(head=0):
+    """(cout, cin) padded shapes of the denoiser layers (head: HEAD_RESIDUAL or HEAD_KERNEL)."""
+    last = 16 if head == 0 else 32
+    return [(WIDTH if l < LAYERS - 1 else last, 16 if l == 0 else WIDTH) for l in range(LAYERS)]
 
 
 def conv3d_bf16(ctx, x, w, bias=None, relu=False):
@@ -441,12 +444,17 @@ def conv3d_bf16(ctx, x, w, bias=None, relu=False):
     return y
 
 
+HEAD_RESIDUAL = 0
+HEAD_KERNEL = 1
+
+
 class Denoiser:
-    def __init__(self, ctx, in_channels, residual_channel=-1, seed=0):
+    def __init__(self, ctx, in_channels, residual_channel=-1, seed=0, head=HEAD_RESIDUAL):
         self.ctx = ctx
         L = _denoiser_lib()
         h = C.c_void_p()
-        ctx.check(L.nsp_denoiser_create(ctx.h, in_channels, residual_channel, seed, C.byref(h)))
+        ctx.check(L.nsp_denoiser_create_head(ctx.h, in_channels, residual_channel, head, seed, C.byref(h)))
+        self.head = head
         self.h = h
         self.cin = in_channels
         self.nparams = L.nsp_denoiser_param_count(h)
@@ -513,6 +521,7 @@ def _batch_lib():
         L.nsp_batch_destroy.restype = None
         L.nsp_batch_paint.argtypes = [P, C.POINTER(PaintParams), C.POINTER(C.c_int32)]
         L.nsp_batch_statistics.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+        L.nsp_batch_set_lanes.argtypes = [P, C.c_int]
         L.nsp_autotune.argtypes = [P, C.POINTER(PaintParams)]
         L._batch_ready = True
     return L
@@ -539,8 +548,11 @@ class Batch:
         except Exception:
             pass
 
-    def paint(self, spp, max_depth=5, seed=1, batch_paths=0, use_graph=False):
-        p = PaintParams(spp, max_depth, 0, PRECISION_FP32, 1, seed, batch_paths, int(use_graph), 0, 0)
+    def set_lanes(self, lanes):
+        self.ctx.check(_batch_lib().nsp_batch_set_lanes(self.h, int(lanes)))
+
+    def paint(self, spp, max_depth=5, seed=1, batch_paths=0, use_graph=False, primary=PRIMARY_AUTO):
+        p = PaintParams(spp, max_depth, 0, PRECISION_FP32, 1, seed, batch_paths, int(use_graph), 0, int(primary))
         stop = C.c_int32(0)
         self.ctx.check(_batch_lib().nsp_batch_paint(self.h, C.byref(p), C.byref(stop)))
 

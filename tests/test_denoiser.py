@@ -6,18 +6,19 @@ import pytest
 from oracle import denoise_ref as R
 
 
-def random_params(seed, scale=0.05):
+def random_params(seed, scale=0.05, head="residual", cin=13):
     rng = np.random.default_rng(seed)
+    used = 3 if head == "residual" else 27
     parts = []
-    for l, (co, ci) in enumerate(R.shapes()):
+    for l, (co, ci) in enumerate(R.shapes(head)):
         w = rng.normal(scale=scale, size=(co, 27, ci))
         if l == 0:
-            w[:, :, 13:] = 0
+            w[:, :, cin:] = 0
         if l == R.LAYERS - 1:
-            w[3:] = 0
+            w[used:] = 0
         b = rng.normal(scale=0.01, size=co)
         if l == R.LAYERS - 1:
-            b[3:] = 0
+            b[used:] = 0
         parts += [w.ravel(), b]
     return np.concatenate(parts)
 
@@ -45,6 +46,46 @@ def test_checker_gradient_matches_finite_differences():
         fd = (R.loss_grad(pp, x, t, 3)[0] - R.loss_grad(pm, x, t, 3)[0]) / (2 * h)
         # central differences: truncation O(h^2 f''') and cancellation O(eps f / h)
         assert abs(fd - g[k]) <= 2e-5 * max(1e-3, abs(g[k])) + 1e-8, (k, fd, g[k])
+
+
+def test_checker_kernel_head_gradient_matches_finite_differences():
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=(1, 3, 4, 3, 16))
+    t = rng.normal(size=(1, 3, 4, 3, 3))
+    p = random_params(5, scale=0.05, head="kernel", cin=16)
+    loss, g = R.loss_grad(p, x, t, 3, head="kernel")
+    offs = np.cumsum([0] + [co * 27 * ci + co for co, ci in R.shapes("kernel")])
+    idx = []
+    for l, (co, ci) in enumerate(R.shapes("kernel")):
+        cr = 27 if l == R.LAYERS - 1 else co
+        for _ in range(3):
+            o, tp, i = rng.integers(cr), rng.integers(27), rng.integers(ci)
+            idx.append(offs[l] + (o * 27 + tp) * ci + i)
+        idx.append(offs[l] + co * 27 * ci + rng.integers(cr))  # a bias
+    h = 1e-6
+    nonzero = 0
+    for k in idx:
+        pp, pm = p.copy(), p.copy()
+        pp[k] += h
+        pm[k] -= h
+        fd = (R.loss_grad(pp, x, t, 3, head="kernel")[0] - R.loss_grad(pm, x, t, 3, head="kernel")[0]) / (2 * h)
+        nonzero += abs(g[k]) > 1e-9
+        assert abs(fd - g[k]) <= 2e-5 * max(1e-3, abs(g[k])) + 1e-8, (k, fd, g[k])
+    assert nonzero >= len(idx) // 2
+
+
+def test_checker_kernel_head_closed_forms():
+    # zero logits: a box filter over the in-volume taps; a dominant centre logit: the identity
+    rng = np.random.default_rng(6)
+    rad = rng.normal(size=(1, 3, 4, 5, 3))
+    views, valid = R._tap_views(rad)
+    logits = np.zeros((1, 3, 4, 5, 32))
+    w = R.kernel_weights(logits, valid)
+    box = (views * valid[..., None]).sum(axis=0) / valid.sum(axis=0)[..., None]
+    assert np.allclose(np.einsum("t...,t...c->...c", w, views), box)
+    logits[..., 13] = 200.0
+    w = R.kernel_weights(logits, valid)
+    assert np.allclose(np.einsum("t...,t...c->...c", w, views), rad)
 
 
 def test_checker_conv_closed_forms():
@@ -94,6 +135,31 @@ def test_denoiser_forward_and_gradient_match_checker(ctx):
         a, b = g[offs[l]:offs[l + 1]], rg[offs[l]:offs[l + 1]]
         errs.append(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
     print(f"forward rel err {ferr:.2e}, loss {loss:.6f} vs {rl:.6f}, grad rel err per layer {np.round(errs, 4)}")
+    assert ferr < 2e-2
+    assert abs(loss - rl) <= 2e-2 * rl
+    assert max(errs) < 5e-2
+
+
+@pytest.mark.gpu
+def test_denoiser_kernel_head_matches_checker(ctx):
+    from math_amd import native
+    rng = np.random.default_rng(9)
+    d = native.Denoiser(ctx, 16, residual_channel=3, seed=12, head=native.HEAD_KERNEL)
+    p = random_params(13, scale=0.05, head="kernel", cin=16).astype(np.float32)
+    d.set_params(p)
+    x = rng.normal(size=(2, 5, 6, 7, 16)).astype(np.float32)
+    t = rng.normal(size=(2, 5, 6, 7, 3)).astype(np.float32)
+    y = d.forward(x)
+    ref, _ = R.forward(p, x, 3, emulate_bf16=True, head="kernel")
+    ferr = np.max(np.abs(y - ref)) / np.max(np.abs(ref))
+    loss, g = d.loss_grad(x, t)
+    rl, rg = R.loss_grad(p, x, t, 3, emulate_bf16=True, head="kernel")
+    offs = np.cumsum([0] + [co * 27 * ci + co for co, ci in R.shapes("kernel")])
+    errs = []
+    for l in range(R.LAYERS):
+        a, b = g[offs[l]:offs[l + 1]], rg[offs[l]:offs[l + 1]]
+        errs.append(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+    print(f"kernel head: forward rel err {ferr:.2e}, loss {loss:.6f} vs {rl:.6f}, grad rel err {np.round(errs, 4)}")
     assert ferr < 2e-2
     assert abs(loss - rl) <= 2e-2 * rl
     assert max(errs) < 5e-2
