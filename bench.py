@@ -197,16 +197,21 @@ def batch_bench(args):
     per_rays = sum(s.statistics()["rays"] for s in per) - r0
     bat = [native.Scene(ctx, m, c) for m, c, _ in meshes]
     b = native.Batch(ctx, bat)
-    b.set_lanes(args.lanes)
-    b.paint(4, 5, seed=1, use_graph=True)  # warm-up
+    paths = 0
+    if args.lanes > 0:
+        b.set_lanes(args.lanes)
+    else:
+        paths = b.autotune(4, 5, seed=1)  # not timed; images are restored
+        log(f"autotuned batch: {b.lanes} lanes, {paths} paths per wavefront batch")
+    b.paint(4, 5, seed=1, use_graph=True, batch_paths=paths)  # warm-up
     for s in bat:
         s.set_image(*[np.zeros_like(a) for a in s.image()])
     t0 = time.perf_counter()
-    b.paint(4, 5, seed=1, use_graph=True)
+    b.paint(4, 5, seed=1, use_graph=True, batch_paths=paths)
     bat_s = time.perf_counter() - t0
     bat_rays = b.statistics()["rays"]
     out = {"metric": "C4 batched launch (one hipGraph, concurrent lanes), Mrays/s", "scenes": args.scenes,
-           "lanes": args.lanes,
+           "lanes": b.lanes, "batch_paths": paths or (1 << 23), "autotuned": args.lanes <= 0,
            "per_scene": {"seconds": round(per_s, 3), "Mrays_s": round(per_rays / per_s / 1e6, 2)},
            "batched": {"seconds": round(bat_s, 3), "Mrays_s": round(bat_rays / bat_s / 1e6, 2)},
            "speedup": round(per_s / bat_s, 3)}
@@ -240,7 +245,7 @@ def main():
     ap.add_argument("--batch-scenes", action="store_true",
                     help="C4: batched persistent launch over many scenes vs one launch sequence per scene")
     ap.add_argument("--scenes", type=int, default=256)
-    ap.add_argument("--lanes", type=int, default=4)
+    ap.add_argument("--lanes", type=int, default=0, help="batch lanes (0: autotune lanes and queue size)")
     args = ap.parse_args()
     if args.denoise:
         return denoise_bench(args)

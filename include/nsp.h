@@ -268,6 +268,12 @@ int nsp_batch_statistics(const nsp_batch* batch, int64_t* ray_count, double* las
  * scene's first pixels (images do not depend on the choice); writes the
  * winner into params->batch_paths. */
 int nsp_autotune(nsp_scene* scene, nsp_paint_params* params);
+/* Chooses the lane count of a batch (kept in the batch) and its queue size
+ * (written into params->batch_paths) by timing whole batched passes; every
+ * scene's image and statistics are restored afterwards. */
+int nsp_batch_autotune(nsp_batch* batch, nsp_paint_params* params);
+/* The batch's current lane count. */
+int nsp_batch_get_lanes(const nsp_batch* batch);
 
 /* Learned denoiser for 3D screens (n = 4 renders).  No reference counterpart
  * (the reference only renders progressively).  8 layers of 3x3x3

@@ -841,6 +841,8 @@ void nsp_batch_destroy(nsp_batch* b)
     delete b;
 }
 
+int nsp_batch_get_lanes(const nsp_batch* b) { return b ? b->lanes : NSP_E_INVALID; }
+
 int nsp_batch_set_lanes(nsp_batch* b, int lanes)
 {
     if (!b || lanes < 1 || lanes > 64) return NSP_E_INVALID;
@@ -1036,6 +1038,81 @@ int nsp_autotune(nsp_scene* scene, nsp_paint_params* params)
     if (rc2 != NSP_OK) return rc2;
     params->batch_paths = best;
     (void)ctx;
+    return NSP_OK;
+}
+
+// Batched launch: lanes first (2, 4, 8 at the default queue size), then the
+// queue size (2^22, 2^23 paths) at the best lane count, each timed on one
+// whole batched pass; every scene's image and statistics are restored.
+int nsp_batch_autotune(nsp_batch* b, nsp_paint_params* params)
+{
+    if (!b || !params) return NSP_E_INVALID;
+    const int ns = (int)b->scenes.size();
+    struct Saved {
+        std::vector<float> sum;
+        std::vector<int32_t> cnt;
+        int64_t passes, pixels, rays, samples;
+        double last;
+    };
+    std::vector<Saved> saved(ns);
+    for (int i = 0; i < ns; ++i) {
+        nsp_scene* sc = b->scenes[i];
+        Saved& sv = saved[i];
+        sv.sum.resize((size_t)3 * sc->pixels);
+        sv.cnt.resize(sc->pixels);
+        int rc = nsp_image_get(sc, sv.sum.data(), sv.cnt.data());
+        if (rc != NSP_OK) return rc;
+        sv.passes = sc->pass_count;
+        sv.pixels = sc->pixel_count;
+        sv.rays = sc->ray_count;
+        sv.samples = sc->sample_count;
+        sv.last = sc->last_pass_s;
+    }
+    nsp_paint_params p = *params;
+    p.passes = 1;
+    int rc = NSP_OK;
+    auto timed = [&](int lanes, int64_t paths) -> double {
+        b->lanes = lanes;
+        p.batch_paths = paths;
+        auto t0 = std::chrono::steady_clock::now();
+        rc = nsp_batch_paint(b, &p, nullptr);
+        return rc == NSP_OK ? std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() : 1e300;
+    };
+    int best_lanes = 4;
+    int64_t best_paths = (int64_t)1 << 23;
+    double best_s = 1e300;
+    for (int lanes : {2, 4, 8}) {
+        if (lanes > ns && lanes != 2) continue;
+        double t = timed(lanes, (int64_t)1 << 23);
+        if (rc != NSP_OK) break;
+        if (t < best_s) {
+            best_s = t;
+            best_lanes = lanes;
+        }
+    }
+    if (rc == NSP_OK) {
+        double t = timed(best_lanes, (int64_t)1 << 22);
+        if (rc == NSP_OK && t < best_s) {
+            best_s = t;
+            best_paths = (int64_t)1 << 22;
+        }
+    }
+    int rc2 = NSP_OK;
+    for (int i = 0; i < ns; ++i) {
+        nsp_scene* sc = b->scenes[i];
+        const Saved& sv = saved[i];
+        int r = nsp_image_set(sc, sv.sum.data(), sv.cnt.data());
+        if (r != NSP_OK) rc2 = r;
+        sc->pass_count = sv.passes;
+        sc->pixel_count = sv.pixels;
+        sc->ray_count = sv.rays;
+        sc->sample_count = sv.samples;
+        sc->last_pass_s = sv.last;
+    }
+    if (rc != NSP_OK) return rc;
+    if (rc2 != NSP_OK) return rc2;
+    b->lanes = best_lanes;
+    params->batch_paths = best_paths;
     return NSP_OK;
 }
 

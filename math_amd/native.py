@@ -94,7 +94,7 @@ EXPORTS = [
     "nsp_denoiser_param_count",
     "nsp_denoiser_get_params", "nsp_denoiser_set_params", "nsp_denoiser_forward", "nsp_denoiser_loss_grad",
     "nsp_denoiser_train", "nsp_conv3d_bf16", "nsp_batch_create", "nsp_batch_destroy", "nsp_batch_paint",
-    "nsp_batch_statistics", "nsp_batch_set_lanes", "nsp_autotune", "nsp_mesh_set_record_layout", "nsp_mesh_get_record_layout",
+    "nsp_batch_statistics", "nsp_batch_set_lanes", "nsp_batch_get_lanes", "nsp_batch_autotune", "nsp_autotune", "nsp_mesh_set_record_layout", "nsp_mesh_get_record_layout",
 ]
 
 _lib = None
@@ -522,6 +522,8 @@ def _batch_lib():
         L.nsp_batch_paint.argtypes = [P, C.POINTER(PaintParams), C.POINTER(C.c_int32)]
         L.nsp_batch_statistics.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
         L.nsp_batch_set_lanes.argtypes = [P, C.c_int]
+        L.nsp_batch_get_lanes.argtypes = [P]
+        L.nsp_batch_autotune.argtypes = [P, C.POINTER(PaintParams)]
         L.nsp_autotune.argtypes = [P, C.POINTER(PaintParams)]
         L._batch_ready = True
     return L
@@ -560,6 +562,16 @@ class Batch:
         r, t = C.c_int64(), C.c_double()
         self.ctx.check(_batch_lib().nsp_batch_statistics(self.h, C.byref(r), C.byref(t)))
         return dict(rays=r.value, last_pass_s=t.value)
+
+    @property
+    def lanes(self):
+        return _batch_lib().nsp_batch_get_lanes(self.h)
+
+    def autotune(self, spp, max_depth=5, seed=1, use_graph=True):
+        """Lane count (kept in the batch) and queue size (returned) chosen by timing; images are restored."""
+        p = PaintParams(spp, max_depth, 0, PRECISION_FP32, 1, seed, 0, int(use_graph), 0, 0)
+        self.ctx.check(_batch_lib().nsp_batch_autotune(self.h, C.byref(p)))
+        return p.batch_paths
 
 
 def autotune(scene, spp, max_depth=5, seed=1):

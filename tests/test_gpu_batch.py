@@ -81,3 +81,18 @@ def test_autotune_restores_state_and_choice_is_image_invariant(ctx):
     t2 = native.Scene(ctx, mesh, cfg)
     t2.paint(4, 3, seed=5, batch_paths=1 << 20)
     assert all(np.array_equal(p, q) for p, q in zip(t1.image(), t2.image()))
+
+
+def test_batch_autotune_restores_state(ctx):
+    specs = [(3, 6, 20, 1), (4, 3, 8, 2), (5, 2, 5, 3)]
+    meshes = [scene_of(ctx, *s) for s in specs]
+    scenes = [native.Scene(ctx, m, c) for m, c in meshes]
+    b = native.Batch(ctx, scenes)
+    b.paint(4, 3, seed=2)
+    before = [s.image() for s in scenes]
+    stats = [s.statistics() for s in scenes]
+    paths = b.autotune(4, 3, seed=2)
+    assert b.lanes in (2, 4, 8) and paths in (1 << 22, 1 << 23)
+    for s, img, st in zip(scenes, before, stats):
+        assert all(np.array_equal(p, q) for p, q in zip(img, s.image()))
+        assert s.statistics() == st
