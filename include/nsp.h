@@ -71,6 +71,46 @@ void nsp_mesh_destroy(nsp_mesh* mesh);
  * single_object_scene). */
 int nsp_mesh_min_max(const nsp_mesh* mesh, double* min, double* max);
 
+/* OBJ / OBJN files (host only).  nsp_obj_read replaces load_from_obj_file<N>
+ * and FileObj<N> (src/obj/file/file_load.cpp:946-1144) with the dimension
+ * rule of obj_file_dimension_and_type (src/obj/file/obj_file.cpp:144-199):
+ * extension "obj" is n = 3, "objN" is n = N.  Lines v / vt / vn / f are read
+ * (vn normalised); a face line of G vertex groups becomes G - (n - 1) facets
+ * (group 0 with groups i+1 .. i+n-1; at most 5 facets for n = 3, 1 otherwise);
+ * indices are 1-based or negative (relative to the items read before the
+ * line); zero indices, texture or normal indices given on only some groups and
+ * lines with fewer than n groups are errors.  After reading (file_load.cpp:
+ * 1402-1430): no facets, an index out of range or a facet bounding box of
+ * zero extent along an axis are errors, and for n = 3 facets whose edge
+ * vectors are parallel are dropped.  The file is read in `threads` line
+ * chunks in parallel (<= 0: the host's threads, chunks of at least 64 KiB).
+ * Errors return NSP_E_INVALID with the message in nsp_obj_last_error(). */
+typedef struct nsp_obj nsp_obj;
+int nsp_obj_read(const char* path, int threads, nsp_obj** out);
+void nsp_obj_destroy(nsp_obj* obj);
+/* Last error message of nsp_obj_read / nsp_obj_write on the calling thread. */
+const char* nsp_obj_last_error(void);
+/* Counts of the file's items (Obj<N>::vertices(), facets(), normals(),
+ * texcoords(), src/obj/obj.h:72-80); any pointer may be NULL. */
+int nsp_obj_info(const nsp_obj* obj, int* n, int64_t* vertices, int64_t* facets, int64_t* normals,
+                 int64_t* texcoords);
+/* Copies the arrays in the layout nsp_mesh_create takes (0-based indices;
+ * facet_normals -1 where a facet has no normals); any pointer may be NULL. */
+int nsp_obj_copy(const nsp_obj* obj, float* vertices, int32_t* facets, float* normals, int32_t* facet_normals);
+/* nsp_mesh_create on the arrays of a read file (Mesh<N,T>(obj, matrix, ...),
+ * src/painter/shapes/mesh.h:66). */
+int nsp_mesh_create_obj(nsp_context* ctx, const nsp_obj* obj, const double* vertex_matrix, nsp_mesh** out);
+/* save_obj_geometry_to_file (src/obj/file/file_save.cpp:36-306): an optional
+ * "# " comment (NULL: none), all vertices scaled into [-1, 1] about the centre
+ * of the bounding box of the facets' vertices (proportions kept), "%12.9f"
+ * coordinates, the normals normalised, facets "f a b c", or
+ * "f a//na b//nb c//nc" when all of a facet's normal indices are >= 0 (for
+ * n = 3 vertices 1 and 2 swap places when all three vertex normals point
+ * against the facet's edge cross product).  The extension must be "obj"
+ * (n = 3) or "objN" (N = n). */
+int nsp_obj_write(const char* path, int n, const float* vertices, int64_t V, const int32_t* facets, int64_t F,
+                  const float* normals, int64_t VN, const int32_t* facet_normals, const char* comment);
+
 typedef struct {
     int n;
     int64_t facets;

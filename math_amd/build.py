@@ -32,6 +32,7 @@ SOURCES = {
     "nsp_raster.hip": [],
     "nsp_api.cpp": ["-x", "hip"],
     "nsp_meshgen.cpp": ["-x", "hip"],
+    "nsp_obj.cpp": ["-ffp-contract=off"],   # float operations as written (file values)
 }
 
 

@@ -303,6 +303,10 @@ int nsp_mesh_create(nsp_context* ctx, int n, const float* vertices, int64_t V, c
         NSP_HIP(hipMemcpyAsync(m.fac.p, facets, sizeof(int32_t) * F * n, hipMemcpyHostToDevice, s));
         std::vector<double> vn;
         if (normals && facet_normals && VN > 0) {
+            // -1 marks a facet vertex without a normal (such facets are shaded flat)
+            for (int64_t k = 0; k < F * n; ++k)
+                if (facet_normals[k] < -1 || facet_normals[k] >= VN)
+                    raise(NSP_E_INVALID, "facet normal index out of range");
             m.has_normals = true;
             m.VN = VN;
             vn.resize((size_t)VN * n);

@@ -95,6 +95,8 @@ EXPORTS = [
     "nsp_denoiser_get_params", "nsp_denoiser_set_params", "nsp_denoiser_forward", "nsp_denoiser_loss_grad",
     "nsp_denoiser_train", "nsp_conv3d_bf16", "nsp_batch_create", "nsp_batch_destroy", "nsp_batch_paint",
     "nsp_batch_statistics", "nsp_batch_set_lanes", "nsp_batch_get_lanes", "nsp_batch_autotune", "nsp_autotune", "nsp_mesh_set_record_layout", "nsp_mesh_get_record_layout",
+    "nsp_obj_read", "nsp_obj_destroy", "nsp_obj_last_error", "nsp_obj_info", "nsp_obj_copy", "nsp_mesh_create_obj",
+    "nsp_obj_write",
 ]
 
 _lib = None
@@ -146,6 +148,15 @@ def lib():
                                         ip, i64p]
         L.nsp_meshgen_cube_sphere.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_uint64, fp,
                                               i64p, ip, i64p]
+        L.nsp_obj_read.argtypes = [C.c_char_p, C.c_int, C.POINTER(P)]
+        L.nsp_obj_destroy.argtypes = [P]
+        L.nsp_obj_destroy.restype = None
+        L.nsp_obj_last_error.argtypes = []
+        L.nsp_obj_last_error.restype = C.c_char_p
+        L.nsp_obj_info.argtypes = [P, C.POINTER(C.c_int), i64p, i64p, i64p, i64p]
+        L.nsp_obj_copy.argtypes = [P, fp, ip, fp, ip]
+        L.nsp_mesh_create_obj.argtypes = [P, P, dp, C.POINTER(P)]
+        L.nsp_obj_write.argtypes = [C.c_char_p, C.c_int, fp, C.c_int64, ip, C.c_int64, fp, C.c_int64, ip, C.c_char_p]
         _lib = L
     return _lib
 
@@ -187,6 +198,63 @@ def meshgen_cube_sphere(n, g, sphere=True, sigma=0.0, cap_angle=0.0, seed=0):
 
 
 # ---------------------------------------------------------------------------
+# OBJ / OBJN files (host only)
+
+class ObjFile:
+    """A read OBJ / OBJN file: n, vertices [V, n] float32, facets [F, n] int32
+    (0-based), normals [VN, n] float32, facet_normals [F, n] int32 (-1: none),
+    texcoords (count of vt lines).  The handle feeds Mesh.from_obj."""
+
+    def __init__(self, path, threads=0):
+        L = lib()
+        h = C.c_void_p()
+        rc = L.nsp_obj_read(os.fsencode(path), int(threads), C.byref(h))
+        if rc:
+            raise NspError(rc, L.nsp_obj_last_error().decode())
+        self._L = L
+        self.h = h
+        n = C.c_int()
+        V, F, VN, VT = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        L.nsp_obj_info(h, C.byref(n), C.byref(V), C.byref(F), C.byref(VN), C.byref(VT))
+        self.n = n.value
+        self.texcoords = VT.value
+        self.vertices = np.zeros((V.value, self.n), np.float32)
+        self.facets = np.zeros((F.value, self.n), np.int32)
+        self.normals = np.zeros((VN.value, self.n), np.float32)
+        self.facet_normals = np.zeros((F.value, self.n), np.int32)
+        L.nsp_obj_copy(h, _ptr(self.vertices, C.c_float), _ptr(self.facets, C.c_int32),
+                       _ptr(self.normals, C.c_float), _ptr(self.facet_normals, C.c_int32))
+
+    def close(self):
+        if self.h:
+            self._L.nsp_obj_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def read_obj(path, threads=0):
+    return ObjFile(path, threads)
+
+
+def write_obj(path, vertices, facets, normals=None, facet_normals=None, comment=None):
+    L = lib()
+    v = np.ascontiguousarray(vertices, np.float32)
+    f = np.ascontiguousarray(facets, np.int32)
+    nrm = None if normals is None else np.ascontiguousarray(normals, np.float32)
+    fn = None if facet_normals is None else np.ascontiguousarray(facet_normals, np.int32)
+    rc = L.nsp_obj_write(os.fsencode(path), v.shape[1], _ptr(v, C.c_float), v.shape[0], _ptr(f, C.c_int32),
+                         f.shape[0], _ptr(nrm, C.c_float), 0 if nrm is None else nrm.shape[0], _ptr(fn, C.c_int32),
+                         None if comment is None else comment.encode())
+    if rc:
+        raise NspError(rc, L.nsp_obj_last_error().decode())
+
+
+# ---------------------------------------------------------------------------
 
 class Context:
     def __init__(self, device=0):
@@ -218,6 +286,21 @@ class Context:
 
 
 class Mesh:
+    @classmethod
+    def from_obj(cls, ctx, obj, matrix=None):
+        """Mesh of a read OBJ / OBJN file (an ObjFile, or a path to read)."""
+        if not isinstance(obj, ObjFile):
+            obj = ObjFile(obj)
+        mat = None if matrix is None else np.ascontiguousarray(matrix, np.float64)
+        h = C.c_void_p()
+        ctx.check(ctx._L.nsp_mesh_create_obj(ctx.h, obj.h, _ptr(mat, C.c_double), C.byref(h)))
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.n = obj.n
+        self.h = h
+        self.F = obj.facets.shape[0]
+        return self
+
     def __init__(self, ctx, vertices, facets, normals=None, facet_normals=None, matrix=None):
         self.ctx = ctx
         L = ctx._L
