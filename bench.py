@@ -159,6 +159,62 @@ def denoise_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def obj_bench(args):
+    """.objN ingest: the C1 mesh written as an OBJ4 file, read back with the
+    library's chunked reader (all host threads, and one), and a mesh built on
+    the GPU from the read file; the oracle's line-by-line reader timed on a
+    bounded prefix of the same file."""
+    import os
+    import tempfile
+
+    import numpy as np
+
+    from math_amd import configs, native
+    from oracle import obj_ref
+
+    v, f, _, _, _ = configs.config("C1")
+    out = {"metric": ".obj4 ingest of the C1 mesh", "unit": "MB/s"}
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "c1.obj4")
+        t0 = time.perf_counter()
+        native.write_obj(path, v, f)
+        out["write_s"] = time.perf_counter() - t0
+        size = os.path.getsize(path)
+        out["file_mb"] = size / 1e6
+        for threads in (0, 1):
+            best = None
+            for _ in range(3):
+                t0 = time.perf_counter()
+                obj = native.read_obj(path, threads)
+                dt = time.perf_counter() - t0
+                best = dt if best is None else min(best, dt)
+            key = "all_threads" if threads == 0 else "one_thread"
+            out[key] = {"seconds": best, "mb_per_s": size / 1e6 / best}
+            log(f"read ({key}): {best * 1e3:.1f} ms, {size / 1e6 / best:.0f} MB/s")
+        assert obj.facets.shape == f.shape
+        ctx = native.Context(0) if not args.no_gpu_mesh else None
+        if ctx is not None:
+            t0 = time.perf_counter()
+            m = native.Mesh.from_obj(ctx, obj)
+            out["mesh_from_obj_s"] = time.perf_counter() - t0
+            info = m.info()
+            out["mesh_build_ms"] = {"basis": info.basis_ms, "tree": info.tree_ms}
+            m.close()
+        # the oracle on a bounded sample: the first facets of the mesh with the
+        # vertices they use
+        used, inv = np.unique(f[:args.obj_sample_facets], return_inverse=True)
+        sample = os.path.join(d, "sample.obj4")
+        native.write_obj(sample, v[used], inv.reshape(-1, f.shape[1]).astype(np.int32))
+        ssize = os.path.getsize(sample)
+        t0 = time.perf_counter()
+        obj_ref.read(sample)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": ssize / 1e6 / dt, "unit": "MB/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle line-by-line reader on the first {args.obj_sample_facets} facets "
+                                         f"of the mesh and their vertices ({ssize / 1e6:.1f} MB)"}
+    print(json.dumps(out), flush=True)
+
+
 def batch_bench(args):
     """C4: many scenes of uneven size and dimension, 4 spp, FP32: one batched
     launch sequence vs one launch sequence per scene."""
@@ -237,11 +293,14 @@ def main():
                     help="C3: train the denoiser on renders of n=4 variants, then time to RMSE<=0.01, "
                          "denoised vs plain")
     ap.add_argument("--train-variants", type=int, default=32)
-    ap.add_argument("--train-steps", type=int, default=6000)
-    ap.add_argument("--eval-scenes", type=int, default=2)
+    ap.add_argument("--train-steps", type=int, default=8000)
+    ap.add_argument("--eval-scenes", type=int, default=3)
     ap.add_argument("--window", type=int, default=64)
     ap.add_argument("--target-spp", type=int, default=2048)
     ap.add_argument("--eval-window", type=int, default=0, help="evaluate on a centred window (0: full 128^3)")
+    ap.add_argument("--obj", action="store_true", help=".obj4 ingest of the C1 mesh (reader MB/s)")
+    ap.add_argument("--no-gpu-mesh", action="store_true", help="with --obj: skip the GPU mesh build")
+    ap.add_argument("--obj-sample-facets", type=int, default=60000, help="with --obj: facets read by the oracle")
     ap.add_argument("--batch-scenes", action="store_true",
                     help="C4: batched persistent launch over many scenes vs one launch sequence per scene")
     ap.add_argument("--scenes", type=int, default=256)
@@ -251,6 +310,8 @@ def main():
         return denoise_bench(args)
     if args.batch_scenes:
         return batch_bench(args)
+    if args.obj:
+        return obj_bench(args)
     if args.copy_calibration:
         import torch
         x = torch.ones(1 << 26, dtype=torch.float32, device="cuda")  # 256 MiB

@@ -27,9 +27,15 @@
 // "f a//n b//n c//n" (1-based); for N = 3 a facet whose vertex normals all
 // point against its edge cross product is written with vertices 1 and 2
 // swapped.
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
 #include <array>
 #include <cerrno>
+#include <cfloat>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -81,18 +87,105 @@ int dimension_of(const std::string& path)
     return d;
 }
 
-// string_to_floats (src/obj/file/file_load.cpp:448-498): strtof in sequence,
-// stopping at the first value that is missing, out of range or not finite
-int read_floats(const char* str, float* out, int count)
+// Exact decimal-to-float conversion for the common case.  A token of at most
+// 19 significant digits m and decimal exponent |e| <= 22 with m < 2^53 gives
+// d = m * 10^e (or m / 10^-e) rounded once to double; rounding d to float then
+// gives the correctly rounded float of the decimal value unless d lies exactly
+// on a midpoint between two floats (no other float midpoint can lie between
+// the decimal value and d, since midpoints are doubles).  Returns false, and
+// leaves the decision to strtof, for anything else: midpoints, results that
+// are zero from non-zero digits, subnormal or infinite, more digits, hex,
+// "inf", "nan" or no digits at all.  Accepted tokens are consumed exactly as
+// strtof consumes them (leading white space, sign, digits, '.', exponent).
+bool fast_float(const char* p, const char* e, float* out, const char** end)
 {
-    errno = 0;
+    static constexpr double P10[23] = {1e0,  1e1,  1e2,  1e3,  1e4,  1e5,  1e6,  1e7,  1e8,  1e9,  1e10, 1e11,
+                                       1e12, 1e13, 1e14, 1e15, 1e16, 1e17, 1e18, 1e19, 1e20, 1e21, 1e22};
+    const char* q = p;
+    while (q < e && is_space(*q)) ++q;
+    bool neg = false;
+    if (q < e && (*q == '+' || *q == '-')) {
+        neg = *q == '-';
+        ++q;
+    }
+    if (q + 1 < e && q[0] == '0' && (q[1] == 'x' || q[1] == 'X')) return false;
+    uint64_t m = 0;
+    int digits = 0, e10 = 0;
+    bool any = false, dropped = false;
+    for (; q < e && is_digit(*q); ++q) {
+        any = true;
+        if (digits < 19) {
+            m = m * 10 + (uint64_t)(*q - '0');
+            digits += m != 0;
+        } else {
+            ++e10;
+            dropped |= *q != '0';
+        }
+    }
+    if (q < e && *q == '.') {
+        for (++q; q < e && is_digit(*q); ++q) {
+            any = true;
+            if (digits < 19) {
+                m = m * 10 + (uint64_t)(*q - '0');
+                digits += m != 0;
+                --e10;
+            } else {
+                dropped |= *q != '0';
+            }
+        }
+    }
+    if (!any || dropped) return false;
+    if (q < e && (*q == 'e' || *q == 'E')) {
+        const char* r = q + 1;
+        bool eneg = false;
+        if (r < e && (*r == '+' || *r == '-')) {
+            eneg = *r == '-';
+            ++r;
+        }
+        if (r < e && is_digit(*r)) {
+            int x = 0;
+            for (; r < e && is_digit(*r); ++r)
+                if (x < 100000) x = x * 10 + (*r - '0');
+            e10 += eneg ? -x : x;
+            q = r;
+        }
+    }
+    if (m == 0) {
+        *out = neg ? -0.0f : 0.0f;
+        *end = q;
+        return true;
+    }
+    if (m >= ((uint64_t)1 << 53) || e10 < -22 || e10 > 22) return false;
+    const double d = e10 >= 0 ? (double)m * P10[e10] : (double)m / P10[-e10];
+    const float f = (float)d;
+    if (!(f >= FLT_MIN) || f > FLT_MAX) return false;
+    uint64_t bits;
+    std::memcpy(&bits, &d, sizeof bits);
+    if ((bits & (((uint64_t)1 << 29) - 1)) == ((uint64_t)1 << 28)) return false;
+    *out = neg ? -f : f;
+    *end = q;
+    return true;
+}
+
+// string_to_floats (src/obj/file/file_load.cpp:448-498): values read as strtof
+// reads them, in sequence from p up to the end e of the line's text, stopping
+// at the first value that is missing, out of range or not finite
+int read_floats(const char* p, const char* e, float* out, int count, std::string& tmp)
+{
     int k = 0;
     while (k < count) {
-        char* end;
-        const float v = std::strtof(str, &end);
-        if (end == str || errno == ERANGE || !std::isfinite(v)) break;
+        const char* end;
+        float v;
+        if (!fast_float(p, e, &v, &end)) {
+            tmp.assign(p, e);  // strtof on a terminated copy of the rest of the line
+            char* te;
+            errno = 0;
+            v = std::strtof(tmp.c_str(), &te);
+            if (te == tmp.c_str() || errno == ERANGE || !std::isfinite(v)) break;
+            end = p + (te - tmp.c_str());
+        }
         out[k++] = v;
-        str = end;
+        p = end;
     }
     return k;
 }
@@ -134,6 +227,30 @@ void run_threads(int T, const std::function<void(int)>& fn)
     for (auto& th : pool) th.join();
 }
 
+// the file's bytes, mapped read-only
+struct Mapped {
+    const char* p = nullptr;
+    size_t n = 0;
+    int fd = -1;
+    explicit Mapped(const char* path)
+    {
+        fd = ::open(path, O_RDONLY);
+        if (fd < 0) fail(std::string("Failed to open file ") + path);
+        struct stat st;
+        if (::fstat(fd, &st) != 0) fail(std::string("Failed to read file ") + path);
+        n = (size_t)st.st_size;
+        if (n == 0) return;
+        void* m = ::mmap(nullptr, n, PROT_READ, MAP_PRIVATE, fd, 0);
+        if (m == MAP_FAILED) fail(std::string("Failed to read file ") + path);
+        p = (const char*)m;
+    }
+    ~Mapped()
+    {
+        if (p) ::munmap((void*)p, n);
+        if (fd >= 0) ::close(fd);
+    }
+};
+
 struct Chunk {
     const char* begin = nullptr;
     const char* end = nullptr;
@@ -142,15 +259,29 @@ struct Chunk {
     // per facet: N (vertex, texture, normal) indices as read, and the chunk's
     // running (v, vt, vn) counts at its line
     std::vector<std::array<int32_t, 3>> g;
-    std::vector<std::array<int64_t, 3>> at;
+    std::vector<std::array<int32_t, 3>> at;
     int64_t lines = 0;
     std::string error;
+    // after index correction: the box of the chunk's facet vertices, and how
+    // many of its triangles have parallel edge vectors (n = 3)
+    std::vector<float> lo, hi;
+    int64_t flat = 0;
 };
 
 void parse_chunk(Chunk& c, int N)
 {
     const int max_facets = N == 3 ? 5 : 1;
     const int max_groups = max_facets + N - 1;
+    {
+        // room for a line of n values or one facet per line
+        int64_t lines = 0;
+        for (const char* p = c.begin; p < c.end; ++lines) {
+            const char* le = (const char*)std::memchr(p, '\n', (size_t)(c.end - p));
+            p = le ? le + 1 : c.end;
+        }
+        c.g.reserve((size_t)(lines * N));
+        c.at.reserve((size_t)lines);
+    }
     std::vector<std::array<int32_t, 3>> grp((size_t)max_groups);
     std::vector<float> tmp((size_t)N + 1);
     std::string text;
@@ -170,8 +301,7 @@ void parse_chunk(Chunk& c, int N)
         const bool vn = kl == 2 && kw[0] == 'v' && kw[1] == 'n';
         const bool f = kl == 1 && kw[0] == 'f';
         if (v || vn) {
-            text.assign(q, ce);
-            if (read_floats(text.c_str(), tmp.data(), N) != N)
+            if (read_floats(q, ce, tmp.data(), N, text) != N)
                 fail("Error read " + std::to_string(N) + " floating points of float type");
             if (v) {
                 c.v.insert(c.v.end(), tmp.begin(), tmp.begin() + N);
@@ -182,8 +312,7 @@ void parse_chunk(Chunk& c, int N)
                 ++c.nvn;
             }
         } else if (vt) {
-            text.assign(q, ce);
-            const int k = read_floats(text.c_str(), tmp.data(), N);
+            const int k = read_floats(q, ce, tmp.data(), N, text);
             if (k != N - 1 && k != N)
                 fail("Error read " + std::to_string(N - 1) + " or " + std::to_string(N) +
                      " floating points of float type");
@@ -226,9 +355,10 @@ void parse_chunk(Chunk& c, int N)
             }
             if (!(tex == 0 || tex == count)) fail("Inconsistent facet texture indices");
             if (!(nrm == 0 || nrm == count)) fail("Inconsistent facet normal indices");
+            if (c.nv > INT32_MAX || c.nvt > INT32_MAX || c.nvn > INT32_MAX) fail("OBJ file too large");
             for (int fi = 0; fi < count - (N - 1); ++fi) {
                 for (int k = 0; k < N; ++k) c.g.push_back(grp[(size_t)(k == 0 ? 0 : fi + k)]);
-                c.at.push_back({c.nv, c.nvt, c.nvn});
+                c.at.push_back({(int32_t)c.nv, (int32_t)c.nvt, (int32_t)c.nvn});
             }
         }
         ++c.lines;
@@ -236,21 +366,14 @@ void parse_chunk(Chunk& c, int N)
     }
 }
 
-// center_and_length_for_min_max (src/obj/alg/alg.h:36-70) over the facets' vertices
-void check_extent(const std::vector<float>& vtx, const std::vector<int32_t>& fac, int N)
+// center_and_length_for_min_max (src/obj/alg/alg.h:36-70) on the box of the facets' vertices
+void check_box(const float* mn, const float* mx, int N)
 {
-    std::vector<float> mn((size_t)N, std::numeric_limits<float>::max()),
-        mx((size_t)N, std::numeric_limits<float>::lowest());
-    for (int32_t v : fac)
-        for (int i = 0; i < N; ++i) {
-            mn[(size_t)i] = std::min(mn[(size_t)i], vtx[(size_t)v * N + i]);
-            mx[(size_t)i] = std::max(mx[(size_t)i], vtx[(size_t)v * N + i]);
-        }
     long double len2 = 0;
     for (int i = 0; i < N; ++i) {
-        if (mn[(size_t)i] >= mx[(size_t)i]) fail("Object size error");
-        const float d = mx[(size_t)i] - mn[(size_t)i];
-        if (!std::isfinite(mn[(size_t)i] + d / 2)) fail("Object center is not finite");
+        if (mn[i] >= mx[i]) fail("Object size error");
+        const float d = mx[i] - mn[i];
+        if (!std::isfinite(mn[i] + d / 2)) fail("Object center is not finite");
         len2 += (long double)d * d;
     }
     if (!std::isfinite((double)std::sqrt(len2))) fail("Object length is not finite");
@@ -272,9 +395,9 @@ bool triangle_has_area(const float* a, const float* b, const float* c)
 
 struct nsp_obj {
     int n = 0;
-    std::vector<float> vertices, normals;
-    std::vector<int32_t> facets, facet_normals;
-    int64_t texcoords = 0;
+    int64_t V = 0, VN = 0, VT = 0, F = 0;
+    std::unique_ptr<float[]> vertices, normals;
+    std::unique_ptr<int32_t[]> facets, facet_normals;
 };
 
 extern "C" {
@@ -290,27 +413,19 @@ int nsp_obj_read(const char* path, int threads, nsp_obj** out)
     *out = nullptr;
     try {
         const int N = dimension_of(path);
-        FILE* f = std::fopen(path, "rb");
-        if (!f) fail(std::string("Failed to open file ") + path);
-        std::vector<char> data;
-        std::fseek(f, 0, SEEK_END);
-        const long size = std::ftell(f);
-        std::fseek(f, 0, SEEK_SET);
-        data.resize((size_t)std::max(0L, size));
-        const size_t got = size > 0 ? std::fread(data.data(), 1, (size_t)size, f) : 0;
-        std::fclose(f);
-        if (got != data.size()) fail(std::string("Failed to read file ") + path);
+        const Mapped file(path);
         // chunks at line boundaries; automatic thread counts give each chunk at
         // least 64 KiB
+        const size_t size = file.n;
         int T = threads > 0 ? std::min(threads, 64)
                             : std::min({(int)std::max(1u, std::thread::hardware_concurrency()), 64,
-                                        1 + (int)std::min<size_t>(data.size() >> 16, 64)});
+                                        1 + (int)std::min<size_t>(size >> 16, 64)});
         std::vector<Chunk> chunks((size_t)T);
-        const char* base = data.data();
-        const char* end = base + data.size();
+        const char* base = file.p;
+        const char* end = base + size;
         const char* cur = base;
         for (int t = 0; t < T; ++t) {
-            const char* e = t == T - 1 ? end : base + data.size() * (size_t)(t + 1) / (size_t)T;
+            const char* e = t == T - 1 ? end : base + size * (size_t)(t + 1) / (size_t)T;
             if (e < cur) e = cur;
             while (e < end && *e != '\n') ++e;
             if (e < end) ++e;
@@ -349,18 +464,27 @@ int nsp_obj_read(const char* path, int threads, nsp_obj** out)
         const int64_t V = tot[0], VT = tot[1], VN = tot[2], F = tot[3];
         if (F == 0) fail("No facets found in OBJ file");
         if (V > INT32_MAX || VN > INT32_MAX || F > INT32_MAX) fail("OBJ file too large");
-        obj->texcoords = VT;
-        obj->vertices.resize((size_t)(V * N));
-        obj->normals.resize((size_t)(VN * N));
-        obj->facets.resize((size_t)(F * N));
-        obj->facet_normals.resize((size_t)(F * N));
-        // correct_indices, then check_facet_indices (src/obj/file/file_load.cpp:1046-1065, 851-897)
-        std::vector<std::string> errors((size_t)T);
+        obj->V = V;
+        obj->VT = VT;
+        obj->VN = VN;
+        obj->F = F;
+        obj->vertices.reset(new float[(size_t)(V * N)]);
+        obj->normals.reset(new float[(size_t)(VN * N)]);
+        obj->facets.reset(new int32_t[(size_t)(F * N)]);
+        obj->facet_normals.reset(new int32_t[(size_t)(F * N)]);
         run_threads(T, [&](int t) {
             const Chunk& c = chunks[(size_t)t];
+            std::copy(c.v.begin(), c.v.end(), obj->vertices.get() + start[(size_t)t][0] * N);
+            std::copy(c.vn.begin(), c.vn.end(), obj->normals.get() + start[(size_t)t][2] * N);
+        });
+        // correct_indices, then check_facet_indices (src/obj/file/file_load.cpp:1046-1065,
+        // 851-897); the box of each chunk's facet vertices and its flat triangles
+        run_threads(T, [&](int t) {
+            Chunk& c = chunks[(size_t)t];
             const std::array<int64_t, 4>& s0 = start[(size_t)t];
-            std::copy(c.v.begin(), c.v.end(), obj->vertices.begin() + s0[0] * N);
-            std::copy(c.vn.begin(), c.vn.end(), obj->normals.begin() + s0[2] * N);
+            c.lo.assign((size_t)N, std::numeric_limits<float>::max());
+            c.hi.assign((size_t)N, std::numeric_limits<float>::lowest());
+            const float* vtx = obj->vertices.get();
             for (size_t k = 0; k < c.at.size(); ++k) {
                 const int64_t nv = s0[0] + c.at[k][0], nt = s0[1] + c.at[k][1], nn = s0[2] + c.at[k][2];
                 const int64_t fo = (s0[3] + (int64_t)k) * N;
@@ -371,48 +495,73 @@ int nsp_obj_read(const char* path, int threads, nsp_obj** out)
                     const int64_t tb = b > 0 ? b - 1 : (b < 0 ? nt + b : -1);
                     const int64_t nd = d > 0 ? d - 1 : (d < 0 ? nn + d : -1);
                     if (va < 0 || va >= V) {
-                        errors[(size_t)t] = "Vertex index " + std::to_string(va) + " is out of bounds [0, " +
-                                            std::to_string(V) + ")";
+                        c.error = "Vertex index " + std::to_string(va) + " is out of bounds [0, " +
+                                  std::to_string(V) + ")";
                         return;
                     }
                     if (b != 0 && (tb < 0 || tb >= VT)) {
-                        errors[(size_t)t] = "Texture coordinate index " + std::to_string(tb) +
-                                            " is out of bounds [0, " + std::to_string(VT) + ")";
+                        c.error = "Texture coordinate index " + std::to_string(tb) + " is out of bounds [0, " +
+                                  std::to_string(VT) + ")";
                         return;
                     }
                     if (d != 0 && (nd < 0 || nd >= VN)) {
-                        errors[(size_t)t] = "Normal index " + std::to_string(nd) + " is out of bounds [0, " +
-                                            std::to_string(VN) + ")";
+                        c.error = "Normal index " + std::to_string(nd) + " is out of bounds [0, " +
+                                  std::to_string(VN) + ")";
                         return;
                     }
                     obj->facets[(size_t)(fo + i)] = (int32_t)va;
                     obj->facet_normals[(size_t)(fo + i)] = (int32_t)nd;
+                    for (int j = 0; j < N; ++j) {
+                        const float x = vtx[va * N + j];
+                        c.lo[(size_t)j] = std::min(c.lo[(size_t)j], x);
+                        c.hi[(size_t)j] = std::max(c.hi[(size_t)j], x);
+                    }
+                }
+                if (N == 3) {
+                    const int32_t* fv = &obj->facets[(size_t)fo];
+                    c.flat += !triangle_has_area(vtx + 3 * (int64_t)fv[0], vtx + 3 * (int64_t)fv[1],
+                                                 vtx + 3 * (int64_t)fv[2]);
                 }
             }
         });
-        for (const std::string& e : errors)
-            if (!e.empty()) fail(e);
-        check_extent(obj->vertices, obj->facets, N);
-        if (N == 3) {
-            // remove_facets_with_incorrect_dimension (src/obj/file/file_load.cpp:902-940)
+        std::vector<float> lo((size_t)N, std::numeric_limits<float>::max()),
+            hi((size_t)N, std::numeric_limits<float>::lowest());
+        int64_t flat = 0;
+        for (const Chunk& c : chunks) {
+            if (!c.error.empty()) fail(c.error);
+            for (int j = 0; j < N; ++j) {
+                lo[(size_t)j] = std::min(lo[(size_t)j], c.lo[(size_t)j]);
+                hi[(size_t)j] = std::max(hi[(size_t)j], c.hi[(size_t)j]);
+            }
+            flat += c.flat;
+        }
+        check_box(lo.data(), hi.data(), N);
+        if (flat > 0) {
+            // remove_facets_with_incorrect_dimension (src/obj/file/file_load.cpp:902-940), n = 3
+            if (flat == F) fail("No 2-facets found in OBJ-3 file");
+            int32_t* fac = obj->facets.get();
+            int32_t* fnr = obj->facet_normals.get();
+            const float* vv = obj->vertices.get();
             int64_t kept = 0;
-            const float* vv = obj->vertices.data();
+            std::fill(lo.begin(), lo.end(), std::numeric_limits<float>::max());
+            std::fill(hi.begin(), hi.end(), std::numeric_limits<float>::lowest());
             for (int64_t k = 0; k < F; ++k) {
-                const int32_t* fv = &obj->facets[(size_t)(3 * k)];
+                const int32_t* fv = fac + 3 * k;
                 if (!triangle_has_area(vv + 3 * (int64_t)fv[0], vv + 3 * (int64_t)fv[1], vv + 3 * (int64_t)fv[2]))
                     continue;
                 for (int i = 0; i < 3; ++i) {
-                    obj->facets[(size_t)(3 * kept + i)] = obj->facets[(size_t)(3 * k + i)];
-                    obj->facet_normals[(size_t)(3 * kept + i)] = obj->facet_normals[(size_t)(3 * k + i)];
+                    const int32_t x = fac[3 * k + i];
+                    fac[3 * kept + i] = x;
+                    fnr[3 * kept + i] = fnr[3 * k + i];
+                    for (int j = 0; j < 3; ++j) {
+                        lo[(size_t)j] = std::min(lo[(size_t)j], vv[3 * (int64_t)x + j]);
+                        hi[(size_t)j] = std::max(hi[(size_t)j], vv[3 * (int64_t)x + j]);
+                    }
                 }
                 ++kept;
             }
-            if (kept != F) {
-                if (kept == 0) fail("No 2-facets found in OBJ-3 file");
-                obj->facets.resize((size_t)(3 * kept));
-                obj->facet_normals.resize((size_t)(3 * kept));
-                check_extent(obj->vertices, obj->facets, N);
-            }
+            obj->F = kept;
+            check_box(lo.data(), hi.data(), N);
         }
         *out = obj.release();
         return NSP_OK;
@@ -432,32 +581,31 @@ int nsp_obj_info(const nsp_obj* obj, int* n, int64_t* vertices, int64_t* facets,
 {
     if (!obj) return NSP_E_INVALID;
     if (n) *n = obj->n;
-    if (vertices) *vertices = (int64_t)obj->vertices.size() / obj->n;
-    if (facets) *facets = (int64_t)obj->facets.size() / obj->n;
-    if (normals) *normals = (int64_t)obj->normals.size() / obj->n;
-    if (texcoords) *texcoords = obj->texcoords;
+    if (vertices) *vertices = obj->V;
+    if (facets) *facets = obj->F;
+    if (normals) *normals = obj->VN;
+    if (texcoords) *texcoords = obj->VT;
     return NSP_OK;
 }
 
 int nsp_obj_copy(const nsp_obj* obj, float* vertices, int32_t* facets, float* normals, int32_t* facet_normals)
 {
     if (!obj) return NSP_E_INVALID;
-    if (vertices) std::copy(obj->vertices.begin(), obj->vertices.end(), vertices);
-    if (facets) std::copy(obj->facets.begin(), obj->facets.end(), facets);
-    if (normals) std::copy(obj->normals.begin(), obj->normals.end(), normals);
-    if (facet_normals) std::copy(obj->facet_normals.begin(), obj->facet_normals.end(), facet_normals);
+    const int n = obj->n;
+    if (vertices) std::copy(obj->vertices.get(), obj->vertices.get() + obj->V * n, vertices);
+    if (facets) std::copy(obj->facets.get(), obj->facets.get() + obj->F * n, facets);
+    if (normals) std::copy(obj->normals.get(), obj->normals.get() + obj->VN * n, normals);
+    if (facet_normals) std::copy(obj->facet_normals.get(), obj->facet_normals.get() + obj->F * n, facet_normals);
     return NSP_OK;
 }
 
 int nsp_mesh_create_obj(nsp_context* ctx, const nsp_obj* obj, const double* vertex_matrix, nsp_mesh** out)
 {
     if (!ctx || !obj || !out) return NSP_E_INVALID;
-    const int n = obj->n;
-    const bool normals = !obj->normals.empty();
-    return nsp_mesh_create(ctx, n, obj->vertices.data(), (int64_t)obj->vertices.size() / n, obj->facets.data(),
-                           (int64_t)obj->facets.size() / n, normals ? obj->normals.data() : nullptr,
-                           (int64_t)obj->normals.size() / n, normals ? obj->facet_normals.data() : nullptr,
-                           vertex_matrix, out);
+    const bool normals = obj->VN > 0;
+    return nsp_mesh_create(ctx, obj->n, obj->vertices.get(), obj->V, obj->facets.get(), obj->F,
+                           normals ? obj->normals.get() : nullptr, obj->VN,
+                           normals ? obj->facet_normals.get() : nullptr, vertex_matrix, out);
 }
 
 int nsp_obj_write(const char* path, int n, const float* vertices, int64_t V, const int32_t* facets, int64_t F,

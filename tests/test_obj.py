@@ -234,6 +234,72 @@ def test_random_corruptions_give_same_error(tmp_path, seed):
         assert str(e1.value) in str(e2.value)
 
 
+def float_tokens(rng, count):
+    """Decimal spellings of many kinds: random magnitudes and digit counts,
+    signs, leading zeros, exponents, float midpoints (shortest and exact
+    spellings), more than 19 digits, integers above 2^24."""
+    from decimal import Decimal
+    out = []
+    for _ in range(count):
+        k = int(rng.integers(9))
+        if k == 0:
+            x = float(rng.normal()) * 10.0 ** int(rng.integers(-30, 31))
+            out.append(repr(x))
+        elif k == 1:
+            out.append("%.*e" % (int(rng.integers(0, 12)), float(rng.normal()) * 10.0 ** int(rng.integers(-20, 21))))
+        elif k == 2:
+            out.append("%.*f" % (int(rng.integers(0, 10)), float(rng.normal()) * 10.0 ** int(rng.integers(-3, 6))))
+        elif k in (3, 4):
+            f = np.float32(rng.normal() * 10.0 ** int(rng.integers(-10, 11)))
+            g = np.nextafter(f, np.float32(np.inf))
+            mid = (float(f) + float(g)) / 2
+            out.append(repr(mid) if k == 3 else str(Decimal(mid)))
+        elif k == 5:
+            out.append(("+" if rng.random() < 0.5 else "-") + "0" * int(rng.integers(0, 4))
+                       + str(int(rng.integers(1, 10 ** 9))) + "." + str(int(rng.integers(0, 10 ** 6))).zfill(6))
+        elif k == 6:
+            out.append("".join(str(int(d)) for d in rng.integers(0, 10, size=int(rng.integers(18, 30))))
+                       + "e" + str(int(rng.integers(-40, 10))))
+        elif k == 7:
+            out.append(str(int(rng.integers(1 << 24, 1 << 40))))
+        else:
+            out.append(["0", "-0", "0.0", ".5", "5.", "-.25e1", "1E3", "7e+0", "000", "0.000000000"][int(rng.integers(10))])
+    return out
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_float_spellings_read_as_strtof_reads_them(tmp_path, seed):
+    # the library's float reading (an exact fast path with strtof deciding the
+    # hard cases) against libc strtof itself, bit for bit
+    rng = np.random.default_rng(seed)
+    tok = float_tokens(rng, 30000)
+    lines = ["v " + " ".join(tok[i:i + 3]) for i in range(0, len(tok) - 2, 3)]
+    lines.append("f 1 2 3")
+    lines.append("f 4 5 6")
+    p = write_text(tmp_path, "fl.obj", "\n".join(lines) + "\n")
+    ref = obj_ref.read(p)  # every spelling here is a finite normal float
+    got = lib_read(p, 4)
+    assert got["vertices"].tobytes() == ref["vertices"].tobytes()
+
+
+@pytest.mark.parametrize("tok", ["1e", "1e+", "0x1p3", "inf", "nan", "1e39", "1e-39", "1e-46", "-1e-45", "+",
+                                 "-", ".", "1.5abc", "--1", "3.4028235e38", "3.4028236e38", "1.17549435e-38",
+                                 "1.1754942e-38", "16777217", "0.1", "1" + "0" * 30])
+def test_float_edge_spellings(tmp_path, tok):
+    # one token at a time, as the first coordinate: both readers agree on the
+    # value or on the error
+    text = f"v {tok} 1 1\nv 0 0 2\nv 0 2 0\nv 3 0 0\nf 2 3 4\n"
+    p = write_text(tmp_path, "edge.obj", text)
+    try:
+        ref = obj_ref.read(p)
+    except obj_ref.ObjError as e:
+        with pytest.raises(native.NspError) as e2:
+            native.read_obj(p, 1)
+        assert str(e) in str(e2.value)
+        return
+    assert lib_read(p, 1)["vertices"].tobytes() == ref["vertices"].tobytes()
+
+
 # ---------------------------------------------------------------------------
 # Writer
 
