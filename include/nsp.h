@@ -208,10 +208,12 @@ typedef struct {
     int max_depth;         /* MAX_RECURSION_LEVEL (painter.cpp:42), reference value 100 */
     int smooth_normal;     /* paint(..., smooth_normal) */
     int precision;         /* NSP_PRECISION_* */
-    int passes;            /* passes to run (>=1); VisibleBarPaintbrush max_pass_count */
+    int passes;            /* passes to run (>= 1), or -1: until *stop (VisibleBarPaintbrush
+                              max_pass_count, paintbrush.h:141-144) */
     uint64_t seed;         /* counter-based RNG key */
     int64_t batch_paths;   /* paths per wavefront batch; 0 = library default/autotuned */
-    int use_graph;         /* capture each batch's per-bounce launches in a hipGraph */
+    int use_graph;         /* capture a batch's launch sequence in a hipGraph once per call and
+                              replay it for every batch of every pass */
     int aov;               /* also accumulate first-hit albedo, normal and direct light per pixel (denoiser inputs) */
     int primary;           /* NSP_PRIMARY_*: how FP32 renders find the primary rays' hits */
 } nsp_paint_params;
@@ -225,10 +227,12 @@ typedef struct {
 #define NSP_PRIMARY_WALK 1
 
 /* Delivery of results: PainterNotifier::painter_pixel_after(pixel, color)
- * (src/painter/painter.h:26-38) in batches of consecutive pixels in
- * GlobalIndex order (x0 fastest); rgb holds the running mean colour of each
- * pixel (Pixels::add_color_and_samples, painter.cpp:57-88).  Called on the
- * thread that called nsp_paint. */
+ * (src/painter/painter.h:26-38) for the pixels of each wavefront batch as the
+ * batch finishes (while the next batch runs), in runs of at most 65536
+ * consecutive pixels in GlobalIndex order (x0 fastest); rgb holds the running
+ * mean colour of each pixel (Pixels::add_color_and_samples, painter.cpp:57-88).
+ * pass_done follows the last batch of each pass (BarPaintbrush::next_pass).
+ * Called on the thread that called nsp_paint. */
 typedef struct {
     void (*pixels_after)(void* user, int64_t first_pixel, int64_t count, const float* rgb_mean);
     void (*pass_done)(void* user, int pass, double seconds);
@@ -240,8 +244,9 @@ typedef struct {
  * thread_count, stop, smooth_normal) (src/painter/painter.h:40-42,
  * painter.cpp:599-625).  Blocks until `passes` passes are done or *stop becomes
  * nonzero (polled between wavefront batches, the analogue of the per-pixel
- * poll at painter.cpp:419).  Errors are reported through callbacks->error_message
- * as in paint(), and returned. */
+ * poll at painter.cpp:419; a callback may set it).  A stopped paint returns
+ * NSP_E_STOPPED with the finished batches kept.  Errors are reported through
+ * callbacks->error_message as in paint(), and returned. */
 int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callbacks* callbacks,
               const volatile int32_t* stop);
 

@@ -83,6 +83,9 @@ struct nsp_context {
     // ray queues and per-path state of nsp_paint; every paint on this context
     // runs on its one stream, so all scenes share this workspace
     RenderBuffers work;
+    // pinned staging of the per-batch pixel delivery (two slots) and its events
+    HostBuf feed;
+    hipEvent_t feed_ev[2] = {nullptr, nullptr};
 };
 
 struct nsp_mesh {
@@ -104,6 +107,7 @@ struct nsp_scene {
     DevBuf pix_sum, pix_cnt;
     DevBuf pix_aov;  // aov_channels(n) * pixels, allocated on the first pass with aov
     RasterPrep raster;  // rasterised primary visibility (boxes and work units), built on first use
+    DevBuf pass_dev;  // int: the pass index the kernels read
     DevBuf totals;  // 3 u64
     StageMarks marks;
     bool counters = false;
@@ -235,6 +239,9 @@ void nsp_context_destroy(nsp_context* ctx)
 {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (hipEvent_t e : ctx->feed_ev)
+        if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -556,6 +563,78 @@ void nsp_scene_destroy(nsp_scene* scene)
 
 int64_t nsp_scene_pixel_count(const nsp_scene* scene) { return scene ? scene->pixels : NSP_E_INVALID; }
 
+// Delivery of the pixels a wavefront batch finished (PainterNotifier::painter_pixel_after
+// for each pixel, src/painter/painter.cpp:410-427): after a batch's launches its
+// pixel range is copied into one of two pinned slots and an event is recorded;
+// the previous batch's slot is delivered while this batch runs.
+namespace {
+struct PixelFeed {
+    nsp_context* ctx;
+    const nsp_scene* scene;
+    const nsp_callbacks* cb;
+    int64_t batch_pixels;
+    int64_t pending_first = -1, pending_count = 0;
+    int pending_slot = 0;
+    std::vector<float> mean;
+
+    PixelFeed(nsp_context* c, const nsp_scene* sc, const nsp_callbacks* k, int64_t bp)
+        : ctx(c), scene(sc), cb(k), batch_pixels(bp)
+    {
+        ctx->feed.alloc((size_t)2 * bp * (3 * sizeof(float) + sizeof(int32_t)));
+        for (hipEvent_t& e : ctx->feed_ev)
+            if (!e) NSP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    float* slot_sum(int k) const { return ctx->feed.as<float>() + (size_t)k * batch_pixels * 4; }
+    int32_t* slot_cnt(int k) const { return reinterpret_cast<int32_t*>(slot_sum(k) + 3 * batch_pixels); }
+
+    // queue the copies of pixels [first, first + count) after the work on s
+    void batch_done(int64_t first, int64_t count, hipStream_t s)
+    {
+        const int k = pending_first < 0 ? 0 : 1 - pending_slot;
+        const int64_t P = scene->pixels;
+        for (int c = 0; c < 3; ++c)
+            NSP_HIP(hipMemcpyAsync(slot_sum(k) + c * batch_pixels, scene->pix_sum.as<float>() + c * P + first,
+                                   sizeof(float) * count, hipMemcpyDeviceToHost, s));
+        NSP_HIP(hipMemcpyAsync(slot_cnt(k), scene->pix_cnt.as<int32_t>() + first, sizeof(int32_t) * count,
+                               hipMemcpyDeviceToHost, s));
+        NSP_HIP(hipEventRecord(ctx->feed_ev[k], s));
+        deliver();
+        pending_first = first;
+        pending_count = count;
+        pending_slot = k;
+    }
+    // hand the pending slot to the callback, in chunks of at most 65536 pixels
+    void deliver()
+    {
+        if (pending_first < 0) return;
+        NSP_HIP(hipEventSynchronize(ctx->feed_ev[pending_slot]));
+        const float* sum = slot_sum(pending_slot);
+        const int32_t* cnt = slot_cnt(pending_slot);
+        const int64_t CH = 65536;
+        mean.resize((size_t)3 * std::min(CH, pending_count));
+        for (int64_t b = 0; b < pending_count; b += CH) {
+            const int64_t c = std::min(CH, pending_count - b);
+            for (int64_t j = 0; j < c; ++j) {
+                const int32_t k = std::max(1, cnt[b + j]);
+                for (int ch = 0; ch < 3; ++ch) mean[(size_t)(j * 3 + ch)] = sum[ch * batch_pixels + b + j] / k;
+            }
+            cb->pixels_after(cb->user, pending_first + b, c, mean.data());
+        }
+        pending_first = -1;
+    }
+};
+
+struct GraphHolder {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~GraphHolder()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+}  // namespace
+
 int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callbacks* cb, const volatile int32_t* stop)
 {
     if (!scene || !params) return NSP_E_INVALID;
@@ -563,23 +642,27 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
     int rc = guarded(ctx, [&] {
         const nsp_paint_params& p = *params;
         if (p.max_depth < 0 || p.max_depth > 250) raise(NSP_E_INVALID, "max_depth must be in [0, 250]");
-        if (p.passes < 1) raise(NSP_E_INVALID, "passes must be >= 1");
+        // BarPaintbrush max_pass_count: -1 or positive (src/painter/paintbrushes/paintbrush.h:141-144)
+        if (!(p.passes == -1 || p.passes > 0)) raise(NSP_E_INVALID, "Error paintbrush max pass count " + std::to_string(p.passes));
+        if (p.passes == -1 && !stop) raise(NSP_E_INVALID, "passes = -1 paints until *stop is set and needs a stop flag");
         if (p.precision != NSP_PRECISION_FP32 && p.precision != NSP_PRECISION_FP64)
             raise(NSP_E_INVALID, "unknown precision");
+        if (p.primary != NSP_PRIMARY_AUTO && p.primary != NSP_PRIMARY_WALK) raise(NSP_E_INVALID, "unknown primary mode");
         launch_gate(ctx);
         const int n = scene->n;
         hipStream_t s = ctx->stream;
+        if (!scene->pass_dev.p) scene->pass_dev.alloc(sizeof(int));
         SceneK<float> kf;
         SceneK<double> kd;
         fill_scene<float>(*scene, p, 0, kf);
         fill_scene<double>(*scene, p, 0, kd);
+        kf.pass_dev = kd.pass_dev = scene->pass_dev.as<int>();
         const int S = kf.S;
         int64_t want = p.batch_paths > 0 ? p.batch_paths : (p.precision ? ((int64_t)1 << 22) : ((int64_t)1 << 23));
         int64_t batch_pixels = std::max<int64_t>(1, want / S);
         batch_pixels = std::min<int64_t>(batch_pixels, scene->pixels);
         const int64_t cap = batch_pixels * S;
         alloc_render_buffers(ctx->work, n, p.precision, cap, p.max_depth);
-        if (p.primary != NSP_PRIMARY_AUTO && p.primary != NSP_PRIMARY_WALK) raise(NSP_E_INVALID, "unknown primary mode");
         const bool raster = p.precision == NSP_PRECISION_FP32 && p.primary == NSP_PRIMARY_AUTO;
         if (raster) {
             raster_prepare(scene->raster, scene->mesh->m, kd, S, s);
@@ -592,36 +675,37 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
                 NSP_HIP(hipMemsetAsync(scene->pix_aov.p, 0, sizeof(float) * aov_channels(n) * scene->pixels, s));
             }
         }
-        for (int pass = 0; pass < p.passes; ++pass) {
+        // the batches live in device memory; each batch's launch sequence
+        // selects its batch there and the kernels read the pass index there,
+        // so one launch sequence (and one captured graph) serves every batch of
+        // every pass
+        std::vector<BatchDesc> descs;
+        for (int64_t px = 0; px < scene->pixels; px += batch_pixels)
+            descs.push_back(BatchDesc{px, std::min(batch_pixels, scene->pixels - px) * S});
+        ctx->work.batch_list.alloc(sizeof(BatchDesc) * descs.size());
+        NSP_HIP(hipMemcpyAsync(ctx->work.batch_list.p, descs.data(), sizeof(BatchDesc) * descs.size(),
+                               hipMemcpyHostToDevice, s));
+        RenderLaunch rl;
+        rl.mesh = &scene->mesh->m;
+        rl.scene = p.precision ? (const void*)&kd : (const void*)&kf;
+        rl.buf = &ctx->work;
+        rl.pix_sum = scene->pix_sum.as<float>();
+        rl.pix_cnt = scene->pix_cnt.as<int32_t>();
+        rl.pix_aov = p.aov ? scene->pix_aov.as<float>() : nullptr;
+        rl.pixels = scene->pixels;
+        rl.max_depth = p.max_depth;
+        rl.counters = scene->counters;
+        rl.marks = p.use_graph ? nullptr : &scene->marks;
+        rl.raster = raster ? &scene->raster : nullptr;
+        GraphHolder gh;
+        std::unique_ptr<PixelFeed> feed;
+        if (cb && cb->pixels_after) feed = std::make_unique<PixelFeed>(ctx, scene, cb, batch_pixels);
+        for (int64_t pass = 0; p.passes == -1 || pass < p.passes; ++pass) {
             if (stop && *stop) raise(NSP_E_STOPPED, "stopped");
-            kf.pass = kd.pass = (int)scene->pass_count + pass;
             auto t0 = std::chrono::steady_clock::now();
             scene->marks.reset();
-            // the pass's batches live in device memory; each batch's launch
-            // sequence selects its batch there, so the sequence is identical
-            // for every batch and can be replayed from one captured graph
-            std::vector<BatchDesc> descs;
-            for (int64_t px = 0; px < scene->pixels; px += batch_pixels)
-                descs.push_back(BatchDesc{px, std::min(batch_pixels, scene->pixels - px) * S});
-            ctx->work.batch_list.alloc(sizeof(BatchDesc) * descs.size());
-            NSP_HIP(hipMemcpyAsync(ctx->work.batch_list.p, descs.data(), sizeof(BatchDesc) * descs.size(),
-                                   hipMemcpyHostToDevice, s));
-            pass_reset(ctx->work, scene->counters, s);
-            RenderLaunch rl;
-            rl.mesh = &scene->mesh->m;
-            rl.scene = p.precision ? (const void*)&kd : (const void*)&kf;
-            rl.buf = &ctx->work;
-            rl.pix_sum = scene->pix_sum.as<float>();
-            rl.pix_cnt = scene->pix_cnt.as<int32_t>();
-            rl.pix_aov = p.aov ? scene->pix_aov.as<float>() : nullptr;
-            rl.pixels = scene->pixels;
-            rl.max_depth = p.max_depth;
-            rl.counters = scene->counters;
-            rl.marks = p.use_graph ? nullptr : &scene->marks;
-            rl.raster = raster ? &scene->raster : nullptr;
-            hipGraph_t graph = nullptr;
-            hipGraphExec_t exec = nullptr;
-            if (p.use_graph) {
+            pass_reset(ctx->work, scene->counters, scene->pass_dev.as<int>(), (int)scene->pass_count, s);
+            if (p.use_graph && !gh.exec) {
                 NSP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
                 try {
                     render_batch(rl, s);
@@ -631,21 +715,21 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
                     if (g2) (void)hipGraphDestroy(g2);
                     throw;
                 }
-                NSP_HIP(hipStreamEndCapture(s, &graph));
-                NSP_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+                NSP_HIP(hipStreamEndCapture(s, &gh.graph));
+                NSP_HIP(hipGraphInstantiate(&gh.exec, gh.graph, nullptr, nullptr, 0));
             }
             for (size_t b = 0; b < descs.size(); ++b) {
                 if (stop && *stop) break;
-                if (exec)
-                    NSP_HIP(hipGraphLaunch(exec, s));
+                if (gh.exec)
+                    NSP_HIP(hipGraphLaunch(gh.exec, s));
                 else
                     render_batch(rl, s);
+                if (feed) feed->batch_done(descs[b].pixel0, descs[b].paths / S, s);
             }
+            if (feed) feed->deliver();
             unsigned long long tot[3] = {0, 0, 0};
             NSP_HIP(hipMemcpyAsync(tot, ctx->work.totals.p, sizeof tot, hipMemcpyDeviceToHost, s));
             NSP_HIP(hipStreamSynchronize(s));
-            if (exec) (void)hipGraphExecDestroy(exec);
-            if (graph) (void)hipGraphDestroy(graph);
             const unsigned long long closest = tot[0], shadow = tot[1], probes = tot[2];
             auto t1 = std::chrono::steady_clock::now();
             double secs = std::chrono::duration<double>(t1 - t0).count();
@@ -687,23 +771,6 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
             scene->sample_count += scene->pixels * S;
             scene->last_pass_s = secs;
             if (cb && cb->pass_done) cb->pass_done(cb->user, (int)scene->pass_count, secs);
-            if (cb && cb->pixels_after) {
-                const int64_t P = scene->pixels;
-                std::vector<float> sum((size_t)3 * P);
-                std::vector<int32_t> cnt(P);
-                NSP_HIP(hipMemcpy(sum.data(), scene->pix_sum.p, sizeof(float) * 3 * P, hipMemcpyDeviceToHost));
-                NSP_HIP(hipMemcpy(cnt.data(), scene->pix_cnt.p, sizeof(int32_t) * P, hipMemcpyDeviceToHost));
-                const int64_t CH = 65536;
-                std::vector<float> mean((size_t)3 * CH);
-                for (int64_t b = 0; b < P; b += CH) {
-                    int64_t c = std::min(CH, P - b);
-                    for (int64_t j = 0; j < c; ++j) {
-                        int32_t k = std::max(1, cnt[b + j]);
-                        for (int ch = 0; ch < 3; ++ch) mean[j * 3 + ch] = sum[(size_t)ch * P + b + j] / k;
-                    }
-                    cb->pixels_after(cb->user, b, c, mean.data());
-                }
-            }
             if (stopped) raise(NSP_E_STOPPED, "stopped");
         }
     });
@@ -885,6 +952,8 @@ int nsp_batch_paint(nsp_batch* b, const nsp_paint_params* params, const volatile
             nsp_scene* sc = b->scenes[i];
             BatchScene& bs = b->per[i];
             fill_scene<float>(*sc, p, (int)sc->pass_count, bs.kf);
+            if (!sc->pass_dev.p) sc->pass_dev.alloc(sizeof(int));
+            bs.kf.pass_dev = sc->pass_dev.as<int>();
             SceneK<double> kd;
             fill_scene<double>(*sc, p, (int)sc->pass_count, kd);
             const int S = bs.kf.S;
@@ -933,7 +1002,8 @@ int nsp_batch_paint(nsp_batch* b, const nsp_paint_params* params, const volatile
                 for (int i : lane_scenes[l]) {
                     nsp_scene* sc = b->scenes[i];
                     BatchScene& bs = b->per[i];
-                    pass_reset_state(bs.totals.as<unsigned long long>(), bs.idx.as<long long>(), ls);
+                    pass_reset_state(bs.totals.as<unsigned long long>(), bs.idx.as<long long>(),
+                                     sc->pass_dev.as<int>(), (int)sc->pass_count, ls);
                     RenderLaunch rl;
                     rl.mesh = &sc->mesh->m;
                     rl.scene = &bs.kf;

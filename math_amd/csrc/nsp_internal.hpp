@@ -66,6 +66,35 @@ struct DevBuf {
     }
 };
 
+// Pinned host memory that only grows (staging of device-to-host copies that
+// overlap later kernels).
+struct HostBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    HostBuf() = default;
+    HostBuf(const HostBuf&) = delete;
+    HostBuf& operator=(const HostBuf&) = delete;
+    ~HostBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t b)
+    {
+        if (b <= bytes && p) return;
+        release();
+        NSP_HIP(hipHostMalloc(&p, b, hipHostMallocDefault));
+        bytes = b;
+    }
+    template <typename T>
+    T* as() const
+    {
+        return static_cast<T*>(p);
+    }
+};
+
 // Per-simplex FP64 record layout (AoS, original simplex order):
 //   normal[N] | vertex0[N] | plane_n[(N-1)*N] | plane_d[N-1]
 __host__ __device__ constexpr int rec64_stride(int n) { return n * n + 2 * n - 1; }

@@ -164,24 +164,28 @@ __global__ void k_batch_totals(const unsigned long long* __restrict__ cnt, int d
 
 // Clears the pass totals and the batch index (a kernel, not a memset, so it
 // may sit inside captured work).
-__global__ void k_pass_reset(unsigned long long* totals, long long* batch_idx, unsigned long long* trav, int ntrav)
+__global__ void k_pass_reset(unsigned long long* totals, long long* batch_idx, unsigned long long* trav, int ntrav,
+                             int* pass_dev, int pass)
 {
     int i = threadIdx.x;
     if (i < 3) totals[i] = 0;
     if (i == 0) *batch_idx = 0;
+    if (i == 0 && pass_dev) *pass_dev = pass;
     if (trav && i < ntrav) trav[i] = 0;
 }
 
-void pass_reset_state(unsigned long long* totals, long long* batch_idx, hipStream_t s)
+void pass_reset_state(unsigned long long* totals, long long* batch_idx, int* pass_dev, int pass, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_pass_reset, dim3(1), dim3(64), 0, s, totals, batch_idx, (unsigned long long*)nullptr, 0);
+    hipLaunchKernelGGL(k_pass_reset, dim3(1), dim3(64), 0, s, totals, batch_idx, (unsigned long long*)nullptr, 0,
+                       pass_dev, pass);
     NSP_HIP(hipGetLastError());
 }
 
-void pass_reset(RenderBuffers& b, bool counters, hipStream_t s)
+void pass_reset(RenderBuffers& b, bool counters, int* pass_dev, int pass, hipStream_t s)
 {
     hipLaunchKernelGGL(k_pass_reset, dim3(1), dim3(64), 0, s, b.totals.as<unsigned long long>(),
-                       b.batch_idx.as<long long>(), counters ? b.trav_counters.as<unsigned long long>() : nullptr, 8);
+                       b.batch_idx.as<long long>(), counters ? b.trav_counters.as<unsigned long long>() : nullptr, 8,
+                       pass_dev, pass);
     NSP_HIP(hipGetLastError());
 }
 

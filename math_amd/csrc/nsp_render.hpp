@@ -47,8 +47,17 @@ struct SceneK {
     int smooth;
     int has_normals;
     uint64_t seed;
-    int pass;
+    int pass;             // pass index, used when pass_dev is null
+    const int* pass_dev;  // pass index in device memory (written by the pass reset), so that
+                          // one captured launch sequence serves every pass
 };
+
+// The pass index of the sample streams.
+template <typename S>
+__device__ __forceinline__ int pass_index(const S& sc)
+{
+    return sc.pass_dev ? *sc.pass_dev : sc.pass;
+}
 
 struct BatchDesc {
     int64_t pixel0;
@@ -157,10 +166,11 @@ struct RenderLaunch {
     unsigned long long* totals = nullptr;
 };
 void render_batch(const RenderLaunch& rl, hipStream_t s);
-// Clears the pass totals, the batch index and (when counting) the traversal counters.
-void pass_reset(RenderBuffers& b, bool counters, hipStream_t s);
+// Clears the pass totals, the batch index and (when counting) the traversal counters,
+// and writes the pass index to pass_dev (when not null).
+void pass_reset(RenderBuffers& b, bool counters, int* pass_dev, int pass, hipStream_t s);
 // Clears a scene's own pass totals (3 u64) and batch index.
-void pass_reset_state(unsigned long long* totals, long long* batch_idx, hipStream_t s);
+void pass_reset_state(unsigned long long* totals, long long* batch_idx, int* pass_dev, int pass, hipStream_t s);
 void alloc_render_buffers(RenderBuffers& b, int n, int precision, int64_t cap, int max_depth);
 
 }  // namespace nsp

@@ -15,8 +15,8 @@ template <int N, typename T>
 __device__ __forceinline__ void primary_ray(const SceneK<T>& sc, int64_t pixel, int sample, T* org, T* dir)
 {
     T u[8];
-    uniforms4<T>(sc.seed, pixel, sample, sc.pass, 0, 0, u);
-    if constexpr (N - 1 > 4) uniforms4<T>(sc.seed, pixel, sample, sc.pass, 0, 1, u + 4);
+    uniforms4<T>(sc.seed, pixel, sample, pass_index(sc), 0, 0, u);
+    if constexpr (N - 1 > 4) uniforms4<T>(sc.seed, pixel, sample, pass_index(sc), 0, 1, u + 4);
     int64_t rem = pixel;
     int sr = sample;
     T sp[N - 1];
@@ -243,8 +243,8 @@ __device__ void shade_one(const SceneK<T>& sc, const TreeArgs& A, const MeshShad
     // diffuse_lighting, painter.cpp:298-323
     if (depth < sc.max_depth) {
         T u[8];
-        uniforms4<T>(sc.seed, pixel, sample, sc.pass, depth, 2, u);
-        if constexpr (N >= 6) uniforms4<T>(sc.seed, pixel, sample, sc.pass, depth, 3, u + 4);
+        uniforms4<T>(sc.seed, pixel, sample, pass_index(sc), depth, 2, u);
+        if constexpr (N >= 6) uniforms4<T>(sc.seed, pixel, sample, pass_index(sc), depth, 3, u + 4);
         T nd[N];
         cosine_hemisphere<N, T>(sn, u, nd);
         normalize<N, T>(nd);

@@ -11,7 +11,8 @@ the time-to-RMSE comparison of denoised against plain FP32 rendering.
              Rendering is progressive: at level j the slots hold j one-sample
              passes and j eight-sample passes, so level 1 is the plain 1-spp +
              4-spp input and the network can be applied at any sample budget
-  target     1024-spp (1331 samples) FP32 render
+  target     FP32 render of target_spp samples per pixel (2048 spp requests
+             13^3 = 2197 stratified samples)
   network    8 layers 3x3x3, 32 channels, BF16/FP32-accumulate, residual on the
              4-spp radiance, trained on 32^3 patches, batch 16, Adam
   judge      RMSE over pixels x 3 linear-RGB channels against a 4096-sample FP64
@@ -25,7 +26,7 @@ from . import configs, native
 
 IN_CHANNELS = 16
 RESIDUAL = 3  # the 4-spp radiance channels
-LEVELS = (1, 2, 4, 8, 16, 32)  # progressive levels stored for training
+LEVELS = (1, 2, 4, 8, 16, 32, 64, 128)  # progressive levels stored for training
 THRESHOLDS = (0.03, 0.02, 0.01)
 
 
@@ -147,7 +148,7 @@ def first_time(curve, threshold):
     return None
 
 
-def time_to_rmse(ctx, mesh, cfg, ref, denoiser, depth=5, thresholds=THRESHOLDS, max_passes=256, max_levels=96,
+def time_to_rmse(ctx, mesh, cfg, ref, denoiser, depth=5, thresholds=THRESHOLDS, max_passes=256, max_levels=160,
                  seed=77):
     """Error-versus-time curves of both methods against `ref`, and the time at
     which each first reaches each threshold.  Only GPU work is timed (paint

@@ -428,16 +428,22 @@ class Scene:
             pass
 
     def paint(self, spp, max_depth=5, precision=PRECISION_FP32, passes=1, seed=1, batch_paths=0, smooth=False,
-              use_graph=False, stop=None, on_pixels=None, aov=False, primary=PRIMARY_AUTO):
+              use_graph=False, stop=None, on_pixels=None, aov=False, primary=PRIMARY_AUTO, on_pass=None):
+        """passes=-1 paints until the stop flag (a ctypes.c_int32, which a
+        callback may set) is nonzero; a stopped paint raises NspError with
+        code NSP_E_STOPPED.  on_pixels(first, count, rgb_mean) follows each
+        wavefront batch, on_pass(pass_count, seconds) each pass."""
         p = PaintParams(spp, max_depth, int(smooth), precision, passes, seed, batch_paths, int(use_graph), int(aov),
                         int(primary))
         cb = None
-        keep = []
-        if on_pixels is not None:
+        if on_pixels is not None or on_pass is not None:
             def _px(user, first, count, rgb):
                 on_pixels(first, count, np.ctypeslib.as_array(rgb, shape=(count * 3,)).reshape(count, 3).copy())
-            cb = Callbacks(PIXELS_AFTER(_px), PASS_DONE(0), ERROR_MSG(0), None)
-            keep.append(cb)
+
+            def _pass(user, k, secs):
+                on_pass(k, secs)
+            cb = Callbacks(PIXELS_AFTER(_px) if on_pixels else PIXELS_AFTER(0),
+                           PASS_DONE(_pass) if on_pass else PASS_DONE(0), ERROR_MSG(0), None)
         stop_flag = C.c_int32(0) if stop is None else stop
         rc = self.ctx._L.nsp_paint(self.h, C.byref(p), C.byref(cb) if cb else None, C.byref(stop_flag))
         self.ctx.check(rc)

@@ -331,6 +331,70 @@ def test_graph_replay_equals_eager(ctx):
     assert eager.statistics()["rays"] == graph.statistics()["rays"]
 
 
+def test_passes_in_one_call_equal_separate_calls(ctx):
+    """k passes in one call (eager, and with one graph replayed across every
+    batch of every pass) equal k calls of one pass, bit for bit: the kernels
+    read the pass index from device memory, written by the pass reset."""
+    v, f, cfg, spp, depth = small_scene(4, 4, 10, spp=8)
+    gm = ctx.mesh(v, f)
+    one = native.Scene(ctx, gm, cfg)
+    for _ in range(3):
+        one.paint(spp, depth, seed=8, batch_paths=27 * 300)
+    ref = one.image()
+    for use_graph in (False, True):
+        print(f"three passes in one call, graph={use_graph}", flush=True)
+        sc = native.Scene(ctx, gm, cfg)
+        sc.paint(spp, depth, seed=8, batch_paths=27 * 300, passes=3, use_graph=use_graph)
+        got = sc.image()
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+        assert sc.statistics()["rays"] == one.statistics()["rays"]
+
+
+def test_unbounded_passes_until_stop(ctx):
+    # passes=-1 (BarPaintbrush max_pass_count -1) runs until the stop flag, here
+    # set by the pass callback after the third pass
+    import ctypes
+    v, f, cfg, spp, depth = small_scene(4, 3, 8, spp=4)
+    gm = ctx.mesh(v, f)
+    sc = native.Scene(ctx, gm, cfg)
+    stop = ctypes.c_int32(0)
+    seen = []
+
+    def on_pass(k, secs):
+        seen.append(k)
+        if k == 3:
+            stop.value = 1
+    with pytest.raises(native.NspError) as e:
+        sc.paint(spp, depth, seed=2, passes=-1, stop=stop, on_pass=on_pass, use_graph=True)
+    assert e.value.code == native.NSP_E_STOPPED
+    assert seen == [1, 2, 3] and sc.statistics()["passes"] == 3
+    ref = native.Scene(ctx, gm, cfg)
+    ref.paint(spp, depth, seed=2, passes=3)
+    assert np.array_equal(sc.image()[0], ref.image()[0])
+    # an unbounded paint without a stop flag is refused before any launch
+    params = native.PaintParams(spp, depth, 0, native.PRECISION_FP32, -1, 2, 0, 0, 0, 0)
+    assert native.lib().nsp_paint(sc.h, ctypes.byref(params), None, None) == native.NSP_E_INVALID
+
+
+def test_pixels_delivered_per_batch(ctx):
+    # each wavefront batch's pixels arrive once per pass, in order, with the
+    # running mean; after the last pass they equal the image
+    v, f, cfg, spp, depth = small_scene(4, 3, 9, spp=8)
+    gm = ctx.mesh(v, f)
+    sc = native.Scene(ctx, gm, cfg)
+    P = sc.pixels
+    runs = []
+    sc.paint(spp, depth, seed=5, batch_paths=27 * 100, passes=2,
+             on_pixels=lambda first, count, rgb: runs.append((first, count, rgb)))
+    starts = [r[0] for r in runs]
+    assert len(runs) == 2 * ((P + 99) // 100)
+    half = len(runs) // 2
+    assert starts[:half] == starts[half:] == list(range(0, P, 100))
+    s, c = sc.image()
+    last = np.concatenate([r[2] for r in runs[half:]])
+    assert np.allclose(last, s / c[:, None], rtol=1e-6, atol=0)
+
+
 @pytest.mark.parametrize("n", [3, 4, 5])
 def test_rasterised_primary_visibility_equals_tree_walk(ctx, n):
     # FP32 renders find the primary hits by rasterising the simplices into the
