@@ -174,14 +174,17 @@ typedef struct {
     double unit_intensity_distance; /* point light only */
 } nsp_light;
 
-#define NSP_PROJECTOR_PARALLEL 0 /* ParallelProjector, src/painter/projectors/projector.h:153-202 */
+#define NSP_PROJECTOR_PARALLEL 0    /* ParallelProjector, src/painter/projectors/projector.h:153-202 */
+#define NSP_PROJECTOR_PERSPECTIVE 1 /* PerspectiveProjector, projector.h:102-151 */
+#define NSP_PROJECTOR_SPHERICAL 2   /* SphericalProjector, projector.h:204-275 */
 typedef struct {
     int kind;
     double camera_org[NSP_MAX_DIM];
     double camera_dir[NSP_MAX_DIM];
     double screen_axes[NSP_MAX_DIM - 1][NSP_MAX_DIM];
-    double units_per_pixel;
+    double units_per_pixel;      /* parallel */
     int screen_size[NSP_MAX_DIM - 1];
+    double view_angle_degrees;   /* perspective and spherical: width_view_angle_degrees */
 } nsp_projector;
 
 /* SurfaceProperties (src/painter/objects.h:27-139): colour, diffuse, light source flag/colour. */
@@ -221,7 +224,8 @@ typedef struct {
 /* Primary visibility of FP32 renders.  Both give the nearest accepted hit of
  * every primary ray (the same FP32 test with the FP64 re-test inline):
  *   NSP_PRIMARY_AUTO    rasterise the simplices into the parallel projection
- *                       (every primary ray has the projector's direction)
+ *                       (every primary ray has the projector's direction);
+ *                       perspective and spherical projections walk
  *   NSP_PRIMARY_WALK    walk the tree per ray, as for the other bounces */
 #define NSP_PRIMARY_AUTO 0
 #define NSP_PRIMARY_WALK 1

@@ -66,6 +66,26 @@ def scene_dict(vertices, facets, screen_edge, light_pos, diffuse=0.8):
     return cfg
 
 
+def projector_dict(vertices, facets, screen_edge, light_pos, kind, view_angle, diffuse=0.8):
+    """scene_dict seen through a perspective or spherical projector
+    (PerspectiveProjector / SphericalProjector, src/painter/projectors/projector.h:102-275):
+    the same camera direction and screen axes, the camera on the last axis at
+    the distance where the view angle spans the object's largest projected
+    extent plus its half-diagonal."""
+    cfg = scene_dict(vertices, facets, screen_edge, light_pos, diffuse)
+    vmin, vmax = mesh_min_max(vertices, facets)
+    size = vmax - vmin
+    center = vmin + size / 2
+    n = vertices.shape[1]
+    half = float(np.max(size[: n - 1])) / 2
+    dist = half / np.tan(np.radians(view_angle) / 2) + float(np.linalg.norm(size)) / 2
+    cam = center.copy()
+    cam[n - 1] = center[n - 1] + dist
+    cfg.update(projector=kind, view_angle=float(view_angle), camera_org=cam.tolist())
+    cfg.pop("units_per_pixel")
+    return cfg
+
+
 CONFIGS = {
     # name: (generator, generator args, screen edge, spp, depth, light)
     "C0": ("torus", dict(R=1.0, r=0.35, nu=200, nv=100, noise=0.01, seed=0), 512, 16, 5, (3.0, 3.0, 3.0)),

@@ -28,6 +28,8 @@ void check_hip(hipError_t e, const char* what)
     raise(code, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+constexpr double PI_D = 3.14159265358979323846;
+
 // TREE_MIN_OBJECTS_PER_BOX and tree_max_depth<N>(), src/painter/shapes/mesh.cpp:33-58
 constexpr int TREE_MIN_OBJECTS_PER_BOX = 10;
 static int tree_max_depth(int n)
@@ -173,12 +175,23 @@ static void fill_scene(const nsp_scene& S, const nsp_paint_params& p, int pass, 
         k.cam_org[i] = (T)pr.camera_org[i];
         k.cam_dir[i] = (T)cd[i];
     }
+    // projector constants (projector.h:110-136 perspective, 161-185 parallel, 212-249 spherical)
+    k.proj = pr.kind;
+    const double scale = pr.kind == NSP_PROJECTOR_PARALLEL ? pr.units_per_pixel : 1.0;
     for (int a = 0; a < n - 1; ++a) {
         double r = pr.screen_axes[a][0] * pr.screen_axes[a][0];
         for (int i = 1; i < n; ++i) r = std::fma(pr.screen_axes[a][i], pr.screen_axes[a][i], r);
         double l = std::sqrt(r);
-        for (int i = 0; i < n; ++i) k.axes[a][i] = (T)((pr.screen_axes[a][i] / l) * pr.units_per_pixel);
+        for (int i = 0; i < n; ++i) k.axes[a][i] = (T)((pr.screen_axes[a][i] / l) * scale);
         k.screen[a] = pr.screen_size[a];
+    }
+    const double half_angle = pr.view_angle_degrees * 0.5 * PI_D / 180;
+    if (pr.kind == NSP_PROJECTOR_PERSPECTIVE) {
+        const double dir_length = pr.screen_size[0] * 0.5 * std::tan(PI_D / 2 - half_angle);
+        for (int i = 0; i < n; ++i) k.cam_dir[i] = (T)(cd[i] * dir_length);
+    } else if (pr.kind == NSP_PROJECTOR_SPHERICAL) {
+        const double r = pr.screen_size[0] * 0.5 / std::sin(half_angle);
+        k.proj_r2 = (T)(r * r);
     }
     k.pixels = S.pixels;
     k.k = one_dimension_size(n - 1, p.samples_per_pixel);
@@ -495,7 +508,9 @@ int nsp_scene_create(nsp_context* ctx, int n, nsp_mesh* mesh, const nsp_surface*
         if (light_count < 0 || (light_count > 0 && !lights)) raise(NSP_E_INVALID, "bad light list");
         if (light_count > 1)
             raise(NSP_E_UNSUPPORTED, "more than one light source is not supported by the GPU painter yet");
-        if (projector->kind != NSP_PROJECTOR_PARALLEL) raise(NSP_E_UNSUPPORTED, "only the parallel projector is supported");
+        if (projector->kind != NSP_PROJECTOR_PARALLEL && projector->kind != NSP_PROJECTOR_PERSPECTIVE &&
+            projector->kind != NSP_PROJECTOR_SPHERICAL)
+            raise(NSP_E_INVALID, "unknown projector kind " + std::to_string(projector->kind));
         int64_t pixels = 1;
         for (int i = 0; i < n - 1; ++i) {
             // screen_org, src/painter/projectors/projector.h:51-66; pixel coordinates are int_least16_t
@@ -505,7 +520,25 @@ int nsp_scene_create(nsp_context* ctx, int n, nsp_mesh* mesh, const nsp_surface*
             if (projector->screen_size[i] > 32767) raise(NSP_E_INVALID, "screen size exceeds int_least16_t pixels");
             pixels *= projector->screen_size[i];
         }
-        if (!(projector->units_per_pixel > 0)) raise(NSP_E_INVALID, "Error units per pixel for parallel projection");
+        const double angle = projector->view_angle_degrees;
+        if (projector->kind == NSP_PROJECTOR_PARALLEL) {
+            if (!(projector->units_per_pixel > 0))
+                raise(NSP_E_INVALID, "Error units per pixel for parallel projection");
+        } else if (projector->kind == NSP_PROJECTOR_PERSPECTIVE) {
+            // PerspectiveProjector, projector.h:120-136
+            if (!(angle > 0 && angle < 180))
+                raise(NSP_E_INVALID, "Perspective projection: error view angle " + std::to_string(angle));
+        } else {
+            // SphericalProjector, projector.h:222-249
+            const double sin_alpha = std::sin(angle * 0.5 * PI_D / 180);
+            if (!(angle > 0))
+                raise(NSP_E_INVALID, "Spherical projection view angle " + std::to_string(angle) + " is not positive");
+            const double k = sin_alpha / projector->screen_size[0];
+            double r = sin_alpha * sin_alpha;
+            for (int i = 1; i < n - 1; ++i) r += (k * projector->screen_size[i]) * (k * projector->screen_size[i]);
+            if (!(r < 1))
+                raise(NSP_E_INVALID, "Spherical projection view angle " + std::to_string(angle) + " is too big");
+        }
         // unit_dir_and_axes + check_vectors_orthogonal, projector.h:30-86
         auto len = [n](const double* v) {
             double r = v[0] * v[0];
@@ -663,7 +696,8 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
         batch_pixels = std::min<int64_t>(batch_pixels, scene->pixels);
         const int64_t cap = batch_pixels * S;
         alloc_render_buffers(ctx->work, n, p.precision, cap, p.max_depth);
-        const bool raster = p.precision == NSP_PRECISION_FP32 && p.primary == NSP_PRIMARY_AUTO;
+        const bool raster = p.precision == NSP_PRECISION_FP32 && p.primary == NSP_PRIMARY_AUTO &&
+                            scene->proj.kind == NSP_PROJECTOR_PARALLEL;
         if (raster) {
             raster_prepare(scene->raster, scene->mesh->m, kd, S, s);
             ctx->work.keys.alloc(sizeof(unsigned long long) * cap);
@@ -967,7 +1001,7 @@ int nsp_batch_paint(nsp_batch* b, const nsp_paint_params* params, const volatile
             bs.totals.alloc(sizeof(unsigned long long) * 3);
             NSP_HIP(hipMemcpyAsync(bs.list.p, descs.data(), sizeof(BatchDesc) * descs.size(), hipMemcpyHostToDevice,
                                    s));
-            if (raster) raster_prepare(sc->raster, sc->mesh->m, kd, S, s);
+            if (raster && sc->proj.kind == NSP_PROJECTOR_PARALLEL) raster_prepare(sc->raster, sc->mesh->m, kd, S, s);
             cost[i] = (double)sc->pixels * S * std::log2(2.0 + (double)sc->mesh->m.F);
         }
         // lanes by longest processing time first
@@ -1015,7 +1049,7 @@ int nsp_batch_paint(nsp_batch* b, const nsp_paint_params* params, const volatile
                     rl.max_depth = p.max_depth;
                     rl.counters = false;
                     rl.marks = nullptr;
-                    rl.raster = raster ? &sc->raster : nullptr;
+                    rl.raster = raster && sc->proj.kind == NSP_PROJECTOR_PARALLEL ? &sc->raster : nullptr;
                     rl.batch_list = bs.list.as<BatchDesc>();
                     rl.batch_idx = bs.idx.as<long long>();
                     rl.totals = bs.totals.as<unsigned long long>();

@@ -21,9 +21,11 @@ constexpr int MAX_LIGHTS = 1;
 // paint_threads, src/painter/painter.cpp:560-575).
 template <typename T>
 struct SceneK {
+    int proj;                // NSP_PROJECTOR_*
+    T proj_r2;               // spherical: squared radius of the sphere the screen is projected onto
     T cam_org[NMAX];
-    T cam_dir[NMAX];
-    T axes[NMAX - 1][NMAX];  // unit screen axes times units_per_pixel
+    T cam_dir[NMAX];         // unit; perspective: times the distance of the screen (in pixels)
+    T axes[NMAX - 1][NMAX];  // unit screen axes; parallel: times units_per_pixel
     int screen[NMAX - 1];
     int64_t pixels;
     int k;        // samples per screen dimension

@@ -1,6 +1,4 @@
-// nsp_stages.hpp -- per-ray bodies of the wavefront stages, shared by the
-// single-scene kernels (nsp_render.hip) and the multi-scene persistent
-// kernels (nsp_batch.hip).
+// nsp_stages.hpp -- per-ray bodies of the wavefront stages (nsp_render.hip).
 #pragma once
 
 #include "nsp_device.hpp"
@@ -9,8 +7,10 @@
 
 namespace nsp {
 
-// paint_pixels' sample + ParallelProjector::ray (src/painter/painter.cpp:419-442,
-// src/painter/sampling/engine.h:86-156, src/painter/projectors/projector.h:194-201)
+// paint_pixels' sample + Projector::ray (src/painter/painter.cpp:419-442,
+// src/painter/sampling/engine.h:86-156, src/painter/projectors/projector.h:143-150
+// (perspective), 194-201 (parallel), 259-274 (spherical)); Ray's constructor
+// normalises the direction (src/com/ray.h:34-36)
 template <int N, typename T>
 __device__ __forceinline__ void primary_ray(const SceneK<T>& sc, int64_t pixel, int sample, T* org, T* dir)
 {
@@ -30,14 +30,34 @@ __device__ __forceinline__ void primary_ray(const SceneK<T>& sc, int64_t pixel, 
         T spt = off + u[i] * sc.k_inv;
         sp[i] = (-(T)sc.screen[i] * (T)0.5) + ((T)x + spt);
     }
+    T sd[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        T sd = sc.axes[0][j] * sp[0];
+        sd[j] = sc.axes[0][j] * sp[0];
 #pragma unroll
-        for (int i = 1; i < N - 1; ++i) sd = sd + sc.axes[i][j] * sp[i];
-        org[j] = sc.cam_org[j] + sd;
-        dir[j] = sc.cam_dir[j];
+        for (int i = 1; i < N - 1; ++i) sd[j] = sd[j] + sc.axes[i][j] * sp[i];
     }
+    if (sc.proj == NSP_PROJECTOR_PARALLEL) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            org[j] = sc.cam_org[j] + sd[j];
+            dir[j] = sc.cam_dir[j];
+        }
+        return;
+    }
+    T z = 1;
+    if (sc.proj == NSP_PROJECTOR_SPHERICAL) {
+        T r = sp[0] * sp[0];
+#pragma unroll
+        for (int i = 1; i < N - 1; ++i) r = fma(sp[i], sp[i], r);
+        z = sqrt(sc.proj_r2 - r);  // positive: the scene constructor bounds the view angle
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        org[j] = sc.cam_org[j];
+        dir[j] = sc.cam_dir[j] * z + sd[j];
+    }
+    normalize<N, T>(dir);
 }
 
 template <int N, typename T>

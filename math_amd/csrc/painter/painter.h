@@ -5,6 +5,7 @@
 // threads (src/painter/objects.h:143-223, src/painter/painter.h:26-42).  Here
 // the concrete classes callers construct -- Mesh, VisibleSharedMesh,
 // VisiblePointLight, VisibleConstantLight, VisibleParallelProjector,
+// VisiblePerspectiveProjector, VisibleSphericalProjector,
 // VisibleBarPaintbrush, single_object_scene -- carry their data, and
 // paint<N,T>() hands that data to the GPU through the C ABI.  Names,
 // argument meaning and error behaviour follow the reference:
@@ -278,32 +279,100 @@ public:
     nsp_light data() const override { return m_light; }
 };
 
+namespace nsp_painter {
+// the camera, axes and screen of a projector (projector.h:51-86)
+template <size_t N, typename T>
+nsp_projector projector_data(int kind, const Vector<N, T>& camera_org, const Vector<N, T>& camera_dir,
+                             const std::array<Vector<N, T>, N - 1>& screen_axes,
+                             const std::array<int, N - 1>& screen_size)
+{
+    nsp_projector p{};
+    p.kind = kind;
+    for (size_t i = 0; i < N; ++i) {
+        p.camera_org[i] = (double)camera_org[i];
+        p.camera_dir[i] = (double)camera_dir[i];
+    }
+    for (size_t a = 0; a + 1 < N; ++a) {
+        for (size_t i = 0; i < N; ++i) p.screen_axes[a][i] = (double)screen_axes[a][i];
+        p.screen_size[a] = screen_size[a];
+        if (screen_size[a] < 1)
+            throw ErrorException("Projection size " + std::to_string(a) + " is not positive (" +
+                                 std::to_string(screen_size[a]) + ")");
+    }
+    return p;
+}
+}  // namespace nsp_painter
+
+// VisiblePerspectiveProjector (src/painter/visible_projectors.h:24-46, PerspectiveProjector
+// projector.h:102-151)
+template <size_t N, typename T>
+class VisiblePerspectiveProjector final : public Projector<N, T> {
+    nsp_projector m_p;
+    std::array<int, N - 1> m_size;
+
+public:
+    VisiblePerspectiveProjector(const Vector<N, T>& camera_org, const Vector<N, T>& camera_dir,
+                                const std::array<Vector<N, T>, N - 1>& screen_axes, T width_view_angle_degrees,
+                                const std::array<int, N - 1>& screen_size)
+        : m_p(nsp_painter::projector_data<N, T>(NSP_PROJECTOR_PERSPECTIVE, camera_org, camera_dir, screen_axes,
+                                                screen_size)),
+          m_size(screen_size)
+    {
+        if (!(width_view_angle_degrees > 0 && width_view_angle_degrees < 180))
+            throw ErrorException("Perspective projection: error view angle " +
+                                 std::to_string((double)width_view_angle_degrees));
+        m_p.view_angle_degrees = (double)width_view_angle_degrees;
+    }
+    const std::array<int, N - 1>& screen_size() const override { return m_size; }
+    nsp_projector data() const override { return m_p; }
+};
+
 // VisibleParallelProjector (src/painter/visible_projectors.h:48-70)
 template <size_t N, typename T>
 class VisibleParallelProjector final : public Projector<N, T> {
-    nsp_projector m_p{};
+    nsp_projector m_p;
     std::array<int, N - 1> m_size;
 
 public:
     VisibleParallelProjector(const Vector<N, T>& camera_org, const Vector<N, T>& camera_dir,
                              const std::array<Vector<N, T>, N - 1>& screen_axes, T units_per_pixel,
                              const std::array<int, N - 1>& screen_size)
-        : m_size(screen_size)
+        : m_p(nsp_painter::projector_data<N, T>(NSP_PROJECTOR_PARALLEL, camera_org, camera_dir, screen_axes,
+                                                screen_size)),
+          m_size(screen_size)
     {
         if (!(units_per_pixel > 0)) throw ErrorException("Error units per pixel for parallel projection");
-        m_p.kind = NSP_PROJECTOR_PARALLEL;
-        for (size_t i = 0; i < N; ++i) {
-            m_p.camera_org[i] = (double)camera_org[i];
-            m_p.camera_dir[i] = (double)camera_dir[i];
-        }
-        for (size_t a = 0; a + 1 < N; ++a) {
-            for (size_t i = 0; i < N; ++i) m_p.screen_axes[a][i] = (double)screen_axes[a][i];
-            m_p.screen_size[a] = screen_size[a];
-            if (screen_size[a] < 1)
-                throw ErrorException("Projection size " + std::to_string(a) + " is not positive (" +
-                                     std::to_string(screen_size[a]) + ")");
-        }
         m_p.units_per_pixel = (double)units_per_pixel;
+    }
+    const std::array<int, N - 1>& screen_size() const override { return m_size; }
+    nsp_projector data() const override { return m_p; }
+};
+
+// VisibleSphericalProjector (src/painter/visible_projectors.h:72-94, SphericalProjector
+// projector.h:204-275)
+template <size_t N, typename T>
+class VisibleSphericalProjector final : public Projector<N, T> {
+    nsp_projector m_p;
+    std::array<int, N - 1> m_size;
+
+public:
+    VisibleSphericalProjector(const Vector<N, T>& camera_org, const Vector<N, T>& camera_dir,
+                              const std::array<Vector<N, T>, N - 1>& screen_axes, T width_view_angle_degrees,
+                              const std::array<int, N - 1>& screen_size)
+        : m_p(nsp_painter::projector_data<N, T>(NSP_PROJECTOR_SPHERICAL, camera_org, camera_dir, screen_axes,
+                                                screen_size)),
+          m_size(screen_size)
+    {
+        const double angle = (double)width_view_angle_degrees;
+        const double sin_alpha = std::sin(angle * 0.5 * 3.14159265358979323846 / 180);
+        if (!(angle > 0))
+            throw ErrorException("Spherical projection view angle " + std::to_string(angle) + " is not positive");
+        const double k = sin_alpha / screen_size[0];
+        double r = sin_alpha * sin_alpha;
+        for (size_t i = 1; i + 1 < N; ++i) r += (k * screen_size[i]) * (k * screen_size[i]);
+        if (!(r < 1))
+            throw ErrorException("Spherical projection view angle " + std::to_string(angle) + " is too big");
+        m_p.view_angle_degrees = angle;
     }
     const std::array<int, N - 1>& screen_size() const override { return m_size; }
     nsp_projector data() const override { return m_p; }

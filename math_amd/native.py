@@ -19,6 +19,10 @@ NSP_E_NOMEM = -3
 NSP_E_STOPPED = -4
 NSP_E_UNSUPPORTED = -5
 
+PROJECTOR_PARALLEL = 0
+PROJECTOR_PERSPECTIVE = 1
+PROJECTOR_SPHERICAL = 2
+
 PRECISION_FP32 = 0
 PRECISION_FP64 = 1
 LIGHT_POINT = 0
@@ -43,7 +47,7 @@ class Light(C.Structure):
 class Projector(C.Structure):
     _fields_ = [("kind", C.c_int), ("camera_org", C.c_double * MAX_DIM), ("camera_dir", C.c_double * MAX_DIM),
                 ("screen_axes", (C.c_double * MAX_DIM) * (MAX_DIM - 1)), ("units_per_pixel", C.c_double),
-                ("screen_size", C.c_int * (MAX_DIM - 1))]
+                ("screen_size", C.c_int * (MAX_DIM - 1)), ("view_angle_degrees", C.c_double)]
 
 
 class Surface(C.Structure):
@@ -402,13 +406,14 @@ class Scene:
         light.rgb[:] = cfg["light_rgb"]
         light.unit_intensity_distance = cfg["light_unit_distance"]
         pr = Projector()
-        pr.kind = 0
+        pr.kind = cfg.get("projector", PROJECTOR_PARALLEL)
         pr.camera_org[:n] = cfg["camera_org"]
         pr.camera_dir[:n] = cfg["camera_dir"]
         for a in range(n - 1):
             pr.screen_axes[a][:n] = cfg["axes"][a]
             pr.screen_size[a] = cfg["screen"][a]
-        pr.units_per_pixel = cfg["units_per_pixel"]
+        pr.units_per_pixel = cfg.get("units_per_pixel", 0.0)
+        pr.view_angle_degrees = cfg.get("view_angle", 0.0)
         h = C.c_void_p()
         ctx.check(L.nsp_scene_create(ctx.h, n, mesh.h, C.byref(obj), C.byref(light), 1, C.byref(pr), C.byref(bg),
                                      C.byref(h)))

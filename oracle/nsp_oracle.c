@@ -971,8 +971,9 @@ void orc_cosine_hemisphere(int n, const double* normal, const double* u, double*
 typedef struct {
     const orc_mesh* m;
     const orc_scene* s;
-    double cam_dir[NM];
-    double axes[NM - 1][NM];
+    double cam_dir[NM];        /* perspective: times the screen distance */
+    double axes[NM - 1][NM];   /* parallel: times units_per_pixel */
+    double square_radius;      /* spherical */
     double ray_offset;
     double coef;
     double light_coef;
@@ -1138,19 +1139,90 @@ static void trace_path(const pctx* c, pstate* st, int level, float color_level, 
     for (int k = 0; k < 3; ++k) out[k] = out[k] + sc[k] * (direct[k] + diffuse[k]);
 }
 
+/* Projector constructors (src/painter/projectors/projector.h): unit_dir_and_axes
+ * (:68-86), then PerspectiveProjector (:110-136): camera_dir times
+ * screen_size[0]/2 * tan(pi/2 - half_angle); ParallelProjector (:161-185): axes
+ * times units_per_pixel; SphericalProjector (:212-249): square_radius =
+ * (screen_size[0]/2 / sin(half_angle))^2, with its view-angle checks. */
+static int projector_setup(const orc_scene* s, int n, double* cam_dir, double axes[][NM], double* square_radius)
+{
+    memcpy(cam_dir, s->cam_dir, sizeof(double) * n);
+    normalize_n(cam_dir, n);
+    for (int a = 0; a < n - 1; ++a) {
+        memcpy(axes[a], s->axes[a], sizeof(double) * n);
+        normalize_n(axes[a], n);
+    }
+    const double PI_D = 3.14159265358979323846;
+    const double half_angle = s->view_angle_degrees * 0.5 * PI_D / 180;
+    *square_radius = 0;
+    if (s->projector == 0) {
+        if (!(s->units_per_pixel > 0)) return fail("Error units per pixel for parallel projection");
+        for (int a = 0; a < n - 1; ++a)
+            for (int i = 0; i < n; ++i) axes[a][i] = axes[a][i] * s->units_per_pixel;
+    } else if (s->projector == 1) {
+        if (!(s->view_angle_degrees > 0 && s->view_angle_degrees < 180))
+            return fail("Perspective projection: error view angle %g", s->view_angle_degrees);
+        double dir_length = s->screen[0] * 0.5 * tan(PI_D / 2 - half_angle);
+        for (int i = 0; i < n; ++i) cam_dir[i] = cam_dir[i] * dir_length;
+    } else if (s->projector == 2) {
+        double sin_alpha = sin(half_angle);
+        if (!(s->view_angle_degrees > 0)) return fail("Spherical projection view angle is not positive");
+        double k = sin_alpha / s->screen[0];
+        double r = sin_alpha * sin_alpha;
+        for (int i = 1; i < n - 1; ++i) r += (k * s->screen[i]) * (k * s->screen[i]);
+        if (!(r < 1)) return fail("Spherical projection view angle %g is too big", s->view_angle_degrees);
+        double rad = s->screen[0] * 0.5 / sin_alpha;
+        *square_radius = rad * rad;
+    } else {
+        return fail("unknown projector %d", s->projector);
+    }
+    return 0;
+}
+
+/* Projector::ray (projector.h:143-150, 194-201, 259-274) at screen point
+ * sp = screen_org + point; the direction is normalised as Ray's constructor
+ * does (src/com/ray.h:34-36). */
+static void projector_ray(const orc_scene* s, int n, const double* cam_dir, double axes[][NM], double square_radius,
+                          const double* sp, double* org, double* dir)
+{
+    double sd[NM];
+    for (int j = 0; j < n; ++j) {
+        sd[j] = axes[0][j] * sp[0];
+        for (int i = 1; i < n - 1; ++i) sd[j] = sd[j] + axes[i][j] * sp[i];
+    }
+    if (s->projector == 0) {
+        for (int j = 0; j < n; ++j) {
+            org[j] = s->cam_org[j] + sd[j];
+            dir[j] = cam_dir[j];
+        }
+        return;
+    }
+    double z = 1;
+    if (s->projector == 2) z = sqrt(square_radius - dotn(sp, sp, n - 1));
+    for (int j = 0; j < n; ++j) {
+        org[j] = s->cam_org[j];
+        dir[j] = cam_dir[j] * z + sd[j];
+    }
+    normalize_n(dir, n);
+}
+
+int orc_projector_ray(const orc_scene* s, const double* point, double* org, double* dir)
+{
+    int n = s->n;
+    double cam_dir[NM], axes[NM - 1][NM], r2, sp[NM - 1];
+    if (projector_setup(s, n, cam_dir, axes, &r2)) return 1;
+    for (int i = 0; i < n - 1; ++i) sp[i] = -s->screen[i] * 0.5 + point[i];
+    projector_ray(s, n, cam_dir, axes, r2, sp, org, dir);
+    return 0;
+}
+
 static int make_ctx(const orc_mesh* m, const orc_scene* s, pctx* c)
 {
     int n = m->n;
     if (s->n != n) return fail("scene dimension %d != mesh dimension %d", s->n, n);
     c->m = m;
     c->s = s;
-    memcpy(c->cam_dir, s->cam_dir, sizeof(double) * n);
-    normalize_n(c->cam_dir, n);
-    for (int a = 0; a < n - 1; ++a) {
-        memcpy(c->axes[a], s->axes[a], sizeof(double) * n);
-        normalize_n(c->axes[a], n);
-        for (int i = 0; i < n; ++i) c->axes[a][i] = c->axes[a][i] * s->units_per_pixel;
-    }
+    if (projector_setup(s, n, c->cam_dir, c->axes, &c->square_radius)) return 1;
     /* compute_ray_offset, painter.cpp:539-558 */
     double all_max = -INFINITY;
     for (int i = 0; i < n; ++i) all_max = fmax(all_max, fmax(fabs(m->mn[i]), fabs(m->mx[i])));
@@ -1187,16 +1259,12 @@ static void sample_color(const pctx* c, pstate* st, float* rgb)
         double off = (double)d / c->k;
         double jitter = u[i] * rk;
         double spt = off + jitter;
-        /* ParallelProjector::ray, projector.h:194-201 */
+        /* screen_org + point, projector.h:51-66 */
         sp[i] = (-s->screen[i] * 0.5) + ((double)x + spt);
     }
-    double org[NM];
-    for (int j = 0; j < n; ++j) {
-        double sd = c->axes[0][j] * sp[0];
-        for (int i = 1; i < n - 1; ++i) sd = sd + c->axes[i][j] * sp[i];
-        org[j] = s->cam_org[j] + sd;
-    }
-    trace_path(c, st, 0, 1.0f, org, c->cam_dir, 0, rgb);
+    double org[NM], dir[NM];
+    projector_ray(s, n, c->cam_dir, (double (*)[NM])c->axes, c->square_radius, sp, org, dir);
+    trace_path(c, st, 0, 1.0f, org, dir, 0, rgb);
 }
 
 void orc_trace_path_sample(const orc_mesh* m, const orc_scene* s, int64_t pixel, int sample, float* rgb,

@@ -87,7 +87,7 @@ void orc_trace_brute(const orc_mesh* m, int64_t count, const float* org, const f
 void orc_trace_tree(const orc_mesh* m, int64_t count, const float* org, const float* dir,
                     int32_t* idx, double* t);
 
-/* Painter scene (single mesh, one point light, parallel projector), the data of
+/* Painter scene (single mesh, one point light, one projector), the data of
  * PaintObjects built by single_object_scene (src/painter/scenes/single_object.h:87-179)
  * with VisiblePointLight instead of the constant light. */
 typedef struct {
@@ -108,7 +108,15 @@ typedef struct {
     int smooth_normal;
     uint64_t seed;
     int pass_index;
+    int projector;             /* 0 parallel, 1 perspective, 2 spherical */
+    double view_angle_degrees; /* perspective and spherical */
 } orc_scene;
+
+/* Projector::ray for a screen point in pixel units (the `point` argument):
+ * PerspectiveProjector, ParallelProjector, SphericalProjector
+ * (src/painter/projectors/projector.h:102-275); dir is unit (Ray ctor).
+ * Returns nonzero on an invalid projector. */
+int orc_projector_ray(const orc_scene* scene, const double* point, double* org, double* dir);
 
 /* StratifiedJitteredSampler::one_dimension_size, src/painter/sampling/sampler.h:31-52 */
 int orc_samples_per_dimension(int screen_dims, int spp);

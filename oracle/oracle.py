@@ -19,7 +19,8 @@ class Scene(C.Structure):
                 ("screen", C.c_int * (NMAX - 1)), ("light_pos", C.c_double * NMAX), ("light_rgb", C.c_float * 3),
                 ("light_unit_distance", C.c_double), ("background_rgb", C.c_float * 3),
                 ("object_rgb", C.c_float * 3), ("diffuse", C.c_float), ("samples_per_pixel", C.c_int),
-                ("max_depth", C.c_int), ("smooth_normal", C.c_int), ("seed", C.c_uint64), ("pass_index", C.c_int)]
+                ("max_depth", C.c_int), ("smooth_normal", C.c_int), ("seed", C.c_uint64), ("pass_index", C.c_int),
+                ("projector", C.c_int), ("view_angle_degrees", C.c_double)]
 
 
 _lib = None
@@ -199,7 +200,9 @@ def make_scene(cfg, spp, max_depth, seed, smooth=False, pass_index=0):
     for a in range(n - 1):
         s.axes[a][:n] = cfg["axes"][a]
         s.screen[a] = cfg["screen"][a]
-    s.units_per_pixel = cfg["units_per_pixel"]
+    s.units_per_pixel = cfg.get("units_per_pixel", 0.0)
+    s.projector = cfg.get("projector", 0)
+    s.view_angle_degrees = cfg.get("view_angle", 0.0)
     s.light_pos[:n] = cfg["light_pos"]
     s.light_rgb[:] = cfg["light_rgb"]
     s.light_unit_distance = cfg["light_unit_distance"]
@@ -232,3 +235,17 @@ def path_sample(mesh, scene, pixel, sample):
     rays = C.c_int64()
     lib().orc_trace_path_sample(mesh.h, C.byref(scene), pixel, sample, _p(rgb, C.c_float), C.byref(rays))
     return rgb, rays.value
+
+
+def projector_ray(cfg, point):
+    """Projector::ray at a screen point in pixel units: (org, unit dir)."""
+    s = make_scene(cfg, 1, 1, 0)
+    n = cfg["n"]
+    p = np.ascontiguousarray(point, np.float64)
+    org, d = np.zeros(n), np.zeros(n)
+    L = lib()
+    L.orc_projector_ray.argtypes = [C.POINTER(Scene), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double)]
+    if L.orc_projector_ray(C.byref(s), _p(p, C.c_double), _p(org, C.c_double), _p(d, C.c_double)):
+        raise ValueError("invalid projector")
+    return org, d

@@ -100,6 +100,51 @@ int main(int argc, char** argv)
             std::printf("FAIL thread-count error not reported\n");
             return 1;
         }
+        // the other projectors through the explicit-projector single_object_scene
+        // (single_object.h:87-100): camera 3.5 units up the last axis, looking down it
+        for (int kind = 0; kind < 2; ++kind) {
+            Vector<4, double> org{0.0, 0.0, 0.0, 3.5}, dir{0.0, 0.0, 0.0, -1.0};
+            std::array<Vector<4, double>, 3> axes = {Vector<4, double>{1.0, 0.0, 0.0, 0.0},
+                                                     Vector<4, double>{0.0, 1.0, 0.0, 0.0},
+                                                     Vector<4, double>{0.0, 0.0, 1.0, 0.0}};
+            std::array<int, 3> sz = {max_screen, max_screen, max_screen};
+            std::unique_ptr<const Projector<4, double>> proj;
+            if (kind == 0)
+                proj = std::make_unique<const VisiblePerspectiveProjector<4, double>>(org, dir, axes, 50.0, sz);
+            else
+                proj = std::make_unique<const VisibleSphericalProjector<4, double>>(org, dir, axes, 50.0, sz);
+            auto light = std::make_unique<const VisiblePointLight<4, double>>(Vector<4, double>{2.0, 2.0, 2.0, 2.0},
+                                                                              Color(1), 4.0);
+            auto sc = single_object_scene<4, double>(BACKGROUND_COLOR, DEFAULT_COLOR, 1, std::move(proj),
+                                                     std::move(light), mesh);
+            Images im;
+            VisibleBarPaintbrush<3> b3(sz, 2, 1);
+            paint(&im, spp, *sc, &b3, 8, &stop, false);
+            size_t want = (size_t)sz[0] * sz[1] * sz[2];
+            const Color bg = BACKGROUND_COLOR;
+            size_t object = 0;
+            for (auto& [p, c] : im.pixels) object += std::fabs(c.r - bg.r) + std::fabs(c.b - bg.b) > 1e-3f;
+            if (im.errors || im.pixels.size() != want || object == 0 || object == want) {
+                std::printf("FAIL projector %d: errors %d (%s), pixels %zu, object pixels %zu\n", kind, im.errors,
+                            im.last_error.c_str(), im.pixels.size(), object);
+                return 1;
+            }
+        }
+        bool threw = false;
+        try {
+            VisiblePerspectiveProjector<4, double> p(Vector<4, double>{0.0, 0.0, 0.0, 3.5},
+                                                     Vector<4, double>{0.0, 0.0, 0.0, -1.0},
+                                                     {Vector<4, double>{1.0, 0.0, 0.0, 0.0},
+                                                      Vector<4, double>{0.0, 1.0, 0.0, 0.0},
+                                                      Vector<4, double>{0.0, 0.0, 1.0, 0.0}},
+                                                     180.0, {4, 4, 4});
+        } catch (const std::exception&) {
+            threw = true;
+        }
+        if (!threw) {
+            std::printf("FAIL perspective view angle 180 accepted\n");
+            return 1;
+        }
         std::printf("OK %zu %lld %.6f %lld\n", images.pixels.size(), rays, lum / expect, passes);
         return 0;
     } catch (const std::exception& e) {

@@ -195,6 +195,40 @@ def test_render_smooth_normals_matches_checker(ctx, n, precision):
     assert abs(st["rays"] - rays) <= 0.002 * rays
 
 
+@pytest.mark.parametrize("n,kind,precision", [(3, native.PROJECTOR_PERSPECTIVE, native.PRECISION_FP64),
+                                              (4, native.PROJECTOR_PERSPECTIVE, native.PRECISION_FP64),
+                                              (4, native.PROJECTOR_SPHERICAL, native.PRECISION_FP64),
+                                              (4, native.PROJECTOR_PERSPECTIVE, native.PRECISION_FP32),
+                                              (3, native.PROJECTOR_SPHERICAL, native.PRECISION_FP32)])
+def test_render_other_projectors_match_checker(ctx, n, kind, precision):
+    # PerspectiveProjector and SphericalProjector (projector.h:102-151, 204-275): primary rays from
+    # the camera point; FP32 renders walk the tree for them (rasterisation needs one direction)
+    g = {3: 8, 4: 4}[n]
+    edge = {3: 24, 4: 10}[n]
+    v, f = sphere_mesh(n, g)
+    cfg = configs.projector_dict(v, f, edge, tuple([2.0] * n), kind, 50.0)
+    print(f"projector {kind} n={n} precision={precision}", flush=True)
+    gm = ctx.mesh(v, f)
+    sc = native.Scene(ctx, gm, cfg)
+    sc.paint(4, 3, precision=precision, seed=12)
+    img, cnt = sc.image()
+    st = sc.statistics()
+    om = O.Mesh(v, f)
+    om.build_tree()
+    ref, rays = O.render(om, O.make_scene(cfg, 4, 3, 12), threads=8)
+    rmse = float(np.sqrt(np.mean((img / cnt[:, None] - ref / cnt[:, None]) ** 2)))
+    lit = float(np.mean(np.abs(ref / cnt[:, None] - np.array(configs.BACKGROUND_RGB)) > 1e-3))
+    print(f"rmse {rmse:.3e} rays gpu {st['rays']} cpu {rays}, object pixels {lit:.2f}", flush=True)
+    assert 0.2 < lit < 0.95  # the object fills part of the view
+    assert rmse < (1e-4 if precision == native.PRECISION_FP64 else 5e-3)
+    if precision == native.PRECISION_FP64:
+        assert abs(st["rays"] - rays) <= 0.001 * rays
+    else:
+        walk = native.Scene(ctx, gm, cfg)
+        walk.paint(4, 3, precision=precision, seed=12, primary=native.PRIMARY_WALK)
+        assert np.array_equal(walk.image()[0], img)
+
+
 def test_record_layouts_identical(ctx):
     # both FP32 record copies hold the same values: hits, distances and images are bit-identical
     v, f = sphere_mesh(4, 4)

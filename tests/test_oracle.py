@@ -193,3 +193,55 @@ def test_tree_leaf_rule_and_containment():
                 found = True
                 break
         assert found
+
+
+# ---------------------------------------------------------------------------
+# Projectors (src/painter/projectors/projector.h): closed forms of the ray directions
+
+def _proj_cfg(n, kind, angle, screen, upp=0.01):
+    cfg = dict(n=n, camera_org=[0.3] * n, camera_dir=[0.0] * (n - 1) + [-2.0],
+               axes=[[3.0 if j == i else 0.0 for j in range(n)] for i in range(n - 1)],
+               screen=list(screen), projector=kind, view_angle=angle, units_per_pixel=upp,
+               light_pos=[1.0] * n, light_rgb=[1, 1, 1], light_unit_distance=1.0, background_rgb=[0, 0, 0],
+               object_rgb=[1, 1, 1], diffuse=0.8)
+    return cfg
+
+
+@pytest.mark.parametrize("n", [3, 4, 5])
+def test_projector_ray_closed_forms(n):
+    rng = np.random.default_rng(n)
+    screen = [int(x) for x in rng.integers(20, 60, size=n - 1)]
+    w = screen[0]
+    for _ in range(20):
+        point = rng.random(n - 1) * screen
+        sp = point - np.array(screen) / 2
+        # parallel: origin moves on the screen plane, direction fixed
+        org, d = O.projector_ray(_proj_cfg(n, 0, 0.0, screen), point)
+        assert np.allclose(org[:n - 1], 0.3 + 0.01 * sp, atol=1e-12) and org[n - 1] == 0.3
+        assert np.allclose(d, [0] * (n - 1) + [-1])
+        # perspective: the screen at distance L = (w/2) / tan(angle/2) in pixel units
+        ang = float(rng.uniform(20, 150))
+        org, d = O.projector_ray(_proj_cfg(n, 1, ang, screen), point)
+        L = (w / 2) / np.tan(np.radians(ang) / 2)
+        expect = np.concatenate([sp, [-L]]) / np.sqrt(sp @ sp + L * L)
+        assert np.allclose(org, 0.3) and np.allclose(d, expect, atol=1e-12)
+        # spherical: the screen point lifted onto the sphere of radius R = (w/2) / sin(angle/2)
+        ang = float(rng.uniform(10, 60))
+        R = (w / 2) / np.sin(np.radians(ang) / 2)
+        if sp @ sp < R * R and (np.sin(np.radians(ang) / 2) ** 2) * (1 + sum((s / w) ** 2 for s in screen[1:])) < 1:
+            org, d = O.projector_ray(_proj_cfg(n, 2, ang, screen), point)
+            expect = np.concatenate([sp, [-np.sqrt(R * R - sp @ sp)]]) / R
+            assert np.allclose(d, expect, atol=1e-12)
+    # the edge of the screen is seen at half the view angle
+    for kind in (1, 2):
+        _, d = O.projector_ray(_proj_cfg(n, kind, 50.0, screen), [0.0] + [s / 2 for s in screen[1:]])
+        assert np.isclose(np.degrees(np.arccos(-d[n - 1])), 25.0)
+
+
+def test_projector_errors():
+    with pytest.raises(ValueError):
+        O.projector_ray(_proj_cfg(3, 1, 180.0, [10, 10]), [1, 1])
+    with pytest.raises(ValueError):
+        O.projector_ray(_proj_cfg(3, 2, 170.0, [10, 10]), [1, 1])  # too big for a square screen
+    with pytest.raises(ValueError):
+        O.projector_ray(_proj_cfg(3, 0, 0.0, [10, 10], upp=0.0), [1, 1])
