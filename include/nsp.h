@@ -204,6 +204,31 @@ int nsp_scene_create(nsp_context* ctx, int n, nsp_mesh* mesh, const nsp_surface*
                      nsp_scene** out);
 void nsp_scene_destroy(nsp_scene* scene);
 
+/* Flat-shaded shapes beside the mesh (src/painter/visible_shapes.h:31-117):
+ *   NSP_SHAPE_PARALLELOTOPE             VisibleParallelotope: org and n edge
+ *                                       vectors (Parallelotope, src/painter/space/parallelotope.h)
+ *   NSP_SHAPE_HYPERPLANE_PARALLELOTOPE  VisibleHyperplaneParallelotope: org and
+ *                                       n-1 edge vectors (space/hyperplane_parallelotope.h)
+ * Each has its own surface properties; a shape that is a light source emits
+ * light_rgb to diffuse bounces and shows rgb to primary rays (trace_path,
+ * painter.cpp:355-358), as the mesh object does. */
+#define NSP_SHAPE_PARALLELOTOPE 0
+#define NSP_SHAPE_HYPERPLANE_PARALLELOTOPE 1
+#define NSP_MAX_SHAPES 64
+typedef struct {
+    int kind;
+    double org[NSP_MAX_DIM];
+    double edges[NSP_MAX_DIM][NSP_MAX_DIM]; /* n (parallelotope) or n-1 edge vectors */
+    nsp_surface surface;
+} nsp_shape;
+/* Sets the scene's shapes (replacing any set before).  The scene's objects()
+ * are then the mesh followed by the shapes in this order; the order decides
+ * the ray counts of shadow rays, which test the objects one after another
+ * until one blocks the light (light_source_is_visible, painter.cpp:172-186).
+ * Smooth-normal renders of meshes with vertex normals are not supported
+ * together with shapes (NSP_E_UNSUPPORTED from nsp_paint). */
+int nsp_scene_set_shapes(nsp_scene* scene, const nsp_shape* shapes, int count);
+
 #define NSP_PRECISION_FP32 0 /* FP32 trace with FP64 near-boundary re-test */
 #define NSP_PRECISION_FP64 1 /* FP64 everywhere (reference-quality renders) */
 typedef struct {

@@ -86,6 +86,56 @@ def projector_dict(vertices, facets, screen_edge, light_pos, kind, view_angle, d
     return cfg
 
 
+def cornell_box(vertices, facets, width, height, size=None, object_rgb=(1.0, 1.0, 1.0), diffuse=1.0,
+                camera_direction=(0.0, 0.0, -1.0), camera_up=(0.0, 1.0, 0.0)):
+    """CornellBoxScene::create_scene (src/painter/scenes/cornell_box.cpp:114-210) for a
+    3-space mesh: five walls and a lamp (hyperplane parallelotopes, the lamp a light
+    source of colour 50), a box (parallelotope), no light sources, a black default
+    surface that is not a light, and the 70-degree perspective projector.  The mesh
+    is scaled to `size` about the origin (model_vertex_matrix); size defaults to 1."""
+    size = 1.0 if size is None else float(size)
+    v = np.asarray(vertices, np.float64)
+    used = v[np.unique(np.asarray(facets).ravel())]
+    lo, hi = used.min(axis=0), used.max(axis=0)
+    centre = (lo + hi) / 2
+    scale = size / float(np.max(hi - lo))
+    mesh_v = ((v - centre) * scale).astype(np.float32)
+    size *= 1.5
+    cdir = np.asarray(camera_direction, np.float64)
+    up_in = np.asarray(camera_up, np.float64)
+    right = np.cross(cdir, up_in)
+    right /= np.linalg.norm(right)
+    up = up_in / np.linalg.norm(up_in)
+    d = cdir / np.linalg.norm(cdir)
+    view_point = (1.0 / 6) * size * up - size * d
+    lower_left = view_point + 0.5 * size * (d - right - up)
+    lower_right = view_point + 0.5 * size * (d + right - up)
+    upper_left = view_point + 0.5 * size * (d - right + up)
+    white, red, green, magenta = [1.0, 1.0, 1.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [1.0, 0.0, 1.0]
+    hp, pp = native.SHAPE_HYPERPLANE_PARALLELOTOPE, native.SHAPE_PARALLELOTOPE
+
+    def shape(kind, org, edges, rgb, light_rgb=None):
+        return dict(kind=kind, org=list(map(float, org)), edges=[list(map(float, e)) for e in edges], rgb=rgb,
+                    diffuse=1.0, light_rgb=light_rgb)
+    upper_center = upper_left - 0.001 * size * up + 0.5 * size * right + 0.5 * size * d
+    shapes = [
+        shape(hp, upper_center - 0.1 * size * d - 0.1 * size * right, [0.2 * size * right, 0.2 * size * d], white,
+              [50.0, 50.0, 50.0]),
+        shape(hp, lower_left + size * d, [size * right, size * up], white),
+        shape(hp, upper_left, [size * d, size * right], white),
+        shape(hp, lower_left, [size * d, size * right], white),
+        shape(hp, lower_left, [size * d, size * up], red),
+        shape(hp, lower_right, [size * d, size * up], green),
+        shape(pp, lower_left + size * (0.7 * d + 0.8 * right + 0.1 * up),
+              [0.1 * size * right, 0.8 * size * up, 0.1 * size * d], magenta),
+    ]
+    cfg = dict(n=3, object_rgb=list(object_rgb), diffuse=diffuse, background_rgb=[0.0, 0.0, 0.0],
+               background_is_light=False, light_pos=None, projector=native.PROJECTOR_PERSPECTIVE, view_angle=70.0,
+               camera_org=view_point.tolist(), camera_dir=d.tolist(), axes=[right.tolist(), up.tolist()],
+               screen=[int(width), int(height)], shapes=shapes)
+    return mesh_v, np.asarray(facets, np.int32), cfg
+
+
 CONFIGS = {
     # name: (generator, generator args, screen edge, spp, depth, light)
     "C0": ("torus", dict(R=1.0, r=0.35, nu=200, nv=100, noise=0.01, seed=0), 512, 16, 5, (3.0, 3.0, 3.0)),

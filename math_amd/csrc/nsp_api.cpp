@@ -80,6 +80,7 @@ using namespace nsp;
 struct nsp_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;  // the shadow stages of nsp_paint (joined back before each batch ends)
     std::string error;
     bool fail_next = false;
     // ray queues and per-path state of nsp_paint; every paint on this context
@@ -110,6 +111,9 @@ struct nsp_scene {
     DevBuf pix_aov;  // aov_channels(n) * pixels, allocated on the first pass with aov
     RasterPrep raster;  // rasterised primary visibility (boxes and work units), built on first use
     DevBuf pass_dev;  // int: the pass index the kernels read
+    std::vector<ShapeRec> shapes;  // the objects after the mesh (nsp_scene_set_shapes)
+    DevBuf shapes_dev;
+    double shape_abs_max = 0;  // largest |coordinate| of the shapes' vertices
     DevBuf totals;  // 3 u64
     StageMarks marks;
     bool counters = false;
@@ -216,10 +220,13 @@ static void fill_scene(const nsp_scene& S, const nsp_paint_params& p, int pass, 
     k.bg_is_light = S.background.light_source;
     k.obj_is_light = S.object.light_source;
     k.diffuse = S.object.diffuse;
-    // compute_ray_offset, painter.cpp:539-558: max |coordinate| * 1000 * epsilon(T)
+    // compute_ray_offset, painter.cpp:539-558: max |coordinate| of every object * 1000 * epsilon(T)
     double all_max = -INFINITY;
     for (int i = 0; i < n; ++i)
         all_max = std::max(all_max, std::max(std::fabs(S.mesh->m.mn[i]), std::fabs(S.mesh->m.mx[i])));
+    if (!S.shapes.empty()) all_max = std::max(all_max, S.shape_abs_max);
+    k.shapes = S.shapes.empty() ? nullptr : S.shapes_dev.as<ShapeRec>();
+    k.shape_count = (int)S.shapes.size();
     const double eps = sizeof(T) == 4 ? 1.1920928955078125e-07 : 2.220446049250313e-16;
     k.ray_offset = (T)(all_max * (1000 * eps));
     k.cos_coef = (T)cosine_sphere_coefficient(n);
@@ -244,6 +251,7 @@ int nsp_context_create(int device, nsp_context** out)
         auto ctx = std::make_unique<nsp_context>();
         ctx->device = device;
         NSP_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        NSP_HIP(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
         *out = ctx.release();
     });
 }
@@ -255,6 +263,8 @@ void nsp_context_destroy(nsp_context* ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (hipEvent_t e : ctx->feed_ev)
         if (e) (void)hipEventDestroy(e);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -586,6 +596,129 @@ int nsp_scene_create(nsp_context* ctx, int n, nsp_mesh* mesh, const nsp_surface*
     });
 }
 
+}  // extern "C"
+
+// Parallelotope::create_planes (parallelotope.h:129-153) and
+// HyperplaneParallelotope::set_data with HyperplaneSimplexGeometry::set_data
+// (hyperplane_parallelotope.h:68-74, hyperplane_geometry.h:80-110), FP64.
+template <int N>
+static ShapeRec make_shape(const nsp_shape& in)
+{
+    ShapeRec r{};
+    r.kind = in.kind;
+    r.light = in.surface.light_source != 0;
+    for (int c = 0; c < 3; ++c) {
+        r.rgb[c] = in.surface.rgb[c];
+        r.light_rgb[c] = in.surface.light_rgb[c];
+    }
+    r.diffuse = in.surface.diffuse;
+    for (int k = 0; k < N; ++k) r.org[k] = in.org[k];
+    if (in.kind == NSP_SHAPE_PARALLELOTOPE) {
+        for (int i = 0; i < N; ++i) {
+            double m[(N - 1) * N];
+            for (int a = 0, row = 0; a < N; ++a)
+                if (a != i) {
+                    for (int k = 0; k < N; ++k) m[row * N + k] = in.edges[a][k];
+                    ++row;
+                }
+            double nn[N];
+            ortho_nn<N, double>(m, nn);
+            normalize<N, double>(nn);
+            double ov[N];
+            for (int k = 0; k < N; ++k) ov[k] = r.org[k] + in.edges[i][k];
+            double d1 = dot<N, double>(r.org, nn);
+            double d2 = -dot<N, double>(ov, nn);
+            if (dot<N, double>(nn, in.edges[i]) > 0) {
+                for (int k = 0; k < N; ++k) nn[k] = -nn[k];
+                d1 = -d1;
+                d2 = -d2;
+            }
+            for (int k = 0; k < N; ++k) r.pn[i][k] = nn[k];
+            r.d1[i] = d1;
+            r.d2[i] = d2;
+        }
+    } else {
+        double vec[(N - 1) * N];
+        for (int a = 0; a < N - 1; ++a)
+            for (int k = 0; k < N; ++k) vec[a * N + k] = in.edges[a][k];
+        ortho_nn<N, double>(vec, r.n);
+        normalize<N, double>(r.n);
+        // the simplex geometry of the points org + e_i and org
+        double pts[N][N];
+        for (int a = 0; a < N - 1; ++a)
+            for (int k = 0; k < N; ++k) pts[a][k] = r.org[k] + in.edges[a][k];
+        for (int k = 0; k < N; ++k) pts[N - 1][k] = r.org[k];
+        for (int a = 0; a < N - 1; ++a)
+            for (int k = 0; k < N; ++k) vec[a * N + k] = pts[a][k] - pts[N - 1][k];
+        for (int i = 0; i < N - 1; ++i) {
+            double m[(N - 1) * N];
+            for (int a = 0; a < N - 1; ++a)
+                for (int k = 0; k < N; ++k) m[a * N + k] = a == i ? r.n[k] : vec[a * N + k];
+            double pn[N];
+            ortho_nn<N, double>(m, pn);
+            double d = dot<N, double>(pts[N - 1], pn);
+            const double dist = dot<N, double>(pts[i], pn) - d;
+            for (int k = 0; k < N; ++k) r.pn[i][k] = pn[k] / dist;
+            r.d1[i] = d / dist;
+        }
+    }
+    return r;
+}
+
+extern "C" {
+
+int nsp_scene_set_shapes(nsp_scene* scene, const nsp_shape* shapes, int count)
+{
+    if (!scene) return NSP_E_INVALID;
+    return guarded(scene->ctx, [&] {
+        if (count < 0 || count > NSP_MAX_SHAPES || (count > 0 && !shapes))
+            raise(NSP_E_INVALID, "shape count must be in [0, " + std::to_string(NSP_MAX_SHAPES) + "]");
+        const int n = scene->n;
+        std::vector<ShapeRec> recs;
+        double abs_max = 0;
+        for (int j = 0; j < count; ++j) {
+            const nsp_shape& sh = shapes[j];
+            if (sh.kind != NSP_SHAPE_PARALLELOTOPE && sh.kind != NSP_SHAPE_HYPERPLANE_PARALLELOTOPE)
+                raise(NSP_E_INVALID, "unknown shape kind " + std::to_string(sh.kind));
+            ShapeRec r{};
+            switch (n) {
+            case 3: r = make_shape<3>(sh); break;
+            case 4: r = make_shape<4>(sh); break;
+            case 5: r = make_shape<5>(sh); break;
+            case 6: r = make_shape<6>(sh); break;
+            default: raise(NSP_E_INVALID, "dimension not supported");
+            }
+            bool finite = true;
+            for (int a = 0; a < n; ++a) {
+                finite = finite && std::isfinite(r.n[a]) && std::isfinite(r.d1[a]) && std::isfinite(r.d2[a]);
+                for (int k = 0; k < n; ++k) finite = finite && std::isfinite(r.pn[a][k]);
+            }
+            if (!finite) raise(NSP_E_INVALID, "shape " + std::to_string(j) + " has dependent edge vectors");
+            // the vertices: org plus every subset of the edges (parallelotope_vertices,
+            // hyperplane_parallelotope_vertices), for compute_ray_offset
+            const int edges = sh.kind == NSP_SHAPE_PARALLELOTOPE ? n : n - 1;
+            for (int mask = 0; mask < (1 << edges); ++mask)
+                for (int k = 0; k < n; ++k) {
+                    double x = sh.org[k];
+                    for (int a = 0; a < edges; ++a)
+                        if (mask & (1 << a)) x += sh.edges[a][k];
+                    abs_max = std::max(abs_max, std::fabs(x));
+                }
+            recs.push_back(r);
+        }
+        hipStream_t s = scene->ctx->stream;
+        NSP_HIP(hipStreamSynchronize(s));
+        if (!recs.empty()) {
+            scene->shapes_dev.alloc(sizeof(ShapeRec) * recs.size());
+            NSP_HIP(hipMemcpyAsync(scene->shapes_dev.p, recs.data(), sizeof(ShapeRec) * recs.size(),
+                                   hipMemcpyHostToDevice, s));
+            NSP_HIP(hipStreamSynchronize(s));
+        }
+        scene->shapes = std::move(recs);
+        scene->shape_abs_max = abs_max;
+    });
+}
+
 void nsp_scene_destroy(nsp_scene* scene)
 {
     if (!scene) return;
@@ -681,6 +814,8 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
         if (p.precision != NSP_PRECISION_FP32 && p.precision != NSP_PRECISION_FP64)
             raise(NSP_E_INVALID, "unknown precision");
         if (p.primary != NSP_PRIMARY_AUTO && p.primary != NSP_PRIMARY_WALK) raise(NSP_E_INVALID, "unknown primary mode");
+        if (!scene->shapes.empty() && p.smooth_normal && scene->mesh->m.has_normals)
+            raise(NSP_E_UNSUPPORTED, "smooth-normal renders of meshes with vertex normals do not support shapes");
         launch_gate(ctx);
         const int n = scene->n;
         hipStream_t s = ctx->stream;
@@ -731,6 +866,7 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
         rl.counters = scene->counters;
         rl.marks = p.use_graph ? nullptr : &scene->marks;
         rl.raster = raster ? &scene->raster : nullptr;
+        rl.side = ctx->side;
         GraphHolder gh;
         std::unique_ptr<PixelFeed> feed;
         if (cb && cb->pixels_after) feed = std::make_unique<PixelFeed>(ctx, scene, cb, batch_pixels);
@@ -985,6 +1121,8 @@ int nsp_batch_paint(nsp_batch* b, const nsp_paint_params* params, const volatile
         for (int i = 0; i < ns; ++i) {
             nsp_scene* sc = b->scenes[i];
             BatchScene& bs = b->per[i];
+            if (!sc->shapes.empty() && p.smooth_normal && sc->mesh->m.has_normals)
+                raise(NSP_E_UNSUPPORTED, "smooth-normal renders of meshes with vertex normals do not support shapes");
             fill_scene<float>(*sc, p, (int)sc->pass_count, bs.kf);
             if (!sc->pass_dev.p) sc->pass_dev.alloc(sizeof(int));
             bs.kf.pass_dev = sc->pass_dev.as<int>();

@@ -25,7 +25,7 @@ struct Vec {
 };
 
 template <int N, typename T>
-__device__ __forceinline__ T dot(const T* a, const T* b)
+__host__ __device__ __forceinline__ T dot(const T* a, const T* b)
 {
     T r = a[0] * b[0];
 #pragma unroll
@@ -34,7 +34,7 @@ __device__ __forceinline__ T dot(const T* a, const T* b)
 }
 
 template <int N, typename T>
-__device__ __forceinline__ void normalize(T* v)
+__host__ __device__ __forceinline__ void normalize(T* v)
 {
     T len = sqrt(dot<N, T>(v, v));
 #pragma unroll
@@ -90,7 +90,7 @@ __device__ __forceinline__ void sincospi_t(double x, double* s, double* c) { sin
 // m (row stride N), columns given by cols.
 template <int N, int K, typename T>
 struct Det {
-    __device__ static T eval(const T* m, int row, const int* cols)
+    __host__ __device__ static T eval(const T* m, int row, const int* cols)
     {
         T s = 0;
 #pragma unroll
@@ -113,19 +113,19 @@ struct Det {
 };
 template <int N, typename T>
 struct Det<N, 2, T> {
-    __device__ static T eval(const T* m, int row, const int* cols)
+    __host__ __device__ static T eval(const T* m, int row, const int* cols)
     {
         return m[row * N + cols[0]] * m[(row + 1) * N + cols[1]] - m[row * N + cols[1]] * m[(row + 1) * N + cols[0]];
     }
 };
 template <int N, typename T>
 struct Det<N, 1, T> {
-    __device__ static T eval(const T* m, int row, const int* cols) { return m[row * N + cols[0]]; }
+    __host__ __device__ static T eval(const T* m, int row, const int* cols) { return m[row * N + cols[0]]; }
 };
 
 // ortho_nn of N-1 vectors (rows of m), src/geometry/core/linear_algebra.h:167-186.
 template <int N, typename T>
-__device__ __forceinline__ void ortho_nn(const T* m, T* out)
+__host__ __device__ __forceinline__ void ortho_nn(const T* m, T* out)
 {
 #pragma unroll
     for (int i = 0; i < N; ++i) {

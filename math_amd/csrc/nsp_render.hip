@@ -153,7 +153,7 @@ __global__ void k_batch_totals(const unsigned long long* __restrict__ cnt, int d
     unsigned long long c = 0, s = 0, p = 0;
     for (int d = 0; d < depth_slots; ++d) {
         c += cnt[d * CNT_STRIDE + 0];
-        s += cnt[d * CNT_STRIDE + 1];
+        s += cnt[d * CNT_STRIDE + 1] + cnt[d * CNT_STRIDE + 7];
         p += cnt[d * CNT_STRIDE + 2];
     }
     totals[0] += c;
@@ -191,7 +191,8 @@ void pass_reset(RenderBuffers& b, bool counters, int* pass_dev, int pass, hipStr
 
 template <int N, typename T>
 __global__ void __launch_bounds__(TRACE_BLOCK) k_raygen(SceneK<T> sc, const BatchDesc* __restrict__ desc,
-                                                        RayQueue<T> q, int64_t cap, float* __restrict__ L)
+                                                        RayQueue<T> q, int64_t cap, float* __restrict__ L,
+                                                        float* __restrict__ Ls)
 {
     const int64_t paths = desc->paths;
     const int64_t pixel0 = desc->pixel0;
@@ -212,6 +213,9 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_raygen(SceneK<T> sc, const Batc
         L[p] = 0.0f;
         L[cap + p] = 0.0f;
         L[2 * cap + p] = 0.0f;
+        Ls[p] = 0.0f;
+        Ls[cap + p] = 0.0f;
+        Ls[2 * cap + p] = 0.0f;
     }
 }
 
@@ -292,6 +296,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK)
             sq.contrib[ss] = out.contrib[0];
             sq.contrib[scap + ss] = out.contrib[1];
             sq.contrib[2 * scap + ss] = out.contrib[2];
+            if (sq.shape_count > 0) sq.shape[ss] = out.sshape;
         }
     }
 }
@@ -304,6 +309,13 @@ template <int N, typename T>
 __device__ __forceinline__ void shadow_unoccluded(const ShadowQueue<T>& sq, int64_t scap, int64_t i,
                                                   float* __restrict__ L, int64_t cap, float* __restrict__ direct)
 {
+    if (sq.shape_count > 0) {
+        // the mesh (the first object) let the ray pass: the shapes were tested
+        // next, up to the first that blocks the light (shade recorded it)
+        const int32_t j = sq.shape[i];
+        atomicAdd(sq.extra, (unsigned long long)(j >= 0 ? j + 1 : sq.shape_count));
+        if (j >= 0) return;
+    }
     const int32_t raw = sq.path[i];
     const int32_t path = raw < 0 ? -raw - 1 : raw;
     const float c0 = sq.contrib[i], c1 = sq.contrib[scap + i], c2 = sq.contrib[2 * scap + i];
@@ -603,7 +615,8 @@ __global__ void __launch_bounds__(256) k_shadow_resolve(TreeArgs A, ShadowQueue<
 
 // Sum of the samples of each pixel in sample order; Pixels::add_color_and_samples.
 __global__ void __launch_bounds__(256) k_accumulate(const BatchDesc* __restrict__ desc, int S,
-                                                   const float* __restrict__ L, int64_t cap,
+                                                   const float* __restrict__ L, const float* __restrict__ Ls,
+                                                   int64_t cap,
                                                    float* __restrict__ pix_sum, int32_t* __restrict__ pix_cnt,
                                                    int64_t pixels, const float* __restrict__ aov,
                                                    float* __restrict__ pix_aov, int aovc)
@@ -614,9 +627,9 @@ __global__ void __launch_bounds__(256) k_accumulate(const BatchDesc* __restrict_
         float s0 = 0, s1 = 0, s2 = 0;
         for (int s = 0; s < S; ++s) {
             int64_t p = j * S + s;
-            s0 += L[p];
-            s1 += L[cap + p];
-            s2 += L[2 * cap + p];
+            s0 += L[p] + Ls[p];
+            s1 += L[cap + p] + Ls[cap + p];
+            s2 += L[2 * cap + p] + Ls[2 * cap + p];
         }
         int64_t px = pixel0 + j;
         pix_sum[px] += s0;
@@ -654,12 +667,18 @@ void alloc_render_buffers(RenderBuffers& b, int n, int precision, int64_t cap, i
         q[6].alloc(ts * cap);
     }
     int64_t scap = cap * MAX_LIGHTS;
-    b.sq[0].alloc(ts * n * scap);
-    b.sq[1].alloc(ts * n * scap);
-    b.sq[2].alloc(ts * scap);
-    b.sq[3].alloc(sizeof(int32_t) * scap);
-    b.sq[4].alloc(sizeof(float) * 3 * scap);
+    for (DevBuf* q : {b.sq[0], b.sq[1]}) {
+        q[0].alloc(ts * n * scap);
+        q[1].alloc(ts * n * scap);
+        q[2].alloc(ts * scap);
+        q[3].alloc(sizeof(int32_t) * scap);
+        q[4].alloc(sizeof(float) * 3 * scap);
+        q[5].alloc(sizeof(int32_t) * scap);
+    }
     b.L.alloc(sizeof(float) * 3 * cap);
+    b.Ls.alloc(sizeof(float) * 3 * cap);
+    for (hipEvent_t* e : {&b.fork_ev, &b.join_ev, &b.done_ev[0], &b.done_ev[1]})
+        if (!*e) NSP_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
     b.redo.alloc(sizeof(int32_t) * scap);
     b.redo_c.alloc(sizeof(int32_t) * cap);
     b.res_c.alloc(sizeof(int32_t) * RES_STRIDE * cap);
@@ -669,6 +688,12 @@ void alloc_render_buffers(RenderBuffers& b, int n, int precision, int64_t cap, i
     b.totals.alloc(sizeof(unsigned long long) * 3);
     b.batch_idx.alloc(sizeof(long long));
     b.trav_counters.alloc(sizeof(unsigned long long) * 8);
+}
+
+RenderBuffers::~RenderBuffers()
+{
+    for (hipEvent_t e : {fork_ev, join_ev, done_ev[0], done_ev[1]})
+        if (e) (void)hipEventDestroy(e);
 }
 
 template <typename T>
@@ -700,13 +725,18 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
                     rl.mesh->has_normals ? rl.mesh->vn64.as<double>() : nullptr,
                     rl.mesh->has_normals ? rl.mesh->fnrm.as<int32_t>() : nullptr, rl.mesh->ntype.as<int32_t>()};
     RayQueue<T> qa = queue_of<T>(b.qa), qb = queue_of<T>(b.qb);
-    ShadowQueue<T> sq{b.sq[0].as<T>(), b.sq[1].as<T>(), b.sq[2].as<T>(), b.sq[3].as<int32_t>(), b.sq[4].as<float>()};
+    ShadowQueue<T> sqs[2];
+    for (int k = 0; k < 2; ++k)
+        sqs[k] = ShadowQueue<T>{b.sq[k][0].as<T>(), b.sq[k][1].as<T>(), b.sq[k][2].as<T>(), b.sq[k][3].as<int32_t>(),
+                                b.sq[k][4].as<float>(), b.sq[k][5].as<int32_t>(), sc.shape_count, nullptr};
     unsigned long long* cnt = b.counters.as<unsigned long long>();
     const BatchDesc* desc = b.desc.as<BatchDesc>();
     const BatchDesc* list = rl.batch_list ? rl.batch_list : b.batch_list.as<BatchDesc>();
     long long* bidx = rl.batch_idx ? rl.batch_idx : b.batch_idx.as<long long>();
     unsigned long long* totals = rl.totals ? rl.totals : b.totals.as<unsigned long long>();
     float* L = b.L.as<float>();
+    float* Ls = b.Ls.as<float>();
+    hipStream_t ss = rl.side ? rl.side : s;  // the shadow stages' stream
     float* aov = nullptr;
     if (rl.pix_aov) {
         // allocated by the caller before any graph capture
@@ -728,26 +758,31 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
     int32_t* res_s = b.res.as<int32_t>();
     int32_t* redo_c = b.redo_c.as<int32_t>();
     int32_t* res_c = b.res_c.as<int32_t>();
-    auto timed = [&](int stage, auto&& launch) {
+    auto timed_on = [&](hipStream_t st, int stage, auto&& launch) {
         if (!rl.marks) {
             launch();
             NSP_HIP(hipGetLastError());
             return;
         }
         hipEvent_t e0 = rl.marks->get(), e1 = rl.marks->get();
-        NSP_HIP(hipEventRecord(e0, s));
+        NSP_HIP(hipEventRecord(e0, st));
         launch();
         NSP_HIP(hipGetLastError());
-        NSP_HIP(hipEventRecord(e1, s));
+        NSP_HIP(hipEventRecord(e1, st));
         rl.marks->marks.emplace_back(stage, e0, e1);
     };
+    auto timed = [&](int stage, auto&& launch) { timed_on(s, stage, launch); };
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(256), 0, s, cnt, CNT_STRIDE * (rl.max_depth + 2),
                        list, bidx, b.desc.as<BatchDesc>());
     NSP_HIP(hipGetLastError());
-    timed(0, [&] { hipLaunchKernelGGL((k_raygen<N, T>), dim3(g), dim3(TRACE_BLOCK), 0, s, sc, desc, qa, cap, L); });
+    timed(0, [&] {
+        hipLaunchKernelGGL((k_raygen<N, T>), dim3(g), dim3(TRACE_BLOCK), 0, s, sc, desc, qa, cap, L, Ls);
+    });
     for (int depth = 0; depth <= rl.max_depth; ++depth) {
         RayQueue<T>& qin = (depth & 1) ? qb : qa;
         RayQueue<T>& qout = (depth & 1) ? qa : qb;
+        ShadowQueue<T> sq = sqs[depth & 1];
+        sq.extra = cnt + depth * CNT_STRIDE + 7;
         unsigned long long* dc = cnt + depth * CNT_STRIDE;
         const bool rasterise = sizeof(T) == 4 && depth == 0 && rl.raster;
         timed(rasterise ? 5 : 1, [&] {
@@ -772,27 +807,38 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
             }
             hipLaunchKernelGGL(kc, dim3(g), dim3(TRACE_BLOCK), 0, s, A, qin, cap, dc);
         });
+        // shade(depth) refills the shadow queue that the shadow stage of depth - 2 read
+        if (rl.side && depth >= 2) NSP_HIP(hipStreamWaitEvent(s, b.done_ev[depth & 1], 0));
         timed(2, [&] {
             hipLaunchKernelGGL(kshade, dim3(g), dim3(TRACE_BLOCK), 0, s, sc, A, M, desc, depth, qin, qout, sq, cap,
                                cnt, L, aov);
         });
-        timed(3, [&] {
+        if (rl.side) {
+            NSP_HIP(hipEventRecord(b.fork_ev, s));
+            NSP_HIP(hipStreamWaitEvent(ss, b.fork_ev, 0));
+        }
+        timed_on(ss, 3, [&] {
             float* direct = (aov && depth == 0) ? aov + (int64_t)(3 + N) * cap : nullptr;
-            hipLaunchKernelGGL(ks, dim3(gs), dim3(TRACE_BLOCK), 0, s, As, sq, scap, dc + 1, L, cap, direct,
+            hipLaunchKernelGGL(ks, dim3(gs), dim3(TRACE_BLOCK), 0, ss, As, sq, scap, dc + 1, Ls, cap, direct,
                                dc + 6, redo_s, dc + 4, res_s);
             NSP_HIP(hipGetLastError());
             if constexpr (sizeof(T) == 4) {
-                hipLaunchKernelGGL((k_shadow_resolve<N>), dim3(gx), dim3(256), 0, s, As, sq, scap, L, cap, direct,
+                hipLaunchKernelGGL((k_shadow_resolve<N>), dim3(gx), dim3(256), 0, ss, As, sq, scap, Ls, cap, direct,
                                    dc + 4, res_s);
                 NSP_HIP(hipGetLastError());
             }
-            hipLaunchKernelGGL((k_shadow_exact<N, T>), dim3(gx), dim3(TRACE_BLOCK), 0, s, As, sq, scap, L, cap,
+            hipLaunchKernelGGL((k_shadow_exact<N, T>), dim3(gx), dim3(TRACE_BLOCK), 0, ss, As, sq, scap, Ls, cap,
                                direct, dc + 6, redo_s);
         });
+        if (rl.side) NSP_HIP(hipEventRecord(b.done_ev[depth & 1], ss));
+    }
+    if (rl.side) {
+        NSP_HIP(hipEventRecord(b.join_ev, ss));
+        NSP_HIP(hipStreamWaitEvent(s, b.join_ev, 0));
     }
     timed(4, [&] {
         hipLaunchKernelGGL(k_accumulate, dim3(grid_for(cap / std::max(1, sc.S), 256, 4096)), dim3(256), 0, s, desc,
-                           sc.S, L, cap, rl.pix_sum, rl.pix_cnt, rl.pixels, aov, rl.pix_aov, aov_channels(N));
+                           sc.S, L, Ls, cap, rl.pix_sum, rl.pix_cnt, rl.pixels, aov, rl.pix_aov, aov_channels(N));
     });
     hipLaunchKernelGGL(k_batch_totals, dim3(1), dim3(64), 0, s, cnt, rl.max_depth + 1, totals, bidx);
     NSP_HIP(hipGetLastError());

@@ -16,6 +16,18 @@ namespace nsp {
 // so the shadow stage adds to a path's radiance without atomics.
 constexpr int MAX_LIGHTS = 1;
 
+// A flat-shaded shape beside the mesh, FP64 (VisibleParallelotope /
+// VisibleHyperplaneParallelotope, src/painter/visible_shapes.h:31-117).
+struct ShapeRec {
+    int kind;                // NSP_SHAPE_*
+    int light;               // a light source (emits light_rgb to diffuse bounces)
+    float rgb[3], light_rgb[3], diffuse;
+    double org[NMAX];
+    double n[NMAX];          // hyperplane parallelotope: unit normal
+    double pn[NMAX][NMAX];   // parallelotope: outward unit plane normals; hyperplane: barycentric planes
+    double d1[NMAX], d2[NMAX];  // parallelotope: offsets of the two planes; hyperplane: d1 of the barycentric planes
+};
+
 // Scene constants passed by value to the render kernels (the data of
 // PaintObjects: src/painter/objects.h:212-223, with the derived constants of
 // paint_threads, src/painter/painter.cpp:560-575).
@@ -49,6 +61,8 @@ struct SceneK {
     int smooth;
     int has_normals;
     uint64_t seed;
+    const ShapeRec* shapes;  // the objects after the mesh, in order (device memory)
+    int shape_count;
     int pass;             // pass index, used when pass_dev is null
     const int* pass_dev;  // pass index in device memory (written by the pass reset), so that
                           // one captured launch sequence serves every pass
@@ -90,6 +104,10 @@ struct ShadowQueue {
     T* tmax;
     int32_t* path;
     float* contrib;  // 3 * cap
+    // with shapes: the first shape (in object order) that blocks the light, or -1
+    int32_t* shape;
+    int shape_count;
+    unsigned long long* extra;  // ray counts of the shape tests (the mesh test is the shadow ray itself)
 };
 
 struct MeshShadeArgs {
@@ -102,8 +120,9 @@ struct MeshShadeArgs {
 
 // Per-depth counters, CNT_STRIDE u64 per depth: 0 queued rays, 1 shadow rays, 2 probe rays,
 // 3 closest-hit FP64 decisions, 4 shadow FP64 decisions, 5 closest-hit re-walks, 6 shadow re-walks,
-// 8-15 closest-hit work index per queue slice, 16-23 shadow work index per queue slice (7 unused).  The stride keeps the counters that one stage's waves append to with atomics
-// (next depth's queue, shadow queue) on different cache lines.
+// 7 ray counts of shadow rays' shape tests, 8-15 closest-hit work index per queue slice,
+// 16-23 shadow work index per queue slice.  The stride keeps the counters that one stage's waves
+// append to with atomics (next depth's queue, shadow queue) on different cache lines.
 constexpr int CNT_STRIDE = 24;
 
 struct RenderBuffers {
@@ -111,8 +130,16 @@ struct RenderBuffers {
     int precision = 0;
     int64_t cap = 0;        // paths per batch
     int depth_slots = 0;
-    DevBuf qa[8], qb[8], sq[6];
-    DevBuf L;               // 3 * cap floats
+    DevBuf qa[8], qb[8];
+    DevBuf sq[2][6];        // shadow-ray queues of even and odd depths (ray, tmax, path, contribution, blocking shape)
+    DevBuf L;               // 3 * cap floats: radiance the paths collect at their hits (background, emitters)
+    DevBuf Ls;              // 3 * cap floats: radiance of the paths' unoccluded shadow rays
+    // ordering of the shadow stages on a second stream (see render_batch)
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr, done_ev[2] = {nullptr, nullptr};
+    RenderBuffers() = default;
+    RenderBuffers(const RenderBuffers&) = delete;
+    RenderBuffers& operator=(const RenderBuffers&) = delete;
+    ~RenderBuffers();
     DevBuf redo;            // int32 shadow-ray indices to walk again with the FP64 re-test inline
     DevBuf res;             // int32 (shadow ray, PENDING candidates) for the FP64 decisions
     DevBuf keys;            // u64 per path: (t, simplex) of the rasterised primary hits
@@ -166,7 +193,16 @@ struct RenderLaunch {
     const BatchDesc* batch_list = nullptr;
     long long* batch_idx = nullptr;
     unsigned long long* totals = nullptr;
+    // a second stream for the shadow stages (null: everything on the launch stream)
+    hipStream_t side = nullptr;
 };
+// One wavefront batch: raygen, then per depth closest hits, shade and the
+// shadow stage, then accumulate.  With rl.side, the shadow stage of depth d
+// runs on the side stream, overlapping the closest hits and shading of depth
+// d+1: it reads the shadow queue of depth d's parity and writes only Ls, and
+// shade(d+2), which refills that queue, waits for it.  The side stream joins
+// the launch stream before accumulate, so a caller sees one stream's work
+// (and a graph captured on it holds both).
 void render_batch(const RenderLaunch& rl, hipStream_t s);
 // Clears the pass totals, the batch index and (when counting) the traversal counters,
 // and writes the pass index to pass_dev (when not null).
@@ -174,5 +210,8 @@ void pass_reset(RenderBuffers& b, bool counters, int* pass_dev, int pass, hipStr
 // Clears a scene's own pass totals (3 u64) and batch index.
 void pass_reset_state(unsigned long long* totals, long long* batch_idx, int* pass_dev, int pass, hipStream_t s);
 void alloc_render_buffers(RenderBuffers& b, int n, int precision, int64_t cap, int max_depth);
+// FP64 records of shapes (org and edge vectors as given) into out (device), on stream s.
+void build_shape_records(int n, const int* kind, const double* org, const double* edges, int count, ShapeRec* out,
+                         hipStream_t s);
 
 }  // namespace nsp

@@ -1,6 +1,8 @@
 // nsp_stages.hpp -- per-ray bodies of the wavefront stages (nsp_render.hip).
 #pragma once
 
+#include <cfloat>
+
 #include "nsp_device.hpp"
 #include "nsp_render.hpp"
 #include "nsp_trace.hpp"
@@ -58,6 +60,129 @@ __device__ __forceinline__ void primary_ray(const SceneK<T>& sc, int64_t pixel, 
         dir[j] = sc.cam_dir[j] * z + sd[j];
     }
     normalize<N, T>(dir);
+}
+
+// Parallelotope::intersect (parallelotope.h:196-246) and
+// HyperplaneParallelotopeGeometry::intersect (hyperplane_geometry.h:41-52,
+// 240-261), in FP64: the nearest t > 0 at which the ray meets the shape.
+template <int N>
+__device__ __forceinline__ bool shape_intersect(const ShapeRec& s, const double* o, const double* d, double* t)
+{
+    if (s.kind == NSP_SHAPE_PARALLELOTOPE) {
+        double f_max = -DBL_MAX, b_min = DBL_MAX;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const double sd = dot<N, double>(d, s.pn[i]);
+            const double od = dot<N, double>(o, s.pn[i]);
+            if (sd == 0) {
+                if (od - s.d1[i] > 0 || -od - s.d2[i] > 0) return false;  // parallel and outside
+                continue;
+            }
+            const double a1 = (s.d1[i] - od) / sd;
+            const double a2 = (s.d2[i] + od) / -sd;
+            if (sd < 0) {
+                f_max = fmax(a1, f_max);
+                b_min = fmin(a2, b_min);
+            } else {
+                b_min = fmin(a1, b_min);
+                f_max = fmax(a2, f_max);
+            }
+            if (b_min <= 0 || b_min < f_max) return false;
+        }
+        *t = f_max > 0 ? f_max : b_min;
+        return true;
+    }
+    const double sd = dot<N, double>(s.n, d);
+    if (sd == 0) return false;
+    double v[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = s.org[k] - o[k];
+    const double tt = dot<N, double>(v, s.n) / sd;
+    if (!(tt > 0)) return false;
+    double p[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) p[k] = o[k] + d[k] * tt;
+#pragma unroll
+    for (int i = 0; i < N - 1; ++i) {
+        const double c = dot<N, double>(p, s.pn[i]) - s.d1[i];
+        if (c <= 0 || c >= 1) return false;
+    }
+    *t = tt;
+    return true;
+}
+
+// Parallelotope::normal (parallelotope.h:261-290): the normal of the nearest
+// face; HyperplaneParallelotope::normal: its plane's normal.
+template <int N>
+__device__ __forceinline__ void shape_normal(const ShapeRec& s, const double* p, double* n)
+{
+    if (s.kind != NSP_SHAPE_PARALLELOTOPE) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) n[k] = s.n[k];
+        return;
+    }
+    double mn = DBL_MAX;
+    int best = 0;
+    bool neg = false;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const double dd = dot<N, double>(p, s.pn[i]);
+        double l = fabs(dd - s.d1[i]);
+        if (l < mn) {
+            mn = l;
+            best = i;
+            neg = false;
+        }
+        l = fabs(-dd - s.d2[i]);
+        if (l < mn) {
+            mn = l;
+            best = i;
+            neg = true;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < N; ++k) n[k] = neg ? -s.pn[best][k] : s.pn[best][k];
+}
+
+// The nearest shape hit before t_max (object order breaks exact ties), or -1.
+template <int N, typename T>
+__device__ __forceinline__ int nearest_shape(const SceneK<T>& sc, const T* o, const T* d, double t_max, double* t)
+{
+    double o64[N], d64[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        o64[k] = (double)o[k];
+        d64[k] = (double)d[k];
+    }
+    int hit = -1;
+    double best = t_max;
+    for (int k = 0; k < sc.shape_count; ++k) {
+        double ts;
+        if (shape_intersect<N>(sc.shapes[k], o64, d64, &ts) && ts < best) {
+            best = ts;
+            hit = k;
+        }
+    }
+    *t = best;
+    return hit;
+}
+
+// light_source_is_visible over the shapes (painter.cpp:143-186): the first
+// shape in object order that meets the ray before the light, or -1.
+template <int N, typename T>
+__device__ __forceinline__ int blocking_shape(const SceneK<T>& sc, const T* o, const T* d, double dist)
+{
+    double o64[N], d64[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        o64[k] = (double)o[k];
+        d64[k] = (double)d[k];
+    }
+    for (int k = 0; k < sc.shape_count; ++k) {
+        double ts;
+        if (shape_intersect<N>(sc.shapes[k], o64, d64, &ts) && ts < dist) return k;
+    }
+    return -1;
 }
 
 template <int N, typename T>
@@ -122,6 +247,7 @@ struct ShadeOut {
     T sorg[N], sdir[N];
     T stmax;
     float contrib[3];  // radiance added when the shadow ray is unoccluded
+    int sshape;        // the first shape blocking the shadow ray, or -1
 };
 
 // One path vertex: trace_path's shading, direct_diffuse_lighting's shadow-ray
@@ -138,41 +264,72 @@ __device__ void shade_one(const SceneK<T>& sc, const TreeArgs& A, const MeshShad
     out.add[0] = out.add[1] = out.add[2] = 0.0f;
     out.next = false;
     out.shadow = false;
+    out.sshape = -1;
     const bool diffuse_reflection = depth > 0;
+    // ray_intersection over the objects (ray_intersection.h:20-80): the mesh
+    // hit, or a nearer shape
+    int shape = -1;
+    if (sc.shape_count > 0) {
+        double ts;
+        shape = nearest_shape<N, T>(sc, o, d, idx >= 0 ? (double)t : DBL_MAX, &ts);
+        if (shape >= 0) t = (T)ts;
+    }
+    const bool hit = idx >= 0 || shape >= 0;
+    const bool mesh = shape < 0;
+    T p[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) p[k] = o[k] + d[k] * t;
+    // the surface properties of the hit (SurfaceProperties, src/painter/objects.h:27-139)
+    T gn[N], sn[N];
+    const float* rgb = sc.obj_rgb;
+    const float* light_rgb = sc.obj_light_rgb;
+    float diffuse = sc.diffuse;
+    bool is_light = sc.obj_is_light;
+    if (shape >= 0) {
+        const ShapeRec& sr = sc.shapes[shape];
+        double p64[N], n64[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) p64[k] = (double)p[k];
+        shape_normal<N>(sr, p64, n64);
+#pragma unroll
+        for (int k = 0; k < N; ++k) gn[k] = (T)n64[k];
+        rgb = sr.rgb;
+        light_rgb = sr.light_rgb;
+        diffuse = sr.diffuse;
+        is_light = sr.light;
+    } else if (idx >= 0) {
+        load_normal<N, T>(M, idx, gn);
+    }
     if (aov) {
         T an[N];
-        if (idx >= 0) {
-            load_normal<N, T>(M, idx, an);
+        if (hit) {
+#pragma unroll
+            for (int k = 0; k < N; ++k) an[k] = gn[k];
             if (dot<N, T>(d, an) > (T)0)
 #pragma unroll
                 for (int k = 0; k < N; ++k) an[k] = -an[k];
         }
 #pragma unroll
-        for (int c = 0; c < 3; ++c) aov[c] = idx >= 0 ? sc.diffuse * sc.obj_rgb[c] : sc.bg_rgb[c];
+        for (int c = 0; c < 3; ++c) aov[c] = hit ? diffuse * rgb[c] : sc.bg_rgb[c];
 #pragma unroll
-        for (int k = 0; k < N; ++k) aov[3 + k] = idx >= 0 ? (float)an[k] : 0.0f;
+        for (int k = 0; k < N; ++k) aov[3 + k] = hit ? (float)an[k] : 0.0f;
     }
-    if (idx < 0) {
+    if (!hit) {
         // miss: background, a light source when reached by a diffuse bounce (painter.cpp:335-340)
         const float* bg = (sc.bg_is_light && diffuse_reflection) ? sc.bg_light_rgb : sc.bg_rgb;
 #pragma unroll
         for (int c = 0; c < 3; ++c) out.add[c] = beta[c] * bg[c];
         return;
     }
-    T p[N];
-#pragma unroll
-    for (int k = 0; k < N; ++k) p[k] = o[k] + d[k] * t;
-    T gn[N], sn[N];
-    load_normal<N, T>(M, idx, gn);
     T dgn = dot<N, T>(d, gn);
     if (fabs(dgn) <= (T)0) return;
-    if (sc.obj_is_light) {
-        const float* lc = diffuse_reflection ? sc.obj_light_rgb : sc.obj_rgb;
+    if (is_light) {
+        const float* lc = diffuse_reflection ? light_rgb : rgb;
 #pragma unroll
         for (int c = 0; c < 3; ++c) out.add[c] = beta[c] * lc[c];
         return;
     }
-    if (sc.smooth && sc.has_normals)
+    if (mesh && sc.smooth && sc.has_normals)
         shading_normal<N, T>(M, idx, p, gn, sn);
     else {
 #pragma unroll
@@ -185,8 +342,8 @@ __device__ void shade_one(const SceneK<T>& sc, const TreeArgs& A, const MeshShad
             sn[k] = -sn[k];
         }
     }
-    if (!(sc.diffuse > 0)) return;
-    float scol[3] = {sc.diffuse * sc.obj_rgb[0], sc.diffuse * sc.obj_rgb[1], sc.diffuse * sc.obj_rgb[2]};
+    if (!(diffuse > 0)) return;
+    float scol[3] = {diffuse * rgb[0], diffuse * rgb[1], diffuse * rgb[2]};
     float new_level = level * fmaxf(scol[0], fmaxf(scol[1], scol[2]));
     if (!(new_level >= 1e-4f)) return;
     // direct_diffuse_lighting, painter.cpp:197-292 (one light)
@@ -217,7 +374,7 @@ __device__ void shade_one(const SceneK<T>& sc, const TreeArgs& A, const MeshShad
 #pragma unroll
             for (int k = 0; k < N; ++k) o1[k] = p[k] + ld[k] * sc.ray_offset;
             T tmaxs = dist;
-            if (SMOOTH && sc.smooth && dot<N, T>(ld, gn) < (T)0) {
+            if (SMOOTH && mesh && sc.smooth && dot<N, T>(ld, gn) < (T)0) {
                 // smooth-normal back face: forward and reverse probes (painter.cpp:238-284)
                 atomicAdd(probe_count, 1ull);
                 double o64[N], d64[N];
@@ -249,6 +406,7 @@ __device__ void shade_one(const SceneK<T>& sc, const TreeArgs& A, const MeshShad
             if (lit) {
                 float w = (float)(sc.cos_coef * dln);
                 out.shadow = true;
+                if (sc.shape_count > 0) out.sshape = blocking_shape<N, T>(sc, o1, ld, (double)dist);
 #pragma unroll
                 for (int k = 0; k < N; ++k) {
                     out.sorg[k] = o1[k];
@@ -268,7 +426,8 @@ __device__ void shade_one(const SceneK<T>& sc, const TreeArgs& A, const MeshShad
         T nd[N];
         cosine_hemisphere<N, T>(sn, u, nd);
         normalize<N, T>(nd);
-        if (dot<N, T>(nd, gn) > (T)0) {
+        // only a mesh's bounce must leave its geometric side (painter.cpp:308-313)
+        if (!mesh || dot<N, T>(nd, gn) > (T)0) {
             out.next = true;
 #pragma unroll
             for (int k = 0; k < N; ++k) {

@@ -55,6 +55,30 @@ class Surface(C.Structure):
                 ("light_rgb", C.c_float * 3)]
 
 
+SHAPE_PARALLELOTOPE = 0
+SHAPE_HYPERPLANE_PARALLELOTOPE = 1
+
+
+class Shape(C.Structure):
+    _fields_ = [("kind", C.c_int), ("org", C.c_double * MAX_DIM), ("edges", (C.c_double * MAX_DIM) * MAX_DIM),
+                ("surface", Surface)]
+
+
+def make_shape(kind, org, edges, rgb, diffuse=1.0, light_rgb=None):
+    """nsp_shape: org and edge vectors; light_rgb makes it a light source."""
+    sh = Shape()
+    sh.kind = kind
+    sh.org[:len(org)] = list(org)
+    for a, e in enumerate(edges):
+        sh.edges[a][:len(e)] = list(e)
+    sh.surface.rgb[:] = list(rgb)
+    sh.surface.diffuse = diffuse
+    sh.surface.light_source = 0 if light_rgb is None else 1
+    if light_rgb is not None:
+        sh.surface.light_rgb[:] = list(light_rgb)
+    return sh
+
+
 class PaintParams(C.Structure):
     _fields_ = [("samples_per_pixel", C.c_int), ("max_depth", C.c_int), ("smooth_normal", C.c_int),
                 ("precision", C.c_int), ("passes", C.c_int), ("seed", C.c_uint64), ("batch_paths", C.c_int64),
@@ -100,7 +124,7 @@ EXPORTS = [
     "nsp_denoiser_train", "nsp_conv3d_bf16", "nsp_batch_create", "nsp_batch_destroy", "nsp_batch_paint",
     "nsp_batch_statistics", "nsp_batch_set_lanes", "nsp_batch_get_lanes", "nsp_batch_autotune", "nsp_autotune", "nsp_mesh_set_record_layout", "nsp_mesh_get_record_layout",
     "nsp_obj_read", "nsp_obj_destroy", "nsp_obj_last_error", "nsp_obj_info", "nsp_obj_copy", "nsp_mesh_create_obj",
-    "nsp_obj_write",
+    "nsp_obj_write", "nsp_scene_set_shapes",
 ]
 
 _lib = None
@@ -152,6 +176,7 @@ def lib():
                                         ip, i64p]
         L.nsp_meshgen_cube_sphere.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_uint64, fp,
                                               i64p, ip, i64p]
+        L.nsp_scene_set_shapes.argtypes = [P, C.POINTER(Shape), C.c_int]
         L.nsp_obj_read.argtypes = [C.c_char_p, C.c_int, C.POINTER(P)]
         L.nsp_obj_destroy.argtypes = [P]
         L.nsp_obj_destroy.restype = None
@@ -397,14 +422,16 @@ class Scene:
         bg = Surface()
         bg.rgb[:] = cfg["background_rgb"]
         bg.diffuse = 1.0
-        bg.light_source = 1
+        bg.light_source = int(cfg.get("background_is_light", True))
         lum = 0.2126 * cfg["background_rgb"][0] + 0.7152 * cfg["background_rgb"][1] + 0.0722 * cfg["background_rgb"][2]
         bg.light_rgb[:] = [lum, lum, lum]
         light = Light()
-        light.kind = LIGHT_POINT
-        light.position[:n] = cfg["light_pos"]
-        light.rgb[:] = cfg["light_rgb"]
-        light.unit_intensity_distance = cfg["light_unit_distance"]
+        nlights = 0 if cfg.get("light_pos") is None else 1
+        if nlights:
+            light.kind = LIGHT_POINT
+            light.position[:n] = cfg["light_pos"]
+            light.rgb[:] = cfg["light_rgb"]
+            light.unit_intensity_distance = cfg["light_unit_distance"]
         pr = Projector()
         pr.kind = cfg.get("projector", PROJECTOR_PARALLEL)
         pr.camera_org[:n] = cfg["camera_org"]
@@ -415,9 +442,13 @@ class Scene:
         pr.units_per_pixel = cfg.get("units_per_pixel", 0.0)
         pr.view_angle_degrees = cfg.get("view_angle", 0.0)
         h = C.c_void_p()
-        ctx.check(L.nsp_scene_create(ctx.h, n, mesh.h, C.byref(obj), C.byref(light), 1, C.byref(pr), C.byref(bg),
-                                     C.byref(h)))
+        ctx.check(L.nsp_scene_create(ctx.h, n, mesh.h, C.byref(obj), C.byref(light) if nlights else None, nlights,
+                                     C.byref(pr), C.byref(bg), C.byref(h)))
         self.h = h
+        if cfg.get("shapes"):
+            self.set_shapes([sh if isinstance(sh, Shape) else
+                             make_shape(sh["kind"], sh["org"], sh["edges"], sh["rgb"], sh.get("diffuse", 1.0),
+                                        sh.get("light_rgb")) for sh in cfg["shapes"]])
         self.pixels = L.nsp_scene_pixel_count(h)
         self.screen = list(cfg["screen"])
 
@@ -467,6 +498,11 @@ class Scene:
         self.ctx.check(self.ctx._L.nsp_image_aov(self.h, _ptr(a, C.c_float), _ptr(nrm, C.c_float),
                                                  _ptr(dl, C.c_float)))
         return a, nrm, dl
+
+    def set_shapes(self, shapes):
+        """nsp_scene_set_shapes: a list of Shape (make_shape), after the mesh in object order."""
+        arr = (Shape * max(1, len(shapes)))(*shapes)
+        self.ctx.check(self.ctx._L.nsp_scene_set_shapes(self.h, arr, len(shapes)))
 
     def set_image(self, s, c):
         s = np.ascontiguousarray(s, np.float32)

@@ -6,6 +6,7 @@
 #define _GNU_SOURCE
 #include "nsp_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <pthread.h>
 #include <stdarg.h>
@@ -968,9 +969,151 @@ void orc_cosine_hemisphere(int n, const double* normal, const double* u, double*
     }
 }
 
+/* A shape's planes (FP64): Parallelotope::create_planes (parallelotope.h:129-153);
+ * HyperplaneParallelotope::set_data and HyperplaneSimplexGeometry::set_data
+ * (hyperplane_parallelotope.h:68-74, hyperplane_geometry.h:80-110). */
+typedef struct {
+    int kind;
+    double org[NM];
+    double n[NM];
+    double pn[NM][NM];
+    double d1[NM], d2[NM];
+} shape_geom;
+
+static void shape_setup(int n, int kind, const double* org, const double (*edges)[NM], shape_geom* g)
+{
+    g->kind = kind;
+    memcpy(g->org, org, sizeof(double) * n);
+    double vec[(NM - 1) * NM];
+    if (kind == 0) {
+        for (int i = 0; i < n; ++i) {
+            for (int a = 0, row = 0; a < n; ++a)
+                if (a != i) {
+                    memcpy(vec + row * n, edges[a], sizeof(double) * n);
+                    ++row;
+                }
+            double nn[NM], ov[NM];
+            orc_ortho_nn(n, vec, nn);
+            normalize_n(nn, n);
+            for (int k = 0; k < n; ++k) ov[k] = org[k] + edges[i][k];
+            double d1 = dotn(org, nn, n), d2 = -dotn(ov, nn, n);
+            if (dotn(nn, edges[i], n) > 0) {
+                for (int k = 0; k < n; ++k) nn[k] = -nn[k];
+                d1 = -d1;
+                d2 = -d2;
+            }
+            memcpy(g->pn[i], nn, sizeof(double) * n);
+            g->d1[i] = d1;
+            g->d2[i] = d2;
+        }
+        return;
+    }
+    for (int a = 0; a < n - 1; ++a) memcpy(vec + a * n, edges[a], sizeof(double) * n);
+    orc_ortho_nn(n, vec, g->n);
+    normalize_n(g->n, n);
+    double pts[NM][NM];
+    for (int a = 0; a < n - 1; ++a)
+        for (int k = 0; k < n; ++k) pts[a][k] = org[k] + edges[a][k];
+    memcpy(pts[n - 1], org, sizeof(double) * n);
+    for (int a = 0; a < n - 1; ++a)
+        for (int k = 0; k < n; ++k) vec[a * n + k] = pts[a][k] - pts[n - 1][k];
+    for (int i = 0; i < n - 1; ++i) {
+        double m[(NM - 1) * NM], pn[NM];
+        memcpy(m, vec, sizeof(double) * (n - 1) * n);
+        memcpy(m + i * n, g->n, sizeof(double) * n);
+        orc_ortho_nn(n, m, pn);
+        double d = dotn(pts[n - 1], pn, n);
+        double dist = dotn(pts[i], pn, n) - d;
+        for (int k = 0; k < n; ++k) g->pn[i][k] = pn[k] / dist;
+        g->d1[i] = d / dist;
+    }
+}
+
+/* Parallelotope::intersect (parallelotope.h:196-246); HyperplaneParallelotopeGeometry::
+ * intersect (hyperplane_geometry.h:41-52, 240-261) */
+static int shape_intersect(int n, const shape_geom* g, const double* o, const double* d, double* t)
+{
+    if (g->kind == 0) {
+        double f_max = -DBL_MAX, b_min = DBL_MAX;
+        for (int i = 0; i < n; ++i) {
+            double sd = dotn(d, g->pn[i], n);
+            double od = dotn(o, g->pn[i], n);
+            if (sd == 0) {
+                if (od - g->d1[i] > 0 || -od - g->d2[i] > 0) return 0;
+                continue;
+            }
+            double a1 = (g->d1[i] - od) / sd;
+            double a2 = (g->d2[i] + od) / -sd;
+            if (sd < 0) {
+                if (a1 > f_max) f_max = a1;
+                if (a2 < b_min) b_min = a2;
+            } else {
+                if (a1 < b_min) b_min = a1;
+                if (a2 > f_max) f_max = a2;
+            }
+            if (b_min <= 0 || b_min < f_max) return 0;
+        }
+        *t = f_max > 0 ? f_max : b_min;
+        return 1;
+    }
+    double sd = dotn(g->n, d, n);
+    if (sd == 0) return 0;
+    double v[NM], p[NM];
+    for (int k = 0; k < n; ++k) v[k] = g->org[k] - o[k];
+    double tt = dotn(v, g->n, n) / sd;
+    if (!(tt > 0)) return 0;
+    for (int k = 0; k < n; ++k) p[k] = o[k] + d[k] * tt;
+    for (int i = 0; i < n - 1; ++i) {
+        double c = dotn(p, g->pn[i], n) - g->d1[i];
+        if (c <= 0 || c >= 1) return 0;
+    }
+    *t = tt;
+    return 1;
+}
+
+/* Parallelotope::normal (parallelotope.h:261-290), HyperplaneParallelotope::normal */
+static void shape_normal(int n, const shape_geom* g, const double* p, double* out)
+{
+    if (g->kind != 0) {
+        memcpy(out, g->n, sizeof(double) * n);
+        return;
+    }
+    double mn = DBL_MAX;
+    int best = 0, neg = 0;
+    for (int i = 0; i < n; ++i) {
+        double dd = dotn(p, g->pn[i], n);
+        double l = fabs(dd - g->d1[i]);
+        if (l < mn) {
+            mn = l;
+            best = i;
+            neg = 0;
+        }
+        l = fabs(-dd - g->d2[i]);
+        if (l < mn) {
+            mn = l;
+            best = i;
+            neg = 1;
+        }
+    }
+    for (int k = 0; k < n; ++k) out[k] = neg ? -g->pn[best][k] : g->pn[best][k];
+}
+
+int orc_shape_intersect(int n, int kind, const double* org, const double* edges, const double* o, const double* d,
+                        double* t, double* normal)
+{
+    shape_geom g;
+    shape_setup(n, kind, org, (const double(*)[NM])edges, &g);
+    if (!shape_intersect(n, &g, o, d, t)) return 0;
+    double p[NM];
+    for (int k = 0; k < n; ++k) p[k] = o[k] + d[k] * *t;
+    shape_normal(n, &g, p, normal);
+    return 1;
+}
+
 typedef struct {
     const orc_mesh* m;
     const orc_scene* s;
+    shape_geom shapes[ORC_MAX_SHAPES];
     double cam_dir[NM];        /* perspective: times the screen distance */
     double axes[NM - 1][NM];   /* parallel: times units_per_pixel */
     double square_radius;      /* spherical */
@@ -1030,12 +1173,37 @@ static void hit_normals(const orc_mesh* m, int64_t f, const double* p, double* g
 
 static int light_visible(const pctx* c, pstate* st, const double* org, const double* dir, double dist)
 {
-    /* light_source_is_visible / object_is_obstacle_to_light, painter.cpp:142-186 */
+    /* light_source_is_visible / object_is_obstacle_to_light, painter.cpp:142-186:
+     * the objects in order (the mesh, then the shapes), one ray count each,
+     * until one blocks the light */
     st->rays += 1;
     double t;
     int64_t f;
-    if (!nearest(c->m, org, dir, &t, &f)) return 1;
-    return !(t < dist);
+    if (nearest(c->m, org, dir, &t, &f) && t < dist) return 0;
+    for (int k = 0; k < c->s->shape_count; ++k) {
+        st->rays += 1;
+        if (shape_intersect(c->m->n, &c->shapes[k], org, dir, &t) && t < dist) return 0;
+    }
+    return 1;
+}
+
+/* ray_intersection over the objects (src/painter/space/ray_intersection.h:20-80):
+ * the nearest of the mesh hit and the shapes' hits; *shape = -1 for the mesh */
+static int nearest_object(const pctx* c, const double* org, const double* dir, double* t, int64_t* f, int* shape)
+{
+    int found = nearest(c->m, org, dir, t, f);
+    *shape = -1;
+    double best = found ? *t : DBL_MAX;
+    for (int k = 0; k < c->s->shape_count; ++k) {
+        double ts;
+        if (shape_intersect(c->m->n, &c->shapes[k], org, dir, &ts) && ts < best) {
+            best = ts;
+            *shape = k;
+            found = 1;
+        }
+    }
+    if (found) *t = best;
+    return found;
 }
 
 static void trace_path(const pctx* c, pstate* st, int level, float color_level, const double* org,
@@ -1044,11 +1212,12 @@ static void trace_path(const pctx* c, pstate* st, int level, float color_level, 
 /* direct_diffuse_lighting, painter.cpp:197-292, one point light (PointLight::properties,
  * src/painter/lights/light_source.h:42-56) */
 static void direct_lighting(const pctx* c, pstate* st, const double* p, const double* gn, const double* sn,
-                            float* out)
+                            int mesh, float* out)
 {
     int n = c->m->n;
     const orc_scene* s = c->s;
     out[0] = out[1] = out[2] = 0;
+    if (s->no_light) return;
     double vl[NM];
     for (int i = 0; i < n; ++i) vl[i] = s->light_pos[i] - p[i];
     double sq = dotn(vl, vl, n);
@@ -1063,7 +1232,7 @@ static void direct_lighting(const pctx* c, pstate* st, const double* p, const do
     if (dln <= 0) return;
     double dist = sqrt(dotn(vl, vl, n));
     double o[NM];
-    int mesh_smooth_back = c->m->vn && s->smooth_normal && dotn(ldir, gn, n) < 0;
+    int mesh_smooth_back = mesh && c->m->vn && s->smooth_normal && dotn(ldir, gn, n) < 0;
     if (!mesh_smooth_back) {
         for (int i = 0; i < n; ++i) o[i] = p[i] + ldir[i] * c->ray_offset;
         if (!light_visible(c, st, o, ldir, dist)) return;
@@ -1101,27 +1270,45 @@ static void trace_path(const pctx* c, pstate* st, int level, float color_level, 
     out[0] = out[1] = out[2] = 0;
     double t;
     int64_t f;
-    if (!nearest(m, org, dir, &t, &f)) {
-        const float* bg = diffuse_reflection ? c->bg_light : s->background_rgb;
+    int shape;
+    if (!nearest_object(c, org, dir, &t, &f, &shape)) {
+        const float* bg = (diffuse_reflection && !s->background_not_light) ? c->bg_light : s->background_rgb;
         out[0] = bg[0]; out[1] = bg[1]; out[2] = bg[2];
         return;
     }
+    const int mesh = shape < 0;
     double p[NM], gn[NM], sn[NM];
     for (int i = 0; i < n; ++i) p[i] = org[i] + dir[i] * t;
-    hit_normals(m, f, p, gn, sn);
+    const float* rgb = s->object_rgb;
+    float dif = s->diffuse;
+    if (mesh) {
+        hit_normals(m, f, p, gn, sn);
+    } else {
+        shape_normal(n, &c->shapes[shape], p, gn);
+        normalize_n(gn, n);  /* set_geometric_normal normalises, objects.h:72-75 */
+        memcpy(sn, gn, sizeof(double) * n);
+        rgb = s->shapes[shape].rgb;
+        dif = s->shapes[shape].diffuse;
+    }
     double dgn = dotn(dir, gn, n);
     if (fabs(dgn) <= 0) return;
-    if (!(m->vn && s->smooth_normal)) memcpy(sn, gn, sizeof(double) * n);
+    if (!mesh && s->shapes[shape].light) {
+        /* trace_path, painter.cpp:355-358 */
+        const float* lc = diffuse_reflection ? s->shapes[shape].light_rgb : s->shapes[shape].rgb;
+        out[0] = lc[0]; out[1] = lc[1]; out[2] = lc[2];
+        return;
+    }
+    if (!(mesh && m->vn && s->smooth_normal)) memcpy(sn, gn, sizeof(double) * n);
     if (dgn > 0) {
         for (int i = 0; i < n; ++i) { gn[i] = -gn[i]; sn[i] = -sn[i]; }
     }
-    if (!(s->diffuse > 0)) return;
+    if (!(dif > 0)) return;
     float sc[3];
-    for (int k = 0; k < 3; ++k) sc[k] = s->diffuse * s->object_rgb[k];
+    for (int k = 0; k < 3; ++k) sc[k] = dif * rgb[k];
     float new_level = color_level * maxf3(sc);
     if (!(new_level >= 1e-4f)) return;
     float direct[3];
-    direct_lighting(c, st, p, gn, sn, direct);
+    direct_lighting(c, st, p, gn, sn, mesh, direct);
     float diffuse[3] = {0, 0, 0};
     if (level < s->max_depth) {
         double u[8];
@@ -1130,7 +1317,7 @@ static void trace_path(const pctx* c, pstate* st, int level, float color_level, 
         double nd[NM];
         orc_cosine_hemisphere(n, sn, u, nd);
         normalize_n(nd, n);  /* Ray ctor */
-        if (dotn(nd, gn, n) > 0) {
+        if (!mesh || dotn(nd, gn, n) > 0) {  /* only a mesh's bounce must leave its side, painter.cpp:308-313 */
             double o[NM];
             for (int i = 0; i < n; ++i) o[i] = p[i] + nd[i] * c->ray_offset;
             trace_path(c, st, level + 1, new_level, o, nd, 1, diffuse);
@@ -1223,9 +1410,21 @@ static int make_ctx(const orc_mesh* m, const orc_scene* s, pctx* c)
     c->m = m;
     c->s = s;
     if (projector_setup(s, n, c->cam_dir, c->axes, &c->square_radius)) return 1;
-    /* compute_ray_offset, painter.cpp:539-558 */
+    /* compute_ray_offset, painter.cpp:539-558, over every object's vertices */
     double all_max = -INFINITY;
     for (int i = 0; i < n; ++i) all_max = fmax(all_max, fmax(fabs(m->mn[i]), fabs(m->mx[i])));
+    if (s->shape_count < 0 || s->shape_count > ORC_MAX_SHAPES) return fail("bad shape count %d", s->shape_count);
+    for (int k = 0; k < s->shape_count; ++k) {
+        shape_setup(n, s->shapes[k].kind, s->shapes[k].org, (const double(*)[NM])s->shapes[k].edges, &c->shapes[k]);
+        int edges = s->shapes[k].kind == 0 ? n : n - 1;
+        for (int mask = 0; mask < (1 << edges); ++mask)
+            for (int i = 0; i < n; ++i) {
+                double x = s->shapes[k].org[i];
+                for (int a = 0; a < edges; ++a)
+                    if (mask & (1 << a)) x += s->shapes[k].edges[a][i];
+                all_max = fmax(all_max, fabs(x));
+            }
+    }
     c->ray_offset = all_max * (1000 * 2.220446049250313e-16);
     c->coef = orc_cosine_sphere_coefficient(n);
     c->light_coef = pow(s->light_unit_distance, n - 1);

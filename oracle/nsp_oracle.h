@@ -36,6 +36,7 @@ extern "C" {
 #endif
 
 #define ORC_NMAX 6
+#define ORC_MAX_SHAPES 16
 
 typedef struct orc_mesh orc_mesh;
 
@@ -110,6 +111,20 @@ typedef struct {
     int pass_index;
     int projector;             /* 0 parallel, 1 perspective, 2 spherical */
     double view_angle_degrees; /* perspective and spherical */
+    /* objects after the mesh, in order (VisibleParallelotope kind 0,
+     * VisibleHyperplaneParallelotope kind 1, src/painter/visible_shapes.h:31-117) */
+    int shape_count;
+    struct {
+        int kind;
+        double org[ORC_NMAX];
+        double edges[ORC_NMAX][ORC_NMAX];
+        float rgb[3];
+        float diffuse;
+        int light;            /* a light source emitting light_rgb */
+        float light_rgb[3];
+    } shapes[ORC_MAX_SHAPES];
+    int no_light;             /* 1: no light source (light_* ignored) */
+    int background_not_light; /* 1: the default surface is not a light source */
 } orc_scene;
 
 /* Projector::ray for a screen point in pixel units (the `point` argument):
@@ -117,6 +132,12 @@ typedef struct {
  * (src/painter/projectors/projector.h:102-275); dir is unit (Ray ctor).
  * Returns nonzero on an invalid projector. */
 int orc_projector_ray(const orc_scene* scene, const double* point, double* org, double* dir);
+
+/* One shape (kind 0 parallelotope with n edges, 1 hyperplane parallelotope with
+ * n-1; edges row-major with row stride ORC_NMAX): the nearest t > 0 along the
+ * ray (o, d) and the normal there; returns 0 on a miss. */
+int orc_shape_intersect(int n, int kind, const double* org, const double* edges, const double* o, const double* d,
+                        double* t, double* normal);
 
 /* StratifiedJitteredSampler::one_dimension_size, src/painter/sampling/sampler.h:31-52 */
 int orc_samples_per_dimension(int screen_dims, int spp);

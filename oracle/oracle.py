@@ -13,6 +13,14 @@ LIB_PATH = os.path.join(HERE, "libnsp_oracle.so")
 NMAX = 6
 
 
+MAX_SHAPES = 16
+
+
+class ShapeC(C.Structure):
+    _fields_ = [("kind", C.c_int), ("org", C.c_double * NMAX), ("edges", (C.c_double * NMAX) * NMAX),
+                ("rgb", C.c_float * 3), ("diffuse", C.c_float), ("light", C.c_int), ("light_rgb", C.c_float * 3)]
+
+
 class Scene(C.Structure):
     _fields_ = [("n", C.c_int), ("cam_org", C.c_double * NMAX), ("cam_dir", C.c_double * NMAX),
                 ("axes", (C.c_double * NMAX) * (NMAX - 1)), ("units_per_pixel", C.c_double),
@@ -20,7 +28,8 @@ class Scene(C.Structure):
                 ("light_unit_distance", C.c_double), ("background_rgb", C.c_float * 3),
                 ("object_rgb", C.c_float * 3), ("diffuse", C.c_float), ("samples_per_pixel", C.c_int),
                 ("max_depth", C.c_int), ("smooth_normal", C.c_int), ("seed", C.c_uint64), ("pass_index", C.c_int),
-                ("projector", C.c_int), ("view_angle_degrees", C.c_double)]
+                ("projector", C.c_int), ("view_angle_degrees", C.c_double), ("shape_count", C.c_int),
+                ("shapes", ShapeC * MAX_SHAPES), ("no_light", C.c_int), ("background_not_light", C.c_int)]
 
 
 _lib = None
@@ -203,9 +212,28 @@ def make_scene(cfg, spp, max_depth, seed, smooth=False, pass_index=0):
     s.units_per_pixel = cfg.get("units_per_pixel", 0.0)
     s.projector = cfg.get("projector", 0)
     s.view_angle_degrees = cfg.get("view_angle", 0.0)
-    s.light_pos[:n] = cfg["light_pos"]
-    s.light_rgb[:] = cfg["light_rgb"]
-    s.light_unit_distance = cfg["light_unit_distance"]
+    if cfg.get("light_pos") is None:
+        s.no_light = 1
+    else:
+        s.light_pos[:n] = cfg["light_pos"]
+        s.light_rgb[:] = cfg["light_rgb"]
+        s.light_unit_distance = cfg["light_unit_distance"]
+    s.background_not_light = 0 if cfg.get("background_is_light", True) else 1
+    shapes = cfg.get("shapes") or []
+    if len(shapes) > MAX_SHAPES:
+        raise ValueError("too many shapes for the checker")
+    s.shape_count = len(shapes)
+    for k, sh in enumerate(shapes):
+        d = s.shapes[k]
+        d.kind = sh["kind"]
+        d.org[:n] = sh["org"]
+        for a, e in enumerate(sh["edges"]):
+            d.edges[a][:n] = e
+        d.rgb[:] = sh["rgb"]
+        d.diffuse = sh.get("diffuse", 1.0)
+        d.light = 0 if sh.get("light_rgb") is None else 1
+        if sh.get("light_rgb") is not None:
+            d.light_rgb[:] = sh["light_rgb"]
     s.background_rgb[:] = cfg["background_rgb"]
     s.object_rgb[:] = cfg["object_rgb"]
     s.diffuse = cfg["diffuse"]
@@ -249,3 +277,23 @@ def projector_ray(cfg, point):
     if L.orc_projector_ray(C.byref(s), _p(p, C.c_double), _p(org, C.c_double), _p(d, C.c_double)):
         raise ValueError("invalid projector")
     return org, d
+
+
+def shape_intersect(kind, org, edges, o, d):
+    """One shape's nearest hit t > 0 along (o, d) and the normal there, or None."""
+    n = len(org)
+    e = np.zeros((NMAX, NMAX))
+    for a, v in enumerate(edges):
+        e[a, :n] = v
+    org_ = np.ascontiguousarray(org, np.float64)
+    o_ = np.ascontiguousarray(o, np.float64)
+    d_ = np.ascontiguousarray(d, np.float64)
+    t = C.c_double()
+    nrm = np.zeros(n)
+    L = lib()
+    L.orc_shape_intersect.argtypes = [C.c_int, C.c_int] + [C.POINTER(C.c_double)] * 4 + \
+        [C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    if not L.orc_shape_intersect(n, kind, _p(org_, C.c_double), _p(e, C.c_double), _p(o_, C.c_double),
+                                 _p(d_, C.c_double), C.byref(t), _p(nrm, C.c_double)):
+        return None
+    return t.value, nrm

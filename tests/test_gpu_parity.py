@@ -219,7 +219,7 @@ def test_render_other_projectors_match_checker(ctx, n, kind, precision):
     rmse = float(np.sqrt(np.mean((img / cnt[:, None] - ref / cnt[:, None]) ** 2)))
     lit = float(np.mean(np.abs(ref / cnt[:, None] - np.array(configs.BACKGROUND_RGB)) > 1e-3))
     print(f"rmse {rmse:.3e} rays gpu {st['rays']} cpu {rays}, object pixels {lit:.2f}", flush=True)
-    assert 0.2 < lit < 0.95  # the object fills part of the view
+    assert 0.05 < lit < 0.95  # the object fills part of the view
     assert rmse < (1e-4 if precision == native.PRECISION_FP64 else 5e-3)
     if precision == native.PRECISION_FP64:
         assert abs(st["rays"] - rays) <= 0.001 * rays
@@ -418,7 +418,8 @@ def test_pixels_delivered_per_batch(ctx):
     sc = native.Scene(ctx, gm, cfg)
     P = sc.pixels
     runs = []
-    sc.paint(spp, depth, seed=5, batch_paths=27 * 100, passes=2,
+    # 8 requested samples in n = 4 are 2^3 = 8 per pixel: batches of 100 pixels
+    sc.paint(spp, depth, seed=5, batch_paths=8 * 100, passes=2,
              on_pixels=lambda first, count, rgb: runs.append((first, count, rgb)))
     starts = [r[0] for r in runs]
     assert len(runs) == 2 * ((P + 99) // 100)

@@ -245,3 +245,72 @@ def test_projector_errors():
         O.projector_ray(_proj_cfg(3, 2, 170.0, [10, 10]), [1, 1])  # too big for a square screen
     with pytest.raises(ValueError):
         O.projector_ray(_proj_cfg(3, 0, 0.0, [10, 10], upp=0.0), [1, 1])
+
+
+# ---------------------------------------------------------------------------
+# Shapes (src/painter/space/parallelotope.h, hyperplane_parallelotope.h): the
+# oracle's plane-based tests against the parameter form p = org + E c, solved
+# by linear algebra
+
+@pytest.mark.parametrize("n", [3, 4, 5])
+def test_parallelotope_intersection_by_coordinates(n):
+    rng = np.random.default_rng(40 + n)
+    checked = 0
+    for _ in range(300):
+        E = rng.normal(size=(n, n)) + 2 * np.eye(n)
+        org = rng.normal(size=n)
+        o = org + rng.normal(size=n) * 3
+        d = (org + E.sum(axis=0) * rng.random() - o)
+        d /= np.linalg.norm(d)
+        Einv = np.linalg.inv(E.T)          # c = Einv (p - org), columns of E.T are the edges
+        c0, c1 = Einv @ (o - org), Einv @ d
+        lo, hi = -np.inf, np.inf
+        for i in range(n):
+            if c1[i] == 0:
+                continue
+            a, b = (0 - c0[i]) / c1[i], (1 - c0[i]) / c1[i]
+            lo, hi = max(lo, min(a, b)), min(hi, max(a, b))
+        hit = hi > max(lo, 0)
+        r = O.shape_intersect(0, org, E, o, d)
+        if not hit or min(abs(hi - lo), abs(lo), abs(hi)) < 1e-9:
+            assert (r is None) == (not hit) or min(abs(hi - lo), abs(lo), abs(hi)) < 1e-9
+            continue
+        assert r is not None
+        t, nrm = r
+        expect = lo if lo > 0 else hi
+        assert abs(t - expect) < 1e-9 * (1 + abs(expect))
+        # the normal of the face the hit lies on: the gradient of the coordinate that is 0 or 1 there
+        c = c0 + t * c1
+        i = int(np.argmin(np.minimum(np.abs(c), np.abs(c - 1))))
+        g = Einv[i] * (1 if abs(c[i] - 1) < abs(c[i]) else -1)
+        assert np.allclose(nrm, g / np.linalg.norm(g), atol=1e-9)
+        checked += 1
+    assert checked > 100
+
+
+@pytest.mark.parametrize("n", [3, 4, 5])
+def test_hyperplane_parallelotope_intersection_by_coordinates(n):
+    rng = np.random.default_rng(60 + n)
+    hits = misses = 0
+    for _ in range(300):
+        E = rng.normal(size=(n - 1, n))
+        org = rng.normal(size=n)
+        o = org + rng.normal(size=n) * 3
+        target = org + E.T @ rng.uniform(-0.3, 1.3, size=n - 1)
+        d = (target - o) * (1 if rng.random() < 0.8 else -1)  # some rays point away
+        d /= np.linalg.norm(d)
+        # org + E^T c = o + t d  ->  [E^T, -d] [c; t] = o - org
+        sol = np.linalg.solve(np.column_stack([E.T, -d]), o - org)
+        c, t = sol[:-1], sol[-1]
+        inside = t > 0 and np.all(c > 0) and np.all(c < 1)
+        if np.min(np.abs(np.concatenate([c, c - 1, [t]]))) < 1e-9:
+            continue
+        r = O.shape_intersect(1, org, E, o, d)
+        assert (r is not None) == inside
+        if inside:
+            assert abs(r[0] - t) < 1e-9 * (1 + abs(t))
+            assert np.allclose(E @ r[1], 0, atol=1e-9) and abs(np.linalg.norm(r[1]) - 1) < 1e-12
+            hits += 1
+        else:
+            misses += 1
+    assert hits > 20 and misses > 20
