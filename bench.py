@@ -394,10 +394,20 @@ def main():
                     bytes_per_launch=int(k["bytes"] / max(k["launches"] // max(args.steps, 1), 1)),
                     per_ray=k["per_ray"])
 
-    dominant = max(kernels, key=lambda name: kernels[name]["ms"])
-    roofline = line(dominant)
-    roofline["other"] = line("k_shadow" if dominant == "k_closest" else "k_closest")
-    roofline["stage_ms_per_step"] = {k: round(v / args.steps, 2) for k, v in stage_ms.items()}
+    if min(kernels[k]["ms"] for k in kernels) > 0:
+        dominant = max(kernels, key=lambda name: kernels[name]["ms"])
+        roofline = line(dominant)
+        roofline["other"] = line("k_shadow" if dominant == "k_closest" else "k_closest")
+        roofline["stage_ms_per_step"] = {k: round(v / args.steps, 2) for k, v in stage_ms.items()}
+        # the shadow walks run on a second stream beside the next depth's closest-hit walks, so
+        # each kernel's launch time includes sharing the GPU with the other; both walks' bytes
+        # over the whole step bound their joint rate from below
+        both = (kernels["k_closest"]["bytes"] + kernels["k_shadow"]["bytes"]) / elapsed / 1e9
+        roofline["both_walks_over_step"] = dict(achieved=round(both, 1), unit="GB/s",
+                                                frac=round(both / HBM_PEAK_GBS, 4))
+    else:
+        roofline = None  # hipGraph replay: no per-stage events inside the graph
+        log("stage times are not measured inside a captured graph: no roofline line")
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(v, f, cfg, spp, depth)

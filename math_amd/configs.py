@@ -14,6 +14,9 @@ unit-intensity distance is its distance from the origin.
       on S^3 with radial N(0, 0.02) (seed 1), light (2,2,2,2), 128^3, 16 spp, depth 5
   C2  n=5 penteract boundary, 10 x 10^4 cells x 24 Kuhn 4-simplices (2.4M before
       the cap), on S^4 with a cap of angular radius 0.5 removed (seed 2), 48^4, 8 spp, depth 4
+
+projector_dict gives the same scenes through a perspective or spherical
+projector; cornell_box builds the reference's Cornell box around a 3-space mesh.
 """
 import numpy as np
 

@@ -89,8 +89,8 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_first_hit(TreeArgs A, int64_t c
                                                            const float* __restrict__ dir, int32_t* __restrict__ idx,
                                                            double* __restrict__ tout, uint8_t* __restrict__ near)
 {
-    __shared__ int stack[MAX_LEVELS * TRACE_BLOCK];
-    int* st = stack + threadIdx.x;
+    __shared__ int2 stack[MAX_LEVELS * TRACE_BLOCK];
+    int2* st = stack + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x; i < count; i += stride) {
         float o[N], d[N];
@@ -219,14 +219,14 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_raygen(SceneK<T> sc, const Batc
     }
 }
 
-template <int N, typename T, bool SMOOTH>
+template <int N, typename T, bool SMOOTH, bool SHAPES>
 __global__ void __launch_bounds__(TRACE_BLOCK)
     k_shade(SceneK<T> sc, TreeArgs A, MeshShadeArgs M, const BatchDesc* __restrict__ desc, int depth,
             RayQueue<T> qin, RayQueue<T> qout, ShadowQueue<T> sq, int64_t cap, unsigned long long* __restrict__ cnt,
             float* __restrict__ L, float* __restrict__ aov)
 {
-    __shared__ int stack[MAX_LEVELS * TRACE_BLOCK];
-    int* st = stack + threadIdx.x;
+    __shared__ int2 stack[MAX_LEVELS * TRACE_BLOCK];
+    int2* st = stack + threadIdx.x;
     const int64_t count = (int64_t)cnt[depth * CNT_STRIDE + 0];
     unsigned long long* next_count = &cnt[(depth + 1) * CNT_STRIDE + 0];
     unsigned long long* shadow_count = &cnt[depth * CNT_STRIDE + 1];
@@ -254,14 +254,14 @@ __global__ void __launch_bounds__(TRACE_BLOCK)
                 d[k] = qin.dir[(int64_t)k * cap + i];
             }
             const int64_t gp = pixel0 * sc.S + path;
-            float av[3 + N];
-            shade_one<N, T, SMOOTH>(sc, A, M, depth, gp / sc.S, (int)(gp % sc.S), o, d, qin.hit[i], qin.hit_t[i], beta,
-                            qin.level[i], st, probe_count, (aov && depth == 0) ? av : nullptr, out);
+            shade_one<N, T, SMOOTH, SHAPES>(sc, A, M, depth, gp / sc.S, (int)(gp % sc.S), o, d, qin.hit[i],
+                                            qin.hit_t[i], beta, qin.level[i], st, probe_count, aov && depth == 0,
+                                            out);
 #pragma unroll
             for (int c = 0; c < 3; ++c) L[c * cap + path] += out.add[c];
             if (aov && depth == 0) {
 #pragma unroll
-                for (int c = 0; c < 3 + N; ++c) aov[c * cap + path] = av[c];
+                for (int c = 0; c < 3 + N; ++c) aov[c * cap + path] = out.av[c];
                 // direct channels start with what the primary ray sees itself
                 // (background or emitter); first-hit shadow rays add to them
 #pragma unroll
@@ -396,8 +396,8 @@ template <int N, typename T, int LAY>
 __global__ void __launch_bounds__(TRACE_BLOCK) k_closest(TreeArgs A, RayQueue<T> q, int64_t cap,
                                                          const unsigned long long* __restrict__ cnt)
 {
-    __shared__ int stack[MAX_LEVELS * TRACE_BLOCK];
-    int* st = stack + threadIdx.x;
+    __shared__ int2 stack[MAX_LEVELS * TRACE_BLOCK];
+    int2* st = stack + threadIdx.x;
     const int64_t count = (int64_t)cnt[0];
     for (int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x; i < count;
          i += (int64_t)gridDim.x * blockDim.x) {
@@ -432,8 +432,8 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_closest_pool(TreeArgs A, RayQue
                                                               unsigned long long* __restrict__ res_count,
                                                               int32_t* __restrict__ res)
 {
-    __shared__ int stack[MAX_LEVELS * TRACE_BLOCK];
-    int* st = stack + threadIdx.x;
+    __shared__ int2 stack[MAX_LEVELS * TRACE_BLOCK];
+    int2* st = stack + threadIdx.x;
     RayPool pool(cnt + 8, (int64_t)cnt[0]);
     RayWalk<N, T, false, LAY, true> w;
     int64_t ray = -1;
@@ -499,8 +499,8 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_closest_exact(TreeArgs A, RayQu
                                                                const unsigned long long* __restrict__ redo_count,
                                                                const int32_t* __restrict__ redo)
 {
-    __shared__ int stack[MAX_LEVELS * TRACE_BLOCK];
-    int* st = stack + threadIdx.x;
+    __shared__ int2 stack[MAX_LEVELS * TRACE_BLOCK];
+    int2* st = stack + threadIdx.x;
     const int64_t count = (int64_t)*redo_count;
     for (int64_t j = (int64_t)xcd_block() * blockDim.x + threadIdx.x; j < count;
          j += (int64_t)gridDim.x * blockDim.x) {
@@ -527,8 +527,8 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_shadow(TreeArgs A, ShadowQueue<
                                                         unsigned long long* __restrict__ res_count,
                                                         int32_t* __restrict__ res)
 {
-    __shared__ int stack[MAX_LEVELS * TRACE_BLOCK];
-    int* st = stack + threadIdx.x;
+    __shared__ int2 stack[MAX_LEVELS * TRACE_BLOCK];
+    int2* st = stack + threadIdx.x;
     RayPool pool(cnt + 15, (int64_t)cnt[0]);
     RayWalk<N, T, true, LAY, true> w;
     int64_t ray = -1;
@@ -571,8 +571,8 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_shadow_exact(TreeArgs A, Shadow
                                                               const unsigned long long* __restrict__ redo_count,
                                                               const int32_t* __restrict__ redo)
 {
-    __shared__ int stack[MAX_LEVELS * TRACE_BLOCK];
-    int* st = stack + threadIdx.x;
+    __shared__ int2 stack[MAX_LEVELS * TRACE_BLOCK];
+    int2* st = stack + threadIdx.x;
     const int64_t count = (int64_t)*redo_count;
     for (int64_t j = (int64_t)xcd_block() * blockDim.x + threadIdx.x; j < count;
          j += (int64_t)gridDim.x * blockDim.x) {
@@ -750,7 +750,8 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
     auto kcp = aos ? k_closest_pool<N, T, 1> : k_closest_pool<N, T, 0>;
     const unsigned gcp = aos ? persistent_grid(k_closest_pool<N, T, 1>) : persistent_grid(k_closest_pool<N, T, 0>);
 
-    auto kshade = sc.smooth ? k_shade<N, T, true> : k_shade<N, T, false>;
+    auto kshade = sc.shape_count > 0 ? (sc.smooth ? k_shade<N, T, true, true> : k_shade<N, T, false, true>)
+                                     : (sc.smooth ? k_shade<N, T, true, false> : k_shade<N, T, false, false>);
     auto ks = aos ? k_shadow<N, T, 1> : k_shadow<N, T, 0>;
     const unsigned gs = aos ? persistent_grid(k_shadow<N, T, 1>) : persistent_grid(k_shadow<N, T, 0>);
     const unsigned gx = 8 * 32;  // FP64 decisions and re-walks of shadow rays: a few per cent or fewer

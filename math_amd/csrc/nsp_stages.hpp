@@ -248,18 +248,20 @@ struct ShadeOut {
     T stmax;
     float contrib[3];  // radiance added when the shadow ray is unoccluded
     int sshape;        // the first shape blocking the shadow ray, or -1
+    float av[3 + N];   // first-hit albedo and normal (when asked for)
 };
 
 // One path vertex: trace_path's shading, direct_diffuse_lighting's shadow-ray
 // set-up and diffuse_lighting's bounce (src/painter/painter.cpp:197-397), for
-// the ray (o, d) whose nearest hit is (idx, t).  aov (3 + N floats) receives
-// the first-hit albedo and normal when non-null.
+// the ray (o, d) whose nearest hit is (idx, t).  With aov, out.av receives
+// the first-hit albedo and normal.
+// SHAPES: false compiles out the shapes beside the mesh (scenes without them).
 // SMOOTH: false compiles out the smooth-normal probes (scenes painted without
 // smooth normals never take them).
-template <int N, typename T, bool SMOOTH = true>
-__device__ void shade_one(const SceneK<T>& sc, const TreeArgs& A, const MeshShadeArgs& M, int depth, int64_t pixel,
-                          int sample, const T* o, const T* d, int idx, T t, const float* beta, float level, int* st,
-                          unsigned long long* probe_count, float* aov, ShadeOut<N, T>& out)
+template <int N, typename T, bool SMOOTH = true, bool SHAPES = true>
+__device__ __forceinline__ void shade_one(const SceneK<T>& sc, const TreeArgs& A, const MeshShadeArgs& M, int depth, int64_t pixel,
+                          int sample, const T* o, const T* d, int idx, T t, const float* beta, float level, int2* st,
+                          unsigned long long* probe_count, bool aov, ShadeOut<N, T>& out)
 {
     out.add[0] = out.add[1] = out.add[2] = 0.0f;
     out.next = false;
@@ -269,7 +271,7 @@ __device__ void shade_one(const SceneK<T>& sc, const TreeArgs& A, const MeshShad
     // ray_intersection over the objects (ray_intersection.h:20-80): the mesh
     // hit, or a nearer shape
     int shape = -1;
-    if (sc.shape_count > 0) {
+    if (SHAPES && sc.shape_count > 0) {
         double ts;
         shape = nearest_shape<N, T>(sc, o, d, idx >= 0 ? (double)t : DBL_MAX, &ts);
         if (shape >= 0) t = (T)ts;
@@ -281,11 +283,15 @@ __device__ void shade_one(const SceneK<T>& sc, const TreeArgs& A, const MeshShad
     for (int k = 0; k < N; ++k) p[k] = o[k] + d[k] * t;
     // the surface properties of the hit (SurfaceProperties, src/painter/objects.h:27-139)
     T gn[N], sn[N];
-    const float* rgb = sc.obj_rgb;
-    const float* light_rgb = sc.obj_light_rgb;
+    float rgb[3], light_rgb[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        rgb[c] = sc.obj_rgb[c];
+        light_rgb[c] = sc.obj_light_rgb[c];
+    }
     float diffuse = sc.diffuse;
     bool is_light = sc.obj_is_light;
-    if (shape >= 0) {
+    if (SHAPES && shape >= 0) {
         const ShapeRec& sr = sc.shapes[shape];
         double p64[N], n64[N];
 #pragma unroll
@@ -293,8 +299,11 @@ __device__ void shade_one(const SceneK<T>& sc, const TreeArgs& A, const MeshShad
         shape_normal<N>(sr, p64, n64);
 #pragma unroll
         for (int k = 0; k < N; ++k) gn[k] = (T)n64[k];
-        rgb = sr.rgb;
-        light_rgb = sr.light_rgb;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            rgb[c] = sr.rgb[c];
+            light_rgb[c] = sr.light_rgb[c];
+        }
         diffuse = sr.diffuse;
         is_light = sr.light;
     } else if (idx >= 0) {
@@ -310,9 +319,9 @@ __device__ void shade_one(const SceneK<T>& sc, const TreeArgs& A, const MeshShad
                 for (int k = 0; k < N; ++k) an[k] = -an[k];
         }
 #pragma unroll
-        for (int c = 0; c < 3; ++c) aov[c] = hit ? diffuse * rgb[c] : sc.bg_rgb[c];
+        for (int c = 0; c < 3; ++c) out.av[c] = hit ? diffuse * rgb[c] : sc.bg_rgb[c];
 #pragma unroll
-        for (int k = 0; k < N; ++k) aov[3 + k] = hit ? (float)an[k] : 0.0f;
+        for (int k = 0; k < N; ++k) out.av[3 + k] = hit ? (float)an[k] : 0.0f;
     }
     if (!hit) {
         // miss: background, a light source when reached by a diffuse bounce (painter.cpp:335-340)
@@ -324,9 +333,8 @@ __device__ void shade_one(const SceneK<T>& sc, const TreeArgs& A, const MeshShad
     T dgn = dot<N, T>(d, gn);
     if (fabs(dgn) <= (T)0) return;
     if (is_light) {
-        const float* lc = diffuse_reflection ? light_rgb : rgb;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) out.add[c] = beta[c] * lc[c];
+        for (int c = 0; c < 3; ++c) out.add[c] = beta[c] * (diffuse_reflection ? light_rgb[c] : rgb[c]);
         return;
     }
     if (mesh && sc.smooth && sc.has_normals)
@@ -406,7 +414,7 @@ __device__ void shade_one(const SceneK<T>& sc, const TreeArgs& A, const MeshShad
             if (lit) {
                 float w = (float)(sc.cos_coef * dln);
                 out.shadow = true;
-                if (sc.shape_count > 0) out.sshape = blocking_shape<N, T>(sc, o1, ld, (double)dist);
+                if (SHAPES && sc.shape_count > 0) out.sshape = blocking_shape<N, T>(sc, o1, ld, (double)dist);
 #pragma unroll
                 for (int k = 0; k < N; ++k) {
                     out.sorg[k] = o1[k];

@@ -321,10 +321,12 @@ struct RayWalk {
     }
 
     // One leaf of the walk; true when the walk has ended.
-    // stack: this lane's LDS stack, stack[l * TRACE_BLOCK].
+    // stack: this lane's LDS stack, stack[l * TRACE_BLOCK] = the node record
+    // of the walk's node at level l, so that a restart after a pop reads the
+    // node from LDS instead of global memory.
     // o64/d64: the ray in FP64 for the near-boundary re-test, or null to use
     // the FP32 ray converted exactly.
-    __device__ __forceinline__ bool step(const TreeArgs& A, int* stack, const double* o64, const double* d64)
+    __device__ __forceinline__ bool step(const TreeArgs& A, int2* stack, const double* o64, const double* d64)
     {
         const T* rlo = TreeBox<T>::lo(A);
         const T* rsz = TreeBox<T>::size(A);
@@ -344,31 +346,30 @@ struct RayWalk {
             T x = floor((o[i] + d[i] * tp - rlo[i]) * rinv[i]);
             Q[i] = (int)fmin(fmax(x, (T)0), (T)(R - 1));
         }
-        int node;
+        int2 nd;
         if (level < 0) {
             level = 0;
-            node = 0;
-            stack[0] = 0;
+            nd = A.nodes[0];
+            ++nvis;
+            stack[0] = nd;
         } else {
             unsigned diff = 0;
 #pragma unroll
             for (int i = 0; i < N; ++i) diff |= (unsigned)((Q[i] >> (lmax - level)) ^ q[i]);
             int pops = diff ? 32 - __clz(diff) : 0;
             level -= pops;
-            node = stack[level * TRACE_BLOCK];
+            nd = stack[level * TRACE_BLOCK];
         }
-        int2 nd = A.nodes[node];
-        ++nvis;
         while (nd.y < 0) {
             int shift = lmax - level - 1;
             int code = 0;
 #pragma unroll
             for (int i = 0; i < N; ++i) code |= ((Q[i] >> shift) & 1) << i;
-            node = nd.x + code * (-nd.y);
+            const int node = nd.x + code * (-nd.y);
             ++level;
-            stack[level * TRACE_BLOCK] = node;
             nd = A.nodes[node];
             ++nvis;
+            stack[level * TRACE_BLOCK] = nd;
         }
 #pragma unroll
         for (int i = 0; i < N; ++i) q[i] = Q[i] >> (lmax - level);
@@ -382,7 +383,8 @@ struct RayWalk {
         }
         T ta, tb;
         if (!slab<N, T>(blo, bsz, o, inv, d, &ta, &tb)) tb = tp;  // rounding: point located outside its own box
-        if (node != last_leaf && nd.y > 0) {
+        // a leaf with entries is known by its first entry (leaves own disjoint entry ranges)
+        if (nd.x != last_leaf && nd.y > 0) {
             ++lvis;
             // outcome of one candidate test; true ends the walk
             auto take = [&](int c, double th, int si, bool nr, int sid) -> bool {
@@ -457,7 +459,7 @@ struct RayWalk {
                 }
             }
         }
-        last_leaf = node;
+        if (nd.y > 0) last_leaf = nd.x;
         if (tb > t_cur) {
             t_cur = tb;
             adv = adv0;
@@ -523,7 +525,7 @@ __device__ __forceinline__ bool pending_resolve(const TreeArgs& A, const float* 
 
 // The whole walk of one ray in one call (first-hit queries and FP64 paths).
 template <int N, typename T, bool ANY>
-__device__ HitResult traverse(const TreeArgs& A, const T* o, const T* d, T tmax, int* stack, const double* o64,
+__device__ HitResult traverse(const TreeArgs& A, const T* o, const T* d, T tmax, int2* stack, const double* o64,
                               const double* d64)
 {
     RayWalk<N, T, ANY> w;
