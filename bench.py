@@ -402,7 +402,7 @@ def main():
         # the shadow walks run on a second stream beside the next depth's closest-hit walks, so
         # each kernel's launch time includes sharing the GPU with the other; both walks' bytes
         # over the whole step bound their joint rate from below
-        both = (kernels["k_closest"]["bytes"] + kernels["k_shadow"]["bytes"]) / elapsed / 1e9
+        both = (kernels["k_closest"]["bytes"] + kernels["k_shadow"]["bytes"]) / (elapsed / args.steps) / 1e9
         roofline["both_walks_over_step"] = dict(achieved=round(both, 1), unit="GB/s",
                                                 frac=round(both / HBM_PEAK_GBS, 4))
     else:
