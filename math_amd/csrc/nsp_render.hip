@@ -33,6 +33,12 @@ static unsigned grid_for(int64_t n, int block = TRACE_BLOCK, int64_t max_blocks 
     return (unsigned)b;
 }
 
+// Share (per cent) of a full persistent grid given to the shadow walks when they
+// run on the second stream.
+#ifndef NSP_SIDE_GRID_PCT
+#define NSP_SIDE_GRID_PCT 100
+#endif
+
 // Blocks of a persistent launch: every CU filled to the kernel's occupancy
 // (computed once per kernel), rounded to a multiple of 8 for the XCD remap.
 template <class K>
@@ -753,7 +759,10 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
     auto kshade = sc.shape_count > 0 ? (sc.smooth ? k_shade<N, T, true, true> : k_shade<N, T, false, true>)
                                      : (sc.smooth ? k_shade<N, T, true, false> : k_shade<N, T, false, false>);
     auto ks = aos ? k_shadow<N, T, 1> : k_shadow<N, T, 0>;
-    const unsigned gs = aos ? persistent_grid(k_shadow<N, T, 1>) : persistent_grid(k_shadow<N, T, 0>);
+    unsigned gs = aos ? persistent_grid(k_shadow<N, T, 1>) : persistent_grid(k_shadow<N, T, 0>);
+    // beside the closest-hit walks, the persistent shadow walk leaves part of the
+    // machine to the launch stream's kernels
+    if (rl.side) gs = std::max(8u, (gs * NSP_SIDE_GRID_PCT / 100 + 7) / 8 * 8);
     const unsigned gx = 8 * 32;  // FP64 decisions and re-walks of shadow rays: a few per cent or fewer
     int32_t* redo_s = b.redo.as<int32_t>();
     int32_t* res_s = b.res.as<int32_t>();
