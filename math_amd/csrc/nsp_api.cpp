@@ -516,8 +516,11 @@ int nsp_scene_create(nsp_context* ctx, int n, nsp_mesh* mesh, const nsp_surface*
         if (!mesh || !object || !projector || !default_surface) raise(NSP_E_INVALID, "null scene argument");
         if (mesh->m.n != n) raise(NSP_E_INVALID, "mesh dimension differs from scene dimension");
         if (light_count < 0 || (light_count > 0 && !lights)) raise(NSP_E_INVALID, "bad light list");
-        if (light_count > 1)
-            raise(NSP_E_UNSUPPORTED, "more than one light source is not supported by the GPU painter yet");
+        if (light_count > MAX_LIGHTS)
+            raise(NSP_E_UNSUPPORTED, "at most " + std::to_string(MAX_LIGHTS) + " light sources are supported");
+        for (int l = 0; l < light_count; ++l)
+            if (lights[l].kind != NSP_LIGHT_POINT && lights[l].kind != NSP_LIGHT_CONSTANT)
+                raise(NSP_E_INVALID, "unknown light kind " + std::to_string(lights[l].kind));
         if (projector->kind != NSP_PROJECTOR_PARALLEL && projector->kind != NSP_PROJECTOR_PERSPECTIVE &&
             projector->kind != NSP_PROJECTOR_SPHERICAL)
             raise(NSP_E_INVALID, "unknown projector kind " + std::to_string(projector->kind));
@@ -830,7 +833,7 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
         int64_t batch_pixels = std::max<int64_t>(1, want / S);
         batch_pixels = std::min<int64_t>(batch_pixels, scene->pixels);
         const int64_t cap = batch_pixels * S;
-        alloc_render_buffers(ctx->work, n, p.precision, cap, p.max_depth);
+        alloc_render_buffers(ctx->work, n, p.precision, cap, p.max_depth, (int)scene->lights.size());
         const bool raster = p.precision == NSP_PRECISION_FP32 && p.primary == NSP_PRIMARY_AUTO &&
                             scene->proj.kind == NSP_PROJECTOR_PARALLEL;
         if (raster) {
@@ -1155,12 +1158,13 @@ int nsp_batch_paint(nsp_batch* b, const nsp_paint_params* params, const volatile
         }
         for (int l = 0; l < lanes; ++l) {
             int64_t cap = 1;
-            int nmax = 3;
+            int nmax = 3, lmax = 1;
             for (int i : lane_scenes[l]) {
                 cap = std::max(cap, b->per[i].batch_pixels * b->per[i].kf.S);
                 nmax = std::max(nmax, b->scenes[i]->n);
+                lmax = std::max(lmax, (int)b->scenes[i]->lights.size());
             }
-            alloc_render_buffers(*b->work[l], nmax, NSP_PRECISION_FP32, cap, p.max_depth);
+            alloc_render_buffers(*b->work[l], nmax, NSP_PRECISION_FP32, cap, p.max_depth, lmax);
             if (raster) b->work[l]->keys.alloc(sizeof(unsigned long long) * cap);
         }
         NSP_HIP(hipStreamSynchronize(s));

@@ -225,7 +225,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_raygen(SceneK<T> sc, const Batc
     }
 }
 
-template <int N, typename T, bool SMOOTH, bool SHAPES>
+template <int N, typename T, bool SMOOTH, bool SHAPES, bool MULTI>
 __global__ void __launch_bounds__(TRACE_BLOCK)
     k_shade(SceneK<T> sc, TreeArgs A, MeshShadeArgs M, const BatchDesc* __restrict__ desc, int depth,
             RayQueue<T> qin, RayQueue<T> qout, ShadowQueue<T> sq, int64_t cap, unsigned long long* __restrict__ cnt,
@@ -240,7 +240,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK)
     const int64_t pixel0 = desc->pixel0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t base = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
-    const int64_t scap = cap * MAX_LIGHTS;
+    const int64_t scap = cap * (sc.light_count > 1 ? sc.light_count : 1);
     // every lane of a wave iterates the same number of times (wave-level appends)
     const int64_t iters = (count + stride - 1) / stride;
     for (int64_t it = 0; it < iters; ++it) {
@@ -260,7 +260,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK)
                 d[k] = qin.dir[(int64_t)k * cap + i];
             }
             const int64_t gp = pixel0 * sc.S + path;
-            shade_one<N, T, SMOOTH, SHAPES>(sc, A, M, depth, gp / sc.S, (int)(gp % sc.S), o, d, qin.hit[i],
+            shade_one<N, T, SMOOTH, SHAPES, MULTI>(sc, A, M, depth, gp / sc.S, (int)(gp % sc.S), o, d, qin.hit[i],
                                             qin.hit_t[i], beta, qin.level[i], st, probe_count, aov && depth == 0,
                                             out);
 #pragma unroll
@@ -288,29 +288,39 @@ __global__ void __launch_bounds__(TRACE_BLOCK)
             qout.beta[2 * cap + slot] = out.nbeta[2];
             qout.level[slot] = out.nlevel;
         }
-        int64_t ss = wave_append(out.shadow, shadow_count);
-        if (ss >= 0) {
-#pragma unroll
-            for (int k = 0; k < N; ++k) {
-                sq.org[(int64_t)k * scap + ss] = out.sorg[k];
-                sq.dir[(int64_t)k * scap + ss] = out.sdir[k];
+        // one shadow ray per light (a single pass when the shade body set it up)
+        const int lights = MULTI ? sc.light_count : 1;
+        for (int l = 0; l < lights; ++l) {
+            if (MULTI) {
+                out.shadow = false;
+                if (valid && out.direct)
+                    light_shadow_ray<N, T, SMOOTH, SHAPES>(sc, A, l, out.sp, out.sgn, out.ssn, out.mesh, out.sbeta,
+                                                           out.sscol, st, probe_count, out);
             }
-            sq.tmax[ss] = out.stmax;
-            // a first-hit shadow ray is marked by a negative path index while
-            // the direct-light AOV is collected
-            sq.path[ss] = (aov && depth == 0) ? -(path + 1) : path;
-            sq.contrib[ss] = out.contrib[0];
-            sq.contrib[scap + ss] = out.contrib[1];
-            sq.contrib[2 * scap + ss] = out.contrib[2];
-            if (sq.shape_count > 0) sq.shape[ss] = out.sshape;
+            int64_t ss = wave_append(out.shadow, shadow_count);
+            if (ss >= 0) {
+#pragma unroll
+                for (int k = 0; k < N; ++k) {
+                    sq.org[(int64_t)k * scap + ss] = out.sorg[k];
+                    sq.dir[(int64_t)k * scap + ss] = out.sdir[k];
+                }
+                sq.tmax[ss] = out.stmax;
+                // a first-hit shadow ray is marked by a negative path index while
+                // the direct-light AOV is collected
+                sq.path[ss] = (aov && depth == 0) ? -(path + 1) : path;
+                sq.contrib[ss] = out.contrib[0];
+                sq.contrib[scap + ss] = out.contrib[1];
+                sq.contrib[2 * scap + ss] = out.contrib[2];
+                if (sq.shape_count > 0) sq.shape[ss] = out.sshape;
+            }
         }
     }
 }
 
 // Adds an unoccluded shadow ray's contribution to its path (and, for a
 // first-hit shadow ray while AOVs are collected, to the direct-light AOV).
-// With one light per path and bounce, each path receives at most one
-// contribution per shadow launch, so the additions need no atomics.
+// With one light, each path receives at most one contribution per shadow
+// launch and the additions need no atomics; with several they are atomic.
 template <int N, typename T>
 __device__ __forceinline__ void shadow_unoccluded(const ShadowQueue<T>& sq, int64_t scap, int64_t i,
                                                   float* __restrict__ L, int64_t cap, float* __restrict__ direct)
@@ -325,6 +335,17 @@ __device__ __forceinline__ void shadow_unoccluded(const ShadowQueue<T>& sq, int6
     const int32_t raw = sq.path[i];
     const int32_t path = raw < 0 ? -raw - 1 : raw;
     const float c0 = sq.contrib[i], c1 = sq.contrib[scap + i], c2 = sq.contrib[2 * scap + i];
+    if (sq.multi) {
+        atomicAdd(&L[path], c0);
+        atomicAdd(&L[cap + path], c1);
+        atomicAdd(&L[2 * cap + path], c2);
+        if (raw < 0 && direct) {
+            atomicAdd(&direct[path], c0);
+            atomicAdd(&direct[cap + path], c1);
+            atomicAdd(&direct[2 * cap + path], c2);
+        }
+        return;
+    }
     L[path] += c0;
     L[cap + path] += c1;
     L[2 * cap + path] += c2;
@@ -654,7 +675,7 @@ __global__ void __launch_bounds__(256) k_accumulate(const BatchDesc* __restrict_
 
 // ---------------------------------------------------------------------------
 
-void alloc_render_buffers(RenderBuffers& b, int n, int precision, int64_t cap, int max_depth)
+void alloc_render_buffers(RenderBuffers& b, int n, int precision, int64_t cap, int max_depth, int lights)
 {
     // DevBuf::alloc keeps a buffer that is already large enough, so scenes of
     // different dimension and precision share one workspace that only grows
@@ -663,6 +684,7 @@ void alloc_render_buffers(RenderBuffers& b, int n, int precision, int64_t cap, i
     b.precision = precision;
     b.cap = cap;
     b.depth_slots = max_depth + 2;
+    b.lights = std::max(1, lights);
     for (DevBuf* q : {b.qa, b.qb}) {
         q[0].alloc(ts * n * cap);
         q[1].alloc(ts * n * cap);
@@ -672,7 +694,7 @@ void alloc_render_buffers(RenderBuffers& b, int n, int precision, int64_t cap, i
         q[5].alloc(sizeof(int32_t) * cap);
         q[6].alloc(ts * cap);
     }
-    int64_t scap = cap * MAX_LIGHTS;
+    int64_t scap = cap * b.lights;
     for (DevBuf* q : {b.sq[0], b.sq[1]}) {
         q[0].alloc(ts * n * scap);
         q[1].alloc(ts * n * scap);
@@ -720,7 +742,8 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
     RenderBuffers& b = *rl.buf;
     const SceneK<T>& sc = *static_cast<const SceneK<T>*>(rl.scene);
     const int64_t cap = b.cap;
-    const int64_t scap = cap * MAX_LIGHTS;
+    if (std::max(1, sc.light_count) > b.lights) raise(NSP_E_INVALID, "render buffers hold fewer lights than the scene");
+    const int64_t scap = cap * std::max(1, sc.light_count);
     TreeArgs A = make_tree_args(*rl.mesh);
     TreeArgs As = A;
     if (rl.counters) {
@@ -734,7 +757,8 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
     ShadowQueue<T> sqs[2];
     for (int k = 0; k < 2; ++k)
         sqs[k] = ShadowQueue<T>{b.sq[k][0].as<T>(), b.sq[k][1].as<T>(), b.sq[k][2].as<T>(), b.sq[k][3].as<int32_t>(),
-                                b.sq[k][4].as<float>(), b.sq[k][5].as<int32_t>(), sc.shape_count, nullptr};
+                                b.sq[k][4].as<float>(), b.sq[k][5].as<int32_t>(), sc.shape_count, nullptr,
+                                sc.light_count > 1};
     unsigned long long* cnt = b.counters.as<unsigned long long>();
     const BatchDesc* desc = b.desc.as<BatchDesc>();
     const BatchDesc* list = rl.batch_list ? rl.batch_list : b.batch_list.as<BatchDesc>();
@@ -756,8 +780,12 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
     auto kcp = aos ? k_closest_pool<N, T, 1> : k_closest_pool<N, T, 0>;
     const unsigned gcp = aos ? persistent_grid(k_closest_pool<N, T, 1>) : persistent_grid(k_closest_pool<N, T, 0>);
 
-    auto kshade = sc.shape_count > 0 ? (sc.smooth ? k_shade<N, T, true, true> : k_shade<N, T, false, true>)
-                                     : (sc.smooth ? k_shade<N, T, true, false> : k_shade<N, T, false, false>);
+    // shade variants: several lights (with shapes compiled in), shapes, or the plain mesh scene
+    auto kshade = sc.light_count > 1
+                      ? (sc.smooth ? k_shade<N, T, true, true, true> : k_shade<N, T, false, true, true>)
+                  : sc.shape_count > 0
+                      ? (sc.smooth ? k_shade<N, T, true, true, false> : k_shade<N, T, false, true, false>)
+                      : (sc.smooth ? k_shade<N, T, true, false, false> : k_shade<N, T, false, false, false>);
     auto ks = aos ? k_shadow<N, T, 1> : k_shadow<N, T, 0>;
     unsigned gs = aos ? persistent_grid(k_shadow<N, T, 1>) : persistent_grid(k_shadow<N, T, 0>);
     // beside the closest-hit walks, the persistent shadow walk leaves part of the

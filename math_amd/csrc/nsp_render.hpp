@@ -12,9 +12,10 @@
 
 namespace nsp {
 
-// One light per scene: each path then has at most one shadow ray per bounce,
-// so the shadow stage adds to a path's radiance without atomics.
-constexpr int MAX_LIGHTS = 1;
+// Light sources per scene.  With one light a path has at most one shadow ray
+// per bounce and the shadow stage adds to its radiance without atomics; with
+// more, the additions are atomic.
+constexpr int MAX_LIGHTS = 8;
 
 // A flat-shaded shape beside the mesh, FP64 (VisibleParallelotope /
 // VisibleHyperplaneParallelotope, src/painter/visible_shapes.h:31-117).
@@ -108,6 +109,7 @@ struct ShadowQueue {
     int32_t* shape;
     int shape_count;
     unsigned long long* extra;  // ray counts of the shape tests (the mesh test is the shadow ray itself)
+    int multi;                  // several shadow rays per path (more than one light): atomic additions
 };
 
 struct MeshShadeArgs {
@@ -129,6 +131,7 @@ struct RenderBuffers {
     int n = 0;
     int precision = 0;
     int64_t cap = 0;        // paths per batch
+    int lights = 1;         // shadow rays per path and bounce the shadow queues hold
     int depth_slots = 0;
     DevBuf qa[8], qb[8];
     DevBuf sq[2][6];        // shadow-ray queues of even and odd depths (ray, tmax, path, contribution, blocking shape)
@@ -209,7 +212,7 @@ void render_batch(const RenderLaunch& rl, hipStream_t s);
 void pass_reset(RenderBuffers& b, bool counters, int* pass_dev, int pass, hipStream_t s);
 // Clears a scene's own pass totals (3 u64) and batch index.
 void pass_reset_state(unsigned long long* totals, long long* batch_idx, int* pass_dev, int pass, hipStream_t s);
-void alloc_render_buffers(RenderBuffers& b, int n, int precision, int64_t cap, int max_depth);
+void alloc_render_buffers(RenderBuffers& b, int n, int precision, int64_t cap, int max_depth, int lights = 1);
 // FP64 records of shapes (org and edge vectors as given) into out (device), on stream s.
 void build_shape_records(int n, const int* kind, const double* org, const double* edges, int count, ShapeRec* out,
                          hipStream_t s);

@@ -249,16 +249,104 @@ struct ShadeOut {
     float contrib[3];  // radiance added when the shadow ray is unoccluded
     int sshape;        // the first shape blocking the shadow ray, or -1
     float av[3 + N];   // first-hit albedo and normal (when asked for)
+    // with several lights: the hit the shadow rays start from (direct: they are due)
+    bool direct, mesh;
+    T sp[N], sgn[N], ssn[N];
+    float sbeta[3], sscol[3];
 };
+
+// The shadow ray of one light source from a hit (direct_diffuse_lighting's
+// loop body, painter.cpp:205-289; PointLight and ConstantLight::properties,
+// src/painter/lights/light_source.h:26-78): sets out.shadow and, when the light
+// can reach the point, the ray, its distance limit, its contribution (path
+// weight beta times surface colour scol times light colour and cosine weight)
+// and the first shape in the way.
+template <int N, typename T, bool SMOOTH, bool SHAPES>
+__device__ __forceinline__ void light_shadow_ray(const SceneK<T>& sc, const TreeArgs& A, int l, const T* p,
+                                                 const T* gn, const T* sn, bool mesh, const float* beta,
+                                                 const float* scol, int2* st, unsigned long long* probe_count,
+                                                 ShadeOut<N, T>& out)
+{
+    out.shadow = false;
+    out.sshape = -1;
+    T vl[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) vl[k] = sc.light_pos[l][k] - p[k];
+    T sq2 = dot<N, T>(vl, vl);
+    T scale;
+    if (sc.light_kind[l] == 1)
+        scale = (T)1;
+    else if constexpr (N == 3)
+        scale = sc.light_coef[l] / sq2;
+    else
+        scale = sc.light_coef[l] / pow(sq2, (T)(N - 1) / (T)2);
+    float lc[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) lc[c] = sc.light_rgb[l][c] * (float)scale;
+    T ld[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) ld[k] = vl[k];
+    normalize<N, T>(ld);
+    T dln = dot<N, T>(ld, sn);
+    bool lit = !(fmaxf(lc[0], fmaxf(lc[1], lc[2])) < 1e-4f) && dln > (T)0;
+    if (!lit) return;
+    T dist = sqrt(dot<N, T>(vl, vl));
+    T o1[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) o1[k] = p[k] + ld[k] * sc.ray_offset;
+    T tmaxs = dist;
+    if (SMOOTH && mesh && sc.smooth && dot<N, T>(ld, gn) < (T)0) {
+        // smooth-normal back face: forward and reverse probes (painter.cpp:238-284)
+        atomicAdd(probe_count, 1ull);
+        double o64[N], d64[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            o64[k] = o1[k];
+            d64[k] = ld[k];
+        }
+        HitResult h1 = traverse<N, T, false>(A, o1, ld, (T)INFINITY, st, o64, d64);
+        T tf = (T)h1.t;
+        lit = h1.idx >= 0 && tf < dist;
+        if (lit) {
+            atomicAdd(probe_count, 1ull);
+            T rd[N], ro[N];
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                rd[k] = -ld[k];
+                ro[k] = o1[k] + rd[k] * ((T)2 * sc.ray_offset);
+                o64[k] = ro[k];
+                d64[k] = rd[k];
+            }
+            HitResult h2 = traverse<N, T, false>(A, ro, rd, (T)INFINITY, st, o64, d64);
+            lit = !(h2.idx >= 0 && (T)h2.t < tf);
+#pragma unroll
+            for (int k = 0; k < N; ++k) o1[k] = o1[k] + ld[k] * (tf + sc.ray_offset);
+            tmaxs = dist - tf;
+        }
+    }
+    if (!lit) return;
+    float w = (float)(sc.cos_coef * dln);
+    out.shadow = true;
+    if (SHAPES && sc.shape_count > 0) out.sshape = blocking_shape<N, T>(sc, o1, ld, (double)dist);
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        out.sorg[k] = o1[k];
+        out.sdir[k] = ld[k];
+    }
+    out.stmax = tmaxs;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out.contrib[c] = beta[c] * (scol[c] * (lc[c] * w));
+}
 
 // One path vertex: trace_path's shading, direct_diffuse_lighting's shadow-ray
 // set-up and diffuse_lighting's bounce (src/painter/painter.cpp:197-397), for
 // the ray (o, d) whose nearest hit is (idx, t).  With aov, out.av receives
 // the first-hit albedo and normal.
 // SHAPES: false compiles out the shapes beside the mesh (scenes without them).
+// MULTI: the shadow rays are left to the caller, one light after another.
 // SMOOTH: false compiles out the smooth-normal probes (scenes painted without
 // smooth normals never take them).
-template <int N, typename T, bool SMOOTH = true, bool SHAPES = true>
+template <int N, typename T, bool SMOOTH = true, bool SHAPES = true, bool MULTI = false>
 __device__ __forceinline__ void shade_one(const SceneK<T>& sc, const TreeArgs& A, const MeshShadeArgs& M, int depth, int64_t pixel,
                           int sample, const T* o, const T* d, int idx, T t, const float* beta, float level, int2* st,
                           unsigned long long* probe_count, bool aov, ShadeOut<N, T>& out)
@@ -267,6 +355,7 @@ __device__ __forceinline__ void shade_one(const SceneK<T>& sc, const TreeArgs& A
     out.next = false;
     out.shadow = false;
     out.sshape = -1;
+    out.direct = false;
     const bool diffuse_reflection = depth > 0;
     // ray_intersection over the objects (ray_intersection.h:20-80): the mesh
     // hit, or a nearer shape
@@ -354,77 +443,24 @@ __device__ __forceinline__ void shade_one(const SceneK<T>& sc, const TreeArgs& A
     float scol[3] = {diffuse * rgb[0], diffuse * rgb[1], diffuse * rgb[2]};
     float new_level = level * fmaxf(scol[0], fmaxf(scol[1], scol[2]));
     if (!(new_level >= 1e-4f)) return;
-    // direct_diffuse_lighting, painter.cpp:197-292 (one light)
-    if (sc.light_count > 0) {
-        T vl[N];
+    // direct_diffuse_lighting, painter.cpp:197-292: one light here; several
+    // lights are taken one after another by the caller from the hit in out
+    if (MULTI) {
+        out.direct = true;
+        out.mesh = mesh;
 #pragma unroll
-        for (int k = 0; k < N; ++k) vl[k] = sc.light_pos[0][k] - p[k];
-        T sq2 = dot<N, T>(vl, vl);
-        T scale;
-        if (sc.light_kind[0] == 1)
-            scale = (T)1;
-        else if constexpr (N == 3)
-            scale = sc.light_coef[0] / sq2;
-        else
-            scale = sc.light_coef[0] / pow(sq2, (T)(N - 1) / (T)2);
-        float lc[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) lc[c] = sc.light_rgb[0][c] * (float)scale;
-        T ld[N];
-#pragma unroll
-        for (int k = 0; k < N; ++k) ld[k] = vl[k];
-        normalize<N, T>(ld);
-        T dln = dot<N, T>(ld, sn);
-        bool lit = !(fmaxf(lc[0], fmaxf(lc[1], lc[2])) < 1e-4f) && dln > (T)0;
-        if (lit) {
-            T dist = sqrt(dot<N, T>(vl, vl));
-            T o1[N];
-#pragma unroll
-            for (int k = 0; k < N; ++k) o1[k] = p[k] + ld[k] * sc.ray_offset;
-            T tmaxs = dist;
-            if (SMOOTH && mesh && sc.smooth && dot<N, T>(ld, gn) < (T)0) {
-                // smooth-normal back face: forward and reverse probes (painter.cpp:238-284)
-                atomicAdd(probe_count, 1ull);
-                double o64[N], d64[N];
-#pragma unroll
-                for (int k = 0; k < N; ++k) {
-                    o64[k] = o1[k];
-                    d64[k] = ld[k];
-                }
-                HitResult h1 = traverse<N, T, false>(A, o1, ld, (T)INFINITY, st, o64, d64);
-                T tf = (T)h1.t;
-                lit = h1.idx >= 0 && tf < dist;
-                if (lit) {
-                    atomicAdd(probe_count, 1ull);
-                    T rd[N], ro[N];
-#pragma unroll
-                    for (int k = 0; k < N; ++k) {
-                        rd[k] = -ld[k];
-                        ro[k] = o1[k] + rd[k] * ((T)2 * sc.ray_offset);
-                        o64[k] = ro[k];
-                        d64[k] = rd[k];
-                    }
-                    HitResult h2 = traverse<N, T, false>(A, ro, rd, (T)INFINITY, st, o64, d64);
-                    lit = !(h2.idx >= 0 && (T)h2.t < tf);
-#pragma unroll
-                    for (int k = 0; k < N; ++k) o1[k] = o1[k] + ld[k] * (tf + sc.ray_offset);
-                    tmaxs = dist - tf;
-                }
-            }
-            if (lit) {
-                float w = (float)(sc.cos_coef * dln);
-                out.shadow = true;
-                if (SHAPES && sc.shape_count > 0) out.sshape = blocking_shape<N, T>(sc, o1, ld, (double)dist);
-#pragma unroll
-                for (int k = 0; k < N; ++k) {
-                    out.sorg[k] = o1[k];
-                    out.sdir[k] = ld[k];
-                }
-                out.stmax = tmaxs;
-#pragma unroll
-                for (int c = 0; c < 3; ++c) out.contrib[c] = beta[c] * (scol[c] * (lc[c] * w));
-            }
+        for (int k = 0; k < N; ++k) {
+            out.sp[k] = p[k];
+            out.sgn[k] = gn[k];
+            out.ssn[k] = sn[k];
         }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            out.sbeta[c] = beta[c];
+            out.sscol[c] = scol[c];
+        }
+    } else if (sc.light_count > 0) {
+        light_shadow_ray<N, T, SMOOTH, SHAPES>(sc, A, 0, p, gn, sn, mesh, beta, scol, st, probe_count, out);
     }
     // diffuse_lighting, painter.cpp:298-323
     if (depth < sc.max_depth) {

@@ -425,13 +425,18 @@ class Scene:
         bg.light_source = int(cfg.get("background_is_light", True))
         lum = 0.2126 * cfg["background_rgb"][0] + 0.7152 * cfg["background_rgb"][1] + 0.0722 * cfg["background_rgb"][2]
         bg.light_rgb[:] = [lum, lum, lum]
-        light = Light()
-        nlights = 0 if cfg.get("light_pos") is None else 1
-        if nlights:
-            light.kind = LIGHT_POINT
-            light.position[:n] = cfg["light_pos"]
-            light.rgb[:] = cfg["light_rgb"]
-            light.unit_intensity_distance = cfg["light_unit_distance"]
+        lights = cfg.get("lights")
+        if lights is None:
+            lights = [] if cfg.get("light_pos") is None else [
+                dict(kind=LIGHT_POINT, pos=cfg["light_pos"], rgb=cfg["light_rgb"],
+                     unit_distance=cfg["light_unit_distance"])]
+        nlights = len(lights)
+        light = (Light * max(1, nlights))()
+        for k, li in enumerate(lights):
+            light[k].kind = li.get("kind", LIGHT_POINT)
+            light[k].position[:n] = li["pos"]
+            light[k].rgb[:] = li["rgb"]
+            light[k].unit_intensity_distance = li.get("unit_distance", 1.0)
         pr = Projector()
         pr.kind = cfg.get("projector", PROJECTOR_PARALLEL)
         pr.camera_org[:n] = cfg["camera_org"]
@@ -442,7 +447,7 @@ class Scene:
         pr.units_per_pixel = cfg.get("units_per_pixel", 0.0)
         pr.view_angle_degrees = cfg.get("view_angle", 0.0)
         h = C.c_void_p()
-        ctx.check(L.nsp_scene_create(ctx.h, n, mesh.h, C.byref(obj), C.byref(light) if nlights else None, nlights,
+        ctx.check(L.nsp_scene_create(ctx.h, n, mesh.h, C.byref(obj), light if nlights else None, nlights,
                                      C.byref(pr), C.byref(bg), C.byref(h)))
         self.h = h
         if cfg.get("shapes"):

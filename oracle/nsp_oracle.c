@@ -1209,21 +1209,20 @@ static int nearest_object(const pctx* c, const double* org, const double* dir, d
 static void trace_path(const pctx* c, pstate* st, int level, float color_level, const double* org,
                        const double* dir, int diffuse_reflection, float* out);
 
-/* direct_diffuse_lighting, painter.cpp:197-292, one point light (PointLight::properties,
- * src/painter/lights/light_source.h:42-56) */
-static void direct_lighting(const pctx* c, pstate* st, const double* p, const double* gn, const double* sn,
-                            int mesh, float* out)
+/* One light of direct_diffuse_lighting (painter.cpp:205-289) with
+ * PointLight::properties or ConstantLight::properties
+ * (src/painter/lights/light_source.h:26-78); adds its term to out. */
+static void direct_one(const pctx* c, pstate* st, const double* p, const double* gn, const double* sn, int mesh,
+                       int kind, const double* pos, const float* rgb, double coef, float* out)
 {
     int n = c->m->n;
     const orc_scene* s = c->s;
-    out[0] = out[1] = out[2] = 0;
-    if (s->no_light) return;
     double vl[NM];
-    for (int i = 0; i < n; ++i) vl[i] = s->light_pos[i] - p[i];
+    for (int i = 0; i < n; ++i) vl[i] = pos[i] - p[i];
     double sq = dotn(vl, vl, n);
-    double scale = (n == 3) ? c->light_coef / sq : c->light_coef / pow(sq, (double)(n - 1) / 2);
+    double scale = kind == 1 ? 1.0 : (n == 3) ? coef / sq : coef / pow(sq, (double)(n - 1) / 2);
     float lc[3];
-    for (int k = 0; k < 3; ++k) lc[k] = s->light_rgb[k] * (float)scale;
+    for (int k = 0; k < 3; ++k) lc[k] = rgb[k] * (float)scale;
     if (maxf3(lc) < 1e-4f) return;
     double ldir[NM];
     memcpy(ldir, vl, sizeof ldir);
@@ -1257,6 +1256,20 @@ static void direct_lighting(const pctx* c, pstate* st, const double* p, const do
     }
     float w = (float)(c->coef * dln);
     for (int k = 0; k < 3; ++k) out[k] = out[k] + lc[k] * w;
+}
+
+/* direct_diffuse_lighting, painter.cpp:197-292: the lights in order */
+static void direct_lighting(const pctx* c, pstate* st, const double* p, const double* gn, const double* sn,
+                            int mesh, float* out)
+{
+    const orc_scene* s = c->s;
+    int n = c->m->n;
+    out[0] = out[1] = out[2] = 0;
+    if (!s->no_light)
+        direct_one(c, st, p, gn, sn, mesh, s->light_kind, s->light_pos, s->light_rgb, c->light_coef, out);
+    for (int l = 0; l < s->extra_light_count && l < ORC_MAX_LIGHTS - 1; ++l)
+        direct_one(c, st, p, gn, sn, mesh, s->extra_lights[l].kind, s->extra_lights[l].pos, s->extra_lights[l].rgb,
+                   pow(s->extra_lights[l].unit_distance, n - 1), out);
 }
 
 /* trace_path, painter.cpp:325-397 and diffuse_lighting, :298-323 */

@@ -37,6 +37,7 @@ extern "C" {
 
 #define ORC_NMAX 6
 #define ORC_MAX_SHAPES 16
+#define ORC_MAX_LIGHTS 8
 
 typedef struct orc_mesh orc_mesh;
 
@@ -125,6 +126,14 @@ typedef struct {
     } shapes[ORC_MAX_SHAPES];
     int no_light;             /* 1: no light source (light_* ignored) */
     int background_not_light; /* 1: the default surface is not a light source */
+    int light_kind;           /* of the first light: 0 PointLight, 1 ConstantLight */
+    int extra_light_count;    /* lights after the first, in order */
+    struct {
+        int kind;
+        double pos[ORC_NMAX];
+        float rgb[3];
+        double unit_distance;
+    } extra_lights[ORC_MAX_LIGHTS - 1];
 } orc_scene;
 
 /* Projector::ray for a screen point in pixel units (the `point` argument):

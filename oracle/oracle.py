@@ -21,6 +21,14 @@ class ShapeC(C.Structure):
                 ("rgb", C.c_float * 3), ("diffuse", C.c_float), ("light", C.c_int), ("light_rgb", C.c_float * 3)]
 
 
+MAX_LIGHTS = 8
+
+
+class LightC(C.Structure):
+    _fields_ = [("kind", C.c_int), ("pos", C.c_double * NMAX), ("rgb", C.c_float * 3),
+                ("unit_distance", C.c_double)]
+
+
 class Scene(C.Structure):
     _fields_ = [("n", C.c_int), ("cam_org", C.c_double * NMAX), ("cam_dir", C.c_double * NMAX),
                 ("axes", (C.c_double * NMAX) * (NMAX - 1)), ("units_per_pixel", C.c_double),
@@ -29,7 +37,8 @@ class Scene(C.Structure):
                 ("object_rgb", C.c_float * 3), ("diffuse", C.c_float), ("samples_per_pixel", C.c_int),
                 ("max_depth", C.c_int), ("smooth_normal", C.c_int), ("seed", C.c_uint64), ("pass_index", C.c_int),
                 ("projector", C.c_int), ("view_angle_degrees", C.c_double), ("shape_count", C.c_int),
-                ("shapes", ShapeC * MAX_SHAPES), ("no_light", C.c_int), ("background_not_light", C.c_int)]
+                ("shapes", ShapeC * MAX_SHAPES), ("no_light", C.c_int), ("background_not_light", C.c_int),
+                ("light_kind", C.c_int), ("extra_light_count", C.c_int), ("extra_lights", LightC * (MAX_LIGHTS - 1))]
 
 
 _lib = None
@@ -212,12 +221,24 @@ def make_scene(cfg, spp, max_depth, seed, smooth=False, pass_index=0):
     s.units_per_pixel = cfg.get("units_per_pixel", 0.0)
     s.projector = cfg.get("projector", 0)
     s.view_angle_degrees = cfg.get("view_angle", 0.0)
-    if cfg.get("light_pos") is None:
+    lights = cfg.get("lights")
+    if lights is None:
+        lights = [] if cfg.get("light_pos") is None else [
+            dict(kind=0, pos=cfg["light_pos"], rgb=cfg["light_rgb"], unit_distance=cfg["light_unit_distance"])]
+    if not lights:
         s.no_light = 1
     else:
-        s.light_pos[:n] = cfg["light_pos"]
-        s.light_rgb[:] = cfg["light_rgb"]
-        s.light_unit_distance = cfg["light_unit_distance"]
+        s.light_kind = lights[0].get("kind", 0)
+        s.light_pos[:n] = lights[0]["pos"]
+        s.light_rgb[:] = lights[0]["rgb"]
+        s.light_unit_distance = lights[0].get("unit_distance", 1.0)
+        s.extra_light_count = len(lights) - 1
+        for k, li in enumerate(lights[1:]):
+            e = s.extra_lights[k]
+            e.kind = li.get("kind", 0)
+            e.pos[:n] = li["pos"]
+            e.rgb[:] = li["rgb"]
+            e.unit_distance = li.get("unit_distance", 1.0)
     s.background_not_light = 0 if cfg.get("background_is_light", True) else 1
     shapes = cfg.get("shapes") or []
     if len(shapes) > MAX_SHAPES:

@@ -112,3 +112,24 @@ def test_shape_errors(ctx):
         ss.paint(4, 2, smooth=True)
     assert e.value.code == native.NSP_E_UNSUPPORTED
     ss.paint(4, 2, smooth=False)
+
+
+@pytest.mark.parametrize("n,precision", [(3, native.PRECISION_FP64), (4, native.PRECISION_FP64),
+                                         (4, native.PRECISION_FP32)])
+def test_several_lights_match_checker(ctx, n, precision):
+    # two point lights and a constant light (PointLight, ConstantLight,
+    # src/painter/lights/light_source.h:26-78): one shadow ray per light and bounce
+    g = {3: 8, 4: 4}[n]
+    v, f = native.meshgen_cube_sphere(n, g, True, 0.02, 0.0, 2)
+    cfg = configs.scene_dict(v, f, {3: 24, 4: 10}[n], tuple([2.0] * n))
+    cfg["lights"] = [dict(kind=native.LIGHT_POINT, pos=[2.0] * n, rgb=[1.0, 0.9, 0.7], unit_distance=2.5),
+                     dict(kind=native.LIGHT_POINT, pos=[-2.0] + [0.5] * (n - 1), rgb=[0.3, 0.4, 1.0],
+                          unit_distance=2.0),
+                     dict(kind=native.LIGHT_CONSTANT, pos=[0.0] * (n - 1) + [3.0], rgb=[0.2, 0.2, 0.2])]
+    print(f"lights n={n} precision={precision}", flush=True)
+    g_, o, rg, ro = render_both(ctx, v, f, cfg, 8, 3, precision, 31)
+    rmse = float(np.sqrt(np.mean((g_ - o) ** 2)))
+    print(f"rmse {rmse:.3e} rays gpu {rg} cpu {ro}", flush=True)
+    assert rmse < (1e-4 if precision == native.PRECISION_FP64 else 5e-3)
+    if precision == native.PRECISION_FP64:
+        assert abs(rg - ro) <= 0.001 * ro

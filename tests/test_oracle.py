@@ -314,3 +314,34 @@ def test_hyperplane_parallelotope_intersection_by_coordinates(n):
         else:
             misses += 1
     assert hits > 20 and misses > 20
+
+
+# ---------------------------------------------------------------------------
+# Several light sources (direct_diffuse_lighting's loop, painter.cpp:205-289)
+
+@pytest.mark.parametrize("n", [3, 4])
+def test_lights_add_linearly(n):
+    # without bounces a hit's colour is the sum of the lights' terms, so the render
+    # with two lights equals the two single-light renders minus the unlit one
+    from math_amd import configs
+    g = {3: 6, 4: 3}[n]
+    v, f = native.meshgen_cube_sphere(n, g, True, 0.02, 0.0, 5)
+    cfg = configs.scene_dict(v, f, 8 if n == 4 else 16, tuple([2.0] * n))
+    la = dict(kind=0, pos=[2.0] * n, rgb=[1.0, 0.8, 0.6], unit_distance=2.0)
+    lb = dict(kind=1, pos=[-1.5] + [1.0] * (n - 1), rgb=[0.2, 0.3, 0.9])
+    m = O.Mesh(v, f)
+    m.build_tree()
+
+    def render(lights):
+        c = dict(cfg)
+        c["lights"] = lights
+        img, rays = O.render(m, O.make_scene(c, 4, 0, 7), threads=4)
+        return img, rays
+    ab, rab = render([la, lb])
+    a, ra = render([la])
+    b, rb = render([lb])
+    z, rz = render([])
+    assert np.allclose(ab, a + b - z, rtol=1e-5, atol=1e-6)
+    # each light's shadow ray counts once: the ray counts add the same way
+    assert rab == ra + rb - rz
+    assert not np.allclose(a, z) and not np.allclose(b, z)
