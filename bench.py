@@ -124,6 +124,9 @@ def denoise_bench(args):
         log(f"training steps {c + k}: lr {lr:g}, loss {chunks[-1][-50:].mean():.6f} ({time.perf_counter() - t0:.1f} s)")
     losses = np.concatenate(chunks)
     train_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    alphas = DE.fit_blend(dn, X, Y)
+    log(f"blend weights per level {alphas} ({time.perf_counter() - t0:.1f} s)")
     del X, Y
     results = []
     for k in range(args.eval_scenes):
@@ -138,7 +141,7 @@ def denoise_bench(args):
         ref, ref_s = DE.render_mean(ctx, mesh, cfg, 512, seed=seed * 7, precision=native.PRECISION_FP64, passes=8,
                                     log=log)
         log(f"eval scene {seed}: FP64 4096-sample reference in {ref_s:.1f} s")
-        r = DE.time_to_rmse(ctx, mesh, cfg, ref, dn, seed=seed * 11)
+        r = DE.time_to_rmse(ctx, mesh, cfg, ref, dn, seed=seed * 11, alphas=alphas)
         r["seed"] = seed
         log(f"eval scene {seed}: {r['thresholds']}")
         log(f"eval scene {seed}: plain curve {[(c, round(t, 3), round(e, 4)) for c, t, e in r['plain_curve']]}")
@@ -151,6 +154,7 @@ def denoise_bench(args):
         "eval": [{"seed": r["seed"], "thresholds": r["thresholds"], "level1_rmse": r["denoised_rmse"],
                   "level1_input_rmse": r["input4_rmse"], "inference_s": r["denoiser_inference_s"]} for r in results],
         "curves": [{"seed": r["seed"], "plain": r["plain_curve"], "denoised": r["denoised_curve"]} for r in results],
+        "blend_weights": alphas,
         "train": {"variants": args.train_variants, "levels": list(DE.LEVELS), "window": args.window,
                   "target_spp": args.target_spp, "steps": args.train_steps, "seconds": round(train_s, 2),
                   "data_seconds": round(gen_s, 2), "loss_first": float(losses[:5].mean()),

@@ -135,6 +135,35 @@ def training_set(ctx, variants, screen=128, window=64, target_spp=1024, depth=5,
     return np.stack(xs), np.stack(ys)
 
 
+def fit_blend(denoiser, X, Y, levels=LEVELS, batch=4):
+    """Per progressive level, the least-squares weight alpha in [0, 1] of the
+    network output against the 4-spp input channels: minimises
+    |alpha * net + (1 - alpha) * r4 - target|^2 over the training volumes of
+    that level (X, Y as training_set returns them, variant-major, level-minor).
+    Returns {level: alpha}."""
+    L = len(levels)
+    num = np.zeros(L)
+    den = np.zeros(L)
+    for i0 in range(0, X.shape[0], batch):
+        xs = X[i0:i0 + batch]
+        ys = denoiser.forward(xs)
+        for j in range(xs.shape[0]):
+            li = (i0 + j) % L
+            r4 = xs[j, ..., RESIDUAL:RESIDUAL + 3].astype(np.float64)
+            dv = ys[j].astype(np.float64) - r4
+            num[li] += float(np.sum(dv * (Y[i0 + j].astype(np.float64) - r4)))
+            den[li] += float(np.sum(dv * dv))
+    return {lv: float(np.clip(num[k] / den[k], 0.0, 1.0)) if den[k] > 0 else 0.0 for k, lv in enumerate(levels)}
+
+
+def blend_weight(alphas, level):
+    """alpha at a progressive level: linear in log2(level) between the fitted
+    levels, the end values outside them."""
+    lv = sorted(alphas)
+    x = np.log2(np.asarray(lv, np.float64))
+    return float(np.interp(np.log2(max(level, 1)), x, [alphas[k] for k in lv]))
+
+
 def _mean_volume(scene, screen):
     s, c = scene.image()
     return to_volume(s / np.maximum(c, 1)[:, None], screen), c
@@ -149,7 +178,7 @@ def first_time(curve, threshold):
 
 
 def time_to_rmse(ctx, mesh, cfg, ref, denoiser, depth=5, thresholds=THRESHOLDS, max_passes=256, max_levels=160,
-                 seed=77):
+                 seed=77, alphas=None):
     """Error-versus-time curves of both methods against `ref`, and the time at
     which each first reaches each threshold.  Only GPU work is timed (paint
     calls and the network forward); image read-back and the RMSE evaluation
@@ -159,8 +188,10 @@ def time_to_rmse(ctx, mesh, cfg, ref, denoiser, depth=5, thresholds=THRESHOLDS, 
       plain     progressive 4-spp (8-sample) FP32 passes
       denoised  progressive levels of ProgressiveInputs (1 + 8 samples per
                 level); after each level the network maps the input to an
-                image, level 1 being the standard 1-spp + 4-spp input.  The
-                time at a level is its passes plus one inference.
+                image, level 1 being the standard 1-spp + 4-spp input, blended
+                with the 4-spp input by the level's fitted weight when alphas
+                (fit_blend) are given.  The time at a level is its passes plus
+                one inference.
     """
     screen = cfg["screen"]
     goal = min(thresholds)
@@ -182,6 +213,9 @@ def time_to_rmse(ctx, mesh, cfg, ref, denoiser, depth=5, thresholds=THRESHOLDS, 
         x = p.volume()
         t1 = time.perf_counter()
         y = denoiser.forward(x[None])[0]
+        if alphas is not None:
+            a = blend_weight(alphas, p.level)
+            y = a * y + (1 - a) * x[..., RESIDUAL:RESIDUAL + 3]
         dt = time.perf_counter() - t1
         e = rmse(y, ref)
         if k == 0:

@@ -200,3 +200,30 @@ def test_trained_denoiser_beats_its_noisy_input_on_heldout_render(ctx):
     print(f"held-out rmse: 4-spp input {e_in:.4f}, denoised {e_out:.4f}", flush=True)
     assert np.all(np.isfinite(y))
     assert e_out < 0.9 * e_in
+
+
+def test_blend_weights_closed_form():
+    # fit_blend's least-squares weight: a network output equal to the target gives 1, one that
+    # overshoots the target twice as far from the 4-spp input gives 0.5, the input itself 0
+    from math_amd import denoise_eval as DE
+    rng = np.random.default_rng(3)
+    levels = (1, 2, 4)
+    V = 2
+    X = rng.random((V * len(levels), 4, 4, 4, DE.IN_CHANNELS)).astype(np.float32)
+    Y = rng.random((V * len(levels), 4, 4, 4, 3)).astype(np.float32)
+    r4 = X[..., DE.RESIDUAL:DE.RESIDUAL + 3]
+
+    class Fake:
+        def __init__(self, out):
+            self.out, self.k = out, 0
+
+        def forward(self, x):
+            y = self.out[self.k:self.k + x.shape[0]]
+            self.k += x.shape[0]
+            return y
+    for out, expect in ((Y, 1.0), (r4 + 2 * (Y - r4), 0.5), (r4 - (Y - r4), 0.0)):
+        a = DE.fit_blend(Fake(out), X, Y, levels=levels, batch=4)
+        assert set(a) == set(levels)
+        assert all(abs(v - expect) < 1e-5 for v in a.values()), (a, expect)
+    a = {1: 1.0, 4: 0.0}
+    assert DE.blend_weight(a, 2) == 0.5 and DE.blend_weight(a, 1000) == 0.0 and DE.blend_weight(a, 1) == 1.0
