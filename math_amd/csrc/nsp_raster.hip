@@ -91,9 +91,30 @@ __global__ void k_raster_clear(const BatchDesc* __restrict__ desc, unsigned long
         keys[p] = ~0ull;
 }
 
+// x / e and x % e for 0 <= x < 2^22 and 1 <= e, with inv = 1 / e in FP32: the
+// product x * inv is within 2^-21 x / e < 2 / e of x / e (exact for e = 1), so
+// its truncation is within one of the quotient and one correction makes it exact.
+__device__ __forceinline__ int small_divmod(int x, int e, float inv, int* rem)
+{
+    int q = (int)((float)x * inv);
+    int r = x - q * e;
+    if (r < 0) {
+        --q;
+        r += e;
+    } else if (r >= e) {
+        ++q;
+        r -= e;
+    }
+    *rem = r;
+    return q;
+}
+
 // One wave per work unit (a chunk of one simplex's box): the record is loaded
 // once and tested against every sample of the chunk that belongs to the batch.
-template <int N>
+// SMALL (every box volume and unit item count below 2^22): lane items advance
+// the (pixel, sample) pair by 64 without a division, and the box coordinates
+// come from float quotients; otherwise 64-bit integer division.
+template <int N, bool SMALL>
 __global__ void __launch_bounds__(RASTER_BLOCK) k_raster(TreeArgs A, const int32_t* __restrict__ box,
                                                          const int2* __restrict__ unit,
                                                          const int64_t* __restrict__ ustart, RasterDims rd,
@@ -111,16 +132,21 @@ __global__ void __launch_bounds__(RASTER_BLOCK) k_raster(TreeArgs A, const int32
     const int64_t u1 = ustart[std::min<int64_t>(rd.last, xhi + 1)];
     const int lane = (int)(threadIdx.x & 63);
     const int64_t waves = (int64_t)gridDim.x * (RASTER_BLOCK / 64);
+    // item it = lane + 64 k is pixel b0 + it / S, sample it % S; 64 = q64 S + r64
+    const int q64 = 64 / S, r64 = 64 % S;
+    const int lane_q = lane / S, lane_r = lane % S;
     // consecutive units (neighbouring boxes) on one XCD, so their samples share its L2
     for (int64_t u = u0 + (int64_t)xcd_block() * (RASTER_BLOCK / 64) + (threadIdx.x >> 6); u < u1; u += waves) {
         const int2 un = unit[u];
         const int sid = un.x;
         int lo[D], ext[D];
+        float inv_ext[D];
         int64_t vol = 1;
 #pragma unroll
         for (int i = 0; i < D; ++i) {
             lo[i] = box[(int64_t)sid * 2 * D + i];
             ext[i] = box[(int64_t)sid * 2 * D + D + i] - lo[i] + 1;
+            inv_ext[i] = 1.0f / (float)ext[i];
             vol *= ext[i];
         }
         float rec[RS];
@@ -137,16 +163,39 @@ __global__ void __launch_bounds__(RASTER_BLOCK) k_raster(TreeArgs A, const int32
         auto get_sid = [&]() -> int { return sid; };
         const int64_t b0 = (int64_t)un.y * rd.ppu, b1 = std::min<int64_t>(vol, b0 + rd.ppu);
         const int64_t items = (b1 - b0) * S;
+        int next_q = lane_q, next_r = lane_r;
         for (int64_t it = lane; it < items; it += 64) {
-            const int64_t b = b0 + it / S;
-            const int s = (int)(it % S);
-            int64_t rem = b, stride = 1, pixel = 0;
+            int64_t pixel = 0;
+            int s;
+            if constexpr (SMALL) {
+                int rem = (int)b0 + next_q;
+                s = next_r;
+                next_r += r64;
+                next_q += q64;
+                if (next_r >= S) {
+                    next_r -= S;
+                    ++next_q;
+                }
+                int64_t stride = 1;
 #pragma unroll
-            for (int i = 0; i < D; ++i) {
-                const int64_t xi = rem % ext[i];
-                rem /= ext[i];
-                pixel += (lo[i] + xi) * stride;
-                stride *= rd.screen[i];
+                for (int i = 0; i < D - 1; ++i) {
+                    int xi;
+                    rem = small_divmod(rem, ext[i], inv_ext[i], &xi);
+                    pixel += (int64_t)(lo[i] + xi) * stride;
+                    stride *= rd.screen[i];
+                }
+                pixel += (int64_t)(lo[D - 1] + rem) * stride;
+            } else {
+                const int64_t b = b0 + it / S;
+                s = (int)(it % S);
+                int64_t rem = b, stride = 1;
+#pragma unroll
+                for (int i = 0; i < D; ++i) {
+                    const int64_t xi = rem % ext[i];
+                    rem /= ext[i];
+                    pixel += (lo[i] + xi) * stride;
+                    stride *= rd.screen[i];
+                }
             }
             if (pixel < pixel0 || pixel >= pixel0 + npix) continue;
             const int64_t path = (pixel - pixel0) * S + s;
@@ -207,11 +256,13 @@ void prepare_t(RasterPrep& rp, const MeshDev& m, const SceneK<double>& kd, int S
     // consecutive units cover neighbouring pixels)
     std::vector<std::tuple<int64_t, int32_t, int32_t>> us;
     int hmax = 0;
+    int64_t max_vol = 0;
     for (int64_t f = 0; f < F; ++f) {
         const int32_t* b = hb.data() + f * 2 * D;
         int64_t vol = 1;
         for (int i = 0; i < D; ++i) vol *= std::max(0, b[D + i] - b[i] + 1);
         if (vol == 0) continue;
+        max_vol = std::max(max_vol, vol);
         hmax = std::max(hmax, b[2 * D - 1] - b[D - 1]);
         const int64_t chunks = (vol + rp.ppu - 1) / rp.ppu;
         int64_t corner = 0;
@@ -228,6 +279,7 @@ void prepare_t(RasterPrep& rp, const MeshDev& m, const SceneK<double>& kd, int S
     for (int v = rp.last - 1; v >= 0; --v) start[v] = std::min(start[v], start[v + 1]);
     rp.hmax = hmax;
     rp.units = (int64_t)hu.size();
+    rp.small = max_vol < (int64_t(1) << 22) && rp.ppu * S < (int64_t(1) << 22);
     rp.unit.alloc(sizeof(int2) * std::max<size_t>(hu.size(), 1));
     rp.ustart.alloc(sizeof(int64_t) * start.size());
     if (!hu.empty()) NSP_HIP(hipMemcpyAsync(rp.unit.p, hu.data(), sizeof(int2) * hu.size(), hipMemcpyHostToDevice, s));
@@ -253,8 +305,12 @@ void launch_t(const RasterPrep& rp, const TreeArgs& A, const BatchDesc* desc, Ra
     rd.ppu = rp.ppu;
     for (int k = 0; k < N; ++k) rd.dir[k] = dir[k];
     const unsigned blocks = 2048;  // 8192 waves; each wave loops over the batch's units
-    hipLaunchKernelGGL(k_raster<N>, dim3(blocks), dim3(RASTER_BLOCK), 0, s, A, rp.box.as<int32_t>(),
-                       rp.unit.as<int2>(), rp.ustart.as<int64_t>(), rd, desc, rp.S, q, cap, keys);
+    if (rp.small)
+        hipLaunchKernelGGL((k_raster<N, true>), dim3(blocks), dim3(RASTER_BLOCK), 0, s, A, rp.box.as<int32_t>(),
+                           rp.unit.as<int2>(), rp.ustart.as<int64_t>(), rd, desc, rp.S, q, cap, keys);
+    else
+        hipLaunchKernelGGL((k_raster<N, false>), dim3(blocks), dim3(RASTER_BLOCK), 0, s, A, rp.box.as<int32_t>(),
+                           rp.unit.as<int2>(), rp.ustart.as<int64_t>(), rd, desc, rp.S, q, cap, keys);
     NSP_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_raster_resolve, dim3(gclear), dim3(256), 0, s, desc, q, keys);
     NSP_HIP(hipGetLastError());

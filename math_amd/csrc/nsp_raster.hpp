@@ -23,6 +23,7 @@ struct RasterPrep {
     int last = 0;         // screen extent of the last screen axis
     int64_t plane = 0;    // pixels per slice of the last screen axis
     int64_t units = 0;
+    bool small = false;   // every box volume and unit item count below 2^22 (float-quotient index split)
     int screen[NSP_MAX_DIM - 1] = {0};
     DevBuf box;           // int32 [F][2(n-1)]: first and last pixel per screen axis
     DevBuf unit;          // int2 (simplex, chunk of its box) in the order of the box's low corner, last axis major

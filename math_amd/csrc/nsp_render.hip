@@ -641,34 +641,111 @@ __global__ void __launch_bounds__(256) k_shadow_resolve(TreeArgs A, ShadowQueue<
 }
 
 // Sum of the samples of each pixel in sample order; Pixels::add_color_and_samples.
-__global__ void __launch_bounds__(256) k_accumulate(const BatchDesc* __restrict__ desc, int S,
-                                                   const float* __restrict__ L, const float* __restrict__ Ls,
-                                                   int64_t cap,
-                                                   float* __restrict__ pix_sum, int32_t* __restrict__ pix_cnt,
-                                                   int64_t pixels, const float* __restrict__ aov,
-                                                   float* __restrict__ pix_aov, int aovc)
+// A pixel's S samples are consecutive in each channel, so a block stages the
+// samples of up to ACC_PIX pixels through a 32 KiB LDS tile with coalesced loads,
+// and each thread then sums one pixel's row from LDS. Rows are padded to an odd
+// word stride, so the 64 lanes of a wave read 64 different banks.
+constexpr int ACC_PIX = 256;
+constexpr int ACC_WORDS = 8192;
+constexpr int ACC_MAX_S = 63;  // larger S: k_accumulate_wide
+
+__host__ __device__ constexpr int acc_pixels_per_block(int S)
+{
+    return ACC_WORDS / (S | 1) < ACC_PIX ? ACC_WORDS / (S | 1) : ACC_PIX;
+}
+
+__device__ __forceinline__ void acc_channel(float* tile, const float* __restrict__ a,
+                                            const float* __restrict__ b, int64_t first, int S, int R,
+                                            int P, float inv_s, float* out)
+{
+    // index k of the block's P * S <= ACC_WORDS samples -> row k / S, column k % S.
+    // The float quotient is within 2^-22 (k + 0.5) / S < 0.002 / S of (k + 0.5) / S,
+    // which is at least 0.5 / S from an integer, so the truncation is exact.
+    const int T = P * S;
+    __syncthreads();
+    for (int k = threadIdx.x; k < T; k += blockDim.x) {
+        const int q = (int)(((float)k + 0.5f) * inv_s);
+        const int r = k - q * S;
+        float v = a[first + k];
+        if (b) v += b[first + k];
+        tile[q * R + r] = v;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < P) {
+        float s = 0;
+        const float* row = tile + threadIdx.x * R;
+        for (int j = 0; j < S; ++j) s += row[j];
+        *out = s;
+    }
+}
+
+__global__ void __launch_bounds__(ACC_PIX) k_accumulate(const BatchDesc* __restrict__ desc, int S,
+                                                        const float* __restrict__ L,
+                                                        const float* __restrict__ Ls, int64_t cap,
+                                                        float* __restrict__ pix_sum,
+                                                        int32_t* __restrict__ pix_cnt, int64_t pixels,
+                                                        const float* __restrict__ aov,
+                                                        float* __restrict__ pix_aov, int aovc)
+{
+    __shared__ float tile[ACC_WORDS];
+    const int64_t npix = desc->paths / S;
+    const int64_t pixel0 = desc->pixel0;
+    const int R = S | 1;
+    const int PB = acc_pixels_per_block(S);
+    const float inv_s = 1.0f / (float)S;
+    for (int64_t j0 = (int64_t)blockIdx.x * PB; j0 < npix; j0 += (int64_t)gridDim.x * PB) {
+        const int P = (int)std::min<int64_t>(PB, npix - j0);
+        const int64_t first = j0 * S;
+        const int64_t px = pixel0 + j0 + threadIdx.x;
+        float s = 0;
+        for (int c = 0; c < 3; ++c) {
+            acc_channel(tile, L + c * cap, Ls + c * cap, first, S, R, P, inv_s, &s);
+            if ((int)threadIdx.x < P) pix_sum[c * pixels + px] += s;
+        }
+        if ((int)threadIdx.x < P) pix_cnt[px] += S;
+        for (int c = 0; aov && c < aovc; ++c) {
+            acc_channel(tile, aov + c * cap, nullptr, first, S, R, P, inv_s, &s);
+            if ((int)threadIdx.x < P) pix_aov[c * pixels + px] += s;
+        }
+    }
+}
+
+// The same sum for S > ACC_MAX_S: one wave per pixel, lanes strided over the
+// samples, then a butterfly sum across the wave.
+__device__ __forceinline__ float wave_sum(float v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+__global__ void __launch_bounds__(256) k_accumulate_wide(const BatchDesc* __restrict__ desc, int S,
+                                                         const float* __restrict__ L,
+                                                         const float* __restrict__ Ls, int64_t cap,
+                                                         float* __restrict__ pix_sum,
+                                                         int32_t* __restrict__ pix_cnt, int64_t pixels,
+                                                         const float* __restrict__ aov,
+                                                         float* __restrict__ pix_aov, int aovc)
 {
     const int64_t npix = desc->paths / S;
     const int64_t pixel0 = desc->pixel0;
-    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < npix; j += (int64_t)gridDim.x * blockDim.x) {
-        float s0 = 0, s1 = 0, s2 = 0;
-        for (int s = 0; s < S; ++s) {
-            int64_t p = j * S + s;
-            s0 += L[p] + Ls[p];
-            s1 += L[cap + p] + Ls[cap + p];
-            s2 += L[2 * cap + p] + Ls[2 * cap + p];
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t j = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); j < npix; j += waves) {
+        const int64_t first = j * S;
+        const int64_t px = pixel0 + j;
+        for (int c = 0; c < 3; ++c) {
+            float v = 0;
+            for (int s = lane; s < S; s += 64) v += L[c * cap + first + s] + Ls[c * cap + first + s];
+            v = wave_sum(v);
+            if (lane == 0) pix_sum[c * pixels + px] += v;
         }
-        int64_t px = pixel0 + j;
-        pix_sum[px] += s0;
-        pix_sum[pixels + px] += s1;
-        pix_sum[2 * pixels + px] += s2;
-        pix_cnt[px] += S;
-        if (aov) {
-            for (int c = 0; c < aovc; ++c) {
-                float a = 0;
-                for (int s = 0; s < S; ++s) a += aov[c * cap + j * S + s];
-                pix_aov[c * pixels + px] += a;
-            }
+        if (lane == 0) pix_cnt[px] += S;
+        for (int c = 0; aov && c < aovc; ++c) {
+            float v = 0;
+            for (int s = lane; s < S; s += 64) v += aov[c * cap + first + s];
+            v = wave_sum(v);
+            if (lane == 0) pix_aov[c * pixels + px] += v;
         }
     }
 }
@@ -875,8 +952,14 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
         NSP_HIP(hipStreamWaitEvent(s, b.join_ev, 0));
     }
     timed(4, [&] {
-        hipLaunchKernelGGL(k_accumulate, dim3(grid_for(cap / std::max(1, sc.S), 256, 4096)), dim3(256), 0, s, desc,
-                           sc.S, L, Ls, cap, rl.pix_sum, rl.pix_cnt, rl.pixels, aov, rl.pix_aov, aov_channels(N));
+        const int64_t max_pix = cap / std::max(1, sc.S);
+        if (sc.S <= ACC_MAX_S)
+            hipLaunchKernelGGL(k_accumulate, dim3(grid_for(max_pix, acc_pixels_per_block(sc.S), 8192)),
+                               dim3(ACC_PIX), 0, s, desc, sc.S, L, Ls, cap, rl.pix_sum, rl.pix_cnt, rl.pixels,
+                               aov, rl.pix_aov, aov_channels(N));
+        else
+            hipLaunchKernelGGL(k_accumulate_wide, dim3(grid_for(max_pix, 4, 8192)), dim3(256), 0, s, desc, sc.S,
+                               L, Ls, cap, rl.pix_sum, rl.pix_cnt, rl.pixels, aov, rl.pix_aov, aov_channels(N));
     });
     hipLaunchKernelGGL(k_batch_totals, dim3(1), dim3(64), 0, s, cnt, rl.max_depth + 1, totals, bidx);
     NSP_HIP(hipGetLastError());

@@ -170,6 +170,25 @@ def test_render_fp32_matches_checker(ctx, n):
     assert rmse < 5e-3
 
 
+@pytest.mark.parametrize("n,spp,edge", [(3, 36, 12), (3, 64, 12), (4, 125, 4)])
+def test_render_many_samples_per_pixel(ctx, n, spp, edge):
+    # per-pixel sums over 36 samples (LDS tile of fewer than 256 pixels), 64 and 125
+    # samples (one wave per pixel)
+    g = {3: 8, 4: 4}[n]
+    v, f, cfg, _, depth = small_scene(n, g, edge, spp=spp, depth=2)
+    gm = ctx.mesh(v, f)
+    sc = native.Scene(ctx, gm, cfg)
+    sc.paint(spp, depth, precision=native.PRECISION_FP64, seed=12)
+    img, cnt = sc.image()
+    om = O.Mesh(v, f)
+    om.build_tree()
+    ref, rays = O.render(om, O.make_scene(cfg, spp, depth, 12), threads=8)
+    assert np.all(cnt == spp)
+    rmse = float(np.sqrt(np.mean((img / cnt[:, None] - ref / cnt[:, None]) ** 2)))
+    print(f"n={n} spp={spp} rmse {rmse:.3e}", flush=True)
+    assert rmse < 1e-4
+
+
 @pytest.mark.parametrize("n,precision", [(3, native.PRECISION_FP64), (4, native.PRECISION_FP64),
                                          (4, native.PRECISION_FP32)])
 def test_render_smooth_normals_matches_checker(ctx, n, precision):
