@@ -219,6 +219,47 @@ def obj_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def hull_bench(args):
+    """Convex hull in 4 dimensions of 100,000 points in the unit ball, of the
+    same count in a hyperplane with one apex, and of 20,000 points on the
+    sphere (test_convex_hull_speed and sphere_mesh's workloads): exact
+    predicates, the first 4096 points on the host, the conflict graph of the rest
+    in one GPU pass, then the host again; beside it the same library with every
+    point on the host, and the oracle (exact Python) on a bounded sample."""
+    import numpy as np
+
+    from math_amd import configs, native
+    from oracle import hull_ref
+
+    ctx = native.Context(0)
+    native.convex_hull(ctx, configs.hull_points(4, 2000, 1))  # warm-up (kernel load)
+    out = {"metric": "convex hull in 4 dimensions", "unit": "ms", "workloads": []}
+    for name, count, sphere, zero in (("ball", 100000, False, False), ("hyperplane+apex", 100000, False, True),
+                                      ("sphere", 20000, True, False)):
+        p = configs.hull_points(4, count, 80, on_sphere=sphere, zero=zero)
+        t0 = time.perf_counter()
+        a = native.convex_hull(ctx, p)
+        ta = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        b = native.convex_hull(ctx, p, count)
+        tb = time.perf_counter() - t0
+        same = set(map(tuple, a.vertices.tolist())) == set(map(tuple, b.vertices.tolist()))
+        w = {"workload": f"{name}, {count} points", "facets": len(a.vertices), "ms": ta * 1e3,
+             "ms_split": {"host_first_4096": a.ms[0], "gpu_conflict_pass": a.ms[1], "host_rest": a.ms[2]},
+             "dropped_by_gpu_pass": a.dropped, "host_only_ms": tb * 1e3, "same_facets_as_host_only": same}
+        log(f"hull {name}: {ta * 1e3:.0f} ms ({a.ms[0]:.0f} + GPU {a.ms[1]:.1f} + {a.ms[2]:.0f}), dropped "
+            f"{a.dropped}, {len(a.vertices)} facets; host only {tb * 1e3:.0f} ms; same {same}")
+        out["workloads"].append(w)
+    # the oracle on a bounded sample of the ball workload
+    p = configs.hull_points(4, args.hull_sample, 80)
+    t0 = time.perf_counter()
+    hull_ref.convex_hull(p, np.arange(len(p)))
+    dt = time.perf_counter() - t0
+    out["cpu_baseline"] = {"value": dt * 1e3, "unit": "ms", "cores": 1, "kind": "port",
+                           "sample": f"oracle (exact Python integers) on {args.hull_sample} points of the ball workload"}
+    print(json.dumps(out), flush=True)
+
+
 def batch_bench(args):
     """C4: many scenes of uneven size and dimension, 4 spp, FP32: one batched
     launch sequence vs one launch sequence per scene."""
@@ -303,6 +344,8 @@ def main():
     ap.add_argument("--target-spp", type=int, default=2048)
     ap.add_argument("--eval-window", type=int, default=0, help="evaluate on a centred window (0: full 128^3)")
     ap.add_argument("--obj", action="store_true", help=".obj4 ingest of the C1 mesh (reader MB/s)")
+    ap.add_argument("--hull", action="store_true", help="convex hulls of 4-dimensional point sets (ms)")
+    ap.add_argument("--hull-sample", type=int, default=3000, help="with --hull: points the oracle takes")
     ap.add_argument("--no-gpu-mesh", action="store_true", help="with --obj: skip the GPU mesh build")
     ap.add_argument("--obj-sample-facets", type=int, default=60000, help="with --obj: facets read by the oracle")
     ap.add_argument("--batch-scenes", action="store_true",
@@ -314,6 +357,8 @@ def main():
         return denoise_bench(args)
     if args.batch_scenes:
         return batch_bench(args)
+    if args.hull:
+        return hull_bench(args)
     if args.obj:
         return obj_bench(args)
     if args.copy_calibration:

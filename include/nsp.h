@@ -111,6 +111,49 @@ int nsp_mesh_create_obj(nsp_context* ctx, const nsp_obj* obj, const double* vert
 int nsp_obj_write(const char* path, int n, const float* vertices, int64_t V, const int32_t* facets, int64_t F,
                   const float* normals, int64_t VN, const int32_t* facet_normals, const char* comment);
 
+/* Convex hull of `count` float points in n = 2..6 dimensions: replaces
+ * compute_convex_hull<N> (src/geometry/core/convex_hull.h:70-72,
+ * convex_hull.cpp:877-886).  The points are shuffled (std::shuffle with
+ * std::mt19937_64(count)), scaled to integers in [0, 2^30 - 1] keeping
+ * proportions (llround((p - min) * (2^30 - 1) / max extent)), duplicates
+ * dropped (the first in shuffled order is kept), and the hull is built
+ * incrementally with exact predicates.  Each facet has n vertices (input point
+ * indices) and a unit outward normal (ConvexHullFacet<N>::vertices / ortho).
+ * Coplanar points are not hull vertices unless needed, and a non-simplicial
+ * face comes out triangulated as the insertion order makes it.  The first
+ * `first_points` points (0: 4096) are inserted on the host; the GPU then
+ * computes the conflicts of all later points with that hull in one pass and
+ * drops the points strictly inside it.  Errors (NSP_E_INVALID): fewer than
+ * n + 1 distinct points, all points equal, no n-simplex among them, non-finite
+ * coordinates; NSP_E_UNSUPPORTED for n outside 2..6. */
+typedef struct nsp_hull nsp_hull;
+int nsp_convex_hull(nsp_context* ctx, int n, const float* points, int64_t count, int64_t first_points,
+                    nsp_hull** out);
+/* Delaunay simplices of points in n = 2..5 dimensions: replaces
+ * compute_delaunay<N> (convex_hull.h:67-69, convex_hull.cpp:864-875): the
+ * points scaled to integers in [0, 2^24 - 1] as above, lifted to the
+ * paraboloid x_{n+1} = |x|^2, and the facets of the lifted hull whose outward
+ * normal has a negative last coordinate kept.  Each simplex has n + 1 vertices
+ * and n + 1 unit normals: normal r is that of the face without vertex r,
+ * pointing away from vertex r (DelaunaySimplex<N>::ortho).  nsp_hull_copy's
+ * `points` receives the integer coordinates the simplices were computed on
+ * (zero for dropped duplicates), as compute_delaunay's `points` output. */
+int nsp_delaunay(nsp_context* ctx, int n, const float* points, int64_t count, int64_t first_points,
+                 nsp_hull** out);
+/* Sizes: facets (hull) or simplices (Delaunay), vertices and unit normals per
+ * facet, and the number of distinct points; any pointer may be NULL. */
+int nsp_hull_info(const nsp_hull* h, int* n, int64_t* facets, int* vertices_per_facet, int* orthos_per_facet,
+                  int64_t* unique_points);
+/* vertices [facets][vertices_per_facet]; orthos [facets][orthos_per_facet][n];
+ * points [count][n] (Delaunay only); order [unique_points]: the input index of
+ * each distinct point in insertion order.  Any pointer may be NULL. */
+int nsp_hull_copy(const nsp_hull* h, int32_t* vertices, double* orthos, double* points, int32_t* order);
+/* Points the GPU pass dropped, facets created in total, and the times in ms of
+ * the host insertions before the GPU pass, the GPU pass, the host insertions
+ * after it (ms[3]). */
+int nsp_hull_stats(const nsp_hull* h, int64_t* dropped, int64_t* facets_created, double* ms);
+void nsp_hull_destroy(nsp_hull* h);
+
 typedef struct {
     int n;
     int64_t facets;

@@ -30,6 +30,7 @@ SOURCES = {
     "nsp_render.hip": [],
     "nsp_denoise.hip": [],
     "nsp_raster.hip": [],
+    "nsp_hull.hip": [],
     "nsp_api.cpp": ["-x", "hip"],
     "nsp_meshgen.cpp": ["-x", "hip"],
     "nsp_obj.cpp": ["-ffp-contract=off"],   # float operations as written (file values)
@@ -72,18 +73,21 @@ def build_oracle(verbose=False):
 
 
 def build_cpp_tests(verbose=False):
-    """The C++17 drop-in painter test (tests/cpp), compiled by the host compiler
+    """The C++17 drop-in tests (tests/cpp), compiled by the host compiler
     against include/nsp.h and libnsp.so."""
-    src = os.path.join(ROOT, "tests", "cpp", "test_painter_api.cpp")
-    exe = os.path.join(ROOT, "tests", "cpp", "test_painter_api")
-    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", CSRC, src, "-o", exe,
-           "-L", HERE, "-lnsp", "-Wl,-rpath,$ORIGIN/../../math_amd"]
-    if verbose:
-        print(" ".join(cmd))
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"C++ test build failed\n{r.stdout}\n{r.stderr}")
-    return exe
+    exes = []
+    for name in ("test_painter_api", "test_hull_order"):
+        src = os.path.join(ROOT, "tests", "cpp", name + ".cpp")
+        exe = os.path.join(ROOT, "tests", "cpp", name)
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", CSRC, src, "-o", exe,
+               "-L", HERE, "-lnsp", "-Wl,-rpath,$ORIGIN/../../math_amd"]
+        if verbose:
+            print(" ".join(cmd))
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"C++ test build failed\n{r.stdout}\n{r.stderr}")
+        exes.append(exe)
+    return exes[0]
 
 
 if __name__ == "__main__":

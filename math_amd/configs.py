@@ -159,6 +159,38 @@ WORKLOADS = {
 }
 
 
+def hull_points(n, count, seed, on_sphere=False, zero=False):
+    """Points for the convex hull: uniform in the unit n-ball, or on its sphere,
+    as generate_random_data (geometry/test/test_convex_hull.cpp:41-73) draws
+    them (here with numpy's generator); with `zero`, the last coordinate is 0
+    except for one apex point at x_n = 1."""
+    rng = np.random.default_rng(seed)
+    out = np.zeros((0, n))
+    k = n - 1 if zero else n
+    while len(out) < count:
+        v = np.zeros((2 * (count - len(out)) + 16, n))
+        v[:, :k] = rng.uniform(-1, 1, (len(v), k))
+        ln = np.linalg.norm(v, axis=1)
+        keep = (ln <= 1) & (ln > 0)
+        v, ln = v[keep], ln[keep]
+        if on_sphere:
+            v = v / ln[:, None]
+        out = np.vstack([out, v])
+    p = out[:count].astype(np.float32)
+    if zero:
+        p[-1, n - 1] = 1
+    return p
+
+
+def sphere_hull_mesh(ctx, n, count, seed=0):
+    """simplex_mesh_of_sphere (painter/shapes/test/sphere_mesh.cpp:65-105): the
+    convex hull of `count` random points on the unit (n-1)-sphere, as
+    (vertices, facets) for a mesh."""
+    p = hull_points(n, count, seed, on_sphere=True)
+    h = native.convex_hull(ctx, p)
+    return p, h.vertices.astype(np.int32)
+
+
 def make_mesh(gen, args):
     if gen == "torus":
         return native.meshgen_torus(**args)

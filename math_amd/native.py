@@ -124,7 +124,8 @@ EXPORTS = [
     "nsp_denoiser_train", "nsp_conv3d_bf16", "nsp_batch_create", "nsp_batch_destroy", "nsp_batch_paint",
     "nsp_batch_statistics", "nsp_batch_set_lanes", "nsp_batch_get_lanes", "nsp_batch_autotune", "nsp_autotune", "nsp_mesh_set_record_layout", "nsp_mesh_get_record_layout",
     "nsp_obj_read", "nsp_obj_destroy", "nsp_obj_last_error", "nsp_obj_info", "nsp_obj_copy", "nsp_mesh_create_obj",
-    "nsp_obj_write", "nsp_scene_set_shapes",
+    "nsp_obj_write", "nsp_scene_set_shapes", "nsp_convex_hull", "nsp_delaunay", "nsp_hull_info", "nsp_hull_copy",
+    "nsp_hull_stats", "nsp_hull_destroy",
 ]
 
 _lib = None
@@ -186,6 +187,13 @@ def lib():
         L.nsp_obj_copy.argtypes = [P, fp, ip, fp, ip]
         L.nsp_mesh_create_obj.argtypes = [P, P, dp, C.POINTER(P)]
         L.nsp_obj_write.argtypes = [C.c_char_p, C.c_int, fp, C.c_int64, ip, C.c_int64, fp, C.c_int64, ip, C.c_char_p]
+        L.nsp_convex_hull.argtypes = [P, C.c_int, fp, C.c_int64, C.c_int64, C.POINTER(P)]
+        L.nsp_delaunay.argtypes = [P, C.c_int, fp, C.c_int64, C.c_int64, C.POINTER(P)]
+        L.nsp_hull_info.argtypes = [P, C.POINTER(C.c_int), i64p, C.POINTER(C.c_int), C.POINTER(C.c_int), i64p]
+        L.nsp_hull_copy.argtypes = [P, ip, dp, dp, ip]
+        L.nsp_hull_stats.argtypes = [P, i64p, i64p, dp]
+        L.nsp_hull_destroy.argtypes = [P]
+        L.nsp_hull_destroy.restype = None
         _lib = L
     return _lib
 
@@ -281,6 +289,59 @@ def write_obj(path, vertices, facets, normals=None, facet_normals=None, comment=
                          None if comment is None else comment.encode())
     if rc:
         raise NspError(rc, L.nsp_obj_last_error().decode())
+
+
+# ---------------------------------------------------------------------------
+# Convex hull and Delaunay simplices (exact integer predicates, GPU conflict pass)
+
+class Hull:
+    """Result of convex_hull / delaunay.  vertices [F, k] int32 (input point
+    indices; k = n for a hull, n + 1 for Delaunay), orthos [F, n] (hull: unit
+    outward normals) or [F, n + 1, n] (Delaunay: normal r of the face without
+    vertex r, away from it), order [unique] (input index of each distinct point
+    in insertion order), points [count, n] (Delaunay: the integer coordinates
+    used), dropped (points the GPU pass found inside), facets_created, ms (host
+    before, GPU pass, host after)."""
+
+
+def _hull(ctx, points, first, delaunay):
+    L = ctx._L
+    p = np.ascontiguousarray(points, np.float32)
+    if p.ndim != 2:
+        raise ValueError("points must be [count, n]")
+    h = C.c_void_p()
+    fn = L.nsp_delaunay if delaunay else L.nsp_convex_hull
+    ctx.check(fn(ctx.h, p.shape[1], _ptr(p, C.c_float), p.shape[0], int(first), C.byref(h)))
+    try:
+        n, per, orth = C.c_int(), C.c_int(), C.c_int()
+        F, U = C.c_int64(), C.c_int64()
+        L.nsp_hull_info(h, C.byref(n), C.byref(F), C.byref(per), C.byref(orth), C.byref(U))
+        r = Hull()
+        r.n = n.value
+        r.vertices = np.zeros((F.value, per.value), np.int32)
+        shape = (F.value, orth.value, n.value) if delaunay else (F.value, n.value)
+        r.orthos = np.zeros(shape, np.float64)
+        r.order = np.zeros(U.value, np.int32)
+        r.points = np.zeros((p.shape[0], n.value), np.float64) if delaunay else None
+        L.nsp_hull_copy(h, _ptr(r.vertices, C.c_int32), _ptr(r.orthos, C.c_double), _ptr(r.points, C.c_double),
+                        _ptr(r.order, C.c_int32))
+        d, fc = C.c_int64(), C.c_int64()
+        ms = (C.c_double * 3)()
+        L.nsp_hull_stats(h, C.byref(d), C.byref(fc), ms)
+        r.dropped, r.facets_created, r.ms = d.value, fc.value, tuple(ms)
+        return r
+    finally:
+        L.nsp_hull_destroy(h)
+
+
+def convex_hull(ctx, points, first=0):
+    """compute_convex_hull (geometry/core/convex_hull.h:70-72) on ctx's GPU."""
+    return _hull(ctx, points, first, False)
+
+
+def delaunay(ctx, points, first=0):
+    """compute_delaunay (geometry/core/convex_hull.h:67-69) on ctx's GPU."""
+    return _hull(ctx, points, first, True)
 
 
 # ---------------------------------------------------------------------------
