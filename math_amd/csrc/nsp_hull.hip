@@ -150,41 +150,51 @@ struct FacetDev {
     int64_t base[M];
 };
 
+// One wave per point, its lanes strided over the facets: a hull of a few
+// thousand points has tens of thousands of facets, so the points alone would
+// not fill the GPU.
 template <int M, typename C>
 __global__ void __launch_bounds__(HULL_BLOCK) k_hull_count(const int64_t* __restrict__ X, const int32_t* __restrict__ pts,
                                                            int64_t npts, const FacetDev<M, C>* __restrict__ F,
                                                            int64_t nf, int32_t* __restrict__ count)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= npts) return;
+    const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = (int)(threadIdx.x & 63);
+    if (w >= npts) return;
     int64_t x[M];
 #pragma unroll
-    for (int j = 0; j < M; ++j) x[j] = X[(int64_t)pts[i] * M + j];
-    int32_t c = 0;
-    for (int64_t f = 0; f < nf; ++f) {
-        const C v = visible_value<M, C>(F[f].ortho, F[f].base, x);
-        c += sign_of(v) > 0;
-    }
-    count[i] = c;
+    for (int j = 0; j < M; ++j) x[j] = X[(int64_t)pts[w] * M + j];
+    int c = 0;
+    for (int64_t f = lane; f < nf; f += 64) c += sign_of(visible_value<M, C>(F[f].ortho, F[f].base, x)) > 0;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) c += __shfl_xor(c, m, 64);
+    if (lane == 0) count[w] = c;
 }
 
+// The same tests again, each wave writing its point's facets in facet order
+// (ballot and prefix count per 64 facets).
 template <int M, typename C>
 __global__ void __launch_bounds__(HULL_BLOCK) k_hull_fill(const int64_t* __restrict__ X, const int32_t* __restrict__ pts,
                                                           int64_t npts, const FacetDev<M, C>* __restrict__ F, int64_t nf,
                                                           const int64_t* __restrict__ offset,
                                                           int32_t* __restrict__ lists)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= npts) return;
-    const int64_t o0 = offset[i], o1 = offset[i + 1];
-    if (o0 == o1) return;
+    const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = (int)(threadIdx.x & 63);
+    if (w >= npts) return;
+    int64_t o = offset[w];
+    const int64_t o1 = offset[w + 1];
+    if (o == o1) return;
     int64_t x[M];
 #pragma unroll
-    for (int j = 0; j < M; ++j) x[j] = X[(int64_t)pts[i] * M + j];
-    int64_t o = o0;
-    for (int64_t f = 0; f < nf && o < o1; ++f) {
-        const C v = visible_value<M, C>(F[f].ortho, F[f].base, x);
-        if (sign_of(v) > 0) lists[o++] = (int32_t)f;
+    for (int j = 0; j < M; ++j) x[j] = X[(int64_t)pts[w] * M + j];
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    for (int64_t f0 = 0; f0 < nf && o < o1; f0 += 64) {
+        const int64_t f = f0 + lane;
+        const bool vis = f < nf && sign_of(visible_value<M, C>(F[f].ortho, F[f].base, x)) > 0;
+        const unsigned long long m = __ballot(vis);
+        if (vis) lists[o + __popcll(m & below)] = (int32_t)f;
+        o += __popcll(m);
     }
 }
 
@@ -198,6 +208,7 @@ struct Builder {
         int v[M];
         int link[M];  // link[r]: the facet across the ridge without v[r]
         C ortho[M];   // outward
+        double od[M]; // ortho rounded to double (the filter of visible_sign)
         std::vector<int> conf;
         bool alive = true;
         bool seen = false;
@@ -219,25 +230,42 @@ struct Builder {
         return visible_value<M, C>(f.ortho, &X[(int64_t)f.v[0] * M], &X[(int64_t)q * M]);
     }
 
+    // Sign of visible(f, q).  The double dot product of the rounded normal with
+    // the (exactly converted) difference is within (M + 3) 2^-53 sum|terms| of
+    // the exact value (normal rounding 3 ulp at most, products 1, sums M - 1), so
+    // a value beyond 16 2^-53 sum|terms| has the exact sign; otherwise the
+    // integer test decides.
+    int visible_sign(const Facet& f, int q) const
+    {
+        const int64_t* b = &X[(int64_t)f.v[0] * M];
+        const int64_t* x = &X[(int64_t)q * M];
+        double s = 0, a = 0;
+        for (int j = 0; j < M; ++j) {
+            const double t = f.od[j] * (double)(x[j] - b[j]);
+            s += t;
+            a += std::fabs(t);
+        }
+        if (std::fabs(s) > a * (16.0 / 9007199254740992.0)) return s > 0 ? 1 : -1;
+        return sign_of(visible(f, q));
+    }
+
     // the facet's normal, pointing away from `inner` (FacetInteger, geometry/core/facet.h:178-207)
     void orient(Facet& f, int inner, const Facet* horizon)
     {
         ortho_of<M, C>(X.data(), f.v, f.ortho);
         const int s = sign_of(visible(f, inner));
-        if (s < 0) return;
-        if (s > 0) {
-            for (int j = 0; j < M; ++j) f.ortho[j] = C(0) - f.ortho[j];
-            return;
+        bool flip = s > 0;
+        if (s == 0) {
+            if (!horizon) raise(NSP_E_INVALID, "convex hull: degenerate initial simplex");
+            // coplanar with the horizon facet: the same direction
+            for (int j = 0; j < M; ++j) {
+                const int a = sign_of(f.ortho[j]), b = sign_of(horizon->ortho[j]);
+                if ((a > 0 && b < 0) || (a < 0 && b > 0)) flip = true;
+            }
         }
-        if (!horizon) raise(NSP_E_INVALID, "convex hull: degenerate initial simplex");
-        // coplanar with the horizon facet: the same direction
-        bool opposite = false;
-        for (int j = 0; j < M; ++j) {
-            const int a = sign_of(f.ortho[j]), b = sign_of(horizon->ortho[j]);
-            if ((a > 0 && b < 0) || (a < 0 && b > 0)) opposite = true;
-        }
-        if (opposite)
+        if (flip)
             for (int j = 0; j < M; ++j) f.ortho[j] = C(0) - f.ortho[j];
+        for (int j = 0; j < M; ++j) f.od[j] = to_double(f.ortho[j]);
     }
 
     // pair up the ridges of the listed facets (all ridges, or those without `apex`)
@@ -316,7 +344,7 @@ struct Builder {
         for (int v : s) enabled[v] = 0;
         for (int id : ids)
             for (int q = 0; q < first; ++q)
-                if (enabled[q] && sign_of(visible(F[id], q)) > 0) {
+                if (enabled[q] && visible_sign(F[id], q) > 0) {
                     F[id].conf.push_back(q);
                     pconf[q].push_back(id);
                 }
@@ -347,10 +375,10 @@ struct Builder {
                 ++stamp;
                 for (int q : F[id].conf) {
                     mark[q] = stamp;
-                    if (q != p && sign_of(visible(nf, q)) > 0) nf.conf.push_back(q);
+                    if (q != p && visible_sign(nf, q) > 0) nf.conf.push_back(q);
                 }
                 for (int q : F[g].conf)
-                    if (mark[q] != stamp && q != p && sign_of(visible(nf, q)) > 0) nf.conf.push_back(q);
+                    if (mark[q] != stamp && q != p && visible_sign(nf, q) > 0) nf.conf.push_back(q);
                 F.push_back(std::move(nf));
                 const int nid = (int)F.size() - 1;
                 F[nid].link[r] = g;
@@ -397,7 +425,7 @@ struct Builder {
         NSP_HIP(hipMemcpyAsync(dX.p, X.data(), sizeof(int64_t) * X.size(), hipMemcpyHostToDevice, s));
         NSP_HIP(hipMemcpyAsync(dP.p, pts.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, s));
         NSP_HIP(hipMemcpyAsync(dF.p, fd.data(), sizeof(FacetDev<M, C>) * nf, hipMemcpyHostToDevice, s));
-        const unsigned grid = (unsigned)((np + HULL_BLOCK - 1) / HULL_BLOCK);
+        const unsigned grid = (unsigned)((np * 64 + HULL_BLOCK - 1) / HULL_BLOCK);
         hipLaunchKernelGGL((k_hull_count<M, C>), dim3(grid), dim3(HULL_BLOCK), 0, s, dX.as<int64_t>(),
                            dP.as<int32_t>(), np, dF.as<FacetDev<M, C>>(), nf, dC.as<int32_t>());
         NSP_HIP(hipGetLastError());
