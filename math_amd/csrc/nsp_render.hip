@@ -795,6 +795,20 @@ void alloc_render_buffers(RenderBuffers& b, int n, int precision, int64_t cap, i
     b.trav_counters.alloc(sizeof(unsigned long long) * 8);
 }
 
+int64_t render_bytes_per_path(int n, int precision, int lights, bool aov, bool raster)
+{
+    // the per-path sizes alloc_render_buffers (and the raster keys, the AOV
+    // planes) allocate
+    const int64_t ts = precision ? sizeof(double) : sizeof(float);
+    const int64_t L = std::max(1, lights);
+    const int64_t queue = 2 * ts * n + 4 + 12 + 4 + 4 + ts;
+    const int64_t shadow = 2 * ts * n + ts + 4 + 12 + 4;
+    int64_t b = 2 * queue + 2 * shadow * L + 24 + 4 * L + 4 + 4 * RES_STRIDE * (1 + L);
+    if (raster) b += 8;
+    if (aov) b += 4 * aov_channels(n);
+    return b;
+}
+
 RenderBuffers::~RenderBuffers()
 {
     for (hipEvent_t e : {fork_ev, join_ev, done_ev[0], done_ev[1]})

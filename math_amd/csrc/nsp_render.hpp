@@ -212,6 +212,8 @@ void render_batch(const RenderLaunch& rl, hipStream_t s);
 void pass_reset(RenderBuffers& b, bool counters, int* pass_dev, int pass, hipStream_t s);
 // Clears a scene's own pass totals (3 u64) and batch index.
 void pass_reset_state(unsigned long long* totals, long long* batch_idx, int* pass_dev, int pass, hipStream_t s);
+// Device bytes per path of one wavefront batch's workspace.
+int64_t render_bytes_per_path(int n, int precision, int lights, bool aov, bool raster);
 void alloc_render_buffers(RenderBuffers& b, int n, int precision, int64_t cap, int max_depth, int lights = 1);
 // FP64 records of shapes (org and edge vectors as given) into out (device), on stream s.
 void build_shape_records(int n, const int* kind, const double* org, const double* edges, int count, ShapeRec* out,
