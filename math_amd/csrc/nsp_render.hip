@@ -34,10 +34,18 @@ static unsigned grid_for(int64_t n, int block = TRACE_BLOCK, int64_t max_blocks 
 }
 
 // Share (per cent) of a full persistent grid given to the shadow walks when they
-// run on the second stream.
-#ifndef NSP_SIDE_GRID_PCT
-#define NSP_SIDE_GRID_PCT 75  // C1: 832 Mrays/s at 75 %, 804 at 100 %, 788 at 50 %
+// run on the second stream.  Measured with one batch per pass: n = 4 (C1) 1312
+// Mrays/s at 75 %, 1270 at 100 %, 1157 at 50 %; n = 5 (C2) 276 at 100 %, 258 at
+// 75 %, 229 at 50 %.  -DNSP_SIDE_GRID_PCT=P sets one share for every dimension.
+template <int N>
+constexpr int side_grid_pct()
+{
+#ifdef NSP_SIDE_GRID_PCT
+    return NSP_SIDE_GRID_PCT;
+#else
+    return N >= 5 ? 100 : 75;
 #endif
+}
 
 // Blocks of a persistent launch: every CU filled to the kernel's occupancy
 // (computed once per kernel), rounded to a multiple of 8 for the XCD remap.
@@ -376,7 +384,10 @@ __device__ __forceinline__ void push_pending(unsigned long long* res_count, int3
 // from its own XCD's slice, so neighbouring rays share that XCD's L2, and
 // moves on to the next slices once its own is empty.  Every wave leaves the
 // loop after all 8 slices are empty and its own rays are done.
-constexpr int REFILL_LANES = 32;
+#ifndef NSP_REFILL_LANES
+#define NSP_REFILL_LANES 32
+#endif
+constexpr int REFILL_LANES = NSP_REFILL_LANES;
 constexpr int XCDS = 8;
 
 struct RayPool {
@@ -881,7 +892,7 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
     unsigned gs = aos ? persistent_grid(k_shadow<N, T, 1>) : persistent_grid(k_shadow<N, T, 0>);
     // beside the closest-hit walks, the persistent shadow walk leaves part of the
     // machine to the launch stream's kernels
-    if (rl.side) gs = std::max(8u, (gs * NSP_SIDE_GRID_PCT / 100 + 7) / 8 * 8);
+    if (rl.side) gs = std::max(8u, (gs * side_grid_pct<N>() / 100 + 7) / 8 * 8);
     const unsigned gx = 8 * 32;  // FP64 decisions and re-walks of shadow rays: a few per cent or fewer
     int32_t* redo_s = b.redo.as<int32_t>();
     int32_t* res_s = b.res.as<int32_t>();
