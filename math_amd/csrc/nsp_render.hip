@@ -378,18 +378,28 @@ __device__ __forceinline__ void push_pending(unsigned long long* res_count, int3
 
 // Persistent traversal (shadow stage): each lane holds one ray walk (RayWalk)
 // across the iterations of the loop and does one leaf step per iteration; when
-// at least REFILL_LANES lanes of a wave have finished, the wave takes that
+// at least refill_lanes<N>() lanes of a wave have finished, the wave takes that
 // many new rays with one atomic.  The queue is split into 8 contiguous slices, one per XCD
 // (workgroups are dispatched round robin over the 8 XCDs): a wave takes rays
 // from its own XCD's slice, so neighbouring rays share that XCD's L2, and
 // moves on to the next slices once its own is empty.  Every wave leaves the
 // loop after all 8 slices are empty and its own rays are done.
-#ifndef NSP_REFILL_LANES
-#define NSP_REFILL_LANES 32
+// Measured with one batch per pass (Mrays/s, repeated runs): n = 4 (C1) 8 lanes 1224;
+// 16 lanes 1330, 1345, 1326, 1333; 24 lanes 1344; 32 lanes 1308, 1312, 1309, 1320; 48 lanes
+// 1321.  n = 5 (C2): 16 lanes 262, 32 lanes 276.  -DNSP_REFILL_LANES=L sets one
+// threshold for every dimension.
+template <int N>
+constexpr int refill_lanes()
+{
+#ifdef NSP_REFILL_LANES
+    return NSP_REFILL_LANES;
+#else
+    return N >= 5 ? 32 : 16;
 #endif
-constexpr int REFILL_LANES = NSP_REFILL_LANES;
+}
 constexpr int XCDS = 8;
 
+template <int REFILL_LANES>
 struct RayPool {
     unsigned long long* work;  // XCDS slice counters
     int64_t count;
@@ -472,7 +482,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_closest_pool(TreeArgs A, RayQue
 {
     __shared__ int2 stack[MAX_LEVELS * TRACE_BLOCK];
     int2* st = stack + threadIdx.x;
-    RayPool pool(cnt + 8, (int64_t)cnt[0]);
+    RayPool<refill_lanes<N>()> pool(cnt + 8, (int64_t)cnt[0]);
     RayWalk<N, T, false, LAY, true> w;
     int64_t ray = -1;
     for (;;) {
@@ -567,7 +577,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_shadow(TreeArgs A, ShadowQueue<
 {
     __shared__ int2 stack[MAX_LEVELS * TRACE_BLOCK];
     int2* st = stack + threadIdx.x;
-    RayPool pool(cnt + 15, (int64_t)cnt[0]);
+    RayPool<refill_lanes<N>()> pool(cnt + 15, (int64_t)cnt[0]);
     RayWalk<N, T, true, LAY, true> w;
     int64_t ray = -1;
     for (;;) {
