@@ -1280,8 +1280,12 @@ int nsp_autotune(nsp_scene* scene, nsp_paint_params* params)
     p.passes = 1;
     int64_t best = 0;
     double best_s = 1e300;
-    for (int lg = 20; lg <= 24; ++lg) {
+    const bool raster = p.precision == NSP_PRECISION_FP32 && p.primary == NSP_PRIMARY_AUTO &&
+                        scene->proj.kind == NSP_PROJECTOR_PARALLEL;
+    const int64_t per = render_bytes_per_path(scene->n, p.precision, (int)scene->lights.size(), p.aov != 0, raster);
+    for (int lg = 20; lg <= (p.precision ? 25 : 26); ++lg) {
         p.batch_paths = (int64_t)1 << lg;
+        if (lg > 20 && p.batch_paths * per > DEFAULT_WORKSPACE) break;
         auto t0 = std::chrono::steady_clock::now();
         rc = nsp_paint(scene, &p, nullptr, nullptr);
         if (rc != NSP_OK) break;
