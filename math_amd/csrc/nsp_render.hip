@@ -399,6 +399,14 @@ constexpr int refill_lanes()
 }
 constexpr int XCDS = 8;
 
+// Minimum waves per SIMD asked of the compiler for the persistent walks
+// (-DNSP_POOL_MIN_WAVES=W); by default their register use alone sets it.
+#ifdef NSP_POOL_MIN_WAVES
+#define NSP_POOL_BOUNDS __launch_bounds__(TRACE_BLOCK, NSP_POOL_MIN_WAVES)
+#else
+#define NSP_POOL_BOUNDS __launch_bounds__(TRACE_BLOCK)
+#endif
+
 template <int REFILL_LANES>
 struct RayPool {
     unsigned long long* work;  // XCDS slice counters
@@ -473,7 +481,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_closest(TreeArgs A, RayQueue<T>
 // k_closest_resolve; walks with more than PENDING of them are walked again by
 // k_closest_exact.
 template <int N, typename T, int LAY>
-__global__ void __launch_bounds__(TRACE_BLOCK) k_closest_pool(TreeArgs A, RayQueue<T> q, int64_t cap,
+__global__ void NSP_POOL_BOUNDS k_closest_pool(TreeArgs A, RayQueue<T> q, int64_t cap,
                                                               unsigned long long* __restrict__ cnt,
                                                               unsigned long long* __restrict__ redo_count,
                                                               int32_t* __restrict__ redo,
@@ -566,7 +574,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_closest_exact(TreeArgs A, RayQu
 }
 
 template <int N, typename T, int LAY>
-__global__ void __launch_bounds__(TRACE_BLOCK) k_shadow(TreeArgs A, ShadowQueue<T> sq, int64_t scap,
+__global__ void NSP_POOL_BOUNDS k_shadow(TreeArgs A, ShadowQueue<T> sq, int64_t scap,
                                                         unsigned long long* __restrict__ cnt,
                                                         float* __restrict__ L, int64_t cap,
                                                         float* __restrict__ direct,
