@@ -21,7 +21,10 @@
 
 namespace nsp {
 
-constexpr int TRACE_BLOCK = 256;
+#ifndef NSP_TRACE_BLOCK
+#define NSP_TRACE_BLOCK 256
+#endif
+constexpr int TRACE_BLOCK = NSP_TRACE_BLOCK;  // threads per block of the trace and stage kernels
 constexpr int MAX_LEVELS = 10;  // reference depth limit, src/painter/space/tree.h:428-431
 
 // Half-width of the barycentric band (in barycentric units, scaled by
