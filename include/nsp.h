@@ -121,9 +121,10 @@ int nsp_obj_write(const char* path, int n, const float* vertices, int64_t V, con
  * indices) and a unit outward normal (ConvexHullFacet<N>::vertices / ortho).
  * Coplanar points are not hull vertices unless needed, and a non-simplicial
  * face comes out triangulated as the insertion order makes it.  The first
- * `first_points` points (0: 4096) are inserted on the host; the GPU then
- * computes the conflicts of all later points with that hull in one pass and
- * drops the points strictly inside it.  Errors (NSP_E_INVALID): fewer than
+ * `first_points` points (0: 4096) are inserted on the host; then, in stages
+ * [K, 4K), [4K, 16K), ..., the GPU computes the conflicts of the stage's points
+ * with the hull built so far in one pass, drops the points strictly inside it,
+ * and the host inserts the rest.  Errors (NSP_E_INVALID): fewer than
  * n + 1 distinct points, all points equal, no n-simplex among them, non-finite
  * coordinates; NSP_E_UNSUPPORTED for n outside 2..6. */
 typedef struct nsp_hull nsp_hull;

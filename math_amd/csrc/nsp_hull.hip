@@ -18,10 +18,11 @@
 // A point strictly inside the hull of the first K points can never see a later
 // facet (later hulls contain this one).  So the host inserts the first K points
 // with conflict lists over those points only, then k_hull_count / k_hull_fill
-// test every remaining point against every facet of that hull on the GPU: points
-// that see no facet are dropped, the others get their conflict lists, and the
-// host continues from point K.  The facets equal those of the plain construction
-// for every input, degenerate ones included.
+// test the points of the next stage, [K, 4K), against every facet of that hull
+// on the GPU: points that see no facet are dropped, the others get their
+// conflict lists, and the host inserts them; then the same for [4K, 16K), and
+// so on.  The facets equal those of the plain construction for every input,
+// degenerate ones included.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -396,11 +397,11 @@ struct Builder {
             for (int q : F[nid].conf) pconf[q].push_back(nid);
     }
 
-    // conflict graph of points [first, P) against the current hull, on the GPU
-    void gpu_conflicts(int first, hipStream_t s, int64_t* dropped)
+    // conflict graph of points [lo, hi) against the current hull, on the GPU
+    void gpu_conflicts(int lo, int hi, hipStream_t s, int64_t* dropped)
     {
         std::vector<int32_t> pts;
-        for (int q = first; q < P; ++q)
+        for (int q = lo; q < hi; ++q)
             if (enabled[q]) pts.push_back(q);
         if (pts.empty()) return;
         std::vector<int> ids;
@@ -461,6 +462,10 @@ struct Builder {
         }
     }
 
+    // Stages [first, 4 first), [4 first, 16 first), ...: at each stage's start the
+    // GPU gives the conflicts of that stage's points only, so the host never
+    // tracks points of later stages, and each later pass meets a larger hull that
+    // more points lie inside.
     void run(int first, hipStream_t s, nsp_hull* out)
     {
         double t0 = now_ms();
@@ -468,15 +473,18 @@ struct Builder {
         start(first);
         for (int q = 0; q < first; ++q)
             if (enabled[q]) insert(q);
-        double t1 = now_ms();
-        gpu_conflicts(first, s, &out->stats[1]);
-        double t2 = now_ms();
-        for (int q = first; q < P; ++q)
-            if (enabled[q]) insert(q);
-        double t3 = now_ms();
-        out->ms[0] = t1 - t0;
-        out->ms[1] = t2 - t1;
-        out->ms[2] = t3 - t2;
+        out->ms[0] = now_ms() - t0;
+        for (int lo = first; lo < P;) {
+            const int hi = (int)std::min<int64_t>(P, (int64_t)lo * 4);
+            double t1 = now_ms();
+            gpu_conflicts(lo, hi, s, &out->stats[1]);
+            double t2 = now_ms();
+            for (int q = lo; q < hi; ++q)
+                if (enabled[q]) insert(q);
+            out->ms[1] += t2 - t1;
+            out->ms[2] += now_ms() - t2;
+            lo = hi;
+        }
         out->stats[2] = (int64_t)F.size();
     }
 };
