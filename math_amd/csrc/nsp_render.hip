@@ -907,10 +907,11 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
                       ? (sc.smooth ? k_shade<N, T, true, true, false> : k_shade<N, T, false, true, false>)
                       : (sc.smooth ? k_shade<N, T, true, false, false> : k_shade<N, T, false, false, false>);
     auto ks = aos ? k_shadow<N, T, 1> : k_shadow<N, T, 0>;
-    unsigned gs = aos ? persistent_grid(k_shadow<N, T, 1>) : persistent_grid(k_shadow<N, T, 0>);
+    const unsigned gs_full = aos ? persistent_grid(k_shadow<N, T, 1>) : persistent_grid(k_shadow<N, T, 0>);
     // beside the closest-hit walks, the persistent shadow walk leaves part of the
-    // machine to the launch stream's kernels
-    if (rl.side) gs = std::max(8u, (gs * side_grid_pct<N>() / 100 + 7) / 8 * 8);
+    // machine to the launch stream's kernels; the last depth's shadow stage has
+    // no closest-hit walk beside it and takes the whole grid
+    const unsigned gs_side = rl.side ? std::max(8u, (gs_full * side_grid_pct<N>() / 100 + 7) / 8 * 8) : gs_full;
     const unsigned gx = 8 * 32;  // FP64 decisions and re-walks of shadow rays: a few per cent or fewer
     int32_t* redo_s = b.redo.as<int32_t>();
     int32_t* res_s = b.res.as<int32_t>();
@@ -977,6 +978,7 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
         }
         timed_on(ss, 3, [&] {
             float* direct = (aov && depth == 0) ? aov + (int64_t)(3 + N) * cap : nullptr;
+            const unsigned gs = depth < rl.max_depth ? gs_side : gs_full;
             hipLaunchKernelGGL(ks, dim3(gs), dim3(TRACE_BLOCK), 0, ss, As, sq, scap, dc + 1, Ls, cap, direct,
                                dc + 6, redo_s, dc + 4, res_s);
             NSP_HIP(hipGetLastError());
