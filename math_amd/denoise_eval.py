@@ -4,13 +4,14 @@ the time-to-RMSE comparison of denoised against plain FP32 rendering.
   variants   n=4 tesseract-boundary meshes of the C1 generator with radial noise
              sigma ~ U[0, 0.05] and a point light at distance 4 in a direction
              uniform on S^3 (seeded)
-  inputs     16 channels per voxel of the 3D screen: the 1-spp radiance and
-             the 4-spp (8-sample) radiance, and from the 4-spp pass the
-             first-hit albedo, normal and direct light (the radiance the first
-             hit receives from the light and the background it sees).
-             Rendering is progressive: at level j the slots hold j one-sample
-             passes and j eight-sample passes, so level 1 is the plain 1-spp +
-             4-spp input and the network can be applied at any sample budget
+  inputs     13 channels per voxel of the 3D screen: the 4-spp (8-sample)
+             radiance and, from the same pass, the first-hit albedo, normal and
+             direct light (the radiance the first hit receives from the light
+             and the background it sees).
+             Rendering is progressive: at level j the slots hold j
+             eight-sample passes, so level 1 is the plain 4-spp input and the
+             network can be applied at any sample budget; a level costs what a
+             plain 4-spp pass costs, plus the AOV writes
   target     FP32 render of target_spp samples per pixel (2048 spp requests
              13^3 = 2197 stratified samples)
   network    8 layers 3x3x3, 32 channels, BF16/FP32-accumulate, residual on the
@@ -24,8 +25,8 @@ import numpy as np
 
 from . import configs, native
 
-IN_CHANNELS = 16
-RESIDUAL = 3  # the 4-spp radiance channels
+IN_CHANNELS = 13
+RESIDUAL = 0  # the 4-spp radiance channels
 LEVELS = (1, 2, 4, 8, 16, 32, 64, 128)  # progressive levels stored for training
 THRESHOLDS = (0.03, 0.02, 0.01)
 
@@ -65,10 +66,9 @@ def _paint_timed(scene, spp, depth, seed, **kw):
 
 class ProgressiveInputs:
     """The denoiser input of one scene, rendered progressively: each advance()
-    adds one 1-spp pass and one 4-spp (8-sample) pass with its AOVs."""
+    adds one 4-spp (8-sample) pass with its AOVs."""
 
     def __init__(self, ctx, mesh, cfg, seed, depth=5):
-        self.s1 = native.Scene(ctx, mesh, cfg)
         self.s4 = native.Scene(ctx, mesh, cfg)
         self.screen = cfg["screen"]
         self.seed, self.depth = seed, depth
@@ -76,22 +76,19 @@ class ProgressiveInputs:
         self.secs = 0.0  # paint seconds so far
 
     def advance(self):
-        self.secs += _paint_timed(self.s1, 1, self.depth, self.seed)
         self.secs += _paint_timed(self.s4, 4, self.depth, self.seed + 7919, aov=True)
         self.level += 1
 
     def volume(self):
-        r1, c1 = self.s1.image()
         r4, c4 = self.s4.image()
         alb, nrm, dl = self.s4.aov()
-        k1 = np.maximum(c1, 1)[:, None]
         k4 = np.maximum(c4, 1)[:, None]
-        x = np.concatenate([r1 / k1, r4 / k4, alb / k4, nrm / k4, dl / k4], axis=1)
+        x = np.concatenate([r4 / k4, alb / k4, nrm / k4, dl / k4], axis=1)
         return to_volume(x.astype(np.float32), self.screen)
 
 
 def render_inputs(ctx, mesh, cfg, seed, depth=5):
-    """Level-1 (1-spp + 4-spp) denoiser input volume and the paint seconds spent."""
+    """Level-1 (4-spp) denoiser input volume and the paint seconds spent."""
     p = ProgressiveInputs(ctx, mesh, cfg, seed, depth)
     p.advance()
     return p.volume(), p.secs
@@ -186,9 +183,9 @@ def time_to_rmse(ctx, mesh, cfg, ref, denoiser, depth=5, thresholds=THRESHOLDS, 
     after max_passes plain passes, max_levels denoised levels).
 
       plain     progressive 4-spp (8-sample) FP32 passes
-      denoised  progressive levels of ProgressiveInputs (1 + 8 samples per
+      denoised  progressive levels of ProgressiveInputs (8 samples per
                 level); after each level the network maps the input to an
-                image, level 1 being the standard 1-spp + 4-spp input, blended
+                image, level 1 being the standard 4-spp input, blended
                 with the 4-spp input by the level's fitted weight when alphas
                 (fit_blend) are given.  The time at a level is its passes plus
                 one inference.
@@ -220,10 +217,10 @@ def time_to_rmse(ctx, mesh, cfg, ref, denoiser, depth=5, thresholds=THRESHOLDS, 
         e = rmse(y, ref)
         if k == 0:
             out["denoised_rmse"] = e
-            out["input4_rmse"] = rmse(x[..., 3:6], ref)
+            out["input4_rmse"] = rmse(x[..., RESIDUAL:RESIDUAL + 3], ref)
             out["denoiser_inference_s"] = dt
         # a render stopped at this level costs its passes and one inference
-        dcurve.append((9 * p.level, p.secs + dt, e))
+        dcurve.append((8 * p.level, p.secs + dt, e))
         if e <= goal:
             break
     out["denoised_curve"] = dcurve

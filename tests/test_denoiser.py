@@ -196,7 +196,7 @@ def test_trained_denoiser_beats_its_noisy_input_on_heldout_render(ctx):
     ref, _ = DE.render_mean(ctx, mesh, cfg, 1024, seed=2001)
     x, _ = DE.render_inputs(ctx, mesh, cfg, seed=2002)
     y = d.forward(x[None])[0]
-    e_in, e_out = DE.rmse(x[..., 3:6], ref), DE.rmse(y, ref)
+    e_in, e_out = DE.rmse(x[..., DE.RESIDUAL:DE.RESIDUAL + 3], ref), DE.rmse(y, ref)
     print(f"held-out rmse: 4-spp input {e_in:.4f}, denoised {e_out:.4f}", flush=True)
     assert np.all(np.isfinite(y))
     assert e_out < 0.9 * e_in
