@@ -223,6 +223,10 @@ struct Builder {
     std::vector<int> mark;
     int stamp = 0;
     int64_t alive_count = 0;
+    // device buffers of the conflict passes, kept across the stages (DevBuf::alloc
+    // keeps a buffer that is large enough); the points go up once
+    DevBuf dX, dP, dF, dC, dO, dL;
+    bool x_uploaded = false;
 
     Builder(const std::vector<int64_t>& x, int p) : X(x), P(p), pconf(p), enabled(p, 1), mark(p, 0) {}
 
@@ -417,13 +421,15 @@ struct Builder {
             ids.push_back(id);
         }
         const int64_t np = (int64_t)pts.size(), nf = (int64_t)fd.size();
-        DevBuf dX, dP, dF, dC, dO, dL;
         dX.alloc(sizeof(int64_t) * X.size());
         dP.alloc(sizeof(int32_t) * np);
         dF.alloc(sizeof(FacetDev<M, C>) * nf);
         dC.alloc(sizeof(int32_t) * np);
         dO.alloc(sizeof(int64_t) * (np + 1));
-        NSP_HIP(hipMemcpyAsync(dX.p, X.data(), sizeof(int64_t) * X.size(), hipMemcpyHostToDevice, s));
+        if (!x_uploaded) {
+            NSP_HIP(hipMemcpyAsync(dX.p, X.data(), sizeof(int64_t) * X.size(), hipMemcpyHostToDevice, s));
+            x_uploaded = true;
+        }
         NSP_HIP(hipMemcpyAsync(dP.p, pts.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, s));
         NSP_HIP(hipMemcpyAsync(dF.p, fd.data(), sizeof(FacetDev<M, C>) * nf, hipMemcpyHostToDevice, s));
         const unsigned grid = (unsigned)((np * 64 + HULL_BLOCK - 1) / HULL_BLOCK);
