@@ -350,7 +350,8 @@ def main():
     ap.add_argument("--obj-sample-facets", type=int, default=60000, help="with --obj: facets read by the oracle")
     ap.add_argument("--batch-scenes", action="store_true",
                     help="C4: batched persistent launch over many scenes vs one launch sequence per scene")
-    ap.add_argument("--scenes", type=int, default=256)
+    ap.add_argument("--scenes", type=int, default=16,
+                    help="C4: the first scenes of the 256 seeded scenes (all 256 meshes do not fit beside the workspaces)")
     ap.add_argument("--lanes", type=int, default=0, help="batch lanes (0: autotune lanes and queue size)")
     args = ap.parse_args()
     if args.denoise:

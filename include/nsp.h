@@ -387,9 +387,10 @@ int nsp_batch_statistics(const nsp_batch* batch, int64_t* ray_count, double* las
  * scene's first pixels (images do not depend on the choice); writes the
  * winner into params->batch_paths. */
 int nsp_autotune(nsp_scene* scene, nsp_paint_params* params);
-/* Chooses the lane count of a batch (kept in the batch) and its queue size
- * (written into params->batch_paths) by timing whole batched passes; every
- * scene's image and statistics are restored afterwards. */
+/* Chooses the lane count of a batch (1, 2, 4 or 8; kept in the batch) and its
+ * queue size (2^22 to 2^25 paths; written into params->batch_paths) by timing
+ * whole batched passes; every scene's image and statistics are restored
+ * afterwards. */
 int nsp_batch_autotune(nsp_batch* batch, nsp_paint_params* params);
 /* The batch's current lane count. */
 int nsp_batch_get_lanes(const nsp_batch* batch);

@@ -402,8 +402,10 @@ int nsp_mesh_set_record_layout(nsp_mesh* mesh, int layout)
         if (mesh->ctx) mesh->ctx->error = "unknown record layout " + std::to_string(layout);
         return NSP_E_INVALID;
     }
-    mesh->m.layout = layout;
-    return NSP_OK;
+    return guarded(mesh->ctx, [&] {
+        if (layout == NSP_LAYOUT_LEAF_SOA) build_soa_device(mesh->m, mesh->ctx->stream);
+        mesh->m.layout = layout;
+    });
 }
 
 int nsp_mesh_get_record_layout(const nsp_mesh* mesh) { return mesh ? mesh->m.layout : NSP_E_INVALID; }
@@ -1348,8 +1350,10 @@ int nsp_batch_autotune(nsp_batch* b, nsp_paint_params* params)
     int best_lanes = 4;
     int64_t best_paths = (int64_t)1 << 23;
     double best_s = 1e300;
-    for (int lanes : {2, 4, 8}) {
-        if (lanes > ns && lanes != 2) continue;
+    // lanes first at 2^23 paths, then the queue size at the best lane count; the
+    // lanes' workspaces together stay within twice the single-scene budget
+    for (int lanes : {1, 2, 4, 8}) {
+        if (lanes > ns && lanes > 2) continue;
         double t = timed(lanes, (int64_t)1 << 23);
         if (rc != NSP_OK) break;
         if (t < best_s) {
@@ -1357,11 +1361,14 @@ int nsp_batch_autotune(nsp_batch* b, nsp_paint_params* params)
             best_lanes = lanes;
         }
     }
-    if (rc == NSP_OK) {
-        double t = timed(best_lanes, (int64_t)1 << 22);
+    for (int lg : {22, 24, 25}) {
+        if (rc != NSP_OK) break;
+        const int64_t paths = (int64_t)1 << lg;
+        if (lg > 23 && (int64_t)best_lanes * paths * 512 > 2 * DEFAULT_WORKSPACE) continue;
+        double t = timed(best_lanes, paths);
         if (rc == NSP_OK && t < best_s) {
             best_s = t;
-            best_paths = (int64_t)1 << 22;
+            best_paths = paths;
         }
     }
     int rc2 = NSP_OK;

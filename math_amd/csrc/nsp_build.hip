@@ -687,14 +687,35 @@ void build_tree_t(MeshDev& m, int max_depth, int min_objects, hipStream_t s)
     for (const int2& nd : hn)
         if (nd.y >= 0) ++leaves;
     t.leaf_count = leaves;
-    // SoA
+    // the leaf-ordered SoA copy of the records is built on demand (build_soa_device)
     t.entry_pad = (entry_count + 63) / 64 * 64;
     if (t.entry_pad == 0) t.entry_pad = 64;
-    t.soa.alloc(sizeof(float) * t.entry_pad * soa_components(N));
-    hipLaunchKernelGGL(k_pack_soa<N>, dim3(blocks_for(entry_count)), dim3(256), 0, s, entry_count, t.entry_pad,
-                       t.entries.as<int32_t>(), m.rec64.as<double>(), t.soa.as<float>());
-    NSP_HIP(hipGetLastError());
     NSP_HIP(hipStreamSynchronize(s));
+}
+
+template <int N>
+static void build_soa_t(MeshDev& m, hipStream_t s)
+{
+    TreeDev& t = m.tree;
+    t.soa.alloc(sizeof(float) * t.entry_pad * soa_components(N));
+    if (t.entry_count > 0) {
+        hipLaunchKernelGGL(k_pack_soa<N>, dim3(blocks_for(t.entry_count)), dim3(256), 0, s, t.entry_count,
+                           t.entry_pad, t.entries.as<int32_t>(), m.rec64.as<double>(), t.soa.as<float>());
+        NSP_HIP(hipGetLastError());
+    }
+    NSP_HIP(hipStreamSynchronize(s));
+}
+
+void build_soa_device(MeshDev& m, hipStream_t s)
+{
+    if (m.tree.soa.p) return;
+    switch (m.n) {
+    case 3: build_soa_t<3>(m, s); break;
+    case 4: build_soa_t<4>(m, s); break;
+    case 5: build_soa_t<5>(m, s); break;
+    case 6: build_soa_t<6>(m, s); break;
+    default: raise(NSP_E_INVALID, "dimension not supported");
+    }
 }
 
 template <int N>
