@@ -283,8 +283,8 @@ typedef struct {
     int passes;            /* passes to run (>= 1), or -1: until *stop (VisibleBarPaintbrush
                               max_pass_count, paintbrush.h:141-144) */
     uint64_t seed;         /* counter-based RNG key */
-    int64_t batch_paths;   /* paths per wavefront batch; 0 = the largest power of two up to 2^26
-                              (FP32) or 2^25 (FP64) whose workspace fits 24 GiB */
+    int64_t batch_paths;   /* paths per wavefront batch; 0 = the largest power of two up to 2^27
+                              (FP32) or 2^26 (FP64) whose workspace fits 48 GiB */
     int use_graph;         /* capture a batch's launch sequence in a hipGraph once per call and
                               replay it for every batch of every pass */
     int aov;               /* also accumulate first-hit albedo, normal and direct light per pixel (denoiser inputs) */

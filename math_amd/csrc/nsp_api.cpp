@@ -132,7 +132,7 @@ std::string* context_error(nsp_context* ctx) { return &ctx->error; }
 }  // namespace nsp
 
 // device workspace a default wavefront batch may take (bytes)
-static constexpr int64_t DEFAULT_WORKSPACE = (int64_t)24 << 30;
+static constexpr int64_t DEFAULT_WORKSPACE = (int64_t)48 << 30;
 
 template <class F>
 static int guarded(nsp_context* ctx, F&& f)
@@ -836,15 +836,16 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
         const int S = kf.S;
         const bool raster = p.precision == NSP_PRECISION_FP32 && p.primary == NSP_PRIMARY_AUTO &&
                             scene->proj.kind == NSP_PROJECTOR_PARALLEL;
-        // Default batch: the largest power of two of paths, at most 2^26 (FP32) or
-        // 2^25 (FP64), whose workspace fits DEFAULT_WORKSPACE.  Large batches keep
+        // Default batch: the largest power of two of paths, at most 2^27 (FP32) or
+        // 2^26 (FP64), whose workspace fits DEFAULT_WORKSPACE.  Large batches keep
         // the persistent walks busy and leave fewer batch tails (C1, 27 samples per
-        // pixel: 898 Mrays/s at 2^23 paths, 1076 at 2^24, 1218 at 2^25, 1308 at 2^26).
+        // pixel: 898 Mrays/s at 2^23 paths, 1076 at 2^24, 1218 at 2^25, 1308 at 2^26;
+        // C2 in one 2^27-path batch 282 against 276 in two of 2^26).
         int64_t want = p.batch_paths;
         if (want <= 0) {
             const int64_t per =
                 render_bytes_per_path(n, p.precision, (int)scene->lights.size(), p.aov != 0, raster);
-            want = (int64_t)1 << (p.precision ? 25 : 26);
+            want = (int64_t)1 << (p.precision ? 26 : 27);
             while (want > ((int64_t)1 << 20) && want * per > DEFAULT_WORKSPACE) want >>= 1;
         }
         int64_t batch_pixels = std::max<int64_t>(1, want / S);
@@ -1285,7 +1286,7 @@ int nsp_autotune(nsp_scene* scene, nsp_paint_params* params)
     const bool raster = p.precision == NSP_PRECISION_FP32 && p.primary == NSP_PRIMARY_AUTO &&
                         scene->proj.kind == NSP_PROJECTOR_PARALLEL;
     const int64_t per = render_bytes_per_path(scene->n, p.precision, (int)scene->lights.size(), p.aov != 0, raster);
-    for (int lg = 20; lg <= (p.precision ? 25 : 26); ++lg) {
+    for (int lg = 20; lg <= (p.precision ? 26 : 27); ++lg) {
         p.batch_paths = (int64_t)1 << lg;
         if (lg > 20 && p.batch_paths * per > DEFAULT_WORKSPACE) break;
         auto t0 = std::chrono::steady_clock::now();
