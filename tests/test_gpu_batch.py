@@ -72,7 +72,7 @@ def test_autotune_restores_state_and_choice_is_image_invariant(ctx):
     before = s.image()
     stats = s.statistics()
     best = native.autotune(s, 4, 3, seed=1)
-    assert best in [1 << k for k in range(20, 27)]
+    assert best in [1 << k for k in range(20, 28)]
     after = s.image()
     assert all(np.array_equal(p, q) for p, q in zip(before, after))
     assert s.statistics() == stats
