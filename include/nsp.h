@@ -383,9 +383,12 @@ int nsp_batch_paint(nsp_batch* batch, const nsp_paint_params* params, const vola
 int nsp_batch_set_lanes(nsp_batch* batch, int lanes);
 /* rays of the last batch pass (reference counting rules) and its wall time */
 int nsp_batch_statistics(const nsp_batch* batch, int64_t* ray_count, double* last_pass_duration);
-/* Chooses batch_paths for a scene by timing candidate queue sizes on the
- * scene's first pixels (images do not depend on the choice); writes the
- * winner into params->batch_paths. */
+/* Chooses batch_paths for a scene by timing one whole pass at each candidate
+ * queue size (2^20 up to the first power of two holding every path of the
+ * screen, within the 48 GiB workspace budget; after one untimed warm-up pass
+ * at the largest), and writes the winner into params->batch_paths.  Images do
+ * not depend on the choice; the scene's image, AOV sums and statistics are as
+ * before the call (the timed passes write no AOVs). */
 int nsp_autotune(nsp_scene* scene, nsp_paint_params* params);
 /* Chooses the lane count of a batch (1, 2, 4 or 8; kept in the batch) and its
  * queue size (2^22 to 2^25 paths; written into params->batch_paths) by timing

@@ -123,6 +123,9 @@ struct nsp_scene {
     nsp_stage_stats stage{};
 };
 
+// pass index of the counter-based sample streams of the scene's next pass
+static int rng_pass(const nsp_scene* s) { return (int)s->pass_count; }
+
 static thread_local std::string g_create_error;
 
 namespace nsp {
@@ -893,7 +896,7 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
             if (stop && *stop) raise(NSP_E_STOPPED, "stopped");
             auto t0 = std::chrono::steady_clock::now();
             scene->marks.reset();
-            pass_reset(ctx->work, scene->counters, scene->pass_dev.as<int>(), (int)scene->pass_count, s);
+            pass_reset(ctx->work, scene->counters, scene->pass_dev.as<int>(), rng_pass(scene), s);
             if (p.use_graph && !gh.exec) {
                 NSP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
                 try {
@@ -907,6 +910,7 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
                 NSP_HIP(hipStreamEndCapture(s, &gh.graph));
                 NSP_HIP(hipGraphInstantiate(&gh.exec, gh.graph, nullptr, nullptr, 0));
             }
+            int64_t done_pixels = 0;
             for (size_t b = 0; b < descs.size(); ++b) {
                 if (stop && *stop) break;
                 if (gh.exec)
@@ -914,6 +918,7 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
                 else
                     render_batch(rl, s);
                 if (feed) feed->batch_done(descs[b].pixel0, descs[b].paths / S, s);
+                done_pixels += descs[b].paths / S;
             }
             if (feed) feed->deliver();
             unsigned long long tot[3] = {0, 0, 0};
@@ -954,12 +959,18 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
             }
             scene->stage = st;
             bool stopped = stop && *stop;
+            // BarPaintbrush statistics (paintbrush.h:177-232): pixels and samples
+            // count as they are painted, while the pass count is the 1-based index
+            // of the pass begun (m_pass_count starts at 1 and next_pass moves it
+            // on, paintbrush.h:118, 200-219), so a stopped pass counts as begun;
+            // pass_done follows only a pass whose batches all ran
+            const bool whole = done_pixels == scene->pixels;
             scene->pass_count += 1;
-            scene->pixel_count += scene->pixels;
+            scene->pixel_count += done_pixels;
             scene->ray_count += (int64_t)(closest + shadow + probes);
-            scene->sample_count += scene->pixels * S;
+            scene->sample_count += done_pixels * S;
             scene->last_pass_s = secs;
-            if (cb && cb->pass_done) cb->pass_done(cb->user, (int)scene->pass_count, secs);
+            if (whole && cb && cb->pass_done) cb->pass_done(cb->user, (int)scene->pass_count, secs);
             if (stopped) raise(NSP_E_STOPPED, "stopped");
         }
     });
@@ -1142,11 +1153,11 @@ int nsp_batch_paint(nsp_batch* b, const nsp_paint_params* params, const volatile
             BatchScene& bs = b->per[i];
             if (!sc->shapes.empty() && p.smooth_normal && sc->mesh->m.has_normals)
                 raise(NSP_E_UNSUPPORTED, "smooth-normal renders of meshes with vertex normals do not support shapes");
-            fill_scene<float>(*sc, p, (int)sc->pass_count, bs.kf);
+            fill_scene<float>(*sc, p, rng_pass(sc), bs.kf);
             if (!sc->pass_dev.p) sc->pass_dev.alloc(sizeof(int));
             bs.kf.pass_dev = sc->pass_dev.as<int>();
             SceneK<double> kd;
-            fill_scene<double>(*sc, p, (int)sc->pass_count, kd);
+            fill_scene<double>(*sc, p, rng_pass(sc), kd);
             const int S = bs.kf.S;
             bs.batch_pixels = std::min<int64_t>(std::max<int64_t>(1, want / S), sc->pixels);
             std::vector<BatchDesc> descs;
@@ -1195,7 +1206,7 @@ int nsp_batch_paint(nsp_batch* b, const nsp_paint_params* params, const volatile
                     nsp_scene* sc = b->scenes[i];
                     BatchScene& bs = b->per[i];
                     pass_reset_state(bs.totals.as<unsigned long long>(), bs.idx.as<long long>(),
-                                     sc->pass_dev.as<int>(), (int)sc->pass_count, ls);
+                                     sc->pass_dev.as<int>(), rng_pass(sc), ls);
                     RenderLaunch rl;
                     rl.mesh = &sc->mesh->m;
                     rl.scene = &bs.kf;
@@ -1281,17 +1292,45 @@ int nsp_autotune(nsp_scene* scene, nsp_paint_params* params)
     const double last = scene->last_pass_s;
     nsp_paint_params p = *params;
     p.passes = 1;
+    // the timed passes write no AOVs: the AOV sums of the scene stay as they are
+    // (the queue size does not depend on them beyond their buffer, which the
+    // chosen size allocates on the caller's first paint with AOVs)
+    p.aov = 0;
     int64_t best = 0;
     double best_s = 1e300;
     const bool raster = p.precision == NSP_PRECISION_FP32 && p.primary == NSP_PRIMARY_AUTO &&
                         scene->proj.kind == NSP_PROJECTOR_PARALLEL;
-    const int64_t per = render_bytes_per_path(scene->n, p.precision, (int)scene->lights.size(), p.aov != 0, raster);
+    const int64_t per = render_bytes_per_path(scene->n, p.precision, (int)scene->lights.size(), false, raster);
+    int64_t S = 0;
+    rc = guarded(ctx, [&] {
+        S = 1;
+        const int k = one_dimension_size(scene->n - 1, p.samples_per_pixel);
+        for (int i = 0; i < scene->n - 1; ++i) S *= k;
+    });
+    if (rc != NSP_OK) return rc;
+    const int64_t all_paths = scene->pixels * S;
+    // candidates: 2^20 .. the first power of two that holds the whole screen in
+    // one batch (larger ones give the same batches), within the workspace budget
+    std::vector<int64_t> cand;
     for (int lg = 20; lg <= (p.precision ? 26 : 27); ++lg) {
-        p.batch_paths = (int64_t)1 << lg;
-        if (lg > 20 && p.batch_paths * per > DEFAULT_WORKSPACE) break;
+        const int64_t paths = (int64_t)1 << lg;
+        if (lg > 20 && paths * per > DEFAULT_WORKSPACE) break;
+        cand.push_back(paths);
+        if (paths >= all_paths) break;
+    }
+    // untimed warm-up at the largest candidate: the workspace (which only grows)
+    // and the raster set-up are then in place before any candidate is timed
+    p.batch_paths = cand.back();
+    rc = nsp_paint(scene, &p, nullptr, nullptr);
+    for (size_t c = 0; rc == NSP_OK && c < cand.size(); ++c) {
+        p.batch_paths = cand[c];
         auto t0 = std::chrono::steady_clock::now();
-        rc = nsp_paint(scene, &p, nullptr, nullptr);
-        if (rc != NSP_OK) break;
+        int r = nsp_paint(scene, &p, nullptr, nullptr);
+        if (r != NSP_OK) {
+            // a failed candidate ends the search; the best size so far stands
+            if (!best) rc = r;
+            break;
+        }
         double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (secs < best_s) {
             best_s = secs;
@@ -1307,13 +1346,14 @@ int nsp_autotune(nsp_scene* scene, nsp_paint_params* params)
     if (rc != NSP_OK) return rc;
     if (rc2 != NSP_OK) return rc2;
     params->batch_paths = best;
-    (void)ctx;
     return NSP_OK;
 }
 
-// Batched launch: lanes first (2, 4, 8 at the default queue size), then the
-// queue size (2^22, 2^23 paths) at the best lane count, each timed on one
-// whole batched pass; every scene's image and statistics are restored.
+// Batched launch: lanes first (1, 2, 4, 8 at 2^23 paths), then the queue size
+// (2^22, 2^24, 2^25 paths) at the best lane count, each timed on one whole
+// batched pass; candidates whose lane workspaces exceed the budget are not
+// tried and one that cannot allocate is skipped; every scene's image and
+// statistics are restored.
 int nsp_batch_autotune(nsp_batch* b, nsp_paint_params* params)
 {
     if (!b || !params) return NSP_E_INVALID;
@@ -1348,29 +1388,52 @@ int nsp_batch_autotune(nsp_batch* b, nsp_paint_params* params)
         rc = nsp_batch_paint(b, &p, nullptr);
         return rc == NSP_OK ? std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() : 1e300;
     };
-    int best_lanes = 4;
+    // workspace bytes per path of a lane: the largest dimension and light count of
+    // the batch's scenes (a lane sizes its workspace for the scenes it holds)
+    int nmax = 3, lmax = 1;
+    for (nsp_scene* sc : b->scenes) {
+        nmax = std::max(nmax, sc->n);
+        lmax = std::max(lmax, (int)sc->lights.size());
+    }
+    const int64_t per_path = render_bytes_per_path(nmax, NSP_PRECISION_FP32, lmax, false, true);
+    // the lanes' workspaces together stay within the single-scene budget
+    auto fits = [&](int lanes, int64_t paths) { return (int64_t)lanes * paths * per_path <= DEFAULT_WORKSPACE; };
+    int best_lanes = 0;
     int64_t best_paths = (int64_t)1 << 23;
     double best_s = 1e300;
-    // lanes first at 2^23 paths, then the queue size at the best lane count; the
-    // lanes' workspaces together stay within twice the single-scene budget
-    for (int lanes : {1, 2, 4, 8}) {
-        if (lanes > ns && lanes > 2) continue;
-        double t = timed(lanes, (int64_t)1 << 23);
-        if (rc != NSP_OK) break;
-        if (t < best_s) {
+    // one candidate: timed, or skipped when its workspace cannot be allocated
+    // (the best so far stands); any other failure ends the search
+    auto try_one = [&](int lanes, int64_t paths) {
+        double t = timed(lanes, paths);
+        if (rc == NSP_E_NOMEM) {
+            (void)hipGetLastError();
+            rc = NSP_OK;
+            return;
+        }
+        if (rc == NSP_OK && t < best_s) {
             best_s = t;
             best_lanes = lanes;
+            best_paths = paths;
         }
+    };
+    // lanes first (1, 2, 4, 8) at 2^23 paths, then the queue size (2^22, 2^24,
+    // 2^25 paths) at the best lane count
+    for (int lanes : {1, 2, 4, 8}) {
+        if (rc != NSP_OK) break;
+        if (lanes > ns && lanes > 2) continue;
+        if (!fits(lanes, (int64_t)1 << 23)) continue;
+        try_one(lanes, (int64_t)1 << 23);
     }
+    const int lanes_found = best_lanes ? best_lanes : 1;
     for (int lg : {22, 24, 25}) {
         if (rc != NSP_OK) break;
         const int64_t paths = (int64_t)1 << lg;
-        if (lg > 23 && (int64_t)best_lanes * paths * 512 > 2 * DEFAULT_WORKSPACE) continue;
-        double t = timed(best_lanes, paths);
-        if (rc == NSP_OK && t < best_s) {
-            best_s = t;
-            best_paths = paths;
-        }
+        if (!fits(lanes_found, paths)) continue;
+        try_one(lanes_found, paths);
+    }
+    if (rc == NSP_OK && !best_lanes) {
+        rc = NSP_E_NOMEM;
+        b->ctx->error = "no batch configuration fits the device memory";
     }
     int rc2 = NSP_OK;
     for (int i = 0; i < ns; ++i) {

@@ -1450,30 +1450,49 @@ static int make_ctx(const orc_mesh* m, const orc_scene* s, pctx* c)
     return 0;
 }
 
+/* The screen point of one (pixel, sample) in pixel units, before the
+ * projector's -screen/2 shift: pixel coordinate (GlobalIndex order, x0
+ * fastest, src/com/global_index.h) plus the sample's stratified jittered point
+ * (StratifiedJitteredSampleEngine::generate, engine.h:86-156: sample index
+ * digits in base k, dimension 0 least significant, offset digit/k plus a
+ * uniform jitter in [0, 1/k)). */
+static void sample_point(const orc_scene* s, int n, int k, int64_t pixel, int sample, double* out)
+{
+    int64_t rem = pixel;
+    double u[8];
+    uniforms4(s->seed, pixel, sample, s->pass_index, 0, 0, u);
+    if (n - 1 > 4) uniforms4(s->seed, pixel, sample, s->pass_index, 0, 1, u + 4);
+    int sr = sample;
+    double rk = 1.0 / k;
+    for (int i = 0; i < n - 1; ++i) {
+        int64_t x = rem % s->screen[i];
+        rem /= s->screen[i];
+        int d = sr % k;
+        sr /= k;
+        double off = (double)d / k;
+        double jitter = u[i] * rk;
+        out[i] = (double)x + (off + jitter);
+    }
+}
+
+int orc_sample_point(const orc_scene* s, int64_t pixel, int sample, double* point)
+{
+    int n = s->n;
+    int k = orc_samples_per_dimension(n - 1, s->samples_per_pixel);
+    if (k < 1) return fail("bad sample count");
+    sample_point(s, n, k, pixel, sample, point);
+    return 0;
+}
+
 /* paint_pixels body for one (pixel, sample), painter.cpp:419-442 */
 static void sample_color(const pctx* c, pstate* st, float* rgb)
 {
     const orc_scene* s = c->s;
     int n = c->m->n;
-    int64_t rem = st->pixel;
     double sp[NM - 1];
-    double u[8];
-    uniforms4(s->seed, st->pixel, st->sample, s->pass_index, 0, 0, u);
-    if (n - 1 > 4) uniforms4(s->seed, st->pixel, st->sample, s->pass_index, 0, 1, u + 4);
-    int sr = st->sample;
-    double rk = 1.0 / c->k;
-    for (int i = 0; i < n - 1; ++i) {
-        int64_t x = rem % s->screen[i];
-        rem /= s->screen[i];
-        int d = sr % c->k;
-        sr /= c->k;
-        /* StratifiedJitteredSampleEngine::generate, engine.h:86-156 */
-        double off = (double)d / c->k;
-        double jitter = u[i] * rk;
-        double spt = off + jitter;
-        /* screen_org + point, projector.h:51-66 */
-        sp[i] = (-s->screen[i] * 0.5) + ((double)x + spt);
-    }
+    sample_point(s, n, c->k, st->pixel, st->sample, sp);
+    /* screen_org + point, projector.h:51-66 */
+    for (int i = 0; i < n - 1; ++i) sp[i] = (-s->screen[i] * 0.5) + sp[i];
     double org[NM], dir[NM];
     projector_ray(s, n, c->cam_dir, (double (*)[NM])c->axes, c->square_radius, sp, org, dir);
     trace_path(c, st, 0, 1.0f, org, dir, 0, rgb);
@@ -1495,6 +1514,7 @@ void orc_trace_path_sample(const orc_mesh* m, const orc_scene* s, int64_t pixel,
 typedef struct {
     const pctx* c;
     int64_t begin, count;
+    const int64_t* list;  /* explicit pixel indices, or NULL: begin + j */
     int64_t* next;
     float* out;
     int64_t rays;
@@ -1509,7 +1529,7 @@ static void* work(void* arg)
         if (b >= w->count) break;
         int64_t e = b + CH < w->count ? b + CH : w->count;
         for (int64_t j = b; j < e; ++j) {
-            pstate st = {w->begin + j, 0, 0};
+            pstate st = {w->list ? w->list[j] : w->begin + j, 0, 0};
             float sum[3] = {0, 0, 0};
             for (int s = 0; s < w->c->nsamples; ++s) {
                 st.sample = s;
@@ -1524,18 +1544,23 @@ static void* work(void* arg)
     return NULL;
 }
 
-int orc_render(const orc_mesh* m, const orc_scene* s, int64_t pixel_begin, int64_t pixel_count, int threads,
-               float* rgb_sum, int64_t* rays)
+static int render_impl(const orc_mesh* m, const orc_scene* s, int64_t pixel_begin, const int64_t* list,
+                       int64_t pixel_count, int threads, float* rgb_sum, int64_t* rays)
 {
     pctx c;
     if (make_ctx(m, s, &c)) return 1;
     if (threads < 1) return fail("Painter thread count (%d) must be greater than 0", threads);
+    int64_t total_pixels = 1;
+    for (int i = 0; i < s->n - 1; ++i) total_pixels *= s->screen[i];
+    for (int64_t j = 0; list && j < pixel_count; ++j)
+        if (list[j] < 0 || list[j] >= total_pixels) return fail("pixel %lld outside the screen", (long long)list[j]);
     int64_t next = 0;
     worker* ws = calloc(threads, sizeof(worker));
     pthread_t* th = calloc(threads, sizeof(pthread_t));
     for (int i = 0; i < threads; ++i) {
         ws[i].c = &c;
         ws[i].begin = pixel_begin;
+        ws[i].list = list;
         ws[i].count = pixel_count;
         ws[i].next = &next;
         ws[i].out = rgb_sum;
@@ -1550,4 +1575,16 @@ int orc_render(const orc_mesh* m, const orc_scene* s, int64_t pixel_begin, int64
     free(th);
     if (rays) *rays = total;
     return 0;
+}
+
+int orc_render(const orc_mesh* m, const orc_scene* s, int64_t pixel_begin, int64_t pixel_count, int threads,
+               float* rgb_sum, int64_t* rays)
+{
+    return render_impl(m, s, pixel_begin, NULL, pixel_count, threads, rgb_sum, rays);
+}
+
+int orc_render_pixels(const orc_mesh* m, const orc_scene* s, const int64_t* pixels, int64_t count, int threads,
+                      float* rgb_sum, int64_t* rays)
+{
+    return render_impl(m, s, 0, pixels, count, threads, rgb_sum, rays);
 }

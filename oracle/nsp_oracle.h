@@ -158,6 +158,16 @@ int orc_samples_per_dimension(int screen_dims, int spp);
 int orc_render(const orc_mesh* m, const orc_scene* scene, int64_t pixel_begin, int64_t pixel_count,
                int threads, float* rgb_sum, int64_t* rays);
 
+/* orc_render over an explicit list of pixel indices (GlobalIndex order);
+ * rgb_sum[j] belongs to pixels[j]. */
+int orc_render_pixels(const orc_mesh* m, const orc_scene* s, const int64_t* pixels, int64_t count, int threads,
+                      float* rgb_sum, int64_t* rays);
+
+/* The screen point (pixel units, the pixel's integer coordinates plus the
+ * stratified jittered sample point in [0,1)^(n-1)) of one (pixel, sample), as
+ * orc_render generates it (painter.cpp:419-433, engine.h:86-156). */
+int orc_sample_point(const orc_scene* s, int64_t pixel, int sample, double* point);
+
 /* One path, for per-path comparison with the GPU: returns the path colour. */
 void orc_trace_path_sample(const orc_mesh* m, const orc_scene* scene, int64_t pixel, int sample,
                            float* rgb, int64_t* rays);

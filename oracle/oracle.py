@@ -82,6 +82,8 @@ def lib():
         L.orc_trace_tree.restype = None
         L.orc_samples_per_dimension.argtypes = [C.c_int, C.c_int]
         L.orc_render.argtypes = [P, C.POINTER(Scene), C.c_int64, C.c_int64, C.c_int, fp, i64p]
+        L.orc_render_pixels.argtypes = [P, C.POINTER(Scene), i64p, C.c_int64, C.c_int, fp, i64p]
+        L.orc_sample_point.argtypes = [C.POINTER(Scene), C.c_int64, C.c_int, dp]
         L.orc_trace_path_sample.argtypes = [P, C.POINTER(Scene), C.c_int64, C.c_int, fp, i64p]
         L.orc_trace_path_sample.restype = None
         L.orc_cosine_hemisphere.argtypes = [C.c_int, dp, dp, dp]
@@ -277,6 +279,29 @@ def render(mesh, scene, pixel_begin=0, pixel_count=None, threads=1):
     if rc:
         raise ValueError(L.orc_last_error().decode())
     return out, rays.value
+
+
+def render_pixels(mesh, scene, pixels, threads=1):
+    """render over an explicit list of pixel indices: (sums [len,3] float32, ray count)."""
+    L = lib()
+    px = np.ascontiguousarray(pixels, np.int64)
+    out = np.zeros((px.shape[0], 3), np.float32)
+    rays = C.c_int64()
+    rc = L.orc_render_pixels(mesh.h, C.byref(scene), _p(px, C.c_int64), px.shape[0], threads, _p(out, C.c_float),
+                             C.byref(rays))
+    if rc:
+        raise ValueError(L.orc_last_error().decode())
+    return out, rays.value
+
+
+def sample_point(scene, pixel, sample):
+    """Screen point (pixel units) of one (pixel, sample): integer pixel coordinates plus the
+    stratified jittered point in [0,1)^(n-1)."""
+    n = scene.n
+    out = np.zeros(n - 1)
+    if lib().orc_sample_point(C.byref(scene), pixel, sample, _p(out, C.c_double)):
+        raise ValueError(lib().orc_last_error().decode())
+    return out
 
 
 def path_sample(mesh, scene, pixel, sample):
