@@ -467,3 +467,44 @@ def test_rasterised_primary_visibility_equals_tree_walk(ctx, n):
         imgs.append(sc.image() + (sc.statistics()["rays"],))
     for a, b in zip(imgs[0], imgs[1]):
         assert np.array_equal(a, b)
+
+
+def test_cpp_single_object_scene_conventions():
+    """single_object_scene(background, colour, diffuse, min, max, mesh) of the C++
+    drop-in against the formulas of src/painter/scenes/single_object.h:102-179:
+    camera at the mesh centre moved to max[N-1] + |size| on the last axis,
+    looking along -e_{N-1}, axes e_0..e_{N-2}, units per pixel = largest
+    projected extent / max screen, screen edges lround(size_i / largest *
+    max) clamped to [min, max], a constant light of colour 1 at max + 100 (max -
+    centre); the mesh min/max over the facets' vertices (mesh.cpp:85-98)."""
+    import math
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "test_conventions")
+    assert os.path.exists(exe), "built by __graft_entry__.build()"
+    r = subprocess.run([exe, "scene", "3", "5", "23"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = {}
+    for ln in r.stdout.splitlines():
+        k, *vals = ln.split()
+        out.setdefault(k, []).append([float(x) for x in vals])
+    v, f = native.meshgen_cube_sphere(4, 3, True, 0.03, 0.0, 7)
+    used = v[np.unique(f)].astype(np.float64)
+    mn, mx = used.min(axis=0), used.max(axis=0)
+    assert np.array_equal(out["min"][0], mn) and np.array_equal(out["max"][0], mx)
+    size = mx - mn
+    centre = mn + size / 2
+    maxp = size[:3].max()
+    cam = centre.copy()
+    cam[3] = mx[3] + math.sqrt(float(size @ size))
+    assert out["projector"][0] == [native.PROJECTOR_PARALLEL]
+    assert np.allclose(out["camera_org"][0], cam, rtol=0, atol=1e-12)
+    assert out["camera_dir"][0] == [0.0, 0.0, 0.0, -1.0]
+    assert np.array_equal(np.array(out["axis"]), np.eye(4)[:3])
+    assert out["units_per_pixel"][0][0] == pytest.approx(maxp / 23, rel=1e-14)
+    assert out["screen"][0] == [min(max(math.floor(size[i] / maxp * 23 + 0.5), 5), 23) for i in range(3)]
+    assert len(out["light"]) == 1
+    light = out["light"][0]
+    assert light[0] == native.LIGHT_CONSTANT
+    assert np.allclose(light[1:5], mx + 100 * (mx - centre), rtol=1e-14, atol=1e-12)
+    assert light[5:] == [1.0, 1.0, 1.0]
