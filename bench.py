@@ -268,19 +268,10 @@ def batch_bench(args):
     from math_amd import configs, native
 
     ctx = native.Context(0)
-    rng = np.random.default_rng(300)
     meshes = []
     t0 = time.perf_counter()
-    for i in range(args.scenes):
-        n = int(rng.integers(3, 6))
-        if n == 3:
-            g, edge = int(rng.integers(8, 60)), int(rng.integers(64, 257))
-        elif n == 4:
-            g, edge = int(rng.integers(3, 16)), int(rng.integers(32, 97))
-        else:
-            g, edge = int(rng.integers(2, 6)), int(rng.integers(32, 97))
-        v, f = native.meshgen_cube_sphere(n, g, True, 0.02, 0.0, 300 + i)
-        cfg = configs.scene_dict(v, f, edge, tuple([2.0] * n))
+    for n, g, edge, seed in configs.c4_scenes(256)[:args.scenes]:
+        v, f, cfg = configs.c4_scene(n, g, edge, seed)
         meshes.append((ctx.mesh(v, f), cfg, len(f)))
     log(f"{args.scenes} meshes built in {time.perf_counter() - t0:.1f} s "
         f"({min(m[2] for m in meshes)}..{max(m[2] for m in meshes)} simplices)")

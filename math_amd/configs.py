@@ -159,6 +159,33 @@ WORKLOADS = {
 }
 
 
+def c4_scenes(count=256):
+    """The batched workload's scenes: (n, g, screen edge, mesh seed) for each of
+    `count` scenes drawn with seed 300: n uniform in {3, 4, 5}; the cube-sphere
+    generator's cells per edge g (n=3: 8..59, 768..41,772 triangles; n=4: 3..15,
+    1,296..162,000 tetrahedra; n=5: 2..5, 3,840..150,000 4-simplices); screen
+    edges 64..256 (n=3) or 32..96 (n >= 4); mesh seed 300 + scene index."""
+    rng = np.random.default_rng(300)
+    out = []
+    for i in range(count):
+        n = int(rng.integers(3, 6))
+        if n == 3:
+            g, edge = int(rng.integers(8, 60)), int(rng.integers(64, 257))
+        elif n == 4:
+            g, edge = int(rng.integers(3, 16)), int(rng.integers(32, 97))
+        else:
+            g, edge = int(rng.integers(2, 6)), int(rng.integers(32, 97))
+        out.append((n, g, edge, 300 + i))
+    return out
+
+
+def c4_scene(n, g, edge, seed):
+    """Mesh (vertices, facets) and scene dict of one batched-workload scene: the
+    cube-sphere on S^(n-1) with radial N(0, 0.02) noise, point light at (2, .., 2)."""
+    v, f = native.meshgen_cube_sphere(n, g, True, 0.02, 0.0, seed)
+    return v, f, scene_dict(v, f, edge, tuple([2.0] * n))
+
+
 def hull_points(n, count, seed, on_sphere=False, zero=False):
     """Points for the convex hull: uniform in the unit n-ball, or on its sphere,
     as generate_random_data (geometry/test/test_convex_hull.cpp:41-73) draws

@@ -555,73 +555,261 @@ static int box_simplex_overlap(const orc_mesh* m, int64_t f, const double* org, 
     return 1;
 }
 
-static int64_t push_node(orc_mesh* m, int level, const int64_t* q)
+/* Node and entry arrays of a tree, or of one subtree built by a worker thread. */
+typedef struct {
+    onode* nodes;
+    int64_t n_nodes, cap_nodes;
+    int32_t* entries;
+    int64_t n_entries, cap_entries;
+} tbuf;
+
+static int64_t push_node(tbuf* B, int n, int level, const int64_t* q)
 {
-    if (m->n_nodes == m->cap_nodes) {
-        m->cap_nodes = m->cap_nodes ? m->cap_nodes * 2 : 1024;
-        m->nodes = realloc(m->nodes, sizeof(onode) * m->cap_nodes);
+    if (B->n_nodes == B->cap_nodes) {
+        B->cap_nodes = B->cap_nodes ? B->cap_nodes * 2 : 1024;
+        B->nodes = realloc(B->nodes, sizeof(onode) * B->cap_nodes);
     }
-    onode* nd = &m->nodes[m->n_nodes];
+    onode* nd = &B->nodes[B->n_nodes];
     nd->level = level;
-    memcpy(nd->q, q, sizeof(int64_t) * m->n);
+    memcpy(nd->q, q, sizeof(int64_t) * n);
     nd->child0 = -1;
     nd->begin = 0;
     nd->count = 0;
-    return m->n_nodes++;
+    return B->n_nodes++;
 }
 
-static void push_entries(orc_mesh* m, const int32_t* list, int64_t count, int64_t* begin)
+static void push_entries(tbuf* B, const int32_t* list, int64_t count, int64_t* begin)
 {
-    if (m->n_entries + count > m->cap_entries) {
-        while (m->n_entries + count > m->cap_entries) m->cap_entries = m->cap_entries ? m->cap_entries * 2 : 4096;
-        m->entries = realloc(m->entries, sizeof(int32_t) * m->cap_entries);
+    if (B->n_entries + count > B->cap_entries) {
+        while (B->n_entries + count > B->cap_entries) B->cap_entries = B->cap_entries ? B->cap_entries * 2 : 4096;
+        B->entries = realloc(B->entries, sizeof(int32_t) * B->cap_entries);
     }
-    *begin = m->n_entries;
-    memcpy(m->entries + m->n_entries, list, sizeof(int32_t) * count);
-    m->n_entries += count;
+    *begin = B->n_entries;
+    memcpy(B->entries + B->n_entries, list, sizeof(int32_t) * count);
+    B->n_entries += count;
+}
+
+/* The simplices of `list` that overlap child c of a node at (level, pq). */
+static int64_t child_list(const orc_mesh* m, int level, const int64_t* pq, int c, const int32_t* list,
+                          int64_t count, int32_t* out)
+{
+    int n = m->n;
+    int64_t q[NM];
+    for (int i = 0; i < n; ++i) q[i] = pq[i] * 2 + ((c >> i) & 1);
+    double org[NM], size[NM];
+    node_box(m, level + 1, q, org, size);
+    int64_t k = 0;
+    for (int64_t j = 0; j < count; ++j)
+        if (box_simplex_overlap(m, list[j], org, size)) out[k++] = list[j];
+    return k;
 }
 
 /* extend(), src/painter/space/tree.h:327-387, depth-first. */
-static void build_rec(orc_mesh* m, int64_t node, int depth, int32_t* list, int64_t count, int min_objects)
+static void build_rec(const orc_mesh* m, tbuf* B, int64_t node, int depth, const int32_t* list, int64_t count,
+                      int min_objects)
 {
     int n = m->n;
     if (depth >= m->max_depth || count <= min_objects) {
         int64_t b;
-        push_entries(m, list, count, &b);
-        m->nodes[node].begin = b;
-        m->nodes[node].count = count;
+        push_entries(B, list, count, &b);
+        B->nodes[node].begin = b;
+        B->nodes[node].count = count;
         return;
     }
-    int level = m->nodes[node].level;
+    int level = B->nodes[node].level;
     int64_t pq[NM];
-    memcpy(pq, m->nodes[node].q, sizeof(int64_t) * n);
+    memcpy(pq, B->nodes[node].q, sizeof(int64_t) * n);
     int nch = 1 << n;
     int64_t first = -1;
     for (int c = 0; c < nch; ++c) {
         int64_t q[NM];
         for (int i = 0; i < n; ++i) q[i] = pq[i] * 2 + ((c >> i) & 1);
-        int64_t id = push_node(m, level + 1, q);
+        int64_t id = push_node(B, n, level + 1, q);
         if (c == 0) first = id;
     }
-    m->nodes[node].child0 = (int)first;
+    B->nodes[node].child0 = (int)first;
     int32_t* sub = malloc(sizeof(int32_t) * (count ? count : 1));
     for (int c = 0; c < nch; ++c) {
-        int64_t id = first + c;
-        double org[NM], size[NM];
-        node_box(m, level + 1, m->nodes[id].q, org, size);
-        int64_t k = 0;
-        for (int64_t j = 0; j < count; ++j)
-            if (box_simplex_overlap(m, list[j], org, size)) sub[k++] = list[j];
+        int64_t k = child_list(m, level, pq, c, list, count, sub);
         int32_t* copy = malloc(sizeof(int32_t) * (k ? k : 1));
         memcpy(copy, sub, sizeof(int32_t) * k);
-        build_rec(m, id, depth + 1, copy, k, min_objects);
+        build_rec(m, B, first + c, depth + 1, copy, k, min_objects);
         free(copy);
     }
     free(sub);
 }
 
+/* The reference builds the tree with worker threads (BoxJobs, tree.h:229-325,
+ * 466-525).  Here the top levels are split on the calling thread (the
+ * children of one split filtered in parallel), and the subtrees below them are
+ * built by worker threads into buffers of their own and appended: the same
+ * boxes and leaf lists as a depth-first build, in another node order. */
+typedef struct {
+    int64_t node;  /* in the main buffer */
+    int level;
+    int64_t q[NM];
+    int depth;
+    int32_t* list;
+    int64_t count;
+    tbuf sub;      /* the subtree below node (its local node 0 is node) */
+} btask;
+
+typedef struct {
+    const orc_mesh* m;
+    btask* tasks;
+    int64_t ntasks;
+    int64_t* next;
+    int min_objects;
+    /* child filtering of one split */
+    int level;
+    const int64_t* pq;
+    const int32_t* list;
+    int64_t count;
+    int32_t** child;
+    int64_t* child_count;
+    int nch;
+} bjob;
+
+static void* split_worker(void* arg)
+{
+    bjob* j = arg;
+    for (;;) {
+        int64_t c = __atomic_fetch_add(j->next, 1, __ATOMIC_RELAXED);
+        if (c >= j->nch) break;
+        j->child[c] = malloc(sizeof(int32_t) * (j->count ? j->count : 1));
+        j->child_count[c] = child_list(j->m, j->level, j->pq, (int)c, j->list, j->count, j->child[c]);
+    }
+    return NULL;
+}
+
+static void* subtree_worker(void* arg)
+{
+    bjob* j = arg;
+    for (;;) {
+        int64_t t = __atomic_fetch_add(j->next, 1, __ATOMIC_RELAXED);
+        if (t >= j->ntasks) break;
+        btask* k = &j->tasks[t];
+        memset(&k->sub, 0, sizeof k->sub);
+        push_node(&k->sub, j->m->n, k->level, k->q);
+        build_rec(j->m, &k->sub, 0, k->depth, k->list, k->count, j->min_objects);
+    }
+    return NULL;
+}
+
+static void run_threads(int threads, void* (*fn)(void*), void* arg)
+{
+    pthread_t* th = calloc(threads, sizeof(pthread_t));
+    for (int i = 0; i < threads; ++i) pthread_create(&th[i], NULL, fn, arg);
+    for (int i = 0; i < threads; ++i) pthread_join(th[i], NULL);
+    free(th);
+}
+
+static void build_parallel(orc_mesh* m, int min_objects, int threads)
+{
+    const int n = m->n, nch = 1 << n;
+    tbuf M;
+    memset(&M, 0, sizeof M);
+    int64_t q0[NM] = {0};
+    push_node(&M, n, 0, q0);
+    btask* front = calloc(1, sizeof(btask));
+    int64_t nfront = 1;
+    front[0].node = 0;
+    front[0].level = 0;
+    front[0].depth = 1;
+    front[0].count = m->F;
+    front[0].list = malloc(sizeof(int32_t) * (m->F ? m->F : 1));
+    for (int64_t f = 0; f < m->F; ++f) front[0].list[f] = (int32_t)f;
+    /* split the top levels here until there are enough subtrees for the workers */
+    for (int round = 0; round < 3 && nfront < 8 * threads; ++round) {
+        btask* next = calloc(nfront * nch, sizeof(btask));
+        int64_t nnext = 0;
+        for (int64_t t = 0; t < nfront; ++t) {
+            btask* k = &front[t];
+            if (k->depth >= m->max_depth || k->count <= min_objects) {
+                int64_t b;
+                push_entries(&M, k->list, k->count, &b);
+                M.nodes[k->node].begin = b;
+                M.nodes[k->node].count = k->count;
+                free(k->list);
+                continue;
+            }
+            int64_t first = -1;
+            for (int c = 0; c < nch; ++c) {
+                int64_t q[NM];
+                for (int i = 0; i < n; ++i) q[i] = k->q[i] * 2 + ((c >> i) & 1);
+                int64_t id = push_node(&M, n, k->level + 1, q);
+                if (c == 0) first = id;
+            }
+            M.nodes[k->node].child0 = (int)first;
+            int32_t* child[1 << NM];
+            int64_t cc[1 << NM];
+            int64_t nx = 0;
+            bjob j = {m, NULL, 0, &nx, min_objects, k->level, k->q, k->list, k->count, child, cc, nch};
+            run_threads(threads < nch ? threads : nch, split_worker, &j);
+            for (int c = 0; c < nch; ++c) {
+                btask* d = &next[nnext++];
+                d->node = first + c;
+                d->level = k->level + 1;
+                memcpy(d->q, M.nodes[first + c].q, sizeof(int64_t) * n);
+                d->depth = k->depth + 1;
+                d->list = child[c];
+                d->count = cc[c];
+            }
+            free(k->list);
+        }
+        free(front);
+        front = next;
+        nfront = nnext;
+    }
+    /* the subtrees, on the workers */
+    int64_t nx = 0;
+    bjob j;
+    memset(&j, 0, sizeof j);
+    j.m = m;
+    j.tasks = front;
+    j.ntasks = nfront;
+    j.next = &nx;
+    j.min_objects = min_objects;
+    run_threads(threads, subtree_worker, &j);
+    /* append them in task order: node 0 of a subtree is its task's node */
+    for (int64_t t = 0; t < nfront; ++t) {
+        btask* k = &front[t];
+        const int64_t off_n = M.n_nodes - 1, off_e = M.n_entries;
+        onode* root = &M.nodes[k->node];
+        const onode* r = &k->sub.nodes[0];
+        root->child0 = r->child0 >= 0 ? (int)(r->child0 + off_n) : -1;
+        root->begin = r->begin + off_e;
+        root->count = r->count;
+        for (int64_t i = 1; i < k->sub.n_nodes; ++i) {
+            onode nd = k->sub.nodes[i];
+            if (nd.child0 >= 0) nd.child0 = (int)(nd.child0 + off_n);
+            else nd.begin += off_e;
+            int64_t id = push_node(&M, n, nd.level, nd.q);
+            M.nodes[id] = nd;
+        }
+        int64_t b;
+        push_entries(&M, k->sub.entries, k->sub.n_entries, &b);
+        free(k->sub.nodes);
+        free(k->sub.entries);
+        free(k->list);
+    }
+    free(front);
+    m->nodes = M.nodes;
+    m->n_nodes = M.n_nodes;
+    m->cap_nodes = M.cap_nodes;
+    m->entries = M.entries;
+    m->n_entries = M.n_entries;
+    m->cap_entries = M.cap_entries;
+}
+
 int orc_tree_build(orc_mesh* m, int max_depth, int min_objects)
 {
+    return orc_tree_build_threads(m, max_depth, min_objects, 1);
+}
+
+int orc_tree_build_threads(orc_mesh* m, int max_depth, int min_objects, int threads)
+{
+    if (threads < 1) return fail("tree build thread count (%d) must be positive", threads);
     int n = m->n;
     /* decompose(), tree.h:466-525 and min_max_and_distance(), tree.h:167-226 */
     if (!(max_depth >= 1 && max_depth <= 10) || !(min_objects >= 2 && min_objects <= 100))
@@ -675,12 +863,7 @@ int orc_tree_build(orc_mesh* m, int max_depth, int min_objects)
     }
     free(m->nodes); m->nodes = NULL; m->n_nodes = m->cap_nodes = 0;
     free(m->entries); m->entries = NULL; m->n_entries = m->cap_entries = 0;
-    int64_t q0[NM] = {0};
-    push_node(m, 0, q0);
-    int32_t* all = malloc(sizeof(int32_t) * m->F);
-    for (int64_t f = 0; f < m->F; ++f) all[f] = (int32_t)f;
-    build_rec(m, 0, 1, all, m->F, min_objects);
-    free(all);
+    build_parallel(m, min_objects, threads);
     m->has_tree = 1;
     return 0;
 }

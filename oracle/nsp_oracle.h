@@ -69,8 +69,12 @@ const char* orc_last_error(void);
 void orc_mesh_records(const orc_mesh* m, double* normal, double* plane_n, double* plane_d, double* vertex0);
 void orc_mesh_min_max(const orc_mesh* m, double* mn, double* mx);
 
-/* SpatialSubdivisionTree::decompose, src/painter/space/tree.h:466-525. */
+/* SpatialSubdivisionTree::decompose, src/painter/space/tree.h:466-525
+ * (orc_tree_build: one thread; orc_tree_build_threads: the subtrees below the
+ * top levels built by `threads` worker threads, as the reference's BoxJobs
+ * workers do; the same boxes and leaf lists in either case). */
 int orc_tree_build(orc_mesh* m, int max_depth, int min_objects);
+int orc_tree_build_threads(orc_mesh* m, int max_depth, int min_objects, int threads);
 /* node_count, leaf_count (non-empty + empty leaves), total leaf entries, max level */
 void orc_tree_info(const orc_mesh* m, int64_t* nodes, int64_t* leaves, int64_t* entries, int* depth);
 /* Sorted leaf lists keyed by the leaf's (level, integer cell coordinates): for

@@ -72,6 +72,7 @@ def lib():
         L.orc_mesh_min_max.argtypes = [P, dp, dp]
         L.orc_mesh_min_max.restype = None
         L.orc_tree_build.argtypes = [P, C.c_int, C.c_int]
+        L.orc_tree_build_threads.argtypes = [P, C.c_int, C.c_int, C.c_int]
         L.orc_tree_info.argtypes = [P, i64p, i64p, i64p, C.POINTER(C.c_int)]
         L.orc_tree_info.restype = None
         L.orc_tree_dump.argtypes = [P, i64p, C.c_int64]
@@ -175,10 +176,13 @@ class Mesh:
         lib().orc_mesh_min_max(self.h, _p(mn, C.c_double), _p(mx, C.c_double))
         return mn, mx
 
-    def build_tree(self, max_depth=None, min_objects=10):
+    def build_tree(self, max_depth=None, min_objects=10, threads=None):
+        """threads: worker threads of the build (default: the CPUs this process may run on)."""
         if max_depth is None:
             max_depth = {3: 10, 4: 8, 5: 6, 6: 5}[self.n]
-        rc = lib().orc_tree_build(self.h, max_depth, min_objects)
+        if threads is None:
+            threads = len(os.sched_getaffinity(0))
+        rc = lib().orc_tree_build_threads(self.h, max_depth, min_objects, threads)
         if rc:
             raise ValueError(lib().orc_last_error().decode())
 

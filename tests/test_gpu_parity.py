@@ -130,10 +130,11 @@ def small_scene(n=4, g=4, edge=10, spp=4, depth=3):
     return v, f, cfg, spp, depth
 
 
-@pytest.mark.parametrize("n", [3, 4, 5])
+@pytest.mark.parametrize("n", [3, 4, 5, 6])
 def test_render_fp64_matches_checker(ctx, n):
-    g = {3: 8, 4: 4, 5: 2}[n]
-    edge = {3: 24, 4: 10, 5: 5}[n]
+    # paint<N,T> is instantiated for N = 3..6 (painter.cpp:627-643)
+    g = {3: 8, 4: 4, 5: 2, 6: 1}[n]
+    edge = {3: 24, 4: 10, 5: 5, 6: 4}[n]
     v, f, cfg, spp, depth = small_scene(n, g, edge)
     print(f"render fp64 n={n}", flush=True)
     gm = ctx.mesh(v, f)
@@ -153,10 +154,10 @@ def test_render_fp64_matches_checker(ctx, n):
     assert abs(st["rays"] - rays) <= 0.001 * rays
 
 
-@pytest.mark.parametrize("n", [3, 4])
+@pytest.mark.parametrize("n", [3, 4, 5, 6])
 def test_render_fp32_matches_checker(ctx, n):
-    g = {3: 8, 4: 4}[n]
-    edge = {3: 32, 4: 12}[n]
+    g = {3: 8, 4: 4, 5: 2, 6: 1}[n]
+    edge = {3: 32, 4: 12, 5: 6, 6: 4}[n]
     v, f, cfg, spp, depth = small_scene(n, g, edge, spp=16)
     gm = ctx.mesh(v, f)
     sc = native.Scene(ctx, gm, cfg)

@@ -1,0 +1,221 @@
+"""Parity at the benchmark configurations' full sizes, through the kernels the
+benchmark times (one nsp_paint pass with its defaults: rasterised primary
+visibility, persistent closest-hit and shadow walks with their deferred FP64
+decisions and re-walks, one wavefront batch per pass), against the FP64
+checker (orc_render, painter.cpp:410-448 restated) on the same counter-based
+sample streams.
+
+  C0  n=3 torus, 40,000 triangles, 512^2, 16 spp, depth 5
+  C1  n=4 noisy S^3, 663,552 tetrahedra, 128^3, 27 samples/pixel, depth 5
+  C2  n=5 S^4 with a cap removed, 2,388,785 4-simplices, 48^4, 16 samples/pixel, depth 4
+  C4  the 256 seeded scenes of the batched workload (n = 3, 4, 5) through one
+      batched launch, each scene's image bit-equal to its own nsp_paint render
+
+The checker renders a seeded sample of pixels spread over the whole screen.
+Bars:
+  * first hit (nsp_trace_first_hit against the checker's brute force over all
+    simplices): indices equal outside the near-boundary band, >= 95 % of rays
+    outside it, >= 99.9 % equal overall, t within 1e-5 relative;
+  * FP64 render (every ray in FP64): per-pixel mean RMSE <= 1e-4 against the
+    checker (the paths are the same; only FP64 rounding of the hardware
+    differs);
+  * FP32 render (the benchmark's precision): per-pixel mean RMSE <= 1e-2 and
+    |mean difference| <= 1e-3 against the FP64 checker.  Bounce origins move
+    1000 FP32 epsilons off the surface (compute_ray_offset, painter.cpp:539-558),
+    so a small share of paths take another branch than in FP64; those give
+    noise, not bias.
+Measured values are written to the JSON file named by NSP_PARITY_REPORT, when
+set, one entry per configuration.
+"""
+import concurrent.futures
+import json
+import os
+import time
+
+import numpy as np
+import pytest
+
+from math_amd import configs, native
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SEED = 4242
+THREADS = len(os.sched_getaffinity(0))
+
+
+def record(config, **fields):
+    path = os.environ.get("NSP_PARITY_REPORT")
+    print(f"parity {config}: {json.dumps(fields)}", flush=True)
+    if not path:
+        return
+    data = {}
+    if os.path.exists(path):
+        with open(path) as fh:
+            data = json.load(fh)
+    data.setdefault(config, {}).update(fields)
+    with open(path, "w") as fh:
+        json.dump(data, fh, indent=1, sort_keys=True)
+
+
+def brute_force(om, org, d):
+    """The checker's nearest hit over all simplices, the rays split over threads."""
+    parts = np.array_split(np.arange(len(org)), THREADS)
+    with concurrent.futures.ThreadPoolExecutor(THREADS) as ex:
+        res = list(ex.map(lambda ix: om.trace_brute(org[ix], d[ix], eps_bary=1e-9, rel_t=1e-9), parts))
+    return tuple(np.concatenate([r[k] for r in res]) for k in range(3))
+
+
+def rays_toward(n, K, seed, radius=3.0, spread=0.6):
+    rng = np.random.default_rng(seed)
+    org = rng.normal(size=(K, n))
+    org = org / np.linalg.norm(org, axis=1, keepdims=True) * radius
+    d = rng.normal(size=(K, n)) * spread - org
+    return org.astype(np.float32), d.astype(np.float32)
+
+
+def first_hit_parity(config, gm, om, n, K, seed):
+    org, d = rays_toward(n, K, seed)
+    t0 = time.perf_counter()
+    gi, gt, gnear = gm.first_hit(org, d)
+    oi, ot, onear = brute_force(om, org, d)
+    ok = (gnear == 0) & (onear == 0)
+    h = (gi == oi) & (oi >= 0)
+    mism = int(np.sum(gi[ok] != oi[ok]))
+    rel = float(np.max(np.abs(gt[h] - ot[h]) / ot[h])) if h.any() else 0.0
+    record(config, first_hit_rays=K, first_hit_outside_band=float(ok.mean()), first_hit_mismatch_outside_band=mism,
+           first_hit_equal=float(np.mean(gi == oi)), first_hit_max_t_rel_err=rel, first_hit_hits=int(h.sum()),
+           first_hit_seconds=round(time.perf_counter() - t0, 2))
+    assert ok.mean() >= 0.95
+    assert mism == 0
+    assert np.mean(gi == oi) >= 0.999
+    assert h.sum() > K // 10
+    assert rel <= 1e-5
+
+
+def sample_pixels(P, count, seed):
+    rng = np.random.default_rng(seed)
+    return np.sort(rng.choice(P, size=min(count, P), replace=False)).astype(np.int64)
+
+
+def render_parity(config, ctx, gm, om, cfg, spp, depth, count):
+    """One benchmark-default pass (FP32) and one FP64 pass of the whole screen
+    on the GPU; the checker on `count` pixels spread over the screen."""
+    P = int(np.prod(cfg["screen"]))
+    px = sample_pixels(P, count, SEED)
+    t0 = time.perf_counter()
+    ref, rays = O.render_pixels(om, O.make_scene(cfg, spp, depth, SEED), px, threads=THREADS)
+    t_cpu = time.perf_counter() - t0
+    out = {}
+    for name, precision in (("fp32", native.PRECISION_FP32), ("fp64", native.PRECISION_FP64)):
+        sc = native.Scene(ctx, gm, cfg)
+        t0 = time.perf_counter()
+        sc.paint(spp, depth, precision=precision, seed=SEED)
+        t_gpu = time.perf_counter() - t0
+        img, cnt = sc.image()
+        sc.close()
+        assert np.all(cnt[px] == cnt[px[0]]) and cnt[px[0]] > 0
+        g = img[px] / cnt[px][:, None]
+        o = ref / cnt[px][:, None]
+        diff = g.astype(np.float64) - o
+        out[name] = dict(rmse=float(np.sqrt(np.mean(diff ** 2))), mean_diff=float(diff.mean()),
+                         max_abs=float(np.abs(diff).max()), pixels_within_1e_3=float(np.mean(np.abs(diff).max(1) < 1e-3)),
+                         gpu_pass_seconds=round(t_gpu, 3))
+    record(config, render_pixels=int(len(px)), render_samples_per_pixel=int(cnt[px[0]]),
+           checker_rays=int(rays), checker_seconds=round(t_cpu, 2), render_fp32=out["fp32"], render_fp64=out["fp64"])
+    assert out["fp64"]["rmse"] <= 1e-4
+    assert out["fp32"]["rmse"] <= 1e-2 and abs(out["fp32"]["mean_diff"]) <= 1e-3
+
+
+@pytest.mark.parametrize("config,hit_rays,pixels", [("C0", 4000, 6000), ("C1", 4000, 6000), ("C2", 1500, 2000)])
+def test_full_size_parity(ctx, config, hit_rays, pixels):
+    v, f, cfg, spp, depth = configs.config(config)
+    print(f"{config}: F={len(f)} screen={cfg['screen']} spp={spp} depth={depth}", flush=True)
+    gm = ctx.mesh(v, f)
+    om = O.Mesh(v, f)
+    t0 = time.perf_counter()
+    om.build_tree()
+    print(f"{config}: checker tree {time.perf_counter() - t0:.1f} s", flush=True)
+    first_hit_parity(config, gm, om, cfg["n"], hit_rays, 17)
+    render_parity(config, ctx, gm, om, cfg, spp, depth, pixels)
+
+
+def test_batched_256_scenes_equal_per_scene(ctx):
+    """All 256 scenes of the batched workload in one batched launch (captured
+    hipGraph, concurrent lanes), 1 sample per pixel (the workload's 4 spp is
+    35G paths per leg, 2.2G at 1), depth 5: every scene's image, sample counts
+    and ray count equal its own nsp_paint render bit for bit; three scenes (one
+    per dimension) checked against the checker on a pixel sample."""
+    spec = configs.c4_scenes(256)
+    meshes, cfgs, scenes = [], [], []
+    t0 = time.perf_counter()
+    for n, g, edge, seed in spec:
+        v, f, cfg = configs.c4_scene(n, g, edge, seed)
+        meshes.append((ctx.mesh(v, f), v, f))
+        cfgs.append(cfg)
+    print(f"256 meshes in {time.perf_counter() - t0:.1f} s", flush=True)
+    scenes = [native.Scene(ctx, m, c) for (m, _, _), c in zip(meshes, cfgs)]
+    batch = native.Batch(ctx, scenes)
+    t0 = time.perf_counter()
+    batch.paint(1, 5, seed=SEED, use_graph=True)
+    print(f"batched pass {time.perf_counter() - t0:.1f} s, {batch.statistics()['rays']} rays", flush=True)
+    equal = 0
+    checked = {}
+    for i, ((m, v, f), cfg) in enumerate(zip(meshes, cfgs)):
+        a_img, a_cnt = scenes[i].image()
+        a_rays = scenes[i].statistics()["rays"]
+        scenes[i].close()
+        one = native.Scene(ctx, m, cfg)
+        one.paint(1, 5, seed=SEED)
+        b_img, b_cnt = one.image()
+        b_rays = one.statistics()["rays"]
+        one.close()
+        same = np.array_equal(a_img, b_img) and np.array_equal(a_cnt, b_cnt) and a_rays == b_rays
+        assert same, f"scene {i} (n={spec[i][0]}) differs"
+        equal += same
+        n = spec[i][0]
+        if n not in checked:
+            om = O.Mesh(v, f)
+            om.build_tree()
+            px = sample_pixels(len(a_cnt), 3000, SEED + i)
+            ref, _ = O.render_pixels(om, O.make_scene(cfg, 1, 5, SEED), px, threads=THREADS)
+            diff = a_img[px].astype(np.float64) - ref
+            checked[n] = dict(scene=i, pixels=int(len(px)), rmse=float(np.sqrt(np.mean(diff ** 2))),
+                              mean_diff=float(diff.mean()))
+            print(f"scene {i} n={n} vs checker: {checked[n]}", flush=True)
+        m.close()
+    record("C4", scenes=256, samples_per_pixel=1, depth=5, scenes_bit_equal_to_per_scene=int(equal),
+           checker_fp32_vs_fp64=checked)
+    assert equal == 256
+    for r in checked.values():
+        # one sample per pixel: single FP32 paths against FP64 paths
+        assert r["rmse"] <= 3e-2 and abs(r["mean_diff"]) <= 3e-3
+
+
+@pytest.mark.parametrize("precision", [native.PRECISION_FP64, native.PRECISION_FP32])
+def test_progressive_pass_images_match_checker(ctx, precision):
+    """The image after passes 1, 2 and 3 of one progressive paint (one call,
+    passes = 3, a graph replayed across the passes) equals the checker's renders
+    with pass_index 0, 1, 2 summed: the pass index drives the sample streams as
+    PainterWindow's repeated passes do (painter_window.cpp:172-187)."""
+    v, f = native.meshgen_cube_sphere(4, 4, True, 0.02, 0.0, 1)
+    cfg = configs.scene_dict(v, f, 10, (2.0,) * 4)
+    gm = ctx.mesh(v, f)
+    om = O.Mesh(v, f)
+    om.build_tree()
+    sc = native.Scene(ctx, gm, cfg)
+    images = []
+    sc.paint(4, 3, precision=precision, seed=21, passes=3, use_graph=True,
+             on_pass=lambda k, s: images.append(sc.image()))
+    assert len(images) == 3
+    acc = None
+    for k in range(3):
+        ref, _ = O.render(om, O.make_scene(cfg, 4, 3, 21, pass_index=k), threads=THREADS)
+        acc = ref if acc is None else acc + ref
+        img, cnt = images[k]
+        assert np.all(cnt == 8 * (k + 1))
+        rmse = float(np.sqrt(np.mean((img / cnt[:, None] - acc / cnt[:, None]) ** 2)))
+        print(f"precision {precision} after pass {k + 1}: rmse {rmse:.3e}", flush=True)
+        assert rmse < (1e-4 if precision == native.PRECISION_FP64 else 5e-3)
+        if k > 0:  # later passes add new samples, not the first pass again
+            assert not np.allclose(img, images[0][0] * (k + 1))

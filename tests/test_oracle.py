@@ -345,3 +345,30 @@ def test_lights_add_linearly(n):
     # each light's shadow ray counts once: the ray counts add the same way
     assert rab == ra + rb - rz
     assert not np.allclose(a, z) and not np.allclose(b, z)
+
+
+@pytest.mark.parametrize("n,g", [(3, 12), (4, 4), (5, 2)])
+def test_tree_build_threads_equal_one_thread(n, g):
+    # the subtrees built by worker threads give the one-thread tree: same boxes,
+    # leaf lists and traced hits (only the node numbering differs)
+    v, f = _mesh(n, g)
+    a, b = O.Mesh(v, f), O.Mesh(v, f)
+    a.build_tree(threads=1)
+    b.build_tree(threads=6)
+    assert a.tree_info() == b.tree_info()
+
+    def leaves(m):
+        d, out, k = m.tree_dump(), {}, 0
+        while k < len(d):
+            c = int(d[k + 1 + n])
+            out[tuple(int(x) for x in d[k:k + 1 + n])] = d[k + 2 + n:k + 2 + n + c].tolist()
+            k += 2 + n + c
+        return out
+    assert leaves(a) == leaves(b)
+    rng = np.random.default_rng(n)
+    org = rng.normal(size=(500, n))
+    org = (org / np.linalg.norm(org, axis=1, keepdims=True) * 3).astype(np.float32)
+    d = (rng.normal(size=(500, n)) * 0.5 - org).astype(np.float32)
+    ia, ta = a.trace_tree(org, d)
+    ib, tb = b.trace_tree(org, d)
+    assert np.array_equal(ia, ib) and np.array_equal(ta, tb)
