@@ -207,6 +207,19 @@ int nsp_trace_first_hit(nsp_mesh* mesh, const float* org, const float* dir, int6
  * asynchronous. */
 int nsp_trace_first_hit_device(nsp_mesh* mesh, const float* org_device, const float* dir_device, int64_t count,
                                int32_t* facet_device, double* t_device, uint8_t* near_device, void* hip_stream);
+/* nsp_trace_first_hit without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code that
+ * must not call hipMalloc, tight serving loops, harnesses that forbid it).
+ * Same arguments, results and errors as nsp_trace_first_hit (host arrays in
+ * and out, same kernel); the rays and hits are staged through a fixed 64 MiB
+ * array in device memory, one per device, set up by nsp_context_create and
+ * held until the process ends.  At most NSP_NOALLOC_MAX_RAYS rays per call
+ * (more: NSP_E_INVALID, nothing done).  Every call on one device shares that
+ * device's array: calls that overlap on one GPU (several contexts or threads
+ * on the same device) race and must be serialised by the caller. */
+#define NSP_NOALLOC_MAX_RAYS (1 << 20)
+int nsp_trace_first_hit_noalloc(nsp_mesh* mesh, const float* org, const float* dir, int64_t count, int32_t* facet,
+                                double* t, uint8_t* near);
 
 /* Scene description: the data behind PaintObjects<N,T> (src/painter/objects.h:212-223). */
 #define NSP_LIGHT_POINT 0    /* PointLight, src/painter/lights/light_source.h:26-57 */

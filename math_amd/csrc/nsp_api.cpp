@@ -77,8 +77,25 @@ static int one_dimension_size(int dims, int spp)
 
 using namespace nsp;
 
+// Device staging of nsp_trace_first_hit_noalloc (rays in, hits out) for at most
+// NSP_NOALLOC_MAX_RAYS rays of up to NSP_MAX_DIM coordinates: org and dir
+// (2 x 24 MiB), t (8 MiB), facet (4 MiB), near (1 MiB), each region 256-byte
+// aligned.  One copy per device, held until the process ends; in host code the
+// array's name is not a device address, so kernels and copies take the address
+// hipGetSymbolAddress gives (nsp_context::scratch).
+static constexpr int64_t NOALLOC_SCRATCH_BYTES = (int64_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+static constexpr int64_t align256(int64_t b) { return (b + 255) & ~(int64_t)255; }
+static_assert(2 * align256((int64_t)sizeof(float) * NSP_NOALLOC_MAX_RAYS * NSP_MAX_DIM) +
+                      align256((int64_t)sizeof(double) * NSP_NOALLOC_MAX_RAYS) +
+                      align256((int64_t)sizeof(int32_t) * NSP_NOALLOC_MAX_RAYS) + align256(NSP_NOALLOC_MAX_RAYS) <=
+                  NOALLOC_SCRATCH_BYTES,
+              "the no-allocation staging must hold NSP_NOALLOC_MAX_RAYS rays");
+
 struct nsp_context {
     int device = 0;
+    unsigned char* scratch = nullptr;  // this device's copy of `scratch` (hipGetSymbolAddress)
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;  // the shadow stages of nsp_paint (joined back before each batch ends)
     std::string error;
@@ -258,6 +275,11 @@ int nsp_context_create(int device, nsp_context** out)
         ctx->device = device;
         NSP_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         NSP_HIP(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+        // the device address of this device's staging array, taken once here
+        // with the device current
+        void* sp = nullptr;
+        NSP_HIP(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+        ctx->scratch = static_cast<unsigned char*>(sp);
         *out = ctx.release();
     });
 }
@@ -510,6 +532,46 @@ int nsp_trace_first_hit(nsp_mesh* mesh, const float* org, const float* dir, int6
         NSP_HIP(hipMemcpyAsync(facet, d_i.p, sizeof(int32_t) * count, hipMemcpyDeviceToHost, s));
         NSP_HIP(hipMemcpyAsync(t, d_t.p, sizeof(double) * count, hipMemcpyDeviceToHost, s));
         if (near) NSP_HIP(hipMemcpyAsync(near, d_n.p, count, hipMemcpyDeviceToHost, s));
+        NSP_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int nsp_trace_first_hit_noalloc(nsp_mesh* mesh, const float* org, const float* dir, int64_t count, int32_t* facet,
+                                double* t, uint8_t* near)
+{
+    if (!mesh || count < 0) return NSP_E_INVALID;
+    if (count == 0) return NSP_OK;
+    if (!org || !dir || !facet || !t) return NSP_E_INVALID;
+    nsp_context* ctx = mesh->ctx;
+    if (count > NSP_NOALLOC_MAX_RAYS) {
+        ctx->error = "nsp_trace_first_hit_noalloc takes at most " + std::to_string(NSP_NOALLOC_MAX_RAYS) +
+                     " rays per call (" + std::to_string(count) + " given)";
+        return NSP_E_INVALID;
+    }
+    return guarded(ctx, [&] {
+        launch_gate(ctx);
+        if (!ctx->scratch) raise(NSP_E_HIP, "the device staging array has no address");
+        const int n = mesh->m.n;
+        hipStream_t s = ctx->stream;
+        // regions of the staging array sized for this call; the kernel writes
+        // every facet, t and near entry below count, so nothing is cleared
+        unsigned char* base = ctx->scratch;
+        int64_t off = 0;
+        float* d_o = reinterpret_cast<float*>(base + off);
+        off += align256((int64_t)sizeof(float) * count * n);
+        float* d_d = reinterpret_cast<float*>(base + off);
+        off += align256((int64_t)sizeof(float) * count * n);
+        double* d_t = reinterpret_cast<double*>(base + off);
+        off += align256((int64_t)sizeof(double) * count);
+        int32_t* d_i = reinterpret_cast<int32_t*>(base + off);
+        off += align256((int64_t)sizeof(int32_t) * count);
+        uint8_t* d_n = base + off;
+        NSP_HIP(hipMemcpyAsync(d_o, org, sizeof(float) * count * n, hipMemcpyHostToDevice, s));
+        NSP_HIP(hipMemcpyAsync(d_d, dir, sizeof(float) * count * n, hipMemcpyHostToDevice, s));
+        launch_first_hit(mesh->m, d_o, d_d, count, d_i, d_t, d_n, nullptr, s);
+        NSP_HIP(hipMemcpyAsync(facet, d_i, sizeof(int32_t) * count, hipMemcpyDeviceToHost, s));
+        NSP_HIP(hipMemcpyAsync(t, d_t, sizeof(double) * count, hipMemcpyDeviceToHost, s));
+        if (near) NSP_HIP(hipMemcpyAsync(near, d_n, count, hipMemcpyDeviceToHost, s));
         NSP_HIP(hipStreamSynchronize(s));
     });
 }

@@ -115,7 +115,8 @@ class StageStats(C.Structure):
 EXPORTS = [
     "nsp_context_create", "nsp_context_destroy", "nsp_last_error", "nsp_debug_fail_next_launch",
     "nsp_mesh_create", "nsp_mesh_destroy", "nsp_mesh_min_max", "nsp_mesh_get_info", "nsp_mesh_records",
-    "nsp_mesh_tree_dump", "nsp_trace_first_hit", "nsp_trace_first_hit_device", "nsp_scene_create",
+    "nsp_mesh_tree_dump", "nsp_trace_first_hit", "nsp_trace_first_hit_device", "nsp_trace_first_hit_noalloc",
+    "nsp_scene_create",
     "nsp_scene_destroy", "nsp_paint", "nsp_statistics", "nsp_image_get", "nsp_image_set", "nsp_image_aov",
     "nsp_scene_pixel_count", "nsp_last_pass_stats", "nsp_set_counters", "nsp_meshgen_torus",
     "nsp_meshgen_cube_sphere", "nsp_denoiser_create", "nsp_denoiser_create_head", "nsp_denoiser_destroy",
@@ -160,6 +161,7 @@ def lib():
         L.nsp_mesh_get_record_layout.argtypes = [P]
         L.nsp_trace_first_hit.argtypes = [P, fp, fp, C.c_int64, ip, dp, C.POINTER(C.c_uint8)]
         L.nsp_trace_first_hit_device.argtypes = [P, P, P, C.c_int64, P, P, P, P]
+        L.nsp_trace_first_hit_noalloc.argtypes = [P, fp, fp, C.c_int64, ip, dp, C.POINTER(C.c_uint8)]
         L.nsp_scene_create.argtypes = [P, C.c_int, P, C.POINTER(Surface), C.POINTER(Light), C.c_int,
                                        C.POINTER(Projector), C.POINTER(Surface), C.POINTER(P)]
         L.nsp_scene_destroy.argtypes = [P]
@@ -466,6 +468,23 @@ class Mesh:
                                                        _ptr(idx, C.c_int32), _ptr(t, C.c_double),
                                                        _ptr(near, C.c_uint8)))
         return idx, t, near
+
+    def first_hit_noalloc(self, org, dir):
+        """first_hit through nsp_trace_first_hit_noalloc (no device allocation;
+        at most NOALLOC_MAX_RAYS rays)."""
+        o = np.ascontiguousarray(org, np.float32)
+        d = np.ascontiguousarray(dir, np.float32)
+        cnt = o.shape[0]
+        idx = np.zeros(cnt, np.int32)
+        t = np.zeros(cnt, np.float64)
+        near = np.zeros(cnt, np.uint8)
+        self.ctx.check(self.ctx._L.nsp_trace_first_hit_noalloc(self.h, _ptr(o, C.c_float), _ptr(d, C.c_float), cnt,
+                                                               _ptr(idx, C.c_int32), _ptr(t, C.c_double),
+                                                               _ptr(near, C.c_uint8)))
+        return idx, t, near
+
+
+NOALLOC_MAX_RAYS = 1 << 20
 
 
 class Scene:
