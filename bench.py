@@ -8,10 +8,12 @@ depth 5, FP32 tracing with FP64 near-boundary re-test.  One step = one full
 render pass of the screen.  Rays are counted by the reference's rules
 (Paintbrush statistics): every trace_path call and every shadow ray.
 
-The JSON line carries the closest-hit kernel's roofline (algorithmic bytes per
-launch from in-kernel traversal counters / its average launch time from HIP
-events on the render stream) and a CPU baseline (the FP64 CPU restatement on
-a bounded crop of the same workload, all host threads).
+The JSON line carries the roofline of the traversal kernel with the most time
+per step (algorithmic bytes per launch from in-kernel traversal counters / its
+average launch time from HIP events on the stream it runs on), a CPU baseline
+(the FP64 CPU restatement on a seeded sample of pixels spread over the whole
+screen, on every CPU the process may run on) and a parity object (one GPU pass
+against the CPU restatement on those pixels and sample streams).
 """
 import argparse
 import json
@@ -51,52 +53,65 @@ def shadow_bytes(st, n):
 def profiled_traffic(config, kernel, layout):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE
     / WRITE_SIZE summary of the same workload and record layout (scaled by
-    the copy-kernel calibration stated there), or None."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_{config.lower()}.md")
+    the copy-kernel calibration stated there) and that source, or (None, None)."""
+    name = f"{kernel}_{config.lower()}.md"
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
-        return None
+        return None, None
     text = open(path).read()
     if f"Record layout: {layout}" not in text:
-        return None
+        return None, None
     for line in text.splitlines():
         if line.startswith("traffic_bytes_per_launch:"):
-            return int(line.split(":")[1])
-    return None
+            return int(line.split(":")[1]), (f"profiles/{name}: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of the "
+                                             f"same workload, copy-calibrated (not measured in this run)")
+    return None, None
 
 
-def cpu_baseline(v, f, cfg, spp, depth, seconds_target=15.0):
-    """FP64 CPU restatement on a crop of the same screen, all host threads."""
+def cpu_baseline(ctx, mesh, v, f, cfg, spp, depth, seconds_target=15.0):
+    """The FP64 CPU restatement on every CPU this process may run on, timed on a
+    seeded sample of pixels spread over the whole screen, with all their
+    samples; the sample grows until its render takes about seconds_target.
+    The same pixels of one GPU pass with the same sample streams (outside the
+    timed region) are compared with the checker's: the `parity` object."""
     import numpy as np
 
+    from math_amd import native
     from oracle import oracle as O
 
-    threads = min(os.cpu_count() or 1, 16)
-    log(f"cpu baseline: building the checker's tree ({len(f)} simplices)")
+    threads = len(os.sched_getaffinity(0))
+    log(f"cpu baseline: building the checker's tree ({len(f)} simplices, {threads} threads)")
     m = O.Mesh(v, f)
-    m.build_tree()
-    sc = O.make_scene(cfg, spp, depth, 12345)
-    screen = cfg["screen"]
-    # crop: consecutive pixels around the middle of the screen, grown until ~seconds_target
-    total = int(np.prod(screen))
-    mid = total // 2 + screen[0] // 2
-    count = 64 * threads
+    m.build_tree(threads=threads)
+    seed = 12345
+    sc = O.make_scene(cfg, spp, depth, seed)
+    total = int(np.prod(cfg["screen"]))
+    order = np.random.default_rng(seed).permutation(total)
+    count = 256 * threads
     while True:
+        px = np.sort(order[:count])
         t0 = time.perf_counter()
-        _, rays = O.render(m, sc, mid, count, threads)
+        ref, rays = O.render_pixels(m, sc, px, threads)
         dt = time.perf_counter() - t0
-        if dt > seconds_target / 4 or count >= total // 8:
+        if dt > seconds_target / 3 or count >= total:
             break
-        count *= 4
-    if dt < seconds_target:
-        count2 = min(int(count * seconds_target / max(dt, 1e-3)), total)
-        start = max(0, min(mid - count2 // 2, total - count2))
-        t0 = time.perf_counter()
-        _, rays = O.render(m, sc, start, count2, threads)
-        dt = time.perf_counter() - t0
-        count = count2
-    return dict(value=rays / dt / 1e6, unit="Mrays/s", cores=threads, kind="port",
-                sample=f"{count} of {total} pixels ({screen}), all samples, from the middle of the screen, FP64, "
-                       f"{rays} rays in {dt:.1f} s")
+        count = min(total, int(count * min(8.0, max(1.5, seconds_target / 2.5 / max(dt, 1e-3)))))
+    cpu = dict(value=rays / dt / 1e6, unit="Mrays/s", cores=threads, kind="port",
+               sample=f"{count} of {total} pixels ({cfg['screen']}) drawn uniformly over the screen (seed {seed}), "
+                      f"all samples, FP64, {rays} rays in {dt:.1f} s on {threads} threads "
+                      f"(the host's CPUs available to this process: {threads}; os.cpu_count() {os.cpu_count()})")
+    # parity: one GPU pass of the benchmark's defaults on the same sample streams
+    gs = native.Scene(ctx, mesh, cfg)
+    gs.paint(spp, depth, seed=seed)
+    img, cnt = gs.image()
+    gs.close()
+    g = img[px] / cnt[px][:, None]
+    o = ref / cnt[px][:, None]
+    diff = g.astype(np.float64) - o
+    parity = dict(checker="FP64 CPU restatement, same pixels and sample streams", pixels=int(count),
+                  rmse=float(np.sqrt(np.mean(diff ** 2))), mean_diff=float(diff.mean()),
+                  max_abs=float(np.abs(diff).max()), bar_rmse=1e-2)
+    return cpu, parity
 
 
 def denoise_bench(args):
@@ -428,8 +443,9 @@ def main():
     def line(name):
         k = kernels[name]
         achieved = k["bytes"] / (k["ms"] / args.steps * 1e-3) / 1e9
+        traffic, source = profiled_traffic(args.config, name, layout_name)
         return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=profiled_traffic(args.config, name, layout_name),
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, traffic_source=source,
                     kernel=name, launches_per_step=k["launches"] // max(args.steps, 1),
                     avg_launch_ms=round(k["ms"] / max(k["launches"], 1), 4),
                     bytes_per_launch=int(k["bytes"] / max(k["launches"] // max(args.steps, 1), 1)),
@@ -449,9 +465,9 @@ def main():
     else:
         roofline = None  # hipGraph replay: no per-stage events inside the graph
         log("stage times are not measured inside a captured graph: no roofline line")
-    cpu = None
+    cpu = parity = None
     if not args.no_cpu:
-        cpu = cpu_baseline(v, f, cfg, spp, depth)
+        cpu, parity = cpu_baseline(ctx, mesh, v, f, cfg, spp, depth)
     out = {
         "metric": METRIC,
         "value": round(mrays, 2),
@@ -471,6 +487,7 @@ def main():
                    "tree": dict(nodes=info.tree_nodes, leaves=info.tree_leaves, entries=info.tree_entries)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "parity": parity,
     }
     print(json.dumps(out), flush=True)
 

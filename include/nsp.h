@@ -439,6 +439,14 @@ void nsp_denoiser_destroy(nsp_denoiser* d);
 int64_t nsp_denoiser_param_count(const nsp_denoiser* d);
 int nsp_denoiser_get_params(const nsp_denoiser* d, float* out);
 int nsp_denoiser_set_params(nsp_denoiser* d, const float* in);
+/* Training checkpoint (no reference counterpart): the parameters, Adam's
+ * first and second moment estimates (param_count floats each; any pointer may
+ * be NULL) and its step count.  The patches of a training step are drawn by
+ * (seed, step count), so training resumed from a checkpoint on another
+ * denoiser of the same shape continues exactly as the unbroken run. */
+int nsp_denoiser_get_state(const nsp_denoiser* d, float* params, float* adam_m, float* adam_v, int64_t* adam_t);
+int nsp_denoiser_set_state(nsp_denoiser* d, const float* params, const float* adam_m, const float* adam_v,
+                           int64_t adam_t);
 /* Denoised radiance output[nb*depth*height*width*3]. */
 int nsp_denoiser_forward(nsp_denoiser* d, const float* input, int nb, int depth, int height, int width,
                          float* output);
@@ -447,7 +455,9 @@ int nsp_denoiser_forward(nsp_denoiser* d, const float* input, int nb, int depth,
 int nsp_denoiser_loss_grad(nsp_denoiser* d, const float* input, const float* target, int nb, int depth, int height,
                            int width, float* loss, float* grad);
 /* Adam training on random patch^3 crops (batch per step) of `volumes`
- * input/target volumes uploaded once; losses[steps] receives each step's loss. */
+ * input/target volumes uploaded once; losses[steps] receives each step's loss
+ * (copied once at the end: the steps run without a host wait).  The crops of a
+ * step are drawn from (seed, the denoiser's Adam step count). */
 int nsp_denoiser_train(nsp_denoiser* d, const float* inputs, const float* targets, int volumes, int depth, int height,
                        int width, int patch, int batch, int steps, float lr, uint64_t seed, float* losses);
 /* One BF16 implicit-GEMM 3x3x3 convolution (zero padding): x [V][cin],

@@ -87,193 +87,306 @@ __device__ __forceinline__ int64_t neighbour(const Vox& p, int tap, const Dims& 
 
 // ---------------------------------------------------------------------------
 // Implicit-GEMM convolution (forward, and backward-data with flipped weights).
+// A block computes a 16 (x) x 8 (y) x 4 (z) tile of output voxels.  Its input
+// halo, 18 x 10 x 6 voxels x CIN channels, is staged once in LDS and read there
+// for all 27 taps.  Wave w takes the z-slice w: eight M-tiles (the x-rows
+// y = 0..7, a lane row per voxel) by all output channels (NT N-tiles of 16).
+// The weights are packed in MFMA fragment order, [K-step][N-tile][lane][8],
+// so each lane's B fragment is one 16-byte load (the 54 KB of one layer stay
+// in L2), prefetched a K-step ahead.  The epilogue stages the tile through LDS
+// and stores 8 channels of a voxel per 16-byte store.
 //   X      [V][CIN] bf16
-//   Wk     [NT*16][KP] bf16, k = tap * CIN + ci, KP = 32 * ceil(27 * CIN / 32)
+//   Wf     fragment-ordered weights, KS = ceil(27 CIN / 32) K-steps of 32
 //   bias   [NT*16] f32 or null
 //   mask   [V][NT*16] bf16 or null: zero the output where mask <= 0 (ReLU' of the
 //          layer whose gradient this is)
 //   out    bf16 [V][NT*16] (out_f32 null) or f32 [V][NT*16]
+constexpr int CT_X = 16, CT_Y = 8, CT_Z = 4;
+constexpr int CH_X = CT_X + 2, CH_Y = CT_Y + 2, CH_Z = CT_Z + 2;
+constexpr int CT_VOX = CT_X * CT_Y * CT_Z;
+constexpr int CH_VOX = CH_X * CH_Y * CH_Z;
+
+// tiles per volume along x, y, z
+struct TileGrid {
+    int tx, ty, tz;
+};
+
 template <int CIN, int NT>
-__global__ void __launch_bounds__(256) k_conv(const unsigned short* __restrict__ X, Dims D,
-                                              const unsigned short* __restrict__ Wk,
+constexpr int conv_lds_bytes()
+{
+    return CH_VOX * CIN * 2 > CT_VOX * NT * 16 * 4 ? CH_VOX * CIN * 2 : CT_VOX * NT * 16 * 4;
+}
+
+template <int CIN, int NT>
+__global__ void __launch_bounds__(256) k_conv(const unsigned short* __restrict__ X, Dims D, TileGrid G,
+                                              const unsigned short* __restrict__ Wf,
                                               const float* __restrict__ bias, int relu,
                                               const unsigned short* __restrict__ mask,
                                               unsigned short* __restrict__ out, float* __restrict__ out_f32)
 {
-    constexpr int KP = ((27 * CIN + 31) / 32) * 32;
-    constexpr int KS = KP / 32;
+    constexpr int KS = (27 * CIN + 31) / 32;
     constexpr int COUT = NT * 16;
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int64_t V = D.voxels();
-    const int64_t base = ((int64_t)blockIdx.x * 4 + wave) * 64;
-    if (base >= V) return;
-    const int row = lane & 15, kg = lane >> 4;
-    Vox p[4];
-    bool valid[4];
-#pragma unroll
-    for (int mt = 0; mt < 4; ++mt) {
-        int64_t v = base + mt * 16 + row;
-        valid[mt] = v < V;
-        p[mt] = decode(valid[mt] ? v : 0, D);
+    constexpr int UPV = CIN / 8;  // 16-byte units per voxel
+    __shared__ __align__(16) unsigned char lds[conv_lds_bytes<CIN, NT>()];
+    unsigned short* halo = reinterpret_cast<unsigned short*>(lds);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int t = blockIdx.x;
+    const int tx = t % G.tx;
+    t /= G.tx;
+    const int ty = t % G.ty;
+    t /= G.ty;
+    const int tz = t % G.tz;
+    const int b = t / G.tz;
+    const int x0 = tx * CT_X, y0 = ty * CT_Y, z0 = tz * CT_Z;
+    for (int u = tid; u < CH_VOX * UPV; u += 256) {
+        const int hv = u / UPV, part = u % UPV;
+        const int hx = hv % CH_X, hy = (hv / CH_X) % CH_Y, hz = hv / (CH_X * CH_Y);
+        const int x = x0 + hx - 1, y = y0 + hy - 1, z = z0 + hz - 1;
+        uint4 val = {0u, 0u, 0u, 0u};
+        if (x >= 0 && x < D.w && y >= 0 && y < D.h && z >= 0 && z < D.d)
+            val = *reinterpret_cast<const uint4*>(X + ((((int64_t)b * D.d + z) * D.h + y) * D.w + x) * CIN + part * 8);
+        *reinterpret_cast<uint4*>(halo + hv * CIN + part * 8) = val;
     }
-    floatx4 acc[4][NT];
+    __syncthreads();
+    const int row = lane & 15, kg = lane >> 4;
+    floatx4 acc[CT_Y][NT];
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
+    for (int m = 0; m < CT_Y; ++m)
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = floatx4{0, 0, 0, 0};
+        for (int nt = 0; nt < NT; ++nt) acc[m][nt] = floatx4{0, 0, 0, 0};
+    const bf16x8* W8 = reinterpret_cast<const bf16x8*>(Wf);
+    bf16x8 bn[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) bn[nt] = W8[nt * 64 + lane];
     for (int s = 0; s < KS; ++s) {
+        bf16x8 bc[NT];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) bc[nt] = bn[nt];
+        if (s + 1 < KS)
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) bn[nt] = W8[((s + 1) * NT + nt) * 64 + lane];
         const int k0 = s * 32 + kg * 8;
-        const int tap = k0 / CIN, c0 = k0 % CIN;
-        bf16x8 b[NT];
+        const int tap_raw = k0 / CIN, c0 = k0 % CIN;
+        // the padded end of K (CIN = 16: lanes 32-63 of the last step) reads tap 26
+        // in bounds and multiplies zero
+        const bool real = tap_raw < 27;
+        const int tap = real ? tap_raw : 26;
+        const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
+        const unsigned short* src = halo + (((wave + dz) * CH_Y + dy) * CH_X + row + dx) * CIN + c0;
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt)
-            b[nt] = *reinterpret_cast<const bf16x8*>(Wk + (int64_t)(nt * 16 + row) * KP + k0);
-#pragma unroll
-        for (int mt = 0; mt < 4; ++mt) {
-            bf16x8 a = zero8();
-            if (tap < 27 && valid[mt]) {
-                int64_t nb = neighbour(p[mt], tap, D);
-                if (nb >= 0) a = *reinterpret_cast<const bf16x8*>(X + nb * CIN + c0);
-            }
+        for (int m = 0; m < CT_Y; ++m) {
+            bf16x8 a = *reinterpret_cast<const bf16x8*>(src + m * CH_X * CIN);
+            if (!real) a = zero8();
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt)
-                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[nt], acc[mt][nt], 0, 0, 0);
+                acc[m][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bc[nt], acc[m][nt], 0, 0, 0);
         }
     }
-    // D layout: lane holds rows 4*kg + i of column `row`
+    __syncthreads();  // the halo is read; its LDS now holds the output tile
+    float* epi = reinterpret_cast<float*>(lds);
+    // D layout: lane holds rows 4*kg + i (x) of column `row` (channel) of each tile
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
+    for (int m = 0; m < CT_Y; ++m)
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                int64_t v = base + mt * 16 + kg * 4 + i;
-                if (v >= V) continue;
-                int co = nt * 16 + row;
-                float y = acc[mt][nt][i];
-                if (bias) y += bias[co];
-                if (relu) y = fmaxf(y, 0.0f);
-                if (mask && !(bf2f(mask[v * COUT + co]) > 0.0f)) y = 0.0f;
-                if (out_f32)
-                    out_f32[v * COUT + co] = y;
-                else
-                    out[v * COUT + co] = f2bf(y);
-            }
+            for (int i = 0; i < 4; ++i)
+                epi[((wave * CT_Y + m) * CT_X + kg * 4 + i) * COUT + nt * 16 + row] = acc[m][nt][i];
+    __syncthreads();
+    constexpr int UPO = COUT / 8;
+    for (int u = tid; u < CT_VOX * UPO; u += 256) {
+        const int vl = u / UPO, part = u % UPO;
+        const int x = x0 + vl % CT_X, y = y0 + (vl / CT_X) % CT_Y, z = z0 + vl / (CT_X * CT_Y);
+        if (x >= D.w || y >= D.h || z >= D.d) continue;
+        const int64_t v = (((int64_t)b * D.d + z) * D.h + y) * D.w + x;
+        const float4 lo = *reinterpret_cast<const float4*>(epi + vl * COUT + part * 8);
+        const float4 hi = *reinterpret_cast<const float4*>(epi + vl * COUT + part * 8 + 4);
+        float r[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+        if (bias)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) r[j] += bias[part * 8 + j];
+        if (relu)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) r[j] = fmaxf(r[j], 0.0f);
+        if (mask) {
+            const uint4 mk = *reinterpret_cast<const uint4*>(mask + v * COUT + part * 8);
+            const unsigned mw[4] = {mk.x, mk.y, mk.z, mk.w};
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (!(bf2f((unsigned short)(mw[j / 2] >> (16 * (j & 1)))) > 0.0f)) r[j] = 0.0f;
+        }
+        if (out_f32) {
+            *reinterpret_cast<float4*>(out_f32 + v * COUT + part * 8) = float4{r[0], r[1], r[2], r[3]};
+            *reinterpret_cast<float4*>(out_f32 + v * COUT + part * 8 + 4) = float4{r[4], r[5], r[6], r[7]};
+        } else {
+            uint4 o;
+            o.x = f2bf(r[0]) | ((unsigned)f2bf(r[1]) << 16);
+            o.y = f2bf(r[2]) | ((unsigned)f2bf(r[3]) << 16);
+            o.z = f2bf(r[4]) | ((unsigned)f2bf(r[5]) << 16);
+            o.w = f2bf(r[6]) | ((unsigned)f2bf(r[7]) << 16);
+            *reinterpret_cast<uint4*>(out + v * COUT + part * 8) = o;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
-// Weight gradient: partial[chunk][tap][co][ci] over CHUNK voxels per block.
-// Block = 4 waves; blockIdx.y selects 3 taps; each wave covers CHUNK/4 voxels.
-constexpr int DW_CHUNK = 2048;
+// Weight and bias gradients: dW[co][tap][ci] = sum_v dY[v, co] X[v + off(tap), ci],
+// db[co] = sum_v dY[v, co].  K runs over voxels, so both operands are staged in
+// LDS transposed, voxels contiguous: dY of a 16 x 8 x 2 tile as [co][voxel], and
+// X of its halo as [ci][z][y][x] with x rows padded to 24.  Wave w takes the
+// nine taps with dz = w; for each (dy, ci tile) two aligned 16-byte reads hold
+// x0..x0+15 of the shifted row, which give the B fragments of dx = 0, 1 and 2
+// (the latter two by 2- and 4-byte shifts).  The bias gradient is one more
+// MFMA against a B of ones (wave 0).  Blocks loop over tiles (tile = block +
+// k * grid) and write one partial per block; k_reduce_dw sums the partials in
+// a fixed order, so the gradients are bit-reproducible.
+constexpr int WT_X = 16, WT_Y = 8, WT_Z = 2;
+constexpr int WH_X = 24, WH_Y = WT_Y + 2, WH_Z = WT_Z + 2;  // the halo's x rows: 18 used, padded to 24
+constexpr int WT_VOX = WT_X * WT_Y * WT_Z;
+constexpr int WH_PLANE = WH_X * WH_Y * WH_Z;
+constexpr int DW_MAX_BLOCKS = 512;
 
 template <int CIN, int NT>
-__global__ void __launch_bounds__(256) k_conv_dw(const unsigned short* __restrict__ X, Dims D,
-                                                 const unsigned short* __restrict__ dY, float* __restrict__ partial)
+__global__ void __launch_bounds__(192) k_conv_dw(const unsigned short* __restrict__ X, Dims D, TileGrid G, int tiles,
+                                                 const unsigned short* __restrict__ dY, float* __restrict__ partial,
+                                                 float* __restrict__ bias_partial)
 {
     constexpr int COUT = NT * 16;
     constexpr int CT = CIN / 16;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int UPV = CIN / 8, UPO = COUT / 8;
+    __shared__ __align__(16) unsigned short xT[CIN * WH_PLANE];
+    __shared__ __align__(16) unsigned short dyT[COUT * WT_VOX];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;  // wave = dz
     const int row = lane & 15, kg = lane >> 4;
-    const int tap0 = blockIdx.y * 3;
-    const int64_t V = D.voxels();
-    const int64_t vbeg = (int64_t)blockIdx.x * DW_CHUNK + wave * (DW_CHUNK / 4);
-    floatx4 acc[3][NT][CT];
+    floatx4 acc[9][NT][CT];
+    floatx4 accb[NT];
 #pragma unroll
-    for (int t = 0; t < 3; ++t)
+    for (int q = 0; q < 9; ++q)
 #pragma unroll
         for (int a = 0; a < NT; ++a)
 #pragma unroll
-            for (int c = 0; c < CT; ++c) acc[t][a][c] = floatx4{0, 0, 0, 0};
-    for (int ks = 0; ks < DW_CHUNK / 4 / 32; ++ks) {
-        const int64_t v0 = vbeg + ks * 32 + kg * 8;
-        // A: dY^T, rows co, K = 8 voxels
-        bf16x8 A[NT];
+            for (int c = 0; c < CT; ++c) acc[q][a][c] = floatx4{0, 0, 0, 0};
 #pragma unroll
-        for (int a = 0; a < NT; ++a) {
-            shortx8 t8;
+    for (int a = 0; a < NT; ++a) accb[a] = floatx4{0, 0, 0, 0};
+    shortx8 ones8;
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                int64_t v = v0 + j;
-                t8[j] = v < V ? (short)dY[v * COUT + a * 16 + row] : (short)0;
-            }
-            A[a] = __builtin_bit_cast(bf16x8, t8);
+    for (int j = 0; j < 8; ++j) ones8[j] = (short)0x3F80;  // bf16 1.0
+    const bf16x8 ones = __builtin_bit_cast(bf16x8, ones8);
+    for (int t = blockIdx.x; t < tiles; t += gridDim.x) {
+        int q = t;
+        const int tx = q % G.tx;
+        q /= G.tx;
+        const int ty = q % G.ty;
+        q /= G.ty;
+        const int tz = q % G.tz;
+        const int b = q / G.tz;
+        const int x0 = tx * WT_X, y0 = ty * WT_Y, z0 = tz * WT_Z;
+        for (int u = tid; u < WH_PLANE * UPV; u += 192) {
+            const int hv = u / UPV, part = u % UPV;
+            const int hx = hv % WH_X, hy = (hv / WH_X) % WH_Y, hz = hv / (WH_X * WH_Y);
+            const int x = x0 + hx - 1, y = y0 + hy - 1, z = z0 + hz - 1;
+            uint4 val = {0u, 0u, 0u, 0u};
+            if (hx < WT_X + 2 && x >= 0 && x < D.w && y >= 0 && y < D.h && z >= 0 && z < D.d)
+                val = *reinterpret_cast<const uint4*>(X + ((((int64_t)b * D.d + z) * D.h + y) * D.w + x) * CIN +
+                                                      part * 8);
+            const unsigned w4[4] = {val.x, val.y, val.z, val.w};
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                xT[(part * 8 + j) * WH_PLANE + hv] = (unsigned short)(w4[j / 2] >> (16 * (j & 1)));
         }
-        Vox pv[8];
-        bool ok[8];
+        for (int u = tid; u < WT_VOX * UPO; u += 192) {
+            const int vl = u / UPO, part = u % UPO;
+            const int x = x0 + vl % WT_X, y = y0 + (vl / WT_X) % WT_Y, z = z0 + vl / (WT_X * WT_Y);
+            uint4 val = {0u, 0u, 0u, 0u};
+            if (x < D.w && y < D.h && z < D.d)
+                val = *reinterpret_cast<const uint4*>(dY + ((((int64_t)b * D.d + z) * D.h + y) * D.w + x) * COUT +
+                                                      part * 8);
+            const unsigned w4[4] = {val.x, val.y, val.z, val.w};
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            ok[j] = v0 + j < V;
-            pv[j] = decode(ok[j] ? v0 + j : 0, D);
+            for (int j = 0; j < 8; ++j)
+                dyT[(part * 8 + j) * WT_VOX + vl] = (unsigned short)(w4[j / 2] >> (16 * (j & 1)));
         }
+        __syncthreads();
+        for (int c = 0; c < WT_VOX / 32; ++c) {
+            // this lane's 8 voxels: x-row r = z * WT_Y + y, x = xs .. xs + 7
+            const int r = 2 * c + (kg >> 1), xs = (kg & 1) * 8;
+            const int yy = r % WT_Y, zz = r / WT_Y;
+            bf16x8 A[NT];
 #pragma unroll
-        for (int t = 0; t < 3; ++t) {
-            const int tap = tap0 + t;
-#pragma unroll
-            for (int c = 0; c < CT; ++c) {
-                shortx8 t8;
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    int64_t nb = ok[j] ? neighbour(pv[j], tap, D) : -1;
-                    t8[j] = nb >= 0 ? (short)X[nb * CIN + c * 16 + row] : (short)0;
-                }
-                bf16x8 B = __builtin_bit_cast(bf16x8, t8);
+            for (int a = 0; a < NT; ++a)
+                A[a] = *reinterpret_cast<const bf16x8*>(dyT + (a * 16 + row) * WT_VOX + r * WT_X + xs);
+            if (wave == 0)
 #pragma unroll
                 for (int a = 0; a < NT; ++a)
-                    acc[t][a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[a], B, acc[t][a][c], 0, 0, 0);
-            }
-        }
-    }
-    // reduce the 4 waves through LDS in a fixed order, then one write per value
-    // per wave: 3 taps x NT x CT tiles x 4 accumulator rows x 64 lanes
-    __shared__ float red[4][3 * NT * CT * 4 * 64];
-    int idx = 0;
+                    accb[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[a], ones, accb[a], 0, 0, 0);
 #pragma unroll
-    for (int t = 0; t < 3; ++t)
+            for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct) {
+                    const unsigned short* p =
+                        xT + (ct * 16 + row) * WH_PLANE + ((zz + wave) * WH_Y + yy + dy) * WH_X + xs;
+                    const uint4 lo = *reinterpret_cast<const uint4*>(p);
+                    const uint4 hi = *reinterpret_cast<const uint4*>(p + 8);
+                    bf16x8 B[3];
+                    B[0] = __builtin_bit_cast(bf16x8, lo);
+                    uint4 s1, s2;
+                    s1.x = (lo.x >> 16) | (lo.y << 16);
+                    s1.y = (lo.y >> 16) | (lo.z << 16);
+                    s1.z = (lo.z >> 16) | (lo.w << 16);
+                    s1.w = (lo.w >> 16) | (hi.x << 16);
+                    s2.x = lo.y;
+                    s2.y = lo.z;
+                    s2.z = lo.w;
+                    s2.w = hi.x;
+                    B[1] = __builtin_bit_cast(bf16x8, s1);
+                    B[2] = __builtin_bit_cast(bf16x8, s2);
+#pragma unroll
+                    for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+                        for (int a = 0; a < NT; ++a)
+                            acc[dy * 3 + dx][a][ct] =
+                                __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[a], B[dx], acc[dy * 3 + dx][a][ct], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+    // D layout: lane holds rows 4*kg + i (co) of column `row` (ci)
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
 #pragma unroll
         for (int a = 0; a < NT; ++a)
 #pragma unroll
             for (int c = 0; c < CT; ++c)
 #pragma unroll
-                for (int i = 0; i < 4; ++i) red[wave][(idx++) * 64 + lane] = acc[t][a][c][i];
+                for (int i = 0; i < 4; ++i) {
+                    const int co = a * 16 + kg * 4 + i, ci = c * 16 + row, tap = wave * 9 + q;
+                    partial[(((int64_t)blockIdx.x * COUT + co) * 27 + tap) * CIN + ci] = acc[q][a][c][i];
+                }
+    if (wave == 0 && row == 0)
+#pragma unroll
+        for (int a = 0; a < NT; ++a)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) bias_partial[(int64_t)blockIdx.x * COUT + a * 16 + kg * 4 + i] = accb[a][i];
+}
+
+// dW = the sum of the blocks' partials (also of the bias partials), in a fixed
+// order: a block of 256 threads takes 64 elements; its four waves sum four
+// fixed ranges of partials in order (loads coalesced over the elements), and
+// the four range sums are added in range order.
+__global__ void __launch_bounds__(256) k_reduce_dw(const float* __restrict__ partial, int blocks, int n,
+                                                   float* __restrict__ dw)
+{
+    __shared__ float part[4][64];
+    const int col = threadIdx.x & 63, r = threadIdx.x >> 6;
+    const int e = blockIdx.x * 64 + col;
+    const int per = (blocks + 3) / 4;
+    const int c1 = min(blocks, (r + 1) * per);
+    float s = 0;
+    if (e < n)
+        for (int c = r * per; c < c1; ++c) s += partial[(int64_t)c * n + e];
+    part[r][col] = s;
     __syncthreads();
-    const int total = 3 * NT * CT * 4 * 64;
-    for (int e = threadIdx.x; e < total; e += 256) {
-        float s = red[0][e] + red[1][e] + red[2][e] + red[3][e];
-        int ln = e % 64, q = e / 64;
-        int i = q % 4;
-        q /= 4;
-        int c = q % CT;
-        q /= CT;
-        int a = q % NT;
-        int t = q / NT;
-        int co = a * 16 + 4 * (ln >> 4) + i;  // D row = output row of A = co
-        int ci = c * 16 + (ln & 15);          // D column = ci
-        int tap = tap0 + t;
-        // master layout [co][tap][ci] per chunk
-        partial[(((int64_t)blockIdx.x * COUT + co) * 27 + tap) * CIN + ci] = s;
-    }
-}
-
-// dW[tap][co][ci] = sum over chunks in order; db[co] = sum_v dY[v, co].
-__global__ void k_reduce_dw(const float* __restrict__ partial, int chunks, int n, float* __restrict__ dw)
-{
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    float s = 0;
-    for (int c = 0; c < chunks; ++c) s += partial[(int64_t)c * n + e];
-    dw[e] = s;
-}
-
-__global__ void k_bias_partial(const unsigned short* __restrict__ dY, int64_t V, int cout, float* __restrict__ part)
-{
-    // block b sums voxels [b*DW_CHUNK, ...) for every channel in a fixed order
-    int co = threadIdx.x;
-    if (co >= cout) return;
-    int64_t v0 = (int64_t)blockIdx.x * DW_CHUNK;
-    float s = 0;
-    for (int64_t v = v0; v < std::min<int64_t>(v0 + DW_CHUNK, V); ++v) s += bf2f(dY[v * cout + co]);
-    part[(int64_t)blockIdx.x * cout + co] = s;
+    if (r == 0 && e < n) dw[e] = ((part[0][col] + part[1][col]) + part[2][col]) + part[3][col];
 }
 
 // ---------------------------------------------------------------------------
@@ -383,6 +496,25 @@ __global__ void k_kp_loss(const float* __restrict__ out, const float* __restrict
     if (threadIdx.x == 0 && loss_part) loss_part[blockIdx.x] = sm[0];
 }
 
+constexpr int LOSS_BLOCKS = 1024;
+
+// One step's loss: the blocks' partial sums in a fixed order (each of 256
+// threads sums the partials b = thread (mod 256) in order, then a tree), times scale.
+__global__ void __launch_bounds__(256) k_loss_total(const float* __restrict__ part, int blocks, float scale,
+                                                    float* __restrict__ out)
+{
+    __shared__ double sm[256];
+    double s = 0;
+    for (int b = threadIdx.x; b < blocks; b += 256) s += part[b];
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) sm[threadIdx.x] += sm[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = (float)(sm[0] * scale);
+}
+
 // Adam (Kingma & Ba 2015) on the FP32 master parameters.
 __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
                        int64_t n, float lr, float b1, float b2, float eps, float c1, float c2)
@@ -398,29 +530,28 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 }
 
 // Pack layer weights W[co][tap][ci] (FP32 master, padded shapes) into the BF16
-// forward layout Wf[co][tap*CIN + ci] and the backward-data layout
-// Wb[ci][tap'*COUT + co] with tap' = 26 - tap.
+// fragment order of k_conv, [K-step][N-tile][lane][8]: for the forward
+// convolution element (s, nt, l, j) is W[co][tap][ci] with co = 16 nt + (l & 15),
+// k = 32 s + 8 (l >> 4) + j = tap * cin + ci (zero for tap >= 27); for the
+// backward-data convolution (cin' = cout, cout' = cin, taps flipped) it is
+// W[c][26 - tap'][o] with o = 16 nt + (l & 15), k = tap' * cout + c.
 __global__ void k_pack_w(const float* __restrict__ W, int cin, int cout, int kpf, int kpb,
                          unsigned short* __restrict__ wf, unsigned short* __restrict__ wb)
 {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    int n = cout * 27 * cin;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = e & 7, l = (e >> 3) & 63, q = e >> 9;
     if (e < cout * kpf) {
-        int co = e / kpf, k = e % kpf;
-        float w = 0;
-        if (k < 27 * cin) w = W[((int64_t)co * 27 + k / cin) * cin + k % cin];
-        wf[e] = f2bf(w);
+        const int nts = cout / 16, nt = q % nts, s = q / nts;
+        const int co = nt * 16 + (l & 15), k = s * 32 + (l >> 4) * 8 + j;
+        const int tap = k / cin, ci = k % cin;
+        wf[e] = tap < 27 ? f2bf(W[((int64_t)co * 27 + tap) * cin + ci]) : (unsigned short)0;
     }
     if (wb && e < cin * kpb) {
-        int ci = e / kpb, k = e % kpb;
-        float w = 0;
-        if (k < 27 * cout) {
-            int tp = k / cout, co = k % cout;
-            w = W[((int64_t)co * 27 + (26 - tp)) * cin + ci];
-        }
-        wb[e] = f2bf(w);
+        const int nts = cin / 16, nt = q % nts, s = q / nts;
+        const int o = nt * 16 + (l & 15), k = s * 32 + (l >> 4) * 8 + j;
+        const int tp = k / cout, c = k % cout;
+        wb[e] = tp < 27 ? f2bf(W[((int64_t)c * 27 + (26 - tp)) * cin + o]) : (unsigned short)0;
     }
-    (void)n;
 }
 
 // Input conversion: f32 [V][cin_real] -> bf16 [V][16]; optional residual copy.
@@ -512,10 +643,9 @@ void dn_alloc(nsp_denoiser& d, int64_t V)
     d.pred.alloc(sizeof(float) * V * 3);
     d.dy_a.alloc(sizeof(unsigned short) * V * DN_WIDTH);
     d.dy_b.alloc(sizeof(unsigned short) * V * DN_WIDTH);
-    int64_t chunks = (V + DW_CHUNK - 1) / DW_CHUNK;
-    d.partial.alloc(sizeof(float) * chunks * 27 * DN_WIDTH * DN_WIDTH);
-    d.bias_part.alloc(sizeof(float) * chunks * DN_WIDTH);
-    d.loss_part.alloc(sizeof(float) * 1024);
+    d.partial.alloc(sizeof(float) * DW_MAX_BLOCKS * 27 * DN_WIDTH * DN_WIDTH);
+    d.bias_part.alloc(sizeof(float) * DW_MAX_BLOCKS * DN_WIDTH);
+    d.loss_part.alloc(sizeof(float) * LOSS_BLOCKS);
     d.cap_v = V;
 }
 
@@ -530,13 +660,20 @@ void pack_weights(nsp_denoiser& d)
     NSP_HIP(hipGetLastError());
 }
 
+static TileGrid tiles_of(Dims D, int tx, int ty, int tz)
+{
+    return TileGrid{(D.w + tx - 1) / tx, (D.h + ty - 1) / ty, (D.d + tz - 1) / tz};
+}
+
 template <int CIN, int NT>
 void conv_launch(const unsigned short* x, Dims D, const unsigned short* w, const float* bias, int relu,
                  const unsigned short* mask, unsigned short* out, float* out_f32, hipStream_t s)
 {
-    int64_t waves = (D.voxels() + 63) / 64;
-    unsigned blocks = (unsigned)((waves + 3) / 4);
-    hipLaunchKernelGGL((k_conv<CIN, NT>), dim3(blocks), dim3(256), 0, s, x, D, w, bias, relu, mask, out, out_f32);
+    const TileGrid G = tiles_of(D, CT_X, CT_Y, CT_Z);
+    const int64_t blocks = (int64_t)D.nb * G.tx * G.ty * G.tz;
+    if (blocks > 0x7fffffff) raise(NSP_E_INVALID, "volume too large for the convolution grid");
+    hipLaunchKernelGGL((k_conv<CIN, NT>), dim3((unsigned)blocks), dim3(256), 0, s, x, D, G, w, bias, relu, mask, out,
+                       out_f32);
     NSP_HIP(hipGetLastError());
 }
 
@@ -550,20 +687,27 @@ void conv(int cin, int cout, const unsigned short* x, Dims D, const unsigned sho
     raise(NSP_E_INVALID, "unsupported conv shape");
 }
 
-void conv_dw(int cin, int cout, const unsigned short* x, Dims D, const unsigned short* dy, float* partial,
-             hipStream_t s)
+// weight and bias gradient partials of one layer; returns the number of blocks (partials)
+int conv_dw(int cin, int cout, const unsigned short* x, Dims D, const unsigned short* dy, float* partial,
+            float* bias_partial, hipStream_t s)
 {
-    int64_t chunks = (D.voxels() + DW_CHUNK - 1) / DW_CHUNK;
-    dim3 g((unsigned)chunks, 9);
+    const TileGrid G = tiles_of(D, WT_X, WT_Y, WT_Z);
+    const int64_t tiles = (int64_t)D.nb * G.tx * G.ty * G.tz;
+    if (tiles > 0x7fffffff) raise(NSP_E_INVALID, "volume too large for the gradient grid");
+    const int blocks = (int)std::min<int64_t>(tiles, DW_MAX_BLOCKS);
     if (cin == 16 && cout == 32)
-        hipLaunchKernelGGL((k_conv_dw<16, 2>), g, dim3(256), 0, s, x, D, dy, partial);
+        hipLaunchKernelGGL((k_conv_dw<16, 2>), dim3(blocks), dim3(192), 0, s, x, D, G, (int)tiles, dy, partial,
+                           bias_partial);
     else if (cin == 32 && cout == 32)
-        hipLaunchKernelGGL((k_conv_dw<32, 2>), g, dim3(256), 0, s, x, D, dy, partial);
+        hipLaunchKernelGGL((k_conv_dw<32, 2>), dim3(blocks), dim3(192), 0, s, x, D, G, (int)tiles, dy, partial,
+                           bias_partial);
     else if (cin == 32 && cout == 16)
-        hipLaunchKernelGGL((k_conv_dw<32, 1>), g, dim3(256), 0, s, x, D, dy, partial);
+        hipLaunchKernelGGL((k_conv_dw<32, 1>), dim3(blocks), dim3(192), 0, s, x, D, G, (int)tiles, dy, partial,
+                           bias_partial);
     else
         raise(NSP_E_INVALID, "unsupported conv shape");
     NSP_HIP(hipGetLastError());
+    return blocks;
 }
 
 // forward pass over act[0] (already packed) -> out_f32, pred (if requested)
@@ -583,22 +727,18 @@ void forward(nsp_denoiser& d, Dims D)
 void backward(nsp_denoiser& d, Dims D, const unsigned short* dout)
 {
     hipStream_t s = d.stream;
-    const int64_t V = D.voxels();
-    const int chunks = (int)((V + DW_CHUNK - 1) / DW_CHUNK);
     const unsigned short* dy = dout;
     for (int l = DN_LAYERS - 1; l >= 0; --l) {
         Layer& L = d.layers[l];
-        // weight gradient
-        conv_dw(L.cin, L.cout, d.act[l].as<unsigned short>(), D, dy, d.partial.as<float>(), s);
+        // weight and bias gradients: per-block partials [block][co][tap][ci] (the
+        // master layout) and [block][co], summed in block order
+        const int blocks =
+            conv_dw(L.cin, L.cout, d.act[l].as<unsigned short>(), D, dy, d.partial.as<float>(), d.bias_part.as<float>(), s);
         int n = 27 * L.cout * L.cin;
-        // partials are [chunk][co][tap][ci], the master layout; sum the chunks in order
-        hipLaunchKernelGGL(k_reduce_dw, dim3((n + 255) / 256), dim3(256), 0, s, d.partial.as<float>(), chunks, n,
+        hipLaunchKernelGGL(k_reduce_dw, dim3((n + 63) / 64), dim3(256), 0, s, d.partial.as<float>(), blocks, n,
                            d.grads.as<float>() + L.w_off);
-        NSP_HIP(hipGetLastError());
-        // bias gradient
-        hipLaunchKernelGGL(k_bias_partial, dim3(chunks), dim3(64), 0, s, dy, V, L.cout, d.bias_part.as<float>());
-        hipLaunchKernelGGL(k_reduce_dw, dim3(1), dim3(64), 0, s, d.bias_part.as<float>(), chunks, L.cout,
-                           d.grads.as<float>() + L.b_off);
+        hipLaunchKernelGGL(k_reduce_dw, dim3((L.cout + 63) / 64), dim3(256), 0, s, d.bias_part.as<float>(), blocks,
+                           L.cout, d.grads.as<float>() + L.b_off);
         NSP_HIP(hipGetLastError());
         if (l == 0) break;
         // data gradient for layer l-1, masked by its ReLU
@@ -610,11 +750,14 @@ void backward(nsp_denoiser& d, Dims D, const unsigned short* dout)
     }
 }
 
-float loss_and_grad(nsp_denoiser& d, Dims D, bool want_grad)
+// The loss kernels of one step write LOSS_BLOCKS partial sums; with loss_out
+// the step's mean squared error is summed into loss_out on the device (no host
+// wait), else it is returned after a wait.
+float loss_and_grad(nsp_denoiser& d, Dims D, bool want_grad, float* loss_out = nullptr)
 {
     hipStream_t s = d.stream;
     const int64_t V = D.voxels();
-    const int blocks = 1024;
+    const int blocks = LOSS_BLOCKS;
     float scale = 1.0f / (float)(3 * V);
     if (d.head == NSP_DENOISER_KERNEL)
         hipLaunchKernelGGL(k_kp_loss, dim3(blocks), dim3(256), 0, s, d.out_f32.as<float>(), d.resid.as<float>(),
@@ -625,6 +768,11 @@ float loss_and_grad(nsp_denoiser& d, Dims D, bool want_grad)
                            d.target.as<float>(), V, scale, want_grad ? d.dy_a.as<unsigned short>() : nullptr,
                            d.pred.as<float>(), d.loss_part.as<float>());
     NSP_HIP(hipGetLastError());
+    if (loss_out) {
+        hipLaunchKernelGGL(k_loss_total, dim3(1), dim3(256), 0, s, d.loss_part.as<float>(), blocks, scale, loss_out);
+        NSP_HIP(hipGetLastError());
+        return 0.0f;
+    }
     std::vector<float> lp(blocks);
     NSP_HIP(hipMemcpyAsync(lp.data(), d.loss_part.p, sizeof(float) * blocks, hipMemcpyDeviceToHost, s));
     NSP_HIP(hipStreamSynchronize(s));
@@ -760,6 +908,43 @@ int nsp_denoiser_set_params(nsp_denoiser* d, const float* in)
     }
 }
 
+int nsp_denoiser_get_state(const nsp_denoiser* d, float* params, float* adam_m, float* adam_v, int64_t* adam_t)
+{
+    if (!d) return NSP_E_INVALID;
+    try {
+        NSP_HIP(hipSetDevice(d->device));
+        NSP_HIP(hipStreamSynchronize(d->stream));
+        if (params) NSP_HIP(hipMemcpy(params, d->params.p, sizeof(float) * d->nparams, hipMemcpyDeviceToHost));
+        if (adam_m) NSP_HIP(hipMemcpy(adam_m, d->adam_m.p, sizeof(float) * d->nparams, hipMemcpyDeviceToHost));
+        if (adam_v) NSP_HIP(hipMemcpy(adam_v, d->adam_v.p, sizeof(float) * d->nparams, hipMemcpyDeviceToHost));
+        if (adam_t) *adam_t = d->adam_t;
+        return NSP_OK;
+    } catch (const Error& e) {
+        *d->err = e.msg;
+        return e.code;
+    }
+}
+
+int nsp_denoiser_set_state(nsp_denoiser* d, const float* params, const float* adam_m, const float* adam_v,
+                           int64_t adam_t)
+{
+    if (!d || !params || !adam_m || !adam_v || adam_t < 0) return NSP_E_INVALID;
+    try {
+        NSP_HIP(hipSetDevice(d->device));
+        NSP_HIP(hipStreamSynchronize(d->stream));
+        NSP_HIP(hipMemcpy(d->params.p, params, sizeof(float) * d->nparams, hipMemcpyHostToDevice));
+        NSP_HIP(hipMemcpy(d->adam_m.p, adam_m, sizeof(float) * d->nparams, hipMemcpyHostToDevice));
+        NSP_HIP(hipMemcpy(d->adam_v.p, adam_v, sizeof(float) * d->nparams, hipMemcpyHostToDevice));
+        d->adam_t = adam_t;
+        pack_weights(*d);
+        NSP_HIP(hipStreamSynchronize(d->stream));
+        return NSP_OK;
+    } catch (const Error& e) {
+        *d->err = e.msg;
+        return e.code;
+    }
+}
+
 static Dims dims_of(int nb, int dd, int hh, int ww) { return Dims{nb, dd, hh, ww}; }
 
 int nsp_denoiser_forward(nsp_denoiser* d, const float* input, int nb, int depth, int height, int width, float* output)
@@ -843,14 +1028,19 @@ int nsp_denoiser_train(nsp_denoiser* d, const float* inputs, const float* target
         Dims D = dims_of(batch, patch, patch, patch);
         const int64_t V = D.voxels();
         dn_alloc(*d, V);
+        DevBuf dloss;
+        dloss.alloc(sizeof(float) * std::max(steps, 1));
         const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
         for (int st = 0; st < steps; ++st) {
+            // the patches of a step are drawn by (seed, optimiser step count), so a
+            // run resumed from a checkpoint draws what the unbroken run would
             hipLaunchKernelGGL(k_gather_patches, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s, vin.as<float>(),
                                vtg.as<float>(), volumes, depth, height, width, d->cin_real, d->resid_ch, patch, batch,
-                               seed, st, d->act[0].as<unsigned short>(), d->resid.as<float>(), d->target.as<float>());
+                               seed, (int)d->adam_t, d->act[0].as<unsigned short>(), d->resid.as<float>(),
+                               d->target.as<float>());
             NSP_HIP(hipGetLastError());
             forward(*d, D);
-            float l = loss_and_grad(*d, D, true);
+            loss_and_grad(*d, D, true, dloss.as<float>() + st);
             backward(*d, D, d->dy_a.as<unsigned short>());
             d->adam_t += 1;
             float c1 = 1.0f - std::pow(b1, (float)d->adam_t), c2 = 1.0f - std::pow(b2, (float)d->adam_t);
@@ -859,8 +1049,9 @@ int nsp_denoiser_train(nsp_denoiser* d, const float* inputs, const float* target
                                d->adam_v.as<float>(), d->nparams, lr, b1, b2, eps, c1, c2);
             NSP_HIP(hipGetLastError());
             pack_weights(*d);
-            if (losses) losses[st] = l;
         }
+        if (losses && steps > 0)
+            NSP_HIP(hipMemcpyAsync(losses, dloss.p, sizeof(float) * steps, hipMemcpyDeviceToHost, s));
         NSP_HIP(hipStreamSynchronize(s));
         return NSP_OK;
     } catch (const Error& e) {

@@ -120,7 +120,7 @@ EXPORTS = [
     "nsp_scene_destroy", "nsp_paint", "nsp_statistics", "nsp_image_get", "nsp_image_set", "nsp_image_aov",
     "nsp_scene_pixel_count", "nsp_last_pass_stats", "nsp_set_counters", "nsp_meshgen_torus",
     "nsp_meshgen_cube_sphere", "nsp_denoiser_create", "nsp_denoiser_create_head", "nsp_denoiser_destroy",
-    "nsp_denoiser_param_count",
+    "nsp_denoiser_param_count", "nsp_denoiser_get_state", "nsp_denoiser_set_state",
     "nsp_denoiser_get_params", "nsp_denoiser_set_params", "nsp_denoiser_forward", "nsp_denoiser_loss_grad",
     "nsp_denoiser_train", "nsp_conv3d_bf16", "nsp_batch_create", "nsp_batch_destroy", "nsp_batch_paint",
     "nsp_batch_statistics", "nsp_batch_set_lanes", "nsp_batch_get_lanes", "nsp_batch_autotune", "nsp_autotune", "nsp_mesh_set_record_layout", "nsp_mesh_get_record_layout",
@@ -625,6 +625,8 @@ def _denoiser_lib():
         L.nsp_denoiser_param_count.restype = C.c_int64
         L.nsp_denoiser_get_params.argtypes = [P, fp]
         L.nsp_denoiser_set_params.argtypes = [P, fp]
+        L.nsp_denoiser_get_state.argtypes = [P, fp, fp, fp, C.POINTER(C.c_int64)]
+        L.nsp_denoiser_set_state.argtypes = [P, fp, fp, fp, C.c_int64]
         L.nsp_denoiser_forward.argtypes = [P, fp, C.c_int, C.c_int, C.c_int, C.c_int, fp]
         L.nsp_denoiser_loss_grad.argtypes = [P, fp, fp, C.c_int, C.c_int, C.c_int, C.c_int, fp, fp]
         L.nsp_denoiser_train.argtypes = [P, fp, fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
@@ -694,6 +696,20 @@ class Denoiser:
         p = np.ascontiguousarray(p, np.float32)
         assert p.size == self.nparams
         self.ctx.check(_denoiser_lib().nsp_denoiser_set_params(self.h, _ptr(p, C.c_float)))
+
+    def state(self):
+        """(params, Adam first moments, Adam second moments, Adam step count)."""
+        p, m, v = (np.zeros(self.nparams, np.float32) for _ in range(3))
+        t = C.c_int64()
+        self.ctx.check(_denoiser_lib().nsp_denoiser_get_state(self.h, _ptr(p, C.c_float), _ptr(m, C.c_float),
+                                                              _ptr(v, C.c_float), C.byref(t)))
+        return p, m, v, t.value
+
+    def set_state(self, p, m, v, t):
+        p, m, v = (np.ascontiguousarray(a, np.float32) for a in (p, m, v))
+        assert p.size == m.size == v.size == self.nparams
+        self.ctx.check(_denoiser_lib().nsp_denoiser_set_state(self.h, _ptr(p, C.c_float), _ptr(m, C.c_float),
+                                                              _ptr(v, C.c_float), int(t)))
 
     def forward(self, x):
         x = np.ascontiguousarray(x, np.float32)

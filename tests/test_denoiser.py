@@ -102,11 +102,15 @@ def test_checker_conv_closed_forms():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("cin,cout", [(16, 32), (32, 32), (32, 16), (16, 16)])
-def test_conv3d_bf16_matches_checker(ctx, cin, cout):
+@pytest.mark.parametrize("cin,cout,shape", [(16, 32, (2, 5, 6, 7)), (32, 32, (2, 5, 6, 7)), (32, 16, (2, 5, 6, 7)),
+                                            (16, 16, (2, 5, 6, 7)), (32, 32, (1, 9, 19, 35)),
+                                            (16, 32, (2, 4, 8, 16))])
+def test_conv3d_bf16_matches_checker(ctx, cin, cout, shape):
+    # shapes: inside one 16 x 8 x 4 output tile; spread over 3 x 3 x 3 tiles with
+    # partial tiles on every axis; exactly one full tile per volume
     from math_amd import native
     rng = np.random.default_rng(cin + cout)
-    x = R.bf16(rng.normal(size=(2, 5, 6, 7, cin)))
+    x = R.bf16(rng.normal(size=shape + (cin,)))
     w = R.bf16(rng.normal(scale=0.1, size=(cout, 27, cin)))
     b = rng.normal(size=cout).astype(np.float32)
     y = native.conv3d_bf16(ctx, x, w, b, relu=False)
@@ -136,6 +140,29 @@ def test_denoiser_forward_and_gradient_match_checker(ctx):
         errs.append(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
     print(f"forward rel err {ferr:.2e}, loss {loss:.6f} vs {rl:.6f}, grad rel err per layer {np.round(errs, 4)}")
     assert ferr < 2e-2
+    assert abs(loss - rl) <= 2e-2 * rl
+    assert max(errs) < 5e-2
+
+
+@pytest.mark.gpu
+def test_denoiser_gradient_over_many_tiles_matches_checker(ctx):
+    # 3 x 16 x 64 x 64 voxels: 768 weight-gradient tiles of 16 x 8 x 2, more than
+    # the 512 blocks of the gradient kernel, so blocks sum several tiles; the
+    # gradients are also bit-reproducible from run to run
+    from math_amd import native
+    rng = np.random.default_rng(17)
+    d = native.Denoiser(ctx, 13, residual_channel=3, seed=5)
+    p = d.params()
+    x = rng.normal(size=(3, 16, 64, 64, 13)).astype(np.float32)
+    t = rng.normal(size=(3, 16, 64, 64, 3)).astype(np.float32)
+    loss, g = d.loss_grad(x, t)
+    loss2, g2 = d.loss_grad(x, t)
+    assert loss == loss2 and np.array_equal(g, g2)
+    rl, rg = R.loss_grad(p, x, t, 3, emulate_bf16=True)
+    offs = np.cumsum([0] + [co * 27 * ci + co for co, ci in R.shapes()])
+    errs = [np.linalg.norm(g[offs[l]:offs[l + 1]] - rg[offs[l]:offs[l + 1]]) /
+            max(np.linalg.norm(rg[offs[l]:offs[l + 1]]), 1e-30) for l in range(R.LAYERS)]
+    print(f"many tiles: loss {loss:.6f} vs {rl:.6f}, grad rel err per layer {np.round(errs, 4)}")
     assert abs(loss - rl) <= 2e-2 * rl
     assert max(errs) < 5e-2
 
@@ -178,6 +205,31 @@ def test_denoiser_training_reduces_loss(ctx):
     print("losses", losses[:3], losses[-3:])
     assert np.all(np.isfinite(losses))
     assert losses[-10:].mean() < 0.5 * losses[:5].mean()
+
+
+@pytest.mark.gpu
+def test_training_resumes_from_checkpoint(ctx):
+    # 40 steps in one call equal 25 steps, a checkpoint (parameters, Adam moments,
+    # step count) restored into a new denoiser, and 15 more steps: bit for bit
+    from math_amd import native
+    rng = np.random.default_rng(8)
+    x = rng.normal(size=(3, 20, 20, 20, 7)).astype(np.float32)
+    y = (x[..., :3] * 0.5).astype(np.float32)
+    a = native.Denoiser(ctx, 7, residual_channel=0, seed=4)
+    la = a.train(x, y, patch=16, batch=2, steps=40, lr=1e-3, seed=9)
+    b = native.Denoiser(ctx, 7, residual_channel=0, seed=4)
+    lb1 = b.train(x, y, patch=16, batch=2, steps=25, lr=1e-3, seed=9)
+    st = b.state()
+    assert st[3] == 25
+    b.close()
+    c = native.Denoiser(ctx, 7, residual_channel=0, seed=99)
+    c.set_state(*st)
+    lb2 = c.train(x, y, patch=16, batch=2, steps=15, lr=1e-3, seed=9)
+    assert np.array_equal(np.concatenate([lb1, lb2]), la)
+    pa, ma, va, ta = a.state()
+    pc, mc, vc, tc = c.state()
+    assert ta == tc == 40
+    assert np.array_equal(pa, pc) and np.array_equal(ma, mc) and np.array_equal(va, vc)
 
 
 @pytest.mark.gpu
