@@ -114,68 +114,79 @@ def cpu_baseline(ctx, mesh, v, f, cfg, spp, depth, seconds_target=15.0):
     return cpu, parity
 
 
-def denoise_bench(args):
-    """C3: denoiser trained on the tracer's own renders; time to RMSE <= 0.03,
-    0.02 and 0.01 against a 4096-sample FP64 render, FP32 + denoiser vs plain
-    FP32, on held-out 128^3 scenes (or a centred window with --eval-window)."""
+def denoise_eval(ctx, scenes, args):
+    """C3: time to RMSE <= 0.03, 0.02 and 0.01 against a 4096-sample FP64 render,
+    FP32 + denoiser vs plain FP32, on held-out 128^3 scenes (seeds 200, 201, ...).
+    The network is the shipped one (math_amd/denoiser_c3.npz, trained by
+    math_amd/denoise_train.py with the library's kernels) unless --denoise-train
+    trains it here first."""
     import numpy as np
 
     from math_amd import denoise_eval as DE
+    from math_amd import denoise_train as DT
+    from math_amd import native
+
+    if args.denoise_train:
+        dn, alphas, recipe = DT.train(ctx, steps=args.train_steps,
+                                      variants=list(DE.TRAIN_SEEDS[:args.train_variants]), log=log)
+    else:
+        params, alphas, recipe = DT.load()
+        dn = native.Denoiser(ctx, DE.IN_CHANNELS, DE.RESIDUAL, seed=1)
+        dn.set_params(params)
+    results = []
+    for seed in DE.HELDOUT_SEEDS[:scenes]:
+        v, f, cfg = DE.variant(seed, 128)
+        mesh = ctx.mesh(v, f)
+        # 8 passes of 8^3 = 512 stratified samples: 4096 samples per pixel
+        ref, ref_s = DE.render_mean(ctx, mesh, cfg, 512, seed=seed * 7, precision=native.PRECISION_FP64, passes=8)
+        log(f"held-out scene {seed}: FP64 4096-sample reference in {ref_s:.1f} s")
+        r = DE.time_to_rmse(ctx, mesh, cfg, ref, dn, seed=seed * 11, alphas=alphas)
+        r["seed"] = seed
+        log(f"held-out scene {seed}: {r['thresholds']}")
+        results.append(r)
+        mesh.close()
+    thr = [str(t) for t in DE.THRESHOLDS]
+    per_scene = []
+    for r in results:
+        per_scene.append({"seed": r["seed"], **{t: r["thresholds"][t] for t in thr},
+                          "level1_rmse": {"input_4spp": round(r["input4_rmse"], 5),
+                                          "denoised": round(r["denoised_rmse"], 5)}})
+    geo = {}
+    for t in thr:
+        sp = [r["thresholds"][t]["speedup"] for r in results if r["thresholds"][t]["speedup"]]
+        geo[t] = float(np.exp(np.mean(np.log(sp)))) if sp else None
+    plain_rays = sum(r["plain_rays"] for r in results)
+    plain_s = sum(r["plain_paint_s"] for r in results)
+    dn_rays = sum(r["denoised_rays"] for r in results)
+    dn_s = sum(r["denoised_paint_s"] for r in results)
+    conv_ms = float(np.median([r["conv_forward_ms"] for r in results]))
+    conv_flops = results[0]["conv_forward_flops"]
+    tflops = conv_flops / (conv_ms * 1e-3) / 1e12
+    return {
+        "metric": "time to RMSE <= thr against a 4096-sample FP64 render at n=4, FP32 + denoiser vs plain FP32",
+        "scenes": f"{len(results)} of the 16 held-out 128^3 scenes (seeds {DE.HELDOUT_SEEDS[0]}-"
+                  f"{DE.HELDOUT_SEEDS[len(results) - 1]})",
+        "per_scene": per_scene,
+        "geomean_speedup": geo,
+        "plain": {"Mrays_s": round(plain_rays / plain_s / 1e6, 1),
+                  "roofline": "the traversal kernels of the Mrays/s line (same kernels, 4-spp passes)"},
+        "denoised": {"Mrays_s": round(dn_rays / dn_s / 1e6, 1),
+                     "denoiser_roofline": {"bound": "mfma", "achieved": round(tflops, 1), "peak": 2500.0,
+                                           "unit": "TFLOP/s", "frac": round(tflops / 2500.0, 4),
+                                           "kernel": "k_conv (8 layers, 128^3, BF16 MFMA, FP32 accumulate)",
+                                           "forward_ms": round(conv_ms, 3),
+                                           "flops": conv_flops}},
+        "blend_weights": alphas,
+        "network": "shipped weights (math_amd/denoiser_c3.npz)" if not args.denoise_train else "trained in this run",
+        "recipe": recipe,
+    }
+
+
+def denoise_bench(args):
     from math_amd import native
 
     ctx = native.Context(0)
-    t0 = time.perf_counter()
-    X, Y = DE.training_set(ctx, list(range(100, 100 + args.train_variants)), window=args.window,
-                           target_spp=args.target_spp, log=log)
-    gen_s = time.perf_counter() - t0
-    log(f"training set: {X.shape[0]} volumes of {X.shape[1]}^3 in {gen_s:.1f} s")
-    dn = native.Denoiser(ctx, DE.IN_CHANNELS, DE.RESIDUAL, seed=1)
-    t0 = time.perf_counter()
-    chunks = []
-    for c in range(0, args.train_steps, 500):
-        k = min(500, args.train_steps - c)
-        lr = 2e-4 if c < 0.8 * args.train_steps else 5e-5
-        chunks.append(dn.train(X, Y, patch=32, batch=16, steps=k, lr=lr, seed=2 + c))
-        log(f"training steps {c + k}: lr {lr:g}, loss {chunks[-1][-50:].mean():.6f} ({time.perf_counter() - t0:.1f} s)")
-    losses = np.concatenate(chunks)
-    train_s = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    alphas = DE.fit_blend(dn, X, Y)
-    log(f"blend weights per level {alphas} ({time.perf_counter() - t0:.1f} s)")
-    del X, Y
-    results = []
-    for k in range(args.eval_scenes):
-        seed = 200 + k
-        if args.eval_window:
-            w = args.eval_window
-            v, f, cfg = DE.variant(seed, 128, ([(128 - w) // 2] * 3, w))
-        else:
-            v, f, cfg = DE.variant(seed, 128)
-        mesh = ctx.mesh(v, f)
-        # 8 passes of 8^3 = 512 stratified samples: 4096 samples per pixel
-        ref, ref_s = DE.render_mean(ctx, mesh, cfg, 512, seed=seed * 7, precision=native.PRECISION_FP64, passes=8,
-                                    log=log)
-        log(f"eval scene {seed}: FP64 4096-sample reference in {ref_s:.1f} s")
-        r = DE.time_to_rmse(ctx, mesh, cfg, ref, dn, seed=seed * 11, alphas=alphas)
-        r["seed"] = seed
-        log(f"eval scene {seed}: {r['thresholds']}")
-        log(f"eval scene {seed}: plain curve {[(c, round(t, 3), round(e, 4)) for c, t, e in r['plain_curve']]}")
-        log(f"eval scene {seed}: denoised curve {[(c, round(t, 3), round(e, 4)) for c, t, e in r['denoised_curve']]}")
-        results.append(r)
-        mesh.close()
-    out = {
-        "metric": "time to RMSE<=thr vs 4096-sample FP64 ref at n=4, FP32+denoiser vs plain FP32",
-        "eval_screen": f"{args.eval_window}^3 centred window" if args.eval_window else "128^3",
-        "eval": [{"seed": r["seed"], "thresholds": r["thresholds"], "level1_rmse": r["denoised_rmse"],
-                  "level1_input_rmse": r["input4_rmse"], "inference_s": r["denoiser_inference_s"]} for r in results],
-        "curves": [{"seed": r["seed"], "plain": r["plain_curve"], "denoised": r["denoised_curve"]} for r in results],
-        "blend_weights": alphas,
-        "train": {"variants": args.train_variants, "levels": list(DE.LEVELS), "window": args.window,
-                  "target_spp": args.target_spp, "steps": args.train_steps, "seconds": round(train_s, 2),
-                  "data_seconds": round(gen_s, 2), "loss_first": float(losses[:5].mean()),
-                  "loss_last": float(losses[-50:].mean())},
-    }
-    print(json.dumps(out), flush=True)
+    print(json.dumps(denoise_eval(ctx, args.eval_scenes, args)), flush=True)
 
 
 def obj_bench(args):
@@ -343,12 +354,14 @@ def main():
     ap.add_argument("--denoise", action="store_true",
                     help="C3: train the denoiser on renders of n=4 variants, then time to RMSE<=0.01, "
                          "denoised vs plain")
-    ap.add_argument("--train-variants", type=int, default=32)
-    ap.add_argument("--train-steps", type=int, default=8000)
-    ap.add_argument("--eval-scenes", type=int, default=3)
-    ap.add_argument("--window", type=int, default=64)
-    ap.add_argument("--target-spp", type=int, default=2048)
-    ap.add_argument("--eval-window", type=int, default=0, help="evaluate on a centred window (0: full 128^3)")
+    ap.add_argument("--denoise-train", action="store_true",
+                    help="with --denoise: train the network here (recipe of math_amd/denoise_train.py) instead of "
+                         "loading the shipped weights")
+    ap.add_argument("--train-variants", type=int, default=64)
+    ap.add_argument("--train-steps", type=int, default=16000)
+    ap.add_argument("--eval-scenes", type=int, default=16, help="with --denoise: held-out scenes to evaluate")
+    ap.add_argument("--denoise-scenes", type=int, default=16,
+                    help="held-out scenes of the denoise object in the default run (0: none)")
     ap.add_argument("--obj", action="store_true", help=".obj4 ingest of the C1 mesh (reader MB/s)")
     ap.add_argument("--hull", action="store_true", help="convex hulls of 4-dimensional point sets (ms)")
     ap.add_argument("--hull-sample", type=int, default=3000, help="with --hull: points the oracle takes")
@@ -489,6 +502,8 @@ def main():
         "cpu_baseline": cpu,
         "parity": parity,
     }
+    if args.denoise_scenes > 0:
+        out["denoise"] = denoise_eval(ctx, args.denoise_scenes, args)
     print(json.dumps(out), flush=True)
 
 

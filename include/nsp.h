@@ -355,6 +355,10 @@ int nsp_image_set(nsp_scene* scene, const float* rgb_sum, const int32_t* samples
  * with aov = 1 (divide by the sample counts of nsp_image_get).  Any pointer may
  * be NULL.  No reference counterpart: these are the denoiser's auxiliary inputs. */
 int nsp_image_aov(const nsp_scene* scene, float* albedo_sum, float* normal_sum, float* direct_sum);
+/* Per-pixel sums (pixels*3) of the radiance of each pixel's first sample, one
+ * per pass painted with aov = 1: a one-sample-per-pixel image per pass, the
+ * denoiser's low-sample input.  No reference counterpart. */
+int nsp_image_first_sample(const nsp_scene* scene, float* rgb_sum);
 /* Number of pixels of the scene's screen. */
 int64_t nsp_scene_pixel_count(const nsp_scene* scene);
 
@@ -447,6 +451,24 @@ int nsp_denoiser_set_params(nsp_denoiser* d, const float* in);
 int nsp_denoiser_get_state(const nsp_denoiser* d, float* params, float* adam_m, float* adam_v, int64_t* adam_t);
 int nsp_denoiser_set_state(nsp_denoiser* d, const float* params, const float* adam_m, const float* adam_v,
                            int64_t adam_t);
+/* The denoised image of a scene painted with aov = 1 (n = 4, a 3D screen),
+ * without host copies of the input: the network's 13-channel input is built on
+ * the device from the scene's per-pixel sums (1-spp radiance: the first-sample
+ * sums over the passes, samples_per_pass samples per pixel and pass; 4-spp
+ * radiance; albedo; normal), and the prediction is blended with the 4-spp
+ * radiance: blend * net + (1 - blend) * input.  output: pixels*3 floats in
+ * pixel order, or NULL to keep the result on the device (nsp_denoiser_output
+ * copies it).  The denoiser must have 13 input channels and residual channel 3.
+ * No reference counterpart. */
+int nsp_denoiser_forward_scene(nsp_denoiser* d, const nsp_scene* scene, int samples_per_pass, float blend,
+                               float* output);
+/* Copies the prediction of the last forward (voxels*3 floats). */
+int nsp_denoiser_output(const nsp_denoiser* d, float* output);
+/* Device time (HIP events on the denoiser's stream) of the last
+ * nsp_denoiser_forward's eight convolutions, without the copies, and their
+ * multiply-add count on the real channels (2 * 27 * cin * cout per voxel and
+ * layer).  No reference counterpart. */
+int nsp_denoiser_last_forward(const nsp_denoiser* d, double* ms, double* flops);
 /* Denoised radiance output[nb*depth*height*width*3]. */
 int nsp_denoiser_forward(nsp_denoiser* d, const float* input, int nb, int depth, int height, int width,
                          float* output);

@@ -146,6 +146,21 @@ static int rng_pass(const nsp_scene* s) { return (int)s->pass_count; }
 static thread_local std::string g_create_error;
 
 namespace nsp {
+// The device buffers of a scene's per-pixel sums (channel-major), for the
+// denoiser's device-resident input: returns false when the scene has no AOV sums.
+bool scene_pixel_buffers(const nsp_scene* s, ScenePixels* out)
+{
+    if (!s || !s->pix_aov.p) return false;
+    out->ctx = s->ctx;
+    out->n = s->n;
+    out->pixels = s->pixels;
+    for (int i = 0; i < s->n - 1; ++i) out->screen[i] = s->proj.screen_size[i];
+    out->sum = s->pix_sum.as<float>();
+    out->cnt = s->pix_cnt.as<int32_t>();
+    out->aov = s->pix_aov.as<float>();
+    return true;
+}
+
 hipStream_t context_stream(nsp_context* ctx) { return ctx->stream; }
 int context_device(nsp_context* ctx) { return ctx->device; }
 std::string* context_error(nsp_context* ctx) { return &ctx->error; }
@@ -924,8 +939,8 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
         if (p.aov) {
             ctx->work.aov.alloc(sizeof(float) * aov_channels(n) * cap);
             if (!scene->pix_aov.p) {
-                scene->pix_aov.alloc(sizeof(float) * aov_channels(n) * scene->pixels);
-                NSP_HIP(hipMemsetAsync(scene->pix_aov.p, 0, sizeof(float) * aov_channels(n) * scene->pixels, s));
+                scene->pix_aov.alloc(sizeof(float) * pix_aov_channels(n) * scene->pixels);
+                NSP_HIP(hipMemsetAsync(scene->pix_aov.p, 0, sizeof(float) * pix_aov_channels(n) * scene->pixels, s));
             }
         }
         // the batches live in device memory; each batch's launch sequence
@@ -1087,7 +1102,7 @@ int nsp_image_aov(const nsp_scene* scene, float* albedo_sum, float* normal_sum, 
     return guarded(scene->ctx, [&] {
         const int64_t P = scene->pixels;
         const int n = scene->n;
-        std::vector<float> a((size_t)aov_channels(n) * P, 0.0f);
+        std::vector<float> a((size_t)pix_aov_channels(n) * P, 0.0f);
         if (scene->pix_aov.p)
             NSP_HIP(hipMemcpy(a.data(), scene->pix_aov.p, sizeof(float) * a.size(), hipMemcpyDeviceToHost));
         for (int64_t j = 0; j < P; ++j) {
@@ -1098,6 +1113,20 @@ int nsp_image_aov(const nsp_scene* scene, float* albedo_sum, float* normal_sum, 
             if (direct_sum)
                 for (int c = 0; c < 3; ++c) direct_sum[j * 3 + c] = a[(size_t)(3 + n + c) * P + j];
         }
+    });
+}
+
+int nsp_image_first_sample(const nsp_scene* scene, float* rgb_sum)
+{
+    if (!scene || !rgb_sum) return NSP_E_INVALID;
+    return guarded(scene->ctx, [&] {
+        const int64_t P = scene->pixels;
+        std::vector<float> a((size_t)3 * P, 0.0f);
+        if (scene->pix_aov.p)
+            NSP_HIP(hipMemcpy(a.data(), scene->pix_aov.as<float>() + (size_t)aov_channels(scene->n) * P,
+                              sizeof(float) * a.size(), hipMemcpyDeviceToHost));
+        for (int64_t j = 0; j < P; ++j)
+            for (int c = 0; c < 3; ++c) rgb_sum[j * 3 + c] = a[(size_t)c * P + j];
     });
 }
 

@@ -27,6 +27,7 @@
 #include "nsp.h"
 #include "nsp_device.hpp"
 #include "nsp_internal.hpp"
+#include "nsp_render.hpp"
 
 namespace nsp {
 
@@ -565,6 +566,40 @@ __global__ void k_pack_input(const float* __restrict__ in, int64_t V, int cin, i
         for (int c = 0; c < 3; ++c) resid[v * 3 + c] = resid_ch >= 0 ? in[v * cin + resid_ch + c] : 0.0f;
 }
 
+// The denoiser input of a 3D-screen scene from its per-pixel sums in device
+// memory (pixel order = voxel order, x fastest): channels [1-spp radiance (the
+// first-sample sums over the passes), 4-spp radiance, albedo, normal (4)],
+// each the mean over its samples, BF16, padded to 16; the residual is the
+// 4-spp radiance.  aovc: offset of the first-sample channels in aov.
+__global__ void k_scene_input(const float* __restrict__ sum, const int32_t* __restrict__ cnt,
+                              const float* __restrict__ aov, int aovc, int64_t P, int spp_pass,
+                              unsigned short* __restrict__ x, float* __restrict__ resid)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= P) return;
+    const int c = cnt[v];
+    const float inv = c > 0 ? 1.0f / (float)c : 0.0f;
+    const int passes = spp_pass > 0 ? c / spp_pass : 0;
+    const float inv1 = passes > 0 ? 1.0f / (float)passes : 0.0f;
+    float in[16];
+    for (int k = 0; k < 3; ++k) {
+        in[k] = aov[(int64_t)(aovc + k) * P + v] * inv1;
+        in[3 + k] = sum[(int64_t)k * P + v] * inv;
+        in[6 + k] = aov[(int64_t)k * P + v] * inv;
+        resid[v * 3 + k] = in[3 + k];
+    }
+    for (int k = 0; k < 4; ++k) in[9 + k] = aov[(int64_t)(3 + k) * P + v] * inv;
+    for (int k = 13; k < 16; ++k) in[k] = 0.0f;
+    for (int k = 0; k < 16; ++k) x[v * DN_IN + k] = f2bf(in[k]);
+}
+
+// pred = blend * pred + (1 - blend) * resid (the network's output blended with its input)
+__global__ void k_blend(float* __restrict__ pred, const float* __restrict__ resid, int64_t n, float blend)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) pred[i] = blend * pred[i] + (1.0f - blend) * resid[i];
+}
+
 // Random training patches: for patch i of the batch, a Philox-drawn volume and
 // corner; gathers inputs (f32 -> packed bf16), residual and target.
 __global__ void k_gather_patches(const float* __restrict__ vin, const float* __restrict__ vtgt, int S, int Dd, int Hh,
@@ -621,6 +656,16 @@ struct nsp_denoiser {
     DevBuf act[DN_LAYERS + 1];
     DevBuf out_f32, resid, target, pred, dy_a, dy_b, partial, bias_part, loss_part;
     int64_t cap_v = 0;
+    // device time of the last forward's eight convolutions (HIP events on the stream)
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_forward_ms = 0;
+    int64_t last_forward_voxels = 0;
+    int64_t last_output_voxels = 0;  // voxels of the prediction the last forward left in pred
+    ~nsp_denoiser()
+    {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+    }
 };
 
 namespace nsp {
@@ -782,6 +827,8 @@ float loss_and_grad(nsp_denoiser& d, Dims D, bool want_grad, float* loss_out = n
 }
 
 }  // namespace
+
+static Dims dims_of(int nb, int dd, int hh, int ww) { return Dims{nb, dd, hh, ww}; }
 
 extern "C" {
 
@@ -945,7 +992,88 @@ int nsp_denoiser_set_state(nsp_denoiser* d, const float* params, const float* ad
     }
 }
 
-static Dims dims_of(int nb, int dd, int hh, int ww) { return Dims{nb, dd, hh, ww}; }
+int nsp_denoiser_forward_scene(nsp_denoiser* d, const nsp_scene* scene, int samples_per_pass, float blend,
+                               float* output)
+{
+    if (!d || !scene || samples_per_pass < 1 || !(blend >= 0.0f && blend <= 1.0f)) return NSP_E_INVALID;
+    try {
+        NSP_HIP(hipSetDevice(d->device));
+        ScenePixels sp;
+        if (!scene_pixel_buffers(scene, &sp)) raise(NSP_E_INVALID, "the scene has no AOV sums (paint it with aov = 1)");
+        if (sp.n != 4) raise(NSP_E_UNSUPPORTED, "the denoiser takes the 3D screens of n = 4 scenes");
+        if (d->cin_real != 13 || d->resid_ch != 3)
+            raise(NSP_E_INVALID, "a scene's input has 13 channels (1-spp, 4-spp radiance, albedo, normal), residual 3");
+        if (sp.ctx != d->ctx) raise(NSP_E_INVALID, "the scene belongs to another context");
+        Dims D = dims_of(1, sp.screen[2], sp.screen[1], sp.screen[0]);
+        const int64_t V = D.voxels();
+        dn_alloc(*d, V);
+        hipStream_t s = d->stream;
+        hipLaunchKernelGGL(k_scene_input, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s, sp.sum, sp.cnt, sp.aov,
+                           aov_channels(4), V, samples_per_pass, d->act[0].as<unsigned short>(), d->resid.as<float>());
+        NSP_HIP(hipGetLastError());
+        if (!d->ev0) NSP_HIP(hipEventCreate(&d->ev0));
+        if (!d->ev1) NSP_HIP(hipEventCreate(&d->ev1));
+        NSP_HIP(hipEventRecord(d->ev0, s));
+        forward(*d, D);
+        NSP_HIP(hipEventRecord(d->ev1, s));
+        if (d->head == NSP_DENOISER_KERNEL)
+            hipLaunchKernelGGL(k_kp_loss, dim3(LOSS_BLOCKS), dim3(256), 0, s, d->out_f32.as<float>(),
+                               d->resid.as<float>(), (const float*)nullptr, D, 1.0f, (unsigned short*)nullptr,
+                               d->pred.as<float>(), (float*)nullptr);
+        else
+            hipLaunchKernelGGL(k_loss, dim3(LOSS_BLOCKS), dim3(256), 0, s, d->out_f32.as<float>(), d->resid.as<float>(),
+                               (const float*)nullptr, V, 1.0f, (unsigned short*)nullptr, d->pred.as<float>(),
+                               (float*)nullptr);
+        if (blend < 1.0f)
+            hipLaunchKernelGGL(k_blend, dim3((unsigned)((3 * V + 255) / 256)), dim3(256), 0, s, d->pred.as<float>(),
+                               d->resid.as<float>(), 3 * V, blend);
+        NSP_HIP(hipGetLastError());
+        if (output) NSP_HIP(hipMemcpyAsync(output, d->pred.p, sizeof(float) * V * 3, hipMemcpyDeviceToHost, s));
+        NSP_HIP(hipStreamSynchronize(s));
+        float ms = 0;
+        NSP_HIP(hipEventElapsedTime(&ms, d->ev0, d->ev1));
+        d->last_forward_ms = ms;
+        d->last_forward_voxels = V;
+        d->last_output_voxels = V;
+        return NSP_OK;
+    } catch (const Error& e) {
+        *d->err = e.msg;
+        return e.code;
+    }
+}
+
+int nsp_denoiser_output(const nsp_denoiser* d, float* output)
+{
+    if (!d || !output) return NSP_E_INVALID;
+    try {
+        NSP_HIP(hipSetDevice(d->device));
+        NSP_HIP(hipMemcpyAsync(output, d->pred.p, sizeof(float) * d->last_output_voxels * 3, hipMemcpyDeviceToHost,
+                               d->stream));
+        NSP_HIP(hipStreamSynchronize(d->stream));
+        return NSP_OK;
+    } catch (const Error& e) {
+        *d->err = e.msg;
+        return e.code;
+    }
+}
+
+int nsp_denoiser_last_forward(const nsp_denoiser* d, double* ms, double* flops)
+{
+    if (!d) return NSP_E_INVALID;
+    if (ms) *ms = d->last_forward_ms;
+    if (flops) {
+        // multiply-adds on the real channels: 2 * 27 * (cin * cout) per voxel and layer
+        double per = 0;
+        for (int l = 0; l < DN_LAYERS; ++l) {
+            const int ci = l == 0 ? d->cin_real : DN_WIDTH;
+            const int co = l == DN_LAYERS - 1 ? (d->head == NSP_DENOISER_KERNEL ? 27 : 3) : DN_WIDTH;
+            per += 2.0 * 27 * ci * co;
+        }
+        *flops = per * (double)d->last_forward_voxels;
+    }
+    return NSP_OK;
+}
+
 
 int nsp_denoiser_forward(nsp_denoiser* d, const float* input, int nb, int depth, int height, int width, float* output)
 {
@@ -962,7 +1090,11 @@ int nsp_denoiser_forward(nsp_denoiser* d, const float* input, int nb, int depth,
         hipLaunchKernelGGL(k_pack_input, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s, in.as<float>(), V,
                            d->cin_real, d->resid_ch, d->act[0].as<unsigned short>(), d->resid.as<float>());
         NSP_HIP(hipGetLastError());
+        if (!d->ev0) NSP_HIP(hipEventCreate(&d->ev0));
+        if (!d->ev1) NSP_HIP(hipEventCreate(&d->ev1));
+        NSP_HIP(hipEventRecord(d->ev0, s));
         forward(*d, D);
+        NSP_HIP(hipEventRecord(d->ev1, s));
         if (d->head == NSP_DENOISER_KERNEL)
             hipLaunchKernelGGL(k_kp_loss, dim3(1024), dim3(256), 0, s, d->out_f32.as<float>(), d->resid.as<float>(),
                                (const float*)nullptr, D, 1.0f, (unsigned short*)nullptr, d->pred.as<float>(),
@@ -974,6 +1106,11 @@ int nsp_denoiser_forward(nsp_denoiser* d, const float* input, int nb, int depth,
         NSP_HIP(hipGetLastError());
         NSP_HIP(hipMemcpyAsync(output, d->pred.p, sizeof(float) * V * 3, hipMemcpyDeviceToHost, s));
         NSP_HIP(hipStreamSynchronize(s));
+        float ms = 0;
+        NSP_HIP(hipEventElapsedTime(&ms, d->ev0, d->ev1));
+        d->last_forward_ms = ms;
+        d->last_forward_voxels = V;
+        d->last_output_voxels = V;
         return NSP_OK;
     } catch (const Error& e) {
         *d->err = e.msg;

@@ -124,6 +124,19 @@ struct TreeDev {
     int64_t entry_pad = 0;
 };
 
+// A scene's per-pixel sums in device memory (nsp_api.cpp keeps the scene):
+// sum [3][pixels], cnt [pixels], aov [pix_aov_channels(n)][pixels].
+struct ScenePixels {
+    nsp_context* ctx = nullptr;
+    int n = 0;
+    int64_t pixels = 0;
+    int screen[NSP_MAX_DIM - 1] = {};
+    const float* sum = nullptr;
+    const int32_t* cnt = nullptr;
+    const float* aov = nullptr;
+};
+bool scene_pixel_buffers(const nsp_scene* s, ScenePixels* out);
+
 struct MeshDev {
     int n = 0;
     int64_t V = 0, F = 0, VN = 0;

@@ -736,6 +736,11 @@ __global__ void __launch_bounds__(ACC_PIX) k_accumulate(const BatchDesc* __restr
             acc_channel(tile, aov + c * cap, nullptr, first, S, R, P, inv_s, &s);
             if ((int)threadIdx.x < P) pix_aov[c * pixels + px] += s;
         }
+        if (aov && (int)threadIdx.x < P) {
+            // the radiance of the pixel's first sample: a one-sample image per pass
+            const int64_t f0 = first + (int64_t)threadIdx.x * S;
+            for (int c = 0; c < 3; ++c) pix_aov[(aovc + c) * pixels + px] += L[c * cap + f0] + Ls[c * cap + f0];
+        }
     }
 }
 
@@ -776,6 +781,8 @@ __global__ void __launch_bounds__(256) k_accumulate_wide(const BatchDesc* __rest
             v = wave_sum(v);
             if (lane == 0) pix_aov[c * pixels + px] += v;
         }
+        if (aov && lane == 0)
+            for (int c = 0; c < 3; ++c) pix_aov[(aovc + c) * pixels + px] += L[c * cap + first] + Ls[c * cap + first];
     }
 }
 

@@ -97,6 +97,9 @@ struct RayQueue {
 // radiance the first hit receives directly (3: background or emitter seen by
 // the primary ray plus the unoccluded first-hit light contributions).
 __host__ __device__ constexpr int aov_channels(int n) { return 6 + n; }
+// Per-pixel AOV sums: the per-path channels, then the radiance of each pass's
+// first sample of the pixel (3: one sample per pixel and pass).
+__host__ __device__ constexpr int pix_aov_channels(int n) { return aov_channels(n) + 3; }
 
 template <typename T>
 struct ShadowQueue {

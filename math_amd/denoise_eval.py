@@ -4,18 +4,19 @@ the time-to-RMSE comparison of denoised against plain FP32 rendering.
   variants   n=4 tesseract-boundary meshes of the C1 generator with radial noise
              sigma ~ U[0, 0.05] and a point light at distance 4 in a direction
              uniform on S^3 (seeded)
-  inputs     13 channels per voxel of the 3D screen: the 4-spp (8-sample)
-             radiance and, from the same pass, the first-hit albedo, normal and
-             direct light (the radiance the first hit receives from the light
-             and the background it sees).
+  inputs     13 channels per voxel of the 3D screen: the 1-spp radiance (the
+             first sample of each pixel), the 4-spp (8-sample) radiance, and the
+             first-hit albedo and normal, all from one 4-spp pass with AOVs.
              Rendering is progressive: at level j the slots hold j
-             eight-sample passes, so level 1 is the plain 4-spp input and the
-             network can be applied at any sample budget; a level costs what a
-             plain 4-spp pass costs, plus the AOV writes
-  target     FP32 render of target_spp samples per pixel (2048 spp requests
-             13^3 = 2197 stratified samples)
+             eight-sample passes (the 1-spp slot the j first samples), so
+             level 1 is the plain 1-spp + 4-spp input and the network can be
+             applied at any sample budget; a level costs what a plain 4-spp
+             pass costs, plus the AOV writes
+  target     FP32 render of target_spp samples per pixel (1024 spp requests
+             11^3 = 1331 stratified samples)
   network    8 layers 3x3x3, 32 channels, BF16/FP32-accumulate, residual on the
              4-spp radiance, trained on 32^3 patches, batch 16, Adam
+  variants   64 for training (seeds 100-163), 16 held out (seeds 200-215)
   judge      RMSE over pixels x 3 linear-RGB channels against a 4096-sample FP64
              render of the same scene; the plain render is the fallback
 """
@@ -26,7 +27,9 @@ import numpy as np
 from . import configs, native
 
 IN_CHANNELS = 13
-RESIDUAL = 0  # the 4-spp radiance channels
+RESIDUAL = 3  # the 4-spp radiance channels (after the 1-spp radiance)
+TRAIN_SEEDS = tuple(range(100, 164))
+HELDOUT_SEEDS = tuple(range(200, 216))
 LEVELS = (1, 2, 4, 8, 16, 32, 64, 128)  # progressive levels stored for training
 THRESHOLDS = (0.03, 0.02, 0.01)
 
@@ -81,9 +84,10 @@ class ProgressiveInputs:
 
     def volume(self):
         r4, c4 = self.s4.image()
-        alb, nrm, dl = self.s4.aov()
+        alb, nrm, _ = self.s4.aov()
+        r1 = self.s4.first_sample()
         k4 = np.maximum(c4, 1)[:, None]
-        x = np.concatenate([r4 / k4, alb / k4, nrm / k4, dl / k4], axis=1)
+        x = np.concatenate([r1 / max(self.level, 1), r4 / k4, alb / k4, nrm / k4], axis=1)
         return to_volume(x.astype(np.float32), self.screen)
 
 
@@ -184,11 +188,11 @@ def time_to_rmse(ctx, mesh, cfg, ref, denoiser, depth=5, thresholds=THRESHOLDS, 
 
       plain     progressive 4-spp (8-sample) FP32 passes
       denoised  progressive levels of ProgressiveInputs (8 samples per
-                level); after each level the network maps the input to an
-                image, level 1 being the standard 4-spp input, blended
-                with the 4-spp input by the level's fitted weight when alphas
-                (fit_blend) are given.  The time at a level is its passes plus
-                one inference.
+                level); after each level the network maps the input, built on
+                the device from the scene's sums, to an image (level 1 being
+                the standard 1-spp + 4-spp input), blended with the 4-spp input
+                by the level's fitted weight when alphas (fit_blend) are
+                given.  The time at a level is its passes plus one inference.
     """
     screen = cfg["screen"]
     goal = min(thresholds)
@@ -203,19 +207,26 @@ def time_to_rmse(ctx, mesh, cfg, ref, denoiser, depth=5, thresholds=THRESHOLDS, 
         if e <= goal:
             break
     out["plain_curve"] = curve
+    out["plain_rays"] = sc.statistics()["rays"]
+    out["plain_paint_s"] = t
     p = ProgressiveInputs(ctx, mesh, cfg, seed + 1, depth)
     dcurve = []
+    conv_ms, conv_flops = [], 0.0
+    P = int(np.prod(screen))
     for k in range(max_levels):
         p.advance()
-        x = p.volume()
+        # the network reads the scene's sums on the device and leaves its
+        # (blended) image there; reading it back to score it is not timed
+        a = blend_weight(alphas, p.level) if alphas is not None else 1.0
         t1 = time.perf_counter()
-        y = denoiser.forward(x[None])[0]
-        if alphas is not None:
-            a = blend_weight(alphas, p.level)
-            y = a * y + (1 - a) * x[..., RESIDUAL:RESIDUAL + 3]
+        denoiser.forward_scene(p.s4, 8, blend=a, copy=False)
         dt = time.perf_counter() - t1
+        y = to_volume(denoiser.output(P), screen)
+        ms, conv_flops = denoiser.last_forward()
+        conv_ms.append(ms)
         e = rmse(y, ref)
         if k == 0:
+            x = p.volume()
             out["denoised_rmse"] = e
             out["input4_rmse"] = rmse(x[..., RESIDUAL:RESIDUAL + 3], ref)
             out["denoiser_inference_s"] = dt
@@ -224,6 +235,10 @@ def time_to_rmse(ctx, mesh, cfg, ref, denoiser, depth=5, thresholds=THRESHOLDS, 
         if e <= goal:
             break
     out["denoised_curve"] = dcurve
+    out["denoised_rays"] = p.s4.statistics()["rays"]
+    out["denoised_paint_s"] = p.secs
+    out["conv_forward_ms"] = float(np.median(conv_ms))
+    out["conv_forward_flops"] = conv_flops
     out["thresholds"] = {}
     for thr in thresholds:
         tp, td = first_time(curve, thr), first_time(dcurve, thr)

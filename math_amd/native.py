@@ -117,10 +117,11 @@ EXPORTS = [
     "nsp_mesh_create", "nsp_mesh_destroy", "nsp_mesh_min_max", "nsp_mesh_get_info", "nsp_mesh_records",
     "nsp_mesh_tree_dump", "nsp_trace_first_hit", "nsp_trace_first_hit_device", "nsp_trace_first_hit_noalloc",
     "nsp_scene_create",
-    "nsp_scene_destroy", "nsp_paint", "nsp_statistics", "nsp_image_get", "nsp_image_set", "nsp_image_aov",
+    "nsp_scene_destroy", "nsp_paint", "nsp_statistics", "nsp_image_get", "nsp_image_set", "nsp_image_aov", "nsp_image_first_sample",
     "nsp_scene_pixel_count", "nsp_last_pass_stats", "nsp_set_counters", "nsp_meshgen_torus",
     "nsp_meshgen_cube_sphere", "nsp_denoiser_create", "nsp_denoiser_create_head", "nsp_denoiser_destroy",
-    "nsp_denoiser_param_count", "nsp_denoiser_get_state", "nsp_denoiser_set_state",
+    "nsp_denoiser_param_count", "nsp_denoiser_get_state", "nsp_denoiser_set_state", "nsp_denoiser_last_forward",
+    "nsp_denoiser_forward_scene", "nsp_denoiser_output",
     "nsp_denoiser_get_params", "nsp_denoiser_set_params", "nsp_denoiser_forward", "nsp_denoiser_loss_grad",
     "nsp_denoiser_train", "nsp_conv3d_bf16", "nsp_batch_create", "nsp_batch_destroy", "nsp_batch_paint",
     "nsp_batch_statistics", "nsp_batch_set_lanes", "nsp_batch_get_lanes", "nsp_batch_autotune", "nsp_autotune", "nsp_mesh_set_record_layout", "nsp_mesh_get_record_layout",
@@ -171,6 +172,7 @@ def lib():
         L.nsp_image_get.argtypes = [P, fp, ip]
         L.nsp_image_set.argtypes = [P, fp, ip]
         L.nsp_image_aov.argtypes = [P, fp, fp, fp]
+        L.nsp_image_first_sample.argtypes = [P, fp]
         L.nsp_scene_pixel_count.argtypes = [P]
         L.nsp_scene_pixel_count.restype = C.c_int64
         L.nsp_last_pass_stats.argtypes = [P, C.POINTER(StageStats)]
@@ -584,6 +586,12 @@ class Scene:
                                                  _ptr(dl, C.c_float)))
         return a, nrm, dl
 
+    def first_sample(self):
+        """Per-pixel sums of the first sample's radiance of each pass painted with aov (nsp_image_first_sample)."""
+        r = np.zeros((self.pixels, 3), np.float32)
+        self.ctx.check(self.ctx._L.nsp_image_first_sample(self.h, _ptr(r, C.c_float)))
+        return r
+
     def set_shapes(self, shapes):
         """nsp_scene_set_shapes: a list of Shape (make_shape), after the mesh in object order."""
         arr = (Shape * max(1, len(shapes)))(*shapes)
@@ -626,6 +634,9 @@ def _denoiser_lib():
         L.nsp_denoiser_get_params.argtypes = [P, fp]
         L.nsp_denoiser_set_params.argtypes = [P, fp]
         L.nsp_denoiser_get_state.argtypes = [P, fp, fp, fp, C.POINTER(C.c_int64)]
+        L.nsp_denoiser_last_forward.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.nsp_denoiser_forward_scene.argtypes = [P, P, C.c_int, C.c_float, fp]
+        L.nsp_denoiser_output.argtypes = [P, fp]
         L.nsp_denoiser_set_state.argtypes = [P, fp, fp, fp, C.c_int64]
         L.nsp_denoiser_forward.argtypes = [P, fp, C.c_int, C.c_int, C.c_int, C.c_int, fp]
         L.nsp_denoiser_loss_grad.argtypes = [P, fp, fp, C.c_int, C.c_int, C.c_int, C.c_int, fp, fp]
@@ -696,6 +707,25 @@ class Denoiser:
         p = np.ascontiguousarray(p, np.float32)
         assert p.size == self.nparams
         self.ctx.check(_denoiser_lib().nsp_denoiser_set_params(self.h, _ptr(p, C.c_float)))
+
+    def forward_scene(self, scene, samples_per_pass, blend=1.0, copy=True):
+        """Denoised image [pixels, 3] of a scene painted with aov (nsp_denoiser_forward_scene);
+        copy=False leaves it on the device (output() copies it)."""
+        out = np.zeros((scene.pixels, 3), np.float32) if copy else None
+        self.ctx.check(_denoiser_lib().nsp_denoiser_forward_scene(self.h, scene.h, int(samples_per_pass), float(blend),
+                                                                  _ptr(out, C.c_float)))
+        return out
+
+    def output(self, pixels):
+        out = np.zeros((pixels, 3), np.float32)
+        self.ctx.check(_denoiser_lib().nsp_denoiser_output(self.h, _ptr(out, C.c_float)))
+        return out
+
+    def last_forward(self):
+        """(device ms of the last forward's convolutions, their multiply-add FLOPs)."""
+        ms, fl = C.c_double(), C.c_double()
+        self.ctx.check(_denoiser_lib().nsp_denoiser_last_forward(self.h, C.byref(ms), C.byref(fl)))
+        return ms.value, fl.value
 
     def state(self):
         """(params, Adam first moments, Adam second moments, Adam step count)."""

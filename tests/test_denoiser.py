@@ -208,6 +208,32 @@ def test_denoiser_training_reduces_loss(ctx):
 
 
 @pytest.mark.gpu
+def test_forward_scene_equals_host_input(ctx):
+    # the device-built input of a painted scene (nsp_denoiser_forward_scene) gives
+    # the image of the host-built volume through nsp_denoiser_forward, blended
+    from math_amd import denoise_eval as DE
+    from math_amd import native
+    v, f, cfg = DE.variant(205, 128, ([40, 50, 60], 24), g=8)
+    mesh = ctx.mesh(v, f)
+    p = DE.ProgressiveInputs(ctx, mesh, cfg, seed=3)
+    p.advance()
+    p.advance()
+    d = native.Denoiser(ctx, DE.IN_CHANNELS, DE.RESIDUAL, seed=4)
+    x = p.volume()
+    a = 0.7
+    y_host = a * d.forward(x[None])[0] + (1 - a) * x[..., DE.RESIDUAL:DE.RESIDUAL + 3]
+    y_dev = DE.to_volume(d.forward_scene(p.s4, 8, blend=a), cfg["screen"])
+    err = np.max(np.abs(y_dev - y_host)) / np.max(np.abs(y_host))
+    print(f"device vs host input: max rel err {err:.2e}")
+    assert err < 1e-2
+    ms, flops = d.last_forward()
+    assert ms > 0 and flops == pytest.approx(2 * 27 * (13 * 32 + 6 * 32 * 32 + 32 * 3) * 24 ** 3)
+    # left on the device, then copied
+    d.forward_scene(p.s4, 8, blend=a, copy=False)
+    assert np.array_equal(DE.to_volume(d.output(p.s4.pixels), cfg["screen"]), y_dev)
+
+
+@pytest.mark.gpu
 def test_training_resumes_from_checkpoint(ctx):
     # 40 steps in one call equal 25 steps, a checkpoint (parameters, Adam moments,
     # step count) restored into a new denoiser, and 15 more steps: bit for bit

@@ -509,3 +509,27 @@ def test_cpp_single_object_scene_conventions():
     assert light[0] == native.LIGHT_CONSTANT
     assert np.allclose(light[1:5], mx + 100 * (mx - centre), rtol=1e-14, atol=1e-12)
     assert light[5:] == [1.0, 1.0, 1.0]
+
+
+def test_first_sample_aov(ctx):
+    """nsp_image_first_sample: per pass, the radiance of each pixel's first sample
+    (the denoiser's 1-spp input).  With one sample per pixel it is the image
+    itself; with 8 it is the checker's path colour of sample 0, summed over the
+    passes."""
+    v, f, cfg, _, _ = small_scene(4, 4, 8)
+    gm = ctx.mesh(v, f)
+    one = native.Scene(ctx, gm, cfg)
+    one.paint(1, 3, seed=6, aov=True)
+    assert np.array_equal(one.first_sample(), one.image()[0])
+    sc = native.Scene(ctx, gm, cfg)
+    sc.paint(4, 3, precision=native.PRECISION_FP64, seed=6, aov=True, passes=2)
+    fs = sc.first_sample()
+    om = O.Mesh(v, f)
+    om.build_tree()
+    rng = np.random.default_rng(2)
+    for pixel in rng.choice(sc.pixels, size=60, replace=False):
+        expect = np.zeros(3, np.float32)
+        for k in range(2):
+            rgb, _ = O.path_sample(om, O.make_scene(cfg, 4, 3, 6, pass_index=k), int(pixel), 0)
+            expect += rgb
+        assert np.allclose(fs[pixel], expect, rtol=1e-5, atol=1e-6), (pixel, fs[pixel], expect)
