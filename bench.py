@@ -151,10 +151,12 @@ def denoise_eval(ctx, scenes, args):
         per_scene.append({"seed": r["seed"], **{t: r["thresholds"][t] for t in thr},
                           "level1_rmse": {"input_4spp": round(r["input4_rmse"], 5),
                                           "denoised": round(r["denoised_rmse"], 5)}})
-    geo = {}
+    geo, reached = {}, {}
     for t in thr:
         sp = [r["thresholds"][t]["speedup"] for r in results if r["thresholds"][t]["speedup"]]
         geo[t] = float(np.exp(np.mean(np.log(sp)))) if sp else None
+        # scenes where both curves reached the threshold within 256 levels of 8 samples
+        reached[t] = f"{len(sp)} of {len(results)}"
     plain_rays = sum(r["plain_rays"] for r in results)
     plain_s = sum(r["plain_paint_s"] for r in results)
     dn_rays = sum(r["denoised_rays"] for r in results)
@@ -168,6 +170,7 @@ def denoise_eval(ctx, scenes, args):
                   f"{DE.HELDOUT_SEEDS[len(results) - 1]})",
         "per_scene": per_scene,
         "geomean_speedup": geo,
+        "geomean_over": reached,
         "plain": {"Mrays_s": round(plain_rays / plain_s / 1e6, 1),
                   "roofline": "the traversal kernels of the Mrays/s line (same kernels, 4-spp passes)"},
         "denoised": {"Mrays_s": round(dn_rays / dn_s / 1e6, 1),

@@ -159,8 +159,12 @@ def fit_blend(denoiser, X, Y, levels=LEVELS, batch=4):
 
 def blend_weight(alphas, level):
     """alpha at a progressive level: linear in log2(level) between the fitted
-    levels, the end values outside them."""
+    levels, the first value below them, and above the last fitted level L the
+    last value times L / level (the input's variance, which the network
+    removes, falls as 1 / level, while its own error does not)."""
     lv = sorted(alphas)
+    if level > lv[-1]:
+        return float(alphas[lv[-1]] * lv[-1] / level)
     x = np.log2(np.asarray(lv, np.float64))
     return float(np.interp(np.log2(max(level, 1)), x, [alphas[k] for k in lv]))
 
@@ -178,7 +182,7 @@ def first_time(curve, threshold):
     return None
 
 
-def time_to_rmse(ctx, mesh, cfg, ref, denoiser, depth=5, thresholds=THRESHOLDS, max_passes=256, max_levels=160,
+def time_to_rmse(ctx, mesh, cfg, ref, denoiser, depth=5, thresholds=THRESHOLDS, max_passes=256, max_levels=256,
                  seed=77, alphas=None):
     """Error-versus-time curves of both methods against `ref`, and the time at
     which each first reaches each threshold.  Only GPU work is timed (paint

@@ -305,3 +305,6 @@ def test_blend_weights_closed_form():
         assert all(abs(v - expect) < 1e-5 for v in a.values()), (a, expect)
     a = {1: 1.0, 4: 0.0}
     assert DE.blend_weight(a, 2) == 0.5 and DE.blend_weight(a, 1000) == 0.0 and DE.blend_weight(a, 1) == 1.0
+    # above the last fitted level the weight falls as 1 / level
+    b = {1: 1.0, 8: 0.4}
+    assert DE.blend_weight(b, 16) == pytest.approx(0.2) and DE.blend_weight(b, 80) == pytest.approx(0.04)
