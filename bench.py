@@ -185,6 +185,32 @@ def denoise_eval(ctx, scenes, args):
     }
 
 
+def denoiser_forward(args):
+    """Five forwards of the shipped network on one 128^3 volume of 13 random input
+    channels (the inference shape of the time-to-RMSE evaluation), with the
+    device time of the eight convolutions: the workload of the denoiser's
+    profiles."""
+    import numpy as np
+
+    from math_amd import denoise_eval as DE
+    from math_amd import denoise_train as DT
+    from math_amd import native
+
+    ctx = native.Context(0)
+    dn = native.Denoiser(ctx, DE.IN_CHANNELS, DE.RESIDUAL, seed=1)
+    dn.set_params(DT.load()[0])
+    x = np.random.default_rng(0).random(size=(1, 128, 128, 128, DE.IN_CHANNELS)).astype(np.float32)
+    ms = []
+    for k in range(5):
+        dn.forward(x)
+        ms.append(dn.last_forward()[0])
+        log(f"forward {k}: {ms[-1]:.3f} ms on the device")
+    flops = dn.last_forward()[1]
+    print(json.dumps({"workload": "denoiser forward, 128^3 voxels, 13 -> 32 x 7 -> 3 channels, 3x3x3",
+                      "device_ms": ms, "flops": flops,
+                      "tflops_best": round(flops / (min(ms) * 1e-3) / 1e12, 1)}), flush=True)
+
+
 def denoise_bench(args):
     from math_amd import native
 
@@ -375,7 +401,11 @@ def main():
     ap.add_argument("--scenes", type=int, default=16,
                     help="C4: the first scenes of the 256 seeded scenes (all 256 meshes do not fit beside the workspaces)")
     ap.add_argument("--lanes", type=int, default=0, help="batch lanes (0: autotune lanes and queue size)")
+    ap.add_argument("--denoiser-forward", action="store_true",
+                    help="five 128^3 forwards of the shipped denoiser (the workload of its profiles)")
     args = ap.parse_args()
+    if args.denoiser_forward:
+        return denoiser_forward(args)
     if args.denoise:
         return denoise_bench(args)
     if args.batch_scenes:

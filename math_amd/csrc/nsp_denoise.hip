@@ -89,8 +89,8 @@ __device__ __forceinline__ int64_t neighbour(const Vox& p, int tap, const Dims& 
 // ---------------------------------------------------------------------------
 // Implicit-GEMM convolution (forward, and backward-data with flipped weights).
 // A block computes a 16 (x) x 8 (y) x 4 (z) tile of output voxels.  Its input
-// halo, 18 x 10 x 6 voxels x CIN channels, is staged once in LDS and read there
-// for all 27 taps.  Wave w takes the z-slice w: eight M-tiles (the x-rows
+// halo, 18 x 10 x 6 voxels x CIN channels, is staged once in LDS (8-channel
+// groups outermost) and read there for all 27 taps.  Wave w takes the z-slice w: eight M-tiles (the x-rows
 // y = 0..7, a lane row per voxel) by all output channels (NT N-tiles of 16).
 // The weights are packed in MFMA fragment order, [K-step][N-tile][lane][8],
 // so each lane's B fragment is one 16-byte load (the 54 KB of one layer stay
@@ -112,10 +112,48 @@ struct TileGrid {
     int tx, ty, tz;
 };
 
+// the epilogue's voxel stride in floats: COUT + 4 puts the four row groups of a
+// wave's writes in different banks and keeps 16-byte alignment
+template <int NT>
+constexpr int epi_stride()
+{
+    return NT * 16 + 4;
+}
+
 template <int CIN, int NT>
 constexpr int conv_lds_bytes()
 {
-    return CH_VOX * CIN * 2 > CT_VOX * NT * 16 * 4 ? CH_VOX * CIN * 2 : CT_VOX * NT * 16 * 4;
+    return CH_VOX * CIN * 2 > CT_VOX * epi_stride<NT>() * 4 ? CH_VOX * CIN * 2 : CT_VOX * epi_stride<NT>() * 4;
+}
+
+// Stages n_vox voxels of CIN channels (8-channel groups outermost) from global
+// memory into LDS.  Unit u: a 16-lane group takes 16 consecutive voxels of one
+// group, so a wave reads 16 voxels x 64 bytes contiguously and each group
+// writes 256 consecutive bytes of LDS; every load of a thread is issued before
+// its stores.  src(hv) gives the voxel's global index, or -1 outside the volume.
+template <int CIN, int NVOX, int THREADS, typename Src>
+__device__ __forceinline__ void stage_groups(const unsigned short* __restrict__ X, unsigned short* lds, Src src)
+{
+    constexpr int UPV = CIN / 8;
+    constexpr int NU = ((NVOX + 15) / 16) * 16 * UPV;
+    constexpr int ITER = (NU + THREADS - 1) / THREADS;
+    uint4 buf[ITER];
+#pragma unroll
+    for (int k = 0; k < ITER; ++k) {
+        const int u = (int)threadIdx.x + k * THREADS;
+        const int hv = (u >> 4) / UPV * 16 + (u & 15), part = (u >> 4) % UPV;
+        buf[k] = uint4{0u, 0u, 0u, 0u};
+        if (u < NU && hv < NVOX) {
+            const int64_t g = src(hv);
+            if (g >= 0) buf[k] = *reinterpret_cast<const uint4*>(X + g * CIN + part * 8);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < ITER; ++k) {
+        const int u = (int)threadIdx.x + k * THREADS;
+        const int hv = (u >> 4) / UPV * 16 + (u & 15), part = (u >> 4) % UPV;
+        if (u < NU && hv < NVOX) *reinterpret_cast<uint4*>(lds + (part * NVOX + hv) * 8) = buf[k];
+    }
 }
 
 template <int CIN, int NT>
@@ -127,7 +165,7 @@ __global__ void __launch_bounds__(256) k_conv(const unsigned short* __restrict__
 {
     constexpr int KS = (27 * CIN + 31) / 32;
     constexpr int COUT = NT * 16;
-    constexpr int UPV = CIN / 8;  // 16-byte units per voxel
+    constexpr int ES = epi_stride<NT>();
     __shared__ __align__(16) unsigned char lds[conv_lds_bytes<CIN, NT>()];
     unsigned short* halo = reinterpret_cast<unsigned short*>(lds);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -139,15 +177,14 @@ __global__ void __launch_bounds__(256) k_conv(const unsigned short* __restrict__
     const int tz = t % G.tz;
     const int b = t / G.tz;
     const int x0 = tx * CT_X, y0 = ty * CT_Y, z0 = tz * CT_Z;
-    for (int u = tid; u < CH_VOX * UPV; u += 256) {
-        const int hv = u / UPV, part = u % UPV;
+    // channel-group-major halo: the 16 lanes of an A read take 16 consecutive
+    // voxels of one 8-channel group, 16 consecutive bytes apart
+    stage_groups<CIN, CH_VOX, 256>(X, halo, [&](int hv) -> int64_t {
         const int hx = hv % CH_X, hy = (hv / CH_X) % CH_Y, hz = hv / (CH_X * CH_Y);
         const int x = x0 + hx - 1, y = y0 + hy - 1, z = z0 + hz - 1;
-        uint4 val = {0u, 0u, 0u, 0u};
-        if (x >= 0 && x < D.w && y >= 0 && y < D.h && z >= 0 && z < D.d)
-            val = *reinterpret_cast<const uint4*>(X + ((((int64_t)b * D.d + z) * D.h + y) * D.w + x) * CIN + part * 8);
-        *reinterpret_cast<uint4*>(halo + hv * CIN + part * 8) = val;
-    }
+        if (x < 0 || x >= D.w || y < 0 || y >= D.h || z < 0 || z >= D.d) return -1;
+        return (((int64_t)b * D.d + z) * D.h + y) * D.w + x;
+    });
     __syncthreads();
     const int row = lane & 15, kg = lane >> 4;
     floatx4 acc[CT_Y][NT];
@@ -173,10 +210,10 @@ __global__ void __launch_bounds__(256) k_conv(const unsigned short* __restrict__
         const bool real = tap_raw < 27;
         const int tap = real ? tap_raw : 26;
         const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
-        const unsigned short* src = halo + (((wave + dz) * CH_Y + dy) * CH_X + row + dx) * CIN + c0;
+        const unsigned short* src = halo + ((c0 / 8) * CH_VOX + ((wave + dz) * CH_Y + dy) * CH_X + row + dx) * 8;
 #pragma unroll
         for (int m = 0; m < CT_Y; ++m) {
-            bf16x8 a = *reinterpret_cast<const bf16x8*>(src + m * CH_X * CIN);
+            bf16x8 a = *reinterpret_cast<const bf16x8*>(src + m * CH_X * 8);
             if (!real) a = zero8();
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt)
@@ -192,7 +229,7 @@ __global__ void __launch_bounds__(256) k_conv(const unsigned short* __restrict__
         for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
             for (int i = 0; i < 4; ++i)
-                epi[((wave * CT_Y + m) * CT_X + kg * 4 + i) * COUT + nt * 16 + row] = acc[m][nt][i];
+                epi[((wave * CT_Y + m) * CT_X + kg * 4 + i) * ES + nt * 16 + row] = acc[m][nt][i];
     __syncthreads();
     constexpr int UPO = COUT / 8;
     for (int u = tid; u < CT_VOX * UPO; u += 256) {
@@ -200,8 +237,8 @@ __global__ void __launch_bounds__(256) k_conv(const unsigned short* __restrict__
         const int x = x0 + vl % CT_X, y = y0 + (vl / CT_X) % CT_Y, z = z0 + vl / (CT_X * CT_Y);
         if (x >= D.w || y >= D.h || z >= D.d) continue;
         const int64_t v = (((int64_t)b * D.d + z) * D.h + y) * D.w + x;
-        const float4 lo = *reinterpret_cast<const float4*>(epi + vl * COUT + part * 8);
-        const float4 hi = *reinterpret_cast<const float4*>(epi + vl * COUT + part * 8 + 4);
+        const float4 lo = *reinterpret_cast<const float4*>(epi + vl * ES + part * 8);
+        const float4 hi = *reinterpret_cast<const float4*>(epi + vl * ES + part * 8 + 4);
         float r[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
         if (bias)
 #pragma unroll
@@ -245,7 +282,48 @@ constexpr int WT_X = 16, WT_Y = 8, WT_Z = 2;
 constexpr int WH_X = 24, WH_Y = WT_Y + 2, WH_Z = WT_Z + 2;  // the halo's x rows: 18 used, padded to 24
 constexpr int WT_VOX = WT_X * WT_Y * WT_Z;
 constexpr int WH_PLANE = WH_X * WH_Y * WH_Z;
+// channel rows padded by 16 bytes: the 16 lanes of a fragment read 16 channel
+// rows, which then start in different LDS banks
+constexpr int WT_ROW = WT_VOX + 8;
+constexpr int WH_ROW = WH_PLANE + 8;
 constexpr int DW_MAX_BLOCKS = 512;
+
+// Stages n_vox voxels of C channels from global memory into LDS transposed,
+// [channel][voxel] with row stride ROW, for a block of 192 threads: a 16-lane
+// group loads 16 consecutive voxels of one 8-channel group, and loads are
+// issued in batches of 5 before their stores.  src(v): global voxel index or -1.
+template <int C, int NVOX, int ROW, typename Src>
+__device__ __forceinline__ void stage_transposed(const unsigned short* __restrict__ X, unsigned short* lds, Src src)
+{
+    constexpr int UPV = C / 8;
+    constexpr int NU = ((NVOX + 15) / 16) * 16 * UPV;
+    constexpr int BATCH = 5;
+#pragma unroll 1
+    for (int u0 = 0; u0 < NU; u0 += 192 * BATCH) {
+        uint4 buf[BATCH];
+#pragma unroll
+        for (int k = 0; k < BATCH; ++k) {
+            const int u = u0 + (int)threadIdx.x + k * 192;
+            const int hv = (u >> 4) / UPV * 16 + (u & 15), part = (u >> 4) % UPV;
+            buf[k] = uint4{0u, 0u, 0u, 0u};
+            if (u < NU && hv < NVOX) {
+                const int64_t g = src(hv);
+                if (g >= 0) buf[k] = *reinterpret_cast<const uint4*>(X + g * C + part * 8);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < BATCH; ++k) {
+            const int u = u0 + (int)threadIdx.x + k * 192;
+            const int hv = (u >> 4) / UPV * 16 + (u & 15), part = (u >> 4) % UPV;
+            if (u < NU && hv < NVOX) {
+                const unsigned w4[4] = {buf[k].x, buf[k].y, buf[k].z, buf[k].w};
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    lds[(part * 8 + j) * ROW + hv] = (unsigned short)(w4[j / 2] >> (16 * (j & 1)));
+            }
+        }
+    }
+}
 
 template <int CIN, int NT>
 __global__ void __launch_bounds__(192) k_conv_dw(const unsigned short* __restrict__ X, Dims D, TileGrid G, int tiles,
@@ -254,9 +332,8 @@ __global__ void __launch_bounds__(192) k_conv_dw(const unsigned short* __restric
 {
     constexpr int COUT = NT * 16;
     constexpr int CT = CIN / 16;
-    constexpr int UPV = CIN / 8, UPO = COUT / 8;
-    __shared__ __align__(16) unsigned short xT[CIN * WH_PLANE];
-    __shared__ __align__(16) unsigned short dyT[COUT * WT_VOX];
+    __shared__ __align__(16) unsigned short xT[CIN * WH_ROW];
+    __shared__ __align__(16) unsigned short dyT[COUT * WT_ROW];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;  // wave = dz
     const int row = lane & 15, kg = lane >> 4;
     floatx4 acc[9][NT][CT];
@@ -282,31 +359,17 @@ __global__ void __launch_bounds__(192) k_conv_dw(const unsigned short* __restric
         const int tz = q % G.tz;
         const int b = q / G.tz;
         const int x0 = tx * WT_X, y0 = ty * WT_Y, z0 = tz * WT_Z;
-        for (int u = tid; u < WH_PLANE * UPV; u += 192) {
-            const int hv = u / UPV, part = u % UPV;
+        stage_transposed<CIN, WH_PLANE, WH_ROW>(X, xT, [&](int hv) -> int64_t {
             const int hx = hv % WH_X, hy = (hv / WH_X) % WH_Y, hz = hv / (WH_X * WH_Y);
             const int x = x0 + hx - 1, y = y0 + hy - 1, z = z0 + hz - 1;
-            uint4 val = {0u, 0u, 0u, 0u};
-            if (hx < WT_X + 2 && x >= 0 && x < D.w && y >= 0 && y < D.h && z >= 0 && z < D.d)
-                val = *reinterpret_cast<const uint4*>(X + ((((int64_t)b * D.d + z) * D.h + y) * D.w + x) * CIN +
-                                                      part * 8);
-            const unsigned w4[4] = {val.x, val.y, val.z, val.w};
-#pragma unroll
-            for (int j = 0; j < 8; ++j)
-                xT[(part * 8 + j) * WH_PLANE + hv] = (unsigned short)(w4[j / 2] >> (16 * (j & 1)));
-        }
-        for (int u = tid; u < WT_VOX * UPO; u += 192) {
-            const int vl = u / UPO, part = u % UPO;
+            if (hx >= WT_X + 2 || x < 0 || x >= D.w || y < 0 || y >= D.h || z < 0 || z >= D.d) return -1;
+            return (((int64_t)b * D.d + z) * D.h + y) * D.w + x;
+        });
+        stage_transposed<COUT, WT_VOX, WT_ROW>(dY, dyT, [&](int vl) -> int64_t {
             const int x = x0 + vl % WT_X, y = y0 + (vl / WT_X) % WT_Y, z = z0 + vl / (WT_X * WT_Y);
-            uint4 val = {0u, 0u, 0u, 0u};
-            if (x < D.w && y < D.h && z < D.d)
-                val = *reinterpret_cast<const uint4*>(dY + ((((int64_t)b * D.d + z) * D.h + y) * D.w + x) * COUT +
-                                                      part * 8);
-            const unsigned w4[4] = {val.x, val.y, val.z, val.w};
-#pragma unroll
-            for (int j = 0; j < 8; ++j)
-                dyT[(part * 8 + j) * WT_VOX + vl] = (unsigned short)(w4[j / 2] >> (16 * (j & 1)));
-        }
+            if (x >= D.w || y >= D.h || z >= D.d) return -1;
+            return (((int64_t)b * D.d + z) * D.h + y) * D.w + x;
+        });
         __syncthreads();
         for (int c = 0; c < WT_VOX / 32; ++c) {
             // this lane's 8 voxels: x-row r = z * WT_Y + y, x = xs .. xs + 7
@@ -315,7 +378,7 @@ __global__ void __launch_bounds__(192) k_conv_dw(const unsigned short* __restric
             bf16x8 A[NT];
 #pragma unroll
             for (int a = 0; a < NT; ++a)
-                A[a] = *reinterpret_cast<const bf16x8*>(dyT + (a * 16 + row) * WT_VOX + r * WT_X + xs);
+                A[a] = *reinterpret_cast<const bf16x8*>(dyT + (a * 16 + row) * WT_ROW + r * WT_X + xs);
             if (wave == 0)
 #pragma unroll
                 for (int a = 0; a < NT; ++a)
@@ -325,7 +388,7 @@ __global__ void __launch_bounds__(192) k_conv_dw(const unsigned short* __restric
 #pragma unroll
                 for (int ct = 0; ct < CT; ++ct) {
                     const unsigned short* p =
-                        xT + (ct * 16 + row) * WH_PLANE + ((zz + wave) * WH_Y + yy + dy) * WH_X + xs;
+                        xT + (ct * 16 + row) * WH_ROW + ((zz + wave) * WH_Y + yy + dy) * WH_X + xs;
                     const uint4 lo = *reinterpret_cast<const uint4*>(p);
                     const uint4 hi = *reinterpret_cast<const uint4*>(p + 8);
                     bf16x8 B[3];
