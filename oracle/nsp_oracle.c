@@ -802,6 +802,18 @@ static void build_parallel(orc_mesh* m, int min_objects, int threads)
     m->cap_entries = M.cap_entries;
 }
 
+int orc_box_simplex_overlap(const orc_mesh* m, int64_t f, const double* org, const double* size)
+{
+    if (!m->has_tree || f < 0 || f >= m->F) return -1;
+    return box_simplex_overlap(m, f, org, size);
+}
+
+void orc_tree_root(const orc_mesh* m, double* lo, double* size)
+{
+    memcpy(lo, m->root_lo, sizeof(double) * m->n);
+    memcpy(size, m->root_size, sizeof(double) * m->n);
+}
+
 int orc_tree_build(orc_mesh* m, int max_depth, int min_objects)
 {
     return orc_tree_build_threads(m, max_depth, min_objects, 1);

@@ -75,6 +75,12 @@ void orc_mesh_min_max(const orc_mesh* m, double* mn, double* mx);
  * workers do; the same boxes and leaf lists in either case). */
 int orc_tree_build(orc_mesh* m, int max_depth, int min_objects);
 int orc_tree_build_threads(orc_mesh* m, int max_depth, int min_objects, int threads);
+/* The conservative box/simplex overlap test the tree build uses (after
+ * orc_tree_build, which sets its guard distance): 1 overlap, 0 none, -1 bad
+ * arguments.  org/size: the box's low corner and edge lengths. */
+int orc_box_simplex_overlap(const orc_mesh* m, int64_t f, const double* org, const double* size);
+/* The tree's root box (low corner and edge lengths, with the guard). */
+void orc_tree_root(const orc_mesh* m, double* lo, double* size);
 /* node_count, leaf_count (non-empty + empty leaves), total leaf entries, max level */
 void orc_tree_info(const orc_mesh* m, int64_t* nodes, int64_t* leaves, int64_t* entries, int* depth);
 /* Sorted leaf lists keyed by the leaf's (level, integer cell coordinates): for

@@ -73,6 +73,9 @@ def lib():
         L.orc_mesh_min_max.restype = None
         L.orc_tree_build.argtypes = [P, C.c_int, C.c_int]
         L.orc_tree_build_threads.argtypes = [P, C.c_int, C.c_int, C.c_int]
+        L.orc_box_simplex_overlap.argtypes = [P, C.c_int64, dp, dp]
+        L.orc_tree_root.argtypes = [P, dp, dp]
+        L.orc_tree_root.restype = None
         L.orc_tree_info.argtypes = [P, i64p, i64p, i64p, C.POINTER(C.c_int)]
         L.orc_tree_info.restype = None
         L.orc_tree_dump.argtypes = [P, i64p, C.c_int64]
@@ -185,6 +188,17 @@ class Mesh:
         rc = lib().orc_tree_build_threads(self.h, max_depth, min_objects, threads)
         if rc:
             raise ValueError(lib().orc_last_error().decode())
+
+    def box_overlap(self, f, org, size):
+        """The tree build's conservative overlap test of simplex f with a box."""
+        o = np.ascontiguousarray(org, np.float64)
+        z = np.ascontiguousarray(size, np.float64)
+        return lib().orc_box_simplex_overlap(self.h, int(f), _p(o, C.c_double), _p(z, C.c_double))
+
+    def tree_root(self):
+        lo, size = np.zeros(self.n), np.zeros(self.n)
+        lib().orc_tree_root(self.h, _p(lo, C.c_double), _p(size, C.c_double))
+        return lo, size
 
     def tree_info(self):
         a, b, c, d = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int()
