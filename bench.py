@@ -316,8 +316,8 @@ def hull_bench(args):
 
 
 def batch_bench(args):
-    """C4: many scenes of uneven size and dimension, 4 spp, FP32: one batched
-    launch sequence vs one launch sequence per scene."""
+    """C4: many scenes of uneven size and dimension, 4 spp, FP32: the batched
+    launch (scenes on concurrent lanes) vs one nsp_paint call per scene."""
     import numpy as np
 
     from math_amd import configs, native
@@ -344,24 +344,35 @@ def batch_bench(args):
     per_rays = sum(s.statistics()["rays"] for s in per) - r0
     bat = [native.Scene(ctx, m, c) for m, c, _ in meshes]
     b = native.Batch(ctx, bat)
-    paths = 0
     if args.lanes > 0:
         b.set_lanes(args.lanes)
-    else:
-        paths = b.autotune(4, 5, seed=1)  # not timed; images are restored
-        log(f"autotuned batch: {b.lanes} lanes, {paths} paths per wavefront batch")
-    b.paint(4, 5, seed=1, use_graph=True, batch_paths=paths)  # warm-up
+    if args.batch_autotune:
+        b.autotune(4, 5, seed=1)  # not timed; images are restored
+        log(f"autotuned batch: {b.lanes} lanes")
+    b.paint(4, 5, seed=1)  # warm-up: workspaces, raster set-up
     for s in bat:
         s.set_image(*[np.zeros_like(a) for a in s.image()])
     t0 = time.perf_counter()
-    b.paint(4, 5, seed=1, use_graph=True, batch_paths=paths)
+    b.paint(4, 5, seed=1)
     bat_s = time.perf_counter() - t0
     bat_rays = b.statistics()["rays"]
-    out = {"metric": "C4 batched launch (one hipGraph, concurrent lanes), Mrays/s", "scenes": args.scenes,
-           "lanes": b.lanes, "batch_paths": paths or (1 << 23), "autotuned": args.lanes <= 0,
+    log(f"batched pass {bat_s:.2f} s on {b.lanes} lanes")
+    graph = None
+    if args.batch_graph:
+        # the same pass replayed from a captured hipGraph (captured and
+        # instantiated by the untimed first call)
+        t0 = time.perf_counter()
+        b.paint(4, 5, seed=1, use_graph=True)
+        log(f"graph capture and first replay {time.perf_counter() - t0:.1f} s")
+        t0 = time.perf_counter()
+        b.paint(4, 5, seed=1, use_graph=True)
+        gs = time.perf_counter() - t0
+        graph = {"seconds": round(gs, 3), "Mrays_s": round(b.statistics()["rays"] / gs / 1e6, 2)}
+    out = {"metric": "C4 batched launch (concurrent lanes), Mrays/s", "scenes": args.scenes,
+           "lanes": b.lanes, "autotuned": bool(args.batch_autotune),
            "per_scene": {"seconds": round(per_s, 3), "Mrays_s": round(per_rays / per_s / 1e6, 2)},
            "batched": {"seconds": round(bat_s, 3), "Mrays_s": round(bat_rays / bat_s / 1e6, 2)},
-           "speedup": round(per_s / bat_s, 3)}
+           "speedup": round(per_s / bat_s, 3), "graph_replay": graph}
     print(json.dumps(out), flush=True)
 
 
@@ -400,7 +411,11 @@ def main():
                     help="C4: batched persistent launch over many scenes vs one launch sequence per scene")
     ap.add_argument("--scenes", type=int, default=16,
                     help="C4: the first scenes of the 256 seeded scenes (all 256 meshes do not fit beside the workspaces)")
-    ap.add_argument("--lanes", type=int, default=0, help="batch lanes (0: autotune lanes and queue size)")
+    ap.add_argument("--lanes", type=int, default=0, help="batch lanes (0: the library's default)")
+    ap.add_argument("--batch-graph", action="store_true",
+                    help="also time the batch replayed from its captured hipGraph")
+    ap.add_argument("--batch-autotune", action="store_true",
+                    help="time lane counts first (two whole batched passes per candidate)")
     ap.add_argument("--denoiser-forward", action="store_true",
                     help="five 128^3 forwards of the shipped denoiser (the workload of its profiles)")
     args = ap.parse_args()

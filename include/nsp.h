@@ -384,19 +384,24 @@ int nsp_last_pass_stats(const nsp_scene* scene, nsp_stage_stats* out);
 int nsp_set_counters(nsp_scene* scene, int enable);
 
 /* Batched launch: one FP32 pass of every scene in the batch (scenes of any
- * size and dimension, all of the batch's context) in one launch.  The scenes
- * are spread over concurrent lanes (streams with their own workspaces) by
- * longest processing time first; each lane runs its scenes' wavefront
- * sequences back to back, and with use_graph the whole batch, every bounce of
- * every scene, is captured into one hipGraph and launched once.  No reference
- * counterpart (the reference paints one scene per paint() call); each scene's
- * image and statistics accumulate exactly as after nsp_paint with the same
- * parameters. */
+ * size and dimension, all of the batch's context).  The scenes go to K lanes
+ * (a stream, a side stream for the shadow stages and a workspace each; lane 0
+ * is the context's) by longest processing time first over the estimate
+ * paths x log2(2 + simplices); each lane paints its scenes costliest first
+ * with nsp_paint's wavefront batches (batch_paths, 0: the largest power of two
+ * up to 2^27 paths within the lane's workspace budget: 48 GiB for up to two
+ * lanes, 96 GiB shared among more), and the lanes run concurrently.  With
+ * use_graph the whole batch is captured into one hipGraph on the first call
+ * and replayed by later calls with the same parameters, lanes, workspaces and
+ * scene constants (the pass indices are written before each launch).  No
+ * reference counterpart (the reference paints one scene per paint() call);
+ * each scene's image and statistics accumulate exactly as after nsp_paint with
+ * the same parameters. */
 typedef struct nsp_batch nsp_batch;
 int nsp_batch_create(nsp_context* ctx, nsp_scene* const* scenes, int count, nsp_batch** out);
 void nsp_batch_destroy(nsp_batch* batch);
 int nsp_batch_paint(nsp_batch* batch, const nsp_paint_params* params, const volatile int32_t* stop);
-/* Number of concurrent lanes (1..64, default 4). */
+/* Number of lanes (1..8, default 2; at most one per scene is used). */
 int nsp_batch_set_lanes(nsp_batch* batch, int lanes);
 /* rays of the last batch pass (reference counting rules) and its wall time */
 int nsp_batch_statistics(const nsp_batch* batch, int64_t* ray_count, double* last_pass_duration);
@@ -407,9 +412,9 @@ int nsp_batch_statistics(const nsp_batch* batch, int64_t* ray_count, double* las
  * not depend on the choice; the scene's image, AOV sums and statistics are as
  * before the call (the timed passes write no AOVs). */
 int nsp_autotune(nsp_scene* scene, nsp_paint_params* params);
-/* Chooses the lane count of a batch (1, 2, 4 or 8; kept in the batch) and its
- * queue size (2^22 to 2^25 paths; written into params->batch_paths) by timing
- * whole batched passes; every scene's image and statistics are restored
+/* Chooses the lane count of a batch (1 to 4, kept in the batch) by timing
+ * whole batched passes (each after an untimed pass, which captures the graph
+ * with use_graph); every scene's image and statistics are restored
  * afterwards. */
 int nsp_batch_autotune(nsp_batch* batch, nsp_paint_params* params);
 /* The batch's current lane count. */

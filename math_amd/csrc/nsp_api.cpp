@@ -1146,31 +1146,102 @@ int nsp_set_counters(nsp_scene* scene, int enable)
 
 }  // extern "C"
 
-// A batch of scenes painted by one launch: every scene keeps its own pass
-// state (batch list, batch index, ray totals); the scenes are spread over
-// `lanes` concurrent streams, each with its own workspace, by longest
-// processing time first on pixels x samples; each lane runs its scenes'
-// wavefront sequences back to back, and the whole batch is captured into one
-// hipGraph (fork and join through events on the context's stream).
+// A batch of scenes painted together on K lanes.  A lane is a stream with a
+// side stream for the shadow stages and a workspace of its own (lane 0: the
+// context's).  The scenes go to the lanes by longest processing time first
+// over the cost estimate paths x log2(2 + simplices); each lane paints its
+// scenes one after another, costliest first, with nsp_paint's wavefront
+// batches (params->batch_paths, 0: the largest power of two of paths up to
+// 2^27 within the lane's workspace budget), and the lanes run concurrently,
+// so one scene's bounce tails and short stages overlap another scene's walks.
+// Every scene keeps its own pass state (batch list, batch index, ray totals).
+// With use_graph the whole batch is captured on the first call and its
+// instantiated graph replayed by later calls with the same parameters, lanes,
+// workspaces and scene constants; the pass indices are written before each
+// replay.
 struct BatchScene {
     DevBuf list, idx, totals;
     SceneK<float> kf;
     int64_t batch_pixels = 0, batches = 0;
+    int lane = 0;
 };
 
 struct nsp_batch {
     nsp_context* ctx;
     int device = 0;
-    int lanes = 4;
+    int lanes = 2;
     std::vector<nsp_scene*> scenes;
     std::vector<BatchScene> per;
+    // lanes 1..K-1: workspace, stream, side stream, join event
     std::vector<std::unique_ptr<RenderBuffers>> work;
-    std::vector<hipStream_t> streams;
+    std::vector<hipStream_t> streams, sides;
     std::vector<hipEvent_t> joins;
     hipEvent_t fork = nullptr;
+    // the cached graph and what it was captured for
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<int64_t> key;
+    DevBuf pass_ptrs, pass_vals;  // each scene's pass_dev, and the pass indices written before a launch
     int64_t rays = 0;
     double last_s = 0;
+    void drop_graph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        key.clear();
+    }
 };
+
+namespace {
+// the device addresses and sizes of a workspace's buffers: a captured graph
+// holds them, so it is replayed only while they are unchanged
+void fingerprint(const RenderBuffers& b, std::vector<int64_t>& out)
+{
+    auto add = [&](const DevBuf& d) {
+        out.push_back((int64_t)(uintptr_t)d.p);
+        out.push_back((int64_t)d.bytes);
+    };
+    for (const DevBuf& d : b.qa) add(d);
+    for (const DevBuf& d : b.qb) add(d);
+    for (const auto& q : b.sq)
+        for (const DevBuf& d : q) add(d);
+    for (const DevBuf* d : {&b.L, &b.Ls, &b.redo, &b.res, &b.keys, &b.redo_c, &b.res_c, &b.aov, &b.counters, &b.desc,
+                            &b.batch_list, &b.batch_idx, &b.totals, &b.trav_counters})
+        add(*d);
+    out.push_back(b.cap);
+    out.push_back(b.n);
+    out.push_back(b.lights);
+    out.push_back(b.depth_slots);
+}
+
+// what a captured segment of one scene holds: its buffers, raster work units
+// and launch constants (FNV-1a over the constants' bytes)
+void fingerprint(const nsp_scene& sc, const BatchScene& bs, std::vector<int64_t>& out)
+{
+    for (const DevBuf* d : {&bs.list, &bs.idx, &bs.totals, &sc.pass_dev, &sc.pix_sum, &sc.pix_cnt, &sc.raster.box,
+                            &sc.raster.unit, &sc.raster.ustart}) {
+        out.push_back((int64_t)(uintptr_t)d->p);
+        out.push_back((int64_t)d->bytes);
+    }
+    const RasterPrep& r = sc.raster;
+    for (int64_t v : {(int64_t)r.n, (int64_t)r.S, r.F, r.ppu, (int64_t)r.hmax, (int64_t)r.last, r.plane, r.units,
+                      (int64_t)r.small, bs.batch_pixels, bs.batches, (int64_t)bs.lane, (int64_t)(uintptr_t)&sc.mesh->m})
+        out.push_back(v);
+    uint64_t h = 1469598103934665603ull;
+    const unsigned char* c = reinterpret_cast<const unsigned char*>(&bs.kf);
+    for (size_t i = 0; i < sizeof(bs.kf); ++i) h = (h ^ c[i]) * 1099511628211ull;
+    out.push_back((int64_t)h);
+}
+
+// writes pass_dev of every scene: ptrs[i] <- vals[i]
+__global__ void k_set_passes(int* const* __restrict__ ptrs, const int* __restrict__ vals, int count)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) *ptrs[i] = vals[i];
+}
+}  // namespace
 
 extern "C" {
 
@@ -1197,7 +1268,9 @@ void nsp_batch_destroy(nsp_batch* b)
     if (!b) return;
     (void)hipSetDevice(b->device);
     (void)hipDeviceSynchronize();
+    b->drop_graph();
     for (hipStream_t st : b->streams) (void)hipStreamDestroy(st);
+    for (hipStream_t st : b->sides) (void)hipStreamDestroy(st);
     for (hipEvent_t e : b->joins) (void)hipEventDestroy(e);
     if (b->fork) (void)hipEventDestroy(b->fork);
     delete b;
@@ -1207,7 +1280,7 @@ int nsp_batch_get_lanes(const nsp_batch* b) { return b ? b->lanes : NSP_E_INVALI
 
 int nsp_batch_set_lanes(nsp_batch* b, int lanes)
 {
-    if (!b || lanes < 1 || lanes > 64) return NSP_E_INVALID;
+    if (!b || lanes < 1 || lanes > 8) return NSP_E_INVALID;
     b->lanes = lanes;
     return NSP_OK;
 }
@@ -1226,31 +1299,74 @@ int nsp_batch_paint(nsp_batch* b, const nsp_paint_params* params, const volatile
         hipStream_t s = ctx->stream;
         const int ns = (int)b->scenes.size();
         const int lanes = std::min(b->lanes, ns);
-        while ((int)b->streams.size() < lanes) {
-            hipStream_t st;
+        const bool raster_mode = p.primary == NSP_PRIMARY_AUTO;
+        // each lane's workspace budget: the single-scene budget for up to two
+        // lanes, twice that shared among more
+        const int64_t budget = lanes <= 2 ? DEFAULT_WORKSPACE : 2 * DEFAULT_WORKSPACE / lanes;
+        while ((int)b->streams.size() < lanes - 1) {
+            hipStream_t st, sd;
             NSP_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
             b->streams.push_back(st);
+            NSP_HIP(hipStreamCreateWithFlags(&sd, hipStreamNonBlocking));
+            b->sides.push_back(sd);
             hipEvent_t e;
             NSP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
             b->joins.push_back(e);
             b->work.push_back(std::make_unique<RenderBuffers>());
         }
-        const bool raster = p.primary == NSP_PRIMARY_AUTO;
-        const int64_t want = p.batch_paths > 0 ? p.batch_paths : ((int64_t)1 << 23);
-        // per scene: its launch constants, batch rule (that of nsp_paint) and pass state
+        auto lane_work = [&](int l) -> RenderBuffers& { return l == 0 ? ctx->work : *b->work[l - 1]; };
+        auto lane_stream = [&](int l) { return l == 0 ? s : b->streams[l - 1]; };
+        auto lane_side = [&](int l) { return l == 0 ? ctx->side : b->sides[l - 1]; };
+        // per scene: launch constants, cost, batch rule
         std::vector<double> cost(ns);
         for (int i = 0; i < ns; ++i) {
             nsp_scene* sc = b->scenes[i];
             BatchScene& bs = b->per[i];
             if (!sc->shapes.empty() && p.smooth_normal && sc->mesh->m.has_normals)
                 raise(NSP_E_UNSUPPORTED, "smooth-normal renders of meshes with vertex normals do not support shapes");
-            fill_scene<float>(*sc, p, rng_pass(sc), bs.kf);
+            fill_scene<float>(*sc, p, 0, bs.kf);
             if (!sc->pass_dev.p) sc->pass_dev.alloc(sizeof(int));
             bs.kf.pass_dev = sc->pass_dev.as<int>();
-            SceneK<double> kd;
-            fill_scene<double>(*sc, p, rng_pass(sc), kd);
             const int S = bs.kf.S;
+            cost[i] = (double)(sc->pixels * S) * std::log2(2.0 + (double)sc->mesh->m.F);
+            const bool raster = raster_mode && sc->proj.kind == NSP_PROJECTOR_PARALLEL;
+            int64_t want = p.batch_paths;
+            if (want <= 0) {
+                const int64_t per =
+                    render_bytes_per_path(sc->n, NSP_PRECISION_FP32, (int)sc->lights.size(), false, raster);
+                want = (int64_t)1 << 27;
+                while (want > ((int64_t)1 << 20) && want * per > budget) want >>= 1;
+            }
             bs.batch_pixels = std::min<int64_t>(std::max<int64_t>(1, want / S), sc->pixels);
+        }
+        // lanes by longest processing time first
+        std::vector<int> order(ns);
+        for (int i = 0; i < ns; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cost[a] > cost[c]; });
+        std::vector<double> load(lanes, 0.0);
+        std::vector<std::vector<int>> lane_scenes(lanes);
+        for (int i : order) {
+            const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            b->per[i].lane = l;
+            lane_scenes[l].push_back(i);
+            load[l] += cost[i];
+        }
+        // workspaces (they only grow), raster set-up, batch lists
+        for (int l = 0; l < lanes; ++l) {
+            int64_t cap = 1;
+            int nmax = 3, lmax = 1;
+            for (int i : lane_scenes[l]) {
+                cap = std::max(cap, b->per[i].batch_pixels * b->per[i].kf.S);
+                nmax = std::max(nmax, b->scenes[i]->n);
+                lmax = std::max(lmax, (int)b->scenes[i]->lights.size());
+            }
+            alloc_render_buffers(lane_work(l), nmax, NSP_PRECISION_FP32, cap, p.max_depth, lmax);
+            if (raster_mode) lane_work(l).keys.alloc(sizeof(unsigned long long) * cap);
+        }
+        for (int i = 0; i < ns; ++i) {
+            nsp_scene* sc = b->scenes[i];
+            BatchScene& bs = b->per[i];
+            const int S = bs.kf.S;
             std::vector<BatchDesc> descs;
             for (int64_t px = 0; px < sc->pixels; px += bs.batch_pixels)
                 descs.push_back(BatchDesc{px, std::min(bs.batch_pixels, sc->pixels - px) * S});
@@ -1260,96 +1376,115 @@ int nsp_batch_paint(nsp_batch* b, const nsp_paint_params* params, const volatile
             bs.totals.alloc(sizeof(unsigned long long) * 3);
             NSP_HIP(hipMemcpyAsync(bs.list.p, descs.data(), sizeof(BatchDesc) * descs.size(), hipMemcpyHostToDevice,
                                    s));
-            if (raster && sc->proj.kind == NSP_PROJECTOR_PARALLEL) raster_prepare(sc->raster, sc->mesh->m, kd, S, s);
-            cost[i] = (double)sc->pixels * S * std::log2(2.0 + (double)sc->mesh->m.F);
-        }
-        // lanes by longest processing time first
-        std::vector<int> order(ns);
-        for (int i = 0; i < ns; ++i) order[i] = i;
-        std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cost[a] > cost[c]; });
-        std::vector<std::vector<int>> lane_scenes(lanes);
-        std::vector<double> load(lanes, 0.0);
-        for (int i : order) {
-            int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-            lane_scenes[l].push_back(i);
-            load[l] += cost[i];
-        }
-        for (int l = 0; l < lanes; ++l) {
-            int64_t cap = 1;
-            int nmax = 3, lmax = 1;
-            for (int i : lane_scenes[l]) {
-                cap = std::max(cap, b->per[i].batch_pixels * b->per[i].kf.S);
-                nmax = std::max(nmax, b->scenes[i]->n);
-                lmax = std::max(lmax, (int)b->scenes[i]->lights.size());
+            if (raster_mode && sc->proj.kind == NSP_PROJECTOR_PARALLEL) {
+                SceneK<double> kd;
+                fill_scene<double>(*sc, p, 0, kd);
+                raster_prepare(sc->raster, sc->mesh->m, kd, S, s);
             }
-            alloc_render_buffers(*b->work[l], nmax, NSP_PRECISION_FP32, cap, p.max_depth, lmax);
-            if (raster) b->work[l]->keys.alloc(sizeof(unsigned long long) * cap);
         }
+        // the pass index of every scene, written before each launch
+        std::vector<int> vals(ns);
+        std::vector<int*> ptrs(ns);
+        for (int i = 0; i < ns; ++i) {
+            vals[i] = rng_pass(b->scenes[i]);
+            ptrs[i] = b->scenes[i]->pass_dev.as<int>();
+        }
+        b->pass_ptrs.alloc(sizeof(int*) * ns);
+        b->pass_vals.alloc(sizeof(int) * ns);
+        NSP_HIP(hipMemcpyAsync(b->pass_ptrs.p, ptrs.data(), sizeof(int*) * ns, hipMemcpyHostToDevice, s));
+        NSP_HIP(hipMemcpyAsync(b->pass_vals.p, vals.data(), sizeof(int) * ns, hipMemcpyHostToDevice, s));
         NSP_HIP(hipStreamSynchronize(s));
         if (stop && *stop) raise(NSP_E_STOPPED, "stopped");
-        auto t0 = std::chrono::steady_clock::now();
+        auto launch_for = [&](int i) {
+            nsp_scene* sc = b->scenes[i];
+            BatchScene& bs = b->per[i];
+            RenderLaunch rl;
+            rl.mesh = &sc->mesh->m;
+            rl.scene = &bs.kf;
+            rl.buf = &lane_work(bs.lane);
+            rl.pix_sum = sc->pix_sum.as<float>();
+            rl.pix_cnt = sc->pix_cnt.as<int32_t>();
+            rl.pix_aov = nullptr;
+            rl.pixels = sc->pixels;
+            rl.max_depth = p.max_depth;
+            rl.counters = false;
+            rl.marks = nullptr;
+            rl.raster = raster_mode && sc->proj.kind == NSP_PROJECTOR_PARALLEL ? &sc->raster : nullptr;
+            rl.batch_list = bs.list.as<BatchDesc>();
+            rl.batch_idx = bs.idx.as<long long>();
+            rl.totals = bs.totals.as<unsigned long long>();
+            rl.side = lane_side(bs.lane);
+            return rl;
+        };
+        // the lanes' launch sequences, submitted round robin one wavefront
+        // batch at a time so that no lane waits for another's submission
         auto enqueue = [&]() {
             NSP_HIP(hipEventRecord(b->fork, s));
-            for (int l = 0; l < lanes; ++l) {
-                hipStream_t ls = b->streams[l];
-                NSP_HIP(hipStreamWaitEvent(ls, b->fork, 0));
-                for (int i : lane_scenes[l]) {
-                    nsp_scene* sc = b->scenes[i];
-                    BatchScene& bs = b->per[i];
-                    pass_reset_state(bs.totals.as<unsigned long long>(), bs.idx.as<long long>(),
-                                     sc->pass_dev.as<int>(), rng_pass(sc), ls);
-                    RenderLaunch rl;
-                    rl.mesh = &sc->mesh->m;
-                    rl.scene = &bs.kf;
-                    rl.buf = b->work[l].get();
-                    rl.pix_sum = sc->pix_sum.as<float>();
-                    rl.pix_cnt = sc->pix_cnt.as<int32_t>();
-                    rl.pix_aov = nullptr;
-                    rl.pixels = sc->pixels;
-                    rl.max_depth = p.max_depth;
-                    rl.counters = false;
-                    rl.marks = nullptr;
-                    rl.raster = raster && sc->proj.kind == NSP_PROJECTOR_PARALLEL ? &sc->raster : nullptr;
-                    rl.batch_list = bs.list.as<BatchDesc>();
-                    rl.batch_idx = bs.idx.as<long long>();
-                    rl.totals = bs.totals.as<unsigned long long>();
-                    for (int64_t k = 0; k < bs.batches; ++k) render_batch(rl, ls);
+            for (int l = 1; l < lanes; ++l) NSP_HIP(hipStreamWaitEvent(lane_stream(l), b->fork, 0));
+            std::vector<size_t> next(lanes, 0);
+            std::vector<int64_t> left(lanes, 0);
+            std::vector<RenderLaunch> cur(lanes);
+            for (bool more = true; more;) {
+                more = false;
+                for (int l = 0; l < lanes; ++l) {
+                    if (left[l] == 0) {
+                        if (next[l] == lane_scenes[l].size()) continue;
+                        const int i = lane_scenes[l][next[l]++];
+                        pass_reset_state(b->per[i].totals.as<unsigned long long>(), b->per[i].idx.as<long long>(),
+                                         nullptr, 0, lane_stream(l));
+                        cur[l] = launch_for(i);
+                        left[l] = b->per[i].batches;
+                    }
+                    render_batch(cur[l], lane_stream(l));
+                    --left[l];
+                    more = true;
                 }
-                NSP_HIP(hipEventRecord(b->joins[l], ls));
-                NSP_HIP(hipStreamWaitEvent(s, b->joins[l], 0));
+            }
+            for (int l = 1; l < lanes; ++l) {
+                NSP_HIP(hipEventRecord(b->joins[l - 1], lane_stream(l)));
+                NSP_HIP(hipStreamWaitEvent(s, b->joins[l - 1], 0));
             }
         };
         if (p.use_graph) {
-            hipGraph_t graph = nullptr;
-            hipGraphExec_t exec = nullptr;
-            NSP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            try {
-                enqueue();
-            } catch (...) {
-                hipGraph_t g2 = nullptr;
-                (void)hipStreamEndCapture(s, &g2);
-                if (g2) (void)hipGraphDestroy(g2);
-                throw;
+            std::vector<int64_t> key = {p.samples_per_pixel, p.max_depth, p.smooth_normal, (int64_t)p.seed,
+                                        p.batch_paths, p.primary, lanes};
+            for (int l = 0; l < lanes; ++l) fingerprint(lane_work(l), key);
+            for (int i = 0; i < ns; ++i) fingerprint(*b->scenes[i], b->per[i], key);
+            if (!b->exec || key != b->key) {
+                b->drop_graph();
+                NSP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                try {
+                    enqueue();
+                } catch (...) {
+                    hipGraph_t g2 = nullptr;
+                    (void)hipStreamEndCapture(s, &g2);
+                    if (g2) (void)hipGraphDestroy(g2);
+                    throw;
+                }
+                NSP_HIP(hipStreamEndCapture(s, &b->graph));
+                NSP_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+                b->key = key;
             }
-            NSP_HIP(hipStreamEndCapture(s, &graph));
-            hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-            if (ie == hipSuccess) ie = hipGraphLaunch(exec, s);
-            if (ie == hipSuccess) ie = hipStreamSynchronize(s);
-            if (exec) (void)hipGraphExecDestroy(exec);
-            (void)hipGraphDestroy(graph);
-            NSP_HIP(ie);
-        } else {
-            enqueue();
-            NSP_HIP(hipStreamSynchronize(s));
         }
+        auto t0 = std::chrono::steady_clock::now();
+        hipLaunchKernelGGL(k_set_passes, dim3((ns + 255) / 256), dim3(256), 0, s, b->pass_ptrs.as<int*>(),
+                           b->pass_vals.as<int>(), ns);
+        NSP_HIP(hipGetLastError());
+        if (p.use_graph)
+            NSP_HIP(hipGraphLaunch(b->exec, s));
+        else
+            enqueue();
+        NSP_HIP(hipStreamSynchronize(s));
         b->last_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         b->rays = 0;
+        std::vector<unsigned long long> tot((size_t)3 * ns);
+        for (int i = 0; i < ns; ++i)
+            NSP_HIP(hipMemcpyAsync(tot.data() + 3 * i, b->per[i].totals.p, 3 * sizeof(unsigned long long),
+                                   hipMemcpyDeviceToHost, s));
+        NSP_HIP(hipStreamSynchronize(s));
         for (int i = 0; i < ns; ++i) {
             nsp_scene* sc = b->scenes[i];
-            unsigned long long tot[3] = {0, 0, 0};
-            NSP_HIP(hipMemcpyAsync(tot, b->per[i].totals.p, sizeof tot, hipMemcpyDeviceToHost, s));
-            NSP_HIP(hipStreamSynchronize(s));
-            const int64_t r = (int64_t)(tot[0] + tot[1] + tot[2]);
+            const int64_t r = (int64_t)(tot[3 * i] + tot[3 * i + 1] + tot[3 * i + 2]);
             b->rays += r;
             sc->pass_count += 1;
             sc->pixel_count += sc->pixels;
@@ -1440,11 +1575,10 @@ int nsp_autotune(nsp_scene* scene, nsp_paint_params* params)
     return NSP_OK;
 }
 
-// Batched launch: lanes first (1, 2, 4, 8 at 2^23 paths), then the queue size
-// (2^22, 2^24, 2^25 paths) at the best lane count, each timed on one whole
-// batched pass; candidates whose lane workspaces exceed the budget are not
-// tried and one that cannot allocate is skipped; every scene's image and
-// statistics are restored.
+// Batched launch: the lane count (1, 2, 3, 4), each timed on one whole batched
+// pass after an untimed pass (which captures the graph with use_graph);
+// candidates whose workspaces cannot be allocated are skipped; every scene's
+// image and statistics are restored.
 int nsp_batch_autotune(nsp_batch* b, nsp_paint_params* params)
 {
     if (!b || !params) return NSP_E_INVALID;
@@ -1472,55 +1606,29 @@ int nsp_batch_autotune(nsp_batch* b, nsp_paint_params* params)
     nsp_paint_params p = *params;
     p.passes = 1;
     int rc = NSP_OK;
-    auto timed = [&](int lanes, int64_t paths) -> double {
-        b->lanes = lanes;
-        p.batch_paths = paths;
-        auto t0 = std::chrono::steady_clock::now();
-        rc = nsp_batch_paint(b, &p, nullptr);
-        return rc == NSP_OK ? std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() : 1e300;
-    };
-    // workspace bytes per path of a lane: the largest dimension and light count of
-    // the batch's scenes (a lane sizes its workspace for the scenes it holds)
-    int nmax = 3, lmax = 1;
-    for (nsp_scene* sc : b->scenes) {
-        nmax = std::max(nmax, sc->n);
-        lmax = std::max(lmax, (int)sc->lights.size());
-    }
-    const int64_t per_path = render_bytes_per_path(nmax, NSP_PRECISION_FP32, lmax, false, true);
-    // the lanes' workspaces together stay within the single-scene budget
-    auto fits = [&](int lanes, int64_t paths) { return (int64_t)lanes * paths * per_path <= DEFAULT_WORKSPACE; };
+    const int lanes0 = b->lanes;
     int best_lanes = 0;
-    int64_t best_paths = (int64_t)1 << 23;
     double best_s = 1e300;
-    // one candidate: timed, or skipped when its workspace cannot be allocated
-    // (the best so far stands); any other failure ends the search
-    auto try_one = [&](int lanes, int64_t paths) {
-        double t = timed(lanes, paths);
+    for (int lanes : {1, 2, 3, 4}) {
+        if (rc != NSP_OK) break;
+        if (lanes > ns) continue;
+        b->lanes = lanes;
+        rc = nsp_batch_paint(b, &p, nullptr);
+        double t = 1e300;
+        if (rc == NSP_OK) {
+            auto t0 = std::chrono::steady_clock::now();
+            rc = nsp_batch_paint(b, &p, nullptr);
+            t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
         if (rc == NSP_E_NOMEM) {
             (void)hipGetLastError();
             rc = NSP_OK;
-            return;
+            continue;
         }
         if (rc == NSP_OK && t < best_s) {
             best_s = t;
             best_lanes = lanes;
-            best_paths = paths;
         }
-    };
-    // lanes first (1, 2, 4, 8) at 2^23 paths, then the queue size (2^22, 2^24,
-    // 2^25 paths) at the best lane count
-    for (int lanes : {1, 2, 4, 8}) {
-        if (rc != NSP_OK) break;
-        if (lanes > ns && lanes > 2) continue;
-        if (!fits(lanes, (int64_t)1 << 23)) continue;
-        try_one(lanes, (int64_t)1 << 23);
-    }
-    const int lanes_found = best_lanes ? best_lanes : 1;
-    for (int lg : {22, 24, 25}) {
-        if (rc != NSP_OK) break;
-        const int64_t paths = (int64_t)1 << lg;
-        if (!fits(lanes_found, paths)) continue;
-        try_one(lanes_found, paths);
     }
     if (rc == NSP_OK && !best_lanes) {
         rc = NSP_E_NOMEM;
@@ -1538,10 +1646,11 @@ int nsp_batch_autotune(nsp_batch* b, nsp_paint_params* params)
         sc->sample_count = sv.samples;
         sc->last_pass_s = sv.last;
     }
-    if (rc != NSP_OK) return rc;
-    if (rc2 != NSP_OK) return rc2;
+    if (rc != NSP_OK || rc2 != NSP_OK) {
+        b->lanes = lanes0;
+        return rc != NSP_OK ? rc : rc2;
+    }
     b->lanes = best_lanes;
-    params->batch_paths = best_paths;
     return NSP_OK;
 }
 

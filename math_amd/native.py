@@ -828,11 +828,11 @@ class Batch:
     def lanes(self):
         return _batch_lib().nsp_batch_get_lanes(self.h)
 
-    def autotune(self, spp, max_depth=5, seed=1, use_graph=True):
-        """Lane count (kept in the batch) and queue size (returned) chosen by timing; images are restored."""
+    def autotune(self, spp, max_depth=5, seed=1, use_graph=False):
+        """Lane count chosen by timing (kept in the batch and returned); images are restored."""
         p = PaintParams(spp, max_depth, 0, PRECISION_FP32, 1, seed, 0, int(use_graph), 0, 0)
         self.ctx.check(_batch_lib().nsp_batch_autotune(self.h, C.byref(p)))
-        return p.batch_paths
+        return self.lanes
 
 
 def autotune(scene, spp, max_depth=5, seed=1):
