@@ -7,10 +7,11 @@
 // leaf's nearest simplex only inside that leaf box.  Here the tree is a
 // hierarchy of regular grids, so a point's cell at every level follows from
 // its integer coordinates at the finest level Q = floor((p - lo) / cell_min).
-// Each lane keeps the node ids of its current root-to-leaf path in an LDS
-// stack (one int per level); after leaving a leaf it pops only the levels
-// whose cell differs (bit length of the XOR of old and new coordinates) and
-// descends from there.  Leaves are visited in ray order and the walk stops
+// Each lane keeps the node records (int2: first child and stride, or entry
+// range) of its current root-to-leaf path in an LDS stack, one per level;
+// after leaving a leaf it pops only the levels whose cell differs (bit length
+// of the XOR of old and new coordinates) and descends from the deepest kept
+// record, so a restart reads its first node from LDS.  Leaves are visited in ray order and the walk stops
 // once the next leaf starts beyond the nearest hit, so the result is the
 // nearest hit over all simplices with the reference's tie rule (smaller
 // index first, src/painter/space/ray_intersection.h:141-150).
