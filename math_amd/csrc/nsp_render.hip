@@ -490,6 +490,8 @@ __global__ void NSP_POOL_BOUNDS k_closest_pool(TreeArgs A, RayQueue<T> q, int64_
 {
     __shared__ int2 stack[MAX_LEVELS * TRACE_BLOCK];
     int2* st = stack + threadIdx.x;
+    constexpr bool COOP = LAY == 1 && sizeof(T) == 4;
+    __shared__ CoopWave coop[COOP ? TRACE_BLOCK / 64 : 1];
     RayPool<refill_lanes<N>()> pool(cnt + 8, (int64_t)cnt[0]);
     RayWalk<N, T, false, LAY, true> w;
     int64_t ray = -1;
@@ -510,7 +512,19 @@ __global__ void NSP_POOL_BOUNDS k_closest_pool(TreeArgs A, RayQueue<T> q, int64_
                 }
             }
         }
-        if (ray >= 0 && w.step(A, st, nullptr, nullptr)) {
+        bool ended;
+        if constexpr (COOP) {
+            // locate every walk's next leaf, test all the wave's candidates
+            // together, then advance
+            int e0 = 0, n = 0;
+            ended = ray >= 0 && w.locate(A, st, &e0, &n);
+            const bool live = ray >= 0 && !ended;
+            coop_test<N, false>(A, coop[threadIdx.x >> 6], w, live, e0, n);
+            if (live) ended = w.deferred || w.advance(A);
+        } else {
+            ended = ray >= 0 && w.step(A, st, nullptr, nullptr);
+        }
+        if (ended) {
             if (w.deferred) {
                 redo[atomicAdd(redo_count, 1ull)] = (int32_t)ray;
             } else {
@@ -585,6 +599,8 @@ __global__ void NSP_POOL_BOUNDS k_shadow(TreeArgs A, ShadowQueue<T> sq, int64_t 
 {
     __shared__ int2 stack[MAX_LEVELS * TRACE_BLOCK];
     int2* st = stack + threadIdx.x;
+    constexpr bool COOP = LAY == 1 && sizeof(T) == 4;
+    __shared__ CoopWave coop[COOP ? TRACE_BLOCK / 64 : 1];
     RayPool<refill_lanes<N>()> pool(cnt + 15, (int64_t)cnt[0]);
     RayWalk<N, T, true, LAY, true> w;
     int64_t ray = -1;
@@ -603,7 +619,18 @@ __global__ void NSP_POOL_BOUNDS k_shadow(TreeArgs A, ShadowQueue<T> sq, int64_t 
                     shadow_unoccluded<N, T>(sq, scap, i, L, cap, direct);
             }
         }
-        if (ray >= 0 && w.step(A, st, nullptr, nullptr)) {
+        bool ended;
+        if constexpr (COOP) {
+            int e0 = 0, n = 0;
+            ended = ray >= 0 && w.locate(A, st, &e0, &n);
+            const bool live = ray >= 0 && !ended;
+            coop_test<N, true>(A, coop[threadIdx.x >> 6], w, live, e0, n);
+            // a hit ends an any-hit walk (coop_test then leaves deferred unset)
+            if (live) ended = w.best >= 0 || w.deferred || w.advance(A);
+        } else {
+            ended = ray >= 0 && w.step(A, st, nullptr, nullptr);
+        }
+        if (ended) {
             if (w.deferred) {
                 redo[atomicAdd(redo_count, 1ull)] = (int32_t)ray;
             } else if (w.best < 0) {

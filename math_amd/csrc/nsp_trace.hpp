@@ -11,10 +11,11 @@
 // range) of its current root-to-leaf path in an LDS stack, one per level;
 // after leaving a leaf it pops only the levels whose cell differs (bit length
 // of the XOR of old and new coordinates) and descends from the deepest kept
-// record, so a restart reads its first node from LDS.  Leaves are visited in ray order and the walk stops
-// once the next leaf starts beyond the nearest hit, so the result is the
-// nearest hit over all simplices with the reference's tie rule (smaller
-// index first, src/painter/space/ray_intersection.h:141-150).
+// record, so a restart reads its first node from LDS.  Leaves are visited in
+// ray order and the walk stops once the next leaf starts beyond the nearest
+// hit, so the result is the nearest hit over all simplices with the
+// reference's tie rule (smaller index first,
+// src/painter/space/ray_intersection.h:141-150).
 #pragma once
 
 #include "nsp_device.hpp"
@@ -132,12 +133,17 @@ __device__ __forceinline__ int fp32_test(const TreeArgs& A, const Comp& comp, co
     if (s == 0.0f) return CAND_MISS;
     float dd = comp(N);
     float no = dot<N, float>(nrm, o);
-    float t = (dd - no) / s;
+    // one hardware reciprocal (v_rcp_f32, within 1 ulp) instead of three
+    // IEEE divisions of about ten instructions each
+    const float r = __builtin_amdgcn_rcpf(s);
+    const float ar = fabsf(r);
+    float t = (dd - no) * r;
     float as = fabsf(s);
-    float band = RETEST_BAND * (1.0f + 1.0f / as);
-    // rounding bound of t; outside it the sign of t and the comparison
-    // with the limit are decided in FP32
-    float terr = 4.0e-7f * (fabsf(dd) + fabsf(no)) / as;
+    float band = RETEST_BAND * (1.0f + ar);
+    // rounding bound of t (the numerator's cancellation, the reciprocal's
+    // ulp and the product's rounding); outside it the sign of t and the
+    // comparison with the limit are decided in FP32
+    float terr = 6.0e-7f * (fabsf(dd) + fabsf(no)) * ar;
     if (t < -terr) return CAND_MISS;
     if (t > tlimit + terr) return CAND_MISS;
     float p[N];
@@ -291,6 +297,7 @@ template <int N, typename T, bool ANY, int LAY = -1, bool DEFER = false>
 struct RayWalk {
     T o[N], d[N], inv[N];
     T t_cur, t_end, adv, tmax;
+    T t_next;  // exit of the located leaf box
     double best_t;
     int best, level, last_leaf;
     int q[N];
@@ -324,13 +331,14 @@ struct RayWalk {
         return true;
     }
 
-    // One leaf of the walk; true when the walk has ended.
+    // The first part of a step: locates the leaf that holds the next point of
+    // the ray and its box exit (t_next).  True when the walk has ended;
+    // otherwise *cnt candidates from entry *e0 are to be tested (0: an empty
+    // leaf, or the leaf just tested).
     // stack: this lane's LDS stack, stack[l * TRACE_BLOCK] = the node record
     // of the walk's node at level l, so that a restart after a pop reads the
     // node from LDS instead of global memory.
-    // o64/d64: the ray in FP64 for the near-boundary re-test, or null to use
-    // the FP32 ray converted exactly.
-    __device__ __forceinline__ bool step(const TreeArgs& A, int2* stack, const double* o64, const double* d64)
+    __device__ __forceinline__ bool locate(const TreeArgs& A, int2* stack, int* e0, int* cnt)
     {
         const T* rlo = TreeBox<T>::lo(A);
         const T* rsz = TreeBox<T>::size(A);
@@ -387,9 +395,40 @@ struct RayWalk {
         }
         T ta, tb;
         if (!slab<N, T>(blo, bsz, o, inv, d, &ta, &tb)) tb = tp;  // rounding: point located outside its own box
-        // a leaf with entries is known by its first entry (leaves own disjoint entry ranges)
+        t_next = tb;
+        *cnt = 0;
         if (nd.x != last_leaf && nd.y > 0) {
             ++lvis;
+            *e0 = nd.x;
+            *cnt = nd.y;
+        }
+        if (nd.y > 0) last_leaf = nd.x;
+        return false;
+    }
+
+    // The last part of a step: moves t_cur to the located leaf's exit.  True
+    // when the walk has ended.
+    __device__ __forceinline__ bool advance(const TreeArgs& A)
+    {
+        const T adv0 = TreeBox<T>::adv(A);
+        if (t_next > t_cur) {
+            t_cur = t_next;
+            adv = adv0;
+        } else {
+            adv = adv * 2;  // no progress: the located cell ended before t_cur (rounding)
+            if (t_cur + adv > t_end) return true;
+        }
+        return false;
+    }
+
+    // One leaf of the walk; true when the walk has ended.
+    // o64/d64: the ray in FP64 for the near-boundary re-test, or null to use
+    // the FP32 ray converted exactly.
+    __device__ __forceinline__ bool step(const TreeArgs& A, int2* stack, const double* o64, const double* d64)
+    {
+        int e_begin = 0, e_count = 0;
+        if (locate(A, stack, &e_begin, &e_count)) return true;
+        if (e_count > 0) {
             // outcome of one candidate test; true ends the walk
             auto take = [&](int c, double th, int si, bool nr, int sid) -> bool {
                 near = near || nr;
@@ -424,12 +463,12 @@ struct RayWalk {
                 }
                 return false;
             };
-            const int64_t e1 = (int64_t)nd.x + nd.y;
+            const int64_t e1 = (int64_t)e_begin + e_count;
             if constexpr (LAY == 1 && sizeof(T) == 4) {
                 // two candidates at a time: both entry ids, then both records, are
                 // loaded before either is tested (two independent load chains)
                 constexpr int RS = rec32_stride(N);
-                for (int64_t e = nd.x; e < e1; e += 2) {
+                for (int64_t e = e_begin; e < e1; e += 2) {
                     const bool two = e + 1 < e1;
                     const int sa = A.entries[e];
                     const int sb = two ? A.entries[e + 1] : sa;
@@ -452,7 +491,7 @@ struct RayWalk {
                     }
                 }
             } else {
-                for (int64_t e = nd.x; e < e1; ++e) {
+                for (int64_t e = e_begin; e < e1; ++e) {
                     ++ntest;
                     double th;
                     int si;
@@ -463,15 +502,7 @@ struct RayWalk {
                 }
             }
         }
-        if (nd.y > 0) last_leaf = nd.x;
-        if (tb > t_cur) {
-            t_cur = tb;
-            adv = adv0;
-        } else {
-            adv = adv * 2;  // no progress: the located cell ended before t_cur (rounding)
-            if (t_cur + adv > t_end) return true;
-        }
-        return false;
+        return advance(A);
     }
 
     __device__ __forceinline__ HitResult result() const
@@ -494,6 +525,149 @@ struct RayWalk {
         }
     }
 };
+
+// LDS of one wave for coop_test.
+struct CoopWave {
+    int incl[64];                 // inclusive count of candidates up to each lane
+    unsigned long long key[64];   // per lane: nearest hit (t bits << 32 | simplex), or any hit
+    int npend[64], ndefer[64];    // per lane: pending candidates, deferred decisions
+    int pend[64 * PENDING];
+};
+
+// Sort key of a hit: t > 0, so its float bits order as unsigned integers; ties
+// go to the smaller simplex index (the walk's tie rule).
+__device__ __forceinline__ unsigned long long hit_key(float t, int sid)
+{
+    return ((unsigned long long)__float_as_uint(t) << 32) | (unsigned)sid;
+}
+constexpr unsigned long long NO_HIT_KEY = (0x7f800000ull << 32) | 0xffffffffull;
+
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Wave-cooperative FP32 test of the leaf candidates of every lane whose walk
+// located a leaf (active, cnt candidates from entry e0; the FP32 AoS records,
+// deferred FP64 decisions).  Called by all 64 lanes together.  The lanes'
+// candidate lists are concatenated (a prefix sum over the wave) and dealt out
+// 64 at a time: lane j of a round tests candidate base + j for its owner, the
+// lane whose list holds it (binary search over the inclusive counts), with
+// the owner's ray and limit read across lanes.  A walk with one candidate and
+// a walk with forty then cost the wave the same rounds, where testing each
+// lane's own list keeps the wave until its longest list is done.
+//   nearest hit (ANY false): the owner's key is lowered with an LDS atomic
+//     min from its hit so far, which keeps the nearest hit and, at equal t,
+//     the smaller index.  Every candidate is tested against the limit the
+//     leaf started with; a hit can only lower it, so the minimum is the one
+//     the sequential walk finds.
+//   any hit (ANY true): any hit before tmax ends the owner's walk.
+//   deferred candidates join the owner's pending list (no duplicates); more
+//     than PENDING set deferred, for a re-walk.
+template <int N, bool ANY, class W>
+__device__ __forceinline__ void coop_test(const TreeArgs& A, CoopWave& L, W& w, bool active, int e0, int cnt)
+{
+    constexpr int RS = rec32_stride(N);
+    const int lane = threadIdx.x & 63;
+    const int c = active ? cnt : 0;
+    int incl = c;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    const int total = __shfl(incl, 63);
+    if (total == 0) return;
+    const float tl = ANY ? (float)w.tmax : (float)w.best_t;
+    const int np0 = w.npend;
+    L.incl[lane] = incl;
+    L.key[lane] = ANY || w.best < 0 ? NO_HIT_KEY : hit_key((float)w.best_t, w.best);
+    L.npend[lane] = np0;
+    L.ndefer[lane] = 0;
+#pragma unroll
+    for (int k = 0; k < PENDING; ++k) L.pend[lane * PENDING + k] = w.pend[k];
+    wave_lds_sync();
+    for (int base = 0; base < total; base += 64) {
+        const int j = base + lane;
+        const int jj = j < total ? j : total - 1;
+        // owner: the first lane whose inclusive count exceeds jj
+        int lo = 0, hi = 63;
+#pragma unroll
+        for (int st = 0; st < 6; ++st) {
+            const int mid = (lo + hi) >> 1;
+            if (L.incl[mid] > jj)
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        const int owner = lo;
+        float oo[N], od[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            oo[k] = __shfl((float)w.o[k], owner);
+            od[k] = __shfl((float)w.d[k], owner);
+        }
+        const float tlo = __shfl(tl, owner);
+        const int e0o = __shfl(e0, owner);
+        const int exo = __shfl(incl - c, owner);
+        const int npo = __shfl(np0, owner);
+        if (j < total) {
+            const int sid = A.entries[e0o + (j - exo)];
+            float rec[RS];
+            load_rec32<N>(A, sid, rec);
+            auto comp = [&](int cc) -> float { return rec[cc]; };
+            auto get_sid = [&]() -> int { return sid; };
+            double th;
+            int si;
+            bool nr = false;
+            unsigned dummy = 0;
+            const int r = fp32_test<N, true>(A, comp, get_sid, oo, od, nullptr, nullptr, tlo, -1, &th, &si, &nr,
+                                             &dummy);
+            if (r == CAND_HIT) {
+                atomicMin(&L.key[owner], ANY ? (unsigned long long)(unsigned)si : hit_key((float)th, si));
+            } else if (r == CAND_DEFER) {
+                atomicAdd(&L.ndefer[owner], 1);
+                bool known = false;
+#pragma unroll
+                for (int k = 0; k < PENDING; ++k) known = known || (k < npo && L.pend[owner * PENDING + k] == sid);
+                if (!known) {
+                    const int slot = atomicAdd(&L.npend[owner], 1);
+                    if (slot < PENDING) L.pend[owner * PENDING + slot] = sid;
+                }
+            }
+        }
+    }
+    wave_lds_sync();
+    if (c > 0) {
+        w.ntest += c;
+        const unsigned long long k = L.key[lane];
+        if (ANY && k != NO_HIT_KEY) {
+            // the walk ends occluded; its pending list no longer matters
+            w.best = (int)(unsigned)k;
+            w.best_t = (double)w.tmax;
+            return;
+        }
+        if (!ANY && k != NO_HIT_KEY) {
+            w.best = (int)(unsigned)(k & 0xffffffffull);
+            w.best_t = (double)__uint_as_float((unsigned)(k >> 32));
+        }
+        const int ndf = L.ndefer[lane];
+        if (ndf > 0) {
+            w.nret += ndf;
+            w.near = true;
+        }
+        const int np = L.npend[lane];
+        if (np > PENDING) {
+            w.deferred = true;
+        } else {
+#pragma unroll
+            for (int k = 0; k < PENDING; ++k)
+                if (k < np) w.pend[k] = L.pend[lane * PENDING + k];
+            w.npend = np;
+        }
+    }
+}
 
 // FP64 decision of the pending candidates pend[0..PENDING) (-1: none) of an
 // FP32 walk whose nearest hit is (best, best_t): the inline re-test's
