@@ -394,7 +394,7 @@ constexpr int refill_lanes()
 #ifdef NSP_REFILL_LANES
     return NSP_REFILL_LANES;
 #else
-    return N >= 5 ? 32 : 16;
+    return 32;
 #endif
 }
 constexpr int XCDS = 8;
