@@ -937,7 +937,7 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
             ctx->work.keys.alloc(sizeof(unsigned long long) * cap);
         }
         if (p.aov) {
-            ctx->work.aov.alloc(sizeof(float) * aov_channels(n) * cap);
+            ctx->work.aov.alloc(sizeof(float) * aov_channels(n) * ctx->work.cap);  // per-path planes of stride cap
             if (!scene->pix_aov.p) {
                 scene->pix_aov.alloc(sizeof(float) * pix_aov_channels(n) * scene->pixels);
                 NSP_HIP(hipMemsetAsync(scene->pix_aov.p, 0, sizeof(float) * pix_aov_channels(n) * scene->pixels, s));

@@ -79,6 +79,7 @@ TreeArgs make_tree_args(const MeshDev& m)
     a.layout = m.layout;
     a.rec64 = m.rec64.as<double>();
     a.lmax = t.levels - 1;
+    if (t.levels > stack_levels(m.n)) raise(NSP_E_INVALID, "tree deeper than the walk's node stack");
     double ext = 0;
     for (int i = 0; i < m.n; ++i) {
         a.lo[i] = (float)t.root_lo[i];
@@ -103,7 +104,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_first_hit(TreeArgs A, int64_t c
                                                            const float* __restrict__ dir, int32_t* __restrict__ idx,
                                                            double* __restrict__ tout, uint8_t* __restrict__ near)
 {
-    __shared__ int2 stack[MAX_LEVELS * TRACE_BLOCK];
+    __shared__ int2 stack[stack_levels(N) * TRACE_BLOCK];
     int2* st = stack + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x; i < count; i += stride) {
@@ -145,17 +146,33 @@ void launch_first_hit(const MeshDev& m, const float* org, const float* dir, int6
 // Render stages
 
 // Selects the current batch from the pass's batch list (device memory, so a
-// captured graph replays unchanged for every batch) and clears its counters.
-__global__ void k_batch_reset(unsigned long long* cnt, int ncnt, const BatchDesc* __restrict__ list,
-                              const long long* __restrict__ batch_idx, BatchDesc* __restrict__ cur)
+// captured graph replays unchanged for every batch), clears its counters and
+// writes the queue segment layout: the primary rays lie contiguous in QSEG
+// segments of ceil(paths / QSEG); the queues of later depths have segments of
+// segcap rays, the shadow queues of ssegcap.
+__global__ void k_batch_reset(unsigned long long* cnt, int depth_slots, const BatchDesc* __restrict__ list,
+                              const long long* __restrict__ batch_idx, BatchDesc* __restrict__ cur, int64_t segcap,
+                              int64_t ssegcap)
 {
-    int i = threadIdx.x;
+    const int i = threadIdx.x;
+    const int ncnt = CNT_STRIDE * depth_slots;
     for (int k = i; k < ncnt; k += blockDim.x) cnt[k] = 0;
     __syncthreads();
+    const BatchDesc d = list[*batch_idx];
+    if (i < depth_slots) {
+        cnt[(int64_t)i * CNT_STRIDE + CNT_SEGCAP] = (unsigned long long)segcap;
+        cnt[(int64_t)i * CNT_STRIDE + CNT_SSEGCAP] = (unsigned long long)ssegcap;
+    }
+    __syncthreads();
     if (i == 0) {
-        BatchDesc d = list[*batch_idx];
         *cur = d;
-        cnt[0] = (unsigned long long)d.paths;
+        const int64_t per = (d.paths + QSEG - 1) / QSEG;
+        cnt[CNT_RAYS] = (unsigned long long)d.paths;
+        cnt[CNT_SEGCAP] = (unsigned long long)per;
+        for (int k = 0; k < QSEG; ++k) {
+            const int64_t left = d.paths - k * per;
+            cnt[cnt_rayseg(k)] = (unsigned long long)(left < 0 ? 0 : left < per ? left : per);
+        }
     }
 }
 
@@ -166,9 +183,13 @@ __global__ void k_batch_totals(const unsigned long long* __restrict__ cnt, int d
     if (threadIdx.x != 0) return;
     unsigned long long c = 0, s = 0, p = 0;
     for (int d = 0; d < depth_slots; ++d) {
-        c += cnt[d * CNT_STRIDE + 0];
-        s += cnt[d * CNT_STRIDE + 1] + cnt[d * CNT_STRIDE + 7];
-        p += cnt[d * CNT_STRIDE + 2];
+        const unsigned long long* dc = cnt + (int64_t)d * CNT_STRIDE;
+        for (int k = 0; k < QSEG; ++k) {
+            c += dc[cnt_rayseg(k)];
+            s += dc[cnt_shadowseg(k)];
+        }
+        s += dc[CNT_EXTRA];
+        p += dc[CNT_PROBE];
     }
     totals[0] += c;
     totals[1] += s;
@@ -239,21 +260,29 @@ __global__ void __launch_bounds__(TRACE_BLOCK)
             RayQueue<T> qin, RayQueue<T> qout, ShadowQueue<T> sq, int64_t cap, unsigned long long* __restrict__ cnt,
             float* __restrict__ L, float* __restrict__ aov)
 {
-    __shared__ int2 stack[MAX_LEVELS * TRACE_BLOCK];
+    __shared__ int2 stack[stack_levels(N) * TRACE_BLOCK];
     int2* st = stack + threadIdx.x;
-    const int64_t count = (int64_t)cnt[depth * CNT_STRIDE + 0];
-    unsigned long long* next_count = &cnt[(depth + 1) * CNT_STRIDE + 0];
-    unsigned long long* shadow_count = &cnt[depth * CNT_STRIDE + 1];
-    unsigned long long* probe_count = &cnt[depth * CNT_STRIDE + 2];
+    // input segment = output segments = blockIdx.x % QSEG (the grid is a
+    // multiple of QSEG blocks); the blocks of a segment stride over its rays
+    const unsigned long long* dc = cnt + (int64_t)depth * CNT_STRIDE;
+    const int seg = (int)(blockIdx.x % QSEG);
+    const int64_t count = (int64_t)dc[cnt_rayseg(seg)];
+    const int64_t in0 = (int64_t)seg * (int64_t)dc[CNT_SEGCAP];
+    unsigned long long* next_count = &cnt[(int64_t)(depth + 1) * CNT_STRIDE + cnt_rayseg(seg)];
+    const int64_t out0 = (int64_t)seg * (int64_t)cnt[(int64_t)(depth + 1) * CNT_STRIDE + CNT_SEGCAP];
+    unsigned long long* shadow_count = &cnt[(int64_t)depth * CNT_STRIDE + cnt_shadowseg(seg)];
+    const int64_t sout0 = (int64_t)seg * (int64_t)dc[CNT_SSEGCAP];
+    unsigned long long* probe_count = &cnt[(int64_t)depth * CNT_STRIDE + CNT_PROBE];
     const int64_t pixel0 = desc->pixel0;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t base = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)(gridDim.x / QSEG) * blockDim.x;
+    const int64_t base = (int64_t)(blockIdx.x / QSEG) * blockDim.x + threadIdx.x;
     const int64_t scap = cap * (sc.light_count > 1 ? sc.light_count : 1);
     // every lane of a wave iterates the same number of times (wave-level appends)
     const int64_t iters = (count + stride - 1) / stride;
     for (int64_t it = 0; it < iters; ++it) {
-        const int64_t i = base + it * stride;
-        const bool valid = i < count;
+        const int64_t j = base + it * stride;
+        const bool valid = j < count;
+        const int64_t i = in0 + j;
         ShadeOut<N, T> out;
         out.next = false;
         out.shadow = false;
@@ -285,6 +314,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK)
         // compaction: one atomic per wave per queue
         int64_t slot = wave_append(out.next, next_count);
         if (slot >= 0) {
+            slot += out0;
 #pragma unroll
             for (int k = 0; k < N; ++k) {
                 qout.org[(int64_t)k * cap + slot] = out.norg[k];
@@ -307,6 +337,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK)
             }
             int64_t ss = wave_append(out.shadow, shadow_count);
             if (ss >= 0) {
+                ss += sout0;
 #pragma unroll
                 for (int k = 0; k < N; ++k) {
                     sq.org[(int64_t)k * scap + ss] = out.sorg[k];
@@ -379,11 +410,12 @@ __device__ __forceinline__ void push_pending(unsigned long long* res_count, int3
 // Persistent traversal (shadow stage): each lane holds one ray walk (RayWalk)
 // across the iterations of the loop and does one leaf step per iteration; when
 // at least refill_lanes<N>() lanes of a wave have finished, the wave takes that
-// many new rays with one atomic.  The queue is split into 8 contiguous slices, one per XCD
-// (workgroups are dispatched round robin over the 8 XCDs): a wave takes rays
-// from its own XCD's slice, so neighbouring rays share that XCD's L2, and
-// moves on to the next slices once its own is empty.  Every wave leaves the
-// loop after all 8 slices are empty and its own rays are done.
+// many new rays with one atomic.  The queue's QSEG segments are the slices of
+// the 8 XCDs (workgroups are dispatched round robin over them, so block b runs
+// on XCD b % 8, the XCD whose shade blocks filled segment b % 8): a wave takes
+// rays from its own XCD's segment, so neighbouring rays share that XCD's L2,
+// and moves on to the next segments once its own is empty.  Every wave leaves
+// the loop after all segments are empty and its own rays are done.
 // Measured with one batch per pass (Mrays/s, repeated runs): n = 4 (C1) 8 lanes 1224;
 // 16 lanes 1330, 1345, 1326, 1333; 24 lanes 1344; 32 lanes 1308, 1312, 1309, 1320; 48 lanes
 // 1321.  n = 5 (C2): 16 lanes 262, 32 lanes 276.  -DNSP_REFILL_LANES=L sets one
@@ -397,7 +429,6 @@ constexpr int refill_lanes()
     return 32;
 #endif
 }
-constexpr int XCDS = 8;
 
 // Minimum waves per SIMD asked of the compiler for the persistent walks
 // (-DNSP_POOL_MIN_WAVES=W); by default their register use alone sets it.
@@ -409,14 +440,23 @@ constexpr int XCDS = 8;
 
 template <int REFILL_LANES>
 struct RayPool {
-    unsigned long long* work;  // XCDS slice counters
-    int64_t count;
+    unsigned long long* work;         // rays taken from segment k: work[k * CNT_LINE]
+    const unsigned long long* segn;   // rays in segment k: segn[k * CNT_LINE]
+    int64_t segcap;                   // segment k starts at k * segcap
+    int64_t lo, hi;                   // the current segment's rays
     int slice, visited;
     bool drained;
 
-    __device__ __forceinline__ RayPool(unsigned long long* w, int64_t n)
-        : work(w), count(n), slice((int)(blockIdx.x % XCDS)), visited(0), drained(false)
+    __device__ __forceinline__ RayPool(unsigned long long* w, const unsigned long long* n, int64_t cap)
+        : work(w), segn(n), segcap(cap), slice((int)(blockIdx.x % QSEG)), visited(0), drained(false)
     {
+        load();
+    }
+
+    __device__ __forceinline__ void load()
+    {
+        lo = (int64_t)slice * segcap;
+        hi = lo + (int64_t)segn[slice * CNT_LINE];
     }
 
     // Ray index for each lane with want set (or -1); wave-uniform call.
@@ -425,20 +465,23 @@ struct RayPool {
         const unsigned long long idle = __ballot(want);
         const int nidle = __popcll(idle);
         if (nidle < REFILL_LANES) return -1;
-        const int64_t lo = count * slice / XCDS, hi = count * (slice + 1) / XCDS;
+        const int64_t l0 = lo, h0 = hi;
         const int lane = (int)(threadIdx.x & 63);
         const int leader = __ffsll((unsigned long long)idle) - 1;
         unsigned long long base = 0;
-        if (lane == leader) base = atomicAdd(&work[slice], (unsigned long long)nidle);
+        if (lane == leader) base = atomicAdd(&work[slice * CNT_LINE], (unsigned long long)nidle);
         const unsigned blo = __shfl((unsigned)base, leader), bhi = __shfl((unsigned)(base >> 32), leader);
         base = ((unsigned long long)bhi << 32) | blo;
-        if (lo + (int64_t)base + nidle >= hi) {
-            slice = (slice + 1) % XCDS;
-            if (++visited == XCDS) drained = true;
+        if (l0 + (int64_t)base + nidle >= h0) {
+            slice = (slice + 1) % QSEG;
+            if (++visited == QSEG)
+                drained = true;
+            else
+                load();
         }
         if (!want) return -1;
-        const int64_t i = lo + (int64_t)base + __popcll(idle & ((1ull << lane) - 1ull));
-        return i < hi ? i : -1;
+        const int64_t i = l0 + (int64_t)base + __popcll(idle & ((1ull << lane) - 1ull));
+        return i < h0 ? i : -1;
     }
 };
 
@@ -452,9 +495,9 @@ template <int N, typename T, int LAY>
 __global__ void __launch_bounds__(TRACE_BLOCK) k_closest(TreeArgs A, RayQueue<T> q, int64_t cap,
                                                          const unsigned long long* __restrict__ cnt)
 {
-    __shared__ int2 stack[MAX_LEVELS * TRACE_BLOCK];
+    __shared__ int2 stack[stack_levels(N) * TRACE_BLOCK];
     int2* st = stack + threadIdx.x;
-    const int64_t count = (int64_t)cnt[0];
+    const int64_t count = (int64_t)cnt[CNT_RAYS];
     for (int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x; i < count;
          i += (int64_t)gridDim.x * blockDim.x) {
         RayWalk<N, T, false, LAY, false> w;
@@ -488,11 +531,11 @@ __global__ void NSP_POOL_BOUNDS k_closest_pool(TreeArgs A, RayQueue<T> q, int64_
                                                               unsigned long long* __restrict__ res_count,
                                                               int32_t* __restrict__ res)
 {
-    __shared__ int2 stack[MAX_LEVELS * TRACE_BLOCK];
+    __shared__ int2 stack[stack_levels(N) * TRACE_BLOCK];
     int2* st = stack + threadIdx.x;
     constexpr bool COOP = LAY == 1 && sizeof(T) == 4;
     __shared__ CoopWave coop[COOP ? TRACE_BLOCK / 64 : 1];
-    RayPool<refill_lanes<N>()> pool(cnt + 8, (int64_t)cnt[0]);
+    RayPool<refill_lanes<N>()> pool(cnt + cnt_cwork(0), cnt + cnt_rayseg(0), (int64_t)cnt[CNT_SEGCAP]);
     RayWalk<N, T, false, LAY, true> w;
     int64_t ray = -1;
     for (;;) {
@@ -569,7 +612,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_closest_exact(TreeArgs A, RayQu
                                                                const unsigned long long* __restrict__ redo_count,
                                                                const int32_t* __restrict__ redo)
 {
-    __shared__ int2 stack[MAX_LEVELS * TRACE_BLOCK];
+    __shared__ int2 stack[stack_levels(N) * TRACE_BLOCK];
     int2* st = stack + threadIdx.x;
     const int64_t count = (int64_t)*redo_count;
     for (int64_t j = (int64_t)xcd_block() * blockDim.x + threadIdx.x; j < count;
@@ -597,11 +640,11 @@ __global__ void NSP_POOL_BOUNDS k_shadow(TreeArgs A, ShadowQueue<T> sq, int64_t 
                                                         unsigned long long* __restrict__ res_count,
                                                         int32_t* __restrict__ res)
 {
-    __shared__ int2 stack[MAX_LEVELS * TRACE_BLOCK];
+    __shared__ int2 stack[stack_levels(N) * TRACE_BLOCK];
     int2* st = stack + threadIdx.x;
     constexpr bool COOP = LAY == 1 && sizeof(T) == 4;
     __shared__ CoopWave coop[COOP ? TRACE_BLOCK / 64 : 1];
-    RayPool<refill_lanes<N>()> pool(cnt + 15, (int64_t)cnt[0]);
+    RayPool<refill_lanes<N>()> pool(cnt + cnt_swork(0), cnt + cnt_shadowseg(0), (int64_t)cnt[CNT_SSEGCAP]);
     RayWalk<N, T, true, LAY, true> w;
     int64_t ray = -1;
     for (;;) {
@@ -654,7 +697,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_shadow_exact(TreeArgs A, Shadow
                                                               const unsigned long long* __restrict__ redo_count,
                                                               const int32_t* __restrict__ redo)
 {
-    __shared__ int2 stack[MAX_LEVELS * TRACE_BLOCK];
+    __shared__ int2 stack[stack_levels(N) * TRACE_BLOCK];
     int2* st = stack + threadIdx.x;
     const int64_t count = (int64_t)*redo_count;
     for (int64_t j = (int64_t)xcd_block() * blockDim.x + threadIdx.x; j < count;
@@ -822,6 +865,7 @@ void alloc_render_buffers(RenderBuffers& b, int n, int precision, int64_t cap, i
     size_t ts = precision ? sizeof(double) : sizeof(float);
     b.n = n;
     b.precision = precision;
+    cap = (cap + QSEG - 1) / QSEG * QSEG;  // whole queue segments
     b.cap = cap;
     b.depth_slots = max_depth + 2;
     b.lights = std::max(1, lights);
@@ -965,8 +1009,8 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
         rl.marks->marks.emplace_back(stage, e0, e1);
     };
     auto timed = [&](int stage, auto&& launch) { timed_on(s, stage, launch); };
-    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(256), 0, s, cnt, CNT_STRIDE * (rl.max_depth + 2),
-                       list, bidx, b.desc.as<BatchDesc>());
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(256), 0, s, cnt, rl.max_depth + 2, list, bidx,
+                       b.desc.as<BatchDesc>(), cap / QSEG, cap / QSEG * std::max(1, sc.light_count));
     NSP_HIP(hipGetLastError());
     timed(0, [&] {
         hipLaunchKernelGGL((k_raygen<N, T>), dim3(g), dim3(TRACE_BLOCK), 0, s, sc, desc, qa, cap, L, Ls);
@@ -975,7 +1019,7 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
         RayQueue<T>& qin = (depth & 1) ? qb : qa;
         RayQueue<T>& qout = (depth & 1) ? qa : qb;
         ShadowQueue<T> sq = sqs[depth & 1];
-        sq.extra = cnt + depth * CNT_STRIDE + 7;
+        sq.extra = cnt + depth * CNT_STRIDE + CNT_EXTRA;
         unsigned long long* dc = cnt + depth * CNT_STRIDE;
         const bool rasterise = sizeof(T) == 4 && depth == 0 && rl.raster;
         timed(rasterise ? 5 : 1, [&] {
@@ -987,15 +1031,16 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
                 }
             }
             if (depth > 0) {
-                hipLaunchKernelGGL(kcp, dim3(gcp), dim3(TRACE_BLOCK), 0, s, A, qin, cap, dc, dc + 5, redo_c, dc + 3,
-                                   res_c);
+                hipLaunchKernelGGL(kcp, dim3(gcp), dim3(TRACE_BLOCK), 0, s, A, qin, cap, dc, dc + CNT_REDO_C, redo_c,
+                                   dc + CNT_RES_C, res_c);
                 NSP_HIP(hipGetLastError());
                 if constexpr (sizeof(T) == 4) {
-                    hipLaunchKernelGGL((k_closest_resolve<N>), dim3(gx), dim3(256), 0, s, A, qin, cap, dc + 3, res_c);
+                    hipLaunchKernelGGL((k_closest_resolve<N>), dim3(gx), dim3(256), 0, s, A, qin, cap, dc + CNT_RES_C,
+                                       res_c);
                     NSP_HIP(hipGetLastError());
                 }
-                hipLaunchKernelGGL((k_closest_exact<N, T>), dim3(gx), dim3(TRACE_BLOCK), 0, s, A, qin, cap, dc + 5,
-                                   redo_c);
+                hipLaunchKernelGGL((k_closest_exact<N, T>), dim3(gx), dim3(TRACE_BLOCK), 0, s, A, qin, cap,
+                                   dc + CNT_REDO_C, redo_c);
                 return;
             }
             hipLaunchKernelGGL(kc, dim3(g), dim3(TRACE_BLOCK), 0, s, A, qin, cap, dc);
@@ -1013,16 +1058,16 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
         timed_on(ss, 3, [&] {
             float* direct = (aov && depth == 0) ? aov + (int64_t)(3 + N) * cap : nullptr;
             const unsigned gs = depth < rl.max_depth ? gs_side : gs_full;
-            hipLaunchKernelGGL(ks, dim3(gs), dim3(TRACE_BLOCK), 0, ss, As, sq, scap, dc + 1, Ls, cap, direct,
-                               dc + 6, redo_s, dc + 4, res_s);
+            hipLaunchKernelGGL(ks, dim3(gs), dim3(TRACE_BLOCK), 0, ss, As, sq, scap, dc, Ls, cap, direct,
+                               dc + CNT_REDO_S, redo_s, dc + CNT_RES_S, res_s);
             NSP_HIP(hipGetLastError());
             if constexpr (sizeof(T) == 4) {
                 hipLaunchKernelGGL((k_shadow_resolve<N>), dim3(gx), dim3(256), 0, ss, As, sq, scap, Ls, cap, direct,
-                                   dc + 4, res_s);
+                                   dc + CNT_RES_S, res_s);
                 NSP_HIP(hipGetLastError());
             }
             hipLaunchKernelGGL((k_shadow_exact<N, T>), dim3(gx), dim3(TRACE_BLOCK), 0, ss, As, sq, scap, Ls, cap,
-                               direct, dc + 6, redo_s);
+                               direct, dc + CNT_REDO_S, redo_s);
         });
         if (rl.side) NSP_HIP(hipEventRecord(b.done_ev[depth & 1], ss));
     }

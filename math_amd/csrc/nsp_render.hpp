@@ -123,17 +123,39 @@ struct MeshShadeArgs {
     const int32_t* ntype;
 };
 
-// Per-depth counters, CNT_STRIDE u64 per depth: 0 queued rays, 1 shadow rays, 2 probe rays,
-// 3 closest-hit FP64 decisions, 4 shadow FP64 decisions, 5 closest-hit re-walks, 6 shadow re-walks,
-// 7 ray counts of shadow rays' shape tests, 8-15 closest-hit work index per queue slice,
-// 16-23 shadow work index per queue slice.  The stride keeps the counters that one stage's waves
-// append to with atomics (next depth's queue, shadow queue) on different cache lines.
-constexpr int CNT_STRIDE = 24;
+// Queue segments.  Device-scope atomics on one address serialise (about 12 ns
+// each on MI355X, against under 2 ns when eight addresses share the load), so
+// a stage that appends to a queue with one atomic per wave would be bound by
+// that counter.  Every ray and shadow queue is therefore split into QSEG
+// segments, one per XCD: segment k holds its rays at [k * segcap, k * segcap +
+// count_k) and has a counter of its own.  The shade stage's block b reads input
+// segment b % QSEG and appends to output segment b % QSEG, so a segment never
+// receives more rays than its input segment held; the persistent walks take
+// segment k as the slice of the blocks on XCD k.
+constexpr int QSEG = 8;
+constexpr int CNT_LINE = 16;  // u64 per 128-byte line: one line per atomically updated counter
+
+// Per-depth counters, CNT_STRIDE u64 per depth.  Line 0 holds the scalars:
+constexpr int CNT_RAYS = 0;      // rays of the depth's input queue (depth 0: the batch's paths)
+constexpr int CNT_PROBE = 2;     // smooth-normal probe rays
+constexpr int CNT_RES_C = 3;     // closest-hit walks with pending FP64 decisions
+constexpr int CNT_RES_S = 4;     // shadow walks with pending FP64 decisions
+constexpr int CNT_REDO_C = 5;    // closest-hit re-walks
+constexpr int CNT_REDO_S = 6;    // shadow re-walks
+constexpr int CNT_EXTRA = 7;     // ray counts of shadow rays' shape tests
+constexpr int CNT_SEGCAP = 8;    // segment capacity of the depth's input ray queue
+constexpr int CNT_SSEGCAP = 9;   // segment capacity of the depth's shadow queue
+// then one line per segment counter:
+__host__ __device__ constexpr int cnt_rayseg(int k) { return CNT_LINE * (1 + k); }              // input rays
+__host__ __device__ constexpr int cnt_shadowseg(int k) { return CNT_LINE * (1 + QSEG + k); }    // shadow rays
+__host__ __device__ constexpr int cnt_cwork(int k) { return CNT_LINE * (1 + 2 * QSEG + k); }    // closest-hit walks taken
+__host__ __device__ constexpr int cnt_swork(int k) { return CNT_LINE * (1 + 3 * QSEG + k); }    // shadow walks taken
+constexpr int CNT_STRIDE = CNT_LINE * (1 + 4 * QSEG);
 
 struct RenderBuffers {
     int n = 0;
     int precision = 0;
-    int64_t cap = 0;        // paths per batch
+    int64_t cap = 0;        // paths per batch, a multiple of QSEG (queue arrays are component-major with stride cap)
     int lights = 1;         // shadow rays per path and bounce the shadow queues hold
     int depth_slots = 0;
     DevBuf qa[8], qb[8];

@@ -27,7 +27,11 @@ namespace nsp {
 #define NSP_TRACE_BLOCK 256
 #endif
 constexpr int TRACE_BLOCK = NSP_TRACE_BLOCK;  // threads per block of the trace and stage kernels
-constexpr int MAX_LEVELS = 10;  // reference depth limit, src/painter/space/tree.h:428-431
+// Levels (the root's included) of a walk's LDS node stack: the tree depth
+// limit per dimension (src/painter/shapes/mesh.cpp:33-58; the tree build never
+// makes more levels), so that an n = 4 or 5 walk keeps 8 or 6 records per lane
+// instead of 10.
+__host__ __device__ constexpr int stack_levels(int n) { return n <= 3 ? 10 : n == 4 ? 8 : n == 5 ? 6 : 5; }
 
 // Half-width of the barycentric band (in barycentric units, scaled by
 // 1 + 1/|cos|) inside which an FP32 simplex decision is re-taken in FP64, and
