@@ -464,8 +464,8 @@ def main():
         scene.paint(spp, depth, seed=1000 + w, batch_paths=args.batch_paths, use_graph=args.graph, primary=primary)
         log(f"warmup {w}: {scene.stage_stats()['ms_total']:.1f} ms")
     r0 = scene.statistics()["rays"]
-    stage_ms = dict(raster=0.0, closest=0.0, shade=0.0, shadow=0.0)
-    launches = dict(closest=0, shadow=0)
+    stage_ms = dict(raster=0.0, closest=0.0, shade=0.0, shadow=0.0, walk=0.0)
+    launches = dict(closest=0, shadow=0, walk=0)
     t0 = time.perf_counter()
     for k in range(args.steps):
         scene.paint(spp, depth, seed=k, batch_paths=args.batch_paths, use_graph=args.graph, primary=primary)
@@ -475,7 +475,7 @@ def main():
         for key in launches:
             launches[key] += st["launches_" + key]
         log(f"step {k}: {st['ms_total']:.1f} ms, raster {st['ms_raster']:.1f} ms, closest {st['ms_closest']:.1f} ms, "
-            f"shade {st['ms_shade']:.1f} ms, shadow {st['ms_shadow']:.1f} ms")
+            f"shade {st['ms_shade']:.1f} ms, shadow {st['ms_shadow']:.1f} ms, walk {st['ms_walk']:.1f} ms")
     elapsed = time.perf_counter() - t0
     rays = scene.statistics()["rays"] - r0
     mrays = rays / elapsed / 1e6
@@ -490,15 +490,20 @@ def main():
     layout_name = ["leaf_soa", "simplex_aos"][mesh.record_layout]
     walk_rays = max(cst["closest_rays"] - cst["primary_rays"], 1)
     sr = max(cst["shadow_rays"], 1)
+    per_closest = dict(rays=cst["closest_rays"] - cst["primary_rays"], nodes=cst["node_visits"] / walk_rays,
+                       leaves=cst["leaf_visits"] / walk_rays, tests=cst["simplex_tests"] / walk_rays,
+                       fp64_frac=cst["retests"] / max(cst["simplex_tests"], 1))
+    per_shadow = dict(rays=cst["shadow_rays"], nodes=cst["shadow_node_visits"] / sr,
+                      leaves=cst["shadow_leaf_visits"] / sr, tests=cst["shadow_simplex_tests"] / sr,
+                      fp64_frac=cst["shadow_retests"] / max(cst["shadow_simplex_tests"], 1))
     kernels = {
         "k_closest": dict(bytes=walk_bytes(cst, n), ms=stage_ms["closest"], launches=launches["closest"],
-                          per_ray=dict(nodes=cst["node_visits"] / walk_rays, leaves=cst["leaf_visits"] / walk_rays,
-                                       tests=cst["simplex_tests"] / walk_rays,
-                                       fp64_frac=cst["retests"] / max(cst["simplex_tests"], 1))),
+                          per_ray=per_closest),
         "k_shadow": dict(bytes=shadow_bytes(cst, n), ms=stage_ms["shadow"], launches=launches["shadow"],
-                         per_ray=dict(nodes=cst["shadow_node_visits"] / sr, leaves=cst["shadow_leaf_visits"] / sr,
-                                      tests=cst["shadow_simplex_tests"] / sr,
-                                      fp64_frac=cst["shadow_retests"] / max(cst["shadow_simplex_tests"], 1))),
+                         per_ray=per_shadow),
+        # one launch per depth d holding the closest-hit walks of depth d + 1 and the shadow walks of depth d
+        "k_walk": dict(bytes=walk_bytes(cst, n) + shadow_bytes(cst, n), ms=stage_ms["walk"],
+                       launches=launches["walk"], per_ray=dict(closest=per_closest, shadow=per_shadow)),
     }
 
     def line(name):
@@ -512,8 +517,12 @@ def main():
                     bytes_per_launch=int(k["bytes"] / max(k["launches"] // max(args.steps, 1), 1)),
                     per_ray=k["per_ray"])
 
-    if min(kernels[k]["ms"] for k in kernels) > 0:
-        dominant = max(kernels, key=lambda name: kernels[name]["ms"])
+    if kernels["k_walk"]["ms"] > 0:
+        # the merged walks: one kernel holds all traversal of the bounce and shadow rays
+        roofline = line("k_walk")
+        roofline["stage_ms_per_step"] = {k: round(v / args.steps, 2) for k, v in stage_ms.items()}
+    elif min(kernels[k]["ms"] for k in ("k_closest", "k_shadow")) > 0:
+        dominant = max(("k_closest", "k_shadow"), key=lambda name: kernels[name]["ms"])
         roofline = line(dominant)
         roofline["other"] = line("k_shadow" if dominant == "k_closest" else "k_closest")
         roofline["stage_ms_per_step"] = {k: round(v / args.steps, 2) for k, v in stage_ms.items()}

@@ -377,6 +377,13 @@ typedef struct {
      * of ms_closest, its rays are part of closest_rays (primary_rays of them) */
     double ms_raster;
     int64_t launches_raster, primary_rays;
+    /* FP32 walks over simplex-ordered records: one walk launch per depth d
+     * holds the closest-hit walks of depth d + 1's bounce rays and the shadow
+     * walks of depth d (with their FP64 decisions); its time is not part of
+     * ms_closest or ms_shadow, its traversal counters are split between the
+     * two kinds as above */
+    double ms_walk;
+    int64_t launches_walk;
 } nsp_stage_stats;
 int nsp_last_pass_stats(const nsp_scene* scene, nsp_stage_stats* out);
 /* Enables per-ray traversal counters (node visits, leaf visits, simplex

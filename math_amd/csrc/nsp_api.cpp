@@ -1016,6 +1016,7 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
                 case 3: st.ms_shadow += ms; ++st.launches_shadow; break;
                 case 4: st.ms_accumulate += ms; break;
                 case 5: st.ms_raster += ms; ++st.launches_raster; break;
+                case 6: st.ms_walk += ms; ++st.launches_walk; break;
                 }
             }
             st.ms_total = secs * 1e3;

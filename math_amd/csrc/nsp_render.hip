@@ -562,7 +562,7 @@ __global__ void NSP_POOL_BOUNDS k_closest_pool(TreeArgs A, RayQueue<T> q, int64_
             int e0 = 0, n = 0;
             ended = ray >= 0 && w.locate(A, st, &e0, &n);
             const bool live = ray >= 0 && !ended;
-            coop_test<N, false>(A, coop[threadIdx.x >> 6], w, live, e0, n);
+            coop_test<N, COOP_NEAREST>(A, coop[threadIdx.x >> 6], w, live, e0, n);
             if (live) ended = w.deferred || w.advance(A);
         } else {
             ended = ray >= 0 && w.step(A, st, nullptr, nullptr);
@@ -667,7 +667,7 @@ __global__ void NSP_POOL_BOUNDS k_shadow(TreeArgs A, ShadowQueue<T> sq, int64_t 
             int e0 = 0, n = 0;
             ended = ray >= 0 && w.locate(A, st, &e0, &n);
             const bool live = ray >= 0 && !ended;
-            coop_test<N, true>(A, coop[threadIdx.x >> 6], w, live, e0, n);
+            coop_test<N, COOP_ANY>(A, coop[threadIdx.x >> 6], w, live, e0, n);
             // a hit ends an any-hit walk (coop_test then leaves deferred unset)
             if (live) ended = w.best >= 0 || w.deferred || w.advance(A);
         } else {
@@ -736,6 +736,156 @@ __global__ void __launch_bounds__(256) k_shadow_resolve(TreeArgs A, ShadowQueue<
         double best_t = (double)sq.tmax[i];
         if (!pending_resolve<N, true>(A, o, d, res + RES_STRIDE * j + 1, sq.tmax[i], &best, &best_t))
             shadow_unoccluded<N, float>(sq, scap, i, L, cap, direct);
+    }
+}
+
+// The rays of one walk launch: the closest-hit rays of depth d + 1 (counters
+// cc) and the shadow rays of depth d (counters sc), both made by shade(d).
+// Segment k of the combined ray space holds segment k's closest-hit rays, then
+// its shadow rays; a wave takes rays from its own XCD's segment first (as
+// RayPool) and then from the others, with one atomic per refill on the
+// segment's counter sc[cnt_swork(k)].
+template <int REFILL_LANES>
+struct WalkPool {
+    const unsigned long long* cc;
+    unsigned long long* sc;
+    int64_t csegcap, ssegcap;
+    int64_t nc, total;  // the current segment's closest-hit rays and all its rays
+    int slice, visited;
+    bool drained;
+
+    __device__ __forceinline__ WalkPool(const unsigned long long* c, unsigned long long* s)
+        : cc(c), sc(s), csegcap((int64_t)c[CNT_SEGCAP]), ssegcap((int64_t)s[CNT_SSEGCAP]),
+          slice((int)(blockIdx.x % QSEG)), visited(0), drained(false)
+    {
+        load();
+    }
+
+    __device__ __forceinline__ void load()
+    {
+        nc = (int64_t)cc[cnt_rayseg(slice)];
+        total = nc + (int64_t)sc[cnt_shadowseg(slice)];
+    }
+
+    // For each lane with want set: the index of its ray in the closest-hit
+    // queue (*shadow false) or in the shadow queue (*shadow true), or -1.
+    // Wave-uniform call.
+    __device__ __forceinline__ int64_t take(bool want, bool* shadow)
+    {
+        const unsigned long long idle = __ballot(want);
+        const int nidle = __popcll(idle);
+        if (nidle < REFILL_LANES) return -1;
+        const int s0 = slice;
+        const int64_t nc0 = nc, tot0 = total;
+        const int lane = (int)(threadIdx.x & 63);
+        const int leader = __ffsll((unsigned long long)idle) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(&sc[cnt_swork(s0)], (unsigned long long)nidle);
+        const unsigned blo = __shfl((unsigned)base, leader), bhi = __shfl((unsigned)(base >> 32), leader);
+        base = ((unsigned long long)bhi << 32) | blo;
+        if ((int64_t)base + nidle >= tot0) {
+            slice = (slice + 1) % QSEG;
+            if (++visited == QSEG)
+                drained = true;
+            else
+                load();
+        }
+        if (!want) return -1;
+        const int64_t x = (int64_t)base + __popcll(idle & ((1ull << lane) - 1ull));
+        if (x >= tot0) return -1;
+        *shadow = x >= nc0;
+        return x < nc0 ? s0 * csegcap + x : s0 * ssegcap + (x - nc0);
+    }
+};
+
+// One persistent walk launch per depth d for FP32 walks over simplex-ordered
+// records: the closest-hit walks of depth d + 1's bounce rays and the shadow
+// walks of depth d share the launch, so a single grid fills the machine and
+// no second persistent kernel competes with it for registers, LDS and
+// dispatch (two such kernels on two streams held back the launch stream's
+// small kernels until the shadow walks ended).  Each lane holds one walk of
+// either kind; the wave tests its lanes' leaf candidates together
+// (coop_test with a per-lane mode).  Finished walks end as k_closest_pool's
+// and k_shadow's do: hits written, FP64 decisions and re-walks listed, an
+// unoccluded shadow ray's contribution added to Ls.
+template <int N>
+__global__ void NSP_POOL_BOUNDS k_walk(TreeArgs A, unsigned long long* __restrict__ shadow_counters,
+                                       RayQueue<float> q, int64_t cap, ShadowQueue<float> sq, int64_t scap,
+                                       unsigned long long* __restrict__ cc, unsigned long long* __restrict__ sc,
+                                       float* __restrict__ Ls, float* __restrict__ direct,
+                                       int32_t* __restrict__ redo_c, int32_t* __restrict__ res_c,
+                                       int32_t* __restrict__ redo_s, int32_t* __restrict__ res_s)
+{
+    __shared__ int2 stack[stack_levels(N) * TRACE_BLOCK];
+    int2* st = stack + threadIdx.x;
+    __shared__ CoopWave coop[TRACE_BLOCK / 64];
+    WalkPool<refill_lanes<N>()> pool(cc, sc);
+    RayWalk<N, float, false, 1, true> w;
+    int64_t ray = -1;
+    bool any = false;  // the lane's walk is a shadow (any-hit) walk
+    for (;;) {
+        if (!pool.drained) {
+            bool shadow = false;
+            const int64_t i = pool.take(ray < 0, &shadow);
+            if (i >= 0) {
+                float tmax;
+                if (shadow) {
+#pragma unroll
+                    for (int k = 0; k < N; ++k) {
+                        w.o[k] = sq.org[(int64_t)k * scap + i];
+                        w.d[k] = sq.dir[(int64_t)k * scap + i];
+                    }
+                    tmax = sq.tmax[i];
+                } else {
+#pragma unroll
+                    for (int k = 0; k < N; ++k) {
+                        w.o[k] = q.org[(int64_t)k * cap + i];
+                        w.d[k] = q.dir[(int64_t)k * cap + i];
+                    }
+                    tmax = INFINITY;
+                }
+                any = shadow;
+                if (w.start(A, tmax)) {
+                    ray = i;
+                } else if (shadow) {
+                    shadow_unoccluded<N, float>(sq, scap, i, Ls, cap, direct);
+                } else {
+                    q.hit[i] = -1;
+                    q.hit_t[i] = 0.0f;
+                }
+            }
+        }
+        int e0 = 0, n = 0;
+        bool ended = ray >= 0 && w.locate(A, st, &e0, &n);
+        const bool live = ray >= 0 && !ended;
+        coop_test<N, COOP_PER_LANE>(A, coop[threadIdx.x >> 6], w, live, e0, n, any);
+        // a hit ends an any-hit walk (coop_test then leaves deferred unset)
+        if (live) ended = (any && w.best >= 0) || w.deferred || w.advance(A);
+        if (ended) {
+            if (any) {
+                if (w.deferred) {
+                    redo_s[atomicAdd(sc + CNT_REDO_S, 1ull)] = (int32_t)ray;
+                } else if (w.best < 0) {
+                    if (w.npend > 0)
+                        push_pending(sc + CNT_RES_S, res_s, ray, w.pend, w.npend);
+                    else
+                        shadow_unoccluded<N, float>(sq, scap, ray, Ls, cap, direct);
+                }
+                w.flush_to(shadow_counters);
+            } else {
+                if (w.deferred) {
+                    redo_c[atomicAdd(cc + CNT_REDO_C, 1ull)] = (int32_t)ray;
+                } else {
+                    const HitResult h = w.result();
+                    q.hit[ray] = h.idx;
+                    q.hit_t[ray] = (float)h.t;
+                    if (w.npend > 0) push_pending(cc + CNT_RES_C, res_c, ray, w.pend, w.npend);
+                }
+                w.flush_counters(A);
+            }
+            ray = -1;
+        }
+        if (pool.drained && __ballot(ray >= 0) == 0) break;
     }
 }
 
@@ -991,6 +1141,13 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
     // no closest-hit walk beside it and takes the whole grid
     const unsigned gs_side = rl.side ? std::max(8u, (gs_full * side_grid_pct<N>() / 100 + 7) / 8 * 8) : gs_full;
     const unsigned gx = 8 * 32;  // FP64 decisions and re-walks of shadow rays: a few per cent or fewer
+    // FP32 walks over the simplex-ordered records: the bounce rays' closest-hit
+    // walks of depth d + 1 and the shadow walks of depth d share one launch
+    // (k_walk) after shade(d), on the launch stream; otherwise the two walk
+    // kernels of each depth run as before, the shadow stage on the side stream
+    const bool merged = sizeof(T) == 4 && aos;
+    unsigned gw = 0;
+    if constexpr (sizeof(T) == 4) gw = persistent_grid(k_walk<N>);
     int32_t* redo_s = b.redo.as<int32_t>();
     int32_t* res_s = b.res.as<int32_t>();
     int32_t* redo_c = b.redo_c.as<int32_t>();
@@ -1022,7 +1179,8 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
         sq.extra = cnt + depth * CNT_STRIDE + CNT_EXTRA;
         unsigned long long* dc = cnt + depth * CNT_STRIDE;
         const bool rasterise = sizeof(T) == 4 && depth == 0 && rl.raster;
-        timed(rasterise ? 5 : 1, [&] {
+        // with merged walks, the closest hits of depth > 0 were found by the walk launch of depth - 1
+        if (depth == 0 || !merged) timed(rasterise ? 5 : 1, [&] {
             if constexpr (sizeof(T) == 4) {
                 if (rasterise) {
                     raster_launch(*rl.raster, *rl.mesh, A, desc, qin, cap, b.keys.as<unsigned long long>(),
@@ -1046,11 +1204,35 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
             hipLaunchKernelGGL(kc, dim3(g), dim3(TRACE_BLOCK), 0, s, A, qin, cap, dc);
         });
         // shade(depth) refills the shadow queue that the shadow stage of depth - 2 read
-        if (rl.side && depth >= 2) NSP_HIP(hipStreamWaitEvent(s, b.done_ev[depth & 1], 0));
+        if (!merged && rl.side && depth >= 2) NSP_HIP(hipStreamWaitEvent(s, b.done_ev[depth & 1], 0));
         timed(2, [&] {
             hipLaunchKernelGGL(kshade, dim3(g), dim3(TRACE_BLOCK), 0, s, sc, A, M, desc, depth, qin, qout, sq, cap,
                                cnt, L, aov);
         });
+        if constexpr (sizeof(T) == 4) {
+            if (merged) {
+                // one walk launch: the bounce rays of depth + 1 (in qout) and the shadow rays of depth
+                unsigned long long* dc1 = cnt + (depth + 1) * CNT_STRIDE;
+                float* direct = (aov && depth == 0) ? aov + (int64_t)(3 + N) * cap : nullptr;
+                timed(6, [&] {
+                    hipLaunchKernelGGL((k_walk<N>), dim3(gw), dim3(TRACE_BLOCK), 0, s, A, As.counters, qout, cap, sq,
+                                       scap, dc1, dc, Ls, direct, redo_c, res_c, redo_s, res_s);
+                    NSP_HIP(hipGetLastError());
+                    hipLaunchKernelGGL((k_closest_resolve<N>), dim3(gx), dim3(256), 0, s, A, qout, cap,
+                                       dc1 + CNT_RES_C, res_c);
+                    NSP_HIP(hipGetLastError());
+                    hipLaunchKernelGGL((k_closest_exact<N, T>), dim3(gx), dim3(TRACE_BLOCK), 0, s, A, qout, cap,
+                                       dc1 + CNT_REDO_C, redo_c);
+                    NSP_HIP(hipGetLastError());
+                    hipLaunchKernelGGL((k_shadow_resolve<N>), dim3(gx), dim3(256), 0, s, As, sq, scap, Ls, cap,
+                                       direct, dc + CNT_RES_S, res_s);
+                    NSP_HIP(hipGetLastError());
+                    hipLaunchKernelGGL((k_shadow_exact<N, T>), dim3(gx), dim3(TRACE_BLOCK), 0, s, As, sq, scap, Ls,
+                                       cap, direct, dc + CNT_REDO_S, redo_s);
+                });
+                continue;
+            }
+        }
         if (rl.side) {
             NSP_HIP(hipEventRecord(b.fork_ev, s));
             NSP_HIP(hipStreamWaitEvent(ss, b.fork_ev, 0));
@@ -1071,7 +1253,7 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
         });
         if (rl.side) NSP_HIP(hipEventRecord(b.done_ev[depth & 1], ss));
     }
-    if (rl.side) {
+    if (rl.side && !merged) {
         NSP_HIP(hipEventRecord(b.join_ev, ss));
         NSP_HIP(hipStreamWaitEvent(s, b.join_ev, 0));
     }

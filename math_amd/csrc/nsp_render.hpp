@@ -188,7 +188,8 @@ void launch_first_hit(const MeshDev& m, const float* org, const float* dir, int6
                       uint8_t* near, unsigned long long* counters, hipStream_t s);
 
 // HIP events around each stage launch (stage ids: 0 raygen, 1 closest,
-// 2 shade, 3 shadow, 4 accumulate, 5 rasterised primary visibility).
+// 2 shade, 3 shadow, 4 accumulate, 5 rasterised primary visibility, 6 merged
+// closest-hit and shadow walks).
 struct StageMarks {
     std::vector<hipEvent_t> pool;
     size_t used = 0;

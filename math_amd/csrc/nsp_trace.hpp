@@ -519,13 +519,17 @@ struct RayWalk {
         return r;
     }
 
-    __device__ __forceinline__ void flush_counters(const TreeArgs& A) const
+    __device__ __forceinline__ void flush_counters(const TreeArgs& A) const { flush_to(A.counters); }
+
+    // adds the walk's traversal counters to c[0..3] (node visits, leaf visits,
+    // simplex tests, FP64 decisions) when c is not null
+    __device__ __forceinline__ void flush_to(unsigned long long* c) const
     {
-        if (A.counters) {
-            atomicAdd(&A.counters[0], (unsigned long long)nvis);
-            atomicAdd(&A.counters[1], (unsigned long long)lvis);
-            atomicAdd(&A.counters[2], (unsigned long long)ntest);
-            atomicAdd(&A.counters[3], (unsigned long long)nret);
+        if (c) {
+            atomicAdd(&c[0], (unsigned long long)nvis);
+            atomicAdd(&c[1], (unsigned long long)lvis);
+            atomicAdd(&c[2], (unsigned long long)ntest);
+            atomicAdd(&c[3], (unsigned long long)nret);
         }
     }
 };
@@ -569,10 +573,15 @@ __device__ __forceinline__ void wave_lds_sync()
 //   any hit (ANY true): any hit before tmax ends the owner's walk.
 //   deferred candidates join the owner's pending list (no duplicates); more
 //     than PENDING set deferred, for a re-walk.
-template <int N, bool ANY, class W>
-__device__ __forceinline__ void coop_test(const TreeArgs& A, CoopWave& L, W& w, bool active, int e0, int cnt)
+// MODE: COOP_NEAREST or COOP_ANY for every lane, or COOP_PER_LANE, where each
+// lane's any_hit says which kind its walk is (a wave may hold both kinds).
+constexpr int COOP_NEAREST = 0, COOP_ANY = 1, COOP_PER_LANE = 2;
+template <int N, int MODE, class W>
+__device__ __forceinline__ void coop_test(const TreeArgs& A, CoopWave& L, W& w, bool active, int e0, int cnt,
+                                          bool any_hit = false)
 {
     constexpr int RS = rec32_stride(N);
+    const bool ANY = MODE == COOP_PER_LANE ? any_hit : MODE == COOP_ANY;
     const int lane = threadIdx.x & 63;
     const int c = active ? cnt : 0;
     int incl = c;
@@ -629,7 +638,9 @@ __device__ __forceinline__ void coop_test(const TreeArgs& A, CoopWave& L, W& w, 
             const int r = fp32_test<N, true>(A, comp, get_sid, oo, od, nullptr, nullptr, tlo, -1, &th, &si, &nr,
                                              &dummy);
             if (r == CAND_HIT) {
-                atomicMin(&L.key[owner], ANY ? (unsigned long long)(unsigned)si : hit_key((float)th, si));
+                // (an any-hit owner only asks whether its key fell below NO_HIT_KEY)
+                atomicMin(&L.key[owner],
+                          MODE == COOP_ANY ? (unsigned long long)(unsigned)si : hit_key((float)th, si));
             } else if (r == CAND_DEFER) {
                 atomicAdd(&L.ndefer[owner], 1);
                 bool known = false;
@@ -648,7 +659,7 @@ __device__ __forceinline__ void coop_test(const TreeArgs& A, CoopWave& L, W& w, 
         const unsigned long long k = L.key[lane];
         if (ANY && k != NO_HIT_KEY) {
             // the walk ends occluded; its pending list no longer matters
-            w.best = (int)(unsigned)k;
+            w.best = (int)(unsigned)(k & 0xffffffffull);
             w.best_t = (double)w.tmax;
             return;
         }

@@ -108,7 +108,8 @@ class StageStats(C.Structure):
                 ("shadow_rays", C.c_int64), ("node_visits", C.c_int64), ("leaf_visits", C.c_int64),
                 ("simplex_tests", C.c_int64), ("retests", C.c_int64), ("shadow_node_visits", C.c_int64),
                 ("shadow_leaf_visits", C.c_int64), ("shadow_simplex_tests", C.c_int64), ("shadow_retests", C.c_int64),
-                ("ms_raster", C.c_double), ("launches_raster", C.c_int64), ("primary_rays", C.c_int64)]
+                ("ms_raster", C.c_double), ("launches_raster", C.c_int64), ("primary_rays", C.c_int64),
+                ("ms_walk", C.c_double), ("launches_walk", C.c_int64)]
 
 
 # every symbol of include/nsp.h
