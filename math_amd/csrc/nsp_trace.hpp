@@ -223,6 +223,27 @@ __device__ __forceinline__ void load_rec32(const TreeArgs& A, int sid, float* re
     }
 }
 
+// The same record with every component in registers at return: the test's
+// early exits would otherwise let the compiler sink the later loads into
+// them, one memory round trip per exit (the normal, then d, then the planes).
+template <int N>
+__device__ __forceinline__ void load_rec32_now(const TreeArgs& A, int sid, float* rec)
+{
+    constexpr int RS = rec32_stride(N);
+    const float4* r4 = reinterpret_cast<const float4*>(A.rec32 + (int64_t)sid * RS);
+    float4 v[RS / 4];
+#pragma unroll
+    for (int k = 0; k < RS / 4; ++k) v[k] = r4[k];
+#pragma unroll
+    for (int k = 0; k < RS / 4; ++k) {
+        asm volatile("" ::"v"(v[k].x), "v"(v[k].y), "v"(v[k].z), "v"(v[k].w));
+        rec[4 * k] = v[k].x;
+        rec[4 * k + 1] = v[k].y;
+        rec[4 * k + 2] = v[k].z;
+        rec[4 * k + 3] = v[k].w;
+    }
+}
+
 template <int N, typename T, int LAY = -1, bool DEFER = false>
 __device__ __forceinline__ int candidate(const TreeArgs& A, int64_t e, const T* o, const T* d, const double* o64,
                                          const double* d64, T tlimit, int best_idx, double* t_hit, int* idx,
@@ -535,8 +556,9 @@ struct RayWalk {
 };
 
 // LDS of one wave for coop_test.
-struct CoopWave {
+struct alignas(16) CoopWave {
     int incl[64];                 // inclusive count of candidates up to each lane
+    int incl8[8];                 // incl of lanes 7, 15, ..., 63 (the owner search's first level)
     unsigned long long key[64];   // per lane: nearest hit (t bits << 32 | simplex), or any hit
     int npend[64], ndefer[64];    // per lane: pending candidates, deferred decisions
     int pend[64 * PENDING];
@@ -595,6 +617,7 @@ __device__ __forceinline__ void coop_test(const TreeArgs& A, CoopWave& L, W& w, 
     const float tl = ANY ? (float)w.tmax : (float)w.best_t;
     const int np0 = w.npend;
     L.incl[lane] = incl;
+    if ((lane & 7) == 7) L.incl8[lane >> 3] = incl;
     L.key[lane] = ANY || w.best < 0 ? NO_HIT_KEY : hit_key((float)w.best_t, w.best);
     L.npend[lane] = np0;
     L.ndefer[lane] = 0;
@@ -604,17 +627,17 @@ __device__ __forceinline__ void coop_test(const TreeArgs& A, CoopWave& L, W& w, 
     for (int base = 0; base < total; base += 64) {
         const int j = base + lane;
         const int jj = j < total ? j : total - 1;
-        // owner: the first lane whose inclusive count exceeds jj
-        int lo = 0, hi = 63;
-#pragma unroll
-        for (int st = 0; st < 6; ++st) {
-            const int mid = (lo + hi) >> 1;
-            if (L.incl[mid] > jj)
-                hi = mid;
-            else
-                lo = mid + 1;
-        }
-        const int owner = lo;
+        // owner: the first lane whose inclusive count exceeds jj, i.e. the
+        // number of lanes whose count is at most jj (the counts never fall):
+        // whole groups of 8 lanes first, then the lanes of the next group
+        const int4 g0 = *reinterpret_cast<const int4*>(&L.incl8[0]);
+        const int4 g1 = *reinterpret_cast<const int4*>(&L.incl8[4]);
+        const int g = (g0.x <= jj) + (g0.y <= jj) + (g0.z <= jj) + (g0.w <= jj) + (g1.x <= jj) + (g1.y <= jj) +
+                      (g1.z <= jj) + (g1.w <= jj);
+        const int4 l0 = *reinterpret_cast<const int4*>(&L.incl[8 * g]);
+        const int4 l1 = *reinterpret_cast<const int4*>(&L.incl[8 * g + 4]);
+        const int owner = 8 * g + (l0.x <= jj) + (l0.y <= jj) + (l0.z <= jj) + (l0.w <= jj) + (l1.x <= jj) +
+                          (l1.y <= jj) + (l1.z <= jj) + (l1.w <= jj);
         float oo[N], od[N];
 #pragma unroll
         for (int k = 0; k < N; ++k) {
@@ -628,7 +651,7 @@ __device__ __forceinline__ void coop_test(const TreeArgs& A, CoopWave& L, W& w, 
         if (j < total) {
             const int sid = A.entries[e0o + (j - exo)];
             float rec[RS];
-            load_rec32<N>(A, sid, rec);
+            load_rec32_now<N>(A, sid, rec);
             auto comp = [&](int cc) -> float { return rec[cc]; };
             auto get_sid = [&]() -> int { return sid; };
             double th;
