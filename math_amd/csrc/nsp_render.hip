@@ -416,17 +416,16 @@ __device__ __forceinline__ void push_pending(unsigned long long* res_count, int3
 // rays from its own XCD's segment, so neighbouring rays share that XCD's L2,
 // and moves on to the next segments once its own is empty.  Every wave leaves
 // the loop after all segments are empty and its own rays are done.
-// Measured with one batch per pass (Mrays/s, repeated runs): n = 4 (C1) 8 lanes 1224;
-// 16 lanes 1330, 1345, 1326, 1333; 24 lanes 1344; 32 lanes 1308, 1312, 1309, 1320; 48 lanes
-// 1321.  n = 5 (C2): 16 lanes 262, 32 lanes 276.  -DNSP_REFILL_LANES=L sets one
-// threshold for every dimension.
+// Measured with one batch per pass and the merged walk launch (C1, Mrays/s, two
+// runs each): 16 lanes 2483, 2474; 32 lanes 2452, 2439; 48 lanes 2396, 2410.
+// -DNSP_REFILL_LANES=L sets one threshold for every dimension.
 template <int N>
 constexpr int refill_lanes()
 {
 #ifdef NSP_REFILL_LANES
     return NSP_REFILL_LANES;
 #else
-    return 32;
+    return N >= 5 ? 32 : 16;
 #endif
 }
 

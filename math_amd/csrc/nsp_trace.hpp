@@ -316,7 +316,13 @@ __device__ __forceinline__ bool slab(const T* blo, const T* bsz, const T* o, con
 // have lowered the limit later candidates are tested against, and the walk
 // ends no earlier for it.  A walk with more pending candidates ends with
 // deferred set, for a re-walk with the re-test inline.
-constexpr int PENDING = 4;
+// (C1, one MI355X: 4 pending candidates ran at 2347 Mrays/s, 8 at 2445, 12 at
+// 2459; a walk that overflows its list is walked again with the FP64 re-test
+// inline, a slow single-lane tail after each walk launch.)
+#ifndef NSP_PENDING
+#define NSP_PENDING 8
+#endif
+constexpr int PENDING = NSP_PENDING;
 
 template <int N, typename T, bool ANY, int LAY = -1, bool DEFER = false>
 struct RayWalk {
@@ -559,6 +565,7 @@ struct RayWalk {
 struct alignas(16) CoopWave {
     int incl[64];                 // inclusive count of candidates up to each lane
     int incl8[8];                 // incl of lanes 7, 15, ..., 63 (the owner search's first level)
+    int ebase[64];                // per lane: the entry of its candidate list's slot 0 (e0 - exclusive count)
     unsigned long long key[64];   // per lane: nearest hit (t bits << 32 | simplex), or any hit
     int npend[64], ndefer[64];    // per lane: pending candidates, deferred decisions
     int pend[64 * PENDING];
@@ -618,26 +625,39 @@ __device__ __forceinline__ void coop_test(const TreeArgs& A, CoopWave& L, W& w, 
     const int np0 = w.npend;
     L.incl[lane] = incl;
     if ((lane & 7) == 7) L.incl8[lane >> 3] = incl;
+    L.ebase[lane] = e0 - (incl - c);
     L.key[lane] = ANY || w.best < 0 ? NO_HIT_KEY : hit_key((float)w.best_t, w.best);
     L.npend[lane] = np0;
     L.ndefer[lane] = 0;
 #pragma unroll
     for (int k = 0; k < PENDING; ++k) L.pend[lane * PENDING + k] = w.pend[k];
     wave_lds_sync();
-    for (int base = 0; base < total; base += 64) {
-        const int j = base + lane;
-        const int jj = j < total ? j : total - 1;
-        // owner: the first lane whose inclusive count exceeds jj, i.e. the
-        // number of lanes whose count is at most jj (the counts never fall):
-        // whole groups of 8 lanes first, then the lanes of the next group
+    // owner of slot jj: the first lane whose inclusive count exceeds jj, i.e.
+    // the number of lanes whose count is at most jj (the counts never fall):
+    // whole groups of 8 lanes first, then the lanes of the next group
+    auto owner_of = [&](int jj) -> int {
         const int4 g0 = *reinterpret_cast<const int4*>(&L.incl8[0]);
         const int4 g1 = *reinterpret_cast<const int4*>(&L.incl8[4]);
         const int g = (g0.x <= jj) + (g0.y <= jj) + (g0.z <= jj) + (g0.w <= jj) + (g1.x <= jj) + (g1.y <= jj) +
                       (g1.z <= jj) + (g1.w <= jj);
         const int4 l0 = *reinterpret_cast<const int4*>(&L.incl[8 * g]);
         const int4 l1 = *reinterpret_cast<const int4*>(&L.incl[8 * g + 4]);
-        const int owner = 8 * g + (l0.x <= jj) + (l0.y <= jj) + (l0.z <= jj) + (l0.w <= jj) + (l1.x <= jj) +
-                          (l1.y <= jj) + (l1.z <= jj) + (l1.w <= jj);
+        return 8 * g + (l0.x <= jj) + (l0.y <= jj) + (l0.z <= jj) + (l0.w <= jj) + (l1.x <= jj) + (l1.y <= jj) +
+               (l1.z <= jj) + (l1.w <= jj);
+    };
+    // each round's owners and entry ids are found during the round before, so
+    // the entry load of the next round travels with this round's record load
+    int owner_n = owner_of(lane < total ? lane : total - 1);
+    int sid_n = A.entries[L.ebase[owner_n] + (lane < total ? lane : total - 1)];
+    for (int base = 0; base < total; base += 64) {
+        const int j = base + lane;
+        const int owner = owner_n;
+        const int sid = sid_n;
+        if (base + 64 < total) {
+            const int jn = base + 64 + lane < total ? base + 64 + lane : total - 1;
+            owner_n = owner_of(jn);
+            sid_n = A.entries[L.ebase[owner_n] + jn];
+        }
         float oo[N], od[N];
 #pragma unroll
         for (int k = 0; k < N; ++k) {
@@ -645,11 +665,8 @@ __device__ __forceinline__ void coop_test(const TreeArgs& A, CoopWave& L, W& w, 
             od[k] = __shfl((float)w.d[k], owner);
         }
         const float tlo = __shfl(tl, owner);
-        const int e0o = __shfl(e0, owner);
-        const int exo = __shfl(incl - c, owner);
         const int npo = __shfl(np0, owner);
         if (j < total) {
-            const int sid = A.entries[e0o + (j - exo)];
             float rec[RS];
             load_rec32_now<N>(A, sid, rec);
             auto comp = [&](int cc) -> float { return rec[cc]; };
