@@ -80,6 +80,7 @@ TreeArgs make_tree_args(const MeshDev& m)
     a.rec64 = m.rec64.as<double>();
     a.lmax = t.levels - 1;
     if (t.levels > stack_levels(m.n)) raise(NSP_E_INVALID, "tree deeper than the walk's node stack");
+    if ((t.levels - 1) * m.n > 32) raise(NSP_E_INVALID, "tree cell coordinates exceed the walk's packed word");
     double ext = 0;
     for (int i = 0; i < m.n; ++i) {
         a.lo[i] = (float)t.root_lo[i];
@@ -517,11 +518,12 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_closest(TreeArgs A, RayQueue<T>
     }
 }
 
-// Closest hit of bounce rays with the persistent loop of the shadow stage:
-// these rays are incoherent, so refilling finished lanes keeps waves full.
-// Candidates within the FP64 re-test band are decided afterwards by
-// k_closest_resolve; walks with more than PENDING of them are walked again by
-// k_closest_exact.
+// Closest hit of bounce rays with the persistent loop of the shadow stage
+// (FP64 walks and FP32 walks over the leaf-ordered SoA records; FP32 walks over
+// simplex-ordered records run in k_walk): these rays are incoherent, so
+// refilling finished lanes keeps waves full.  Candidates within the FP64
+// re-test band are decided afterwards by k_closest_resolve; walks with more
+// than PENDING of them are walked again by k_closest_exact.
 template <int N, typename T, int LAY>
 __global__ void NSP_POOL_BOUNDS k_closest_pool(TreeArgs A, RayQueue<T> q, int64_t cap,
                                                               unsigned long long* __restrict__ cnt,
@@ -532,8 +534,6 @@ __global__ void NSP_POOL_BOUNDS k_closest_pool(TreeArgs A, RayQueue<T> q, int64_
 {
     __shared__ int2 stack[stack_levels(N) * TRACE_BLOCK];
     int2* st = stack + threadIdx.x;
-    constexpr bool COOP = LAY == 1 && sizeof(T) == 4;
-    __shared__ CoopWave coop[COOP ? TRACE_BLOCK / 64 : 1];
     RayPool<refill_lanes<N>()> pool(cnt + cnt_cwork(0), cnt + cnt_rayseg(0), (int64_t)cnt[CNT_SEGCAP]);
     RayWalk<N, T, false, LAY, true> w;
     int64_t ray = -1;
@@ -554,18 +554,7 @@ __global__ void NSP_POOL_BOUNDS k_closest_pool(TreeArgs A, RayQueue<T> q, int64_
                 }
             }
         }
-        bool ended;
-        if constexpr (COOP) {
-            // locate every walk's next leaf, test all the wave's candidates
-            // together, then advance
-            int e0 = 0, n = 0;
-            ended = ray >= 0 && w.locate(A, st, &e0, &n);
-            const bool live = ray >= 0 && !ended;
-            coop_test<N, COOP_NEAREST>(A, coop[threadIdx.x >> 6], w, live, e0, n);
-            if (live) ended = w.deferred || w.advance(A);
-        } else {
-            ended = ray >= 0 && w.step(A, st, nullptr, nullptr);
-        }
+        const bool ended = ray >= 0 && w.step(A, st, nullptr, nullptr);
         if (ended) {
             if (w.deferred) {
                 redo[atomicAdd(redo_count, 1ull)] = (int32_t)ray;
@@ -641,8 +630,6 @@ __global__ void NSP_POOL_BOUNDS k_shadow(TreeArgs A, ShadowQueue<T> sq, int64_t 
 {
     __shared__ int2 stack[stack_levels(N) * TRACE_BLOCK];
     int2* st = stack + threadIdx.x;
-    constexpr bool COOP = LAY == 1 && sizeof(T) == 4;
-    __shared__ CoopWave coop[COOP ? TRACE_BLOCK / 64 : 1];
     RayPool<refill_lanes<N>()> pool(cnt + cnt_swork(0), cnt + cnt_shadowseg(0), (int64_t)cnt[CNT_SSEGCAP]);
     RayWalk<N, T, true, LAY, true> w;
     int64_t ray = -1;
@@ -661,17 +648,7 @@ __global__ void NSP_POOL_BOUNDS k_shadow(TreeArgs A, ShadowQueue<T> sq, int64_t 
                     shadow_unoccluded<N, T>(sq, scap, i, L, cap, direct);
             }
         }
-        bool ended;
-        if constexpr (COOP) {
-            int e0 = 0, n = 0;
-            ended = ray >= 0 && w.locate(A, st, &e0, &n);
-            const bool live = ray >= 0 && !ended;
-            coop_test<N, COOP_ANY>(A, coop[threadIdx.x >> 6], w, live, e0, n);
-            // a hit ends an any-hit walk (coop_test then leaves deferred unset)
-            if (live) ended = w.best >= 0 || w.deferred || w.advance(A);
-        } else {
-            ended = ray >= 0 && w.step(A, st, nullptr, nullptr);
-        }
+        const bool ended = ray >= 0 && w.step(A, st, nullptr, nullptr);
         if (ended) {
             if (w.deferred) {
                 redo[atomicAdd(redo_count, 1ull)] = (int32_t)ray;
@@ -806,9 +783,21 @@ struct WalkPool {
 // either kind; the wave tests its lanes' leaf candidates together
 // (coop_test with a per-lane mode).  Finished walks end as k_closest_pool's
 // and k_shadow's do: hits written, FP64 decisions and re-walks listed, an
-// unoccluded shadow ray's contribution added to Ls.
+// unoccluded shadow ray's contribution added to Ls.  COUNT: the traversal
+// counters are kept (a separate instance, so that the timed kernel carries
+// no counter registers).
+// Waves per SIMD asked of the compiler for k_walk: 5 for n <= 4 (C1: 2665
+// Mrays/s at 5 waves with 96 VGPRs and 2 spilled, 2516 at the 4 waves its 105
+// VGPRs allow; 5 blocks of 30.8 KB of LDS fill a CU); 4 for n = 5 and 3 for
+// n = 6, where 5 waves would spill 33 and 55 VGPRs (and 4 waves 6 at n = 6).
 template <int N>
-__global__ void NSP_POOL_BOUNDS k_walk(TreeArgs A, unsigned long long* __restrict__ shadow_counters,
+constexpr int walk_min_waves()
+{
+    return N <= 4 ? 5 : N == 5 ? 4 : 3;
+}
+
+template <int N, bool COUNT>
+__global__ void __launch_bounds__(TRACE_BLOCK, walk_min_waves<N>()) k_walk(TreeArgs A, unsigned long long* __restrict__ shadow_counters,
                                        RayQueue<float> q, int64_t cap, ShadowQueue<float> sq, int64_t scap,
                                        unsigned long long* __restrict__ cc, unsigned long long* __restrict__ sc,
                                        float* __restrict__ Ls, float* __restrict__ direct,
@@ -818,9 +807,11 @@ __global__ void NSP_POOL_BOUNDS k_walk(TreeArgs A, unsigned long long* __restric
     __shared__ int2 stack[stack_levels(N) * TRACE_BLOCK];
     int2* st = stack + threadIdx.x;
     __shared__ CoopWave coop[TRACE_BLOCK / 64];
+    CoopWave& L = coop[threadIdx.x >> 6];
+    const int lane = (int)(threadIdx.x & 63);
     WalkPool<refill_lanes<N>()> pool(cc, sc);
     RayWalk<N, float, false, 1, true> w;
-    int64_t ray = -1;
+    int32_t ray = -1;  // ray indices fit 32 bits (at most 2^27 paths times 8 lights)
     bool any = false;  // the lane's walk is a shadow (any-hit) walk
     for (;;) {
         if (!pool.drained) {
@@ -844,8 +835,9 @@ __global__ void NSP_POOL_BOUNDS k_walk(TreeArgs A, unsigned long long* __restric
                     tmax = INFINITY;
                 }
                 any = shadow;
+                L.npend[lane] = 0;  // the walk's pending list (coop_test keeps it in L)
                 if (w.start(A, tmax)) {
-                    ray = i;
+                    ray = (int32_t)i;
                 } else if (shadow) {
                     shadow_unoccluded<N, float>(sq, scap, i, Ls, cap, direct);
                 } else {
@@ -857,7 +849,7 @@ __global__ void NSP_POOL_BOUNDS k_walk(TreeArgs A, unsigned long long* __restric
         int e0 = 0, n = 0;
         bool ended = ray >= 0 && w.locate(A, st, &e0, &n);
         const bool live = ray >= 0 && !ended;
-        coop_test<N, COOP_PER_LANE>(A, coop[threadIdx.x >> 6], w, live, e0, n, any);
+        coop_test<N>(A, L, w, live, e0, n, any);
         // a hit ends an any-hit walk (coop_test then leaves deferred unset)
         if (live) ended = (any && w.best >= 0) || w.deferred || w.advance(A);
         if (ended) {
@@ -865,12 +857,13 @@ __global__ void NSP_POOL_BOUNDS k_walk(TreeArgs A, unsigned long long* __restric
                 if (w.deferred) {
                     redo_s[atomicAdd(sc + CNT_REDO_S, 1ull)] = (int32_t)ray;
                 } else if (w.best < 0) {
-                    if (w.npend > 0)
-                        push_pending(sc + CNT_RES_S, res_s, ray, w.pend, w.npend);
+                    const int np = L.npend[lane];
+                    if (np > 0)
+                        push_pending(sc + CNT_RES_S, res_s, ray, &L.pend[lane * PENDING], np);
                     else
                         shadow_unoccluded<N, float>(sq, scap, ray, Ls, cap, direct);
                 }
-                w.flush_to(shadow_counters);
+                if constexpr (COUNT) w.flush_to(shadow_counters);
             } else {
                 if (w.deferred) {
                     redo_c[atomicAdd(cc + CNT_REDO_C, 1ull)] = (int32_t)ray;
@@ -878,9 +871,10 @@ __global__ void NSP_POOL_BOUNDS k_walk(TreeArgs A, unsigned long long* __restric
                     const HitResult h = w.result();
                     q.hit[ray] = h.idx;
                     q.hit_t[ray] = (float)h.t;
-                    if (w.npend > 0) push_pending(cc + CNT_RES_C, res_c, ray, w.pend, w.npend);
+                    const int np = L.npend[lane];
+                    if (np > 0) push_pending(cc + CNT_RES_C, res_c, ray, &L.pend[lane * PENDING], np);
                 }
-                w.flush_counters(A);
+                if constexpr (COUNT) w.flush_counters(A);
             }
             ray = -1;
         }
@@ -1124,8 +1118,9 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
     // FP32 walks read the record layout chosen at compile time; FP64 walks read the FP64 records
     const bool aos = sizeof(T) == 4 && A.layout == 1;
     auto kc = aos ? k_closest<N, T, 1> : k_closest<N, T, 0>;
-    auto kcp = aos ? k_closest_pool<N, T, 1> : k_closest_pool<N, T, 0>;
-    const unsigned gcp = aos ? persistent_grid(k_closest_pool<N, T, 1>) : persistent_grid(k_closest_pool<N, T, 0>);
+    // (the persistent two-kernel walks serve FP64 walks and the leaf-ordered SoA records)
+    auto kcp = k_closest_pool<N, T, 0>;
+    const unsigned gcp = persistent_grid(k_closest_pool<N, T, 0>);
 
     // shade variants: several lights (with shapes compiled in), shapes, or the plain mesh scene
     auto kshade = sc.light_count > 1
@@ -1133,8 +1128,8 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
                   : sc.shape_count > 0
                       ? (sc.smooth ? k_shade<N, T, true, true, false> : k_shade<N, T, false, true, false>)
                       : (sc.smooth ? k_shade<N, T, true, false, false> : k_shade<N, T, false, false, false>);
-    auto ks = aos ? k_shadow<N, T, 1> : k_shadow<N, T, 0>;
-    const unsigned gs_full = aos ? persistent_grid(k_shadow<N, T, 1>) : persistent_grid(k_shadow<N, T, 0>);
+    auto ks = k_shadow<N, T, 0>;
+    const unsigned gs_full = persistent_grid(k_shadow<N, T, 0>);
     // beside the closest-hit walks, the persistent shadow walk leaves part of the
     // machine to the launch stream's kernels; the last depth's shadow stage has
     // no closest-hit walk beside it and takes the whole grid
@@ -1146,7 +1141,7 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
     // kernels of each depth run as before, the shadow stage on the side stream
     const bool merged = sizeof(T) == 4 && aos;
     unsigned gw = 0;
-    if constexpr (sizeof(T) == 4) gw = persistent_grid(k_walk<N>);
+    if constexpr (sizeof(T) == 4) gw = rl.counters ? persistent_grid(k_walk<N, true>) : persistent_grid(k_walk<N, false>);
     int32_t* redo_s = b.redo.as<int32_t>();
     int32_t* res_s = b.res.as<int32_t>();
     int32_t* redo_c = b.redo_c.as<int32_t>();
@@ -1214,8 +1209,12 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
                 unsigned long long* dc1 = cnt + (depth + 1) * CNT_STRIDE;
                 float* direct = (aov && depth == 0) ? aov + (int64_t)(3 + N) * cap : nullptr;
                 timed(6, [&] {
-                    hipLaunchKernelGGL((k_walk<N>), dim3(gw), dim3(TRACE_BLOCK), 0, s, A, As.counters, qout, cap, sq,
-                                       scap, dc1, dc, Ls, direct, redo_c, res_c, redo_s, res_s);
+                    if (rl.counters)
+                        hipLaunchKernelGGL((k_walk<N, true>), dim3(gw), dim3(TRACE_BLOCK), 0, s, A, As.counters, qout,
+                                           cap, sq, scap, dc1, dc, Ls, direct, redo_c, res_c, redo_s, res_s);
+                    else
+                        hipLaunchKernelGGL((k_walk<N, false>), dim3(gw), dim3(TRACE_BLOCK), 0, s, A, As.counters, qout,
+                                           cap, sq, scap, dc1, dc, Ls, direct, redo_c, res_c, redo_s, res_s);
                     NSP_HIP(hipGetLastError());
                     hipLaunchKernelGGL((k_closest_resolve<N>), dim3(gx), dim3(256), 0, s, A, qout, cap,
                                        dc1 + CNT_RES_C, res_c);

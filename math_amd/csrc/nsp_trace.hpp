@@ -18,6 +18,8 @@
 // src/painter/space/ray_intersection.h:141-150).
 #pragma once
 
+#include <type_traits>
+
 #include "nsp_device.hpp"
 #include "nsp_internal.hpp"
 
@@ -326,12 +328,16 @@ constexpr int PENDING = NSP_PENDING;
 
 template <int N, typename T, bool ANY, int LAY = -1, bool DEFER = false>
 struct RayWalk {
+    // FP32 walks with deferred FP64 decisions find FP32 distances only
+    using BestT = typename std::conditional<DEFER && sizeof(T) == 4, float, double>::type;
     T o[N], d[N], inv[N];
     T t_cur, t_end, adv, tmax;
     T t_next;  // exit of the located leaf box
-    double best_t;
+    BestT best_t;
     int best, level, last_leaf;
-    int q[N];
+    // finest-level cell coordinates of the last located point, lmax bits each
+    // (n * lmax <= 28 for every supported dimension)
+    unsigned qp;
     int pend[PENDING];
     int npend;
     unsigned steps;
@@ -342,7 +348,7 @@ struct RayWalk {
     __device__ __forceinline__ bool start(const TreeArgs& A, T tmax_)
     {
         tmax = tmax_;
-        best_t = (double)tmax;
+        best_t = (BestT)tmax;
         best = -1;
         level = -1;
         last_leaf = -1;
@@ -397,8 +403,10 @@ struct RayWalk {
             stack[0] = nd;
         } else {
             unsigned diff = 0;
+            const unsigned cmask = (1u << lmax) - 1u;
 #pragma unroll
-            for (int i = 0; i < N; ++i) diff |= (unsigned)((Q[i] >> (lmax - level)) ^ q[i]);
+            for (int i = 0; i < N; ++i)
+                diff |= ((unsigned)Q[i] ^ ((qp >> (i * lmax)) & cmask)) >> (lmax - level);
             int pops = diff ? 32 - __clz(diff) : 0;
             level -= pops;
             nd = stack[level * TRACE_BLOCK];
@@ -414,15 +422,16 @@ struct RayWalk {
             ++nvis;
             stack[level * TRACE_BLOCK] = nd;
         }
+        qp = 0;
 #pragma unroll
-        for (int i = 0; i < N; ++i) q[i] = Q[i] >> (lmax - level);
+        for (int i = 0; i < N; ++i) qp |= (unsigned)Q[i] << (i * lmax);
         // leaf box interval
         T blo[N], bsz[N];
 #pragma unroll
         for (int i = 0; i < N; ++i) {
             T cell = ldexp(rsz[i], -level);
             bsz[i] = cell;
-            blo[i] = rlo[i] + (T)q[i] * cell;
+            blo[i] = rlo[i] + (T)(Q[i] >> (lmax - level)) * cell;
         }
         T ta, tb;
         if (!slab<N, T>(blo, bsz, o, inv, d, &ta, &tb)) tb = tp;  // rounding: point located outside its own box
@@ -483,12 +492,12 @@ struct RayWalk {
                 }
                 if (c == CAND_HIT) {
                     if (ANY) {
-                        best_t = th;
+                        best_t = (BestT)th;
                         best = si;
                         return true;
                     }
                     if (th < best_t || (th == best_t && si < best)) {
-                        best_t = th;
+                        best_t = (BestT)th;
                         best = si;
                     }
                 }
@@ -561,14 +570,15 @@ struct RayWalk {
     }
 };
 
-// LDS of one wave for coop_test.
+// LDS of one wave for coop_test, and the pending lists of its lanes' walks.
 struct alignas(16) CoopWave {
     int incl[64];                 // inclusive count of candidates up to each lane
     int incl8[8];                 // incl of lanes 7, 15, ..., 63 (the owner search's first level)
     int ebase[64];                // per lane: the entry of its candidate list's slot 0 (e0 - exclusive count)
     unsigned long long key[64];   // per lane: nearest hit (t bits << 32 | simplex), or any hit
-    int npend[64], ndefer[64];    // per lane: pending candidates, deferred decisions
-    int pend[64 * PENDING];
+    int npend[64];                // per lane: its walk's pending candidates (more than PENDING: overflowed)
+    int ndefer[64];               // per lane: deferred decisions of the current leaf
+    int pend[64 * PENDING];       // per lane: its walk's pending candidates (the walk's list lives here)
 };
 
 // Sort key of a hit: t > 0, so its float bits order as unsigned integers; ties
@@ -587,30 +597,29 @@ __device__ __forceinline__ void wave_lds_sync()
 
 // Wave-cooperative FP32 test of the leaf candidates of every lane whose walk
 // located a leaf (active, cnt candidates from entry e0; the FP32 AoS records,
-// deferred FP64 decisions).  Called by all 64 lanes together.  The lanes'
+// deferred FP64 decisions).  Called by all 64 lanes together; each lane's
+// any_hit says which kind its walk is (a wave may hold both kinds).  The lanes'
 // candidate lists are concatenated (a prefix sum over the wave) and dealt out
 // 64 at a time: lane j of a round tests candidate base + j for its owner, the
-// lane whose list holds it (binary search over the inclusive counts), with
-// the owner's ray and limit read across lanes.  A walk with one candidate and
-// a walk with forty then cost the wave the same rounds, where testing each
-// lane's own list keeps the wave until its longest list is done.
-//   nearest hit (ANY false): the owner's key is lowered with an LDS atomic
-//     min from its hit so far, which keeps the nearest hit and, at equal t,
-//     the smaller index.  Every candidate is tested against the limit the
-//     leaf started with; a hit can only lower it, so the minimum is the one
-//     the sequential walk finds.
-//   any hit (ANY true): any hit before tmax ends the owner's walk.
-//   deferred candidates join the owner's pending list (no duplicates); more
-//     than PENDING set deferred, for a re-walk.
-// MODE: COOP_NEAREST or COOP_ANY for every lane, or COOP_PER_LANE, where each
-// lane's any_hit says which kind its walk is (a wave may hold both kinds).
-constexpr int COOP_NEAREST = 0, COOP_ANY = 1, COOP_PER_LANE = 2;
-template <int N, int MODE, class W>
+// lane whose list holds it, with the owner's ray and limit read across lanes.
+// A walk with one candidate and a walk with forty then cost the wave the same
+// rounds, where testing each lane's own list keeps the wave until its longest
+// list is done.
+//   nearest hit: the owner's key is lowered with an LDS atomic min from its
+//     hit so far, which keeps the nearest hit and, at equal t, the smaller
+//     index.  Every candidate is tested against the limit the leaf started
+//     with; a hit can only lower it, so the minimum is the one the sequential
+//     walk finds.
+//   any hit: any hit before tmax ends the owner's walk.
+//   deferred candidates join the owner's pending list in L (no duplicates);
+//     more than PENDING set deferred, for a re-walk.  The list stays in LDS for
+//     the whole walk: the caller empties it (L.npend[lane] = 0) when the lane
+//     starts a walk and reads it when the walk ends.
+template <int N, class W>
 __device__ __forceinline__ void coop_test(const TreeArgs& A, CoopWave& L, W& w, bool active, int e0, int cnt,
-                                          bool any_hit = false)
+                                          bool any)
 {
     constexpr int RS = rec32_stride(N);
-    const bool ANY = MODE == COOP_PER_LANE ? any_hit : MODE == COOP_ANY;
     const int lane = threadIdx.x & 63;
     const int c = active ? cnt : 0;
     int incl = c;
@@ -621,16 +630,13 @@ __device__ __forceinline__ void coop_test(const TreeArgs& A, CoopWave& L, W& w, 
     }
     const int total = __shfl(incl, 63);
     if (total == 0) return;
-    const float tl = ANY ? (float)w.tmax : (float)w.best_t;
-    const int np0 = w.npend;
+    const float tl = any ? (float)w.tmax : (float)w.best_t;
+    const int np0 = L.npend[lane];  // at most PENDING: a walk whose list overflowed has ended
     L.incl[lane] = incl;
     if ((lane & 7) == 7) L.incl8[lane >> 3] = incl;
     L.ebase[lane] = e0 - (incl - c);
-    L.key[lane] = ANY || w.best < 0 ? NO_HIT_KEY : hit_key((float)w.best_t, w.best);
-    L.npend[lane] = np0;
+    L.key[lane] = any || w.best < 0 ? NO_HIT_KEY : hit_key((float)w.best_t, w.best);
     L.ndefer[lane] = 0;
-#pragma unroll
-    for (int k = 0; k < PENDING; ++k) L.pend[lane * PENDING + k] = w.pend[k];
     wave_lds_sync();
     // owner of slot jj: the first lane whose inclusive count exceeds jj, i.e.
     // the number of lanes whose count is at most jj (the counts never fall):
@@ -679,8 +685,7 @@ __device__ __forceinline__ void coop_test(const TreeArgs& A, CoopWave& L, W& w, 
                                              &dummy);
             if (r == CAND_HIT) {
                 // (an any-hit owner only asks whether its key fell below NO_HIT_KEY)
-                atomicMin(&L.key[owner],
-                          MODE == COOP_ANY ? (unsigned long long)(unsigned)si : hit_key((float)th, si));
+                atomicMin(&L.key[owner], hit_key((float)th, si));
             } else if (r == CAND_DEFER) {
                 atomicAdd(&L.ndefer[owner], 1);
                 bool known = false;
@@ -697,30 +702,22 @@ __device__ __forceinline__ void coop_test(const TreeArgs& A, CoopWave& L, W& w, 
     if (c > 0) {
         w.ntest += c;
         const unsigned long long k = L.key[lane];
-        if (ANY && k != NO_HIT_KEY) {
+        if (any && k != NO_HIT_KEY) {
             // the walk ends occluded; its pending list no longer matters
             w.best = (int)(unsigned)(k & 0xffffffffull);
-            w.best_t = (double)w.tmax;
+            w.best_t = w.tmax;
             return;
         }
-        if (!ANY && k != NO_HIT_KEY) {
+        if (!any && k != NO_HIT_KEY) {
             w.best = (int)(unsigned)(k & 0xffffffffull);
-            w.best_t = (double)__uint_as_float((unsigned)(k >> 32));
+            w.best_t = __uint_as_float((unsigned)(k >> 32));
         }
         const int ndf = L.ndefer[lane];
         if (ndf > 0) {
             w.nret += ndf;
             w.near = true;
         }
-        const int np = L.npend[lane];
-        if (np > PENDING) {
-            w.deferred = true;
-        } else {
-#pragma unroll
-            for (int k = 0; k < PENDING; ++k)
-                if (k < np) w.pend[k] = L.pend[lane * PENDING + k];
-            w.npend = np;
-        }
+        if (L.npend[lane] > PENDING) w.deferred = true;
     }
 }
 
