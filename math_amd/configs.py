@@ -209,6 +209,49 @@ def hull_points(n, count, seed, on_sphere=False, zero=False):
     return p
 
 
+def sphere_with_notch(n, count, bound=False, seed=None):
+    """Points of the reconstruction tests' "sphere with a notch"
+    (generate_points_sphere_with_notch, geometry/objects/points.cpp:212-236),
+    drawn with numpy's generator (seeded by the count unless `seed` is given):
+    uniform on the unit (n-1)-sphere, points with a positive last coordinate z
+    scaled along the last axis by 1 - |z^5| / 2.  With `bound`, only the part
+    with z >= -0.3 (points.cpp:152-164, COS_FOR_BOUND).  A point is kept only
+    if its coordinates rounded at 1e-5 differ from every earlier point's
+    (DiscretePoints, points.cpp:70-128)."""
+    rng = np.random.default_rng(count if seed is None else seed)
+    out, seen = [], set()
+    while len(out) < count:
+        v = rng.uniform(-1, 1, (2 * (count - len(out)) + 64, n))
+        sq = np.sum(v * v, axis=1)
+        v = v[(sq <= 1) & (sq > 0)]
+        v = v / np.linalg.norm(v, axis=1, keepdims=True)
+        if bound:
+            v = v[v[:, n - 1] >= -0.3]
+        z = v[:, n - 1]
+        v[:, n - 1] = np.where(z > 0, z * (1 - np.abs(0.5 * z ** 5)), z)
+        for p in v:
+            key = tuple(np.rint(p * 100000).astype(np.int64))
+            if key in seen:
+                continue
+            seen.add(key)
+            out.append(p)
+            if len(out) == count:
+                break
+    return np.array(out).astype(np.float32)
+
+
+def clone_points(points, shift=3.0):
+    """The points and 2^n shifted copies (clone_object,
+    geometry/test/test_reconstruction.cpp:118-146): copy k is shifted by +shift
+    along axis i where bit i of k is set and by -shift elsewhere."""
+    n = points.shape[1]
+    parts = [points]
+    for k in range(1 << n):
+        s = np.array([shift if (k >> i) & 1 else -shift for i in range(n)], np.float32)
+        parts.append(points + s)
+    return np.concatenate(parts).astype(np.float32)
+
+
 def sphere_hull_mesh(ctx, n, count, seed=0):
     """simplex_mesh_of_sphere (painter/shapes/test/sphere_mesh.cpp:65-105): the
     convex hull of `count` random points on the unit (n-1)-sphere, as

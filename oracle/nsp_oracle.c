@@ -29,6 +29,8 @@ static int fail(const char* fmt, ...)
     return 1;
 }
 
+void orc_set_error(const char* msg) { fail("%s", msg); }
+
 /* ------------------------------------------------------------------------ */
 /* Philox4x32-10.  Constants from Salmon et al. (SC'11), Random123.           */
 

@@ -186,6 +186,20 @@ void orc_trace_path_sample(const orc_mesh* m, const orc_scene* scene, int64_t pi
  * the given uniforms (u[0] radius^2, u[1..] tangent direction). */
 void orc_cosine_hemisphere(int n, const double* normal, const double* u, double* dir_out);
 
+/* Cocone (mode 0) or BoundCocone(rho, alpha) (mode 1) surface reconstruction
+ * (src/geometry/cocone/reconstruction.cpp) of a Delaunay triangulation:
+ * points [count][n] (doubles), simplices [S][n+1], orthos [S][n+1][n] (ortho r:
+ * the unit normal of the face without vertex r, pointing away from it).
+ * Returns the number of facets, or -1 (orc_last_error).  facets_out (when not
+ * NULL) receives up to facets_cap sorted vertex tuples [*][n] in facet order;
+ * normals_out [count][n] the positive-pole normal of each used point (zero
+ * otherwise).  n = 2..5. */
+int64_t orc_cocone(int n, const double* points, int64_t count, const int32_t* simplices, const double* orthos,
+                   int64_t S, int mode, double rho, double alpha, int32_t* facets_out, int64_t facets_cap,
+                   double* normals_out);
+/* compute_voronoi_vertex: the centre of the sphere through a simplex's n + 1 points. */
+void orc_voronoi_vertex(int n, const double* points, const int32_t* simplex, double* out);
+
 #ifdef __cplusplus
 }
 #endif

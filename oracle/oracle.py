@@ -92,6 +92,11 @@ def lib():
         L.orc_trace_path_sample.restype = None
         L.orc_cosine_hemisphere.argtypes = [C.c_int, dp, dp, dp]
         L.orc_cosine_hemisphere.restype = None
+        L.orc_cocone.argtypes = [C.c_int, dp, C.c_int64, ip, dp, C.c_int64, C.c_int, C.c_double, C.c_double, ip,
+                                 C.c_int64, dp]
+        L.orc_cocone.restype = C.c_int64
+        L.orc_voronoi_vertex.argtypes = [C.c_int, dp, ip, dp]
+        L.orc_voronoi_vertex.restype = None
         _lib = L
     return _lib
 
@@ -361,3 +366,32 @@ def shape_intersect(kind, org, edges, o, d):
                                  _p(d_, C.c_double), C.byref(t), _p(nrm, C.c_double)):
         return None
     return t.value, nrm
+
+
+def cocone(points, simplices, orthos, bound=False, rho=0.3, alpha=0.14):
+    """Cocone (or BoundCocone(rho, alpha)) of a Delaunay triangulation: points
+    [count, n] (the coordinates the triangulation was computed on), simplices
+    [S, n+1], orthos [S, n+1, n].  Returns (facets [F, n] sorted vertex tuples in
+    facet order, normals [count, n]); raises ValueError with the checker's
+    message when the reconstruction fails."""
+    pts = np.ascontiguousarray(points, np.float64)
+    sim = np.ascontiguousarray(simplices, np.int32)
+    ort = np.ascontiguousarray(orthos, np.float64)
+    count, n = pts.shape
+    S = sim.shape[0]
+    cap = max(1, 4 * S * (n + 1))
+    facets = np.zeros((cap, n), np.int32)
+    normals = np.zeros((count, n))
+    k = lib().orc_cocone(n, _p(pts, C.c_double), count, _p(sim, C.c_int32), _p(ort, C.c_double), S, int(bound),
+                         rho, alpha, _p(facets, C.c_int32), cap, _p(normals, C.c_double))
+    if k < 0:
+        raise ValueError(lib().orc_last_error().decode())
+    return facets[:k].copy(), normals
+
+
+def voronoi_vertex(points, simplex):
+    pts = np.ascontiguousarray(points, np.float64)
+    sim = np.ascontiguousarray(simplex, np.int32)
+    out = np.zeros(pts.shape[1])
+    lib().orc_voronoi_vertex(pts.shape[1], _p(pts, C.c_double), _p(sim, C.c_int32), _p(out, C.c_double))
+    return out
