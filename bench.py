@@ -315,6 +315,66 @@ def hull_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def cocone_bench(args):
+    """Cocone and BoundCocone (rho 0.3, alpha 0.14) of the reconstruction
+    test's notched spheres at its largest sizes: 3000 points in 3 dimensions,
+    the same with its 8 clones (27,000 points), 20,000 points in 4 dimensions.
+    Per workload: the constructor (exact Delaunay, facets, per-point GPU pass),
+    then each reconstruction; beside it the oracle (C, one core) on the same
+    triangulation, i.e. every step after the Delaunay."""
+    from math_amd import configs, native
+    from oracle import oracle as O
+
+    ctx = native.Context(0)
+    native.ManifoldConstructor(ctx, configs.sphere_with_notch(3, 200)).cocone()  # warm-up (kernel load)
+    out = {"metric": "Cocone and BoundCocone reconstruction", "unit": "ms", "workloads": []}
+    oracle_ms = lib_ms = 0.0
+    for name, n, count, clones in (("notched 2-sphere", 3, 3000, False), ("notched 2-sphere, 9 objects", 3, 3000, True),
+                                   ("notched 3-sphere", 4, 20000, False)):
+        p = configs.sphere_with_notch(n, count)
+        if clones:
+            p = configs.clone_points(p)
+        t0 = time.perf_counter()
+        m = native.ManifoldConstructor(ctx, p)
+        t_create = time.perf_counter() - t0
+        create_ms = m.stats()
+        t0 = time.perf_counter()
+        _, fc = m.cocone()
+        t_cocone = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        _, fb = m.bound_cocone(0.3, 0.14)
+        t_bound = time.perf_counter() - t0
+        bound_ms = m.stats()
+        w = {"workload": f"{name}, {len(p)} points", "delaunay_objects": m.objects, "cocone_facets": len(fc),
+             "bound_cocone_facets": len(fb), "constructor_ms": t_create * 1e3, "cocone_ms": t_cocone * 1e3,
+             "bound_cocone_ms": t_bound * 1e3,
+             "constructor_split_ms": {"delaunay": create_ms[0], "facets_host": create_ms[1], "gpu": create_ms[2]},
+             "bound_cocone_split_ms": {"gpu": bound_ms[3], "prune_host": bound_ms[4], "extract_host": bound_ms[5]}}
+        m.close()
+        if not args.no_cpu:
+            d = native.delaunay(ctx, p)
+            t0 = time.perf_counter()
+            O.cocone(d.points, d.vertices, d.orthos)
+            O.cocone(d.points, d.vertices, d.orthos, bound=True, rho=0.3, alpha=0.14)
+            o_ms = (time.perf_counter() - t0) * 1e3
+            l_ms = (t_create * 1e3 - create_ms[0]) + (t_cocone + t_bound) * 1e3
+            w["after_delaunay_ms"] = l_ms
+            w["oracle_after_delaunay_ms"] = o_ms
+            oracle_ms += o_ms
+            lib_ms += l_ms
+        log(f"cocone {w['workload']}: {m.objects} simplices, Cocone {len(fc)}, BoundCocone {len(fb)}; constructor "
+            f"{t_create * 1e3:.0f} ms (Delaunay {create_ms[0]:.0f}, facets {create_ms[1]:.0f}, GPU {create_ms[2]:.1f}), "
+            f"cocone {t_cocone * 1e3:.1f} ms, bound {t_bound * 1e3:.1f} ms"
+            + (f"; after Delaunay {w['after_delaunay_ms']:.0f} ms vs oracle {w['oracle_after_delaunay_ms']:.0f} ms"
+               if not args.no_cpu else ""))
+        out["workloads"].append(w)
+    if not args.no_cpu:
+        out["cpu_baseline"] = {"value": oracle_ms, "unit": "ms", "cores": 1, "kind": "port",
+                               "sample": "oracle (C) Cocone + BoundCocone on the library's triangulations of the "
+                                         f"three workloads; the library's same steps take {lib_ms:.0f} ms"}
+    print(json.dumps(out), flush=True)
+
+
 def batch_bench(args):
     """C4: many scenes of uneven size and dimension, 4 spp, FP32: the batched
     launch (scenes on concurrent lanes) vs one nsp_paint call per scene."""
@@ -405,6 +465,7 @@ def main():
     ap.add_argument("--obj", action="store_true", help=".obj4 ingest of the C1 mesh (reader MB/s)")
     ap.add_argument("--hull", action="store_true", help="convex hulls of 4-dimensional point sets (ms)")
     ap.add_argument("--hull-sample", type=int, default=3000, help="with --hull: points the oracle takes")
+    ap.add_argument("--cocone", action="store_true", help="Cocone and BoundCocone reconstructions (ms)")
     ap.add_argument("--no-gpu-mesh", action="store_true", help="with --obj: skip the GPU mesh build")
     ap.add_argument("--obj-sample-facets", type=int, default=60000, help="with --obj: facets read by the oracle")
     ap.add_argument("--batch-scenes", action="store_true",
@@ -427,6 +488,8 @@ def main():
         return batch_bench(args)
     if args.hull:
         return hull_bench(args)
+    if args.cocone:
+        return cocone_bench(args)
     if args.obj:
         return obj_bench(args)
     if args.copy_calibration:

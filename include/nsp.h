@@ -155,6 +155,58 @@ int nsp_hull_copy(const nsp_hull* h, int32_t* vertices, double* orthos, double* 
 int nsp_hull_stats(const nsp_hull* h, int64_t* dropped, int64_t* facets_created, double* ms);
 void nsp_hull_destroy(nsp_hull* h);
 
+/* Cocone and BoundCocone surface reconstruction of `count` float points in
+ * n = 2..5 dimensions.  nsp_manifold_create replaces
+ * create_manifold_constructor<N> (cocone_only = 0) and
+ * create_manifold_constructor_cocone<N> (cocone_only = 1)
+ * (src/geometry/cocone/reconstruction.h:50-57, reconstruction.cpp:386-421):
+ * the Delaunay simplices of nsp_delaunay, their Voronoi vertices, and per
+ * point the positive pole and the cocone flags of its Delaunay facets (with
+ * cocone_only = 0 also the height and radius of its Voronoi cell).  Errors
+ * (NSP_E_INVALID): fewer than n + 2 points ("Error point count ..."), the
+ * Delaunay errors, "Voronoi vertex is not finite", and the reference's per-point
+ * errors ("Positive pole vector not finite", "Negative pole vector not
+ * found", "Cocone intersection not found, ...", "Voronoi cell radius is not
+ * positive"); NSP_E_UNSUPPORTED for n outside 2..5.  The messages are in
+ * nsp_last_error(ctx). */
+typedef struct nsp_manifold nsp_manifold;
+int nsp_manifold_create(nsp_context* ctx, int n, const float* points, int64_t count, int cocone_only,
+                        nsp_manifold** out);
+/* ManifoldConstructor::cocone (reconstruction.cpp:321-339): the facets whose
+ * Voronoi edge meets the cocones of all their vertices, pruned at sharp ridges
+ * (prune_facets.h:117-200) and reduced to the manifold reachable from outside
+ * (extract_manifold.h:105-131).  *facets receives the facet count; the facets
+ * and normals are read with nsp_manifold_result.  Errors (NSP_E_INVALID, the
+ * reference's messages): "Cocone facets not found[ after prune| after
+ * manifold extraction]. (n-1)-manifold is not reconstructable." */
+int nsp_manifold_cocone(nsp_context* ctx, nsp_manifold* m, int64_t* facets);
+/* ManifoldConstructor::bound_cocone(rho, alpha) (reconstruction.cpp:341-373):
+ * interior points (thin Voronoi cells whose poles agree with their cocone
+ * neighbours' within alpha, reconstruction.cpp:72-160), the facets whose
+ * interior vertices all have the cocone flag, then pruning and extraction as
+ * above.  Errors (NSP_E_INVALID): rho or alpha outside (0, 1) ("Rho must be in
+ * the interval (0, 1), but rho = ..."), a constructor created with
+ * cocone_only, "Interior vertices not found. ...", "Cocone interior facets not
+ * found. ...", and those of nsp_manifold_cocone. */
+int nsp_manifold_bound_cocone(nsp_context* ctx, nsp_manifold* m, double rho, double alpha, int64_t* facets);
+/* The last reconstruction's result: normals [points][n] (the positive pole of
+ * each facet vertex, zero for other points) and facets [facets][n] (sorted
+ * point indices; facets in the lexicographic order of their tuples).  Either
+ * pointer may be NULL. */
+int nsp_manifold_result(const nsp_manifold* m, double* normals, int32_t* facets);
+/* Point count, Delaunay objects (simplices) and Delaunay facets; any pointer
+ * may be NULL. */
+int nsp_manifold_info(const nsp_manifold* m, int* n, int64_t* points, int64_t* delaunay_objects,
+                      int64_t* delaunay_facets);
+/* ManifoldConstructor::delaunay_objects (reconstruction.cpp:375-384): the
+ * simplices' vertex indices, [delaunay_objects][n + 1]. */
+int nsp_manifold_objects(const nsp_manifold* m, int32_t* objects);
+/* Times in ms (ms[6]): Delaunay, facet grouping on the host, the GPU passes of
+ * the constructor, and of the last reconstruction its GPU passes, pruning, and
+ * extraction with the output. */
+int nsp_manifold_stats(const nsp_manifold* m, double* ms);
+void nsp_manifold_destroy(nsp_manifold* m);
+
 typedef struct {
     int n;
     int64_t facets;

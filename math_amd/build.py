@@ -31,6 +31,7 @@ SOURCES = {
     "nsp_denoise.hip": [],
     "nsp_raster.hip": [],
     "nsp_hull.hip": [],
+    "nsp_cocone.hip": ["-ffp-contract=off"],  # FP64 reconstruction equal to the CPU checker's
     "nsp_api.cpp": ["-x", "hip"],
     "nsp_meshgen.cpp": ["-x", "hip"],
     "nsp_obj.cpp": ["-ffp-contract=off"],   # float operations as written (file values)

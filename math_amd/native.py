@@ -128,7 +128,8 @@ EXPORTS = [
     "nsp_batch_statistics", "nsp_batch_set_lanes", "nsp_batch_get_lanes", "nsp_batch_autotune", "nsp_autotune", "nsp_mesh_set_record_layout", "nsp_mesh_get_record_layout",
     "nsp_obj_read", "nsp_obj_destroy", "nsp_obj_last_error", "nsp_obj_info", "nsp_obj_copy", "nsp_mesh_create_obj",
     "nsp_obj_write", "nsp_scene_set_shapes", "nsp_convex_hull", "nsp_delaunay", "nsp_hull_info", "nsp_hull_copy",
-    "nsp_hull_stats", "nsp_hull_destroy",
+    "nsp_hull_stats", "nsp_hull_destroy", "nsp_manifold_create", "nsp_manifold_cocone", "nsp_manifold_bound_cocone",
+    "nsp_manifold_result", "nsp_manifold_info", "nsp_manifold_objects", "nsp_manifold_stats", "nsp_manifold_destroy",
 ]
 
 _lib = None
@@ -199,6 +200,15 @@ def lib():
         L.nsp_hull_stats.argtypes = [P, i64p, i64p, dp]
         L.nsp_hull_destroy.argtypes = [P]
         L.nsp_hull_destroy.restype = None
+        L.nsp_manifold_create.argtypes = [P, C.c_int, fp, C.c_int64, C.c_int, C.POINTER(P)]
+        L.nsp_manifold_cocone.argtypes = [P, P, i64p]
+        L.nsp_manifold_bound_cocone.argtypes = [P, P, C.c_double, C.c_double, i64p]
+        L.nsp_manifold_result.argtypes = [P, dp, ip]
+        L.nsp_manifold_info.argtypes = [P, C.POINTER(C.c_int), i64p, i64p, i64p]
+        L.nsp_manifold_objects.argtypes = [P, ip]
+        L.nsp_manifold_stats.argtypes = [P, dp]
+        L.nsp_manifold_destroy.argtypes = [P]
+        L.nsp_manifold_destroy.restype = None
         _lib = L
     return _lib
 
@@ -347,6 +357,70 @@ def convex_hull(ctx, points, first=0):
 def delaunay(ctx, points, first=0):
     """compute_delaunay (geometry/core/convex_hull.h:67-69) on ctx's GPU."""
     return _hull(ctx, points, first, True)
+
+
+# ---------------------------------------------------------------------------
+# Cocone and BoundCocone surface reconstruction
+
+class ManifoldConstructor:
+    """create_manifold_constructor / create_manifold_constructor_cocone
+    (geometry/cocone/reconstruction.h:50-57) on ctx's GPU.  cocone() and
+    bound_cocone(rho, alpha) return (normals [count, n] float64, facets [k, n]
+    int32) as ManifoldConstructor::cocone / bound_cocone fill them;
+    delaunay_objects() the simplices [S, n + 1]."""
+
+    def __init__(self, ctx, points, cocone_only=False):
+        self.ctx = ctx
+        L = ctx._L
+        p = np.ascontiguousarray(points, np.float32)
+        if p.ndim != 2:
+            raise ValueError("points must be [count, n]")
+        h = C.c_void_p()
+        ctx.check(L.nsp_manifold_create(ctx.h, p.shape[1], _ptr(p, C.c_float), p.shape[0], int(cocone_only),
+                                        C.byref(h)))
+        self.h = h
+        n, cnt, S, F = C.c_int(), C.c_int64(), C.c_int64(), C.c_int64()
+        L.nsp_manifold_info(h, C.byref(n), C.byref(cnt), C.byref(S), C.byref(F))
+        self.n, self.count, self.objects, self.delaunay_facets = n.value, cnt.value, S.value, F.value
+
+    def _result(self, k):
+        normals = np.zeros((self.count, self.n), np.float64)
+        facets = np.zeros((k, self.n), np.int32)
+        self.ctx._L.nsp_manifold_result(self.h, _ptr(normals, C.c_double), _ptr(facets, C.c_int32))
+        return normals, facets
+
+    def cocone(self):
+        k = C.c_int64()
+        self.ctx.check(self.ctx._L.nsp_manifold_cocone(self.ctx.h, self.h, C.byref(k)))
+        return self._result(k.value)
+
+    def bound_cocone(self, rho, alpha):
+        k = C.c_int64()
+        self.ctx.check(self.ctx._L.nsp_manifold_bound_cocone(self.ctx.h, self.h, float(rho), float(alpha), C.byref(k)))
+        return self._result(k.value)
+
+    def delaunay_objects(self):
+        out = np.zeros((self.objects, self.n + 1), np.int32)
+        self.ctx._L.nsp_manifold_objects(self.h, _ptr(out, C.c_int32))
+        return out
+
+    def stats(self):
+        """ms: Delaunay, facet grouping, constructor GPU passes, reconstruction
+        GPU passes, pruning, extraction and output."""
+        ms = (C.c_double * 6)()
+        self.ctx._L.nsp_manifold_stats(self.h, ms)
+        return tuple(ms)
+
+    def close(self):
+        if self.h:
+            self.ctx._L.nsp_manifold_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------------------

@@ -185,3 +185,112 @@ def test_oracle_notch_generator():
         assert np.allclose(c[-500:] - p, 3, atol=1e-6)                  # copy 2^n - 1: +3 on every axis
         assert np.allclose((c[1000:1500] - p)[:, 0], 3, atol=1e-6)      # copy 1: +3 on axis 0 only
         assert np.allclose((c[1000:1500] - p)[:, 1:], -3, atol=1e-6)
+
+
+# ---------------------------------------------------------------- library (GPU)
+
+def library_vs_oracle(ctx, pts, bound_surface):
+    """Runs the library's reconstructions and the oracle's on the library's own
+    Delaunay triangulation; returns {algorithm: facets} after asserting that
+    they agree."""
+    from math_amd import native
+    m = native.ManifoldConstructor(ctx, pts)
+    d = native.delaunay(ctx, pts)
+    assert np.array_equal(m.delaunay_objects(), d.vertices)
+    out = {}
+    algos = ["bound"] if bound_surface else ["cocone", "bound"]
+    for algo in algos:
+        if algo == "cocone":
+            nl, fl = m.cocone()
+            fo, no = O.cocone(d.points, d.vertices, d.orthos)
+        else:
+            nl, fl = m.bound_cocone(RHO, ALPHA)
+            fo, no = O.cocone(d.points, d.vertices, d.orthos, bound=True, rho=RHO, alpha=ALPHA)
+        print(f"  {algo}: library {len(fl)} facets, oracle {len(fo)}, ms {m.stats()}", flush=True)
+        assert np.array_equal(fl, fo)
+        assert np.max(np.abs(nl - no)) <= 1e-12
+        out[algo] = fl
+    m.close()
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,P", [(2, 100), (2, 1000), (3, 2000), (3, 3000), (4, 20000)])
+def test_gpu_reference_counts(ctx, n, P):
+    """The reference's test (test_reconstruction.cpp:184-281) on the library:
+    Cocone and BoundCocone of the notched sphere, BoundCocone of the bounded
+    notched sphere, in its facet-count ranges, each result equal to the
+    oracle's on the same triangulation.  The 2^n clones run in 2 and 3
+    dimensions (in 4 they would be 340,000 points)."""
+    lo, hi, blo, bhi = reference_counts(n, P)
+    for bound_surface in (False, True):
+        pts = configs.sphere_with_notch(n, P, bound=bound_surface)
+        for objects in ((1, 1 + (1 << n)) if n < 4 else (1,)):
+            p = pts if objects == 1 else configs.clone_points(pts)
+            print(f"n={n} P={P} bound_surface={bound_surface} objects={objects}", flush=True)
+            got = library_vs_oracle(ctx, p, bound_surface)
+            if "cocone" in got:
+                assert lo * objects <= len(got["cocone"]) <= hi * objects
+                assert all(c == 2 for c in ridge_counts(got["cocone"]).values())
+            assert blo * objects <= len(got["bound"]) <= bhi * objects
+
+
+def outcome(fn):
+    try:
+        return fn()
+    except Exception as e:  # the message, with the dimension suffix cut
+        return str(e).split(": ", 1)[-1].split(". ")[0]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,P", [(3, 300), (4, 3000), (5, 3000)])
+def test_gpu_sparse_samples(ctx, n, P):
+    """Samples too sparse for some of the reconstructions (the reference's test
+    has no 5-space case): the library and the oracle succeed with equal results
+    or fail at the same step."""
+    from math_amd import native
+    print(f"n={n} P={P}", flush=True)
+    pts = configs.sphere_with_notch(n, P)
+    m = native.ManifoldConstructor(ctx, pts)
+    d = native.delaunay(ctx, pts)
+    for bound in (False, True):
+        lib = outcome(lambda: m.bound_cocone(RHO, ALPHA) if bound else m.cocone())
+        ora = outcome(lambda: O.cocone(d.points, d.vertices, d.orthos, bound=bound, rho=RHO, alpha=ALPHA)[::-1])
+        print(f"  bound={bound}: {lib if isinstance(lib, str) else len(lib[1])}", flush=True)
+        assert type(lib) is type(ora)
+        if isinstance(lib, str):
+            assert lib == ora
+        else:
+            assert np.array_equal(lib[1], ora[1]) and np.max(np.abs(lib[0] - ora[0])) <= 1e-12
+
+
+@pytest.mark.gpu
+def test_gpu_cocone_only_constructor(ctx):
+    from math_amd import native
+    pts = configs.sphere_with_notch(3, 2000)
+    a = native.ManifoldConstructor(ctx, pts)
+    b = native.ManifoldConstructor(ctx, pts, cocone_only=True)
+    na, fa = a.cocone()
+    nb, fb = b.cocone()
+    assert np.array_equal(fa, fb) and np.array_equal(na, nb)
+    with pytest.raises(native.NspError, match="created for Cocone and not for BoundCocone"):
+        b.bound_cocone(RHO, ALPHA)
+    # a constructor runs several reconstructions; each result replaces the last
+    n1, f1 = a.bound_cocone(RHO, ALPHA)
+    n2, f2 = a.cocone()
+    assert np.array_equal(f2, fa) and np.array_equal(n2, na)
+
+
+@pytest.mark.gpu
+def test_gpu_errors(ctx):
+    from math_amd import native
+    with pytest.raises(native.NspError, match="Error point count 4 for cocone manifold reconstruction in 3-space"):
+        native.ManifoldConstructor(ctx, np.eye(4, 3, dtype=np.float32))
+    m = native.ManifoldConstructor(ctx, configs.sphere_with_notch(2, 100))
+    with pytest.raises(native.NspError, match=r"Rho must be in the interval \(0, 1\), but rho = 1"):
+        m.bound_cocone(1.0, ALPHA)
+    with pytest.raises(native.NspError, match=r"Alpha must be in the interval \(0, 1\), but alpha = 0"):
+        m.bound_cocone(RHO, 0.0)
+    sparse = native.ManifoldConstructor(ctx, configs.sphere_with_notch(3, 120))
+    with pytest.raises(native.NspError, match="Interior vertices not found. 2-manifold is not reconstructable."):
+        sparse.bound_cocone(RHO, ALPHA)
