@@ -77,7 +77,7 @@ def build_cpp_tests(verbose=False):
     """The C++17 drop-in tests (tests/cpp), compiled by the host compiler
     against include/nsp.h and libnsp.so."""
     exes = []
-    for name in ("test_painter_api", "test_hull_order", "test_conventions"):
+    for name in ("test_painter_api", "test_hull_order", "test_conventions", "test_reconstruction"):
         src = os.path.join(ROOT, "tests", "cpp", name + ".cpp")
         exe = os.path.join(ROOT, "tests", "cpp", name)
         cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", CSRC, src, "-o", exe,

@@ -294,3 +294,16 @@ def test_gpu_errors(ctx):
     sparse = native.ManifoldConstructor(ctx, configs.sphere_with_notch(3, 120))
     with pytest.raises(native.NspError, match="Interior vertices not found. 2-manifold is not reconstructable."):
         sparse.bound_cocone(RHO, ALPHA)
+
+
+@pytest.mark.gpu
+def test_cpp_reconstruction():
+    """The reference's reconstruction test through the C++ drop-in
+    (tests/cpp/test_reconstruction.cpp)."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "test_reconstruction")
+    assert os.path.exists(exe), "built by __graft_entry__.build()"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0 and r.stdout.rstrip().endswith("OK"), r.stdout
