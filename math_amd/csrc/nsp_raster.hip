@@ -35,14 +35,24 @@ struct RasterDims {
 // configurations' scales) with two orders of magnitude to spare; a hit the
 // FP32 test accepts lies inside the simplex by more than the re-test band.
 constexpr double BOX_MARGIN = 0.01;
+// Besides the box, the planes of the projected simplex (a (N-1)-simplex of
+// the screen): per vertex k, the unit normal (in pixels) of the projected face
+// opposite k, pointing towards k, and its offset, so that n.u + c is the
+// distance of screen point u from that face's plane, positive inside.  A pixel
+// [x, x + 1)^(N-1) whose largest distance over the pixel is below -BOX_MARGIN
+// for some face holds no sample inside the simplex dilated by BOX_MARGIN, the
+// same margin the box keeps.  When the projection is degenerate (a face or the
+// simplex of near-zero extent on the screen), the offsets are +inf and only the
+// box limits the samples.
 template <int N>
 __global__ void __launch_bounds__(256) k_raster_box(int64_t F, const double* __restrict__ vtx,
                                                     const int32_t* __restrict__ fac, SceneK<double> kd,
-                                                    int32_t* __restrict__ box)
+                                                    int32_t* __restrict__ box, float* __restrict__ planes)
 {
     constexpr int D = N - 1;
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= F) return;
+    double P[N][D];
     double mn[D], mx[D];
 #pragma unroll
     for (int i = 0; i < D; ++i) {
@@ -60,6 +70,7 @@ __global__ void __launch_bounds__(256) k_raster_box(int64_t F, const double* __r
                 den += kd.axes[i][j] * kd.axes[i][j];
             }
             const double u = num / den + 0.5 * kd.screen[i];
+            P[v][i] = u;
             mn[i] = fmin(mn[i], u);
             mx[i] = fmax(mx[i], u);
         }
@@ -81,6 +92,52 @@ __global__ void __launch_bounds__(256) k_raster_box(int64_t F, const double* __r
     if (empty) {
         b[0] = 1;
         b[D] = 0;
+    }
+    // face planes; scale: the box diagonal, for the degeneracy thresholds
+    double scale = 0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) scale += (mx[i] - mn[i]) * (mx[i] - mn[i]);
+    scale = sqrt(scale);
+    double nrm[N][D], off[N];
+    bool degenerate = !(scale > 0);
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const int j0 = k == 0 ? 1 : 0;
+        double m[(D > 1 ? D - 1 : 1) * D];
+        int r = 0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            if (j == k || j == j0) continue;
+#pragma unroll
+            for (int i = 0; i < D; ++i) m[r * D + i] = P[j][i] - P[j0][i];
+            ++r;
+        }
+        double nk[D];
+        ortho_nn<D, double>(m, nk);
+        double len = 0, h = 0;
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            len += nk[i] * nk[i];
+            h += nk[i] * (P[k][i] - P[j0][i]);
+        }
+        len = sqrt(len);
+        h = h / len;
+        if (!(len > 1e-12 * pow(scale, (double)(D - 1))) || !(fabs(h) > 1e-9 * scale)) degenerate = true;
+        const double sg = h < 0 ? -1.0 / len : 1.0 / len;
+        double c = 0;
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            nrm[k][i] = nk[i] * sg;
+            c -= nrm[k][i] * P[j0][i];
+        }
+        off[k] = c;
+    }
+    float* pl = planes + s * N * N;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+#pragma unroll
+        for (int i = 0; i < D; ++i) pl[k * N + i] = degenerate ? 0.0f : (float)nrm[k][i];
+        pl[k * N + D] = degenerate ? INFINITY : (float)off[k];
     }
 }
 
@@ -109,13 +166,16 @@ __device__ __forceinline__ int small_divmod(int x, int e, float inv, int* rem)
     return q;
 }
 
-// One wave per work unit (a chunk of one simplex's box): the record is loaded
-// once and tested against every sample of the chunk that belongs to the batch.
-// SMALL (every box volume and unit item count below 2^22): lane items advance
-// the (pixel, sample) pair by 64 without a division, and the box coordinates
-// come from float quotients; otherwise 64-bit integer division.
+// One wave per work unit (a chunk of one simplex's box): the record and face
+// planes are loaded once.  The wave takes the chunk's pixels 64 at a time: each
+// lane decodes one pixel and keeps it if it belongs to the batch and is not
+// culled by a face plane; the kept pixels are packed to the low lanes
+// (ds_permute), and then every (kept pixel, sample) pair gets the FP32 test.
+// SMALL (every box volume below 2^22): the box coordinates come from float
+// quotients; otherwise 64-bit integer division.
 template <int N, bool SMALL>
 __global__ void __launch_bounds__(RASTER_BLOCK) k_raster(TreeArgs A, const int32_t* __restrict__ box,
+                                                         const float* __restrict__ planes,
                                                          const int2* __restrict__ unit,
                                                          const int64_t* __restrict__ ustart, RasterDims rd,
                                                          const BatchDesc* __restrict__ desc, int S,
@@ -132,9 +192,10 @@ __global__ void __launch_bounds__(RASTER_BLOCK) k_raster(TreeArgs A, const int32
     const int64_t u1 = ustart[std::min<int64_t>(rd.last, xhi + 1)];
     const int lane = (int)(threadIdx.x & 63);
     const int64_t waves = (int64_t)gridDim.x * (RASTER_BLOCK / 64);
-    // item it = lane + 64 k is pixel b0 + it / S, sample it % S; 64 = q64 S + r64
-    const int q64 = 64 / S, r64 = 64 % S;
-    const int lane_q = lane / S, lane_r = lane % S;
+    const float inv_s = 1.0f / (float)S;
+    // (pair index) / S by a float quotient while pair indices stay below 2^22
+    const bool small_s = (int64_t)S * 128 < (int64_t(1) << 22);
+    const float margin = (float)BOX_MARGIN;
     // consecutive units (neighbouring boxes) on one XCD, so their samples share its L2
     for (int64_t u = u0 + (int64_t)xcd_block() * (RASTER_BLOCK / 64) + (threadIdx.x >> 6); u < u1; u += waves) {
         const int2 un = unit[u];
@@ -149,6 +210,9 @@ __global__ void __launch_bounds__(RASTER_BLOCK) k_raster(TreeArgs A, const int32
             inv_ext[i] = 1.0f / (float)ext[i];
             vol *= ext[i];
         }
+        float pl[N * N];
+#pragma unroll
+        for (int k = 0; k < N * N; ++k) pl[k] = planes[(int64_t)sid * N * N + k];
         float rec[RS];
         const float4* r4 = reinterpret_cast<const float4*>(A.rec32 + (int64_t)sid * RS);
 #pragma unroll
@@ -162,58 +226,83 @@ __global__ void __launch_bounds__(RASTER_BLOCK) k_raster(TreeArgs A, const int32
         auto comp = [&](int c) -> float { return rec[c]; };
         auto get_sid = [&]() -> int { return sid; };
         const int64_t b0 = (int64_t)un.y * rd.ppu, b1 = std::min<int64_t>(vol, b0 + rd.ppu);
-        const int64_t items = (b1 - b0) * S;
-        int next_q = lane_q, next_r = lane_r;
-        for (int64_t it = lane; it < items; it += 64) {
-            int64_t pixel = 0;
-            int s;
-            if constexpr (SMALL) {
-                int rem = (int)b0 + next_q;
-                s = next_r;
-                next_r += r64;
-                next_q += q64;
-                if (next_r >= S) {
-                    next_r -= S;
-                    ++next_q;
-                }
-                int64_t stride = 1;
+        for (int64_t g = b0; g < b1; g += 64) {
+            // this lane's pixel of the group
+            const int64_t pb = g + lane;
+            bool keep = pb < b1;
+            int rel = 0;
+            if (keep) {
+                int x[D];
+                if constexpr (SMALL) {
+                    int rem = (int)pb;
 #pragma unroll
-                for (int i = 0; i < D - 1; ++i) {
-                    int xi;
-                    rem = small_divmod(rem, ext[i], inv_ext[i], &xi);
-                    pixel += (int64_t)(lo[i] + xi) * stride;
-                    stride *= rd.screen[i];
+                    for (int i = 0; i < D - 1; ++i) rem = small_divmod(rem, ext[i], inv_ext[i], &x[i]);
+                    x[D - 1] = rem;
+                } else {
+                    int64_t rem = pb;
+#pragma unroll
+                    for (int i = 0; i < D; ++i) {
+                        x[i] = (int)(rem % ext[i]);
+                        rem /= ext[i];
+                    }
                 }
-                pixel += (int64_t)(lo[D - 1] + rem) * stride;
-            } else {
-                const int64_t b = b0 + it / S;
-                s = (int)(it % S);
-                int64_t rem = b, stride = 1;
+                int64_t pixel = 0, stride = 1;
 #pragma unroll
                 for (int i = 0; i < D; ++i) {
-                    const int64_t xi = rem % ext[i];
-                    rem /= ext[i];
-                    pixel += (lo[i] + xi) * stride;
+                    x[i] += lo[i];
+                    pixel += (int64_t)x[i] * stride;
                     stride *= rd.screen[i];
                 }
-            }
-            if (pixel < pixel0 || pixel >= pixel0 + npix) continue;
-            const int64_t path = (pixel - pixel0) * S + s;
-            float o[N], d[N];
+                keep = pixel >= pixel0 && pixel < pixel0 + npix;
+                rel = (int)(pixel - pixel0);
 #pragma unroll
-            for (int k = 0; k < N; ++k) {
-                o[k] = q.org[(int64_t)k * cap + path];
-                d[k] = rd.dir[k];
+                for (int k = 0; k < N; ++k) {
+                    // the largest distance from face k's plane over the pixel
+                    float dmax = pl[k * N + D];
+#pragma unroll
+                    for (int i = 0; i < D; ++i) {
+                        const float w = pl[k * N + i];
+                        dmax += w * (float)(x[i] + (w > 0.0f ? 1 : 0));
+                    }
+                    keep = keep && !(dmax < -margin);
+                }
             }
-            double th;
-            int si;
-            bool nr = false;
-            unsigned nret = 0;
-            if (fp32_test<N, false>(A, comp, get_sid, o, d, nullptr, nullptr, INFINITY, -1, &th, &si, &nr, &nret) ==
-                CAND_HIT) {
-                const float tf = (float)th;
-                const unsigned long long key = ((unsigned long long)__float_as_uint(tf) << 32) | (unsigned)sid;
-                atomicMin(&keys[path], key);
+            const uint64_t mask = __ballot(keep);
+            if (mask == 0) continue;
+            const int cnt = __popcll(mask);
+            const int below = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0));
+            // kept pixels to lanes 0..cnt-1 in lane order, the others after them
+            const int dst = keep ? below : cnt + (lane - below);
+            const int packed = __builtin_amdgcn_ds_permute(dst << 2, rel);
+            const int items = cnt * S;
+            for (int base = 0; base < items; base += 64) {
+                const int it = base + lane;
+                int s, r;
+                if (small_s) {
+                    r = small_divmod(it, S, inv_s, &s);
+                } else {
+                    r = it / S;
+                    s = it - r * S;
+                }
+                const int prel = __builtin_amdgcn_ds_bpermute((r < 64 ? r : 63) << 2, packed);
+                if (it >= items) continue;
+                const int64_t path = (int64_t)prel * S + s;
+                float o[N], d[N];
+#pragma unroll
+                for (int k = 0; k < N; ++k) {
+                    o[k] = q.org[(int64_t)k * cap + path];
+                    d[k] = rd.dir[k];
+                }
+                double th;
+                int si;
+                bool nr = false;
+                unsigned nret = 0;
+                if (fp32_test<N, false>(A, comp, get_sid, o, d, nullptr, nullptr, INFINITY, -1, &th, &si, &nr,
+                                        &nret) == CAND_HIT) {
+                    const float tf = (float)th;
+                    const unsigned long long key = ((unsigned long long)__float_as_uint(tf) << 32) | (unsigned)sid;
+                    atomicMin(&keys[path], key);
+                }
             }
         }
     }
@@ -238,9 +327,11 @@ void prepare_t(RasterPrep& rp, const MeshDev& m, const SceneK<double>& kd, int S
     constexpr int D = N - 1;
     const int64_t F = m.F;
     rp.box.alloc(sizeof(int32_t) * 2 * D * std::max<int64_t>(F, 1));
+    rp.planes.alloc(sizeof(float) * N * N * std::max<int64_t>(F, 1));
     if (F > 0) {
         hipLaunchKernelGGL(k_raster_box<N>, dim3((unsigned)((F + 255) / 256)), dim3(256), 0, s, F,
-                           m.vtx64.as<double>(), m.fac.as<int32_t>(), kd, rp.box.as<int32_t>());
+                           m.vtx64.as<double>(), m.fac.as<int32_t>(), kd, rp.box.as<int32_t>(),
+                           rp.planes.as<float>());
         NSP_HIP(hipGetLastError());
     }
     std::vector<int32_t> hb((size_t)2 * D * F);
@@ -279,7 +370,8 @@ void prepare_t(RasterPrep& rp, const MeshDev& m, const SceneK<double>& kd, int S
     for (int v = rp.last - 1; v >= 0; --v) start[v] = std::min(start[v], start[v + 1]);
     rp.hmax = hmax;
     rp.units = (int64_t)hu.size();
-    rp.small = max_vol < (int64_t(1) << 22) && rp.ppu * S < (int64_t(1) << 22);
+    rp.small = max_vol < (int64_t(1) << 22);
+    if ((int64_t)S * 128 > INT32_MAX) raise(NSP_E_UNSUPPORTED, "too many samples per pixel for rasterising");
     rp.unit.alloc(sizeof(int2) * std::max<size_t>(hu.size(), 1));
     rp.ustart.alloc(sizeof(int64_t) * start.size());
     if (!hu.empty()) NSP_HIP(hipMemcpyAsync(rp.unit.p, hu.data(), sizeof(int2) * hu.size(), hipMemcpyHostToDevice, s));
@@ -307,10 +399,12 @@ void launch_t(const RasterPrep& rp, const TreeArgs& A, const BatchDesc* desc, Ra
     const unsigned blocks = 2048;  // 8192 waves; each wave loops over the batch's units
     if (rp.small)
         hipLaunchKernelGGL((k_raster<N, true>), dim3(blocks), dim3(RASTER_BLOCK), 0, s, A, rp.box.as<int32_t>(),
-                           rp.unit.as<int2>(), rp.ustart.as<int64_t>(), rd, desc, rp.S, q, cap, keys);
+                           rp.planes.as<float>(), rp.unit.as<int2>(), rp.ustart.as<int64_t>(), rd, desc, rp.S, q,
+                           cap, keys);
     else
         hipLaunchKernelGGL((k_raster<N, false>), dim3(blocks), dim3(RASTER_BLOCK), 0, s, A, rp.box.as<int32_t>(),
-                           rp.unit.as<int2>(), rp.ustart.as<int64_t>(), rd, desc, rp.S, q, cap, keys);
+                           rp.planes.as<float>(), rp.unit.as<int2>(), rp.ustart.as<int64_t>(), rd, desc, rp.S, q,
+                           cap, keys);
     NSP_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_raster_resolve, dim3(gclear), dim3(256), 0, s, desc, q, keys);
     NSP_HIP(hipGetLastError());

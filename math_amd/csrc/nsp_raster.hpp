@@ -26,6 +26,7 @@ struct RasterPrep {
     bool small = false;   // every box volume and unit item count below 2^22 (float-quotient index split)
     int screen[NSP_MAX_DIM - 1] = {0};
     DevBuf box;           // int32 [F][2(n-1)]: first and last pixel per screen axis
+    DevBuf planes;        // float [F][n][n]: per vertex k, the unit normal and offset of the projected face opposite k
     DevBuf unit;          // int2 (simplex, chunk of its box) in the order of the box's low corner, last axis major
     DevBuf ustart;        // int64 [last + 1]: first unit whose box starts at last-axis pixel v
 };

@@ -123,6 +123,24 @@ __device__ __forceinline__ void load_vec_f(const float* __restrict__ soa, int64_
 // Results of a candidate test.
 constexpr int CAND_MISS = 0, CAND_HIT = 1, CAND_DEFER = 2;
 
+// The order of hits along a ray: the smaller distance, then the smaller
+// simplex index.  FP32 walks compare distances rounded to FP32 (the distance a
+// hit keeps), so that every walk and the rasteriser, which orders hits by
+// (FP32 distance, index) keys, choose the same simplex among hits whose
+// distances round alike, whatever order they meet them in; FP64 walks compare
+// FP64 distances.  A candidate test accepts a hit at the limit distance when
+// its index is below best_idx, the index of the hit holding the limit.
+template <typename T>
+__device__ __forceinline__ bool nearer(double t, int sid, double best_t, int best)
+{
+    if constexpr (sizeof(T) == 4) {
+        const float a = (float)t, b = (float)best_t;
+        return a < b || (a == b && sid < best);
+    } else {
+        return t < best_t || (t == best_t && sid < best);
+    }
+}
+
 // The FP32 test of one simplex against the ray (o, d): comp(c) returns
 // component c of the simplex's FP32 record (normal[N], d, then the N-1
 // barycentric planes), sid() its index.  Shared by the tree walk and the
@@ -175,7 +193,7 @@ __device__ __forceinline__ int fp32_test(const TreeArgs& A, const Comp& comp, co
         if (!in || !(t > 0.0f)) return CAND_MISS;
         const int sid = get_sid();
         if (fabsf(t - tlimit) < terr && sid != best_idx) *near = true;  // near-tie with another simplex
-        if (!(t < tlimit)) return CAND_MISS;
+        if (!(t < tlimit || (t == tlimit && sid < best_idx))) return CAND_MISS;
         *t_hit = (double)t;
         *idx = sid;
         return CAND_HIT;
@@ -197,7 +215,8 @@ __device__ __forceinline__ int fp32_test(const TreeArgs& A, const Comp& comp, co
     }
     double t64;
     if (!simplex_test64<N>(A.rec64 + (int64_t)sidx * rec64_stride(N), o64, d64, &t64)) return CAND_MISS;
-    if (!((float)t64 < tlimit)) return CAND_MISS;
+    const float tf = (float)t64;
+    if (!(tf < tlimit || (tf == tlimit && sidx < best_idx))) return CAND_MISS;
     *t_hit = t64;
     *idx = sidx;
     return CAND_HIT;
@@ -255,7 +274,7 @@ __device__ __forceinline__ int candidate(const TreeArgs& A, int64_t e, const T* 
         int s = A.entries[e];
         double t;
         if (!simplex_test64<N>(A.rec64 + (int64_t)s * rec64_stride(N), o, d, &t)) return CAND_MISS;
-        if (!(t < tlimit)) return CAND_MISS;
+        if (!(t < tlimit || (t == tlimit && s < best_idx))) return CAND_MISS;
         *t_hit = t;
         *idx = s;
         return CAND_HIT;
@@ -298,6 +317,27 @@ __device__ __forceinline__ bool slab(const T* blo, const T* bsz, const T* o, con
     *tn = a;
     *tf = b;
     return a <= b;
+}
+
+// Exit of the ray from the located leaf box [blo, blo + bsz].  Axes along
+// which the ray does not move are left out: the leaf was located from the
+// ray's own coordinate there, so the ray stays in it along those axes even
+// when that coordinate lies within rounding of a cell boundary and outside the
+// box's computed bounds.  (Testing them made such an axis-parallel ray creep
+// through its leaves one advance step at a time until the step limit ended
+// the walk without its hit.)  An exit at or before t_cur (rounding along a
+// moving axis) makes advance() double its step.
+template <int N, typename T>
+__device__ __forceinline__ T box_exit(const T* blo, const T* bsz, const T* o, const T* inv, const T* d)
+{
+    T b = INFINITY;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        if (d[i] == 0) continue;
+        const T x0 = (blo[i] - o[i]) * inv[i], x1 = (blo[i] + bsz[i] - o[i]) * inv[i];
+        b = fmin(b, fmax(x0, x1));
+    }
+    return b;
 }
 
 // The tree walk of one ray, split into start() and step() so that a lane can
@@ -433,9 +473,7 @@ struct RayWalk {
             bsz[i] = cell;
             blo[i] = rlo[i] + (T)(Q[i] >> (lmax - level)) * cell;
         }
-        T ta, tb;
-        if (!slab<N, T>(blo, bsz, o, inv, d, &ta, &tb)) tb = tp;  // rounding: point located outside its own box
-        t_next = tb;
+        t_next = box_exit<N, T>(blo, bsz, o, inv, d);
         *cnt = 0;
         if (nd.x != last_leaf && nd.y > 0) {
             ++lvis;
@@ -496,7 +534,7 @@ struct RayWalk {
                         best = si;
                         return true;
                     }
-                    if (th < best_t || (th == best_t && si < best)) {
+                    if (nearer<T>(th, si, best_t, best)) {
                         best_t = (BestT)th;
                         best = si;
                     }
@@ -671,6 +709,7 @@ __device__ __forceinline__ void coop_test(const TreeArgs& A, CoopWave& L, W& w, 
             od[k] = __shfl((float)w.d[k], owner);
         }
         const float tlo = __shfl(tl, owner);
+        const int bo = __shfl(any ? -1 : w.best, owner);
         const int npo = __shfl(np0, owner);
         if (j < total) {
             float rec[RS];
@@ -681,7 +720,7 @@ __device__ __forceinline__ void coop_test(const TreeArgs& A, CoopWave& L, W& w, 
             int si;
             bool nr = false;
             unsigned dummy = 0;
-            const int r = fp32_test<N, true>(A, comp, get_sid, oo, od, nullptr, nullptr, tlo, -1, &th, &si, &nr,
+            const int r = fp32_test<N, true>(A, comp, get_sid, oo, od, nullptr, nullptr, tlo, bo, &th, &si, &nr,
                                              &dummy);
             if (r == CAND_HIT) {
                 // (an any-hit owner only asks whether its key fell below NO_HIT_KEY)
@@ -741,10 +780,11 @@ __device__ __forceinline__ bool pending_resolve(const TreeArgs& A, const float* 
         if (sid < 0) continue;
         double t64;
         if (!simplex_test64<N>(A.rec64 + (int64_t)sid * rec64_stride(N), o64, d64, &t64)) continue;
-        const float lim = ANY ? tmax : (float)*best_t;
-        if (!((float)t64 < lim)) continue;
-        if (ANY) return true;
-        if (t64 < *best_t || (t64 == *best_t && sid < *best)) {
+        if (ANY) {
+            if ((float)t64 < tmax) return true;
+            continue;
+        }
+        if (nearer<float>(t64, sid, *best_t, *best)) {
             *best_t = t64;
             *best = sid;
             any = true;

@@ -219,3 +219,22 @@ def test_progressive_pass_images_match_checker(ctx, precision):
         assert rmse < (1e-4 if precision == native.PRECISION_FP64 else 5e-3)
         if k > 0:  # later passes add new samples, not the first pass again
             assert not np.allclose(img, images[0][0] * (k + 1))
+
+
+@pytest.mark.parametrize("config", ["C0", "C1", "C2"])
+def test_rasterised_primaries_equal_walk_full_size(ctx, config):
+    """The benchmark pass with rasterised primary visibility (its default)
+    against the same pass with every primary ray walked through the tree, at
+    the configuration's full size: the primary hits are the same, so the
+    images, sample counts and ray counts are bit-identical."""
+    v, f, cfg, spp, depth = configs.config(config)
+    gm = ctx.mesh(v, f)
+    out = []
+    for primary in (native.PRIMARY_AUTO, native.PRIMARY_WALK):
+        print(f"{config}: primary mode {primary}", flush=True)
+        sc = native.Scene(ctx, gm, cfg)
+        sc.paint(spp, depth, seed=SEED, primary=primary)
+        out.append(sc.image() + (sc.statistics()["rays"],))
+        sc.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
