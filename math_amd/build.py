@@ -30,6 +30,7 @@ SOURCES = {
     "nsp_render.hip": [],
     "nsp_denoise.hip": [],
     "nsp_raster.hip": [],
+    "nsp_shadow_raster.hip": [],
     "nsp_hull.hip": [],
     "nsp_cocone.hip": ["-ffp-contract=off"],  # FP64 reconstruction equal to the CPU checker's
     "nsp_api.cpp": ["-x", "hip"],

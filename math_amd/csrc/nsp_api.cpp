@@ -127,6 +127,7 @@ struct nsp_scene {
     DevBuf pix_sum, pix_cnt;
     DevBuf pix_aov;  // aov_channels(n) * pixels, allocated on the first pass with aov
     RasterPrep raster;  // rasterised primary visibility (boxes and work units), built on first use
+    ShadowRasterPrep sraster;  // shadow rays rasterised from the light (footprints, grid), built on first use
     DevBuf pass_dev;  // int: the pass index the kernels read
     std::vector<ShapeRec> shapes;  // the objects after the mesh (nsp_scene_set_shapes)
     DevBuf shapes_dev;
@@ -916,6 +917,12 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
         const int S = kf.S;
         const bool raster = p.precision == NSP_PRECISION_FP32 && p.primary == NSP_PRIMARY_AUTO &&
                             scene->proj.kind == NSP_PROJECTOR_PARALLEL;
+        // shadow rays rasterised from the light: FP32 walks over simplex-ordered
+        // records, one point light, no shapes (and the light sees the mesh in
+        // front of it, decided by shadow_raster_prepare)
+        bool sraster = p.precision == NSP_PRECISION_FP32 && p.primary == NSP_PRIMARY_AUTO &&
+                       scene->lights.size() == 1 && scene->lights[0].kind == NSP_LIGHT_POINT &&
+                       scene->shapes.empty() && scene->mesh->m.layout == NSP_LAYOUT_SIMPLEX_AOS;
         // Default batch: the largest power of two of paths, at most 2^27 (FP32) or
         // 2^26 (FP64), whose workspace fits DEFAULT_WORKSPACE.  Large batches keep
         // the persistent walks busy and leave fewer batch tails (C1, 27 samples per
@@ -924,7 +931,7 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
         int64_t want = p.batch_paths;
         if (want <= 0) {
             const int64_t per =
-                render_bytes_per_path(n, p.precision, (int)scene->lights.size(), p.aov != 0, raster);
+                render_bytes_per_path(n, p.precision, (int)scene->lights.size(), p.aov != 0, raster, sraster);
             want = (int64_t)1 << (p.precision ? 26 : 27);
             while (want > ((int64_t)1 << 20) && want * per > DEFAULT_WORKSPACE) want >>= 1;
         }
@@ -936,6 +943,8 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
             raster_prepare(scene->raster, scene->mesh->m, kd, S, s);
             ctx->work.keys.alloc(sizeof(unsigned long long) * cap);
         }
+        if (sraster) sraster = shadow_raster_prepare(scene->sraster, scene->mesh->m, kd.light_pos[0], s);
+        if (sraster) shadow_raster_buffers(ctx->work, n, cap);
         if (p.aov) {
             ctx->work.aov.alloc(sizeof(float) * aov_channels(n) * ctx->work.cap);  // per-path planes of stride cap
             if (!scene->pix_aov.p) {
@@ -965,6 +974,7 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
         rl.counters = scene->counters;
         rl.marks = p.use_graph ? nullptr : &scene->marks;
         rl.raster = raster ? &scene->raster : nullptr;
+        rl.sraster = sraster ? &scene->sraster : nullptr;
         rl.side = ctx->side;
         GraphHolder gh;
         std::unique_ptr<PixelFeed> feed;

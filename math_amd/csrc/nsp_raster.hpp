@@ -41,4 +41,51 @@ void raster_prepare(RasterPrep& rp, const MeshDev& m, const SceneK<double>& kd, 
 void raster_launch(const RasterPrep& rp, const MeshDev& m, const TreeArgs& A, const BatchDesc* desc,
                    RayQueue<float> q, int64_t cap, unsigned long long* keys, const float* dir, hipStream_t s);
 
+// Shadow rays towards one point light, rasterised as the light sees the mesh:
+// every shadow ray of the depth ends at the light, so, projected from the
+// light, it is a single point, and the simplices it can hit are those whose
+// projection covers that point.  The rays are binned by the cell of their
+// origin's projection (a count, a scan and a scatter into cell order); then
+// each simplex's footprint cells are culled by its projected face planes and
+// every ray of the kept cells gets the any-hit test the shadow walk applies to
+// a leaf candidate.  The occlusion of each ray is therefore the walk's.  Used
+// when every vertex lies well in front of the light along the axis from the
+// light to the mesh centre (then a simplex's footprint is the simplex of its
+// projected vertices).
+struct SRGrid {
+    double light[NSP_MAX_DIM];
+    double axis[NSP_MAX_DIM];
+    double basis[NSP_MAX_DIM - 1][NSP_MAX_DIM];
+    double smin[NSP_MAX_DIM - 1];
+    double inv_cell[NSP_MAX_DIM - 1];
+    int grid[NSP_MAX_DIM - 1];
+};
+
+struct ShadowRasterPrep {
+    bool ok = false;      // the light sees the mesh in front of it (else the walks take the shadow rays)
+    int n = 0;
+    int64_t F = 0;
+    double light[NSP_MAX_DIM] = {0};
+    SRGrid grid{};
+    int64_t cells = 0;
+    int64_t units = 0;
+    DevBuf box;           // int32 [F][2(n-1)]: first and last cell per axis of each footprint
+    DevBuf planes;        // float [F][n][n]: the footprint's face planes in cell coordinates
+    DevBuf unit;          // int2 (simplex, chunk of 64 cells of its box)
+    DevBuf count, start, tiles;  // u32 per cell: rays, first ray in cell order; scan tiles
+};
+
+// Projection, grid, footprints and work units for the light at `light` (once
+// per mesh and light position); false when the light does not see the whole
+// mesh in front of it.
+bool shadow_raster_prepare(ShadowRasterPrep& sp, const MeshDev& m, const double* light, hipStream_t s);
+// Per-ray buffers of the binned shadow rays (scap rays).
+void shadow_raster_buffers(RenderBuffers& b, int n, int64_t scap);
+// Occlusion of the shadow rays of one depth (queue sq, counters dc); the
+// contributions of the rays that reach the light are added to Ls (and to the
+// direct-light AOV when direct is set), as the shadow walks add them.
+void shadow_raster_launch(const ShadowRasterPrep& sp, const MeshDev& m, const TreeArgs& A, ShadowQueue<float> sq,
+                          int64_t scap, const unsigned long long* dc, RenderBuffers& b, float* Ls, int64_t cap,
+                          float* direct, hipStream_t s);
+
 }  // namespace nsp

@@ -357,45 +357,6 @@ __global__ void __launch_bounds__(TRACE_BLOCK)
     }
 }
 
-// Adds an unoccluded shadow ray's contribution to its path (and, for a
-// first-hit shadow ray while AOVs are collected, to the direct-light AOV).
-// With one light, each path receives at most one contribution per shadow
-// launch and the additions need no atomics; with several they are atomic.
-template <int N, typename T>
-__device__ __forceinline__ void shadow_unoccluded(const ShadowQueue<T>& sq, int64_t scap, int64_t i,
-                                                  float* __restrict__ L, int64_t cap, float* __restrict__ direct)
-{
-    if (sq.shape_count > 0) {
-        // the mesh (the first object) let the ray pass: the shapes were tested
-        // next, up to the first that blocks the light (shade recorded it)
-        const int32_t j = sq.shape[i];
-        atomicAdd(sq.extra, (unsigned long long)(j >= 0 ? j + 1 : sq.shape_count));
-        if (j >= 0) return;
-    }
-    const int32_t raw = sq.path[i];
-    const int32_t path = raw < 0 ? -raw - 1 : raw;
-    const float c0 = sq.contrib[i], c1 = sq.contrib[scap + i], c2 = sq.contrib[2 * scap + i];
-    if (sq.multi) {
-        atomicAdd(&L[path], c0);
-        atomicAdd(&L[cap + path], c1);
-        atomicAdd(&L[2 * cap + path], c2);
-        if (raw < 0 && direct) {
-            atomicAdd(&direct[path], c0);
-            atomicAdd(&direct[cap + path], c1);
-            atomicAdd(&direct[2 * cap + path], c2);
-        }
-        return;
-    }
-    L[path] += c0;
-    L[cap + path] += c1;
-    L[2 * cap + path] += c2;
-    if (raw < 0 && direct) {
-        direct[path] += c0;
-        direct[cap + path] += c1;
-        direct[2 * cap + path] += c2;
-    }
-}
-
 // Entries of the FP64-decision lists: ray index, then PENDING candidates (-1: none).
 constexpr int RES_STRIDE = 1 + PENDING;
 
@@ -729,10 +690,11 @@ struct WalkPool {
     int64_t nc, total;  // the current segment's closest-hit rays and all its rays
     int slice, visited;
     bool drained;
+    bool shadows;  // false: the shadow rays are rasterised from the light, the pool holds closest-hit rays only
 
-    __device__ __forceinline__ WalkPool(const unsigned long long* c, unsigned long long* s)
+    __device__ __forceinline__ WalkPool(const unsigned long long* c, unsigned long long* s, bool with_shadows)
         : cc(c), sc(s), csegcap((int64_t)c[CNT_SEGCAP]), ssegcap((int64_t)s[CNT_SSEGCAP]),
-          slice((int)(blockIdx.x % QSEG)), visited(0), drained(false)
+          slice((int)(blockIdx.x % QSEG)), visited(0), drained(false), shadows(with_shadows)
     {
         load();
     }
@@ -740,7 +702,7 @@ struct WalkPool {
     __device__ __forceinline__ void load()
     {
         nc = (int64_t)cc[cnt_rayseg(slice)];
-        total = nc + (int64_t)sc[cnt_shadowseg(slice)];
+        total = nc + (shadows ? (int64_t)sc[cnt_shadowseg(slice)] : 0);
     }
 
     // For each lane with want set: the index of its ray in the closest-hit
@@ -802,14 +764,14 @@ __global__ void __launch_bounds__(TRACE_BLOCK, walk_min_waves<N>()) k_walk(TreeA
                                        unsigned long long* __restrict__ cc, unsigned long long* __restrict__ sc,
                                        float* __restrict__ Ls, float* __restrict__ direct,
                                        int32_t* __restrict__ redo_c, int32_t* __restrict__ res_c,
-                                       int32_t* __restrict__ redo_s, int32_t* __restrict__ res_s)
+                                       int32_t* __restrict__ redo_s, int32_t* __restrict__ res_s, int shadows)
 {
     __shared__ int2 stack[stack_levels(N) * TRACE_BLOCK];
     int2* st = stack + threadIdx.x;
     __shared__ CoopWave coop[TRACE_BLOCK / 64];
     CoopWave& L = coop[threadIdx.x >> 6];
     const int lane = (int)(threadIdx.x & 63);
-    WalkPool<refill_lanes<N>()> pool(cc, sc);
+    WalkPool<refill_lanes<N>()> pool(cc, sc, shadows != 0);
     RayWalk<N, float, false, 1, true> w;
     int32_t ray = -1;  // ray indices fit 32 bits (at most 2^27 paths times 8 lights)
     bool any = false;  // the lane's walk is a shadow (any-hit) walk
@@ -1045,16 +1007,17 @@ void alloc_render_buffers(RenderBuffers& b, int n, int precision, int64_t cap, i
     b.trav_counters.alloc(sizeof(unsigned long long) * 8);
 }
 
-int64_t render_bytes_per_path(int n, int precision, int lights, bool aov, bool raster)
+int64_t render_bytes_per_path(int n, int precision, int lights, bool aov, bool raster, bool shadow_raster)
 {
     // the per-path sizes alloc_render_buffers (and the raster keys, the AOV
-    // planes) allocate
+    // planes, the binned shadow rays) allocate
     const int64_t ts = precision ? sizeof(double) : sizeof(float);
     const int64_t L = std::max(1, lights);
     const int64_t queue = 2 * ts * n + 4 + 12 + 4 + 4 + ts;
     const int64_t shadow = 2 * ts * n + ts + 4 + 12 + 4;
     int64_t b = 2 * queue + 2 * shadow * L + 24 + 4 * L + 4 + 4 * RES_STRIDE * (1 + L);
     if (raster) b += 8;
+    if (shadow_raster) b += (4 + 4 + 4 * ((2 * n + 4) / 4 * 4) + 4 + 1) * L;
     if (aov) b += 4 * aov_channels(n);
     return b;
 }
@@ -1208,13 +1171,20 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
                 // one walk launch: the bounce rays of depth + 1 (in qout) and the shadow rays of depth
                 unsigned long long* dc1 = cnt + (depth + 1) * CNT_STRIDE;
                 float* direct = (aov && depth == 0) ? aov + (int64_t)(3 + N) * cap : nullptr;
+                // the shadow rays of the primary hits are rasterised from the light;
+                // the fewer of later depths are walked (a rasterising launch costs
+                // about 0.5 ms for C1's 663,552 footprints even with few rays)
+                const bool sr = rl.sraster != nullptr && depth == 0;
+                if (sr) timed(3, [&] {
+                    shadow_raster_launch(*rl.sraster, *rl.mesh, A, sq, scap, dc, b, Ls, cap, direct, s);
+                });
                 timed(6, [&] {
                     if (rl.counters)
                         hipLaunchKernelGGL((k_walk<N, true>), dim3(gw), dim3(TRACE_BLOCK), 0, s, A, As.counters, qout,
-                                           cap, sq, scap, dc1, dc, Ls, direct, redo_c, res_c, redo_s, res_s);
+                                           cap, sq, scap, dc1, dc, Ls, direct, redo_c, res_c, redo_s, res_s, sr ? 0 : 1);
                     else
                         hipLaunchKernelGGL((k_walk<N, false>), dim3(gw), dim3(TRACE_BLOCK), 0, s, A, As.counters, qout,
-                                           cap, sq, scap, dc1, dc, Ls, direct, redo_c, res_c, redo_s, res_s);
+                                           cap, sq, scap, dc1, dc, Ls, direct, redo_c, res_c, redo_s, res_s, sr ? 0 : 1);
                     NSP_HIP(hipGetLastError());
                     hipLaunchKernelGGL((k_closest_resolve<N>), dim3(gx), dim3(256), 0, s, A, qout, cap,
                                        dc1 + CNT_RES_C, res_c);
@@ -1222,6 +1192,7 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
                     hipLaunchKernelGGL((k_closest_exact<N, T>), dim3(gx), dim3(TRACE_BLOCK), 0, s, A, qout, cap,
                                        dc1 + CNT_REDO_C, redo_c);
                     NSP_HIP(hipGetLastError());
+                    if (sr) return;
                     hipLaunchKernelGGL((k_shadow_resolve<N>), dim3(gx), dim3(256), 0, s, As, sq, scap, Ls, cap,
                                        direct, dc + CNT_RES_S, res_s);
                     NSP_HIP(hipGetLastError());

@@ -179,7 +179,52 @@ struct RenderBuffers {
     DevBuf batch_idx;       // long long: index of the next batch
     DevBuf totals;          // 3 u64: closest, shadow and probe rays of the pass
     DevBuf trav_counters;   // 8 u64: traversal counters (closest stage, shadow stage)
+    // binned shadow rays of the light-space rasteriser (nsp_raster.hpp)
+    DevBuf sr_cell;         // int32 per shadow ray: its cell
+    DevBuf sr_rank;         // u32 per shadow ray: its place among its cell's rays
+    DevBuf sr_rays;         // per shadow ray in cell order: origin, direction, limit (16-byte padded)
+    DevBuf sr_ray;          // int32 per binned ray: its shadow-queue index
+    DevBuf sr_occl;         // u8 per binned ray: occluded
 };
+
+// Adds an unoccluded shadow ray's contribution to its path (and, for a
+// first-hit shadow ray while AOVs are collected, to the direct-light AOV).
+// With one light, each path receives at most one contribution per shadow
+// launch and the additions need no atomics; with several they are atomic.
+template <int N, typename T>
+__device__ __forceinline__ void shadow_unoccluded(const ShadowQueue<T>& sq, int64_t scap, int64_t i,
+                                                  float* __restrict__ L, int64_t cap, float* __restrict__ direct)
+{
+    if (sq.shape_count > 0) {
+        // the mesh (the first object) let the ray pass: the shapes were tested
+        // next, up to the first that blocks the light (shade recorded it)
+        const int32_t j = sq.shape[i];
+        atomicAdd(sq.extra, (unsigned long long)(j >= 0 ? j + 1 : sq.shape_count));
+        if (j >= 0) return;
+    }
+    const int32_t raw = sq.path[i];
+    const int32_t path = raw < 0 ? -raw - 1 : raw;
+    const float c0 = sq.contrib[i], c1 = sq.contrib[scap + i], c2 = sq.contrib[2 * scap + i];
+    if (sq.multi) {
+        atomicAdd(&L[path], c0);
+        atomicAdd(&L[cap + path], c1);
+        atomicAdd(&L[2 * cap + path], c2);
+        if (raw < 0 && direct) {
+            atomicAdd(&direct[path], c0);
+            atomicAdd(&direct[cap + path], c1);
+            atomicAdd(&direct[2 * cap + path], c2);
+        }
+        return;
+    }
+    L[path] += c0;
+    L[cap + path] += c1;
+    L[2 * cap + path] += c2;
+    if (raw < 0 && direct) {
+        direct[path] += c0;
+        direct[cap + path] += c1;
+        direct[2 * cap + path] += c2;
+    }
+}
 
 TreeArgs make_tree_args(const MeshDev& m);
 
@@ -224,6 +269,9 @@ struct RenderLaunch {
     unsigned long long* totals = nullptr;
     // a second stream for the shadow stages (null: everything on the launch stream)
     hipStream_t side = nullptr;
+    // FP32 walks over simplex-ordered records: the shadow rays rasterised from
+    // the light (nsp_raster.hpp), or null to walk them
+    const struct ShadowRasterPrep* sraster = nullptr;
 };
 // One wavefront batch: raygen, then per depth closest hits, shade and the
 // shadow stage, then accumulate.  With rl.side, the shadow stage of depth d
@@ -239,7 +287,7 @@ void pass_reset(RenderBuffers& b, bool counters, int* pass_dev, int pass, hipStr
 // Clears a scene's own pass totals (3 u64) and batch index.
 void pass_reset_state(unsigned long long* totals, long long* batch_idx, int* pass_dev, int pass, hipStream_t s);
 // Device bytes per path of one wavefront batch's workspace.
-int64_t render_bytes_per_path(int n, int precision, int lights, bool aov, bool raster);
+int64_t render_bytes_per_path(int n, int precision, int lights, bool aov, bool raster, bool shadow_raster = false);
 void alloc_render_buffers(RenderBuffers& b, int n, int precision, int64_t cap, int max_depth, int lights = 1);
 // FP64 records of shapes (org and edge vectors as given) into out (device), on stream s.
 void build_shape_records(int n, const int* kind, const double* org, const double* edges, int count, ShapeRec* out,
