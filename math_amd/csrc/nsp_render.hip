@@ -775,6 +775,9 @@ __global__ void __launch_bounds__(TRACE_BLOCK, walk_min_waves<N>()) k_walk(TreeA
     RayWalk<N, float, false, 1, true> w;
     int32_t ray = -1;  // ray indices fit 32 bits (at most 2^27 paths times 8 lights)
     bool any = false;  // the lane's walk is a shadow (any-hit) walk
+    // COUNT: the lane's traversal counters over its walks (closest-hit, then
+    // shadow), added to the global counters once per wave at the end
+    unsigned long long acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (;;) {
         if (!pool.drained) {
             bool shadow = false;
@@ -825,7 +828,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK, walk_min_waves<N>()) k_walk(TreeA
                     else
                         shadow_unoccluded<N, float>(sq, scap, ray, Ls, cap, direct);
                 }
-                if constexpr (COUNT) w.flush_to(shadow_counters);
+                if constexpr (COUNT) w.add_to(acc + 4);
             } else {
                 if (w.deferred) {
                     redo_c[atomicAdd(cc + CNT_REDO_C, 1ull)] = (int32_t)ray;
@@ -836,11 +839,21 @@ __global__ void __launch_bounds__(TRACE_BLOCK, walk_min_waves<N>()) k_walk(TreeA
                     const int np = L.npend[lane];
                     if (np > 0) push_pending(cc + CNT_RES_C, res_c, ray, &L.pend[lane * PENDING], np);
                 }
-                if constexpr (COUNT) w.flush_counters(A);
+                if constexpr (COUNT) w.add_to(acc);
             }
             ray = -1;
         }
         if (pool.drained && __ballot(ray >= 0) == 0) break;
+    }
+    if constexpr (COUNT) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            unsigned long long v = acc[k];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+            unsigned long long* dst = k < 4 ? A.counters : shadow_counters;
+            if (lane == 0 && dst) atomicAdd(&dst[k & 3], v);
+        }
     }
 }
 

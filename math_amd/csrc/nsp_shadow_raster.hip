@@ -317,33 +317,41 @@ __global__ void __launch_bounds__(SR_BLOCK) k_sr_raster(TreeArgs A, const int2* 
     const int64_t waves = (int64_t)gridDim.x * (SR_BLOCK / 64);
     const float margin = (float)SR_MARGIN;
     for (int64_t u = (int64_t)xcd_block() * (SR_BLOCK / 64) + (threadIdx.x >> 6); u < units; u += waves) {
+        // the unit is the wave's: its simplex's box, planes and record are read
+        // with scalar loads
         const int2 un = unit[u];
-        const int sid = un.x;
+        const int sid = __builtin_amdgcn_readfirstlane(un.x);
+        const int chunk = __builtin_amdgcn_readfirstlane(un.y);
         int lo[D], ext[D];
-        int64_t vol = 1;
+        int vol = 1;
 #pragma unroll
         for (int i = 0; i < D; ++i) {
             lo[i] = box[(int64_t)sid * 2 * D + i];
             ext[i] = box[(int64_t)sid * 2 * D + D + i] - lo[i] + 1;
             vol *= ext[i];
         }
-        const int64_t pb = (int64_t)un.y * SR_CELLS_PER_UNIT + lane;
+        float pl[N * N];
+#pragma unroll
+        for (int k = 0; k < N * N; ++k) pl[k] = planes[(int64_t)sid * N * N + k];
+        const int pb = chunk * SR_CELLS_PER_UNIT + lane;
         bool keep = pb < vol;
         uint32_t first = 0, cnt = 0;
         if (keep) {
             int x[D];
-            int64_t rem = pb;
+            int rem = pb;
 #pragma unroll
-            for (int i = 0; i < D; ++i) {
-                x[i] = (int)(rem % ext[i]) + lo[i];
-                rem /= ext[i];
+            for (int i = 0; i < D - 1; ++i) {
+                const int q = rem / ext[i];
+                x[i] = rem - q * ext[i] + lo[i];
+                rem = q;
             }
+            x[D - 1] = rem + lo[D - 1];
 #pragma unroll
             for (int k = 0; k < N; ++k) {
-                float dmax = planes[(int64_t)sid * N * N + k * N + D];
+                float dmax = pl[k * N + D];
 #pragma unroll
                 for (int i = 0; i < D; ++i) {
-                    const float w = planes[(int64_t)sid * N * N + k * N + i];
+                    const float w = pl[k * N + i];
                     dmax += w * (float)(x[i] + (w > 0.0f ? 1 : 0));
                 }
                 keep = keep && !(dmax < -margin);
@@ -572,8 +580,25 @@ void prepare_t(ShadowRasterPrep& sp, const MeshDev& m, const double* light, hipS
             hb[(size_t)f * 2 * D + D + i] = hi;
             vol *= std::max(0, hi - lo + 1);
         }
+        if (vol > INT32_MAX / 2) return;
         const int64_t chunks = (vol + SR_CELLS_PER_UNIT - 1) / SR_CELLS_PER_UNIT;
         for (int64_t c = 0; c < chunks; ++c) hu.push_back(int2{(int)f, (int)c});
+    }
+    // units in the order of their boxes' low corners (last axis major), so that
+    // the simplices that cover a region of cells run together on one XCD and
+    // its rays stay in that XCD's L2 between them
+    {
+        std::vector<std::pair<int64_t, int64_t>> key(hu.size());
+        for (size_t k = 0; k < hu.size(); ++k) {
+            const int32_t* bx = hb.data() + (size_t)hu[k].x * 2 * D;
+            int64_t corner = 0;
+            for (int i = D - 1; i >= 0; --i) corner = corner * g.grid[i] + bx[i];
+            key[k] = {corner, (int64_t)k};
+        }
+        std::sort(key.begin(), key.end());
+        std::vector<int2> sorted(hu.size());
+        for (size_t k = 0; k < hu.size(); ++k) sorted[k] = hu[(size_t)key[k].second];
+        hu.swap(sorted);
     }
     sp.units = (int64_t)hu.size();
     sp.box.alloc(sizeof(int32_t) * hb.size());

@@ -595,6 +595,15 @@ struct RayWalk {
 
     __device__ __forceinline__ void flush_counters(const TreeArgs& A) const { flush_to(A.counters); }
 
+    // adds the walk's traversal counters to a lane's running sums acc[0..3]
+    __device__ __forceinline__ void add_to(unsigned long long* acc) const
+    {
+        acc[0] += nvis;
+        acc[1] += lvis;
+        acc[2] += ntest;
+        acc[3] += nret;
+    }
+
     // adds the walk's traversal counters to c[0..3] (node visits, leaf visits,
     // simplex tests, FP64 decisions) when c is not null
     __device__ __forceinline__ void flush_to(unsigned long long* c) const
