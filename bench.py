@@ -42,11 +42,17 @@ def walk_bytes(st, n):
             + (4 + 4 * (n * n + n)) * st["simplex_tests"])
 
 
+def shadow_walks(st):
+    """Shadow rays walked through the tree (the shadow rays of the primary hits
+    are rasterised from the light when the scene has one point light)."""
+    return st["shadow_rays"] - st.get("shadow_rays_rasterised", 0)
+
+
 def shadow_bytes(st, n):
     """Algorithmic bytes of the shadow (any-hit) walks of one pass: ray read
-    4(2N+1) + path 4 + contribution 12 per ray, 8 per node visit, 4 + 4(N^2+N)
-    per simplex test."""
-    return (st["shadow_rays"] * (4 * (2 * n + 1) + 16) + 8 * st["shadow_node_visits"]
+    4(2N+1) + path 4 + contribution 12 per walked ray, 8 per node visit,
+    4 + 4(N^2+N) per simplex test."""
+    return (shadow_walks(st) * (4 * (2 * n + 1) + 16) + 8 * st["shadow_node_visits"]
             + (4 + 4 * (n * n + n)) * st["shadow_simplex_tests"])
 
 
@@ -552,11 +558,11 @@ def main():
     # other one is reported beside it
     layout_name = ["leaf_soa", "simplex_aos"][mesh.record_layout]
     walk_rays = max(cst["closest_rays"] - cst["primary_rays"], 1)
-    sr = max(cst["shadow_rays"], 1)
+    sr = max(shadow_walks(cst), 1)
     per_closest = dict(rays=cst["closest_rays"] - cst["primary_rays"], nodes=cst["node_visits"] / walk_rays,
                        leaves=cst["leaf_visits"] / walk_rays, tests=cst["simplex_tests"] / walk_rays,
                        fp64_frac=cst["retests"] / max(cst["simplex_tests"], 1))
-    per_shadow = dict(rays=cst["shadow_rays"], nodes=cst["shadow_node_visits"] / sr,
+    per_shadow = dict(rays=shadow_walks(cst), nodes=cst["shadow_node_visits"] / sr,
                       leaves=cst["shadow_leaf_visits"] / sr, tests=cst["shadow_simplex_tests"] / sr,
                       fp64_frac=cst["shadow_retests"] / max(cst["shadow_simplex_tests"], 1))
     kernels = {
@@ -584,6 +590,12 @@ def main():
         # the merged walks: one kernel holds all traversal of the bounce and shadow rays
         roofline = line("k_walk")
         roofline["stage_ms_per_step"] = {k: round(v / args.steps, 2) for k, v in stage_ms.items()}
+        if cst["shadow_rays_rasterised"]:
+            # the shadow rays of the primary hits, rasterised from the light (stage "shadow")
+            roofline["shadow_raster"] = dict(rays=cst["shadow_rays_rasterised"],
+                                             ms_per_step=round(stage_ms["shadow"] / args.steps, 3),
+                                             Mrays_s=round(cst["shadow_rays_rasterised"] /
+                                                           (stage_ms["shadow"] / args.steps * 1e-3) / 1e6, 1))
     elif min(kernels[k]["ms"] for k in ("k_closest", "k_shadow")) > 0:
         dominant = max(("k_closest", "k_shadow"), key=lambda name: kernels[name]["ms"])
         roofline = line(dominant)

@@ -436,6 +436,11 @@ typedef struct {
      * two kinds as above */
     double ms_walk;
     int64_t launches_walk;
+    /* shadow rays of the primary hits rasterised from the light (one point
+     * light, FP32 walks over simplex-ordered records): their time is
+     * ms_shadow, they are part of shadow_rays but not of the walks' shadow
+     * traversal counters (filled when nsp_set_counters is on) */
+    int64_t shadow_rays_rasterised;
 } nsp_stage_stats;
 int nsp_last_pass_stats(const nsp_scene* scene, nsp_stage_stats* out);
 /* Enables per-ray traversal counters (node visits, leaf visits, simplex

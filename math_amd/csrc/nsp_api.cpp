@@ -1034,7 +1034,7 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
             st.primary_rays = raster ? scene->pixels * S : 0;
             st.shadow_rays = (int64_t)shadow;
             if (scene->counters) {
-                unsigned long long tc[8];
+                unsigned long long tc[9];
                 NSP_HIP(hipMemcpy(tc, ctx->work.trav_counters.p, sizeof tc, hipMemcpyDeviceToHost));
                 st.node_visits = (int64_t)tc[0];
                 st.leaf_visits = (int64_t)tc[1];
@@ -1044,6 +1044,7 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
                 st.shadow_leaf_visits = (int64_t)tc[5];
                 st.shadow_simplex_tests = (int64_t)tc[6];
                 st.shadow_retests = (int64_t)tc[7];
+                st.shadow_rays_rasterised = (int64_t)tc[8];
             }
             scene->stage = st;
             bool stopped = stop && *stop;

@@ -84,8 +84,9 @@ void shadow_raster_buffers(RenderBuffers& b, int n, int64_t scap);
 // Occlusion of the shadow rays of one depth (queue sq, counters dc); the
 // contributions of the rays that reach the light are added to Ls (and to the
 // direct-light AOV when direct is set), as the shadow walks add them.
+// rays: when not null, the number of rays is added to it.
 void shadow_raster_launch(const ShadowRasterPrep& sp, const MeshDev& m, const TreeArgs& A, ShadowQueue<float> sq,
                           int64_t scap, const unsigned long long* dc, RenderBuffers& b, float* Ls, int64_t cap,
-                          float* direct, hipStream_t s);
+                          float* direct, unsigned long long* rays, hipStream_t s);
 
 }  // namespace nsp

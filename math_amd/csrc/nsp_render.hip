@@ -220,7 +220,7 @@ void pass_reset_state(unsigned long long* totals, long long* batch_idx, int* pas
 void pass_reset(RenderBuffers& b, bool counters, int* pass_dev, int pass, hipStream_t s)
 {
     hipLaunchKernelGGL(k_pass_reset, dim3(1), dim3(64), 0, s, b.totals.as<unsigned long long>(),
-                       b.batch_idx.as<long long>(), counters ? b.trav_counters.as<unsigned long long>() : nullptr, 8,
+                       b.batch_idx.as<long long>(), counters ? b.trav_counters.as<unsigned long long>() : nullptr, 9,
                        pass_dev, pass);
     NSP_HIP(hipGetLastError());
 }
@@ -1017,7 +1017,7 @@ void alloc_render_buffers(RenderBuffers& b, int n, int precision, int64_t cap, i
     b.desc.alloc(sizeof(BatchDesc));
     b.totals.alloc(sizeof(unsigned long long) * 3);
     b.batch_idx.alloc(sizeof(long long));
-    b.trav_counters.alloc(sizeof(unsigned long long) * 8);
+    b.trav_counters.alloc(sizeof(unsigned long long) * 9);
 }
 
 int64_t render_bytes_per_path(int n, int precision, int lights, bool aov, bool raster, bool shadow_raster)
@@ -1030,7 +1030,7 @@ int64_t render_bytes_per_path(int n, int precision, int lights, bool aov, bool r
     const int64_t shadow = 2 * ts * n + ts + 4 + 12 + 4;
     int64_t b = 2 * queue + 2 * shadow * L + 24 + 4 * L + 4 + 4 * RES_STRIDE * (1 + L);
     if (raster) b += 8;
-    if (shadow_raster) b += (4 + 4 + 4 * ((2 * n + 4) / 4 * 4) + 4 + 1) * L;
+    if (shadow_raster) b += (4 + 4 + 4 * ((2 * n + 4) / 4 * 4) + 4 + 2) * L;
     if (aov) b += 4 * aov_channels(n);
     return b;
 }
@@ -1189,7 +1189,8 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
                 // about 0.5 ms for C1's 663,552 footprints even with few rays)
                 const bool sr = rl.sraster != nullptr && depth == 0;
                 if (sr) timed(3, [&] {
-                    shadow_raster_launch(*rl.sraster, *rl.mesh, A, sq, scap, dc, b, Ls, cap, direct, s);
+                    shadow_raster_launch(*rl.sraster, *rl.mesh, A, sq, scap, dc, b, Ls, cap, direct,
+                                         rl.counters ? b.trav_counters.as<unsigned long long>() + 8 : nullptr, s);
                 });
                 timed(6, [&] {
                     if (rl.counters)

@@ -178,13 +178,14 @@ struct RenderBuffers {
     DevBuf batch_list;      // BatchDesc per batch of the pass
     DevBuf batch_idx;       // long long: index of the next batch
     DevBuf totals;          // 3 u64: closest, shadow and probe rays of the pass
-    DevBuf trav_counters;   // 8 u64: traversal counters (closest stage, shadow stage)
+    DevBuf trav_counters;   // 9 u64: traversal counters (closest stage, shadow stage), rasterised shadow rays
     // binned shadow rays of the light-space rasteriser (nsp_raster.hpp)
     DevBuf sr_cell;         // int32 per shadow ray: its cell
     DevBuf sr_rank;         // u32 per shadow ray: its place among its cell's rays
     DevBuf sr_rays;         // per shadow ray in cell order: origin, direction, limit (16-byte padded)
     DevBuf sr_ray;          // int32 per binned ray: its shadow-queue index
     DevBuf sr_occl;         // u8 per binned ray: occluded
+    DevBuf sr_occq;         // u8 per shadow ray in queue order: occluded
 };
 
 // Adds an unoccluded shadow ray's contribution to its path (and, for a

@@ -270,7 +270,8 @@ __global__ void __launch_bounds__(SR_BLOCK) k_sr_scatter(ShadowQueue<float> sq, 
                                                          const int32_t* __restrict__ cell_of,
                                                          const uint32_t* __restrict__ rank,
                                                          const uint32_t* __restrict__ start, float* __restrict__ rays,
-                                                         int32_t* __restrict__ ray_of, uint8_t* __restrict__ occl)
+                                                         int32_t* __restrict__ ray_of, uint8_t* __restrict__ occl,
+                                                         uint8_t* __restrict__ occl_q)
 {
     const int seg = (int)(blockIdx.x % QSEG);
     const int64_t segcap = (int64_t)dc[CNT_SSEGCAP];
@@ -294,6 +295,7 @@ __global__ void __launch_bounds__(SR_BLOCK) k_sr_scatter(ShadowQueue<float> sq, 
         for (int k = 0; k < R / 4; ++k) dst[k] = make_float4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
         ray_of[pos] = (int32_t)i;
         occl[pos] = 0;
+        occl_q[i] = 0;
     }
 }
 
@@ -309,7 +311,8 @@ __global__ void __launch_bounds__(SR_BLOCK) k_sr_raster(TreeArgs A, const int2* 
                                                         const float* __restrict__ planes, SRGrid g,
                                                         const uint32_t* __restrict__ start,
                                                         const float* __restrict__ rays, int64_t scap,
-                                                        uint8_t* __restrict__ occl)
+                                                        uint8_t* __restrict__ occl, const int32_t* __restrict__ ray_of,
+                                                        uint8_t* __restrict__ occl_q)
 {
     constexpr int D = N - 1;
     constexpr int RS = rec32_stride(N);
@@ -413,24 +416,34 @@ __global__ void __launch_bounds__(SR_BLOCK) k_sr_raster(TreeArgs A, const int2* 
             bool nr = false;
             unsigned nret = 0;
             if (fp32_test<N, false>(A, comp, get_sid, o, d, nullptr, nullptr, tmax, -1, &th, &si, &nr, &nret) ==
-                CAND_HIT)
+                CAND_HIT) {
                 occl[pos] = 1;
+                occl_q[ray_of[pos]] = 1;
+            }
         }
     }
 }
 
-// The contributions of the rays that reached the light.
+// The contributions of the rays that reached the light, in queue order (the
+// queue's paths, contributions and the paths' radiance are read in order).
+// Block b takes queue segment b % QSEG.
 template <int N>
-__global__ void __launch_bounds__(256) k_sr_finish(ShadowQueue<float> sq, int64_t scap, int64_t cells,
-                                                   const uint32_t* __restrict__ start,
-                                                   const int32_t* __restrict__ ray_of,
-                                                   const uint8_t* __restrict__ occl, float* __restrict__ Ls,
-                                                   int64_t cap, float* __restrict__ direct)
+__global__ void __launch_bounds__(SR_BLOCK) k_sr_finish(ShadowQueue<float> sq, int64_t scap,
+                                                        const unsigned long long* __restrict__ dc, int64_t cells,
+                                                        const uint32_t* __restrict__ start,
+                                                        const uint8_t* __restrict__ occl_q, float* __restrict__ Ls,
+                                                        int64_t cap, float* __restrict__ direct,
+                                                        unsigned long long* __restrict__ rays)
 {
-    const int64_t total = (int64_t)start[cells];
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < total;
-         p += (int64_t)gridDim.x * blockDim.x)
-        if (!occl[p]) shadow_unoccluded<N, float>(sq, scap, ray_of[p], Ls, cap, direct);
+    if (rays && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(rays, (unsigned long long)start[cells]);
+    const int seg = (int)(blockIdx.x % QSEG);
+    const int64_t segcap = (int64_t)dc[CNT_SSEGCAP];
+    const int64_t n = (int64_t)dc[cnt_shadowseg(seg)];
+    const int64_t stride = (int64_t)(gridDim.x / QSEG) * blockDim.x;
+    for (int64_t j = (int64_t)(blockIdx.x / QSEG) * blockDim.x + threadIdx.x; j < n; j += stride) {
+        const int64_t i = seg * segcap + j;
+        if (!occl_q[i]) shadow_unoccluded<N, float>(sq, scap, i, Ls, cap, direct);
+    }
 }
 
 template <int N>
@@ -584,16 +597,17 @@ void prepare_t(ShadowRasterPrep& sp, const MeshDev& m, const double* light, hipS
         const int64_t chunks = (vol + SR_CELLS_PER_UNIT - 1) / SR_CELLS_PER_UNIT;
         for (int64_t c = 0; c < chunks; ++c) hu.push_back(int2{(int)f, (int)c});
     }
-    // units in the order of their boxes' low corners (last axis major), so that
-    // the simplices that cover a region of cells run together on one XCD and
-    // its rays stay in that XCD's L2 between them
+    // units in the Morton order of their boxes' low corners, so that the
+    // simplices that cover a compact region of cells run together on one XCD
+    // and the region's rays stay in that XCD's L2 between them
     {
-        std::vector<std::pair<int64_t, int64_t>> key(hu.size());
+        std::vector<std::pair<uint64_t, int64_t>> key(hu.size());
         for (size_t k = 0; k < hu.size(); ++k) {
             const int32_t* bx = hb.data() + (size_t)hu[k].x * 2 * D;
-            int64_t corner = 0;
-            for (int i = D - 1; i >= 0; --i) corner = corner * g.grid[i] + bx[i];
-            key[k] = {corner, (int64_t)k};
+            uint64_t code = 0;
+            for (int bit = 11; bit >= 0; --bit)
+                for (int i = D - 1; i >= 0; --i) code = (code << 1) | (uint64_t)((bx[i] >> bit) & 1);
+            key[k] = {code, (int64_t)k};
         }
         std::sort(key.begin(), key.end());
         std::vector<int2> sorted(hu.size());
@@ -624,7 +638,8 @@ void prepare_t(ShadowRasterPrep& sp, const MeshDev& m, const double* light, hipS
 
 template <int N>
 void launch_t(const ShadowRasterPrep& sp, const TreeArgs& A, ShadowQueue<float> sq, int64_t scap,
-              const unsigned long long* dc, RenderBuffers& b, float* Ls, int64_t cap, float* direct, hipStream_t s)
+              const unsigned long long* dc, RenderBuffers& b, float* Ls, int64_t cap, float* direct,
+              unsigned long long* rays, hipStream_t s)
 {
     const SRGrid& g = sp.grid;
     const unsigned gq = 4096;  // a multiple of QSEG
@@ -643,14 +658,15 @@ void launch_t(const ShadowRasterPrep& sp, const TreeArgs& A, ShadowQueue<float> 
     NSP_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_sr_scatter<N>, dim3(gq), dim3(SR_BLOCK), 0, s, sq, scap, dc, b.sr_cell.as<int32_t>(),
                        b.sr_rank.as<uint32_t>(), sp.start.as<uint32_t>(), b.sr_rays.as<float>(),
-                       b.sr_ray.as<int32_t>(), b.sr_occl.as<uint8_t>());
+                       b.sr_ray.as<int32_t>(), b.sr_occl.as<uint8_t>(), b.sr_occq.as<uint8_t>());
     NSP_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_sr_raster<N>, dim3(2048), dim3(SR_BLOCK), 0, s, A, sp.unit.as<int2>(), sp.units,
                        sp.box.as<int32_t>(), sp.planes.as<float>(), g, sp.start.as<uint32_t>(),
-                       b.sr_rays.as<float>(), scap, b.sr_occl.as<uint8_t>());
+                       b.sr_rays.as<float>(), scap, b.sr_occl.as<uint8_t>(), b.sr_ray.as<int32_t>(),
+                       b.sr_occq.as<uint8_t>());
     NSP_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_sr_finish<N>, dim3(2048), dim3(256), 0, s, sq, scap, sp.cells, sp.start.as<uint32_t>(),
-                       b.sr_ray.as<int32_t>(), b.sr_occl.as<uint8_t>(), Ls, cap, direct);
+    hipLaunchKernelGGL(k_sr_finish<N>, dim3(gq), dim3(SR_BLOCK), 0, s, sq, scap, dc, sp.cells,
+                       sp.start.as<uint32_t>(), b.sr_occq.as<uint8_t>(), Ls, cap, direct, rays);
     NSP_HIP(hipGetLastError());
 }
 
@@ -684,17 +700,18 @@ void shadow_raster_buffers(RenderBuffers& b, int n, int64_t scap)
     b.sr_rays.alloc(sizeof(float) * sr_record(n) * scap);
     b.sr_ray.alloc(sizeof(int32_t) * scap);
     b.sr_occl.alloc(scap);
+    b.sr_occq.alloc(scap);
 }
 
 void shadow_raster_launch(const ShadowRasterPrep& sp, const MeshDev& m, const TreeArgs& A, ShadowQueue<float> sq,
                           int64_t scap, const unsigned long long* dc, RenderBuffers& b, float* Ls, int64_t cap,
-                          float* direct, hipStream_t s)
+                          float* direct, unsigned long long* rays, hipStream_t s)
 {
     switch (m.n) {
-    case 3: launch_t<3>(sp, A, sq, scap, dc, b, Ls, cap, direct, s); break;
-    case 4: launch_t<4>(sp, A, sq, scap, dc, b, Ls, cap, direct, s); break;
-    case 5: launch_t<5>(sp, A, sq, scap, dc, b, Ls, cap, direct, s); break;
-    case 6: launch_t<6>(sp, A, sq, scap, dc, b, Ls, cap, direct, s); break;
+    case 3: launch_t<3>(sp, A, sq, scap, dc, b, Ls, cap, direct, rays, s); break;
+    case 4: launch_t<4>(sp, A, sq, scap, dc, b, Ls, cap, direct, rays, s); break;
+    case 5: launch_t<5>(sp, A, sq, scap, dc, b, Ls, cap, direct, rays, s); break;
+    case 6: launch_t<6>(sp, A, sq, scap, dc, b, Ls, cap, direct, rays, s); break;
     default: raise(NSP_E_INVALID, "dimension not supported");
     }
 }

@@ -109,7 +109,7 @@ class StageStats(C.Structure):
                 ("simplex_tests", C.c_int64), ("retests", C.c_int64), ("shadow_node_visits", C.c_int64),
                 ("shadow_leaf_visits", C.c_int64), ("shadow_simplex_tests", C.c_int64), ("shadow_retests", C.c_int64),
                 ("ms_raster", C.c_double), ("launches_raster", C.c_int64), ("primary_rays", C.c_int64),
-                ("ms_walk", C.c_double), ("launches_walk", C.c_int64)]
+                ("ms_walk", C.c_double), ("launches_walk", C.c_int64), ("shadow_rays_rasterised", C.c_int64)]
 
 
 # every symbol of include/nsp.h
