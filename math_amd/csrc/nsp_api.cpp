@@ -1176,6 +1176,7 @@ struct BatchScene {
     SceneK<float> kf;
     int64_t batch_pixels = 0, batches = 0;
     int lane = 0;
+    bool sraster = false;  // the primary hits' shadow rays rasterised from the light
 };
 
 struct nsp_batch {
@@ -1220,7 +1221,8 @@ void fingerprint(const RenderBuffers& b, std::vector<int64_t>& out)
     for (const auto& q : b.sq)
         for (const DevBuf& d : q) add(d);
     for (const DevBuf* d : {&b.L, &b.Ls, &b.redo, &b.res, &b.keys, &b.redo_c, &b.res_c, &b.aov, &b.counters, &b.desc,
-                            &b.batch_list, &b.batch_idx, &b.totals, &b.trav_counters})
+                            &b.batch_list, &b.batch_idx, &b.totals, &b.trav_counters, &b.sr_cell, &b.sr_rank,
+                            &b.sr_rays, &b.sr_ray, &b.sr_occl, &b.sr_occq})
         add(*d);
     out.push_back(b.cap);
     out.push_back(b.n);
@@ -1233,13 +1235,16 @@ void fingerprint(const RenderBuffers& b, std::vector<int64_t>& out)
 void fingerprint(const nsp_scene& sc, const BatchScene& bs, std::vector<int64_t>& out)
 {
     for (const DevBuf* d : {&bs.list, &bs.idx, &bs.totals, &sc.pass_dev, &sc.pix_sum, &sc.pix_cnt, &sc.raster.box,
-                            &sc.raster.unit, &sc.raster.ustart}) {
+                            &sc.raster.unit, &sc.raster.ustart, &sc.raster.planes, &sc.sraster.box,
+                            &sc.sraster.planes, &sc.sraster.unit, &sc.sraster.count, &sc.sraster.start,
+                            &sc.sraster.tiles}) {
         out.push_back((int64_t)(uintptr_t)d->p);
         out.push_back((int64_t)d->bytes);
     }
     const RasterPrep& r = sc.raster;
     for (int64_t v : {(int64_t)r.n, (int64_t)r.S, r.F, r.ppu, (int64_t)r.hmax, (int64_t)r.last, r.plane, r.units,
-                      (int64_t)r.small, bs.batch_pixels, bs.batches, (int64_t)bs.lane, (int64_t)(uintptr_t)&sc.mesh->m})
+                      (int64_t)r.small, bs.batch_pixels, bs.batches, (int64_t)bs.lane, (int64_t)(uintptr_t)&sc.mesh->m,
+                      (int64_t)bs.sraster, sc.sraster.cells, sc.sraster.units})
         out.push_back(v);
     uint64_t h = 1469598103934665603ull;
     const unsigned char* c = reinterpret_cast<const unsigned char*>(&bs.kf);
@@ -1342,10 +1347,13 @@ int nsp_batch_paint(nsp_batch* b, const nsp_paint_params* params, const volatile
             const int S = bs.kf.S;
             cost[i] = (double)(sc->pixels * S) * std::log2(2.0 + (double)sc->mesh->m.F);
             const bool raster = raster_mode && sc->proj.kind == NSP_PROJECTOR_PARALLEL;
+            // as in nsp_paint; shadow_raster_prepare may still turn it off below
+            bs.sraster = raster_mode && sc->lights.size() == 1 && sc->lights[0].kind == NSP_LIGHT_POINT &&
+                         sc->shapes.empty() && sc->mesh->m.layout == NSP_LAYOUT_SIMPLEX_AOS;
             int64_t want = p.batch_paths;
             if (want <= 0) {
-                const int64_t per =
-                    render_bytes_per_path(sc->n, NSP_PRECISION_FP32, (int)sc->lights.size(), false, raster);
+                const int64_t per = render_bytes_per_path(sc->n, NSP_PRECISION_FP32, (int)sc->lights.size(), false,
+                                                          raster, bs.sraster);
                 want = (int64_t)1 << 27;
                 while (want > ((int64_t)1 << 20) && want * per > budget) want >>= 1;
             }
@@ -1365,15 +1373,18 @@ int nsp_batch_paint(nsp_batch* b, const nsp_paint_params* params, const volatile
         }
         // workspaces (they only grow), raster set-up, batch lists
         for (int l = 0; l < lanes; ++l) {
-            int64_t cap = 1;
+            int64_t cap = 1, srcap = 0;
             int nmax = 3, lmax = 1;
             for (int i : lane_scenes[l]) {
-                cap = std::max(cap, b->per[i].batch_pixels * b->per[i].kf.S);
+                const int64_t c = b->per[i].batch_pixels * b->per[i].kf.S;
+                cap = std::max(cap, c);
+                if (b->per[i].sraster) srcap = std::max(srcap, c);
                 nmax = std::max(nmax, b->scenes[i]->n);
                 lmax = std::max(lmax, (int)b->scenes[i]->lights.size());
             }
             alloc_render_buffers(lane_work(l), nmax, NSP_PRECISION_FP32, cap, p.max_depth, lmax);
             if (raster_mode) lane_work(l).keys.alloc(sizeof(unsigned long long) * cap);
+            if (srcap > 0) shadow_raster_buffers(lane_work(l), nmax, srcap);
         }
         for (int i = 0; i < ns; ++i) {
             nsp_scene* sc = b->scenes[i];
@@ -1388,10 +1399,11 @@ int nsp_batch_paint(nsp_batch* b, const nsp_paint_params* params, const volatile
             bs.totals.alloc(sizeof(unsigned long long) * 3);
             NSP_HIP(hipMemcpyAsync(bs.list.p, descs.data(), sizeof(BatchDesc) * descs.size(), hipMemcpyHostToDevice,
                                    s));
-            if (raster_mode && sc->proj.kind == NSP_PROJECTOR_PARALLEL) {
+            if (raster_mode && (sc->proj.kind == NSP_PROJECTOR_PARALLEL || bs.sraster)) {
                 SceneK<double> kd;
                 fill_scene<double>(*sc, p, 0, kd);
-                raster_prepare(sc->raster, sc->mesh->m, kd, S, s);
+                if (sc->proj.kind == NSP_PROJECTOR_PARALLEL) raster_prepare(sc->raster, sc->mesh->m, kd, S, s);
+                if (bs.sraster) bs.sraster = shadow_raster_prepare(sc->sraster, sc->mesh->m, kd.light_pos[0], s);
             }
         }
         // the pass index of every scene, written before each launch
@@ -1422,6 +1434,7 @@ int nsp_batch_paint(nsp_batch* b, const nsp_paint_params* params, const volatile
             rl.counters = false;
             rl.marks = nullptr;
             rl.raster = raster_mode && sc->proj.kind == NSP_PROJECTOR_PARALLEL ? &sc->raster : nullptr;
+            rl.sraster = bs.sraster ? &sc->sraster : nullptr;
             rl.batch_list = bs.list.as<BatchDesc>();
             rl.batch_idx = bs.idx.as<long long>();
             rl.totals = bs.totals.as<unsigned long long>();
