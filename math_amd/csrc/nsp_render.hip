@@ -1142,6 +1142,7 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
     timed(0, [&] {
         hipLaunchKernelGGL((k_raygen<N, T>), dim3(g), dim3(TRACE_BLOCK), 0, s, sc, desc, qa, cap, L, Ls);
     });
+    bool sr_pending = false;  // the shadow raster of depth 0 runs on the side stream
     for (int depth = 0; depth <= rl.max_depth; ++depth) {
         RayQueue<T>& qin = (depth & 1) ? qb : qa;
         RayQueue<T>& qout = (depth & 1) ? qa : qb;
@@ -1188,10 +1189,28 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
                 // the fewer of later depths are walked (a rasterising launch costs
                 // about 0.5 ms for C1's 663,552 footprints even with few rays)
                 const bool sr = rl.sraster != nullptr && depth == 0;
-                if (sr) timed(3, [&] {
-                    shadow_raster_launch(*rl.sraster, *rl.mesh, A, sq, scap, dc, b, Ls, cap, direct,
-                                         rl.counters ? b.trav_counters.as<unsigned long long>() + 8 : nullptr, s);
-                });
+                // a walk launch that takes shadow rays adds to Ls: it waits for the
+                // shadow raster of depth 0, which ran on the side stream beside the
+                // depth-0 walk and shade(1) (neither reads or writes Ls)
+                if (sr_pending && !sr) {
+                    NSP_HIP(hipStreamWaitEvent(s, b.join_ev, 0));
+                    sr_pending = false;
+                }
+                if (sr) {
+                    if (rl.side) {
+                        NSP_HIP(hipEventRecord(b.fork_ev, s));
+                        NSP_HIP(hipStreamWaitEvent(ss, b.fork_ev, 0));
+                    }
+                    timed_on(ss, 3, [&] {
+                        shadow_raster_launch(*rl.sraster, *rl.mesh, A, sq, scap, dc, b, Ls, cap, direct,
+                                             rl.counters ? b.trav_counters.as<unsigned long long>() + 8 : nullptr,
+                                             ss);
+                    });
+                    if (rl.side) {
+                        NSP_HIP(hipEventRecord(b.join_ev, ss));
+                        sr_pending = true;
+                    }
+                }
                 timed(6, [&] {
                     if (rl.counters)
                         hipLaunchKernelGGL((k_walk<N, true>), dim3(gw), dim3(TRACE_BLOCK), 0, s, A, As.counters, qout,
@@ -1240,6 +1259,7 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
         NSP_HIP(hipEventRecord(b.join_ev, ss));
         NSP_HIP(hipStreamWaitEvent(s, b.join_ev, 0));
     }
+    if (sr_pending) NSP_HIP(hipStreamWaitEvent(s, b.join_ev, 0));
     timed(4, [&] {
         const int64_t max_pix = cap / std::max(1, sc.S);
         if (sc.S <= ACC_MAX_S)
