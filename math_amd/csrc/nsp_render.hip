@@ -225,10 +225,14 @@ void pass_reset(RenderBuffers& b, bool counters, int* pass_dev, int pass, hipStr
     NSP_HIP(hipGetLastError());
 }
 
+// Primary rays of the batch.  slim (primary visibility rasterised, parallel
+// projection): only the origins, which the rasteriser reads, and the cleared
+// shadow sums; shade of depth 0 then takes the path index from the queue
+// position, the direction from the projector, beta and level 1, and stores L.
 template <int N, typename T>
 __global__ void __launch_bounds__(TRACE_BLOCK) k_raygen(SceneK<T> sc, const BatchDesc* __restrict__ desc,
                                                         RayQueue<T> q, int64_t cap, float* __restrict__ L,
-                                                        float* __restrict__ Ls)
+                                                        float* __restrict__ Ls, int slim)
 {
     const int64_t paths = desc->paths;
     const int64_t pixel0 = desc->pixel0;
@@ -237,10 +241,13 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_raygen(SceneK<T> sc, const Batc
         T o[N], d[N];
         primary_ray<N, T>(sc, pixel0 + p / sc.S, (int)(p % sc.S), o, d);
 #pragma unroll
-        for (int j = 0; j < N; ++j) {
-            q.org[(int64_t)j * cap + p] = o[j];
-            q.dir[(int64_t)j * cap + p] = d[j];
-        }
+        for (int j = 0; j < N; ++j) q.org[(int64_t)j * cap + p] = o[j];
+        Ls[p] = 0.0f;
+        Ls[cap + p] = 0.0f;
+        Ls[2 * cap + p] = 0.0f;
+        if (slim) continue;
+#pragma unroll
+        for (int j = 0; j < N; ++j) q.dir[(int64_t)j * cap + p] = d[j];
         q.path[p] = (int32_t)p;
         q.beta[p] = 1.0f;
         q.beta[cap + p] = 1.0f;
@@ -249,9 +256,6 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_raygen(SceneK<T> sc, const Batc
         L[p] = 0.0f;
         L[cap + p] = 0.0f;
         L[2 * cap + p] = 0.0f;
-        Ls[p] = 0.0f;
-        Ls[cap + p] = 0.0f;
-        Ls[2 * cap + p] = 0.0f;
     }
 }
 
@@ -259,7 +263,7 @@ template <int N, typename T, bool SMOOTH, bool SHAPES, bool MULTI>
 __global__ void __launch_bounds__(TRACE_BLOCK)
     k_shade(SceneK<T> sc, TreeArgs A, MeshShadeArgs M, const BatchDesc* __restrict__ desc, int depth,
             RayQueue<T> qin, RayQueue<T> qout, ShadowQueue<T> sq, int64_t cap, unsigned long long* __restrict__ cnt,
-            float* __restrict__ L, float* __restrict__ aov)
+            float* __restrict__ L, float* __restrict__ aov, int prim)
 {
     __shared__ int2 stack[stack_levels(N) * TRACE_BLOCK];
     int2* st = stack + threadIdx.x;
@@ -289,20 +293,43 @@ __global__ void __launch_bounds__(TRACE_BLOCK)
         out.shadow = false;
         int32_t path = 0;
         if (valid) {
-            path = qin.path[i];
-            float beta[3] = {qin.beta[i], qin.beta[cap + i], qin.beta[2 * cap + i]};
+            const int hidx = qin.hit[i];
+            float beta[3], level;
             T o[N], d[N];
+            if (prim) {
+                // the primary queue of a slim raygen: the queue position is the
+                // path index; a miss without shapes reads no ray
+                path = (int32_t)i;
+                beta[0] = beta[1] = beta[2] = 1.0f;
+                level = 1.0f;
+                const bool ray = SHAPES || hidx >= 0;
 #pragma unroll
-            for (int k = 0; k < N; ++k) {
-                o[k] = qin.org[(int64_t)k * cap + i];
-                d[k] = qin.dir[(int64_t)k * cap + i];
+                for (int k = 0; k < N; ++k) {
+                    o[k] = ray ? qin.org[(int64_t)k * cap + i] : (T)0;
+                    d[k] = sc.cam_dir[k];
+                }
+            } else {
+                path = qin.path[i];
+                beta[0] = qin.beta[i];
+                beta[1] = qin.beta[cap + i];
+                beta[2] = qin.beta[2 * cap + i];
+                level = qin.level[i];
+#pragma unroll
+                for (int k = 0; k < N; ++k) {
+                    o[k] = qin.org[(int64_t)k * cap + i];
+                    d[k] = qin.dir[(int64_t)k * cap + i];
+                }
             }
             const int64_t gp = pixel0 * sc.S + path;
-            shade_one<N, T, SMOOTH, SHAPES, MULTI>(sc, A, M, depth, gp / sc.S, (int)(gp % sc.S), o, d, qin.hit[i],
-                                            qin.hit_t[i], beta, qin.level[i], st, probe_count, aov && depth == 0,
-                                            out);
+            shade_one<N, T, SMOOTH, SHAPES, MULTI>(sc, A, M, depth, gp / sc.S, (int)(gp % sc.S), o, d, hidx,
+                                            qin.hit_t[i], beta, level, st, probe_count, aov && depth == 0, out);
+            if (prim) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) L[c * cap + path] += out.add[c];
+                for (int c = 0; c < 3; ++c) L[c * cap + path] = out.add[c];
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) L[c * cap + path] += out.add[c];
+            }
             if (aov && depth == 0) {
 #pragma unroll
                 for (int c = 0; c < 3 + N; ++c) aov[c * cap + path] = out.av[c];
@@ -1139,8 +1166,10 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(256), 0, s, cnt, rl.max_depth + 2, list, bidx,
                        b.desc.as<BatchDesc>(), cap / QSEG, cap / QSEG * std::max(1, sc.light_count));
     NSP_HIP(hipGetLastError());
+    // rasterised primaries: raygen writes only what the rasteriser and shade(0) read
+    const bool slim = sizeof(T) == 4 && rl.raster != nullptr;
     timed(0, [&] {
-        hipLaunchKernelGGL((k_raygen<N, T>), dim3(g), dim3(TRACE_BLOCK), 0, s, sc, desc, qa, cap, L, Ls);
+        hipLaunchKernelGGL((k_raygen<N, T>), dim3(g), dim3(TRACE_BLOCK), 0, s, sc, desc, qa, cap, L, Ls, (int)slim);
     });
     bool sr_pending = false;  // the shadow raster of depth 0 runs on the side stream
     for (int depth = 0; depth <= rl.max_depth; ++depth) {
@@ -1178,7 +1207,7 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
         if (!merged && rl.side && depth >= 2) NSP_HIP(hipStreamWaitEvent(s, b.done_ev[depth & 1], 0));
         timed(2, [&] {
             hipLaunchKernelGGL(kshade, dim3(g), dim3(TRACE_BLOCK), 0, s, sc, A, M, desc, depth, qin, qout, sq, cap,
-                               cnt, L, aov);
+                               cnt, L, aov, (int)(slim && depth == 0));
         });
         if constexpr (sizeof(T) == 4) {
             if (merged) {
