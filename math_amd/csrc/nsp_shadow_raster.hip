@@ -34,7 +34,7 @@ constexpr int SR_MAX_TILES = 4096;  // at most 2^24 cells
 
 // Floats per binned shadow ray: origin, direction and limit, padded to whole
 // 16-byte words (one to four 16-byte loads per ray).
-__host__ __device__ constexpr int sr_record(int n) { return (2 * n + 1 + 3) / 4 * 4; }
+__host__ __device__ constexpr int sr_record(int n) { return 2 * n + 1; }
 
 // Screen coordinates of a point x as the light sees it: s_i = e_i.(x - light) /
 // a.(x - light), a the axis from the light to the mesh centre, e_i an
@@ -288,11 +288,9 @@ __global__ void __launch_bounds__(SR_BLOCK) k_sr_scatter(ShadowQueue<float> sq, 
             r[N + k] = sq.dir[(int64_t)k * scap + i];
         }
         r[2 * N] = sq.tmax[i];
+        float* dst = rays + pos * R;
 #pragma unroll
-        for (int k = 2 * N + 1; k < R; ++k) r[k] = 0.0f;
-        float4* dst = reinterpret_cast<float4*>(rays + pos * R);
-#pragma unroll
-        for (int k = 0; k < R / 4; ++k) dst[k] = make_float4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
+        for (int k = 0; k < R; ++k) dst[k] = r[k];
         ray_of[pos] = (int32_t)i;
         occl[pos] = 0;
         occl_q[i] = 0;
@@ -395,15 +393,9 @@ __global__ void __launch_bounds__(SR_BLOCK) k_sr_raster(TreeArgs A, const int2* 
             if (j >= total || occl[pos]) continue;
             constexpr int R = sr_record(N);
             float r[R];
-            const float4* src = reinterpret_cast<const float4*>(rays + pos * R);
+            const float* src = rays + pos * R;
 #pragma unroll
-            for (int k = 0; k < R / 4; ++k) {
-                const float4 v = src[k];
-                r[4 * k] = v.x;
-                r[4 * k + 1] = v.y;
-                r[4 * k + 2] = v.z;
-                r[4 * k + 3] = v.w;
-            }
+            for (int k = 0; k < R; ++k) r[k] = src[k];
             float o[N], d[N];
 #pragma unroll
             for (int k = 0; k < N; ++k) {
