@@ -1222,7 +1222,7 @@ void fingerprint(const RenderBuffers& b, std::vector<int64_t>& out)
         for (const DevBuf& d : q) add(d);
     for (const DevBuf* d : {&b.L, &b.Ls, &b.redo, &b.res, &b.keys, &b.redo_c, &b.res_c, &b.aov, &b.counters, &b.desc,
                             &b.batch_list, &b.batch_idx, &b.totals, &b.trav_counters, &b.sr_cell, &b.sr_rank,
-                            &b.sr_rays, &b.sr_ray, &b.sr_occl, &b.sr_occq})
+                            &b.sr_rays, &b.sr_ray, &b.sr_occq})
         add(*d);
     out.push_back(b.cap);
     out.push_back(b.n);

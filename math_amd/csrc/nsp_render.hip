@@ -1057,7 +1057,7 @@ int64_t render_bytes_per_path(int n, int precision, int lights, bool aov, bool r
     const int64_t shadow = 2 * ts * n + ts + 4 + 12 + 4;
     int64_t b = 2 * queue + 2 * shadow * L + 24 + 4 * L + 4 + 4 * RES_STRIDE * (1 + L);
     if (raster) b += 8;
-    if (shadow_raster) b += (4 + 4 + 4 * ((2 * n + 4) / 4 * 4) + 4 + 2) * L;
+    if (shadow_raster) b += (4 + 4 + 4 * (2 * n + 1) + 4 + 1) * L;
     if (aov) b += 4 * aov_channels(n);
     return b;
 }

@@ -182,9 +182,8 @@ struct RenderBuffers {
     // binned shadow rays of the light-space rasteriser (nsp_raster.hpp)
     DevBuf sr_cell;         // int32 per shadow ray: its cell
     DevBuf sr_rank;         // u32 per shadow ray: its place among its cell's rays
-    DevBuf sr_rays;         // per shadow ray in cell order: origin, direction, limit (16-byte padded)
+    DevBuf sr_rays;         // per shadow ray in cell order: origin, direction, limit (negative once occluded)
     DevBuf sr_ray;          // int32 per binned ray: its shadow-queue index
-    DevBuf sr_occl;         // u8 per binned ray: occluded
     DevBuf sr_occq;         // u8 per shadow ray in queue order: occluded
 };
 

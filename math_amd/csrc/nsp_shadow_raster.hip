@@ -270,8 +270,7 @@ __global__ void __launch_bounds__(SR_BLOCK) k_sr_scatter(ShadowQueue<float> sq, 
                                                          const int32_t* __restrict__ cell_of,
                                                          const uint32_t* __restrict__ rank,
                                                          const uint32_t* __restrict__ start, float* __restrict__ rays,
-                                                         int32_t* __restrict__ ray_of, uint8_t* __restrict__ occl,
-                                                         uint8_t* __restrict__ occl_q)
+                                                         int32_t* __restrict__ ray_of, uint8_t* __restrict__ occl_q)
 {
     const int seg = (int)(blockIdx.x % QSEG);
     const int64_t segcap = (int64_t)dc[CNT_SSEGCAP];
@@ -292,7 +291,6 @@ __global__ void __launch_bounds__(SR_BLOCK) k_sr_scatter(ShadowQueue<float> sq, 
 #pragma unroll
         for (int k = 0; k < R; ++k) dst[k] = r[k];
         ray_of[pos] = (int32_t)i;
-        occl[pos] = 0;
         occl_q[i] = 0;
     }
 }
@@ -302,14 +300,16 @@ __global__ void __launch_bounds__(SR_BLOCK) k_sr_scatter(ShadowQueue<float> sq, 
 // planes, and every shadow ray of the kept cells gets the any-hit test the
 // shadow walk applies to a leaf candidate (FP32 with the FP64 re-test inline).
 // The rays of the kept cells are dealt 64 at a time: a wave scan of the cells'
-// ray counts, and a lane finds its cell by a binary search over the scan.
+// ray counts, and a lane finds its cell by a binary search over the scan.  A
+// hit makes the ray's limit negative in its record, so the record load that
+// brings a ray to a later simplex also tells it to skip the test.
 template <int N>
 __global__ void __launch_bounds__(SR_BLOCK) k_sr_raster(TreeArgs A, const int2* __restrict__ unit, int64_t units,
                                                         const int32_t* __restrict__ box,
                                                         const float* __restrict__ planes, SRGrid g,
                                                         const uint32_t* __restrict__ start,
-                                                        const float* __restrict__ rays, int64_t scap,
-                                                        uint8_t* __restrict__ occl, const int32_t* __restrict__ ray_of,
+                                                        float* __restrict__ rays, int64_t scap,
+                                                        const int32_t* __restrict__ ray_of,
                                                         uint8_t* __restrict__ occl_q)
 {
     constexpr int D = N - 1;
@@ -390,7 +390,7 @@ __global__ void __launch_bounds__(SR_BLOCK) k_sr_raster(TreeArgs A, const int2* 
             }
             const int ow = owner < 64 ? owner : 63;
             const int64_t pos = (int64_t)j + __shfl(delta, ow);
-            if (j >= total || occl[pos]) continue;
+            if (j >= total) continue;
             constexpr int R = sr_record(N);
             float r[R];
             const float* src = rays + pos * R;
@@ -402,14 +402,17 @@ __global__ void __launch_bounds__(SR_BLOCK) k_sr_raster(TreeArgs A, const int2* 
                 o[k] = r[k];
                 d[k] = r[N + k];
             }
+            // an occluded ray's record carries a negative limit (no hit lies
+            // below a limit of zero, so the test would find none either)
             const float tmax = r[2 * N];
+            if (!(tmax > 0.0f)) continue;
             double th;
             int si;
             bool nr = false;
             unsigned nret = 0;
             if (fp32_test<N, false>(A, comp, get_sid, o, d, nullptr, nullptr, tmax, -1, &th, &si, &nr, &nret) ==
                 CAND_HIT) {
-                occl[pos] = 1;
+                rays[pos * R + 2 * N] = -1.0f;
                 occl_q[ray_of[pos]] = 1;
             }
         }
@@ -650,11 +653,11 @@ void launch_t(const ShadowRasterPrep& sp, const TreeArgs& A, ShadowQueue<float> 
     NSP_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_sr_scatter<N>, dim3(gq), dim3(SR_BLOCK), 0, s, sq, scap, dc, b.sr_cell.as<int32_t>(),
                        b.sr_rank.as<uint32_t>(), sp.start.as<uint32_t>(), b.sr_rays.as<float>(),
-                       b.sr_ray.as<int32_t>(), b.sr_occl.as<uint8_t>(), b.sr_occq.as<uint8_t>());
+                       b.sr_ray.as<int32_t>(), b.sr_occq.as<uint8_t>());
     NSP_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_sr_raster<N>, dim3(2048), dim3(SR_BLOCK), 0, s, A, sp.unit.as<int2>(), sp.units,
                        sp.box.as<int32_t>(), sp.planes.as<float>(), g, sp.start.as<uint32_t>(),
-                       b.sr_rays.as<float>(), scap, b.sr_occl.as<uint8_t>(), b.sr_ray.as<int32_t>(),
+                       b.sr_rays.as<float>(), scap, b.sr_ray.as<int32_t>(),
                        b.sr_occq.as<uint8_t>());
     NSP_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_sr_finish<N>, dim3(gq), dim3(SR_BLOCK), 0, s, sq, scap, dc, sp.cells,
@@ -691,7 +694,6 @@ void shadow_raster_buffers(RenderBuffers& b, int n, int64_t scap)
     b.sr_rank.alloc(sizeof(uint32_t) * scap);
     b.sr_rays.alloc(sizeof(float) * sr_record(n) * scap);
     b.sr_ray.alloc(sizeof(int32_t) * scap);
-    b.sr_occl.alloc(scap);
     b.sr_occq.alloc(scap);
 }
 
