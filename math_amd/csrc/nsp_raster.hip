@@ -197,9 +197,14 @@ __global__ void __launch_bounds__(RASTER_BLOCK) k_raster(TreeArgs A, const int32
     const bool small_s = (int64_t)S * 128 < (int64_t(1) << 22);
     const float margin = (float)BOX_MARGIN;
     // consecutive units (neighbouring boxes) on one XCD, so their samples share its L2
-    for (int64_t u = u0 + (int64_t)xcd_block() * (RASTER_BLOCK / 64) + (threadIdx.x >> 6); u < u1; u += waves) {
-        const int2 un = unit[u];
-        const int sid = un.x;
+    // the unit is the wave's: its simplex's box, planes and record are read with
+    // scalar loads, and the next unit's index is loaded during this one
+    int64_t u = u0 + (int64_t)xcd_block() * (RASTER_BLOCK / 64) + (threadIdx.x >> 6);
+    int2 un_next = u < u1 ? unit[u] : int2{0, 0};
+    for (; u < u1; u += waves) {
+        const int sid = __builtin_amdgcn_readfirstlane(un_next.x);
+        const int chunk = __builtin_amdgcn_readfirstlane(un_next.y);
+        if (u + waves < u1) un_next = unit[u + waves];
         int lo[D], ext[D];
         float inv_ext[D];
         int64_t vol = 1;
@@ -225,7 +230,7 @@ __global__ void __launch_bounds__(RASTER_BLOCK) k_raster(TreeArgs A, const int32
         }
         auto comp = [&](int c) -> float { return rec[c]; };
         auto get_sid = [&]() -> int { return sid; };
-        const int64_t b0 = (int64_t)un.y * rd.ppu, b1 = std::min<int64_t>(vol, b0 + rd.ppu);
+        const int64_t b0 = (int64_t)chunk * rd.ppu, b1 = std::min<int64_t>(vol, b0 + rd.ppu);
         for (int64_t g = b0; g < b1; g += 64) {
             // this lane's pixel of the group
             const int64_t pb = g + lane;

@@ -317,12 +317,14 @@ __global__ void __launch_bounds__(SR_BLOCK) k_sr_raster(TreeArgs A, const int2* 
     const int lane = (int)(threadIdx.x & 63);
     const int64_t waves = (int64_t)gridDim.x * (SR_BLOCK / 64);
     const float margin = (float)SR_MARGIN;
-    for (int64_t u = (int64_t)xcd_block() * (SR_BLOCK / 64) + (threadIdx.x >> 6); u < units; u += waves) {
-        // the unit is the wave's: its simplex's box, planes and record are read
-        // with scalar loads
-        const int2 un = unit[u];
-        const int sid = __builtin_amdgcn_readfirstlane(un.x);
-        const int chunk = __builtin_amdgcn_readfirstlane(un.y);
+    // the unit is the wave's: its simplex's box, planes and record are read with
+    // scalar loads, and the next unit's index is loaded during this one
+    int64_t u = (int64_t)xcd_block() * (SR_BLOCK / 64) + (threadIdx.x >> 6);
+    int2 un_next = u < units ? unit[u] : int2{0, 0};
+    for (; u < units; u += waves) {
+        const int sid = __builtin_amdgcn_readfirstlane(un_next.x);
+        const int chunk = __builtin_amdgcn_readfirstlane(un_next.y);
+        if (u + waves < units) un_next = unit[u + waves];
         int lo[D], ext[D];
         int vol = 1;
 #pragma unroll
@@ -334,6 +336,8 @@ __global__ void __launch_bounds__(SR_BLOCK) k_sr_raster(TreeArgs A, const int2* 
         float pl[N * N];
 #pragma unroll
         for (int k = 0; k < N * N; ++k) pl[k] = planes[(int64_t)sid * N * N + k];
+        float rec[RS];
+        load_rec32<N>(A, sid, rec);
         const int pb = chunk * SR_CELLS_PER_UNIT + lane;
         bool keep = pb < vol;
         uint32_t first = 0, cnt = 0;
@@ -375,8 +379,6 @@ __global__ void __launch_bounds__(SR_BLOCK) k_sr_raster(TreeArgs A, const int2* 
         const uint32_t total = __shfl(incl, 63);
         // a ray's place in cell order: its slot j plus its cell's first ray minus the rays before its cell
         const int64_t delta = (int64_t)first - (int64_t)(incl - cnt);
-        float rec[RS];
-        load_rec32<N>(A, sid, rec);
         auto comp = [&](int cc) -> float { return rec[cc]; };
         auto get_sid = [&]() -> int { return sid; };
         for (uint32_t base = 0; base < total; base += 64) {
