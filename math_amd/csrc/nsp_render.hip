@@ -259,8 +259,19 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_raygen(SceneK<T> sc, const Batc
     }
 }
 
+// Waves per SIMD asked of the compiler for k_shade: 5 for the plain FP32
+// variant at n <= 4 (C1: 96 VGPRs with 72 bytes of spills per lane instead of
+// 122 VGPRs at 4 waves, shade 6.0 against 6.4 ms per pass; 6 and 8 waves spill
+// 140 and 176 bytes and are slower), otherwise what the register use allows
+// (the smooth-normal and several-light variants would spill 480 and 256 bytes).
+template <int N, typename T, bool PLAIN>
+constexpr int shade_min_waves()
+{
+    return PLAIN && N <= 4 && sizeof(T) == 4 ? 5 : 1;
+}
+
 template <int N, typename T, bool SMOOTH, bool SHAPES, bool MULTI>
-__global__ void __launch_bounds__(TRACE_BLOCK)
+__global__ void __launch_bounds__(TRACE_BLOCK, (shade_min_waves<N, T, !SMOOTH && !SHAPES && !MULTI>()))
     k_shade(SceneK<T> sc, TreeArgs A, MeshShadeArgs M, const BatchDesc* __restrict__ desc, int depth,
             RayQueue<T> qin, RayQueue<T> qout, ShadowQueue<T> sq, int64_t cap, unsigned long long* __restrict__ cnt,
             float* __restrict__ L, float* __restrict__ aov, int prim)
