@@ -918,11 +918,14 @@ int nsp_paint(nsp_scene* scene, const nsp_paint_params* params, const nsp_callba
         const bool raster = p.precision == NSP_PRECISION_FP32 && p.primary == NSP_PRIMARY_AUTO &&
                             scene->proj.kind == NSP_PROJECTOR_PARALLEL;
         // shadow rays rasterised from the light: FP32 walks over simplex-ordered
-        // records, one point light, no shapes (and the light sees the mesh in
-        // front of it, decided by shadow_raster_prepare)
+        // records, one point light, no shapes, no smooth-normal probes (their
+        // rays do not start at the hit; the rasteriser rebuilds each ray from
+        // its hit point), and the light sees the mesh in front of it, decided
+        // by shadow_raster_prepare
         bool sraster = p.precision == NSP_PRECISION_FP32 && p.primary == NSP_PRIMARY_AUTO &&
                        scene->lights.size() == 1 && scene->lights[0].kind == NSP_LIGHT_POINT &&
-                       scene->shapes.empty() && scene->mesh->m.layout == NSP_LAYOUT_SIMPLEX_AOS;
+                       scene->shapes.empty() && scene->mesh->m.layout == NSP_LAYOUT_SIMPLEX_AOS &&
+                       !(p.smooth_normal && scene->mesh->m.has_normals);
         // Default batch: the largest power of two of paths, at most 2^27 (FP32) or
         // 2^26 (FP64), whose workspace fits DEFAULT_WORKSPACE.  Large batches keep
         // the persistent walks busy and leave fewer batch tails (C1, 27 samples per
@@ -1349,7 +1352,8 @@ int nsp_batch_paint(nsp_batch* b, const nsp_paint_params* params, const volatile
             const bool raster = raster_mode && sc->proj.kind == NSP_PROJECTOR_PARALLEL;
             // as in nsp_paint; shadow_raster_prepare may still turn it off below
             bs.sraster = raster_mode && sc->lights.size() == 1 && sc->lights[0].kind == NSP_LIGHT_POINT &&
-                         sc->shapes.empty() && sc->mesh->m.layout == NSP_LAYOUT_SIMPLEX_AOS;
+                         sc->shapes.empty() && sc->mesh->m.layout == NSP_LAYOUT_SIMPLEX_AOS &&
+                         !(p.smooth_normal && sc->mesh->m.has_normals);
             int64_t want = p.batch_paths;
             if (want <= 0) {
                 const int64_t per = render_bytes_per_path(sc->n, NSP_PRECISION_FP32, (int)sc->lights.size(), false,

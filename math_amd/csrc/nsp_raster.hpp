@@ -84,9 +84,13 @@ void shadow_raster_buffers(RenderBuffers& b, int n, int64_t scap);
 // Occlusion of the shadow rays of one depth (queue sq, counters dc); the
 // contributions of the rays that reach the light are added to Ls (and to the
 // direct-light AOV when direct is set), as the shadow walks add them.
-// rays: when not null, the number of rays is added to it.
+// rays: when not null, the number of rays is added to it.  The queue holds
+// each ray's hit point alone (shade's spoint mode); the ray is rebuilt from it
+// towards the light at light32 (the FP32 position shading used) with the ray
+// offset shading used.
 void shadow_raster_launch(const ShadowRasterPrep& sp, const MeshDev& m, const TreeArgs& A, ShadowQueue<float> sq,
                           int64_t scap, const unsigned long long* dc, RenderBuffers& b, float* Ls, int64_t cap,
-                          float* direct, unsigned long long* rays, hipStream_t s);
+                          float* direct, const float* light32, float offset, unsigned long long* rays,
+                          hipStream_t s);
 
 }  // namespace nsp

@@ -274,7 +274,7 @@ template <int N, typename T, bool SMOOTH, bool SHAPES, bool MULTI>
 __global__ void __launch_bounds__(TRACE_BLOCK, (shade_min_waves<N, T, !SMOOTH && !SHAPES && !MULTI>()))
     k_shade(SceneK<T> sc, TreeArgs A, MeshShadeArgs M, const BatchDesc* __restrict__ desc, int depth,
             RayQueue<T> qin, RayQueue<T> qout, ShadowQueue<T> sq, int64_t cap, unsigned long long* __restrict__ cnt,
-            float* __restrict__ L, float* __restrict__ aov, int prim)
+            float* __restrict__ L, float* __restrict__ aov, int prim, int spoint)
 {
     __shared__ int2 stack[stack_levels(N) * TRACE_BLOCK];
     int2* st = stack + threadIdx.x;
@@ -302,6 +302,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK, (shade_min_waves<N, T, !SMOOTH &&
         ShadeOut<N, T> out;
         out.next = false;
         out.shadow = false;
+        out.spoint = spoint != 0;
         int32_t path = 0;
         if (valid) {
             const int hidx = qin.hit[i];
@@ -377,12 +378,14 @@ __global__ void __launch_bounds__(TRACE_BLOCK, (shade_min_waves<N, T, !SMOOTH &&
             int64_t ss = wave_append(out.shadow, shadow_count);
             if (ss >= 0) {
                 ss += sout0;
+                // spoint: the hit point alone (the shadow raster rebuilds the ray)
 #pragma unroll
-                for (int k = 0; k < N; ++k) {
-                    sq.org[(int64_t)k * scap + ss] = out.sorg[k];
-                    sq.dir[(int64_t)k * scap + ss] = out.sdir[k];
+                for (int k = 0; k < N; ++k) sq.org[(int64_t)k * scap + ss] = out.sorg[k];
+                if (!spoint) {
+#pragma unroll
+                    for (int k = 0; k < N; ++k) sq.dir[(int64_t)k * scap + ss] = out.sdir[k];
+                    sq.tmax[ss] = out.stmax;
                 }
-                sq.tmax[ss] = out.stmax;
                 // a first-hit shadow ray is marked by a negative path index while
                 // the direct-light AOV is collected
                 sq.path[ss] = (aov && depth == 0) ? -(path + 1) : path;
@@ -1068,7 +1071,7 @@ int64_t render_bytes_per_path(int n, int precision, int lights, bool aov, bool r
     const int64_t shadow = 2 * ts * n + ts + 4 + 12 + 4;
     int64_t b = 2 * queue + 2 * shadow * L + 24 + 4 * L + 4 + 4 * RES_STRIDE * (1 + L);
     if (raster) b += 8;
-    if (shadow_raster) b += (4 + 4 + 4 * (2 * n + 1) + 4 + 1) * L;
+    if (shadow_raster) b += (4 + 4 + 4 * n + 4 + 1) * L;
     if (aov) b += 4 * aov_channels(n);
     return b;
 }
@@ -1218,7 +1221,7 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
         if (!merged && rl.side && depth >= 2) NSP_HIP(hipStreamWaitEvent(s, b.done_ev[depth & 1], 0));
         timed(2, [&] {
             hipLaunchKernelGGL(kshade, dim3(g), dim3(TRACE_BLOCK), 0, s, sc, A, M, desc, depth, qin, qout, sq, cap,
-                               cnt, L, aov, (int)(slim && depth == 0));
+                               cnt, L, aov, (int)(slim && depth == 0), (int)(rl.sraster != nullptr && depth == 0));
         });
         if constexpr (sizeof(T) == 4) {
             if (merged) {
@@ -1243,6 +1246,7 @@ static void render_batch_t(const RenderLaunch& rl, hipStream_t s)
                     }
                     timed_on(ss, 3, [&] {
                         shadow_raster_launch(*rl.sraster, *rl.mesh, A, sq, scap, dc, b, Ls, cap, direct,
+                                             sc.light_pos[0], sc.ray_offset,
                                              rl.counters ? b.trav_counters.as<unsigned long long>() + 8 : nullptr,
                                              ss);
                     });

@@ -8,6 +8,7 @@
 
 #include "nsp_device.hpp"
 #include "nsp_raster.hpp"
+#include "nsp_stages.hpp"
 #include "nsp_trace.hpp"
 
 namespace nsp {
@@ -32,9 +33,15 @@ constexpr int SR_SCAN_TILE = 4096;  // cells per block of the count scan (16 per
 constexpr double SR_CELL_DIV = NSP_SR_CELL_DIV;
 constexpr int SR_MAX_TILES = 4096;  // at most 2^24 cells
 
-// Floats per binned shadow ray: origin, direction and limit, padded to whole
-// 16-byte words (one to four 16-byte loads per ray).
-__host__ __device__ constexpr int sr_record(int n) { return 2 * n + 1; }
+// Floats per binned shadow ray: its hit point (the ray is rebuilt from it with
+// shadow_ray_from, as shading built it).
+__host__ __device__ constexpr int sr_record(int n) { return n; }
+
+// The light as shading saw it: FP32 position and the ray offset.
+struct SRLight {
+    float pos[NSP_MAX_DIM];
+    float offset;
+};
 
 // Screen coordinates of a point x as the light sees it: s_i = e_i.(x - light) /
 // a.(x - light), a the axis from the light to the mesh centre, e_i an
@@ -263,7 +270,7 @@ __global__ void __launch_bounds__(256) k_sr_tile_down(uint32_t* __restrict__ cou
     if (blockIdx.x == 0 && threadIdx.x == 0) start[cells] = *total;
 }
 
-// Each shadow ray's origin, direction and limit to its place in cell order.
+// Each shadow ray's hit point to its place in cell order.
 template <int N>
 __global__ void __launch_bounds__(SR_BLOCK) k_sr_scatter(ShadowQueue<float> sq, int64_t scap,
                                                          const unsigned long long* __restrict__ dc,
@@ -282,11 +289,7 @@ __global__ void __launch_bounds__(SR_BLOCK) k_sr_scatter(ShadowQueue<float> sq, 
         constexpr int R = sr_record(N);
         float r[R];
 #pragma unroll
-        for (int k = 0; k < N; ++k) {
-            r[k] = sq.org[(int64_t)k * scap + i];
-            r[N + k] = sq.dir[(int64_t)k * scap + i];
-        }
-        r[2 * N] = sq.tmax[i];
+        for (int k = 0; k < N; ++k) r[k] = sq.org[(int64_t)k * scap + i];
         float* dst = rays + pos * R;
 #pragma unroll
         for (int k = 0; k < R; ++k) dst[k] = r[k];
@@ -301,8 +304,8 @@ __global__ void __launch_bounds__(SR_BLOCK) k_sr_scatter(ShadowQueue<float> sq, 
 // shadow walk applies to a leaf candidate (FP32 with the FP64 re-test inline).
 // The rays of the kept cells are dealt 64 at a time: a wave scan of the cells'
 // ray counts, and a lane finds its cell by a binary search over the scan.  A
-// hit makes the ray's limit negative in its record, so the record load that
-// brings a ray to a later simplex also tells it to skip the test.
+// hit writes a NaN into the ray's record, so the record load that brings a ray
+// to a later simplex also tells it to skip the test.
 template <int N>
 __global__ void __launch_bounds__(SR_BLOCK) k_sr_raster(TreeArgs A, const int2* __restrict__ unit, int64_t units,
                                                         const int32_t* __restrict__ box,
@@ -310,7 +313,7 @@ __global__ void __launch_bounds__(SR_BLOCK) k_sr_raster(TreeArgs A, const int2* 
                                                         const uint32_t* __restrict__ start,
                                                         float* __restrict__ rays, int64_t scap,
                                                         const int32_t* __restrict__ ray_of,
-                                                        uint8_t* __restrict__ occl_q)
+                                                        uint8_t* __restrict__ occl_q, SRLight lt)
 {
     constexpr int D = N - 1;
     constexpr int RS = rec32_stride(N);
@@ -398,23 +401,17 @@ __global__ void __launch_bounds__(SR_BLOCK) k_sr_raster(TreeArgs A, const int2* 
             const float* src = rays + pos * R;
 #pragma unroll
             for (int k = 0; k < R; ++k) r[k] = src[k];
-            float o[N], d[N];
-#pragma unroll
-            for (int k = 0; k < N; ++k) {
-                o[k] = r[k];
-                d[k] = r[N + k];
-            }
-            // an occluded ray's record carries a negative limit (no hit lies
-            // below a limit of zero, so the test would find none either)
-            const float tmax = r[2 * N];
-            if (!(tmax > 0.0f)) continue;
+            // an occluded ray's record holds a NaN (a hit point is finite)
+            if (!(r[0] == r[0])) continue;
+            float vl[N], o[N], d[N], tmax;
+            shadow_ray_from<N, float>(lt.pos, r, lt.offset, vl, d, &tmax, o);
             double th;
             int si;
             bool nr = false;
             unsigned nret = 0;
             if (fp32_test<N, false>(A, comp, get_sid, o, d, nullptr, nullptr, tmax, -1, &th, &si, &nr, &nret) ==
                 CAND_HIT) {
-                rays[pos * R + 2 * N] = -1.0f;
+                rays[pos * R] = __int_as_float(0x7fc00000);
                 occl_q[ray_of[pos]] = 1;
             }
         }
@@ -636,8 +633,11 @@ void prepare_t(ShadowRasterPrep& sp, const MeshDev& m, const double* light, hipS
 template <int N>
 void launch_t(const ShadowRasterPrep& sp, const TreeArgs& A, ShadowQueue<float> sq, int64_t scap,
               const unsigned long long* dc, RenderBuffers& b, float* Ls, int64_t cap, float* direct,
-              unsigned long long* rays, hipStream_t s)
+              const float* light32, float offset, unsigned long long* rays, hipStream_t s)
 {
+    SRLight lt{};
+    for (int k = 0; k < N; ++k) lt.pos[k] = light32[k];
+    lt.offset = offset;
     const SRGrid& g = sp.grid;
     const unsigned gq = 4096;  // a multiple of QSEG
     uint32_t* count = sp.count.as<uint32_t>();
@@ -660,7 +660,7 @@ void launch_t(const ShadowRasterPrep& sp, const TreeArgs& A, ShadowQueue<float> 
     hipLaunchKernelGGL(k_sr_raster<N>, dim3(2048), dim3(SR_BLOCK), 0, s, A, sp.unit.as<int2>(), sp.units,
                        sp.box.as<int32_t>(), sp.planes.as<float>(), g, sp.start.as<uint32_t>(),
                        b.sr_rays.as<float>(), scap, b.sr_ray.as<int32_t>(),
-                       b.sr_occq.as<uint8_t>());
+                       b.sr_occq.as<uint8_t>(), lt);
     NSP_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_sr_finish<N>, dim3(gq), dim3(SR_BLOCK), 0, s, sq, scap, dc, sp.cells,
                        sp.start.as<uint32_t>(), b.sr_occq.as<uint8_t>(), Ls, cap, direct, rays);
@@ -701,13 +701,14 @@ void shadow_raster_buffers(RenderBuffers& b, int n, int64_t scap)
 
 void shadow_raster_launch(const ShadowRasterPrep& sp, const MeshDev& m, const TreeArgs& A, ShadowQueue<float> sq,
                           int64_t scap, const unsigned long long* dc, RenderBuffers& b, float* Ls, int64_t cap,
-                          float* direct, unsigned long long* rays, hipStream_t s)
+                          float* direct, const float* light32, float offset, unsigned long long* rays,
+                          hipStream_t s)
 {
     switch (m.n) {
-    case 3: launch_t<3>(sp, A, sq, scap, dc, b, Ls, cap, direct, rays, s); break;
-    case 4: launch_t<4>(sp, A, sq, scap, dc, b, Ls, cap, direct, rays, s); break;
-    case 5: launch_t<5>(sp, A, sq, scap, dc, b, Ls, cap, direct, rays, s); break;
-    case 6: launch_t<6>(sp, A, sq, scap, dc, b, Ls, cap, direct, rays, s); break;
+    case 3: launch_t<3>(sp, A, sq, scap, dc, b, Ls, cap, direct, light32, offset, rays, s); break;
+    case 4: launch_t<4>(sp, A, sq, scap, dc, b, Ls, cap, direct, light32, offset, rays, s); break;
+    case 5: launch_t<5>(sp, A, sq, scap, dc, b, Ls, cap, direct, light32, offset, rays, s); break;
+    case 6: launch_t<6>(sp, A, sq, scap, dc, b, Ls, cap, direct, light32, offset, rays, s); break;
     default: raise(NSP_E_INVALID, "dimension not supported");
     }
 }

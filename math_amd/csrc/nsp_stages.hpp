@@ -253,7 +253,28 @@ struct ShadeOut {
     bool direct, mesh;
     T sp[N], sgn[N], ssn[N];
     float sbeta[3], sscol[3];
+    // set by the caller: the shadow ray is kept as its hit point alone (sorg),
+    // to be rebuilt by shadow_ray_from where it is tested
+    bool spoint;
 };
+
+// The shadow ray from hit point p towards a point light at lpos: vl = lpos - p,
+// dist its length, ld = vl * (1 / dist) its direction and o1 = p + ld * offset
+// (fused multiply-adds, one IEEE square root and one IEEE division), so every
+// kernel that rebuilds the ray from p gets the same bits.
+template <int N, typename T>
+__device__ __forceinline__ void shadow_ray_from(const T* lpos, const T* p, T offset, T* vl, T* ld, T* dist, T* o1)
+{
+#pragma unroll
+    for (int k = 0; k < N; ++k) vl[k] = lpos[k] - p[k];
+    const T len = sqrt(dot<N, T>(vl, vl));
+    const T inv = (T)1 / len;
+#pragma unroll
+    for (int k = 0; k < N; ++k) ld[k] = vl[k] * inv;
+    *dist = len;
+#pragma unroll
+    for (int k = 0; k < N; ++k) o1[k] = fma(ld[k], offset, p[k]);
+}
 
 // The shadow ray of one light source from a hit (direct_diffuse_lighting's
 // loop body, painter.cpp:205-289; PointLight and ConstantLight::properties,
@@ -269,9 +290,8 @@ __device__ __forceinline__ void light_shadow_ray(const SceneK<T>& sc, const Tree
 {
     out.shadow = false;
     out.sshape = -1;
-    T vl[N];
-#pragma unroll
-    for (int k = 0; k < N; ++k) vl[k] = sc.light_pos[l][k] - p[k];
+    T vl[N], ld[N], dist, o1[N];
+    shadow_ray_from<N, T>(sc.light_pos[l], p, sc.ray_offset, vl, ld, &dist, o1);
     T sq2 = dot<N, T>(vl, vl);
     T scale;
     if (sc.light_kind[l] == 1)
@@ -283,17 +303,9 @@ __device__ __forceinline__ void light_shadow_ray(const SceneK<T>& sc, const Tree
     float lc[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) lc[c] = sc.light_rgb[l][c] * (float)scale;
-    T ld[N];
-#pragma unroll
-    for (int k = 0; k < N; ++k) ld[k] = vl[k];
-    normalize<N, T>(ld);
     T dln = dot<N, T>(ld, sn);
     bool lit = !(fmaxf(lc[0], fmaxf(lc[1], lc[2])) < 1e-4f) && dln > (T)0;
     if (!lit) return;
-    T dist = sqrt(dot<N, T>(vl, vl));
-    T o1[N];
-#pragma unroll
-    for (int k = 0; k < N; ++k) o1[k] = p[k] + ld[k] * sc.ray_offset;
     T tmaxs = dist;
     if (SMOOTH && mesh && sc.smooth && dot<N, T>(ld, gn) < (T)0) {
         // smooth-normal back face: forward and reverse probes (painter.cpp:238-284)
@@ -330,7 +342,7 @@ __device__ __forceinline__ void light_shadow_ray(const SceneK<T>& sc, const Tree
     if (SHAPES && sc.shape_count > 0) out.sshape = blocking_shape<N, T>(sc, o1, ld, (double)dist);
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        out.sorg[k] = o1[k];
+        out.sorg[k] = out.spoint ? p[k] : o1[k];
         out.sdir[k] = ld[k];
     }
     out.stmax = tmaxs;
