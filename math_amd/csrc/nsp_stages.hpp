@@ -259,19 +259,24 @@ struct ShadeOut {
 };
 
 // The shadow ray from hit point p towards a point light at lpos: vl = lpos - p,
-// dist its length, ld = vl * (1 / dist) its direction and o1 = p + ld * offset
-// (fused multiply-adds, one IEEE square root and one IEEE division), so every
+// ld = vl / |vl| its direction, dist = |vl| its length and o1 = p + ld * offset,
+// from fused multiply-adds and, in FP32, one hardware reciprocal square root
+// (v_rsq_f32, within 1 ulp; FP64: an IEEE square root and division), so every
 // kernel that rebuilds the ray from p gets the same bits.
 template <int N, typename T>
 __device__ __forceinline__ void shadow_ray_from(const T* lpos, const T* p, T offset, T* vl, T* ld, T* dist, T* o1)
 {
 #pragma unroll
     for (int k = 0; k < N; ++k) vl[k] = lpos[k] - p[k];
-    const T len = sqrt(dot<N, T>(vl, vl));
-    const T inv = (T)1 / len;
+    const T len2 = dot<N, T>(vl, vl);
+    T inv;
+    if constexpr (sizeof(T) == 4)
+        inv = __builtin_amdgcn_rsqf(len2);
+    else
+        inv = (T)1 / sqrt(len2);
 #pragma unroll
     for (int k = 0; k < N; ++k) ld[k] = vl[k] * inv;
-    *dist = len;
+    *dist = len2 * inv;
 #pragma unroll
     for (int k = 0; k < N; ++k) o1[k] = fma(ld[k], offset, p[k]);
 }
