@@ -44,8 +44,8 @@ void raster_launch(const RasterPrep& rp, const MeshDev& m, const TreeArgs& A, co
 // Shadow rays towards one point light, rasterised as the light sees the mesh:
 // every shadow ray of the depth ends at the light, so, projected from the
 // light, it is a single point, and the simplices it can hit are those whose
-// projection covers that point.  The rays are binned by the cell of their
-// origin's projection (a count, a scan and a scatter into cell order); then
+// projection covers that point.  The rays are binned by the cell of their hit
+// point's projection (a count, a scan and a scatter into cell order); then
 // each simplex's footprint cells are culled by its projected face planes and
 // every ray of the kept cells gets the any-hit test the shadow walk applies to
 // a leaf candidate.  The occlusion of each ray is therefore the walk's.  Used

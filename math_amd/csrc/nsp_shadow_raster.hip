@@ -17,10 +17,11 @@ namespace {
 
 constexpr int SR_BLOCK = 256;
 // Margin, in cells, by which footprints are widened.  A shadow ray o + t d runs
-// from its origin to the light; its FP32 direction passes the light within
-// about 1e-7 of its length, so where it meets a simplex its projection lies
-// within about 1e-7 (distance / depth) of its origin's, far below the margin
-// at the cell sizes used (at least 1e-4 of the projected extent).
+// from its origin (its hit point moved by the ray offset towards the light) to
+// the light; its FP32 direction passes the light within about 1e-7 of its
+// length, so where it meets a simplex its projection lies within about 1e-7
+// (distance / depth) of its hit point's, far below the margin at the cell sizes
+// used (at least 1e-4 of the projected extent).
 constexpr double SR_MARGIN = 0.01;
 constexpr int SR_CELLS_PER_UNIT = 64;
 constexpr int SR_SCAN_TILE = 4096;  // cells per block of the count scan (16 per thread)
@@ -164,7 +165,7 @@ __global__ void __launch_bounds__(256) k_sr_planes(int64_t F, const int32_t* __r
     }
 }
 
-// The cell of each shadow ray of the depth (its origin's), counted per cell.
+// The cell of each shadow ray of the depth (its hit point's), counted per cell.
 // Block b takes queue segment b % QSEG.
 template <int N>
 __global__ void __launch_bounds__(SR_BLOCK) k_sr_count(ShadowQueue<float> sq, int64_t scap,
@@ -497,7 +498,7 @@ void prepare_t(ShadowRasterPrep& sp, const MeshDev& m, const double* light, hipS
     NSP_HIP(hipMemcpyAsync(hf.data(), m.fac.p, sizeof(int32_t) * hf.size(), hipMemcpyDeviceToHost, s));
     NSP_HIP(hipStreamSynchronize(s));
     // every vertex well in front of the light (so that the footprints are the
-    // projected simplices and every shadow ray's origin projects)
+    // projected simplices and every shadow ray's hit point projects)
     double dist = 0;
     for (int k = 0; k < N; ++k) dist = std::max(dist, std::max(std::fabs(m.mn[k] - light[k]), std::fabs(m.mx[k] - light[k])));
     double zmin = INFINITY;
