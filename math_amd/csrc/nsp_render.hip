@@ -419,16 +419,18 @@ __device__ __forceinline__ void push_pending(unsigned long long* res_count, int3
 // rays from its own XCD's segment, so neighbouring rays share that XCD's L2,
 // and moves on to the next segments once its own is empty.  Every wave leaves
 // the loop after all segments are empty and its own rays are done.
-// Measured with one batch per pass and the merged walk launch (C1, Mrays/s, two
-// runs each): 16 lanes 2483, 2474; 32 lanes 2452, 2439; 48 lanes 2396, 2410.
-// -DNSP_REFILL_LANES=L sets one threshold for every dimension.
+// Measured with the shadow rays of the primary hits rasterised (C1, Mrays/s,
+// two runs each): 8 lanes 3183, 3194; 16 lanes 3517, 3516; 24 lanes 3646,
+// 3640; 32 lanes 3646, 3647; 40 lanes 3651, 3628.  (When the depth-0 launch
+// also walked those shadow rays, 16 lanes were best: 2483 against 2452 at 32.)
+// -DNSP_REFILL_LANES=L sets another threshold.
 template <int N>
 constexpr int refill_lanes()
 {
 #ifdef NSP_REFILL_LANES
     return NSP_REFILL_LANES;
 #else
-    return N >= 5 ? 32 : 16;
+    return 32;
 #endif
 }
 
