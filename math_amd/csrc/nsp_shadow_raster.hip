@@ -23,6 +23,9 @@ constexpr int SR_BLOCK = 256;
 // (distance / depth) of its hit point's, far below the margin at the cell sizes
 // used (at least 1e-4 of the projected extent).
 constexpr double SR_MARGIN = 0.01;
+#ifndef NSP_SR_RASTER_BLOCKS
+#define NSP_SR_RASTER_BLOCKS 4096
+#endif
 constexpr int SR_CELLS_PER_UNIT = 64;
 constexpr int SR_SCAN_TILE = 4096;  // cells per block of the count scan (16 per thread)
 // Cell edge: the mean footprint extent per axis over SR_CELL_DIV.  Every ray is
@@ -658,7 +661,7 @@ void launch_t(const ShadowRasterPrep& sp, const TreeArgs& A, ShadowQueue<float> 
                        b.sr_rank.as<uint32_t>(), sp.start.as<uint32_t>(), b.sr_rays.as<float>(),
                        b.sr_ray.as<int32_t>(), b.sr_occq.as<uint8_t>());
     NSP_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_sr_raster<N>, dim3(2048), dim3(SR_BLOCK), 0, s, A, sp.unit.as<int2>(), sp.units,
+    hipLaunchKernelGGL(k_sr_raster<N>, dim3(NSP_SR_RASTER_BLOCKS), dim3(SR_BLOCK), 0, s, A, sp.unit.as<int2>(), sp.units,
                        sp.box.as<int32_t>(), sp.planes.as<float>(), g, sp.start.as<uint32_t>(),
                        b.sr_rays.as<float>(), scap, b.sr_ray.as<int32_t>(),
                        b.sr_occq.as<uint8_t>(), lt);
